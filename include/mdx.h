@@ -1,0 +1,190 @@
+/*
+ * mdx.h -- C ABI of the MI355X force engine for MACSIMUS `cook`.
+ *
+ * The engine is a drop-in for cook's force call
+ *     void forces(ToIntPtr B, ToIntPtr A)            cook/forces.h:1, cook/forces.c:93-224
+ * and the per-step work beside it (SHAKE + leap-frog + Berendsen,
+ * sim/constrd.c:622-1081; Drude SCF and ASPC, sim/scf.c:12-393, sim/pred2.c).
+ *
+ * Conventions are cook's own (sim/units.h:30-108): lengths in Angstrom,
+ * energies in K (k_B = 1), time in ps, charges in program units
+ * (e = 408.779826922048), Coulomb constant 1, forces in K/Angstrom.
+ * Site arrays are cook's `rp` layout: double[3] per site, molecule-major
+ * (molec[n].ir, sim/siminit.c:497).  The virial tensor order is
+ * xx, yy, zz, yz, zx, xy (sim/simglob.h:487) and is NOT divided by V.
+ *
+ * Every entry point returns 0 on success or a negative MDX_E* code; the
+ * message of the last failure on the calling thread is mdx_last_error().
+ * Nothing here calls exit() (cook's ERROR() does, gen/ground.h:157).
+ * A context is used from one host thread at a time; all GPU work of a
+ * context runs on the context's own HIP stream.
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_OK          0
+#define MDX_E_ARG      -1   /* invalid argument or inconsistent system */
+#define MDX_E_GPU      -2   /* HIP runtime failure */
+#define MDX_E_RANGE    -3   /* site out of box / geometry cook would reject */
+#define MDX_E_NOCONV   -4   /* SHAKE or SCF did not converge */
+#define MDX_E_NODEV    -5   /* no GPU device visible */
+
+/* precision of a context */
+#define MDX_FP64   0   /* plain FP64 GPU path: every kernel in double */
+#define MDX_MIXED  1   /* FP32 pair and k-space arithmetic, FP64 accumulation,
+                          FP64 exclusions, corrections and global sums */
+
+/* Site-site table of one type pair: cook's sitesite_t by value
+   (sim/interpot.h:29-57 + pairaux_t of sim/lj/sitesite.h:44-52).
+   u(r) = E4*((Sq/r^2)^6-(Sq/r^2)^3)          for r^2 <  C1q
+        = A*(r^2-C2q)^2                        for C1q <= r^2 < C2q
+   f(r)/r = E48*y*(y-1/2)*x, x=Sq/r^2, y=x^3  resp.  A4*(r^2-C2q)
+   (sim/setss.c:137-306; cook/ljqqnp.c:203-213) */
+typedef struct mdx_lj_s {
+  double Sq, E4, E24, E48;
+  double C1q, C2q, A, A4;
+} mdx_lj_t;
+
+/* one interval of cook's hyperbolic r-space splines, ertab_t
+   (sim/elst.h:62-68): eru(x)=Bu/(Cu+x)+Au, erd(x)=Bd/(Cd+x)+Ad,
+   x=r^2, interval index (int)(x*sgrid) (sim/elst.h:122-130) */
+typedef struct mdx_erfc_s {
+  double Au, Bu, Cu, Ad, Bd, Cd;
+} mdx_erfc_t;
+
+/* Everything cook's globals hand to forces(): the `same site arrays'
+   plus the tables initss()/initerfc()/Ewald(0) prepare.  All arrays are
+   copied by mdx_create(); the caller keeps ownership. */
+typedef struct mdx_system_s {
+  int ns;                    /* No.s: LJ/charge centres (without Drudes) */
+  int nmol;                  /* No.N: molecules */
+  int ntypes;                /* number of site types (nsites in cook) */
+  const int *mol_first;      /* [nmol+1] first site of molecule n */
+  const int *site_type;      /* [ns] si->st */
+  const double *charge;      /* [ns] si->charge (p.u.), central charge */
+  const unsigned *excl;      /* [ns] bit k: site mol_first[mol]+k is an
+                                EXCL partner (1-2, 1-3; exc->type==EXCL) */
+  const mdx_lj_t *sstab;     /* [ntypes*ntypes] sstab[i][j] */
+  double L[3];               /* box.L */
+  double cutoff;             /* box.cutoff: r-space cutoff (Coulomb) */
+  /* Ewald, sim/ewald.c:258-1003 */
+  double alpha;              /* el.alpha (k-space screening) */
+  double kappa;              /* el.kappa = K/L after the almost-integer fix */
+  double epsinf;             /* el.epsinf (3e33 = tin foil) */
+  /* r-space splines, sim/erfc.c:119-191 */
+  double alphar;             /* Erfc.alpha (el.alphar) */
+  double erfc_sgrid;         /* Erfc.sgrid (= grid*alphar^2) */
+  int erfc_n;                /* number of intervals */
+  const mdx_erfc_t *erfc_tab;/* [erfc_n] Erfc.tab, already shifted */
+  /* linked cells, cook/linklist.c:304-382 */
+  int ncell[3];              /* No.cell; 0 lets the engine choose */
+  /* Drude (POLAR), sim/scf.c; NULL chargepol means nonpolar */
+  const double *chargepol;   /* [ns] si->chargepol (p.u.), 0 = no Drude */
+  const double *alpha_qq;    /* [ns] si->alpha_qq = alphapol/chargepol^2 */
+  /* constraint dynamics (SHAKE=1, VERLET=3), sim/constrd.c:622-1081 */
+  const double *mass;        /* [ns] si->mass (p.u.); NULL = no dynamics */
+  int ncons;                 /* number of constraints */
+  const int *cons_first;     /* [nmol+1] first constraint of molecule n */
+  const int *cons_ij;        /* [2*ncons] global site indices i,j */
+  const double *cons_bondq;  /* [ncons] squared bond lengths */
+  const double *mol_omega;   /* [nmol] SHAKE relaxation omega (= omegac/2) */
+} mdx_system_t;
+
+/* what forces() leaves in the globals of sim/simglob.h:436-577 */
+typedef struct mdx_out_s {
+  double En_el;        /* En.el: Ewald k-space + r-space + self + dipole (SET) */
+  double En_pot;       /* En.pot += U_LJ (+ bonded intramolecular = 0 here) */
+  double En_vir;       /* En.vir += sum rr*f over LJ terms */
+  double Pvir[6];      /* En.Pvir[0..5] += virial tensor (not /V) */
+  double En_kspace;    /* k-space energy incl. self and dipole terms */
+  long long npair;     /* census: site pairs with rr < cutoff^2 */
+  long long nlj;       /* of those, pairs inside C2q with nonzero LJ */
+} mdx_out_t;
+
+typedef struct mdx_ctx_s mdx_ctx_t;
+
+const char *mdx_last_error(void);
+
+/* Creates a context holding the system on the current HIP device.
+   mode: MDX_FP64 or MDX_MIXED.  Replaces the one-time set-up that cook
+   does in lcsetup() (cook/linklist.c:304), Ewald(0) (sim/ewald.c:334-560)
+   and the static scratch re-allocation of linklist()/Ewald(). */
+int mdx_create(mdx_ctx_t **ctx, const mdx_system_t *sys, int mode);
+int mdx_destroy(mdx_ctx_t *ctx);
+
+/* Replaces forces(B,A) (cook/forces.c:93-224) for nonpolar systems.
+   r: [ns*3] configuration A->rp;  f: [ns*3] forces B->rp (overwritten,
+   like sdszero(B) at cook/forces.c:183).  measure!=0 computes energies,
+   virial and pressure tensor (LJqqm/ewpass2m.c variants), otherwise
+   only forces (out->npair is still filled).  Host pointers. */
+int mdx_forces(mdx_ctx_t *ctx, const double *r, double *f, mdx_out_t *out,
+               int measure);
+
+/* Box change between calls (box.L); k-vector set stays as in
+   sim/ewald.c:570-613 (ekk recomputed, same ktab). */
+int mdx_set_box(mdx_ctx_t *ctx, const double L[3]);
+
+/* ---- device-resident MD (the captured step) ---------------------------
+   State lives on the GPU: r = cfg[0] (r(t)), r1 = cfg[1] (r(t)-r(t-h)). */
+typedef struct mdx_step_s {
+  double h;            /* time step [ps] */
+  double eps;          /* SHAKE relative tolerance (epsc) */
+  int maxit;           /* max SHAKE sweeps per molecule */
+  int thermostat;      /* 0 = none (NVE), 1 = Berendsen (sim/thermo.c:141-160) */
+  double T;            /* Berendsen target temperature [K] */
+  double tau;          /* Berendsen time constant [ps] */
+  double nf;           /* No.f degrees of freedom */
+  int measure_every;   /* energies/virial on every n-th step (cook: noint) */
+} mdx_step_t;
+
+typedef struct mdx_step_out_s {
+  mdx_out_t frc;       /* force-call outputs of the last measured step */
+  double En_kin;       /* kinetic energy (VERLET=3 leap-frog estimate) */
+  double Pkin[6];      /* kinetic tensor (not /V) */
+  double virc;         /* constraint virial En.virc */
+  double PTvirc[6];    /* constraint virial tensor */
+  double lambda;       /* last Berendsen scaling factor */
+  int shake_iter_max;  /* max SHAKE sweeps used by a molecule */
+} mdx_step_out_t;
+
+int mdx_upload_state(mdx_ctx_t *ctx, const double *r, const double *r1);
+int mdx_download_state(mdx_ctx_t *ctx, double *r, double *r1);
+/* runs nsteps of Shake() (sim/constrd.c:622) with forces computed on the GPU */
+int mdx_md_steps(mdx_ctx_t *ctx, const mdx_step_t *p, int nsteps,
+                 mdx_step_out_t *out);
+
+/* ---- set-up helpers restating cook's init code (product, not oracle) -- */
+
+/* combining rule (sim/lj/sitesite.c:66-112) + smooth cutoff (sim/setss.c:
+   137-306).  EvdW in K (negative), RvdW in Angstrom, comb_rule as in the
+   ble file, C2 = LJcutoff.  Writes ss; returns the homogeneous cutoff
+   correction corr (sim/setss.c:455-501) in *corr if corr != NULL. */
+int mdx_setss(mdx_lj_t *ss, double EvdW_i, double RvdW_i, double EvdW_j,
+              double RvdW_j, int comb_rule, double C2, double *corr);
+
+/* initerfc()+makesplines() with SPLINE=-2, SUBGRID=1 (sim/erfc.c:119-191,
+   sim/splines.c:23-158): fills tab[0..*n) (caller provides room for
+   mdx_erfc_size(...) entries). */
+int mdx_erfc_size(int grid, double maxr, double alpha);
+int mdx_erfc_table(mdx_erfc_t *tab, int *n, double *sgrid, int grid,
+                   double maxr, double cutoff, double alpha, int shift);
+
+/* Ewaldparm(): alpha, kappa from epsr, epsk (sim/ewald.c:1027-1067);
+   qmax = charges.max, sumq = charges.sumq */
+int mdx_ewald_autoset(double *alpha, double *kappa, double qmax, double sumq,
+                      double cutoff, const double L[3], double epsr,
+                      double epsk);
+/* Ewald(0): the almost-integer kappa fix and the number of k-vectors
+   Q->N (sim/ewald.c:484-522); kappa is updated in place */
+int mdx_ewald_kvectors(double *kappa, const double L[3], int *nk);
+/* lcsetup(): No.cell from No.s (cook/linklist.c:294-337) */
+int mdx_lcsetup(int ncell[3], int ns, double percell, const double L[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDX_H */

@@ -1,0 +1,203 @@
+"""Host-side mirror of cook's set-up of the force path.
+
+Builds the `mdx_system_t` that cook's globals would hand to forces():
+site types, charges, exclusions, site-site tables (initss, cook/simdef.c:
+1494-1587), r-space splines (initrspace -> initerfc, cook/simdef.c:
+1705-1721), Ewald alpha/kappa (Ewaldtest -> Ewaldparm, sim/ewald.c:1027)
+and the linked-cell grid (lcsetup, cook/linklist.c:304).
+
+The numerical set-up is done by a `provider` object exposing the five
+helpers of include/mdx.h (setss, erfc_table, ewald_autoset,
+ewald_kvectors, lcsetup).  The engine library is the default provider.
+
+Units: cook program units (sim/units.h).  The force field is given in
+cook's ble-file units (EvdW in kcal/mol, RvdW in Angstrom, charges in e,
+masses in g/mol) and converted like readblend() (cook/simdef.c:620-631,
+715-760).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+ELECTRON = 408.779826922048          # e in program units (sim/units.h:108)
+KCAL = 4184.0
+EUNIT = 8.31446261815324             # J/mol per program energy unit
+MUNIT = 0.831446261815324            # g/mol per program mass unit
+PUNIT = 13806490.0                   # Pa per program pressure unit
+EUNIT_KCAL = KCAL / EUNIT            # eunit default (cook/simdef.c:43)
+
+
+class Provider:
+    """Set-up helpers of a library with the mdx.h signatures."""
+
+    def __init__(self, lib, prefix):
+        self.lib = lib
+        self.p = prefix
+
+    def _f(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def setss(self, Ei, Ri, Ej, Rj, comb_rule, C2):
+        ss = abi.LJ()
+        corr = C.c_double(0)
+        rc = self._f("setss")(C.byref(ss), Ei, Ri, Ej, Rj, comb_rule, C2, C.byref(corr))
+        if rc:
+            raise RuntimeError("setss failed (%d)" % rc)
+        return ss, corr.value
+
+    def erfc_table(self, grid, maxr, cutoff, alpha, shift):
+        nmax = int((maxr * alpha) ** 2 * grid) + 2
+        tab = (abi.Erfc * nmax)()
+        n = C.c_int(0)
+        sgrid = C.c_double(0)
+        rc = self._f("erfc_table")(tab, C.byref(n), C.byref(sgrid), grid, maxr,
+                                   cutoff, alpha, shift)
+        if rc:
+            raise RuntimeError("erfc_table failed (%d)" % rc)
+        return tab, n.value, sgrid.value
+
+    def ewald_autoset(self, qmax, sumq, cutoff, L, epsr, epsk, kappa0=0.2):
+        a = C.c_double(0)
+        k = C.c_double(kappa0)
+        rc = self._f("ewald_autoset")(C.byref(a), C.byref(k), qmax, sumq, cutoff,
+                                      (C.c_double * 3)(*L), epsr, epsk)
+        if rc:
+            raise RuntimeError("ewald_autoset failed (%d)" % rc)
+        return a.value, k.value
+
+    def ewald_kvectors(self, kappa, L):
+        k = C.c_double(kappa)
+        nk = C.c_int(0)
+        rc = self._f("ewald_kvectors")(C.byref(k), (C.c_double * 3)(*L), C.byref(nk))
+        if rc:
+            raise RuntimeError("ewald_kvectors failed (%d)" % rc)
+        return k.value, nk.value
+
+    def lcsetup(self, ns, L, ncell=(0, 0, 0), percell=0.0):
+        nc = (C.c_int * 3)(*ncell)
+        self._f("lcsetup")(nc, ns, percell, (C.c_double * 3)(*L))
+        return list(nc)
+
+
+def engine_provider():
+    return Provider(abi.load_engine(), "mdx_")
+
+
+class BuiltSystem:
+    """An mdx_system_t plus the numpy arrays it points into."""
+
+    def __init__(self):
+        self.keep = []
+        self.struct = abi.System()
+        self.info = {}
+
+    def ptr(self, arr, ctype):
+        arr = np.ascontiguousarray(arr)
+        self.keep.append(arr)
+        return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
+                 rplus=2.5, rshift=3, epsinf=3e33, ncell=(0, 0, 0),
+                 alpha=None, kappa=None, lj_cutoff=None, provider=None,
+                 omega=0.6):
+    """ff: force-field dict as stored in tests/golden/*.json['forcefield']:
+         types, type_mass, type_EvdW_kcal, type_RvdW,
+         species[{site_type, site_charge_e, site_excl, bonds}]
+       counts: molecules per species (cook's N[i]); molecules are ordered
+       by species like cook's molec[] (sim/siminit.c:464-647)."""
+    prov = provider or engine_provider()
+    bs = BuiltSystem()
+    S = bs.struct
+    nt = len(ff["types"])
+    EvdW = [e * EUNIT_KCAL for e in ff["type_EvdW_kcal"]]
+    RvdW = list(ff["type_RvdW"])
+    C2 = cutoff if lj_cutoff is None else lj_cutoff
+    sstab = (abi.LJ * (nt * nt))()
+    corr = np.zeros((nt, nt))
+    for i in range(nt):
+        for j in range(i + 1):
+            if EvdW[i] == 0 or EvdW[j] == 0:
+                ss, c = abi.LJ(), 0.0
+            else:
+                ss, c = prov.setss(EvdW[i], RvdW[i], EvdW[j], RvdW[j], 1, C2)
+            sstab[i * nt + j] = ss
+            sstab[j * nt + i] = ss
+            corr[i, j] = corr[j, i] = c
+    bs.keep.append(sstab)
+
+    site_type, charge, mass, excl, mol_first = [], [], [], [], [0]
+    cons_first, cons_ij, cons_bondq = [0], [], []
+    for isp, n in enumerate(counts):
+        sp = ff["species"][isp]
+        ns = len(sp["site_type"])
+        masks = [0] * ns
+        for i, ex in enumerate(sp["site_excl"]):
+            for j in ex:
+                masks[i] |= 1 << j
+                masks[j] |= 1 << i
+        for m in range(n):
+            base = mol_first[-1]
+            for i in range(ns):
+                site_type.append(sp["site_type"][i])
+                charge.append(sp["site_charge_e"][i] * ELECTRON)
+                mass.append(ff["type_mass"][sp["site_type"][i]] / MUNIT)
+                excl.append(masks[i])
+            for (a, b, bl) in sp["bonds"]:
+                cons_ij += [base + a, base + b]
+                cons_bondq.append(bl * bl)
+            mol_first.append(base + ns)
+            cons_first.append(len(cons_bondq))
+    ns_tot = mol_first[-1]
+    nmol = len(mol_first) - 1
+    q = np.array(charge)
+    qmax = np.abs(q).max() if ns_tot else 0.0
+    sumq = float(np.sum(q * q))
+    if alpha is None or kappa is None:
+        alpha, kappa = prov.ewald_autoset(qmax, sumq, cutoff, L, epsr, epsk)
+    kappa, nk = prov.ewald_kvectors(kappa, L)
+    tab, nerf, sgrid = prov.erfc_table(grid, cutoff + rplus, cutoff, alpha, rshift)
+    bs.keep.append(tab)
+    if not any(ncell):
+        ncell = prov.lcsetup(ns_tot, L)
+
+    S.ns = ns_tot
+    S.nmol = nmol
+    S.ntypes = nt
+    S.mol_first = bs.ptr(np.array(mol_first, np.int32), C.c_int)
+    S.site_type = bs.ptr(np.array(site_type, np.int32), C.c_int)
+    S.charge = bs.ptr(q, C.c_double)
+    S.excl = bs.ptr(np.array(excl, np.uint32), C.c_uint)
+    S.sstab = C.cast(sstab, C.POINTER(abi.LJ))
+    S.L = (C.c_double * 3)(*L)
+    S.cutoff = cutoff
+    S.alpha = alpha
+    S.kappa = kappa
+    S.epsinf = epsinf
+    S.alphar = alpha
+    S.erfc_sgrid = sgrid
+    S.erfc_n = nerf
+    S.erfc_tab = C.cast(tab, C.POINTER(abi.Erfc))
+    S.ncell = (C.c_int * 3)(*ncell)
+    S.mass = bs.ptr(np.array(mass), C.c_double)
+    S.ncons = len(cons_bondq)
+    S.cons_first = bs.ptr(np.array(cons_first, np.int32), C.c_int)
+    S.cons_ij = bs.ptr(np.array(cons_ij if cons_ij else [0, 0], np.int32), C.c_int)
+    S.cons_bondq = bs.ptr(np.array(cons_bondq if cons_bondq else [1.0]), C.c_double)
+    S.mol_omega = bs.ptr(np.full(max(nmol, 1), omega), C.c_double)
+    # No.f: 3 per site minus constraints (cook subtracts 3 for the total
+    # momentum only when it is conserved; callers can override)
+    bs.info = dict(alpha=alpha, kappa=kappa, nk=nk, ncell=list(ncell),
+                   erfc_n=nerf, erfc_sgrid=sgrid, qmax=qmax, sumq=sumq,
+                   ns=ns_tot, nmol=nmol, ncons=len(cons_bondq),
+                   corr=corr, sstab=sstab, erfc_tab=tab,
+                   mass=np.array(mass), charge=q,
+                   site_type=np.array(site_type, np.int32),
+                   mol_first=np.array(mol_first, np.int32),
+                   excl=np.array(excl, np.uint32),
+                   cons_first=np.array(cons_first, np.int32),
+                   cons_ij=np.array(cons_ij, np.int32).reshape(-1, 2),
+                   cons_bondq=np.array(cons_bondq))
+    return bs

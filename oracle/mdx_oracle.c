@@ -1,0 +1,876 @@
+/*
+ * mdx_oracle.c -- CPU FP64 restatement of cook's force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the MI355X engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  It is never linked into the engine (macsimus_amd/), and the
+ * engine never falls back to it.
+ *
+ * It restates, in plain C, the reference's algorithm for cook's force call
+ * with the compile-time variant COULOMB=-2, SPLINE=-2, LINKCELL (serial),
+ * WORM, PRESSURETENSOR=7, nonpolar LJ:
+ *   combining rule + smooth cutoff  sim/lj/sitesite.c:66-112, sim/setss.c:137-306
+ *   tail correction                 sim/setss.c:455-501
+ *   erfc hyperbolic splines         sim/erfc.c:68-191, sim/splines.c:23-158
+ *   exact exclusion erf             sim/erexact.c:25-53
+ *   Ewald alpha/kappa auto set      sim/ewald.c:1027-1067
+ *   k-vector set, almostint         sim/ewald.c:240-250, 484-534
+ *   Ewald pass 1 / pass 2m          sim/ewald.c:562-692, sim/ewpass1.c, sim/ewpass2m.c:189-300
+ *   dipole term, self term          sim/ewald.c:865-997, sim/ewaldf.c
+ *   linked-cell list                cook/linklist.c:15-29, 37-269, 294-337
+ *   half-shell traversal            cook/lc.c:217-460
+ *   LJ + Coulomb pair kernel        cook/ljqqnp.c:43-259, sim/intermac.h:47-64
+ *   SHAKE + leap-frog + Berendsen   sim/constrd.c:622-1081, sim/verlet.c, sim/virc.c,
+ *                                   sim/shakev3.c, sim/thermo.c:141-160, sim/constrd.c:98-107
+ * Everything is double precision, serial except the k-space sums, which
+ * split the charges over OpenMP threads as cook's PARALLEL=1 does
+ * (sim/ewald.c:197-244).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "../include/mdx.h"
+
+#define PI 3.14159265358979323846
+#define SQRTPI 1.772453850905516
+
+static double sqr(double x) { return x * x; }
+static double pow6(double x) { double y = x * x * x; return y * y; }
+
+/* ------------------------------------------------------------------ LJ */
+
+/* sigvdW/epsvdW of a pair (sim/lj/sitesite.c:66-93), square-root energy
+   rule (comb_rule&1, or alpha<=0) with additive (0) or geometric (2)
+   diameters.  EvdW < 0 in K. */
+static void orc_combrule(double *sig, double *eps, double Ei, double Ri,
+                         double Ej, double Rj, int comb_rule)
+{
+  double r, A, B, Esi = sqrt(-2 * Ei), Esj = sqrt(-2 * Ej);
+  if (comb_rule & 2) { r = 4 * Ri * Rj; r = r * r * r; }
+  else r = pow6(Ri + Rj);
+  B = Esi * Esj * r;
+  A = B * r;
+  *sig = B == 0 ? 0 : pow(A / B, 1. / 6);
+  *eps = A == 0 ? 0 : B * B / (-2 * A);
+}
+
+static double orc_U(double sig, double eps, double r)
+{ double x = pow6(sig / r); return -eps * (x * x - 2 * x); }
+static double orc_F(double sig, double eps, double r)
+{ double x = pow6(sig / r); return -12 * eps * (x * x - x) / r; }
+
+int oracle_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej, double Rj,
+                 int comb_rule, double C2, double *corr)
+{
+  const double neps = 1.1102230246251565e-16; /* No.eps = machprec() */
+  double sig, eps, C1, oldC1;
+  int n;
+  orc_combrule(&sig, &eps, Ei, Ri, Ej, Rj, comb_rule);
+  memset(ss, 0, sizeof(*ss));
+  ss->Sq = sig * sig * 0.7937005259840998;
+  ss->E4 = -4 * eps;
+  if (ss->Sq != 0) { ss->E24 = -24 * eps / ss->Sq; ss->E48 = -48 * eps / ss->Sq; }
+  if (eps == 0 || C2 == 0) {           /* no LJ: C1=C2=A=0 */
+    if (corr) *corr = 0;
+    return 0;                          /* A4 left 0 (cook leaves it NaN, never read) */
+  }
+  C1 = 0.775 * C2;
+  n = 0;
+  do {
+    double den = orc_F(sig, eps, C1);
+    if (fabs(den) < 1e-50) return MDX_E_ARG;
+    oldC1 = C1;
+    C1 = 1 + 4 * orc_U(sig, eps, C1) / C1 / den;
+    if (n++ > 2000 || C1 < 0) return MDX_E_ARG;
+    C1 = C2 / sqrt(C1);
+  } while (fabs(1 - C1 / oldC1) > neps * n);
+  ss->A = orc_U(sig, eps, C1) / sqr(C2 * C2 - C1 * C1);
+  ss->A4 = -orc_F(sig, eps, C1) / (C2 * C2 - C1 * C1) / C1;
+  ss->C2q = C2 * C2;
+  ss->C1q = C1 * C1;
+  if (corr) {
+    /* 4 pi int_C1^C2 A(r^2-C2^2)^2 r^2 analytically (with minus sign) +
+       4 pi int_C1^inf U r^2 by 2-point Gauss in x=1/r, NGauss4 intervals */
+    const int NG = 100;
+    const double g4 = 0.2113248654051871;
+    double c0 = 4 * PI * ss->A *
+      (C1 * (pow6(C1) / 7 - 0.4 * sqr(C1) * sqr(C1) * sqr(C2) + sqr(C1) * sqr(C2) * sqr(C2) / 3)
+       - 8.0 / 105 * pow(C2, 7));
+    double h = 1.0 / NG / C1, c1 = 0, r;
+    int k;
+    for (k = 0; k < NG; k++) {
+      r = 1 / ((k + g4) * h); c1 += sqr(r * r) * orc_U(sig, eps, r);
+      r = 1 / ((k + (1 - g4)) * h); c1 += sqr(r * r) * orc_U(sig, eps, r);
+    }
+    *corr = c0 + c1 * 2 * PI * h;
+  }
+  return 0;
+}
+
+/* --------------------------------------------------------- erfc splines */
+
+int oracle_erfc_table(mdx_erfc_t *tab, int *nout, double *sgridout, int grid,
+                      double maxr, double cutoff, double alpha, int shift)
+{
+  int ito = (int)(sqr(maxr * alpha) * grid) + 1, n = ito, i;
+  double h = 1.0 / grid, Du, Dd, x, xh, z, sgrid;
+  double (*f)[2] = malloc(sizeof(double[2]) * (n + 3));
+  if (!f) return MDX_E_ARG;
+  for (i = n + 2; i >= 0; i--) {        /* erfcs(), SUBGRID=1 */
+    if (i) {
+      double rr = i * h, r = sqrt(rr), er = erfc(r);
+      f[i][0] = er / r;
+      f[i][1] = ((2 / SQRTPI) * exp(-rr) + er / r) / rr;
+    } else f[i][0] = f[i][1] = 9e9;
+  }
+  Du = (-f[n + 2][0] + 8 * f[n + 1][0] - 8 * f[n - 1][0] + f[n - 2][0]) / (12 * h);
+  Dd = (-f[n + 2][1] + 8 * f[n + 1][1] - 8 * f[n - 1][1] + f[n - 2][1]) / (12 * h);
+  x = n * h;
+  while (ito--) {
+    mdx_erfc_t *p = tab + ito;
+    double du, dd;
+    xh = x;
+    i = ito;
+    x = i * h;
+    du = f[i + 1][0] - f[i][0];
+    p->Cu = (x * du - xh * Du * h) / (Du * h - du);
+    p->Bu = -Du * sqr(p->Cu + xh);
+    p->Au = f[i + 1][0] + Du * (p->Cu + xh);
+    Du = -p->Bu / sqr(p->Cu + x);
+    dd = f[i + 1][1] - f[i][1];
+    p->Cd = (x * dd - xh * Dd * h) / (Dd * h - dd);
+    p->Bd = -Dd * sqr(p->Cd + xh);
+    p->Ad = f[i + 1][1] + Dd * (p->Cd + xh);
+    Dd = -p->Bd / sqr(p->Cd + x);
+    p->Au *= alpha; p->Ad *= alpha * alpha * alpha;
+    p->Bu /= alpha; p->Bd *= alpha;
+    p->Cu /= sqr(alpha); p->Cd /= sqr(alpha);
+  }
+  free(f);
+  sgrid = grid * sqr(alpha);
+  z = cutoff * cutoff;
+  if (shift & 1) {
+    mdx_erfc_t *p = tab + (int)(z * sgrid);
+    double s = p->Bu / (p->Cu + z) + p->Au;
+    for (i = 0; i < n; i++) tab[i].Au -= s;
+  }
+  if (shift & 2) {
+    mdx_erfc_t *p = tab + (int)(z * sgrid);
+    double s = p->Bd / (p->Cd + z) + p->Ad;
+    for (i = 0; i < n; i++) tab[i].Ad -= s;
+  }
+  *nout = n;
+  *sgridout = sgrid;
+  return 0;
+}
+
+/* exacterud_sqrt(): -erf(alpha r)/r and its force/r (SUBGRID=1 branch) */
+static double orc_exact(double rr, double alpha, double *erd)
+{
+  double alphaq = alpha * alpha, A = alpha * (-2 / SQRTPI), B = A * alphaq * 2;
+  double err = alphaq * rr;
+  if (err < 0.01) {
+    int n = 1;
+    double su = 0, sd = 0, q = 1, r;
+    for (;;) {
+      r = q / (2 * n + 1);
+      sd += r * n;
+      su += r;
+      if (fabs(q) < 1e-16) break;
+      q *= -err / ++n;
+    }
+    *erd = B * sd;
+    return A * (1 - su * err);
+  } else {
+    double r = sqrt(rr), y = -erf(alpha * r) / r;
+    *erd = (y - A * exp(-err)) / rr;
+    return y;
+  }
+}
+
+/* ------------------------------------------------------- Ewald set-up */
+
+int oracle_ewald_autoset(double *alpha, double *kappa, double qmax, double sumq,
+                         double cutoff, const double L[3], double epsr, double epsk)
+{
+  double V = L[0] * L[1] * L[2], a, k = *kappa, x;
+  int nit = 10000;
+  a = log(2 * qmax * sqrt(sumq / cutoff / V) / epsr);
+  if (a < 0) return MDX_E_ARG;
+  a = sqrt(a) / cutoff;
+  do {
+    if (!nit--) return MDX_E_NOCONV;
+    x = k;
+    k = log(qmax * a / (PI * epsk) * sqrt(8 * sumq / (k * V)));
+    if (k <= 0) return MDX_E_ARG;
+    k = a * sqrt(k) / PI;
+  } while (fabs(x - k) > 1e-9);
+  *alpha = a;
+  *kappa = k;
+  return 0;
+}
+
+static int orc_almostint(double x)
+{
+  x -= (int)x;
+  return fabs(0.5 - fabs(x - 0.5)) < 1e-9;
+}
+
+int oracle_ewald_kvectors(double *kappa, const double L[3], int *nk)
+{
+  double dK = 2e-9;
+  int nagain = 0;
+again:
+  if (nagain++ > 25) return MDX_E_NOCONV;
+  {
+    double Kk = sqr(*kappa);
+    int kx, ky, kz, kxmax = (int)(L[0] * *kappa), iQ = 0;
+    for (kx = 0; kx <= kxmax; kx++) {
+      double kxq = Kk - sqr(kx / L[0]), aux = L[1] * sqrt(kxq);
+      int kymax;
+      if (orc_almostint(aux)) { *kappa += *kappa * dK; dK *= 1.4; goto again; }
+      kymax = (int)aux;
+      for (ky = 0; ky <= kymax; ky++) {
+        double kyq = kxq - sqr(ky / L[1]);
+        int kzmax;
+        aux = L[2] * sqrt(kyq);
+        if (orc_almostint(aux)) { *kappa += *kappa * dK; dK *= 1.4; goto again; }
+        kzmax = (int)aux;
+        for (kz = !(kx + ky); kz <= kzmax; kz++)
+          iQ += 1 << ((kx > 0) + (ky > 0) + (kz > 0) - 1);
+      }
+    }
+    *nk = iQ;
+  }
+  return 0;
+}
+
+static int orc_logint(double x)
+{
+  int i = x + 0.9999;
+  if (i > 1)
+    if (x / (i - 1) < i / x) i--;
+  return i;
+}
+
+int oracle_lcsetup(int ncell[3], int ns, double percell, const double L[3])
+{
+  int i, prdc = 1, nundef = 0;
+  double prdL = 1;
+  if (percell <= 0) percell = 1 + log((double)ns);
+  for (i = 0; i < 3; i++)
+    if (ncell[i] == 0) { nundef++; prdL *= L[i]; }
+    else prdc *= ncell[i];
+  for (i = 0; i < 3; i++)
+    if (ncell[i] == 0) {
+      double x = pow(ns / (percell * prdc * prdL), 1. / nundef);
+      ncell[i] = orc_logint(x * L[i]);
+      if (ncell[i] < 2) ncell[i] = 2;
+      prdL /= L[i];
+      prdc *= ncell[i];
+      nundef--;
+    }
+  return 0;
+}
+
+/* ------------------------------------------------------------- Ewald */
+
+typedef struct { double re, im; } cplx;
+
+typedef struct {
+  int kxmax, *kymax, **kzmax;   /* ktab */
+  int ntrip, nk;
+} orc_kset;
+
+static void orc_kset_build(orc_kset *K, double kappa, const double L[3])
+{
+  double Kk = kappa * kappa;
+  int kx, ky, kz;
+  K->kxmax = (int)(L[0] * kappa);
+  K->kymax = malloc(sizeof(int) * (K->kxmax + 1));
+  K->kzmax = malloc(sizeof(int *) * (K->kxmax + 1));
+  K->ntrip = K->nk = 0;
+  for (kx = 0; kx <= K->kxmax; kx++) {
+    double kxq = Kk - sqr(kx / L[0]);
+    K->kymax[kx] = (int)(L[1] * sqrt(kxq));
+    K->kzmax[kx] = malloc(sizeof(int) * (K->kymax[kx] + 1));
+    for (ky = 0; ky <= K->kymax[kx]; ky++) {
+      double kyq = kxq - sqr(ky / L[1]);
+      K->kzmax[kx][ky] = (int)(L[2] * sqrt(kyq));
+      for (kz = !(kx + ky); kz <= K->kzmax[kx][ky]; kz++) {
+        K->ntrip++;
+        K->nk += 1 << ((kx > 0) + (ky > 0) + (kz > 0) - 1);
+      }
+    }
+  }
+}
+
+static void orc_kset_free(orc_kset *K)
+{
+  int kx;
+  for (kx = 0; kx <= K->kxmax; kx++) free(K->kzmax[kx]);
+  free(K->kzmax);
+  free(K->kymax);
+}
+
+#define CXE(A, E) { double c_ = (A).re; (A).re = c_ * (E).re - (A).im * (E).im; (A).im = c_ * (E).im + (A).im * (E).re; }
+#define CXIE(A, E) { double c_ = (A).re; (A).re = c_ * (E).re + (A).im * (E).im; (A).im = (A).im * (E).re - c_ * (E).im; }
+
+/* per charge recursion state: the qtab_t of sim/ewald.c:115-138 */
+typedef struct { cplx x, xy, xY, xyz, xyZ, xYz, xYZ; } orc_qtab;
+
+/* One thread's share [i0,i1) of the charges.  pass 1 sums Q(k) into Qk;
+   pass 2 adds q*ek*Im(conj(Q) e) to F. Loop order and recursion as in
+   sim/ewpass1.c and sim/ewpass2m.c. */
+static void orc_ewald_pass(int pass, const orc_kset *K, const double *L,
+                           int i0, int i1, const double *q, const cplx *ex,
+                           const cplx *ey, const cplx *ez, orc_qtab *qt,
+                           cplx *Qk, const double *ekk, double (*F)[3])
+{
+  int kx, ky, kz, iQ = 0, ikk = 0, i;
+  for (i = i0; i < i1; i++) { qt[i].x.re = q[i]; qt[i].x.im = 0; }
+  for (kx = 0; kx <= K->kxmax; kx++) {
+    if (kx) for (i = i0; i < i1; i++) CXE(qt[i].x, ex[i]);
+    for (i = i0; i < i1; i++) qt[i].xy = qt[i].xY = qt[i].x;
+    for (ky = 0; ky <= K->kymax[kx]; ky++) {
+      if (ky) for (i = i0; i < i1; i++) { CXE(qt[i].xy, ey[i]); CXIE(qt[i].xY, ey[i]); }
+      for (i = i0; i < i1; i++) {
+        qt[i].xyz = qt[i].xyZ = qt[i].xy;
+        qt[i].xYz = qt[i].xYZ = qt[i].xY;
+      }
+      for (kz = !(kx + ky); kz <= K->kzmax[kx][ky]; kz++) {
+        int ncase = (kx > 0) + (ky > 0) + (kz > 0);
+        if (kz) for (i = i0; i < i1; i++) {
+          CXE(qt[i].xyz, ez[i]); CXIE(qt[i].xyZ, ez[i]);
+          CXE(qt[i].xYz, ez[i]); CXIE(qt[i].xYZ, ez[i]);
+        }
+        if (pass == 1) {
+          cplx A = {0, 0}, B = {0, 0}, C = {0, 0}, D = {0, 0};
+          if (ncase == 3) {
+            for (i = i0; i < i1; i++) {
+              A.re += qt[i].xyz.re; A.im += qt[i].xyz.im;
+              B.re += qt[i].xyZ.re; B.im += qt[i].xyZ.im;
+              C.re += qt[i].xYz.re; C.im += qt[i].xYz.im;
+              D.re += qt[i].xYZ.re; D.im += qt[i].xYZ.im;
+            }
+            Qk[iQ++] = A; Qk[iQ++] = B; Qk[iQ++] = C; Qk[iQ++] = D;
+          } else if (ncase == 2) {
+            if (kz == 0)
+              for (i = i0; i < i1; i++) {
+                A.re += qt[i].xyz.re; A.im += qt[i].xyz.im;
+                B.re += qt[i].xYz.re; B.im += qt[i].xYz.im;
+              }
+            else
+              for (i = i0; i < i1; i++) {
+                A.re += qt[i].xyz.re; A.im += qt[i].xyz.im;
+                B.re += qt[i].xyZ.re; B.im += qt[i].xyZ.im;
+              }
+            Qk[iQ++] = A; Qk[iQ++] = B;
+          } else {
+            for (i = i0; i < i1; i++) { A.re += qt[i].xyz.re; A.im += qt[i].xyz.im; }
+            Qk[iQ++] = A;
+          }
+        } else {
+          double e = ekk[ikk], ek[3];
+          ek[0] = e * (kx / L[0]); ek[1] = e * (ky / L[1]); ek[2] = e * (kz / L[2]);
+          if (ncase == 3) {
+            cplx A = Qk[iQ], B = Qk[iQ + 1], C = Qk[iQ + 2], D = Qk[iQ + 3];
+            iQ += 4;
+            for (i = i0; i < i1; i++) {
+              double FA = A.re * qt[i].xyz.im - A.im * qt[i].xyz.re;
+              double FB = B.re * qt[i].xyZ.im - B.im * qt[i].xyZ.re;
+              double FC = C.re * qt[i].xYz.im - C.im * qt[i].xYz.re;
+              double FD = D.re * qt[i].xYZ.im - D.im * qt[i].xYZ.re;
+              F[i][0] += ek[0] * ((FA + FB) + (FC + FD));
+              F[i][1] += ek[1] * ((FA + FB) - (FC + FD));
+              F[i][2] += ek[2] * (FA - FB + FC - FD);
+            }
+          } else if (ncase == 2) {
+            cplx A = Qk[iQ], B = Qk[iQ + 1];
+            iQ += 2;
+            if (kz) {
+              for (i = i0; i < i1; i++) {
+                double FA = A.re * qt[i].xyz.im - A.im * qt[i].xyz.re;
+                double FB = B.re * qt[i].xyZ.im - B.im * qt[i].xyZ.re;
+                if (kx) F[i][0] += ek[0] * (FA + FB);
+                else F[i][1] += ek[1] * (FA + FB);
+                F[i][2] += ek[2] * (FA - FB);
+              }
+            } else {
+              for (i = i0; i < i1; i++) {
+                double FA = A.re * qt[i].xyz.im - A.im * qt[i].xyz.re;
+                double FB = B.re * qt[i].xYz.im - B.im * qt[i].xYz.re;
+                F[i][0] += ek[0] * (FA + FB);
+                F[i][1] += ek[1] * (FA - FB);
+              }
+            }
+          } else {
+            cplx A = Qk[iQ++];
+            for (i = i0; i < i1; i++) {
+              double FA = A.re * qt[i].xyz.im - A.im * qt[i].xyz.re;
+              if (kx) F[i][0] += ek[0] * FA;
+              else if (ky) F[i][1] += ek[1] * FA;
+              else F[i][2] += ek[2] * FA;
+            }
+          }
+        }
+        ikk++;
+      }
+    }
+  }
+}
+
+/* k-space part: returns En.el of Ewald(2) (measure) and adds forces of
+   the charges to f[idx[i]]; energy/tensor bookkeeping of pass 2m */
+static double orc_ewald(const mdx_system_t *S, int nq, const int *idx,
+                        const double *q, const double (*pos)[3], double (*f)[3],
+                        int measure, double Pvir[6], int nthreads)
+{
+  orc_kset K;
+  const double *L = S->L;
+  double V = L[0] * L[1] * L[2], PIaq = -sqr(PI / S->alpha);
+  double *ekk, Energy = 0, PVIR[6] = {0, 0, 0, 0, 0, 0}, sumq = 0, M[3] = {0, 0, 0};
+  double PIL[3] = {2 * PI / L[0], 2 * PI / L[1], 2 * PI / L[2]};
+  cplx *ex, *ey, *ez, *Qk, *Qth;
+  orc_qtab *qt;
+  double (*F)[3];
+  int i, kx, ky, kz, ikk, iQ, t;
+
+  orc_kset_build(&K, S->kappa, L);
+  ekk = malloc(sizeof(double) * K.ntrip);
+  ikk = 0;
+  for (kx = 0; kx <= K.kxmax; kx++)
+    for (ky = 0; ky <= K.kymax[kx]; ky++) {
+      double aux = sqr(ky / L[1]) + sqr(kx / L[0]);
+      for (kz = !(kx + ky); kz <= K.kzmax[kx][ky]; kz++) {
+        double kk = sqr(kz / L[2]) + aux;
+        ekk[ikk++] = 4 / (kk * (L[0] * L[1] * L[2])) * exp(PIaq * kk);
+      }
+    }
+  ex = malloc(sizeof(cplx) * nq); ey = malloc(sizeof(cplx) * nq); ez = malloc(sizeof(cplx) * nq);
+  qt = malloc(sizeof(orc_qtab) * nq);
+  F = calloc(nq, sizeof(*F));
+  Qk = calloc(K.nk, sizeof(cplx));
+  Qth = calloc((size_t)K.nk * nthreads, sizeof(cplx));
+  for (i = 0; i < nq; i++) {
+    double a;
+    M[0] += q[i] * pos[i][0]; M[1] += q[i] * pos[i][1]; M[2] += q[i] * pos[i][2];
+    ex[i].re = cos(a = PIL[0] * pos[i][0]); ex[i].im = sin(a);
+    ey[i].re = cos(a = PIL[1] * pos[i][1]); ey[i].im = sin(a);
+    ez[i].re = cos(a = PIL[2] * pos[i][2]); ez[i].im = sin(a);
+    sumq += q[i] * q[i];
+  }
+  /* pass 1: per-thread Q(k), summed serially (sim/ewald.c:756-780) */
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (t = 0; t < nthreads; t++)
+    orc_ewald_pass(1, &K, L, (int)((long)nq * t / nthreads), (int)((long)nq * (t + 1) / nthreads),
+                   q, ex, ey, ez, qt, Qth + (size_t)K.nk * t, ekk, F);
+  for (t = 0; t < nthreads; t++)
+    for (iQ = 0; iQ < K.nk; iQ++) {
+      Qk[iQ].re += Qth[(size_t)K.nk * t + iQ].re;
+      Qk[iQ].im += Qth[(size_t)K.nk * t + iQ].im;
+    }
+  /* pass 2: forces */
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (t = 0; t < nthreads; t++)
+    orc_ewald_pass(2, &K, L, (int)((long)nq * t / nthreads), (int)((long)nq * (t + 1) / nthreads),
+                   q, ex, ey, ez, qt, Qk, ekk, F);
+  /* energy and tensor (ewpass2m.c:194-297) */
+  if (measure) {
+    ikk = iQ = 0;
+    for (kx = 0; kx <= K.kxmax; kx++)
+      for (ky = 0; ky <= K.kymax[kx]; ky++)
+        for (kz = !(kx + ky); kz <= K.kzmax[kx][ky]; kz++) {
+          double e = ekk[ikk++], kk, ptf, E1;
+          int ncase = (kx > 0) + (ky > 0) + (kz > 0);
+          double a2, b2 = 0, c2 = 0, d2 = 0, kxl = kx / L[0], kyl = ky / L[1], kzl = kz / L[2];
+          kk = sqr(kxl) + sqr(kyl) + sqr(kzl);
+          ptf = 1 / kk - PIaq;
+          a2 = sqr(Qk[iQ].re) + sqr(Qk[iQ].im);
+          if (ncase >= 2) b2 = sqr(Qk[iQ + 1].re) + sqr(Qk[iQ + 1].im);
+          if (ncase == 3) {
+            c2 = sqr(Qk[iQ + 2].re) + sqr(Qk[iQ + 2].im);
+            d2 = sqr(Qk[iQ + 3].re) + sqr(Qk[iQ + 3].im);
+          }
+          iQ += 1 << (ncase - 1);
+          Energy += E1 = (a2 + b2 + c2 + d2) * e;
+          PVIR[0] += E1 * ptf * sqr(kxl);
+          PVIR[1] += E1 * ptf * sqr(kyl);
+          PVIR[2] += E1 * ptf * sqr(kzl);
+          if (ncase == 3) {
+            PVIR[3] += e * ptf * (a2 - b2 - c2 + d2) * kyl * kzl;
+            PVIR[4] += e * ptf * (a2 - b2 + c2 - d2) * kxl * kzl;
+            PVIR[5] += e * ptf * (a2 + b2 - c2 - d2) * kxl * kyl;
+          } else if (ncase == 2) {
+            if (kz) {
+              PVIR[3] += e * ptf * (a2 - b2) * kyl * kzl;
+              PVIR[4] += e * ptf * (a2 - b2) * kxl * kzl;
+              PVIR[5] += e * ptf * (a2 + b2) * kxl * kyl;
+            } else {
+              PVIR[3] += e * ptf * (a2 - b2) * kyl * kzl;
+              PVIR[4] += e * ptf * (a2 + b2) * kxl * kzl;
+              PVIR[5] += e * ptf * (a2 - b2) * kxl * kyl;
+            }
+          } else {
+            PVIR[3] += e * ptf * a2 * kyl * kzl;
+            PVIR[4] += e * ptf * a2 * kxl * kzl;
+            PVIR[5] += e * ptf * a2 * kxl * kyl;
+          }
+        }
+    Energy /= 4 * PI;
+    for (i = 0; i < 3; i++) Pvir[i] += Energy;
+    for (i = 0; i < 6; i++) Pvir[i] -= 2 / (4 * PI) * PVIR[i];
+  }
+  /* dipole term, tin-foil limit for epsinf -> inf (sim/ewald.c:865-897) */
+  {
+    double c = 4 * PI / (2 * S->epsinf + 1) / V, ek[3], Edip;
+    ek[0] = -c * M[0]; ek[1] = -c * M[1]; ek[2] = -c * M[2];
+    Edip = c / 2 * (sqr(M[0]) + sqr(M[1]) + sqr(M[2]));
+    if (measure) for (i = 0; i < 3; i++) Pvir[i] += Edip;
+    for (i = 0; i < nq; i++) {
+      double *fi = f[idx[i]];
+      fi[0] += F[i][0] + q[i] * ek[0];
+      fi[1] += F[i][1] + q[i] * ek[1];
+      fi[2] += F[i][2] + q[i] * ek[2];
+      if (measure) {
+        Pvir[0] += q[i] * ek[0] * pos[i][0];
+        Pvir[1] += q[i] * ek[1] * pos[i][1];
+        Pvir[2] += q[i] * ek[2] * pos[i][2];
+        Pvir[3] += q[i] * ek[1] * pos[i][2];
+        Pvir[4] += q[i] * ek[2] * pos[i][0];
+        Pvir[5] += q[i] * ek[0] * pos[i][1];
+      }
+    }
+    Energy += Edip;
+  }
+  free(ex); free(ey); free(ez); free(qt); free(F); free(Qk); free(Qth); free(ekk);
+  orc_kset_free(&K);
+  if (!measure) return 0;
+  return Energy - S->alphar / sqrt(PI) * sumq;
+}
+
+/* -------------------------------------------------------- linked cells */
+
+static double orc_minuseps(double x)
+{
+  double y = x * 0.999999999999999667, z;
+  int i;
+  for (i = 0; i < 5; i++) {
+    volatile double zz = (x + y) / 2;
+    z = zz;
+    if (z >= x) return y;
+    y = z;
+  }
+  return y;
+}
+
+typedef struct orc_site_s {
+  double r[3], f[3];
+  int i, n, st;
+  double q;
+  struct orc_site_s *next;
+} orc_site;
+
+typedef struct {
+  const mdx_system_t *S;
+  double U, Enel, Envir, Pvir[6];
+  long long npair, nlj;
+  int measure;
+} orc_acc;
+
+/* LJqq[m] for the central site ls against the list l (cook/ljqqnp.c) */
+static void orc_ljqq(orc_acc *a, orc_site *l, orc_site *ls)
+{
+  const mdx_system_t *S = a->S;
+  double cq = S->cutoff * S->cutoff;
+  for (; l; l = l->next) {
+    double dr[3], rr, f = 0, U = 0, qq, fdr;
+    const mdx_lj_t *ss;
+    dr[0] = ls->r[0] - l->r[0];
+    dr[1] = ls->r[1] - l->r[1];
+    dr[2] = ls->r[2] - l->r[2];
+    rr = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    if (rr >= cq) continue;
+    a->npair++;
+    ss = S->sstab + ls->st * S->ntypes + l->st;
+    if (ls->n == l->n) {
+      int m0 = S->mol_first[ls->n];
+      int ii = ls->i - m0, jj = l->i - m0;
+      if ((S->excl[ls->i] >> jj & 1) || (S->excl[l->i] >> ii & 1)) {
+        /* EXCL: exact Ewald erf correction (cook/ljqqnp.c:127-144) */
+        if ((qq = ls->q * l->q) != 0) {
+          double erd, u = orc_exact(rr, S->alphar, &erd);
+          if (a->measure) a->Enel += u * qq;
+          f = erd * qq;
+        }
+        goto addUf;
+      }
+    }
+    if (rr < ss->C2q) {
+      if (rr < ss->C1q) {
+        double x = ss->Sq / rr, y = x * x * x, z = y * y;
+        U = ss->E4 * (z - y);
+        f = ss->E24 * (z + z - y) * x;
+      } else {
+        double x = rr - ss->C2q;
+        U = x * x * ss->A;
+        f = ss->A4 * x;
+      }
+      a->Envir += rr * f;
+      a->nlj += ss->A != 0;
+    }
+    if ((qq = ls->q * l->q) != 0) {
+      const mdx_erfc_t *p = S->erfc_tab + (int)(rr * S->erfc_sgrid);
+      double byerd = p->Bd / (p->Cd + rr) + p->Ad;
+      double eru = p->Bu / (p->Cu + rr) + p->Au;
+      if (a->measure) a->Enel += eru * qq;
+      f += byerd * qq;
+    }
+  addUf:
+    a->U += U;
+    ls->f[0] += fdr = f * dr[0]; l->f[0] -= fdr;
+    if (a->measure) { a->Pvir[0] += dr[0] * fdr; a->Pvir[5] += dr[1] * fdr; }
+    ls->f[1] += fdr = f * dr[1]; l->f[1] -= fdr;
+    if (a->measure) { a->Pvir[1] += dr[1] * fdr; a->Pvir[3] += dr[2] * fdr; }
+    ls->f[2] += fdr = f * dr[2]; l->f[2] -= fdr;
+    if (a->measure) { a->Pvir[2] += dr[2] * fdr; a->Pvir[4] += dr[0] * fdr; }
+  }
+}
+
+/* linklist() + serial linkcell(): half-shell traversal of cook/lc.c:217-460
+   (periodic z, no SLIT).  Sites get cell ids from FP64 wrapped coordinates
+   with the shaved rL of cook/linklist.c:81-86. Returns 0 or MDX_E_RANGE. */
+static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cellid)
+{
+  const mdx_system_t *S = a->S;
+  const int *nc = S->ncell;
+  int ns = S->ns, i, n, k;
+  double rL[3], cell[3], cq = S->cutoff * S->cutoff;
+  orc_site *st = calloc(ns, sizeof(orc_site));
+  orc_site **list = calloc((size_t)nc[0] * nc[1] * nc[2], sizeof(orc_site *));
+  int ixs, iys, izs, err = 0;
+#define LIST(X, Y, Z) list[((size_t)(X) * nc[1] + (Y)) * nc[2] + (Z)]
+  for (k = 0; k < 3; k++) {
+    double x;
+    volatile double t = nc[k] / S->L[k];
+    rL[k] = t;
+    x = orc_minuseps(S->L[k]);
+    while ((int)(x * rL[k]) >= nc[k]) rL[k] = orc_minuseps(rL[k]);
+  }
+  for (n = 0; n < S->nmol; n++)
+    for (i = S->mol_first[n]; i < S->mol_first[n + 1]; i++) {
+      orc_site *l = st + i;
+      int ic[3];
+      for (k = 0; k < 3; k++) {
+        l->r[k] = r[i][k];
+        while (l->r[k] >= S->L[k]) l->r[k] -= S->L[k];
+        ic[k] = (int)(l->r[k] * rL[k]);
+        if (ic[k] >= nc[k] || ic[k] < 0) err = MDX_E_RANGE;
+      }
+      if (err) break;
+      if (cellid) cellid[i] = (ic[0] * nc[1] + ic[1]) * nc[2] + ic[2];
+      l->i = i; l->n = n; l->st = S->site_type[i]; l->q = S->charge[i];
+      l->next = LIST(ic[0], ic[1], ic[2]);
+      LIST(ic[0], ic[1], ic[2]) = l;
+    }
+  if (err) { free(st); free(list); return err; }
+  for (k = 0; k < 3; k++) { cell[k] = S->L[k] / nc[k]; rL[k] = nc[k] / S->L[k]; }
+
+  for (ixs = 0; ixs < nc[0]; ixs++)
+    for (iys = 0; iys < nc[1]; iys++)
+      for (izs = 0; izs < nc[2]; izs++) {
+        orc_site *ls;
+        for (ls = LIST(ixs, iys, izs); ls; ls = ls->next) {
+          orc_site lsloc = *ls;
+          double cellr[3], DX, DY, dy, dz, dyq;
+          int ix, ixm, ix1, iy, iym, iy0, iy1, iz, izm, iz0, iz1;
+          for (k = 0; k < 3; k++) cellr[k] = rL[k] * lsloc.r[k];
+          ix1 = cellr[0] + S->cutoff * rL[0];
+          for (ix = ixm = ixs; ix <= ix1; ix++) {
+            DX = ix == ixs ? 0 : ix - cellr[0];
+            if (ixm == nc[0]) { lsloc.r[0] -= S->L[0]; ixm -= nc[0]; }
+            dyq = cq - sqr(DX * cell[0]);
+            if (dyq < 0) { if (dyq / cq < -0.1) err = MDX_E_RANGE; dyq = 0; }
+            dy = sqrt(dyq) * rL[1];
+            if (ixm == ixs) iy0 = iym = iys;
+            else {
+              iym = iy0 = (int)(cellr[1] + nc[1] - dy) - nc[1];
+              if (iym < 0) { lsloc.r[1] += S->L[1]; iym += nc[1]; }
+            }
+            if (iym < 0 || iym > nc[1]) err = MDX_E_RANGE;
+            iy1 = (int)(cellr[1] + dy);
+            for (iy = iy0; iy <= iy1; iy++) {
+              if (iym == nc[1]) { lsloc.r[1] -= S->L[1]; iym -= nc[1]; }
+              if (iy > iys) DY = iy - cellr[1];
+              else if (iy < iys) DY = iy + 1 - cellr[1];
+              else DY = 0;
+              dz = dyq - sqr(DY * cell[1]);
+              if (dz < 0) { if (dz / cq < -0.1) err = MDX_E_RANGE; dz = 0; }
+              dz = sqrt(dz) * rL[2];
+              iz1 = (int)(cellr[2] + dz);
+              if (ixm == ixs && iym == iys) {
+                izm = iz0 = izs;
+                for (iz = iz0; iz <= iz1; iz++) {
+                  if (izm == nc[2]) { lsloc.r[2] -= S->L[2]; izm -= nc[2]; }
+                  if (izm == izs) orc_ljqq(a, lsloc.next, &lsloc);
+                  else orc_ljqq(a, LIST(ixm, iym, izm), &lsloc);
+                  izm++;
+                }
+              } else {
+                izm = iz0 = (int)(cellr[2] + nc[2] - dz) - nc[2];
+                if (izm < 0) { lsloc.r[2] += S->L[2]; izm += nc[2]; }
+                for (iz = iz0; iz <= iz1; iz++) {
+                  if (izm == nc[2]) { lsloc.r[2] -= S->L[2]; izm -= nc[2]; }
+                  orc_ljqq(a, LIST(ixm, iym, izm), &lsloc);
+                  izm++;
+                }
+              }
+              lsloc.r[2] = ls->r[2];
+              iym++;
+            }
+            lsloc.r[1] = ls->r[1];
+            ixm++;
+          }
+          ls->f[0] = lsloc.f[0]; ls->f[1] = lsloc.f[1]; ls->f[2] = lsloc.f[2];
+        }
+      }
+  for (i = 0; i < ns; i++) for (k = 0; k < 3; k++) f[i][k] += st[i].f[k];
+#undef LIST
+  free(st);
+  free(list);
+  return err;
+}
+
+/* ------------------------------------------------------------ forces */
+
+/* forces(B,A) of cook (cook/forces.c:180-203, serial): f is zeroed,
+   Ewald(1), En.el=Ewald(2), pairforces() with En.el saved/restored. */
+int oracle_forces(const mdx_system_t *S, const double *r, double *f,
+                  mdx_out_t *out, int measure, int nthreads, int *cellid)
+{
+  int ns = S->ns, i, nq = 0, err;
+  int *idx = malloc(sizeof(int) * ns);
+  double *q = malloc(sizeof(double) * ns);
+  double (*pos)[3] = malloc(sizeof(double[3]) * ns);
+  double Pvir[6] = {0, 0, 0, 0, 0, 0}, Enel;
+  orc_acc a;
+  if (nthreads < 1) nthreads = 1;
+  memset(f, 0, sizeof(double) * 3 * ns);
+  for (i = 0; i < ns; i++)
+    if (S->charge[i] != 0) {
+      idx[nq] = i; q[nq] = S->charge[i];
+      pos[nq][0] = r[3 * i]; pos[nq][1] = r[3 * i + 1]; pos[nq][2] = r[3 * i + 2];
+      nq++;
+    }
+  Enel = nq ? orc_ewald(S, nq, idx, q, (const double (*)[3])pos, (double (*)[3])f,
+                        measure, Pvir, nthreads) : 0;
+  memset(&a, 0, sizeof(a));
+  a.S = S;
+  a.measure = measure;
+  err = orc_pairs(&a, (const double (*)[3])r, (double (*)[3])f, cellid);
+  memset(out, 0, sizeof(*out));
+  out->En_kspace = Enel;
+  out->En_el = a.Enel + Enel;
+  out->En_pot = a.U;
+  out->En_vir = a.Envir;
+  for (i = 0; i < 6; i++) out->Pvir[i] = Pvir[i] + a.Pvir[i];
+  out->npair = a.npair;
+  out->nlj = a.nlj;
+  free(idx); free(q); free(pos);
+  return err;
+}
+
+/* ------------------------------------------------------------- MD step */
+
+/* One Shake() step (sim/constrd.c:622-1081) for SHAKE=1, VERLET=3,
+   thermostat none or Berendsen, no Nose/NPT.  r = cfg[0], r1 = cfg[1]
+   are updated in place; f is scratch (cfg[2]).  Energies returned in out. */
+int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
+                   const mdx_step_t *p, int measure, mdx_step_out_t *out,
+                   int nthreads)
+{
+  int ns = S->ns, n, i, k, al, err;
+  double hh = p->h * p->h, hscale = 0.5 / hh;
+  double Ekin = 0, Pkin[6] = {0, 0, 0, 0, 0, 0}, virc = 0, PTvirc[6] = {0, 0, 0, 0, 0, 0};
+  double *pp = malloc(sizeof(double) * 3 * ns);
+  int maxit0 = -16 * log(p->eps * 0.1), itmax = 0;
+  memset(out, 0, sizeof(*out));
+  err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
+  if (err) { free(pp); return err; }
+  for (n = 0; n < S->nmol; n++) {
+    int i0 = S->mol_first[n], i1 = S->mol_first[n + 1];
+    int c0 = S->cons_first[n], c1 = S->cons_first[n + 1], nc = c1 - c0, it = 0;
+    int maxit = p->maxit > 0 ? p->maxit : maxit0 * (nc + 4);
+    double omega = S->mol_omega ? S->mol_omega[n] : 0.6, maxsq;
+    for (i = i0; i < i1; i++)        /* verlet.c, no thermostat branch */
+      for (k = 0; k < 3; k++)
+        pp[3 * i + k] = hh / S->mass[i] * f[3 * i + k] + r[3 * i + k] + r1[3 * i + k];
+    if (nc) do {
+      it++;
+      maxsq = 0;
+      for (al = c0; al < c1; al++) {
+        int a = S->cons_ij[2 * al], b = S->cons_ij[2 * al + 1];
+        double bondq = S->cons_bondq[al], sq = bondq, pij[3], rij[3];
+        double mi = 1 / S->mass[a], mj = 1 / S->mass[b];
+        for (k = 0; k < 3; k++) pij[k] = pp[3 * a + k] - pp[3 * b + k];
+        sq -= pij[0] * pij[0] + pij[1] * pij[1] + pij[2] * pij[2];
+        if (fabs(sq / bondq) > maxsq) maxsq = fabs(sq / bondq);
+        for (k = 0; k < 3; k++) rij[k] = r[3 * a + k] - r[3 * b + k];
+        sq *= omega / (mi + mj);
+        if (measure) {
+          PTvirc[0] += (sq / bondq) * (rij[0] * rij[0]);
+          PTvirc[1] += (sq / bondq) * (rij[1] * rij[1]);
+          PTvirc[2] += (sq / bondq) * (rij[2] * rij[2]);
+          PTvirc[3] += (sq / bondq) * (rij[1] * rij[2]);
+          PTvirc[4] += (sq / bondq) * (rij[2] * rij[0]);
+          PTvirc[5] += (sq / bondq) * (rij[0] * rij[1]);
+        }
+        virc += sq;
+        sq /= bondq;                     /* SHAKE>0: rxp=bondq */
+        for (k = 0; k < 3; k++) {
+          rij[k] *= sq;
+          pp[3 * a + k] += mi * rij[k];
+          pp[3 * b + k] -= mj * rij[k];
+        }
+      }
+    } while (maxsq > p->eps && it < maxit);
+    if (nc && it == maxit && maxsq > p->eps) { free(pp); return MDX_E_NOCONV; }
+    if (it > itmax) itmax = it;
+    for (i = i0; i < i1; i++) {        /* shakev3.c */
+      double V1[3], V2[3], m = S->mass[i];
+      for (k = 0; k < 3; k++) { V1[k] = r1[3 * i + k]; V2[k] = pp[3 * i + k] - r[3 * i + k]; }
+      Ekin += (V1[0] * V1[0] + V1[1] * V1[1] + V1[2] * V1[2] +
+               V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]) * m;
+      Pkin[0] += (V1[0] * V1[0] + V2[0] * V2[0]) * m;
+      Pkin[1] += (V1[1] * V1[1] + V2[1] * V2[1]) * m;
+      Pkin[2] += (V1[2] * V1[2] + V2[2] * V2[2]) * m;
+      Pkin[3] += (V1[1] * V1[2] + V2[1] * V2[2]) * m;
+      Pkin[4] += (V1[2] * V1[0] + V2[2] * V2[0]) * m;
+      Pkin[5] += (V1[0] * V1[1] + V2[0] * V2[1]) * m;
+      for (k = 0; k < 3; k++) { r1[3 * i + k] = V2[k]; r[3 * i + k] = pp[3 * i + k]; }
+    }
+  }
+  Ekin *= hscale;                          /* twice the kinetic energy */
+  for (k = 0; k < 6; k++) { Pkin[k] *= hscale; PTvirc[k] /= hh; }
+  virc /= hh;
+  out->lambda = 1;
+  if (p->thermostat == 1) {               /* Berendsen, sim/thermo.c:156-160 */
+    double T = Ekin / p->nf, b;
+    const double exp2 = 7.38905609893065022723;
+    if (T > p->T * exp2) b = 1;
+    else if (T * exp2 < p->T) b = -1;
+    else b = log(T / p->T) / 2;
+    out->lambda = exp(b * -p->h / p->tau);
+    for (i = 0; i < 3 * ns; i++) r1[i] *= out->lambda;
+  }
+  out->En_kin = Ekin / 2;
+  for (k = 0; k < 6; k++) { out->Pkin[k] = Pkin[k]; out->PTvirc[k] = PTvirc[k]; }
+  out->virc = virc;
+  out->shake_iter_max = itmax;
+  free(pp);
+  return 0;
+}

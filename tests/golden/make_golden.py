@@ -1,0 +1,287 @@
+#!/usr/bin/env python3
+"""Writes the golden force fixtures under tests/golden/ by running the
+reference `cook` program (compiled from the untouched MACSIMUS sources by
+oracle/Makefile.ref into oracle/_ref/) on hand-written input decks.
+
+Runs only in the build container, where /root/reference exists.  The
+fixtures are data: positions, charges, forces, energies and pressures
+that cook printed, plus the force-field parameters of the deck, so that
+the tests can rebuild the same system without the reference.
+
+What cook prints and where it comes from:
+  * `-v128` makes forces() dump r, q, f per site with 12/9 decimals
+    (sim/simils.c:2259-2297, called at cook/forces.c:222).
+  * `el.test=-2` runs Ewaldtest once on the initial configuration and
+    prints Epot, the virial pressure, the trace pressure and the six
+    pressure-tensor entries (sim/ewald.c:1320-1347).  Diagonal entries are
+    Pvir[i]/(3V)*Punit; off-diagonal entries are raw Pvir[i]*Punit
+    (VO() divides the first three components only).
+  * alpha, kappa, Q->N and No.cell come from the protocol lines
+    "Ewald final values" and "No.cell=" (sim/ewald.c:1212 ff.,
+    cook/linklist.c:341).
+
+Usage:  python3 tests/golden/make_golden.py [case ...]
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+COOK = os.path.join(REPO, "oracle", "_ref", "cookewslc")
+COOKPOL = os.path.join(REPO, "oracle", "_ref", "cookewslcpol")
+
+KCAL_PER_KJ = 1.0 / 4.184
+TWO16 = 2.0 ** (1.0 / 6.0)
+
+
+def lj_site(name, mass, sigma, eps_kjmol):
+    """MACSIMUS LJ table entry: EvdW = -eps [kcal/mol], RvdW = r_min/2."""
+    return dict(name=name, mass=mass, EvdW=-eps_kjmol * KCAL_PER_KJ,
+                RvdW=sigma * TWO16 / 2.0 if eps_kjmol else 0.5)
+
+
+# SPC/E (Berendsen 1987) and Smith-Dang ions, sigma [A], eps [kJ/mol]
+SITES = {
+    "H": lj_site("H", 1.00800, 0.0, 0.0),
+    "O": lj_site("O", 15.99940, 3.166, 0.650),
+    "NA": lj_site("NA", 22.98977, 2.35, 0.544),
+    "CL": lj_site("CL", 35.45300, 4.40, 0.419),
+}
+
+HOH = np.deg2rad(109.47)
+HH = 2.0 * np.sin(HOH / 2.0)
+
+
+def spce_species():
+    s, c = np.sin(HOH / 2), np.cos(HOH / 2)
+    return dict(name="SPCE", sites=[("H", 0.4238, (s, c, 0.0), []),
+                                    ("H", 0.4238, (-s, c, 0.0), [0]),
+                                    ("O", -0.8476, (0.0, 0.0, 0.0), [0, 1])],
+                bonds=[(0, 2, 1.0), (1, 2, 1.0), (0, 1, HH)])
+
+
+def ion_species(name, q):
+    return dict(name=name, sites=[(name, q, (0.0, 0.0, 0.0), [])], bonds=[])
+
+
+def write_ble(path, species, types, polar=None):
+    """polar: dict type -> (alphapol [A^3], chargepol [e]) for Drude sites."""
+    out = []
+    out.append("! hand-written force field for golden fixtures (LJ form)")
+    out.append("nspec=%d" % len(species))
+    out.append("  nsites=%d  nnbfixes=0  factor14=0.5  distance14=0  comb_rule=1"
+               % len(types))
+    out.append("  polar=%d  nparms=0;" % (1 if polar else 0))
+    out.append("")
+    out.append("Lennard-Jones")
+    out.append("! i atom  mass  alpha  EvdW[kcal/mol]  RvdW[A]"
+               + ("  alphapol chargepol Esat kappa" if polar else ""))
+    for i, t in enumerate(types):
+        st = SITES[t]
+        line = "  %d %-4s %10.5f  0.000  %.12f  %.12f" % (
+            i, t, st["mass"], st["EvdW"], st["RvdW"])
+        if polar is not None:
+            a, qd = polar.get(t, (0.0, 0.0))
+            line += "  %.6f  %.6f  0  0" % (a, qd)
+        out.append(line)
+    out.append("")
+    out.append("nbfixes")
+    out.append("! atom atom Eref R")
+    out.append("")
+    for isp, sp in enumerate(species):
+        out.append("!!!!!!!!!!")
+        out.append("species %s" % sp["name"])
+        out.append("!!!!!!!!!!")
+        out.append("i=%d  N=0  config=0  water=0" % isp)
+        out.append("ns=%d  nc=%d  nangles=0  ndihedrals=0  nimpropers=0  naromatics=0"
+                   % (len(sp["sites"]), len(sp["bonds"])))
+        out.append("ndependants=0 ;")
+        out.append("")
+        out.append("sites")
+        out.append("! i atom charge x y z # excluded")
+        for i, (t, q, x, exc) in enumerate(sp["sites"]):
+            out.append("  %d %-4s %.6f  %.9f %.9f %.9f  %d %s" % (
+                i, t, q, x[0], x[1], x[2], len(exc), " ".join(map(str, exc))))
+        out.append("")
+        out.append("bonds")
+        out.append("! i atom i atom K r calc Upot")
+        for (i, j, b) in sp["bonds"]:
+            ti, tj = sp["sites"][i][0], sp["sites"][j][0]
+            out.append(" %d %-4s %d %-4s 0 %.12f %.12f 0.0" % (i, ti, j, tj, b, b))
+        for tab in ("angles", "dihedrals", "impropers", "aromatics", "dependants"):
+            out.append("")
+            out.append(tab)
+        out.append("")
+        out.append("")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def parse_dump(path, polar=False):
+    rows = []
+    with open(path) as f:
+        f.readline()
+        L = [float(x) for x in f.readline().split()[:3]]
+        for line in f:
+            tok = line.split()
+            if polar:
+                rows.append([float(x) for x in tok[:14]] + [int(tok[14]), int(tok[15])])
+            else:
+                rows.append([float(x) for x in tok[:7]] + [int(tok[7]), int(tok[8])])
+    a = np.array(rows)
+    d = dict(L=np.array(L))
+    d["r"] = a[:, 0:3]
+    d["q"] = a[:, 3]
+    if polar:
+        d["rpol"] = a[:, 4:7]
+        d["qpol"] = a[:, 7]
+        d["f"] = a[:, 8:11]
+        d["fpol"] = a[:, 11:14]
+    else:
+        d["f"] = a[:, 4:7]
+    d["mol"] = a[:, -2].astype(np.int32)
+    d["site"] = a[:, -1].astype(np.int32)
+    return d
+
+
+def parse_prt(path):
+    txt = open(path).read()
+    out = {}
+    m = re.search(r"Ewald final values: cutoff=(\S+) el.alpha=(\S+) el.kappa=(\S+) Q->N=(\d+)", txt)
+    if m:
+        out["cutoff"] = float(m.group(1))
+        out["alpha_printed"] = float(m.group(2))
+        out["kappa_printed"] = float(m.group(3))
+        out["Nk"] = int(m.group(4))
+    m = re.search(r"initializing Ewald: kappa=K/L=(\S+)", txt)
+    if m:
+        out["kappa_init"] = float(m.group(1))
+    m = re.findall(r"No.cell=\((\d+),(\d+),(\d+)\)", txt)
+    if m:
+        out["ncell"] = [int(x) for x in m[-1]]
+    m = re.search(r"reference set TTTTTT\s*\nEpot = (\S+) K\s*\n"
+                  r"sqrt\(sum forces\^2\) = (\S+), max force = (\S+)\s*\n"
+                  r".*?\nP \(from E by virial theorem\) = (\S+) \[Pa\]\s*\n"
+                  r"P \(trace of pressure tensor\) = (\S+) \[Pa\]\s*\n"
+                  r"pressure tensor \[Pa\]:\s*\n\s*(\S+) (\S+) (\S+)\s*\n\s*(\S+) (\S+) (\S+)",
+                  txt, re.S)
+    if m:
+        g = [float(x) for x in m.groups()]
+        out["Epot"] = g[0]
+        out["fnorm"] = g[1]
+        out["fmax"] = g[2]
+        out["P_virial_Pa"] = g[3]
+        out["P_trace_Pa"] = g[4]
+        out["Ptens_printed_Pa"] = g[5:11]
+    m = re.search(r"En\.corr=(\S+) \[K AA3\]", txt)
+    if m:
+        out["Ecorr_KAA3"] = float(m.group(1))
+    m = re.search(r"POLAR: (\d+) iterations", txt)
+    if m:
+        out["scf_iterations"] = int(m.group(1))
+    return out
+
+
+def run(cmd, cwd):
+    p = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, timeout=900)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout.decode(errors="replace")[-3000:])
+        raise RuntimeError("cook failed: %s" % cmd)
+    return p.stdout.decode(errors="replace")
+
+
+def ff_arrays(species, types, polar=None):
+    """Force-field description stored with every fixture."""
+    d = dict(types=types,
+             type_mass=[SITES[t]["mass"] for t in types],
+             type_EvdW_kcal=[SITES[t]["EvdW"] for t in types],
+             type_RvdW=[SITES[t]["RvdW"] for t in types],
+             species=[])
+    for sp in species:
+        d["species"].append(dict(
+            name=sp["name"],
+            site_type=[types.index(s[0]) for s in sp["sites"]],
+            site_charge_e=[s[1] for s in sp["sites"]],
+            site_xyz=[list(s[2]) for s in sp["sites"]],
+            site_excl=[s[3] for s in sp["sites"]],
+            bonds=[list(b) for b in sp["bonds"]]))
+    if polar:
+        d["polar"] = {t: list(v) for t, v in polar.items()}
+    return d
+
+
+def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
+               extra_def="", polar=None, scf=None, md_steps=0):
+    """One Ewaldtest force evaluation on cook's own initial configuration.
+
+    md_steps>0 first runs that many MD steps (SHAKE, leap-frog, Berendsen)
+    from the lattice start and then evaluates the forces on the result."""
+    exe = COOKPOL if polar else COOK
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types, polar)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff",
+                  "el.epsr=%g" % epsr, "el.epsk=%g" % epsk,
+                  "h=0.002", "noint=%d" % max(md_steps, 1), "T=298",
+                  "epsc=1e-10"]
+        if scf:
+            lines += scf
+        if extra_def:
+            lines.append(extra_def)
+        lines.append(";")
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        if md_steps:
+            open(os.path.join(td, "eq.get"), "w").write(
+                'init="crystal"\nthermostat="Berendsen"\ntau.T=0.05\nno=1;\n')
+            run([exe, "sys", "eq"], td)
+            start = 'init="start"\n'
+            os.rename(os.path.join(td, "eq.cfg"), os.path.join(td, "t.cfg"))
+        else:
+            start = 'init="crystal"\n'
+        open(os.path.join(td, "t.get"), "w").write(
+            start + "el.test=-2\nno=0;\nel.test=-2;\nel.test=0;\n")
+        run([exe, "sys", "t", "-v128"], td)
+        dump = parse_dump(os.path.join(td, "t.0.dump"), polar=bool(polar))
+        meta = parse_prt(os.path.join(td, "t.prt"))
+    meta.update(dict(name=name, counts=counts, L=L, epsr=epsr, epsk=epsk,
+                     md_steps=md_steps,
+                     forcefield=ff_arrays(species, types, polar)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **dump)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, {k: meta[k] for k in ("Epot", "Nk", "ncell") if k in meta})
+
+
+def main(argv):
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8", "all"],
+                   cwd=REPO, check=True)
+    spce = spce_species()
+    cases = {
+        # 1000 SPC/E, L=31.04 A, 9 A cutoff, Ewald tuned to 0.05 K/A
+        "spce1000_lattice": lambda: force_case(
+            "spce1000_lattice", [spce], ["H", "O"], [1000], 31.04, 9.0, 0.05, 0.05),
+        "spce1000_md": lambda: force_case(
+            "spce1000_md", [spce], ["H", "O"], [1000], 31.04, 9.0, 0.05, 0.05,
+            md_steps=40),
+        # small brine: 3 NaCl + 500 SPC/E, L chosen for ~1.03 g/cm3
+        "brine506": lambda: force_case(
+            "brine506", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 0.05, 0.05,
+            md_steps=20),
+    }
+    todo = argv or list(cases)
+    for c in todo:
+        cases[c]()
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

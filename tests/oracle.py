@@ -1,0 +1,91 @@
+"""ctypes access to the CPU FP64 restatement in oracle/ (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module; the engine never does.
+"""
+import ctypes as C
+import json
+import os
+import subprocess
+
+import numpy as np
+
+from macsimus_amd import abi
+from macsimus_amd.system import Provider, build_system
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "libmdx_oracle.so")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], check=True)
+        lib = C.CDLL(ORACLE_SO)
+        abi.declare_setup(lib, "oracle_")
+        dp = C.POINTER(C.c_double)
+        lib.oracle_forces.argtypes = [C.POINTER(abi.System), dp, dp,
+                                      C.POINTER(abi.Out), C.c_int, C.c_int,
+                                      C.POINTER(C.c_int)]
+        lib.oracle_forces.restype = C.c_int
+        lib.oracle_md_step.argtypes = [C.POINTER(abi.System), dp, dp, dp,
+                                       C.POINTER(abi.Step), C.c_int,
+                                       C.POINTER(abi.StepOut), C.c_int]
+        lib.oracle_md_step.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def provider():
+    return Provider(load(), "oracle_")
+
+
+def load_fixture(name):
+    d = dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+    meta = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    return d, meta
+
+
+def fixture_system(meta, provider_=None, **kw):
+    L = [meta["L"]] * 3
+    args = dict(counts=meta["counts"], L=L, cutoff=meta["cutoff"],
+                epsr=meta["epsr"], epsk=meta["epsk"],
+                ncell=tuple(meta.get("ncell", (0, 0, 0))))
+    args.update(kw)
+    return build_system(meta["forcefield"], provider=provider_ or provider(), **args)
+
+
+def forces(bs, r, measure=1, nthreads=None, cellid=False):
+    lib = load()
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    f = np.zeros_like(r)
+    out = abi.Out()
+    nt = nthreads or os.cpu_count() or 1
+    cid = np.zeros(bs.struct.ns, np.int32) if cellid else None
+    rc = lib.oracle_forces(C.byref(bs.struct),
+                           r.ctypes.data_as(C.POINTER(C.c_double)),
+                           f.ctypes.data_as(C.POINTER(C.c_double)),
+                           C.byref(out), measure, nt,
+                           cid.ctypes.data_as(C.POINTER(C.c_int)) if cellid else None)
+    if rc:
+        raise RuntimeError("oracle_forces failed (%d)" % rc)
+    return (f, out, cid) if cellid else (f, out)
+
+
+def md_step(bs, r, r1, p, measure=1, nthreads=None):
+    lib = load()
+    f = np.zeros_like(r)
+    out = abi.StepOut()
+    nt = nthreads or os.cpu_count() or 1
+    rc = lib.oracle_md_step(C.byref(bs.struct),
+                            r.ctypes.data_as(C.POINTER(C.c_double)),
+                            r1.ctypes.data_as(C.POINTER(C.c_double)),
+                            f.ctypes.data_as(C.POINTER(C.c_double)),
+                            C.byref(p), measure, C.byref(out), nt)
+    if rc:
+        raise RuntimeError("oracle_md_step failed (%d)" % rc)
+    return out
