@@ -122,6 +122,8 @@ def load_engine(path=None):
     lib.mdx_md_steps.argtypes = [C.c_void_p, C.POINTER(Step), C.c_int,
                                  C.POINTER(StepOut)]
     declare_setup(lib, "mdx_")
+    from .engine import declare_extras
+    declare_extras(lib)
     if path is None:
         _engine = lib
     return lib

@@ -1,0 +1,790 @@
+// MI355X force engine for MACSIMUS cook: context, binning, real-space pair
+// kernel, exclusion corrections, reductions and the C ABI entry points.
+//
+// Reference path: forces(B,A), cook/forces.c:93-224 (serial variant): zero
+// B, Ewald k-space, then pairforces() (cook/lc.c:466-583) with the linked
+// cells of cook/linklist.c and the pair kernel LJqq[m] of cook/ljqqnp.c.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mdx_internal.h"
+
+namespace mdx {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+}  // namespace mdx
+
+using namespace mdx;
+
+extern "C" const char *mdx_last_error(void) { return g_last_error.c_str(); }
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kPi = 3.14159265358979323846;
+
+// ------------------------------------------------------------ reductions
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// sums NV per-thread values over the block into part[blockIdx.x*ACC_N+idx[k]]
+template <int NV>
+__device__ inline void block_store(double (&v)[NV], const int (&idx)[NV], double *part) {
+  __shared__ double red[kBlock / 64][NV];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[w][threadIdx.x];
+    part[(size_t)blockIdx.x * ACC_N + idx[threadIdx.x]] += s;
+  }
+}
+
+__global__ void k_zero(double *p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// final reduction of per-block partials; one block, ACC_N columns
+__global__ void k_reduce(const double *part, int nblocks, double *acc) {
+  __shared__ double red[kBlock];
+  for (int a = 0; a < ACC_N; a++) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * ACC_N + a];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[a] = red[0];
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------- binning
+// Counting sort of the sites into the engine's cell grid.  Coordinates
+// are wrapped into [0,L) in FP64 (cook wraps by subtraction,
+// cook/linklist.c:155-159; the engine grid is its own, so it also wraps
+// negative coordinates).
+
+struct Grid {
+  int nx, ny, nz;
+  double Lx, Ly, Lz, rx, ry, rz;  // r* = n/L
+};
+
+__device__ inline double wrap1(double x, double L) {
+  double y = x - L * floor(x / L);
+  return y >= L ? y - L : y;
+}
+
+__global__ void k_cellid(const double *__restrict__ r, int ns, Grid g,
+                         int *__restrict__ cellid, int *__restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  double x = wrap1(r[3 * i], g.Lx), y = wrap1(r[3 * i + 1], g.Ly), z = wrap1(r[3 * i + 2], g.Lz);
+  int cx = min((int)(x * g.rx), g.nx - 1), cy = min((int)(y * g.ry), g.ny - 1),
+      cz = min((int)(z * g.rz), g.nz - 1);
+  int c = (cx * g.ny + cy) * g.nz + cz;
+  cellid[i] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+// exclusive scan of cnt[0..n) into start[0..n], single block
+__global__ void k_scan(const int *__restrict__ cnt, int n, int *__restrict__ start,
+                       int *__restrict__ cursor) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (n + 1023) / 1024;
+  const int b = min(t * per, n), e = min(b + per, n);
+  int s = 0;
+  for (int i = b; i < e; i++) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  s = t ? part[t - 1] : 0;
+  for (int i = b; i < e; i++) {
+    start[i] = s;
+    cursor[i] = 0;
+    s += cnt[i];
+  }
+  if (t == 1023) start[n] = part[1023];
+}
+
+__global__ void k_scatter(const int *__restrict__ cellid, int ns, const int *__restrict__ start,
+                          int *__restrict__ cursor, int *__restrict__ sidx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  int c = cellid[i];
+  sidx[start[c] + atomicAdd(&cursor[c], 1)] = i;
+}
+
+// makes the order inside each cell ascending in site index (deterministic
+// sums), then gathers cell-ordered coordinates and site records
+__global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__restrict__ sidx) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; i++) {
+    int v = sidx[i], k = i - 1;
+    while (k >= b && sidx[k] > v) { sidx[k + 1] = sidx[k]; k--; }
+    sidx[k + 1] = v;
+  }
+}
+
+__global__ void k_gather(int ns, Grid g, const int *__restrict__ sidx, const double *__restrict__ r,
+                         const double *__restrict__ q, const int *__restrict__ type,
+                         const int *__restrict__ mol, const int *__restrict__ molfirst,
+                         const unsigned *__restrict__ excl, const int *__restrict__ cellid,
+                         double4 *__restrict__ spos, int4 *__restrict__ sinfo,
+                         int *__restrict__ scell) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  int i = sidx[s];
+  spos[s] = make_double4(wrap1(r[3 * i], g.Lx), wrap1(r[3 * i + 1], g.Ly),
+                         wrap1(r[3 * i + 2], g.Lz), q[i]);
+  int m = mol[i];
+  sinfo[s] = make_int4(type[i] | ((i - molfirst[m]) << 16), m, (int)excl[i], i);
+  scell[s] = cellid[i];
+}
+
+// cook's own cell id of every site: WORM wrap by subtraction, C truncation,
+// rL shaved so that (int)(L*rL) < No.cell (cook/linklist.c:81-86,155-176).
+// Used to check cell membership against the reference bit for bit.
+__global__ void k_cook_cellid(const double *__restrict__ r, int ns, int nx, int ny, int nz,
+                              double Lx, double Ly, double Lz, double rLx, double rLy,
+                              double rLz, int *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  double x = r[3 * i], y = r[3 * i + 1], z = r[3 * i + 2];
+  while (x >= Lx) x -= Lx;
+  while (y >= Ly) y -= Ly;
+  while (z >= Lz) z -= Lz;
+  int cx = (int)(x * rLx), cy = (int)(y * rLy), cz = (int)(z * rLz);
+  bool ok = cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz;
+  out[i] = ok ? (cx * ny + cy) * nz + cz : -1;
+}
+
+// ------------------------------------------------------------- pair kernel
+// Full-shell traversal of the (2w+1)^3 neighbour cells of each site; each
+// unordered pair is met twice, forces go to the central site only (no
+// atomics), energies/virial/census are halved at the end.  Geometry and the
+// cutoff test are FP64 (so the pair census is the reference's, which tests
+// rr<cutoff^2 in double, cook/ljqqnp.c:85); potential arithmetic is R.
+// Same-molecule EXCL pairs are skipped here and corrected in FP64 by
+// k_excl (cook/ljqqnp.c:104-144).
+
+template <typename R>
+struct PairTab {
+  const R *lj;    // [nt*nt*8] Sq E4 E24 E48 C1q C2q A A4
+  const R *erf;   // [nerf*8] f0u bu Du f0d bd Dd (stable form) or Au Bu Cu Ad Bd Cd
+};
+
+// eru/erd of the hyperbolic spline.  FP64: cook's form B/(C+x)+A
+// (sim/elst.h:122-130).  FP32: the same function written around the
+// interval start x0 = i/sgrid, f0 - b t/(D+t) with D=C+x0, b=B/D,
+// f0=b+A, t=x-x0, which avoids the cancellation of B/(C+x) against A.
+template <typename R>
+__device__ inline void spline(const R *e, double rr, double sgrid, R &eru, R &erd, bool want_u);
+
+template <>
+__device__ inline void spline<double>(const double *e, double rr, double sgrid, double &eru,
+                                      double &erd, bool want_u) {
+  const double *p = e + 8 * (int)(rr * sgrid);
+  erd = p[4] / (p[5] + rr) + p[3];
+  if (want_u) eru = p[1] / (p[2] + rr) + p[0];
+}
+
+template <>
+__device__ inline void spline<float>(const float *e, double rr, double sgrid, float &eru,
+                                     float &erd, bool want_u) {
+  const int i = (int)(rr * sgrid);
+  const float *p = e + 8 * i;
+  const float t = (float)(rr - i / sgrid);
+  erd = p[3] - p[4] * t * __frcp_rn(p[5] + t);
+  if (want_u) eru = p[0] - p[1] * t * __frcp_rn(p[2] + t);
+}
+
+template <typename R>
+__device__ inline R rcp(R x);
+template <>
+__device__ inline float rcp<float>(float x) { return __frcp_rn(x); }
+template <>
+__device__ inline double rcp<double>(double x) { return 1.0 / x; }
+
+template <typename R, bool MEASURE>
+__global__ __launch_bounds__(kBlock) void k_pair(
+    int ns, Grid g, int w, double cq, const double4 *__restrict__ spos,
+    const int4 *__restrict__ sinfo, const int *__restrict__ scell,
+    const int *__restrict__ cstart, int nt, PairTab<R> tab, double sgrid,
+    double *__restrict__ f, double *__restrict__ part) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  double fx = 0, fy = 0, fz = 0;
+  double U = 0, Eel = 0, Vir = 0, pv[6] = {0, 0, 0, 0, 0, 0};
+  double npair = 0, nlj = 0;
+  if (s < ns) {
+    const double4 pi = spos[s];
+    const int4 ii = sinfo[s];
+    const int ti = ii.x & 0xffff;
+    const int c = scell[s];
+    const int cz = c % g.nz, cy = (c / g.nz) % g.ny, cx = c / (g.nz * g.ny);
+    const R qi = (R)pi.w;
+    const R *ljrow = tab.lj + (size_t)ti * nt * 8;
+    for (int dx = -w; dx <= w; dx++) {
+      int nx = cx + dx;
+      double sx = 0;
+      if (nx < 0) { nx += g.nx; sx = -g.Lx; } else if (nx >= g.nx) { nx -= g.nx; sx = g.Lx; }
+      for (int dy = -w; dy <= w; dy++) {
+        int ny = cy + dy;
+        double sy = 0;
+        if (ny < 0) { ny += g.ny; sy = -g.Ly; } else if (ny >= g.ny) { ny -= g.ny; sy = g.Ly; }
+        R ax = 0, ay = 0, az = 0;
+        for (int dz = -w; dz <= w; dz++) {
+          int nz = cz + dz;
+          double sz = 0;
+          if (nz < 0) { nz += g.nz; sz = -g.Lz; } else if (nz >= g.nz) { nz -= g.nz; sz = g.Lz; }
+          const int nc = (nx * g.ny + ny) * g.nz + nz;
+          const int jb = cstart[nc], je = cstart[nc + 1];
+          for (int j = jb; j < je; j++) {
+            const double4 pj = spos[j];
+            const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy), dzr = pi.z - (pj.z + sz);
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            if (rr >= cq || j == s) continue;
+            npair += 1;
+            const int4 jj = sinfo[j];
+            if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) continue;  // EXCL
+            const int tj = jj.x & 0xffff;
+            const R *ss = ljrow + tj * 8;
+            R fr = 0, u = 0;
+            if (rr < (double)ss[5]) {
+              if (rr < (double)ss[4]) {
+                const R x = ss[0] * rcp<R>((R)rr), y = x * x * x;
+                if (MEASURE) {
+                  const R z = y * y;
+                  u = ss[1] * (z - y);
+                  fr = ss[2] * (z + z - y) * x;
+                } else {
+                  fr = ss[3] * y * (y - (R)0.5) * x;
+                }
+              } else {
+                const R x = (R)(rr - (double)ss[5]);
+                if (MEASURE) u = x * x * ss[6];
+                fr = ss[7] * x;
+              }
+              if (MEASURE) Vir += rr * (double)fr;
+              nlj += ss[6] != 0;
+            }
+            const R qq = qi * (R)pj.w;
+            if (qq != 0) {
+              R eru = 0, erd;
+              spline<R>(tab.erf, rr, sgrid, eru, erd, MEASURE);
+              fr += qq * erd;
+              if (MEASURE) Eel += (double)(qq * eru);
+            }
+            const R ddx = (R)dxr, ddy = (R)dyr, ddz = (R)dzr;
+            const R fdx = fr * ddx, fdy = fr * ddy, fdz = fr * ddz;
+            ax += fdx; ay += fdy; az += fdz;
+            if (MEASURE) {
+              U += u;
+              pv[0] += (double)(ddx * fdx); pv[1] += (double)(ddy * fdy); pv[2] += (double)(ddz * fdz);
+              pv[3] += (double)(ddz * fdy); pv[4] += (double)(ddx * fdz); pv[5] += (double)(ddy * fdx);
+            }
+          }
+        }
+        fx += ax; fy += ay; fz += az;
+      }
+    }
+    const int site = ii.w;
+    f[3 * site] = fx;
+    f[3 * site + 1] = fy;
+    f[3 * site + 2] = fz;
+  }
+  if (MEASURE) {
+    double v[12] = {0.5 * U, 0.5 * Eel, 0.5 * Vir, 0.5 * pv[0], 0.5 * pv[1], 0.5 * pv[2],
+                    0.5 * pv[3], 0.5 * pv[4], 0.5 * pv[5], 0.5 * npair, 0.5 * nlj, 0};
+    const int idx[12] = {ACC_U, ACC_EEL, ACC_VIR, ACC_PV0, ACC_PV0 + 1, ACC_PV0 + 2,
+                         ACC_PV0 + 3, ACC_PV0 + 4, ACC_PV0 + 5, ACC_NPAIR, ACC_NLJ, ACC_SUMQ};
+    block_store<12>(v, idx, part);
+  } else {
+    double v[2] = {0.5 * npair, 0.5 * nlj};
+    const int idx[2] = {ACC_NPAIR, ACC_NLJ};
+    block_store<2>(v, idx, part);
+  }
+}
+
+// --------------------------------------------------- exclusion corrections
+// exacterud_sqrt(): -erf(alpha r)/r and its force/r (sim/erexact.c:25-53),
+// added for same-molecule EXCL pairs inside the cutoff, FP64, one thread per
+// molecule (no write conflicts: the thread owns the molecule's sites).
+__device__ inline double exact_erud(double rr, double alpha, double &erd) {
+  const double alphaq = alpha * alpha, A = alpha * (-2 / 1.772453850905516), B = A * alphaq * 2;
+  const double err = alphaq * rr;
+  if (err < 0.01) {
+    double su = 0, sd = 0, q = 1;
+    for (int n = 1;; ) {
+      const double t = q / (2 * n + 1);
+      sd += t * n;
+      su += t;
+      if (fabs(q) < 1e-16) break;
+      q *= -err / ++n;
+    }
+    erd = B * sd;
+    return A * (1 - su * err);
+  }
+  const double r = sqrt(rr), y = -erf(alpha * r) / r;
+  erd = (y - A * exp(-err)) / rr;
+  return y;
+}
+
+template <bool MEASURE>
+__global__ __launch_bounds__(kBlock) void k_excl(int nmol, const int *__restrict__ molfirst,
+                                                 const double *__restrict__ r,
+                                                 const double *__restrict__ q,
+                                                 const unsigned *__restrict__ excl, double Lx,
+                                                 double Ly, double Lz, double cq, double alpha,
+                                                 double *__restrict__ f, double *__restrict__ part) {
+  const int m = blockIdx.x * kBlock + threadIdx.x;
+  double Eel = 0, pv[6] = {0, 0, 0, 0, 0, 0};
+  if (m < nmol) {
+    const int b = molfirst[m], e = molfirst[m + 1];
+    for (int a = b; a < e; a++) {
+      const unsigned mask = excl[a];
+      for (int c = a + 1; c < e; c++) {
+        if (!((mask >> (c - b)) & 1)) continue;
+        const double qq = q[a] * q[c];
+        if (qq == 0) continue;
+        double d[3] = {r[3 * a] - r[3 * c], r[3 * a + 1] - r[3 * c + 1], r[3 * a + 2] - r[3 * c + 2]};
+        d[0] -= Lx * rint(d[0] / Lx);
+        d[1] -= Ly * rint(d[1] / Ly);
+        d[2] -= Lz * rint(d[2] / Lz);
+        const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (rr >= cq) continue;
+        double erd;
+        const double u = exact_erud(rr, alpha, erd);
+        const double fr = erd * qq;
+        for (int k = 0; k < 3; k++) {
+          f[3 * a + k] += fr * d[k];
+          f[3 * c + k] -= fr * d[k];
+        }
+        if (MEASURE) {
+          Eel += u * qq;
+          pv[0] += d[0] * fr * d[0]; pv[1] += d[1] * fr * d[1]; pv[2] += d[2] * fr * d[2];
+          pv[3] += d[2] * fr * d[1]; pv[4] += d[0] * fr * d[2]; pv[5] += d[1] * fr * d[0];
+        }
+      }
+    }
+  }
+  if (MEASURE) {
+    double v[7] = {Eel, pv[0], pv[1], pv[2], pv[3], pv[4], pv[5]};
+    const int idx[7] = {ACC_EEL, ACC_PV0, ACC_PV0 + 1, ACC_PV0 + 2, ACC_PV0 + 3, ACC_PV0 + 4,
+                        ACC_PV0 + 5};
+    block_store<7>(v, idx, part);
+  }
+}
+
+inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+Grid make_grid(const Ctx &c, const int *n) {
+  Grid g;
+  g.nx = n[0]; g.ny = n[1]; g.nz = n[2];
+  g.Lx = c.L[0]; g.Ly = c.L[1]; g.Lz = c.L[2];
+  g.rx = n[0] / c.L[0]; g.ry = n[1] / c.L[1]; g.rz = n[2] / c.L[2];
+  return g;
+}
+
+// engine cell grid: cells at least cutoff/w wide, at least 2w+1 per side
+int choose_grid(Ctx &c, int w) {
+  for (; w >= 1; w--) {
+    bool ok = true;
+    for (int d = 0; d < 3; d++) {
+      c.nc[d] = (int)std::floor(c.L[d] * w / c.cutoff);
+      if (c.nc[d] < 2 * w + 1) ok = false;
+    }
+    if (ok) { c.wst = w; return 0; }
+  }
+  return fail(MDX_E_ARG, "box too small for the cutoff (need L >= 3*cutoff)");
+}
+
+int alloc_grid(Ctx &c) {
+  const int ncell = c.nc[0] * c.nc[1] * c.nc[2];
+  int rc;
+  if ((rc = c.cnt.alloc(ncell)) || (rc = c.cstart.alloc(ncell + 1)) || (rc = c.cursor.alloc(ncell)))
+    return rc;
+  c.part_blocks = std::max(nblk(c.ns), nblk(c.nmol));
+  c.part_blocks = std::max(c.part_blocks, 1024);
+  return c.part.alloc((size_t)c.part_blocks * ACC_N);
+}
+
+int bin_sites(Ctx &c, const double *d_r) {
+  const Grid g = make_grid(c, c.nc);
+  const int ncell = c.nc[0] * c.nc[1] * c.nc[2];
+  MDX_HIP(hipMemsetAsync(c.cnt.p, 0, sizeof(int) * ncell, c.stream));
+  k_cellid<<<nblk(c.ns), kBlock, 0, c.stream>>>(d_r, c.ns, g, c.cellid.p, c.cnt.p);
+  k_scan<<<1, 1024, 0, c.stream>>>(c.cnt.p, ncell, c.cstart.p, c.cursor.p);
+  k_scatter<<<nblk(c.ns), kBlock, 0, c.stream>>>(c.cellid.p, c.ns, c.cstart.p, c.cursor.p, c.sidx.p);
+  k_cellsort<<<nblk(ncell), kBlock, 0, c.stream>>>(ncell, c.cstart.p, c.sidx.p);
+  k_gather<<<nblk(c.ns), kBlock, 0, c.stream>>>(
+      c.ns, g, c.sidx.p, d_r, c.charge.p, c.type.p, c.mol.p, c.molfirst.p, c.excl.p, c.cellid.p,
+      (double4 *)c.spos.p, (int4 *)c.sinfo.p, c.scan_tmp.p);
+  MDX_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int launch_pair(Ctx &c, int measure) {
+  const Grid g = make_grid(c, c.nc);
+  PairTab<R> tab;
+  if constexpr (sizeof(R) == 4) { tab.lj = c.lj32.p; tab.erf = c.erf32.p; }
+  else { tab.lj = (const R *)c.lj64.p; tab.erf = (const R *)c.erf64.p; }
+  const double cq = c.cutoff * c.cutoff;
+  if (measure)
+    k_pair<R, true><<<nblk(c.ns), kBlock, 0, c.stream>>>(
+        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
+        c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p, c.part.p);
+  else
+    k_pair<R, false><<<nblk(c.ns), kBlock, 0, c.stream>>>(
+        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
+        c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p, c.part.p);
+  MDX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+namespace mdx {
+
+// one force evaluation on device-resident positions c.r -> c.f, sums in c.acc
+int forces_device(Ctx &c, int measure) {
+  int rc;
+  const size_t npart = (size_t)c.part_blocks * ACC_N;
+  k_zero<<<nblk(npart), kBlock, 0, c.stream>>>(c.part.p, npart);
+  if (c.timing) c.tm.start(T_BIN, c.stream);
+  if ((rc = bin_sites(c, c.r.p))) return rc;
+  if (c.timing) { c.tm.stop(T_BIN, c.stream); c.tm.start(T_PAIR, c.stream); }
+  rc = c.mode == MDX_FP64 ? launch_pair<double>(c, measure) : launch_pair<float>(c, measure);
+  if (rc) return rc;
+  if (c.timing) { c.tm.stop(T_PAIR, c.stream); c.tm.start(T_EXCL, c.stream); }
+  if (measure)
+    k_excl<true><<<nblk(c.nmol), kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.r.p, c.charge.p,
+                                                        c.excl.p, c.L[0], c.L[1], c.L[2],
+                                                        c.cutoff * c.cutoff, c.alphar, c.f.p,
+                                                        c.part.p);
+  else
+    k_excl<false><<<nblk(c.nmol), kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.r.p,
+                                                         c.charge.p, c.excl.p, c.L[0], c.L[1],
+                                                         c.L[2], c.cutoff * c.cutoff, c.alphar,
+                                                         c.f.p, c.part.p);
+  MDX_HIP(hipGetLastError());
+  if (c.timing) c.tm.stop(T_EXCL, c.stream);
+  if (c.nq && (rc = kspace_forces(c, measure))) return rc;
+  k_reduce<<<1, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, c.acc.p);
+  MDX_HIP(hipGetLastError());
+  return 0;
+}
+
+void fill_out(const Ctx &c, const double *acc, int measure, mdx_out_t *out) {
+  std::memset(out, 0, sizeof(*out));
+  out->npair = (long long)std::llround(acc[ACC_NPAIR]);
+  out->nlj = (long long)std::llround(acc[ACC_NLJ]);
+  if (!measure) return;
+  const double V = c.L[0] * c.L[1] * c.L[2];
+  // k-space: Energy = sum/(4 pi); diag += Energy; all -= 2/(4 pi) PVIR
+  // (sim/ewald.c:853-854); dipole term for epsinf (sim/ewald.c:885-897)
+  const double Ek = acc[ACC_EK] / (4 * kPi);
+  const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
+  const double M2 = acc[ACC_M0] * acc[ACC_M0] + acc[ACC_M0 + 1] * acc[ACC_M0 + 1] +
+                    acc[ACC_M0 + 2] * acc[ACC_M0 + 2];
+  const double Edip = cdip / 2 * M2;
+  double Pk[6];
+  for (int i = 0; i < 6; i++) Pk[i] = -2 / (4 * kPi) * acc[ACC_PK0 + i];
+  for (int i = 0; i < 3; i++) Pk[i] += Ek + Edip;
+  // dipole-field virial sum_i q_i ek r_i with ek=-c M: -c M M^T
+  Pk[0] -= cdip * acc[ACC_M0] * acc[ACC_M0];
+  Pk[1] -= cdip * acc[ACC_M0 + 1] * acc[ACC_M0 + 1];
+  Pk[2] -= cdip * acc[ACC_M0 + 2] * acc[ACC_M0 + 2];
+  Pk[3] -= cdip * acc[ACC_M0 + 1] * acc[ACC_M0 + 2];
+  Pk[4] -= cdip * acc[ACC_M0 + 2] * acc[ACC_M0];
+  Pk[5] -= cdip * acc[ACC_M0] * acc[ACC_M0 + 1];
+  out->En_kspace = Ek + Edip - c.alphar / std::sqrt(kPi) * c.sumq2;
+  out->En_el = out->En_kspace + acc[ACC_EEL];
+  out->En_pot = acc[ACC_U];
+  out->En_vir = acc[ACC_VIR];
+  for (int i = 0; i < 6; i++) out->Pvir[i] = Pk[i] + acc[ACC_PV0 + i];
+}
+
+}  // namespace mdx
+
+// ================================================================ C ABI
+
+extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
+  if (!out || !s) return fail(MDX_E_ARG, "mdx_create: NULL argument");
+  *out = nullptr;
+  if (mode != MDX_FP64 && mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_create: bad mode");
+  if (s->ns <= 0 || s->nmol <= 0 || s->ntypes <= 0 || !s->mol_first || !s->site_type ||
+      !s->charge || !s->excl || !s->sstab || !s->erfc_tab || s->erfc_n <= 0)
+    return fail(MDX_E_ARG, "mdx_create: incomplete system");
+  if (s->ntypes > 0xffff) return fail(MDX_E_ARG, "mdx_create: too many site types");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(MDX_E_NODEV, "mdx_create: no HIP device");
+  for (int n = 0; n < s->nmol; n++) {
+    if (s->mol_first[n + 1] - s->mol_first[n] > 32)
+      return fail(MDX_E_ARG, "mdx_create: molecules with more than 32 sites are not supported");
+    if (s->mol_first[n + 1] < s->mol_first[n]) return fail(MDX_E_ARG, "mdx_create: mol_first");
+  }
+  if (s->mol_first[0] != 0 || s->mol_first[s->nmol] != s->ns)
+    return fail(MDX_E_ARG, "mdx_create: mol_first does not cover the sites");
+  for (int d = 0; d < 3; d++)
+    if (!(s->L[d] > 2 * s->cutoff)) return fail(MDX_E_ARG, "mdx_create: cutoff >= L/2");
+  if ((int)(s->cutoff * s->cutoff * s->erfc_sgrid) >= s->erfc_n)
+    return fail(MDX_E_ARG, "mdx_create: erfc table shorter than the cutoff");
+
+  mdx_ctx_s *h = new mdx_ctx_s;
+  Ctx &c = h->c;
+  int rc = 0;
+  auto bail = [&](int code) {
+    mdx_destroy(h);
+    return code;
+  };
+  c.mode = mode;
+  (void)hipGetDevice(&c.device);
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(MDX_E_GPU, "hipStreamCreate failed"));
+  c.ns = s->ns;
+  c.nmol = s->nmol;
+  c.ntypes = s->ntypes;
+  for (int d = 0; d < 3; d++) { c.L[d] = s->L[d]; c.cook_ncell[d] = s->ncell[d]; }
+  c.cutoff = s->cutoff;
+  c.alpha = s->alpha;
+  c.alphar = s->alphar > 0 ? s->alphar : s->alpha;
+  c.kappa = s->kappa;
+  c.epsinf = s->epsinf;
+
+  // host copies of the static site data
+  std::vector<int> mol(c.ns), qsite;
+  for (int n = 0; n < c.nmol; n++)
+    for (int i = s->mol_first[n]; i < s->mol_first[n + 1]; i++) mol[i] = n;
+  c.sumq2 = 0;
+  for (int i = 0; i < c.ns; i++)
+    if (s->charge[i] != 0) { qsite.push_back(i); c.sumq2 += s->charge[i] * s->charge[i]; }
+  c.nq = (int)qsite.size();
+
+  // LJ table, FP64 and FP32 (A4 of LJ-free pairs is zero)
+  const int nt2 = c.ntypes * c.ntypes;
+  std::vector<double> lj(nt2 * 8);
+  for (int k = 0; k < nt2; k++) {
+    const mdx_lj_t &t = s->sstab[k];
+    double v[8] = {t.Sq, t.E4, t.E24, t.E48, t.C1q, t.C2q, t.A, t.A4};
+    if (!(t.C2q > 0)) for (int m = 4; m < 8; m++) v[m] = 0;
+    std::copy(v, v + 8, lj.begin() + 8 * k);
+  }
+  std::vector<float> lj32(lj.begin(), lj.end());
+  // splines: FP64 cook form, FP32 stable form around the interval start
+  c.nerf = s->erfc_n;
+  c.erf_sgrid = s->erfc_sgrid;
+  std::vector<double> e64(8 * c.nerf);
+  std::vector<float> e32(8 * c.nerf);
+  for (int i = 0; i < c.nerf; i++) {
+    const mdx_erfc_t &p = s->erfc_tab[i];
+    double v[8] = {p.Au, p.Bu, p.Cu, p.Ad, p.Bd, p.Cd, 0, 0};
+    std::copy(v, v + 8, e64.begin() + 8 * i);
+    const double x0 = i / c.erf_sgrid;
+    const double Du = p.Cu + x0, bu = p.Bu / Du, Dd = p.Cd + x0, bd = p.Bd / Dd;
+    const float w[8] = {(float)(bu + p.Au), (float)bu, (float)Du, (float)(bd + p.Ad), (float)bd,
+                        (float)Dd, 0, 0};
+    std::copy(w, w + 8, e32.begin() + 8 * i);
+  }
+
+  if ((rc = c.r.alloc(3 * c.ns)) || (rc = c.f.alloc(3 * c.ns)) || (rc = c.type.alloc(c.ns)) ||
+      (rc = c.mol.alloc(c.ns)) || (rc = c.molfirst.alloc(c.nmol + 1)) ||
+      (rc = c.charge.alloc(c.ns)) || (rc = c.excl.alloc(c.ns)) || (rc = c.q_site.alloc(c.nq)) ||
+      (rc = c.lj64.alloc(lj.size())) || (rc = c.lj32.alloc(lj32.size())) ||
+      (rc = c.erf64.alloc(e64.size())) || (rc = c.erf32.alloc(e32.size())) ||
+      (rc = c.cellid.alloc(c.ns)) || (rc = c.sidx.alloc(c.ns)) || (rc = c.spos.alloc(4 * c.ns)) ||
+      (rc = c.sinfo.alloc(4 * c.ns)) || (rc = c.scan_tmp.alloc(c.ns)) || (rc = c.acc.alloc(ACC_N)))
+    return bail(rc);
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up(c.type.p, s->site_type, sizeof(int) * c.ns) || !up(c.mol.p, mol.data(), sizeof(int) * c.ns) ||
+      !up(c.molfirst.p, s->mol_first, sizeof(int) * (c.nmol + 1)) ||
+      !up(c.charge.p, s->charge, sizeof(double) * c.ns) ||
+      !up(c.excl.p, s->excl, sizeof(unsigned) * c.ns) ||
+      (c.nq && !up(c.q_site.p, qsite.data(), sizeof(int) * c.nq)) ||
+      !up(c.lj64.p, lj.data(), sizeof(double) * lj.size()) ||
+      !up(c.lj32.p, lj32.data(), sizeof(float) * lj32.size()) ||
+      !up(c.erf64.p, e64.data(), sizeof(double) * e64.size()) ||
+      !up(c.erf32.p, e32.data(), sizeof(float) * e32.size()))
+    return bail(fail(MDX_E_GPU, "mdx_create: upload failed"));
+
+  // dynamics data (optional)
+  if (s->mass) {
+    c.ncons = s->ncons;
+    if ((rc = c.r1.alloc(3 * c.ns)) || (rc = c.pwork.alloc(3 * c.ns)) || (rc = c.mass.alloc(c.ns)) ||
+        (rc = c.cons_first.alloc(c.nmol + 1)) || (rc = c.cons_ij.alloc(2 * std::max(1, c.ncons))) ||
+        (rc = c.cons_bondq.alloc(std::max(1, c.ncons))) || (rc = c.mol_omega.alloc(c.nmol)))
+      return bail(rc);
+    std::vector<double> om(c.nmol, 0.6);
+    if (s->mol_omega) std::copy(s->mol_omega, s->mol_omega + c.nmol, om.begin());
+    std::vector<int> cf(c.nmol + 1, 0);
+    if (s->cons_first) std::copy(s->cons_first, s->cons_first + c.nmol + 1, cf.begin());
+    if (!up(c.mass.p, s->mass, sizeof(double) * c.ns) ||
+        !up(c.cons_first.p, cf.data(), sizeof(int) * (c.nmol + 1)) ||
+        (c.ncons && !up(c.cons_ij.p, s->cons_ij, sizeof(int) * 2 * c.ncons)) ||
+        (c.ncons && !up(c.cons_bondq.p, s->cons_bondq, sizeof(double) * c.ncons)) ||
+        !up(c.mol_omega.p, om.data(), sizeof(double) * c.nmol))
+      return bail(fail(MDX_E_GPU, "mdx_create: upload failed"));
+  }
+
+  if ((rc = choose_grid(c, 2)) || (rc = alloc_grid(c))) return bail(rc);
+  if (c.nq && (rc = kspace_setup(c))) return bail(rc);
+  *out = h;
+  return 0;
+}
+
+extern "C" int mdx_destroy(mdx_ctx_t *h) {
+  if (!h) return 0;
+  Ctx &c = h->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  DBuf<double> *dbl[] = {&c.r, &c.f, &c.charge, &c.lj64, &c.erf64, &c.spos, &c.part, &c.acc,
+                         &c.kpart, &c.tsum, &c.coef64, &c.kfrc, &c.r1, &c.pwork, &c.mass,
+                         &c.cons_bondq, &c.mol_omega};
+  for (auto *b : dbl) b->release();
+  DBuf<int> *ints[] = {&c.type, &c.mol, &c.molfirst, &c.q_site, &c.cellid, &c.cnt, &c.cstart,
+                       &c.cursor, &c.sidx, &c.sinfo, &c.scan_tmp, &c.d_row_kx, &c.d_row_ky,
+                       &c.d_row_kzmax, &c.d_row_coef0, &c.d_seg_blk, &c.d_seg_row0,
+                       &c.d_seg_rowend, &c.cons_first, &c.cons_ij};
+  for (auto *b : ints) b->release();
+  c.excl.release();
+  c.lj32.release();
+  c.erf32.release();
+  c.coef32.release();
+  c.tm.release();
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+  return 0;
+}
+
+extern "C" int mdx_forces(mdx_ctx_t *h, const double *r, double *f, mdx_out_t *out, int measure) {
+  if (!h || !r || !f || !out) return fail(MDX_E_ARG, "mdx_forces: NULL argument");
+  Ctx &c = h->c;
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
+  int rc = forces_device(c, measure);
+  if (rc) return rc;
+  double acc[ACC_N];
+  MDX_HIP(hipMemcpyAsync(f, c.f.p, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  fill_out(c, acc, measure, out);
+  return 0;
+}
+
+extern "C" int mdx_set_box(mdx_ctx_t *h, const double L[3]) {
+  if (!h || !L) return fail(MDX_E_ARG, "mdx_set_box: NULL argument");
+  Ctx &c = h->c;
+  for (int d = 0; d < 3; d++)
+    if (!(L[d] > 2 * c.cutoff)) return fail(MDX_E_ARG, "mdx_set_box: cutoff >= L/2");
+  for (int d = 0; d < 3; d++) c.L[d] = L[d];
+  int rc;
+  if ((rc = choose_grid(c, c.wst)) || (rc = alloc_grid(c))) return rc;
+  return 0;  // k-space uses c.L at every call (ekk recomputed, same k-set)
+}
+
+// ------------------------------------------------- debugging / test hooks
+
+// cook's cell id of every site for cook's No.cell (membership check)
+extern "C" int mdx_cook_cells(mdx_ctx_t *h, const double *r, int *cellid) {
+  if (!h || !r || !cellid) return fail(MDX_E_ARG, "mdx_cook_cells: NULL argument");
+  Ctx &c = h->c;
+  const int *n = c.cook_ncell;
+  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(MDX_E_ARG, "mdx_cook_cells: ncell not set");
+  double rL[3];
+  for (int k = 0; k < 3; k++) {
+    // rL shaved until (int)(minuseps(L)*rL) < No.cell (cook/linklist.c:15-29,81-86)
+    auto minuseps = [](double x) {
+      double y = x * 0.999999999999999667;
+      for (int i = 0; i < 5; i++) {
+        volatile double z = (x + y) / 2;
+        if (z >= x) return y;
+        y = z;
+      }
+      return y;
+    };
+    volatile double t = n[k] / c.L[k];
+    rL[k] = t;
+    const double x = minuseps(c.L[k]);
+    while ((int)(x * rL[k]) >= n[k]) rL[k] = minuseps(rL[k]);
+  }
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
+  k_cook_cellid<<<nblk(c.ns), kBlock, 0, c.stream>>>(c.r.p, c.ns, n[0], n[1], n[2], c.L[0], c.L[1],
+                                                     c.L[2], rL[0], rL[1], rL[2], c.cellid.p);
+  MDX_HIP(hipGetLastError());
+  MDX_HIP(hipMemcpyAsync(cellid, c.cellid.p, sizeof(int) * c.ns, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+// kernel timing (HIP events on the context stream) of the following calls
+extern "C" int mdx_set_timing(mdx_ctx_t *h, int on) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_timing: NULL");
+  h->c.timing = on != 0;
+  return 0;
+}
+
+// total ms and launch count per kernel family since the last call:
+// bin, pair, excl, kpass1, kcoef, kpass2, shake (n <= 7)
+extern "C" int mdx_get_timing(mdx_ctx_t *h, double *ms, int *count, int n) {
+  if (!h || !ms || !count) return fail(MDX_E_ARG, "mdx_get_timing: NULL");
+  Ctx &c = h->c;
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  double m[T_NFAM];
+  int k[T_NFAM];
+  c.tm.collect(m, k);
+  for (int i = 0; i < n && i < T_NFAM; i++) { ms[i] = m[i]; count[i] = k[i]; }
+  return 0;
+}
+
+// engine cell grid (stencil half width w), for the autotuner
+extern "C" int mdx_set_stencil(mdx_ctx_t *h, int w) {
+  if (!h || w < 1 || w > 4) return fail(MDX_E_ARG, "mdx_set_stencil: w in 1..4");
+  Ctx &c = h->c;
+  int rc;
+  if ((rc = choose_grid(c, w)) || (rc = alloc_grid(c))) return rc;
+  return c.wst == w ? 0 : fail(MDX_E_ARG, "mdx_set_stencil: box too small for this w");
+}
+
+extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
+  if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
+  Ctx &c = h->c;
+  int v[12] = {c.nc[0], c.nc[1], c.nc[2], c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+               c.K.ntrip, c.nsplit, c.K.kxmax, c.mode};
+  for (int i = 0; i < n && i < 12; i++) iv[i] = v[i];
+  return 0;
+}

@@ -1,0 +1,193 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mdx.h"
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#endif
+
+namespace mdx {
+
+// records the message of the last failure of this thread, returns code
+int fail(int code, const std::string &msg);
+
+#ifdef __HIPCC__
+
+#define MDX_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      return ::mdx::fail(MDX_E_GPU, std::string(#call ": ") +             \
+                                        hipGetErrorString(e_));          \
+  } while (0)
+
+// a device buffer owned by a context
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    if (count <= n && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) count = 1;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+      return fail(MDX_E_GPU, "hipMalloc failed");
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// number of accumulated scalars per block in the energy/virial reductions
+enum Acc {
+  ACC_U = 0,      // LJ energy
+  ACC_EEL,        // r-space Coulomb energy (incl. exclusion corrections)
+  ACC_VIR,        // sum rr*f over LJ terms
+  ACC_PV0,        // 6 virial tensor components xx yy zz yz zx xy
+  ACC_NPAIR = ACC_PV0 + 6,
+  ACC_NLJ,
+  ACC_EK,         // k-space energy sum_k w e 4 sum T^2
+  ACC_PK0,        // 6 k-space tensor sums (before the 2/(4 pi) factor)
+  ACC_M0 = ACC_PK0 + 6,  // dipole moment M (3)
+  ACC_SUMQ = ACC_M0 + 3, // sum q^2 (constant, kept for checking)
+  ACC_N
+};
+
+// Kernel timing by family with HIP events recorded on the launching stream
+// around each launch; totals are read (after a stream sync) by mdx_get_timing.
+enum TimerFamily { T_BIN = 0, T_PAIR, T_EXCL, T_KPASS1, T_KCOEF, T_KPASS2, T_SHAKE, T_NFAM };
+
+struct Timer {
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> rec;
+  size_t used = 0;
+  hipEvent_t open[T_NFAM] = {};
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void start(int fam, hipStream_t s) {
+    open[fam] = get();
+    if (open[fam]) (void)hipEventRecord(open[fam], s);
+  }
+  void stop(int fam, hipStream_t s) {
+    hipEvent_t e = get();
+    if (!e || !open[fam]) return;
+    (void)hipEventRecord(e, s);
+    rec.push_back({fam, {open[fam], e}});
+    open[fam] = nullptr;
+  }
+  // sums the recorded intervals per family (stream must be synchronized)
+  void collect(double *ms, int *count) {
+    for (int f = 0; f < T_NFAM; f++) { ms[f] = 0; count[f] = 0; }
+    for (auto &r : rec) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, r.second.first, r.second.second) == hipSuccess) {
+        ms[r.first] += t;
+        count[r.first]++;
+      }
+    }
+    rec.clear();
+    used = 0;
+  }
+  void release() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    rec.clear();
+    used = 0;
+  }
+};
+
+// k-space layout: rows (kx,ky) sorted by kz limit, 8-wide kz blocks
+struct KLayout {
+  int kxmax = 0, nrows = 0, nblk = 0, nseg = 0, ntrip = 0;
+  int kzpad = 0;                       // 8*nblk
+  std::vector<int> row_kx, row_ky, row_kzmax, row_coef0;
+  std::vector<int> seg_blk, seg_row0, seg_rowend;
+};
+
+struct Ctx {
+  int mode = MDX_MIXED;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // system
+  int ns = 0, nmol = 0, ntypes = 0, nq = 0;
+  double L[3] = {0, 0, 0};
+  double cutoff = 0, alpha = 0, alphar = 0, kappa = 0, epsinf = 3e33;
+  double sumq2 = 0;
+  int cook_ncell[3] = {0, 0, 0};
+  // site data
+  DBuf<double> r, f;          // [3ns] input positions, output forces
+  DBuf<int> type, mol, molfirst, q_site;
+  DBuf<double> charge;        // [ns]
+  DBuf<unsigned> excl;        // [ns]
+  DBuf<double> lj64;          // [ntypes^2*8]
+  DBuf<float> lj32;
+  DBuf<double> erf64;         // [nerf*8] stable spline form
+  DBuf<float> erf32;
+  int nerf = 0;
+  double erf_sgrid = 0;
+  // engine cell grid (pair kernel)
+  int nc[3] = {0, 0, 0};
+  int wst = 2;                 // stencil half width in cells
+  DBuf<int> cellid, cnt, cstart, cursor, sidx;
+  DBuf<double> spos;           // [4ns] x,y,z wrapped, q  (cell order)
+  DBuf<int> sinfo;             // [4ns] type, mol, excl, site
+  DBuf<int> scan_tmp;
+  // reductions
+  DBuf<double> part;           // per-block partial sums [nblocks*ACC_N]
+  int part_blocks = 0;
+  DBuf<double> acc;            // [ACC_N] final sums
+  // k-space
+  KLayout K;
+  int nsplit = 1;
+  DBuf<int> d_row_kx, d_row_ky, d_row_kzmax, d_row_coef0;
+  DBuf<int> d_seg_blk, d_seg_row0, d_seg_rowend;
+  DBuf<double> kpart;          // pass-1 partials [nsplit][nseg][64][64]
+  DBuf<double> tsum;           // reduced T sums [nseg][64][64]
+  DBuf<float> coef32;          // pass-2 coefficients [ntrip*16]
+  DBuf<double> coef64;
+  DBuf<double> kfrc;           // [3nq] k-space forces per charge (FP64)
+  // MD state
+  DBuf<double> r1, pwork;
+  DBuf<double> mass;
+  DBuf<int> cons_first, cons_ij;
+  DBuf<double> cons_bondq, mol_omega;
+  int ncons = 0;
+  bool state_on_device = false;
+  // kernel timing with HIP events on the context stream (off by default)
+  bool timing = false;
+  Timer tm;
+};
+
+// set-up of the k-space layout for the context box and kappa
+int kspace_setup(Ctx &c);
+// forces of the charges from the k-space sum (adds into c.f); energy and
+// tensor sums into c.acc when measure
+int kspace_forces(Ctx &c, int measure);
+
+#endif  // __HIPCC__
+
+}  // namespace mdx
+
+#ifdef __HIPCC__
+// the opaque handle of include/mdx.h
+struct mdx_ctx_s {
+  mdx::Ctx c;
+};
+#endif

@@ -1,0 +1,510 @@
+// Ewald k-space on MI355X: structure-factor sums, energy / pressure tensor,
+// and the per-charge forces.
+//
+// Reference: Ewald(1) and Ewald(2) of sim/ewald.c:562-1003 with
+// sim/ewpass1.c (Q(k) sums by recursive phase products over the half
+// space kx>=0 with the four sign variants of (ky,kz)), sim/ewpass2m.c:
+// 189-300 (energy, analytic pressure tensor, forces) and sim/ewaldf.c.
+//
+// Formulation used here.  For a triple (kx,ky,kz) of non-negative integers
+// with phases a=2pi kx x/Lx, b=2pi ky y/Ly, c=2pi kz z/Lz of a charge, the
+// four structure factors Q(kx,+-ky,+-kz) of the reference are fixed by the
+// eight real sums
+//     T_pqr = sum_j q_j P_p(a_j) P_q(b_j) P_r(c_j),   P_0=cos, P_1=sin,
+// (e.g. Re Q(kx,s ky,t kz) = T_ccc - st T_css - s T_ssc - t T_scs), and
+//     sum_{s,t} |Q|^2 = 4 sum T_pqr^2,
+//     force on j = -q_j grad_j sum_triples w e(k)/(2pi) 4 sum T_pqr P_p P_q P_r,
+// where e(k) = 4 exp(-pi^2 k^2/alpha^2)/(k^2 V) is the reference's ekk and w
+// = 2^(#nonzero components)/8 removes the sign variants that coincide for
+// zero components.  Pass 1 is then a real GEMM: rows (kx,ky) x 4 cos/sin
+// products of a and b, columns kz x {cos,sin} of c, summed over charges.
+// Pass 2 contracts per charge the coefficients 4 w e T over kz and takes
+// the gradient analytically.  Both are FP32 VALU work in MDX_MIXED with
+// FP64 accumulation across charge chunks, and FP64 throughout in MDX_FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "mdx_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kJC = 32;          // charges per LDS chunk in pass 1
+constexpr int kFlush = 8;        // FP32 chunk sums added to FP64 every kFlush chunks
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename R>
+struct C2 { R x, y; };
+
+inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+__device__ inline double frac1(double x) { return x - floor(x); }
+
+// cos, sin of 2 pi t in FP64
+__device__ inline void cs2pi(double t, double &c, double &s) { sincospi(2 * frac1(t), &s, &c); }
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------ pass 1
+// blockIdx.x: group of 4 segments (a segment = 64 rows of one kz block);
+// blockIdx.y: charge split.  Lane <-> row, the charge loop is shared, the
+// kz block of a wave is uniform so its cos/sin of c are broadcast reads.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_sfac(
+    int nq, const int *__restrict__ qsite, const double *__restrict__ r,
+    const double *__restrict__ charge, double iLx, double iLy, double iLz, int nseg,
+    const int *__restrict__ seg_blk, const int *__restrict__ seg_row0,
+    const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
+    const int *__restrict__ row_ky, int KXT, int KYT, int KZT, int jper,
+    double *__restrict__ kpart) {
+  extern __shared__ char smem[];
+  C2<R> *Xs = (C2<R> *)smem;            // [kJC][KXT], q folded in
+  C2<R> *Ys = Xs + kJC * KXT;           // [kJC][KYT]
+  C2<R> *Zs = Ys + kJC * KYT;           // [kJC][KZT]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int seg = blockIdx.x * 4 + wv;
+  const bool segok = seg < nseg;
+  int blk = 0, kx = 0, ky = 0;
+  if (segok) {
+    blk = seg_blk[seg];
+    const int row = seg_row0[seg] + lane;
+    if (row < seg_rowend[seg]) { kx = row_kx[row]; ky = row_ky[row]; }
+  }
+  blk = __builtin_amdgcn_readfirstlane(blk);
+  const int j0 = blockIdx.y * jper, j1 = min(nq, j0 + jper);
+  const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
+
+  // R=float: chunk sums in acc, added every kFlush chunks (256 charges) to
+  // mid; the splits are then summed in FP64.  R=double: acc only.
+  R acc[8][8], mid[8][8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int m = 0; m < 8; m++) { acc[k][m] = 0; mid[k][m] = 0; }
+
+  int nchunk = 0;
+  for (int jc = j0; jc < j1; jc += kJC) {
+    // cos/sin tables of this chunk: anchors e^{2 pi i 8m s} and the base
+    // e^{2 pi i s} in FP64, 7 recursion steps in R
+    for (int t = threadIdx.x; t < kJC * ntask; t += kBlock) {
+      const int jj = t / ntask, task = t % ntask;
+      int d, m, KT;
+      C2<R> *dst;
+      if (task < ntx) { d = 0; m = task; KT = KXT; dst = Xs + jj * KXT; }
+      else if (task < ntx + nty) { d = 1; m = task - ntx; KT = KYT; dst = Ys + jj * KYT; }
+      else { d = 2; m = task - ntx - nty; KT = KZT; dst = Zs + jj * KZT; }
+      const int j = jc + jj;
+      if (j < j1) {
+        const int site = qsite[j];
+        const double iL = d == 0 ? iLx : (d == 1 ? iLy : iLz);
+        const double sd = frac1(r[3 * site + d] * iL);
+        const double qf = d == 0 ? charge[site] : 1.0;
+        double c0, s0, c1, s1;
+        cs2pi(8.0 * m * sd, c0, s0);
+        cs2pi(sd, c1, s1);
+        c0 *= qf; s0 *= qf;
+        R ec = (R)c0, es = (R)s0;
+        const R bc = (R)c1, bs = (R)s1;
+        for (int k = 8 * m; k < min(8 * m + 8, KT); k++) {
+          dst[k].x = ec;
+          dst[k].y = es;
+          const R nc = ec * bc - es * bs;
+          es = ec * bs + es * bc;
+          ec = nc;
+        }
+      } else {
+        for (int k = 8 * m; k < min(8 * m + 8, KT); k++) { dst[k].x = 0; dst[k].y = 0; }
+      }
+    }
+    __syncthreads();
+    const C2<R> *zrow = Zs + 8 * blk;
+#pragma unroll 1
+    for (int jj = 0; jj < kJC; jj++) {
+      const C2<R> X = Xs[jj * KXT + kx], Y = Ys[jj * KYT + ky];
+      const R A[4] = {X.x * Y.x, X.x * Y.y, X.y * Y.x, X.y * Y.y};
+      const C2<R> *z = zrow + jj * KZT;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const C2<R> zk = z[k];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+          acc[k][2 * p] += A[p] * zk.x;
+          acc[k][2 * p + 1] += A[p] * zk.y;
+        }
+      }
+    }
+    __syncthreads();
+    if (sizeof(R) == 4 && ++nchunk == kFlush) {
+      nchunk = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+#pragma unroll
+        for (int m = 0; m < 8; m++) { mid[k][m] += acc[k][m]; acc[k][m] = 0; }
+    }
+  }
+  if (!segok) return;
+  double *out = kpart + (((size_t)blockIdx.y * nseg + seg) * 64 + lane) * 64;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int m = 0; m < 8; m++) out[k * 8 + m] = (double)mid[k][m] + (double)acc[k][m];
+}
+
+// sum of the charge splits: tsum[i] = sum_s kpart[s*n + i]
+__global__ void k_ksum(const double *__restrict__ kpart, int nsplit, size_t n,
+                       double *__restrict__ tsum) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  for (int k = 0; k < nsplit; k++) s += kpart[(size_t)k * n + i];
+  tsum[i] = s;
+}
+
+// ------------------------------------------------- energy, tensor, coefs
+// one thread per (segment, lane, kz in block).  Energy and tensor follow
+// sim/ewpass2m.c:194-222 written with the T sums; coefficients for pass 2:
+// [0..8): 4 w e T_pq{c,s}, [8..16): kz/Lz * 4 w e (T_pqs, -T_pqc).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_kcoef(
+    int nseg, const int *__restrict__ seg_blk, const int *__restrict__ seg_row0,
+    const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
+    const int *__restrict__ row_ky, const int *__restrict__ row_kzmax,
+    const int *__restrict__ row_coef0, const double *__restrict__ tsum, double Lx, double Ly,
+    double Lz, double alpha, R *__restrict__ coef, double *__restrict__ part, int measure) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int seg = t >> 9, lane = (t >> 3) & 63, k = t & 7;
+  double E = 0, pk[6] = {0, 0, 0, 0, 0, 0};
+  if (seg < nseg) {
+    const int row = seg_row0[seg] + lane;
+    const int kz = 8 * seg_blk[seg] + k;
+    if (row < seg_rowend[seg] && kz <= row_kzmax[row]) {
+      const int kx = row_kx[row], ky = row_ky[row];
+      R *cf = coef + (size_t)(row_coef0[row] + kz) * 16;
+      if (kx == 0 && ky == 0 && kz == 0) {
+        for (int m = 0; m < 16; m++) cf[m] = 0;
+      } else {
+        const double *T = tsum + ((size_t)seg * 64 + lane) * 64 + k * 8;
+        const int nz = (kx > 0) + (ky > 0) + (kz > 0);
+        const double w = (double)(1 << (nz - 1)) * 0.25;
+        const double kxl = kx / Lx, kyl = ky / Ly, kzl = kz / Lz;
+        const double kk = kxl * kxl + kyl * kyl + kzl * kzl;
+        const double PIaq = -(kPi / alpha) * (kPi / alpha);
+        const double e = 4 / (kk * (Lx * Ly * Lz)) * exp(PIaq * kk);
+        const double we4 = 4 * w * e;
+        for (int m = 0; m < 4; m++) {
+          cf[2 * m] = (R)(we4 * T[2 * m]);
+          cf[2 * m + 1] = (R)(we4 * T[2 * m + 1]);
+          cf[8 + 2 * m] = (R)(kzl * we4 * T[2 * m + 1]);
+          cf[8 + 2 * m + 1] = (R)(-kzl * we4 * T[2 * m]);
+        }
+        if (measure) {
+          double s2 = 0;
+          for (int m = 0; m < 8; m++) s2 += T[m] * T[m];
+          const double ptf = 1 / kk - PIaq;
+          const double E1 = w * e * 4 * s2;
+          // sum_{s,t} st|Q|^2, sum t|Q|^2, sum s|Q|^2 in terms of T
+          double sst = 0, st = 0, ss = 0;
+          for (int p = 0; p < 2; p++)
+            sst += T[p * 4 + 3] * T[p * 4 + 0] - T[p * 4 + 2] * T[p * 4 + 1];
+          for (int q = 0; q < 2; q++)
+            st += T[4 + q * 2 + 1] * T[q * 2 + 0] - T[4 + q * 2 + 0] * T[q * 2 + 1];
+          for (int rr = 0; rr < 2; rr++) ss += T[6 + rr] * T[rr] - T[4 + rr] * T[2 + rr];
+          E = E1;
+          pk[0] = E1 * ptf * kxl * kxl;
+          pk[1] = E1 * ptf * kyl * kyl;
+          pk[2] = E1 * ptf * kzl * kzl;
+          pk[3] = w * e * ptf * (-8 * sst) * kyl * kzl;
+          pk[4] = w * e * ptf * (-8 * st) * kxl * kzl;
+          pk[5] = w * e * ptf * (-8 * ss) * kxl * kyl;
+        }
+      }
+    }
+  }
+  if (measure) {
+    __shared__ double red[kBlock / 64][7];
+    const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double v[7] = {E, pk[0], pk[1], pk[2], pk[3], pk[4], pk[5]};
+    for (int m = 0; m < 7; m++) {
+      const double s = wave_sum(v[m]);
+      if (ln == 0) red[wv][m] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+      double s = 0;
+      for (int w2 = 0; w2 < kBlock / 64; w2++) s += red[w2][threadIdx.x];
+      part[(size_t)blockIdx.x * mdx::ACC_N + mdx::ACC_EK + threadIdx.x] += s;
+    }
+  }
+}
+
+// dipole moment M = sum q r of the unwrapped sites (sim/ewald.c:644)
+__global__ __launch_bounds__(kBlock) void k_dipole(int nq, const int *__restrict__ qsite,
+                                                   const double *__restrict__ r,
+                                                   const double *__restrict__ charge,
+                                                   double *__restrict__ part) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  double m[3] = {0, 0, 0};
+  if (j < nq) {
+    const int s = qsite[j];
+    for (int d = 0; d < 3; d++) m[d] = charge[s] * r[3 * s + d];
+  }
+  __shared__ double red[kBlock / 64][3];
+  const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int d = 0; d < 3; d++) {
+    const double s = wave_sum(m[d]);
+    if (ln == 0) red[wv][d] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[w][threadIdx.x];
+    part[(size_t)blockIdx.x * mdx::ACC_N + mdx::ACC_M0 + threadIdx.x] += s;
+  }
+}
+
+__global__ void k_reduce_cols(const double *part, int nblocks, int a0, int a1, double *acc) {
+  __shared__ double red[kBlock];
+  for (int a = a0; a < a1; a++) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * mdx::ACC_N + a];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[a] = red[0];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ pass 2
+// Lane <-> charge.  Rows (kx,ky) in lexicographic order; cos/sin of c for
+// kz < KZ live in registers, of a from FP64 sincospi at each new kx, of b by
+// FP64 recursion along ky.  The 16 coefficients of a triple are uniform
+// across the wave (scalar loads).  G, Gz accumulate over kz in R, forces
+// in FP64.  Adds -q_j grad Psi (+ dipole field) to f.
+template <typename R, int KZ>
+__global__ __launch_bounds__(kBlock) void k_kforce(
+    int nq, const int *__restrict__ qsite, const double *__restrict__ r,
+    const double *__restrict__ charge, double Lx, double Ly, double Lz, int kxmax,
+    const int *__restrict__ lex_kymax, const int *__restrict__ lex_row0,
+    const int *__restrict__ row_kzmax_lex, const int *__restrict__ row_coef0_lex,
+    const R *__restrict__ coef, const double *__restrict__ acc, double cdip,
+    double *__restrict__ f) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  const int site = qsite[min(j, nq - 1)];
+  const double sx = frac1(r[3 * site] / Lx), sy = frac1(r[3 * site + 1] / Ly),
+               sz = frac1(r[3 * site + 2] / Lz);
+  R zc[KZ], zs[KZ];
+  {
+    double c1, s1;
+    cs2pi(sz, c1, s1);
+#pragma unroll
+    for (int m = 0; m < KZ / 8; m++) {
+      double c, s;
+      cs2pi(8.0 * m * sz, c, s);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        zc[8 * m + k] = (R)c;
+        zs[8 * m + k] = (R)s;
+        const double nc = c * c1 - s * s1;
+        s = c * s1 + s * c1;
+        c = nc;
+      }
+    }
+  }
+  double cb1, sb1;
+  cs2pi(sy, cb1, sb1);
+  double Fx = 0, Fy = 0, Fz = 0;
+  for (int kx = 0; kx <= kxmax; kx++) {
+    double ca, sa;
+    cs2pi(kx * sx, ca, sa);
+    const R rca = (R)ca, rsa = (R)sa;
+    const R kxl = (R)(kx / Lx);
+    const int kym = lex_kymax[kx], rbase = lex_row0[kx];
+    double cb = 1, sb = 0;
+    for (int ky = 0; ky <= kym; ky++) {
+      const int row = rbase + ky;
+      const int kzm = __builtin_amdgcn_readfirstlane(row_kzmax_lex[row]);
+      const R *cf = coef + (size_t)__builtin_amdgcn_readfirstlane(row_coef0_lex[row]) * 16;
+      R G0 = 0, G1 = 0, G2 = 0, G3 = 0, H0 = 0, H1 = 0, H2 = 0, H3 = 0;
+      // fully unrolled (register arrays); kzm is wave-uniform
+#pragma unroll
+      for (int kz = 0; kz < KZ; kz++) {
+        if (kz > kzm) continue;
+        const R *e = cf + kz * 16;
+        const R c = zc[kz], s = zs[kz];
+        G0 += e[0] * c + e[1] * s;
+        G1 += e[2] * c + e[3] * s;
+        G2 += e[4] * c + e[5] * s;
+        G3 += e[6] * c + e[7] * s;
+        H0 += e[8] * c + e[9] * s;
+        H1 += e[10] * c + e[11] * s;
+        H2 += e[12] * c + e[13] * s;
+        H3 += e[14] * c + e[15] * s;
+      }
+      const R rcb = (R)cb, rsb = (R)sb;
+      const R kyl = (R)(ky / Ly);
+      Fx += (double)(kxl * (rca * (rcb * G2 + rsb * G3) - rsa * (rcb * G0 + rsb * G1)));
+      Fy += (double)(kyl * (rca * (rcb * G1 - rsb * G0) + rsa * (rcb * G3 - rsb * G2)));
+      Fz += (double)(rca * (rcb * H0 + rsb * H1) + rsa * (rcb * H2 + rsb * H3));
+      const double nc = cb * cb1 - sb * sb1;
+      sb = cb * sb1 + sb * cb1;
+      cb = nc;
+    }
+  }
+  if (j >= nq) return;
+  const double q = charge[site];
+  const double *M = acc + mdx::ACC_M0;
+  f[3 * site] += -q * Fx - q * cdip * M[0];
+  f[3 * site + 1] += -q * Fy - q * cdip * M[1];
+  f[3 * site + 2] += -q * Fz - q * cdip * M[2];
+}
+
+template <typename R>
+int run_kspace(mdx::Ctx &c, int measure) {
+  using namespace mdx;
+  const KLayout &K = c.K;
+  const int KXT = K.kxmax + 1, KYT = K.row_ky.empty() ? 1 : *std::max_element(K.row_ky.begin(), K.row_ky.end()) + 1;
+  const int KZT = K.kzpad;
+  const size_t shmem = sizeof(C2<R>) * kJC * (KXT + KYT + KZT);
+  const int jper = (c.nq + c.nsplit - 1) / c.nsplit;
+  dim3 g1((K.nseg + 3) / 4, c.nsplit);
+  if (c.timing) c.tm.start(T_KPASS1, c.stream);
+  k_sfac<R><<<g1, kBlock, shmem, c.stream>>>(
+      c.nq, c.q_site.p, c.r.p, c.charge.p, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
+      c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p, KXT, KYT, KZT,
+      jper, c.kpart.p);
+  MDX_HIP(hipGetLastError());
+  if (c.timing) { c.tm.stop(T_KPASS1, c.stream); c.tm.start(T_KCOEF, c.stream); }
+  const size_t nt = (size_t)K.nseg * 64 * 64;
+  k_ksum<<<nblk(nt), kBlock, 0, c.stream>>>(c.kpart.p, c.nsplit, nt, c.tsum.p);
+  R *coef = nullptr;
+  if constexpr (sizeof(R) == 4) coef = c.coef32.p; else coef = (R *)c.coef64.p;
+  k_kcoef<R><<<nblk((long)K.nseg * 512), kBlock, 0, c.stream>>>(
+      K.nseg, c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p,
+      c.d_row_kzmax.p, c.d_row_coef0.p, c.tsum.p, c.L[0], c.L[1], c.L[2], c.alpha, coef, c.part.p,
+      measure);
+  MDX_HIP(hipGetLastError());
+  const double V = c.L[0] * c.L[1] * c.L[2];
+  const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
+  k_dipole<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.q_site.p, c.r.p, c.charge.p, c.part.p);
+  k_reduce_cols<<<1, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, ACC_M0, ACC_M0 + 3, c.acc.p);
+  if (c.timing) { c.tm.stop(T_KCOEF, c.stream); c.tm.start(T_KPASS2, c.stream); }
+  const int *lk = c.d_row_kx.p + K.nrows;  // lexicographic tables stored after the sorted ones
+  const int *lr0 = c.d_row_ky.p + K.nrows;
+  const int *lkz = c.d_row_kzmax.p + K.nrows;
+  const int *lc0 = c.d_row_coef0.p + K.nrows;
+#define MDX_KF(KZV)                                                                          \
+  k_kforce<R, KZV><<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.q_site.p, c.r.p, c.charge.p, \
+                                                         c.L[0], c.L[1], c.L[2], K.kxmax, lk, \
+                                                         lr0, lkz, lc0, coef, c.acc.p, cdip, c.f.p)
+  if (KZT <= 16) MDX_KF(16);
+  else if (KZT <= 32) MDX_KF(32);
+  else if (KZT <= 48) MDX_KF(48);
+  else if (KZT <= 64) MDX_KF(64);
+  else return fail(MDX_E_ARG, "k-space: kz limit above 63 is not supported");
+#undef MDX_KF
+  MDX_HIP(hipGetLastError());
+  if (c.timing) c.tm.stop(T_KPASS2, c.stream);
+  return 0;
+}
+
+}  // namespace
+
+namespace mdx {
+
+// k-vector set of Ewald(0) for box L and kappa (sim/ewald.c:501-534): the
+// rows (kx,ky), kx in [0,(int)(L0 kappa)], ky in [0,kymax(kx)], with kz
+// from !(kx+ky) to kzmax(kx,ky).  Rows are sorted by kz limit (pass 1
+// segments) and also kept in lexicographic order (pass 2).
+int kspace_setup(Ctx &c) {
+  KLayout &K = c.K;
+  K = KLayout();
+  const double k2 = c.kappa * c.kappa;
+  K.kxmax = (int)(c.L[0] * c.kappa);
+  std::vector<int> lx, ly, lz, lkymax(K.kxmax + 1), lrow0(K.kxmax + 1);
+  for (int kx = 0; kx <= K.kxmax; kx++) {
+    const double rx = k2 - (kx / c.L[0]) * (kx / c.L[0]);
+    const int kym = (int)(c.L[1] * std::sqrt(rx));
+    lkymax[kx] = kym;
+    lrow0[kx] = (int)lx.size();
+    for (int ky = 0; ky <= kym; ky++) {
+      const double ry = rx - (ky / c.L[1]) * (ky / c.L[1]);
+      lx.push_back(kx);
+      ly.push_back(ky);
+      lz.push_back((int)(c.L[2] * std::sqrt(ry)));
+    }
+  }
+  K.nrows = (int)lx.size();
+  int kzmax_all = *std::max_element(lz.begin(), lz.end());
+  K.nblk = kzmax_all / 8 + 1;
+  K.kzpad = 8 * K.nblk;
+  // coefficient offsets in lexicographic order
+  std::vector<int> lcoef(K.nrows);
+  int off = 0;
+  for (int i = 0; i < K.nrows; i++) { lcoef[i] = off; off += lz[i] + 1; }
+  K.ntrip = off;
+  // rows sorted by kz limit (descending), stable
+  std::vector<int> ord(K.nrows);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return lz[a] > lz[b]; });
+  for (int i : ord) {
+    K.row_kx.push_back(lx[i]);
+    K.row_ky.push_back(ly[i]);
+    K.row_kzmax.push_back(lz[i]);
+    K.row_coef0.push_back(lcoef[i]);
+  }
+  for (int b = 0; b < K.nblk; b++) {
+    int n = 0;
+    while (n < K.nrows && K.row_kzmax[n] >= 8 * b) n++;
+    for (int r0 = 0; r0 < n; r0 += 64) {
+      K.seg_blk.push_back(b);
+      K.seg_row0.push_back(r0);
+      K.seg_rowend.push_back(std::min(r0 + 64, n));
+    }
+  }
+  K.nseg = (int)K.seg_blk.size();
+  // charge splits: about 8 workgroups per CU, at least 256 charges each
+  const int wg = (K.nseg + 3) / 4;
+  c.nsplit = std::max(1, std::min(2048 / std::max(wg, 1), c.nq / 256));
+  // device copies; the lexicographic tables follow the sorted ones
+  auto cat = [](std::vector<int> a, const std::vector<int> &b) {
+    a.insert(a.end(), b.begin(), b.end());
+    return a;
+  };
+  std::vector<int> kx_all = cat(K.row_kx, lkymax), ky_all = cat(K.row_ky, lrow0);
+  std::vector<int> kz_all = cat(K.row_kzmax, lz), c0_all = cat(K.row_coef0, lcoef);
+  int rc;
+  if ((rc = c.d_row_kx.alloc(kx_all.size())) || (rc = c.d_row_ky.alloc(ky_all.size())) ||
+      (rc = c.d_row_kzmax.alloc(kz_all.size())) || (rc = c.d_row_coef0.alloc(c0_all.size())) ||
+      (rc = c.d_seg_blk.alloc(K.nseg)) || (rc = c.d_seg_row0.alloc(K.nseg)) ||
+      (rc = c.d_seg_rowend.alloc(K.nseg)) ||
+      (rc = c.kpart.alloc((size_t)c.nsplit * K.nseg * 64 * 64)) ||
+      (rc = c.tsum.alloc((size_t)K.nseg * 64 * 64)))
+    return rc;
+  if (c.mode == MDX_MIXED) { if ((rc = c.coef32.alloc((size_t)K.ntrip * 16))) return rc; }
+  else if ((rc = c.coef64.alloc((size_t)K.ntrip * 16))) return rc;
+  auto up = [&](int *dst, const std::vector<int> &v) {
+    return hipMemcpy(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up(c.d_row_kx.p, kx_all) || !up(c.d_row_ky.p, ky_all) || !up(c.d_row_kzmax.p, kz_all) ||
+      !up(c.d_row_coef0.p, c0_all) || !up(c.d_seg_blk.p, K.seg_blk) ||
+      !up(c.d_seg_row0.p, K.seg_row0) || !up(c.d_seg_rowend.p, K.seg_rowend))
+    return fail(MDX_E_GPU, "kspace_setup: upload failed");
+  return 0;
+}
+
+int kspace_forces(Ctx &c, int measure) {
+  return c.mode == MDX_FP64 ? run_kspace<double>(c, measure) : run_kspace<float>(c, measure);
+}
+
+}  // namespace mdx

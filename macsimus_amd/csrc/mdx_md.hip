@@ -1,0 +1,239 @@
+// Device-resident MD step: leap-frog Verlet + SHAKE per molecule, kinetic
+// energy/tensor, constraint virial and the Berendsen thermostat, following
+// Shake() of sim/constrd.c:622-1081 for SHAKE=1 (simplified formula,
+// sim/constrd.c:805-839), VERLET=3 (sim/shakev3.c), sim/verlet.c (no
+// thermostat branch), sim/virc.c and sim/thermo.c:156-160.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "mdx_internal.h"
+
+namespace mdx {
+int forces_device(Ctx &c, int measure);
+void fill_out(const Ctx &c, const double *acc, int measure, mdx_out_t *out);
+}  // namespace mdx
+
+using namespace mdx;
+
+namespace {
+
+constexpr int kBlock = 256;
+enum { MD_EKIN = 0, MD_PKIN = 1, MD_VIRC = 7, MD_PTV = 8, MD_ITMAX = 14, MD_FAIL = 15, MD_N = 16 };
+
+inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// one thread per molecule: p = h^2 f/m + r + r1, SHAKE sweeps until the
+// largest relative violation is <= eps, then r1 = p - r, r = p
+__global__ __launch_bounds__(kBlock) void k_shake(
+    int nmol, const int *__restrict__ molfirst, const int *__restrict__ cons_first,
+    const int *__restrict__ cons_ij, const double *__restrict__ bondq,
+    const double *__restrict__ omega, const double *__restrict__ mass, double h, double eps,
+    int maxit, int measure, double *__restrict__ r, double *__restrict__ r1,
+    const double *__restrict__ f, double *__restrict__ p, double *__restrict__ part) {
+  const int m = blockIdx.x * kBlock + threadIdx.x;
+  double ek = 0, pk[6] = {0, 0, 0, 0, 0, 0}, virc = 0, ptv[6] = {0, 0, 0, 0, 0, 0};
+  double its = 0, bad = 0;
+  if (m < nmol) {
+    const double hh = h * h;
+    const int b = molfirst[m], e = molfirst[m + 1];
+    for (int i = b; i < e; i++) {
+      const double im = hh / mass[i];
+      for (int k = 0; k < 3; k++) p[3 * i + k] = im * f[3 * i + k] + r[3 * i + k] + r1[3 * i + k];
+    }
+    const int c0 = cons_first[m], c1 = cons_first[m + 1], nc = c1 - c0;
+    const double om = omega[m];
+    const int mit = maxit > 0 ? maxit : (int)(-16 * log(eps * 0.1)) * (nc + 4);
+    int it = 0;
+    if (nc) {
+      double maxsq;
+      do {
+        it++;
+        maxsq = 0;
+        for (int al = c0; al < c1; al++) {
+          const int a = cons_ij[2 * al], c = cons_ij[2 * al + 1];
+          const double bq = bondq[al], mi = 1 / mass[a], mj = 1 / mass[c];
+          double pij[3], rij[3];
+          for (int k = 0; k < 3; k++) { pij[k] = p[3 * a + k] - p[3 * c + k]; rij[k] = r[3 * a + k] - r[3 * c + k]; }
+          double sq = bq - (pij[0] * pij[0] + pij[1] * pij[1] + pij[2] * pij[2]);
+          maxsq = fmax(maxsq, fabs(sq / bq));
+          sq *= om / (mi + mj);
+          if (measure) {
+            const double w = sq / bq;
+            ptv[0] += w * rij[0] * rij[0]; ptv[1] += w * rij[1] * rij[1]; ptv[2] += w * rij[2] * rij[2];
+            ptv[3] += w * rij[1] * rij[2]; ptv[4] += w * rij[2] * rij[0]; ptv[5] += w * rij[0] * rij[1];
+          }
+          virc += sq;
+          sq /= bq;
+          for (int k = 0; k < 3; k++) {
+            const double d = rij[k] * sq;
+            p[3 * a + k] += mi * d;
+            p[3 * c + k] -= mj * d;
+          }
+        }
+      } while (maxsq > eps && it < mit);
+      if (maxsq > eps) bad = 1;
+    }
+    its = it;
+    for (int i = b; i < e; i++) {
+      double V1[3], V2[3];
+      for (int k = 0; k < 3; k++) { V1[k] = r1[3 * i + k]; V2[k] = p[3 * i + k] - r[3 * i + k]; }
+      const double mi = mass[i];
+      ek += (V1[0] * V1[0] + V1[1] * V1[1] + V1[2] * V1[2] + V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]) * mi;
+      pk[0] += (V1[0] * V1[0] + V2[0] * V2[0]) * mi;
+      pk[1] += (V1[1] * V1[1] + V2[1] * V2[1]) * mi;
+      pk[2] += (V1[2] * V1[2] + V2[2] * V2[2]) * mi;
+      pk[3] += (V1[1] * V1[2] + V2[1] * V2[2]) * mi;
+      pk[4] += (V1[2] * V1[0] + V2[2] * V2[0]) * mi;
+      pk[5] += (V1[0] * V1[1] + V2[0] * V2[1]) * mi;
+      for (int k = 0; k < 3; k++) { r1[3 * i + k] = V2[k]; r[3 * i + k] = p[3 * i + k]; }
+    }
+  }
+  __shared__ double red[kBlock / 64][MD_N];
+  const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double v[MD_N] = {ek, pk[0], pk[1], pk[2], pk[3], pk[4], pk[5], virc,
+                    ptv[0], ptv[1], ptv[2], ptv[3], ptv[4], ptv[5], its, bad};
+  for (int k = 0; k < MD_N; k++) {
+    const double s = k >= MD_ITMAX ? wave_max(v[k]) : wave_sum(v[k]);
+    if (ln == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < MD_N) {
+    double s = threadIdx.x >= MD_ITMAX ? 0 : 0;
+    for (int w = 0; w < kBlock / 64; w++)
+      s = threadIdx.x >= MD_ITMAX ? fmax(s, red[w][threadIdx.x]) : s + red[w][threadIdx.x];
+    part[(size_t)blockIdx.x * MD_N + threadIdx.x] = s;
+  }
+}
+
+__global__ void k_md_reduce(const double *part, int nblocks, double *acc) {
+  __shared__ double red[kBlock];
+  for (int a = 0; a < MD_N; a++) {
+    const bool mx = a >= MD_ITMAX;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock)
+      s = mx ? fmax(s, part[(size_t)b * MD_N + a]) : s + part[(size_t)b * MD_N + a];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o)
+        red[threadIdx.x] = mx ? fmax(red[threadIdx.x], red[threadIdx.x + o])
+                              : red[threadIdx.x] + red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[a] = red[0];
+    __syncthreads();
+  }
+}
+
+// Berendsen: r1 *= exp(blogT(T_kin, T) * -h/tau) with blogT = ln(E/T)/2
+// clamped to [-1,1] (sim/constrd.c:98-107); T_kin = 2 E_kin / No.f with
+// acc[MD_EKIN] = sum m (V1^2+V2^2) = 4 h^2 E_kin (leap-frog estimate)
+__global__ void k_berendsen(int n3, double *r1, const double *acc, double h, double T,
+                            double tau, double nf, double *lambda_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double ekin2 = acc[MD_EKIN] * 0.5 / (h * h);  // En.kin (doubled) after hscale
+  const double Tk = ekin2 / nf;
+  const double e2 = 7.38905609893065022723;
+  const double b = Tk > T * e2 ? 1 : (Tk * e2 < T ? -1 : log(Tk / T) / 2);
+  const double lam = exp(b * -h / tau);
+  if (i < n3) r1[i] *= lam;
+  if (i == 0 && lambda_out) *lambda_out = lam;
+}
+
+}  // namespace
+
+extern "C" int mdx_upload_state(mdx_ctx_t *h, const double *r, const double *r1) {
+  if (!h || !r || !r1) return fail(MDX_E_ARG, "mdx_upload_state: NULL argument");
+  Ctx &c = h->c;
+  if (!c.mass.p) return fail(MDX_E_ARG, "mdx_upload_state: system has no masses");
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
+  MDX_HIP(hipMemcpyAsync(c.r1.p, r1, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  c.state_on_device = true;
+  return 0;
+}
+
+extern "C" int mdx_download_state(mdx_ctx_t *h, double *r, double *r1) {
+  if (!h || !r || !r1) return fail(MDX_E_ARG, "mdx_download_state: NULL argument");
+  Ctx &c = h->c;
+  if (!c.state_on_device) return fail(MDX_E_ARG, "mdx_download_state: no state uploaded");
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipMemcpyAsync(r, c.r.p, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(r1, c.r1.p, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_step_out_t *out) {
+  if (!h || !p || !out || nsteps < 0) return fail(MDX_E_ARG, "mdx_md_steps: bad argument");
+  Ctx &c = h->c;
+  if (!c.state_on_device) return fail(MDX_E_ARG, "mdx_md_steps: upload the state first");
+  if (!(p->h > 0) || !(p->eps > 0)) return fail(MDX_E_ARG, "mdx_md_steps: h and eps must be > 0");
+  if (p->thermostat && !(p->tau > 0 && p->nf > 0 && p->T > 0))
+    return fail(MDX_E_ARG, "mdx_md_steps: Berendsen needs T, tau, nf > 0");
+  MDX_HIP(hipSetDevice(c.device));
+  const int nb = nblk(c.nmol);
+  DBuf<double> mdpart, mdacc, lam;
+  int rc;
+  if ((rc = mdpart.alloc((size_t)nb * MD_N)) || (rc = mdacc.alloc(MD_N)) || (rc = lam.alloc(1)))
+    return rc;
+  std::memset(out, 0, sizeof(*out));
+  out->lambda = 1;
+  const int every = p->measure_every > 0 ? p->measure_every : nsteps;
+  double acc[ACC_N], md[MD_N];
+  for (int s = 1; s <= nsteps; s++) {
+    const int measure = (s % every == 0) || s == nsteps;
+    if ((rc = forces_device(c, measure))) break;
+    if (c.timing) c.tm.start(T_SHAKE, c.stream);
+    k_shake<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.cons_first.p, c.cons_ij.p,
+                                         c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
+                                         p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
+                                         mdpart.p);
+    if (c.timing) c.tm.stop(T_SHAKE, c.stream);
+    k_md_reduce<<<1, kBlock, 0, c.stream>>>(mdpart.p, nb, mdacc.p);
+    if (p->thermostat == 1)
+      k_berendsen<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r1.p, mdacc.p, p->h,
+                                                            p->T, p->tau, p->nf, lam.p);
+    if ((rc = hipGetLastError()) != hipSuccess) { rc = fail(MDX_E_GPU, "mdx_md_steps: launch"); break; }
+    if (measure) {
+      if (hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+          hipMemcpyAsync(md, mdacc.p, sizeof(md), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+          hipStreamSynchronize(c.stream) != hipSuccess) {
+        rc = fail(MDX_E_GPU, "mdx_md_steps: download");
+        break;
+      }
+      if (md[MD_FAIL] > 0) { rc = fail(MDX_E_NOCONV, "SHAKE: too many iterations"); break; }
+      const double hh = p->h * p->h, hscale = 0.5 / hh;
+      fill_out(c, acc, 1, &out->frc);
+      out->En_kin = md[MD_EKIN] * hscale / 2;
+      for (int k = 0; k < 6; k++) {
+        out->Pkin[k] = md[MD_PKIN + k] * hscale;
+        out->PTvirc[k] = md[MD_PTV + k] / hh;
+      }
+      out->virc = md[MD_VIRC] / hh;
+      out->shake_iter_max = (int)md[MD_ITMAX];
+      if (p->thermostat == 1) {
+        double l = 1;
+        (void)hipMemcpy(&l, lam.p, sizeof(double), hipMemcpyDeviceToHost);
+        out->lambda = l;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(c.stream);
+  mdpart.release();
+  mdacc.release();
+  lam.release();
+  return rc;
+}

@@ -1,0 +1,246 @@
+// Host-side set-up of the force path: the numbers cook computes once at
+// start-up and hands to forces() through its globals.  Each helper states
+// the reference routine whose arithmetic it reproduces; the formulas have
+// to be the reference's (they define the model), the code is this
+// project's own.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mdx.h"
+#include "mdx_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrtPi = 1.772453850905516;
+// No.eps = machprec() (sim/siminit.c:64-66), the convergence scale of setss
+constexpr double kMachEps = 1.1102230246251565e-16;
+
+inline double p6(double x) { double y = x * x * x; return y * y; }
+
+// LJ in van der Waals form u = -eps[(sig/r)^12 - 2(sig/r)^6], eps < 0
+// (sim/lj/sitesite.h:79-80)
+struct VdW {
+  double sig, eps;
+  double u(double r) const { double x = p6(sig / r); return -eps * (x * x - 2 * x); }
+  double f(double r) const { double x = p6(sig / r); return -12 * eps * (x * x - x) / r; }
+};
+
+// combrule() for the square-root energy rule (comb_rule&1, also chosen by
+// cook when a site has no polarizability), sim/lj/sitesite.c:66-93
+VdW combine(double Ei, double Ri, double Ej, double Rj, int comb_rule) {
+  double r = (comb_rule & 2) ? std::pow(4 * Ri * Rj, 3) : p6(Ri + Rj);
+  double B = std::sqrt(-2 * Ei) * std::sqrt(-2 * Ej) * r;
+  double A = B * r;
+  VdW v;
+  v.sig = (B == 0) ? 0 : std::pow(A / B, 1.0 / 6);
+  v.eps = (A == 0) ? 0 : B * B / (-2 * A);
+  return v;
+}
+
+int logint(double x) {  // cook/linklist.c:294-302
+  int i = (int)(x + 0.9999);
+  if (i > 1 && x / (i - 1) < i / x) i--;
+  return i;
+}
+
+bool almost_integer(double x) {  // sim/ewald.c:240-250, machineprec=1e-9
+  x -= (int)x;
+  return std::fabs(0.5 - std::fabs(x - 0.5)) < 1e-9;
+}
+
+}  // namespace
+
+extern "C" int mdx_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej,
+                         double Rj, int comb_rule, double C2, double *corr) {
+  if (!ss) return mdx::fail(MDX_E_ARG, "mdx_setss: ss is NULL");
+  std::memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (Ei > 0 || Ej > 0) return mdx::fail(MDX_E_ARG, "mdx_setss: EvdW must be <= 0");
+  VdW v = combine(Ei, Ri, Ej, Rj, comb_rule);
+  // initssaux(), sim/lj/sitesite.c:96-104: LJsigma^2 = sig^2 / 2^(1/3)
+  ss->Sq = v.sig * v.sig * 0.7937005259840998;
+  ss->E4 = -4 * v.eps;
+  if (ss->Sq != 0) {
+    ss->E24 = -24 * v.eps / ss->Sq;
+    ss->E48 = -48 * v.eps / ss->Sq;
+  }
+  if (v.eps == 0 || C2 == 0) return 0;  // no LJ term: C1=C2=A=A4=0
+  // smooth cutoff: fixed point for C1 such that the quartic
+  // A (r^2 - C2^2)^2 on [C1,C2] matches u and f at C1 (sim/setss.c:244-296)
+  double C1 = 0.775 * C2, prev;
+  int n = 0;
+  do {
+    double fc = v.f(C1);
+    if (std::fabs(fc) < 1e-50) return mdx::fail(MDX_E_ARG, "mdx_setss: zero force at C1");
+    prev = C1;
+    double t = 1 + 4 * v.u(C1) / C1 / fc;
+    if (n++ > 2000 || t < 0)
+      return mdx::fail(MDX_E_NOCONV, "mdx_setss: no consistent smooth cutoff (LJcutoff too short)");
+    C1 = C2 / std::sqrt(t);
+  } while (std::fabs(1 - C1 / prev) > kMachEps * n);
+  if (C1 > 0.999 * C2 || C1 < 0.5 * C2) return mdx::fail(MDX_E_ARG, "mdx_setss: bad cutoff");
+  double d = C2 * C2 - C1 * C1;
+  ss->A = v.u(C1) / (d * d);
+  ss->A4 = -v.f(C1) / d / C1;
+  ss->C1q = C1 * C1;
+  ss->C2q = C2 * C2;
+  if (corr) {
+    // homogeneous tail: analytic quartic part + 2-point Gauss in 1/r on
+    // [C1,inf) with 100 intervals (sim/setss.c:470-499)
+    const int ng = 100;
+    const double g = 0.2113248654051871;
+    double c0 = 4 * kPi * ss->A *
+                (C1 * (p6(C1) / 7 - 0.4 * std::pow(C1, 4) * C2 * C2 + C1 * C1 * std::pow(C2, 4) / 3) -
+                 8.0 / 105 * std::pow(C2, 7));
+    double h = 1.0 / ng / C1, c1 = 0;
+    for (int k = 0; k < ng; k++) {
+      double r1 = 1 / ((k + g) * h), r2 = 1 / ((k + 1 - g) * h);
+      c1 += std::pow(r1, 4) * v.u(r1) + std::pow(r2, 4) * v.u(r2);
+    }
+    *corr = c0 + 2 * kPi * h * c1;
+  }
+  return 0;
+}
+
+extern "C" int mdx_erfc_size(int grid, double maxr, double alpha) {
+  return (int)(maxr * alpha * maxr * alpha * grid) + 1;
+}
+
+// Hyperbolic splines A + B/(C+x), x = r^2, of eru = erfc(alpha r)/r and
+// erd = -(1/r) d eru/dr on the grid 1/grid in (alpha r)^2, built from the
+// top down so that value and slope are continuous (sim/splines.c:23-100,
+// SPLINE=-2, SUBGRID=1 with library erfc, sim/erfc.c:68-80); shift&1/&2
+// subtract eru/erd at the cutoff (sim/erfc.c:177-187).
+extern "C" int mdx_erfc_table(mdx_erfc_t *tab, int *nout, double *sgrid,
+                              int grid, double maxr, double cutoff,
+                              double alpha, int shift) {
+  if (!tab || !nout || !sgrid || grid <= 0 || alpha <= 0 || maxr < cutoff)
+    return mdx::fail(MDX_E_ARG, "mdx_erfc_table: bad arguments");
+  const int n = mdx_erfc_size(grid, maxr, alpha);
+  const double h = 1.0 / grid;
+  std::vector<double> u(n + 3), d(n + 3);
+  for (int i = n + 2; i >= 0; i--) {
+    if (i == 0) { u[0] = d[0] = 9e9; continue; }
+    double x = i * h, r = std::sqrt(x), e = std::erfc(r);
+    u[i] = e / r;
+    d[i] = ((2 / kSqrtPi) * std::exp(-x) + e / r) / x;
+  }
+  // 4th-order one-sided start of the slope at the top node
+  double su = (-u[n + 2] + 8 * u[n + 1] - 8 * u[n - 1] + u[n - 2]) / (12 * h);
+  double sd = (-d[n + 2] + 8 * d[n + 1] - 8 * d[n - 1] + d[n - 2]) / (12 * h);
+  const double a1 = alpha, a2 = alpha * alpha, a3 = a2 * alpha;
+  double xhi = n * h;
+  for (int i = n - 1; i >= 0; i--) {
+    double xlo = i * h;
+    mdx_erfc_t &p = tab[i];
+    // fit B/(C+x)+A through (xlo,y[i]), (xhi,y[i+1]) with slope s at xhi
+    auto fit = [&](const std::vector<double> &y, double &s, double &A, double &B, double &C) {
+      double dy = y[i + 1] - y[i];
+      C = (xlo * dy - xhi * s * h) / (s * h - dy);
+      B = -s * (C + xhi) * (C + xhi);
+      A = y[i + 1] + s * (C + xhi);
+      s = -B / ((C + xlo) * (C + xlo));  // slope at the lower node
+    };
+    fit(u, su, p.Au, p.Bu, p.Cu);
+    fit(d, sd, p.Ad, p.Bd, p.Cd);
+    // rescale from the alpha=1 variable (alpha r)^2 to r^2
+    p.Au *= a1; p.Bu /= a1; p.Cu /= a2;
+    p.Ad *= a3; p.Bd *= a1; p.Cd /= a2;
+    xhi = xlo;
+  }
+  const double sg = grid * a2, z = cutoff * cutoff;
+  const int iz = (int)(z * sg);
+  if (iz >= n) return mdx::fail(MDX_E_ARG, "mdx_erfc_table: cutoff outside the table");
+  if (shift & 1) {
+    double s = tab[iz].Bu / (tab[iz].Cu + z) + tab[iz].Au;
+    for (int i = 0; i < n; i++) tab[i].Au -= s;
+  }
+  if (shift & 2) {
+    double s = tab[iz].Bd / (tab[iz].Cd + z) + tab[iz].Ad;
+    for (int i = 0; i < n; i++) tab[i].Ad -= s;
+  }
+  *nout = n;
+  *sgrid = sg;
+  return 0;
+}
+
+// Ewaldparm(): alpha from the r-space error target, kappa by fixed-point
+// iteration from the k-space target (random-charge estimates),
+// sim/ewald.c:1044-1057.
+extern "C" int mdx_ewald_autoset(double *alpha, double *kappa, double qmax,
+                                 double sumq, double cutoff, const double L[3],
+                                 double epsr, double epsk) {
+  if (!alpha || !kappa || sumq <= 0 || cutoff <= 0)
+    return mdx::fail(MDX_E_ARG, "mdx_ewald_autoset: bad arguments");
+  const double V = L[0] * L[1] * L[2];
+  double a = std::log(2 * qmax * std::sqrt(sumq / cutoff / V) / epsr);
+  if (a < 0) return mdx::fail(MDX_E_ARG, "mdx_ewald_autoset: cannot determine alpha (epsr)");
+  a = std::sqrt(a) / cutoff;
+  double k = *kappa > 0 ? *kappa : 0.2, prev;
+  int left = 10000;
+  do {
+    if (!left--) return mdx::fail(MDX_E_NOCONV, "mdx_ewald_autoset: kappa iteration");
+    prev = k;
+    double t = std::log(qmax * a / (kPi * epsk) * std::sqrt(8 * sumq / (k * V)));
+    if (t <= 0) return mdx::fail(MDX_E_ARG, "mdx_ewald_autoset: cannot determine kappa (epsk)");
+    k = a * std::sqrt(t) / kPi;
+  } while (std::fabs(prev - k) > 1e-9);
+  *alpha = a;
+  *kappa = k;
+  return 0;
+}
+
+// k-vector census of Ewald(0) with the fix for limits that are almost
+// integers (kappa raised by 2e-9 relative, growing by 1.4 each retry),
+// sim/ewald.c:484-522.  Counts each (kx,ky,kz) with its sign variants.
+extern "C" int mdx_ewald_kvectors(double *kappa, const double L[3], int *nk) {
+  if (!kappa || !nk || *kappa <= 0) return mdx::fail(MDX_E_ARG, "mdx_ewald_kvectors: bad kappa");
+  double dK = 2e-9;
+  for (int attempt = 0; attempt <= 25; attempt++) {
+    const double K2 = *kappa * *kappa;
+    const int kxmax = (int)(L[0] * *kappa);
+    long count = 0;
+    bool retry = false;
+    for (int kx = 0; kx <= kxmax && !retry; kx++) {
+      double rx = K2 - (kx / L[0]) * (kx / L[0]);
+      double ly = L[1] * std::sqrt(rx);
+      if (almost_integer(ly)) { retry = true; break; }
+      for (int ky = 0; ky <= (int)ly; ky++) {
+        double ry = rx - (ky / L[1]) * (ky / L[1]);
+        double lz = L[2] * std::sqrt(ry);
+        if (almost_integer(lz)) { retry = true; break; }
+        for (int kz = (kx + ky) ? 0 : 1; kz <= (int)lz; kz++)
+          count += 1L << ((kx > 0) + (ky > 0) + (kz > 0) - 1);
+      }
+    }
+    if (!retry) { *nk = (int)count; return 0; }
+    *kappa += *kappa * dK;
+    dK *= 1.4;
+  }
+  return mdx::fail(MDX_E_NOCONV, "mdx_ewald_kvectors: cannot resolve rounding of the k-limits");
+}
+
+// lcsetup(): number of linked cells per direction from the target of
+// 1+ln(No.s) sites per cell (cook/linklist.c:304-337, serial build)
+extern "C" int mdx_lcsetup(int ncell[3], int ns, double percell, const double L[3]) {
+  if (!ncell || ns <= 0) return mdx::fail(MDX_E_ARG, "mdx_lcsetup: bad arguments");
+  if (percell <= 0) percell = 1 + std::log((double)ns);
+  int nundef = 0, prdc = 1;
+  double prdL = 1;
+  for (int i = 0; i < 3; i++) {
+    if (ncell[i] == 0) { nundef++; prdL *= L[i]; }
+    else prdc *= ncell[i];
+  }
+  for (int i = 0; i < 3; i++) {
+    if (ncell[i] != 0) continue;
+    double x = std::pow(ns / (percell * prdc * prdL), 1.0 / nundef);
+    ncell[i] = logint(x * L[i]);
+    if (ncell[i] < 2) ncell[i] = 2;
+    prdL /= L[i];
+    prdc *= ncell[i];
+    nundef--;
+  }
+  return 0;
+}

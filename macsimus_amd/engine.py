@@ -1,0 +1,104 @@
+"""Python handle on the MI355X engine (libmdx.so) through its C ABI.
+
+Mirrors cook's force call: `Engine.forces(r, measure)` is forces(B,A) of
+cook/forces.c:93-224 with A->rp = r and B->rp = the returned forces.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class Engine:
+    def __init__(self, built_system, mode=abi.MDX_MIXED, lib=None):
+        self.lib = lib or abi.load_engine()
+        self.bs = built_system            # keeps the arrays alive
+        self.ns = built_system.struct.ns
+        self.ctx = C.c_void_p()
+        rc = self.lib.mdx_create(C.byref(self.ctx), C.byref(built_system.struct), mode)
+        abi.check(self.lib, rc, "mdx_create")
+        self.mode = mode
+
+    def close(self):
+        if self.ctx:
+            self.lib.mdx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forces(self, r, measure=1):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.ns, 3)
+        f = np.empty_like(r)
+        out = abi.Out()
+        rc = self.lib.mdx_forces(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                 f.ctypes.data_as(C.POINTER(C.c_double)), C.byref(out),
+                                 int(measure))
+        abi.check(self.lib, rc, "mdx_forces")
+        return f, out
+
+    def cook_cells(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        cid = np.empty(self.ns, np.int32)
+        rc = self.lib.mdx_cook_cells(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                     cid.ctypes.data_as(C.POINTER(C.c_int)))
+        abi.check(self.lib, rc, "mdx_cook_cells")
+        return cid
+
+    def info(self):
+        iv = (C.c_int * 12)()
+        self.lib.mdx_info(self.ctx, iv, 12)
+        keys = ("ncx", "ncy", "ncz", "stencil", "nq", "krows", "kblocks", "ksegments",
+                "ktriples", "ksplits", "kxmax", "mode")
+        return dict(zip(keys, list(iv)))
+
+    def set_timing(self, on=True):
+        self.lib.mdx_set_timing(self.ctx, int(on))
+
+    FAMILIES = ("bin", "pair", "excl", "kpass1", "kcoef", "kpass2", "shake")
+
+    def timing(self):
+        """{family: (total ms, launches)} since the last call (HIP events)."""
+        ms = (C.c_double * 7)()
+        cnt = (C.c_int * 7)()
+        rc = self.lib.mdx_get_timing(self.ctx, ms, cnt, 7)
+        abi.check(self.lib, rc, "mdx_get_timing")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.FAMILIES)}
+
+    def set_stencil(self, w):
+        rc = self.lib.mdx_set_stencil(self.ctx, int(w))
+        abi.check(self.lib, rc, "mdx_set_stencil")
+
+    # ---- device-resident MD
+    def upload_state(self, r, r1):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        r1 = np.ascontiguousarray(r1, dtype=np.float64)
+        rc = self.lib.mdx_upload_state(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                       r1.ctypes.data_as(C.POINTER(C.c_double)))
+        abi.check(self.lib, rc, "mdx_upload_state")
+
+    def download_state(self):
+        r = np.empty((self.ns, 3))
+        r1 = np.empty((self.ns, 3))
+        rc = self.lib.mdx_download_state(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                         r1.ctypes.data_as(C.POINTER(C.c_double)))
+        abi.check(self.lib, rc, "mdx_download_state")
+        return r, r1
+
+    def md_steps(self, params, nsteps):
+        out = abi.StepOut()
+        rc = self.lib.mdx_md_steps(self.ctx, C.byref(params), int(nsteps), C.byref(out))
+        abi.check(self.lib, rc, "mdx_md_steps")
+        return out
+
+
+def declare_extras(lib):
+    lib.mdx_cook_cells.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.mdx_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
+    lib.mdx_set_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
+    lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int]

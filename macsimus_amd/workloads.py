@@ -1,0 +1,106 @@
+"""Synthetic seeded configurations of the benchmark workloads.
+
+SPC/E water (Berendsen et al. 1987: q_O=-0.8476 e, q_H=+0.4238 e, O-H 1 A,
+HOH 109.47 deg, O-O LJ sigma 3.166 A, eps 0.650 kJ/mol) and Smith-Dang
+Na+/Cl- (sigma 2.35/4.40 A, eps 0.544/0.419 kJ/mol), Lorentz-Berthelot
+combining, in cook's ble-file units (EvdW [kcal/mol] = -eps, RvdW = r_min/2).
+Molecules are placed on a cubic lattice in a random (seeded) order with
+random orientations, the way cook's init="crystal" does (sim/simils.c:
+436-600).
+"""
+import numpy as np
+
+AVOGADRO = 6.02214076e23
+KCAL_PER_KJ = 1 / 4.184
+TWO16 = 2 ** (1 / 6)
+HOH = np.deg2rad(109.47)
+HH = 2 * np.sin(HOH / 2)
+M_H, M_O, M_NA, M_CL = 1.008, 15.9994, 22.98977, 35.453
+
+
+def _type(mass, sigma, eps_kj):
+    return (mass, -eps_kj * KCAL_PER_KJ, sigma * TWO16 / 2 if eps_kj else 0.5)
+
+
+def brine_forcefield(with_ions=True):
+    types = ["H", "O"] + (["NA", "CL"] if with_ions else [])
+    tab = {"H": _type(M_H, 0, 0), "O": _type(M_O, 3.166, 0.650),
+           "NA": _type(M_NA, 2.35, 0.544), "CL": _type(M_CL, 4.40, 0.419)}
+    s, c = np.sin(HOH / 2), np.cos(HOH / 2)
+    water = dict(name="SPCE", site_type=[0, 0, 1], site_charge_e=[0.4238, 0.4238, -0.8476],
+                 site_xyz=[[s, c, 0], [-s, c, 0], [0, 0, 0]], site_excl=[[], [0], [0, 1]],
+                 bonds=[[0, 2, 1.0], [1, 2, 1.0], [0, 1, HH]])
+    species = []
+    if with_ions:
+        species.append(dict(name="NA", site_type=[2], site_charge_e=[1.0], site_xyz=[[0, 0, 0]],
+                            site_excl=[[]], bonds=[]))
+        species.append(dict(name="CL", site_type=[3], site_charge_e=[-1.0], site_xyz=[[0, 0, 0]],
+                            site_excl=[[]], bonds=[]))
+    species.append(water)
+    return dict(types=types, type_mass=[tab[t][0] for t in types],
+                type_EvdW_kcal=[tab[t][1] for t in types],
+                type_RvdW=[tab[t][2] for t in types], species=species)
+
+
+def box_length(counts_masses, density_g_cm3):
+    mass_g = sum(n * m for n, m in counts_masses) / AVOGADRO
+    return (mass_g / density_g_cm3 * 1e24) ** (1 / 3)
+
+
+def random_rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def lattice_config(ff, counts, L, nlat, seed):
+    """Positions (molecule-major, cook's order: species by species) of
+    counts[i] molecules of each species on an nlat^3 lattice, random slot
+    order and random orientations."""
+    rng = np.random.default_rng(seed)
+    nmol = sum(counts)
+    if nmol > nlat ** 3:
+        raise ValueError("lattice too small")
+    slots = rng.permutation(nlat ** 3)[:nmol]
+    a = L / nlat
+    centres = np.stack([slots // (nlat * nlat), (slots // nlat) % nlat, slots % nlat], 1) * a + a / 2
+    pos = []
+    m = 0
+    for isp, n in enumerate(counts):
+        geo = np.array(ff["species"][isp]["site_xyz"], float)
+        com = geo.mean(0)
+        geo = geo - com
+        rot = random_rotations(rng, n) if len(geo) > 1 else np.tile(np.eye(3), (n, 1, 1))
+        pos.append((centres[m:m + n, None, :] + np.einsum("nij,sj->nsi", rot, geo)).reshape(-1, 3))
+        m += n
+    return np.concatenate(pos)
+
+
+def brine_c2():
+    """64,000 SPC/E + 1,152 Na+/Cl- pairs = 194,304 sites, 1.03 g/cm3,
+    random placement on a 41^3 lattice (seed 2), 12 A cutoff."""
+    ff = brine_forcefield(True)
+    counts = [1152, 1152, 64000]
+    L = box_length([(1152, M_NA), (1152, M_CL), (64000, 2 * M_H + M_O)], 1.03)
+    r = lattice_config(ff, counts, L, 41, seed=2)
+    return dict(name="brine_194304", ff=ff, counts=counts, L=L, cutoff=12.0, r=r)
+
+
+def water_c1():
+    """1000 SPC/E on a 10^3 lattice, L=31.04 A, 9 A cutoff (seed 1)."""
+    ff = brine_forcefield(False)
+    counts = [1000]
+    r = lattice_config(ff, counts, 31.04, 10, seed=1)
+    return dict(name="spce_3000", ff=ff, counts=counts, L=31.04, cutoff=9.0, r=r)
+
+
+def degrees_of_freedom(ff, counts):
+    """No.f: 3 per site minus constraints minus 3 (total momentum)."""
+    nf = 0
+    for sp, n in zip(ff["species"], counts):
+        nf += n * (3 * len(sp["site_type"]) - len(sp["bonds"]))
+    return nf - 3

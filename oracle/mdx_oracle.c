@@ -874,3 +874,50 @@ int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
   free(pp);
   return 0;
 }
+
+/* ------------------------------------------------- CPU baseline sample */
+
+#include <time.h>
+static double orc_now(void)
+{
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+/* Times the two halves of one force call on a bounded sample: the full
+   r-space pass (linked cells + pair kernel) and the k-space passes over
+   every kstride-th charge only (the k-space cost is linear in the number
+   of charges for a fixed k-set).  t[0] = r-space seconds, t[1] = sampled
+   k-space seconds, t[2] = charges in the sample. */
+int oracle_time_sample(const mdx_system_t *S, const double *r, int nthreads,
+                       int kstride, double t[3])
+{
+  int ns = S->ns, i, nq = 0, err;
+  int *idx = malloc(sizeof(int) * ns);
+  double *q = malloc(sizeof(double) * ns);
+  double (*pos)[3] = malloc(sizeof(double[3]) * ns);
+  double *f = calloc(3 * (size_t)ns, sizeof(double));
+  double Pvir[6] = {0, 0, 0, 0, 0, 0}, t0;
+  orc_acc a;
+  int k = 0;
+  if (kstride < 1) kstride = 1;
+  for (i = 0; i < ns; i++)
+    if (S->charge[i] != 0 && (k++ % kstride) == 0) {
+      idx[nq] = i; q[nq] = S->charge[i];
+      pos[nq][0] = r[3 * i]; pos[nq][1] = r[3 * i + 1]; pos[nq][2] = r[3 * i + 2];
+      nq++;
+    }
+  memset(&a, 0, sizeof(a));
+  a.S = S;
+  a.measure = 0;
+  t0 = orc_now();
+  err = orc_pairs(&a, (const double (*)[3])r, (double (*)[3])f, NULL);
+  t[0] = orc_now() - t0;
+  t0 = orc_now();
+  if (nq) orc_ewald(S, nq, idx, q, (const double (*)[3])pos, (double (*)[3])f, 0, Pvir, nthreads);
+  t[1] = orc_now() - t0;
+  t[2] = nq;
+  free(idx); free(q); free(pos); free(f);
+  return err;
+}

@@ -261,6 +261,42 @@ def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
     print(name, {k: meta[k] for k in ("Epot", "Nk", "ncell") if k in meta})
 
 
+def traj_case(name, species, types, counts, L, cutoff, nsteps, thermostat="none"):
+    """MD trajectory of cook's Shake() (SHAKE + leap-frog, h=2 fs): the
+    positions of every force call are dumped by -v128 (at most 10 dumps).
+    dump 0 is the initial force evaluation, dump k (k>=1) holds r(t_{k-1})
+    and the forces of step k."""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "noint=%d" % nsteps, "T=298", "epsc=1e-12", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="%s"\ntau.T=0.1\nno=1;\n' % thermostat)
+        run([COOK, "sys", "md", "-v128"], td)
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "md.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "md.%d.dump" % k)))
+            k += 1
+        meta = parse_prt(os.path.join(td, "md.prt"))
+        txt = open(os.path.join(td, "md.prt")).read()
+        m = re.findall(r"omegac=(\S+) \(", txt)
+        if m:
+            meta["omegac_last"] = float(m[-1])
+    out = dict(L=dumps[0]["L"], q=dumps[0]["q"], mol=dumps[0]["mol"], site=dumps[0]["site"],
+               r=np.stack([d["r"] for d in dumps]), f=np.stack([d["f"] for d in dumps]))
+    meta.update(dict(name=name, counts=counts, L=L, epsr=0.05, epsk=0.05, h=0.002,
+                     thermostat=thermostat, ndumps=len(dumps),
+                     forcefield=ff_arrays(species, types)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps")
+
+
 def main(argv):
     subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8", "all"],
                    cwd=REPO, check=True)
@@ -277,6 +313,9 @@ def main(argv):
             "brine506", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 0.05, 0.05,
             md_steps=20),
+        # NVE trajectory of 216 SPC/E (SHAKE + leap-frog)
+        "spce216_nve": lambda: traj_case(
+            "spce216_nve", [spce], ["H", "O"], [216], 18.6206, 8.0, 8),
     }
     todo = argv or list(cases)
     for c in todo:
