@@ -139,10 +139,20 @@ typedef struct mdx_step_s {
   double tau;          /* Berendsen time constant [ps] */
   double nf;           /* No.f degrees of freedom */
   int measure_every;   /* energies/virial on every n-th step (cook: noint) */
+  /* POLAR: always-stable predictor of the Drude offsets (sim/pred2.c) of
+     order aspc_k (cook -p option /100), then scf_iter selffield()
+     iterations with mixing scf_omega (0 = the ASPC stability value
+     (k+2)/(2k+3), sim/pred2.c:75) */
+  int aspc_k;
+  int scf_iter;
+  double scf_omega;
 } mdx_step_t;
 
 typedef struct mdx_step_out_s {
-  mdx_out_t frc;       /* force-call outputs of the last measured step */
+  mdx_out_t frc;       /* force-call outputs of the last measured step (POLAR:
+                          incl. the selffield() terms of En_el and Pvir) */
+  double En_self;      /* POLAR: En.self, Drude self energy */
+  double scf_maxerr;   /* POLAR: max |d rpol| q_D/omega of the last iteration */
   double En_kin;       /* kinetic energy (VERLET=3 leap-frog estimate) */
   double Pkin[6];      /* kinetic tensor (not /V) */
   double virc;         /* constraint virial En.virc */
@@ -150,6 +160,15 @@ typedef struct mdx_step_out_s {
   double lambda;       /* last Berendsen scaling factor */
   int shake_iter_max;  /* max SHAKE sweeps used by a molecule */
 } mdx_step_out_t;
+
+/* Replaces one selffield() iteration (sim/scf.c:12-240, POLAR=0):
+   r, f: 2*ns rows (centres, then Drude offsets / Drude forces) as left by
+   mdx_forces; the Drude offsets in r are moved, the Drude forces are
+   passed to the centres in f; out->En_el and out->Pvir get the
+   centre-Drude Ewald correction and the Drude virial.  *Eself = En.self,
+   *maxerr = largest |d rpol| q_D / omega. */
+int mdx_selffield(mdx_ctx_t *ctx, double *r, double *f, mdx_out_t *out,
+                  double omega, double *Eself, double *maxerr);
 
 int mdx_upload_state(mdx_ctx_t *ctx, const double *r, const double *r1);
 int mdx_download_state(mdx_ctx_t *ctx, double *r, double *r1);

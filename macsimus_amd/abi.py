@@ -66,12 +66,14 @@ class Step(C.Structure):
         ("h", C.c_double), ("eps", C.c_double), ("maxit", C.c_int),
         ("thermostat", C.c_int), ("T", C.c_double), ("tau", C.c_double),
         ("nf", C.c_double), ("measure_every", C.c_int),
+        ("aspc_k", C.c_int), ("scf_iter", C.c_int), ("scf_omega", C.c_double),
     ]
 
 
 class StepOut(C.Structure):
     _fields_ = [
-        ("frc", Out), ("En_kin", C.c_double), ("Pkin", C.c_double * 6),
+        ("frc", Out), ("En_self", C.c_double), ("scf_maxerr", C.c_double),
+        ("En_kin", C.c_double), ("Pkin", C.c_double * 6),
         ("virc", C.c_double), ("PTvirc", C.c_double * 6),
         ("lambda_", C.c_double), ("shake_iter_max", C.c_int),
     ]
@@ -117,6 +119,7 @@ def load_engine(path=None):
     lib.mdx_destroy.argtypes = [C.c_void_p]
     lib.mdx_forces.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(Out), C.c_int]
     lib.mdx_set_box.argtypes = [C.c_void_p, C.c_double * 3]
+    lib.mdx_selffield.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(Out), C.c_double, _dp, _dp]
     lib.mdx_upload_state.argtypes = [C.c_void_p, _dp, _dp]
     lib.mdx_download_state.argtypes = [C.c_void_p, _dp, _dp]
     lib.mdx_md_steps.argtypes = [C.c_void_p, C.POINTER(Step), C.c_int,

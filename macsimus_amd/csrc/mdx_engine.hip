@@ -65,6 +65,11 @@ __global__ void k_zero(double *p, size_t n) {
   if (i < n) p[i] = 0;
 }
 
+__global__ void k_zero_int(int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // final reduction of per-block partials; one block, ACC_N columns
 __global__ void k_reduce(const double *part, int nblocks, double *acc) {
   __shared__ double red[kBlock];
@@ -157,16 +162,19 @@ __global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__rest
 }
 
 __global__ void k_gather(int ns, Grid g, const int *__restrict__ sidx, const double *__restrict__ r,
-                         const double *__restrict__ q, const int *__restrict__ type,
+                         const double *__restrict__ q, const double *__restrict__ qpol,
+                         const int *__restrict__ type,
                          const int *__restrict__ mol, const int *__restrict__ molfirst,
                          const unsigned *__restrict__ excl, const int *__restrict__ cellid,
-                         double4 *__restrict__ spos, int4 *__restrict__ sinfo,
-                         int *__restrict__ scell) {
+                         double4 *__restrict__ spos, double4 *__restrict__ spol,
+                         int4 *__restrict__ sinfo, int *__restrict__ scell) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   int i = sidx[s];
   spos[s] = make_double4(wrap1(r[3 * i], g.Lx), wrap1(r[3 * i + 1], g.Ly),
                          wrap1(r[3 * i + 2], g.Lz), q[i]);
+  if (qpol)  // Drude offset (relative to the centre) and charge
+    spol[s] = make_double4(r[3 * (ns + i)], r[3 * (ns + i) + 1], r[3 * (ns + i) + 2], qpol[i]);
   int m = mol[i];
   sinfo[s] = make_int4(type[i] | ((i - molfirst[m]) << 16), m, (int)excl[i], i);
   scell[s] = cellid[i];
@@ -236,23 +244,48 @@ __device__ inline float rcp<float>(float x) { return __frcp_rn(x); }
 template <>
 __device__ inline double rcp<double>(double x) { return 1.0 / x; }
 
+// one charge-charge term of the pair kernel: force/r from the spline at
+// the FP64 separation d, scattered to the accumulator of the central site
 template <typename R, bool MEASURE>
+__device__ inline void qq_term(const R *erf, double sgrid, R qq, const double (&d)[3],
+                               R (&acc)[3], double &Eel, double (&pv)[6]) {
+  const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  R eru = 0, erd;
+  spline<R>(erf, rr, sgrid, eru, erd, MEASURE);
+  const R fr = qq * erd;
+  const R dx = (R)d[0], dy = (R)d[1], dz = (R)d[2];
+  const R fx = fr * dx, fy = fr * dy, fz = fr * dz;
+  acc[0] += fx; acc[1] += fy; acc[2] += fz;
+  if (MEASURE) {
+    Eel += (double)(qq * eru);
+    pv[0] += (double)(dx * fx); pv[1] += (double)(dy * fy); pv[2] += (double)(dz * fz);
+    pv[3] += (double)(dz * fy); pv[4] += (double)(dx * fz); pv[5] += (double)(dy * fx);
+  }
+}
+
+// POLAR (cook/ljqqpol.c, POLAR=0): the pair is chosen by the centre-centre
+// distance; the Drude of a site sits at r+rpol with charge qpol; the
+// central site collects the forces on its centre (cc, cp) and on its
+// Drude (pc, pp); LJ acts between centres only.
+template <typename R, bool MEASURE, bool POLAR>
 __global__ __launch_bounds__(kBlock) void k_pair(
     int ns, Grid g, int w, double cq, const double4 *__restrict__ spos,
-    const int4 *__restrict__ sinfo, const int *__restrict__ scell,
-    const int *__restrict__ cstart, int nt, PairTab<R> tab, double sgrid,
-    double *__restrict__ f, double *__restrict__ part) {
+    const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
+    const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
+    double sgrid, double *__restrict__ f, double *__restrict__ part) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
-  double fx = 0, fy = 0, fz = 0;
+  double fx = 0, fy = 0, fz = 0, px = 0, py = 0, pz = 0;
   double U = 0, Eel = 0, Vir = 0, pv[6] = {0, 0, 0, 0, 0, 0};
   double npair = 0, nlj = 0;
   if (s < ns) {
     const double4 pi = spos[s];
+    double4 ri = make_double4(0, 0, 0, 0);
+    if (POLAR) ri = spol[s];
     const int4 ii = sinfo[s];
     const int ti = ii.x & 0xffff;
     const int c = scell[s];
     const int cz = c % g.nz, cy = (c / g.nz) % g.ny, cx = c / (g.nz * g.ny);
-    const R qi = (R)pi.w;
+    const R qi = (R)pi.w, qpi = (R)ri.w;
     const R *ljrow = tab.lj + (size_t)ti * nt * 8;
     for (int dx = -w; dx <= w; dx++) {
       int nx = cx + dx;
@@ -262,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(
         int ny = cy + dy;
         double sy = 0;
         if (ny < 0) { ny += g.ny; sy = -g.Ly; } else if (ny >= g.ny) { ny -= g.ny; sy = g.Ly; }
-        R ax = 0, ay = 0, az = 0;
+        R ac[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
         for (int dz = -w; dz <= w; dz++) {
           int nz = cz + dz;
           double sz = 0;
@@ -307,21 +340,45 @@ __global__ __launch_bounds__(kBlock) void k_pair(
             }
             const R ddx = (R)dxr, ddy = (R)dyr, ddz = (R)dzr;
             const R fdx = fr * ddx, fdy = fr * ddy, fdz = fr * ddz;
-            ax += fdx; ay += fdy; az += fdz;
+            ac[0] += fdx; ac[1] += fdy; ac[2] += fdz;
             if (MEASURE) {
               U += u;
               pv[0] += (double)(ddx * fdx); pv[1] += (double)(ddy * fdy); pv[2] += (double)(ddz * fdz);
               pv[3] += (double)(ddz * fdy); pv[4] += (double)(ddx * fdz); pv[5] += (double)(ddy * fdx);
             }
+            if (POLAR) {
+              const double4 rj = spol[j];
+              const R qpj = (R)rj.w, qj = (R)pj.w;
+              if (qpi != 0 && qj != 0) {        // Drude i - centre j
+                const double d[3] = {dxr + ri.x, dyr + ri.y, dzr + ri.z};
+                qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qj, d, ap, Eel, pv);
+              }
+              if (qpj != 0) {
+                if (qi != 0) {                  // centre i - Drude j
+                  const double d[3] = {dxr - rj.x, dyr - rj.y, dzr - rj.z};
+                  qq_term<R, MEASURE>(tab.erf, sgrid, qi * qpj, d, ac, Eel, pv);
+                }
+                if (qpi != 0) {                 // Drude i - Drude j
+                  const double d[3] = {dxr - rj.x + ri.x, dyr - rj.y + ri.y, dzr - rj.z + ri.z};
+                  qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qpj, d, ap, Eel, pv);
+                }
+              }
+            }
           }
         }
-        fx += ax; fy += ay; fz += az;
+        fx += ac[0]; fy += ac[1]; fz += ac[2];
+        if (POLAR) { px += ap[0]; py += ap[1]; pz += ap[2]; }
       }
     }
     const int site = ii.w;
     f[3 * site] = fx;
     f[3 * site + 1] = fy;
     f[3 * site + 2] = fz;
+    if (POLAR) {
+      f[3 * (ns + site)] = px;
+      f[3 * (ns + site) + 1] = py;
+      f[3 * (ns + site) + 2] = pz;
+    }
   }
   if (MEASURE) {
     double v[12] = {0.5 * U, 0.5 * Eel, 0.5 * Vir, 0.5 * pv[0], 0.5 * pv[1], 0.5 * pv[2],
@@ -360,10 +417,33 @@ __device__ inline double exact_erud(double rr, double alpha, double &erd) {
   return y;
 }
 
+// adds the exact correction of one charge pair (separation d, FP64)
 template <bool MEASURE>
-__global__ __launch_bounds__(kBlock) void k_excl(int nmol, const int *__restrict__ molfirst,
+__device__ inline void excl_term(double qq, const double (&d)[3], double alpha, double *fa,
+                                 double *fb, double &Eel, double (&pv)[6]) {
+  const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double erd;
+  const double u = exact_erud(rr, alpha, erd);
+  const double fr = erd * qq;
+  for (int k = 0; k < 3; k++) {
+    fa[k] += fr * d[k];
+    fb[k] -= fr * d[k];
+  }
+  if (MEASURE) {
+    Eel += u * qq;
+    pv[0] += d[0] * fr * d[0]; pv[1] += d[1] * fr * d[1]; pv[2] += d[2] * fr * d[2];
+    pv[3] += d[2] * fr * d[1]; pv[4] += d[0] * fr * d[2]; pv[5] += d[1] * fr * d[0];
+  }
+}
+
+// POLAR: all four centre/Drude combinations of an excluded pair get the
+// exact correction (cook/ljqqpol.c EXCL branch).  r, f: 2ns rows when
+// polar (Drude offsets / Drude forces after the centres).
+template <bool MEASURE>
+__global__ __launch_bounds__(kBlock) void k_excl(int nmol, int ns, const int *__restrict__ molfirst,
                                                  const double *__restrict__ r,
                                                  const double *__restrict__ q,
+                                                 const double *__restrict__ qpol,
                                                  const unsigned *__restrict__ excl, double Lx,
                                                  double Ly, double Lz, double cq, double alpha,
                                                  double *__restrict__ f, double *__restrict__ part) {
@@ -375,25 +455,27 @@ __global__ __launch_bounds__(kBlock) void k_excl(int nmol, const int *__restrict
       const unsigned mask = excl[a];
       for (int c = a + 1; c < e; c++) {
         if (!((mask >> (c - b)) & 1)) continue;
-        const double qq = q[a] * q[c];
-        if (qq == 0) continue;
         double d[3] = {r[3 * a] - r[3 * c], r[3 * a + 1] - r[3 * c + 1], r[3 * a + 2] - r[3 * c + 2]};
         d[0] -= Lx * rint(d[0] / Lx);
         d[1] -= Ly * rint(d[1] / Ly);
         d[2] -= Lz * rint(d[2] / Lz);
         const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (rr >= cq) continue;
-        double erd;
-        const double u = exact_erud(rr, alpha, erd);
-        const double fr = erd * qq;
-        for (int k = 0; k < 3; k++) {
-          f[3 * a + k] += fr * d[k];
-          f[3 * c + k] -= fr * d[k];
+        if (q[a] * q[c] != 0) excl_term<MEASURE>(q[a] * q[c], d, alpha, f + 3 * a, f + 3 * c, Eel, pv);
+        if (!qpol) continue;
+        const double *ra = r + 3 * (ns + a), *rc = r + 3 * (ns + c);
+        double *fpa = f + 3 * (ns + a), *fpc = f + 3 * (ns + c);
+        if (qpol[a] * q[c] != 0) {
+          const double dd[3] = {d[0] + ra[0], d[1] + ra[1], d[2] + ra[2]};
+          excl_term<MEASURE>(qpol[a] * q[c], dd, alpha, fpa, f + 3 * c, Eel, pv);
         }
-        if (MEASURE) {
-          Eel += u * qq;
-          pv[0] += d[0] * fr * d[0]; pv[1] += d[1] * fr * d[1]; pv[2] += d[2] * fr * d[2];
-          pv[3] += d[2] * fr * d[1]; pv[4] += d[0] * fr * d[2]; pv[5] += d[1] * fr * d[0];
+        if (q[a] * qpol[c] != 0) {
+          const double dd[3] = {d[0] - rc[0], d[1] - rc[1], d[2] - rc[2]};
+          excl_term<MEASURE>(q[a] * qpol[c], dd, alpha, f + 3 * a, fpc, Eel, pv);
+        }
+        if (qpol[a] * qpol[c] != 0) {
+          const double dd[3] = {d[0] - rc[0] + ra[0], d[1] - rc[1] + ra[1], d[2] - rc[2] + ra[2]};
+          excl_term<MEASURE>(qpol[a] * qpol[c], dd, alpha, fpa, fpc, Eel, pv);
         }
       }
     }
@@ -442,35 +524,43 @@ int alloc_grid(Ctx &c) {
 int bin_sites(Ctx &c, const double *d_r) {
   const Grid g = make_grid(c, c.nc);
   const int ncell = c.nc[0] * c.nc[1] * c.nc[2];
-  MDX_HIP(hipMemsetAsync(c.cnt.p, 0, sizeof(int) * ncell, c.stream));
+  k_zero_int<<<nblk(ncell), kBlock, 0, c.stream>>>(c.cnt.p, ncell);
   k_cellid<<<nblk(c.ns), kBlock, 0, c.stream>>>(d_r, c.ns, g, c.cellid.p, c.cnt.p);
   k_scan<<<1, 1024, 0, c.stream>>>(c.cnt.p, ncell, c.cstart.p, c.cursor.p);
   k_scatter<<<nblk(c.ns), kBlock, 0, c.stream>>>(c.cellid.p, c.ns, c.cstart.p, c.cursor.p, c.sidx.p);
   k_cellsort<<<nblk(ncell), kBlock, 0, c.stream>>>(ncell, c.cstart.p, c.sidx.p);
   k_gather<<<nblk(c.ns), kBlock, 0, c.stream>>>(
-      c.ns, g, c.sidx.p, d_r, c.charge.p, c.type.p, c.mol.p, c.molfirst.p, c.excl.p, c.cellid.p,
-      (double4 *)c.spos.p, (int4 *)c.sinfo.p, c.scan_tmp.p);
+      c.ns, g, c.sidx.p, d_r, c.charge.p, c.polar ? c.chargepol.p : nullptr, c.type.p, c.mol.p,
+      c.molfirst.p, c.excl.p, c.cellid.p, (double4 *)c.spos.p, (double4 *)c.spol.p,
+      (int4 *)c.sinfo.p, c.scan_tmp.p);
   MDX_HIP(hipGetLastError());
   return 0;
 }
 
-template <typename R>
-int launch_pair(Ctx &c, int measure) {
+template <typename R, bool POLAR>
+int launch_pair_t(Ctx &c, int measure) {
   const Grid g = make_grid(c, c.nc);
   PairTab<R> tab;
   if constexpr (sizeof(R) == 4) { tab.lj = c.lj32.p; tab.erf = c.erf32.p; }
   else { tab.lj = (const R *)c.lj64.p; tab.erf = (const R *)c.erf64.p; }
   const double cq = c.cutoff * c.cutoff;
   if (measure)
-    k_pair<R, true><<<nblk(c.ns), kBlock, 0, c.stream>>>(
-        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
-        c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p, c.part.p);
+    k_pair<R, true, POLAR><<<nblk(c.ns), kBlock, 0, c.stream>>>(
+        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p,
+        c.part.p);
   else
-    k_pair<R, false><<<nblk(c.ns), kBlock, 0, c.stream>>>(
-        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
-        c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p, c.part.p);
+    k_pair<R, false, POLAR><<<nblk(c.ns), kBlock, 0, c.stream>>>(
+        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p,
+        c.part.p);
   MDX_HIP(hipGetLastError());
   return 0;
+}
+
+template <typename R>
+int launch_pair(Ctx &c, int measure) {
+  return c.polar ? launch_pair_t<R, true>(c, measure) : launch_pair_t<R, false>(c, measure);
 }
 
 }  // namespace
@@ -488,16 +578,17 @@ int forces_device(Ctx &c, int measure) {
   rc = c.mode == MDX_FP64 ? launch_pair<double>(c, measure) : launch_pair<float>(c, measure);
   if (rc) return rc;
   if (c.timing) { c.tm.stop(T_PAIR, c.stream); c.tm.start(T_EXCL, c.stream); }
-  if (measure)
-    k_excl<true><<<nblk(c.nmol), kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.r.p, c.charge.p,
-                                                        c.excl.p, c.L[0], c.L[1], c.L[2],
-                                                        c.cutoff * c.cutoff, c.alphar, c.f.p,
-                                                        c.part.p);
-  else
-    k_excl<false><<<nblk(c.nmol), kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.r.p,
-                                                         c.charge.p, c.excl.p, c.L[0], c.L[1],
-                                                         c.L[2], c.cutoff * c.cutoff, c.alphar,
-                                                         c.f.p, c.part.p);
+  {
+    const double *qp = c.polar ? c.chargepol.p : nullptr;
+    if (measure)
+      k_excl<true><<<nblk(c.nmol), kBlock, 0, c.stream>>>(
+          c.nmol, c.ns, c.molfirst.p, c.r.p, c.charge.p, qp, c.excl.p, c.L[0], c.L[1], c.L[2],
+          c.cutoff * c.cutoff, c.alphar, c.f.p, c.part.p);
+    else
+      k_excl<false><<<nblk(c.nmol), kBlock, 0, c.stream>>>(
+          c.nmol, c.ns, c.molfirst.p, c.r.p, c.charge.p, qp, c.excl.p, c.L[0], c.L[1], c.L[2],
+          c.cutoff * c.cutoff, c.alphar, c.f.p, c.part.p);
+  }
   MDX_HIP(hipGetLastError());
   if (c.timing) c.tm.stop(T_EXCL, c.stream);
   if (c.nq && (rc = kspace_forces(c, measure))) return rc;
@@ -584,14 +675,24 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   c.kappa = s->kappa;
   c.epsinf = s->epsinf;
 
-  // host copies of the static site data
+  // host copies of the static site data; the charge list holds the
+  // centres (index i) and, POLAR, the Drudes (index ns+i: rows of the
+  // Drude part of r and f, cook's polar_off)
+  c.polar = s->chargepol != nullptr;
+  if (c.polar && !s->alpha_qq) return bail(fail(MDX_E_ARG, "mdx_create: chargepol without alpha_qq"));
   std::vector<int> mol(c.ns), qsite;
   for (int n = 0; n < c.nmol; n++)
     for (int i = s->mol_first[n]; i < s->mol_first[n + 1]; i++) mol[i] = n;
   c.sumq2 = 0;
-  for (int i = 0; i < c.ns; i++)
-    if (s->charge[i] != 0) { qsite.push_back(i); c.sumq2 += s->charge[i] * s->charge[i]; }
+  for (int i = 0; i < c.ns; i++) {
+    const double qp = c.polar ? s->chargepol[i] : 0;
+    if (s->charge[i] != 0) qsite.push_back(i);
+    if (qp != 0) qsite.push_back(c.ns + i);
+    // self term: squared total site charge (sim/ewald.c:402-424)
+    c.sumq2 += (s->charge[i] + qp) * (s->charge[i] + qp);
+  }
   c.nq = (int)qsite.size();
+  const int nrow = c.polar ? 2 * c.ns : c.ns;
 
   // LJ table, FP64 and FP32 (A4 of LJ-free pairs is zero)
   const int nt2 = c.ntypes * c.ntypes;
@@ -619,7 +720,10 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     std::copy(w, w + 8, e32.begin() + 8 * i);
   }
 
-  if ((rc = c.r.alloc(3 * c.ns)) || (rc = c.f.alloc(3 * c.ns)) || (rc = c.type.alloc(c.ns)) ||
+  if ((rc = c.r.alloc(3 * nrow)) || (rc = c.f.alloc(3 * nrow)) || (rc = c.type.alloc(c.ns)) ||
+      (rc = c.qpos.alloc(4 * std::max(c.nq, 1))) ||
+      (c.polar && ((rc = c.chargepol.alloc(c.ns)) || (rc = c.alpha_qq.alloc(c.ns)) ||
+                   (rc = c.spol.alloc(4 * c.ns)))) ||
       (rc = c.mol.alloc(c.ns)) || (rc = c.molfirst.alloc(c.nmol + 1)) ||
       (rc = c.charge.alloc(c.ns)) || (rc = c.excl.alloc(c.ns)) || (rc = c.q_site.alloc(c.nq)) ||
       (rc = c.lj64.alloc(lj.size())) || (rc = c.lj32.alloc(lj32.size())) ||
@@ -638,7 +742,9 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
       !up(c.lj64.p, lj.data(), sizeof(double) * lj.size()) ||
       !up(c.lj32.p, lj32.data(), sizeof(float) * lj32.size()) ||
       !up(c.erf64.p, e64.data(), sizeof(double) * e64.size()) ||
-      !up(c.erf32.p, e32.data(), sizeof(float) * e32.size()))
+      !up(c.erf32.p, e32.data(), sizeof(float) * e32.size()) ||
+      (c.polar && (!up(c.chargepol.p, s->chargepol, sizeof(double) * c.ns) ||
+                   !up(c.alpha_qq.p, s->alpha_qq, sizeof(double) * c.ns))))
     return bail(fail(MDX_E_GPU, "mdx_create: upload failed"));
 
   // dynamics data (optional)
@@ -673,7 +779,8 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   DBuf<double> *dbl[] = {&c.r, &c.f, &c.charge, &c.lj64, &c.erf64, &c.spos, &c.part, &c.acc,
                          &c.kpart, &c.tsum, &c.coef64, &c.kfrc, &c.r1, &c.pwork, &c.mass,
-                         &c.cons_bondq, &c.mol_omega};
+                         &c.cons_bondq, &c.mol_omega, &c.chargepol, &c.alpha_qq, &c.spol,
+                         &c.qpos};
   for (auto *b : dbl) b->release();
   DBuf<int> *ints[] = {&c.type, &c.mol, &c.molfirst, &c.q_site, &c.cellid, &c.cnt, &c.cstart,
                        &c.cursor, &c.sidx, &c.sinfo, &c.scan_tmp, &c.d_row_kx, &c.d_row_ky,
@@ -694,11 +801,12 @@ extern "C" int mdx_forces(mdx_ctx_t *h, const double *r, double *f, mdx_out_t *o
   if (!h || !r || !f || !out) return fail(MDX_E_ARG, "mdx_forces: NULL argument");
   Ctx &c = h->c;
   MDX_HIP(hipSetDevice(c.device));
-  MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
+  const size_t nrow = c.polar ? 2 * c.ns : c.ns;
+  MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * nrow, hipMemcpyHostToDevice, c.stream));
   int rc = forces_device(c, measure);
   if (rc) return rc;
   double acc[ACC_N];
-  MDX_HIP(hipMemcpyAsync(f, c.f.p, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(f, c.f.p, sizeof(double) * 3 * nrow, hipMemcpyDeviceToHost, c.stream));
   MDX_HIP(hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream));
   MDX_HIP(hipStreamSynchronize(c.stream));
   fill_out(c, acc, measure, out);

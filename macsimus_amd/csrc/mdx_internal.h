@@ -115,7 +115,7 @@ struct Timer {
 
 // k-space layout: rows (kx,ky) sorted by kz limit, 8-wide kz blocks
 struct KLayout {
-  int kxmax = 0, nrows = 0, nblk = 0, nseg = 0, ntrip = 0;
+  int kxmax = 0, nrows = 0, nblk = 0, nseg = 0, ntrip = 0, ncoef = 0;
   int kzpad = 0;                       // 8*nblk
   std::vector<int> row_kx, row_ky, row_kzmax, row_coef0;
   std::vector<int> seg_blk, seg_row0, seg_rowend;
@@ -135,6 +135,10 @@ struct Ctx {
   DBuf<double> r, f;          // [3ns] input positions, output forces
   DBuf<int> type, mol, molfirst, q_site;
   DBuf<double> charge;        // [ns]
+  bool polar = false;          // Drude charges present (cook POLAR)
+  DBuf<double> chargepol, alpha_qq;  // [ns]
+  DBuf<double> spol;           // [4ns] Drude offset, qpol (cell order)
+  DBuf<double> qpos;           // [4nq] charge positions and charges
   DBuf<unsigned> excl;        // [ns]
   DBuf<double> lj64;          // [ntypes^2*8]
   DBuf<float> lj32;

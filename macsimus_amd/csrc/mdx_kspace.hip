@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kJC = 32;          // charges per LDS chunk in pass 1
-constexpr int kFlush = 8;        // FP32 chunk sums added to FP64 every kFlush chunks
+constexpr int kKB = 4;           // pass-2 coefficient rows are padded to kKB kz
 constexpr double kPi = 3.14159265358979323846;
 
 template <typename R>
@@ -52,14 +52,30 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// positions and charges of the charge list: index i < ns is the centre of
+// site i, ns+i its Drude at r_i + rpol_i (POLAR)
+__global__ void k_qgather(int nq, int ns, const int *__restrict__ qsite,
+                          const double *__restrict__ r, const double *__restrict__ charge,
+                          const double *__restrict__ chargepol, double4 *__restrict__ qpos) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nq) return;
+  const int k = qsite[j];
+  if (k < ns) {
+    qpos[j] = make_double4(r[3 * k], r[3 * k + 1], r[3 * k + 2], charge[k]);
+  } else {
+    const int i = k - ns;
+    qpos[j] = make_double4(r[3 * i] + r[3 * k], r[3 * i + 1] + r[3 * k + 1],
+                           r[3 * i + 2] + r[3 * k + 2], chargepol[i]);
+  }
+}
+
 // ------------------------------------------------------------------ pass 1
 // blockIdx.x: group of 4 segments (a segment = 64 rows of one kz block);
 // blockIdx.y: charge split.  Lane <-> row, the charge loop is shared, the
 // kz block of a wave is uniform so its cos/sin of c are broadcast reads.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_sfac(
-    int nq, const int *__restrict__ qsite, const double *__restrict__ r,
-    const double *__restrict__ charge, double iLx, double iLy, double iLz, int nseg,
+    int nq, const double4 *__restrict__ qpos, double iLx, double iLy, double iLz, int nseg,
     const int *__restrict__ seg_blk, const int *__restrict__ seg_row0,
     const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
     const int *__restrict__ row_ky, int KXT, int KYT, int KZT, int jper,
@@ -81,15 +97,15 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
   const int j0 = blockIdx.y * jper, j1 = min(nq, j0 + jper);
   const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
 
-  // R=float: chunk sums in acc, added every kFlush chunks (256 charges) to
-  // mid; the splits are then summed in FP64.  R=double: acc only.
-  R acc[8][8], mid[8][8];
+  // R=float: the sums of one chunk (32 charges) in acc are added to the
+  // FP64 mid after every chunk.  R=double: acc only.
+  R acc[8][8];
+  double mid[8][8];
 #pragma unroll
   for (int k = 0; k < 8; k++)
 #pragma unroll
     for (int m = 0; m < 8; m++) { acc[k][m] = 0; mid[k][m] = 0; }
 
-  int nchunk = 0;
   for (int jc = j0; jc < j1; jc += kJC) {
     // cos/sin tables of this chunk: anchors e^{2 pi i 8m s} and the base
     // e^{2 pi i s} in FP64, 7 recursion steps in R
@@ -102,10 +118,11 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
       else { d = 2; m = task - ntx - nty; KT = KZT; dst = Zs + jj * KZT; }
       const int j = jc + jj;
       if (j < j1) {
-        const int site = qsite[j];
+        const double4 qp = qpos[j];
         const double iL = d == 0 ? iLx : (d == 1 ? iLy : iLz);
-        const double sd = frac1(r[3 * site + d] * iL);
-        const double qf = d == 0 ? charge[site] : 1.0;
+        const double x = d == 0 ? qp.x : (d == 1 ? qp.y : qp.z);
+        const double sd = frac1(x * iL);
+        const double qf = d == 0 ? qp.w : 1.0;
         double c0, s0, c1, s1;
         cs2pi(8.0 * m * sd, c0, s0);
         cs2pi(sd, c1, s1);
@@ -141,12 +158,11 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
       }
     }
     __syncthreads();
-    if (sizeof(R) == 4 && ++nchunk == kFlush) {
-      nchunk = 0;
+    if (sizeof(R) == 4) {
 #pragma unroll
       for (int k = 0; k < 8; k++)
 #pragma unroll
-        for (int m = 0; m < 8; m++) { mid[k][m] += acc[k][m]; acc[k][m] = 0; }
+        for (int m = 0; m < 8; m++) { mid[k][m] += (double)acc[k][m]; acc[k][m] = 0; }
     }
   }
   if (!segok) return;
@@ -245,15 +261,13 @@ __global__ __launch_bounds__(kBlock) void k_kcoef(
 }
 
 // dipole moment M = sum q r of the unwrapped sites (sim/ewald.c:644)
-__global__ __launch_bounds__(kBlock) void k_dipole(int nq, const int *__restrict__ qsite,
-                                                   const double *__restrict__ r,
-                                                   const double *__restrict__ charge,
+__global__ __launch_bounds__(kBlock) void k_dipole(int nq, const double4 *__restrict__ qpos,
                                                    double *__restrict__ part) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   double m[3] = {0, 0, 0};
   if (j < nq) {
-    const int s = qsite[j];
-    for (int d = 0; d < 3; d++) m[d] = charge[s] * r[3 * s + d];
+    const double4 p = qpos[j];
+    m[0] = p.w * p.x; m[1] = p.w * p.y; m[2] = p.w * p.z;
   }
   __shared__ double red[kBlock / 64][3];
   const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -293,16 +307,15 @@ __global__ void k_reduce_cols(const double *part, int nblocks, int a0, int a1, d
 // in FP64.  Adds -q_j grad Psi (+ dipole field) to f.
 template <typename R, int KZ>
 __global__ __launch_bounds__(kBlock) void k_kforce(
-    int nq, const int *__restrict__ qsite, const double *__restrict__ r,
-    const double *__restrict__ charge, double Lx, double Ly, double Lz, int kxmax,
+    int nq, const int *__restrict__ qsite, const double4 *__restrict__ qpos, double Lx,
+    double Ly, double Lz, int kxmax,
     const int *__restrict__ lex_kymax, const int *__restrict__ lex_row0,
-    const int *__restrict__ row_kzmax_lex, const int *__restrict__ row_coef0_lex,
+    const int *__restrict__ row_nblk_lex, const int *__restrict__ row_coef0_lex,
     const R *__restrict__ coef, const double *__restrict__ acc, double cdip,
     double *__restrict__ f) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
-  const int site = qsite[min(j, nq - 1)];
-  const double sx = frac1(r[3 * site] / Lx), sy = frac1(r[3 * site + 1] / Ly),
-               sz = frac1(r[3 * site + 2] / Lz);
+  const double4 qp = qpos[min(j, nq - 1)];
+  const double sx = frac1(qp.x / Lx), sy = frac1(qp.y / Ly), sz = frac1(qp.z / Lz);
   R zc[KZ], zs[KZ];
   {
     double c1, s1;
@@ -333,23 +346,29 @@ __global__ __launch_bounds__(kBlock) void k_kforce(
     double cb = 1, sb = 0;
     for (int ky = 0; ky <= kym; ky++) {
       const int row = rbase + ky;
-      const int kzm = __builtin_amdgcn_readfirstlane(row_kzmax_lex[row]);
+      // number of kKB-blocks of kz with coefficients (rows padded with zeros)
+      const int nb = __builtin_amdgcn_readfirstlane(row_nblk_lex[row]);
       const R *cf = coef + (size_t)__builtin_amdgcn_readfirstlane(row_coef0_lex[row]) * 16;
       R G0 = 0, G1 = 0, G2 = 0, G3 = 0, H0 = 0, H1 = 0, H2 = 0, H3 = 0;
-      // fully unrolled (register arrays); kzm is wave-uniform
+      // fully unrolled over the register arrays; nb is wave-uniform
 #pragma unroll
-      for (int kz = 0; kz < KZ; kz++) {
-        if (kz > kzm) continue;
-        const R *e = cf + kz * 16;
-        const R c = zc[kz], s = zs[kz];
-        G0 += e[0] * c + e[1] * s;
-        G1 += e[2] * c + e[3] * s;
-        G2 += e[4] * c + e[5] * s;
-        G3 += e[6] * c + e[7] * s;
-        H0 += e[8] * c + e[9] * s;
-        H1 += e[10] * c + e[11] * s;
-        H2 += e[12] * c + e[13] * s;
-        H3 += e[14] * c + e[15] * s;
+      for (int b = 0; b < KZ / kKB; b++) {
+        if (b < nb) {
+          const R *e = cf + b * (16 * kKB);
+#pragma unroll
+          for (int k = 0; k < kKB; k++) {
+            const R c = zc[b * kKB + k], s = zs[b * kKB + k];
+            const R *ek = e + 16 * k;
+            G0 = fma(ek[0], c, G0); G0 = fma(ek[1], s, G0);
+            G1 = fma(ek[2], c, G1); G1 = fma(ek[3], s, G1);
+            G2 = fma(ek[4], c, G2); G2 = fma(ek[5], s, G2);
+            G3 = fma(ek[6], c, G3); G3 = fma(ek[7], s, G3);
+            H0 = fma(ek[8], c, H0); H0 = fma(ek[9], s, H0);
+            H1 = fma(ek[10], c, H1); H1 = fma(ek[11], s, H1);
+            H2 = fma(ek[12], c, H2); H2 = fma(ek[13], s, H2);
+            H3 = fma(ek[14], c, H3); H3 = fma(ek[15], s, H3);
+          }
+        }
       }
       const R rcb = (R)cb, rsb = (R)sb;
       const R kyl = (R)(ky / Ly);
@@ -362,7 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_kforce(
     }
   }
   if (j >= nq) return;
-  const double q = charge[site];
+  const int site = qsite[j];   // row of f (Drudes after the centres)
+  const double q = qp.w;
   const double *M = acc + mdx::ACC_M0;
   f[3 * site] += -q * Fx - q * cdip * M[0];
   f[3 * site + 1] += -q * Fy - q * cdip * M[1];
@@ -378,9 +398,13 @@ int run_kspace(mdx::Ctx &c, int measure) {
   const size_t shmem = sizeof(C2<R>) * kJC * (KXT + KYT + KZT);
   const int jper = (c.nq + c.nsplit - 1) / c.nsplit;
   dim3 g1((K.nseg + 3) / 4, c.nsplit);
+  const double4 *qpos = (const double4 *)c.qpos.p;
+  k_qgather<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.ns, c.q_site.p, c.r.p, c.charge.p,
+                                                  c.polar ? c.chargepol.p : nullptr,
+                                                  (double4 *)c.qpos.p);
   if (c.timing) c.tm.start(T_KPASS1, c.stream);
   k_sfac<R><<<g1, kBlock, shmem, c.stream>>>(
-      c.nq, c.q_site.p, c.r.p, c.charge.p, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
+      c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
       c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p, KXT, KYT, KZT,
       jper, c.kpart.p);
   MDX_HIP(hipGetLastError());
@@ -396,15 +420,15 @@ int run_kspace(mdx::Ctx &c, int measure) {
   MDX_HIP(hipGetLastError());
   const double V = c.L[0] * c.L[1] * c.L[2];
   const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
-  k_dipole<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.q_site.p, c.r.p, c.charge.p, c.part.p);
+  k_dipole<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, qpos, c.part.p);
   k_reduce_cols<<<1, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, ACC_M0, ACC_M0 + 3, c.acc.p);
   if (c.timing) { c.tm.stop(T_KCOEF, c.stream); c.tm.start(T_KPASS2, c.stream); }
   const int *lk = c.d_row_kx.p + K.nrows;  // lexicographic tables stored after the sorted ones
   const int *lr0 = c.d_row_ky.p + K.nrows;
-  const int *lkz = c.d_row_kzmax.p + K.nrows;
+  const int *lkz = c.d_row_kzmax.p + K.nrows;  // kKB-blocks per lexicographic row
   const int *lc0 = c.d_row_coef0.p + K.nrows;
 #define MDX_KF(KZV)                                                                          \
-  k_kforce<R, KZV><<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.q_site.p, c.r.p, c.charge.p, \
+  k_kforce<R, KZV><<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.q_site.p, qpos,              \
                                                          c.L[0], c.L[1], c.L[2], K.kxmax, lk, \
                                                          lr0, lkz, lc0, coef, c.acc.p, cdip, c.f.p)
   if (KZT <= 16) MDX_KF(16);
@@ -448,11 +472,17 @@ int kspace_setup(Ctx &c) {
   int kzmax_all = *std::max_element(lz.begin(), lz.end());
   K.nblk = kzmax_all / 8 + 1;
   K.kzpad = 8 * K.nblk;
-  // coefficient offsets in lexicographic order
-  std::vector<int> lcoef(K.nrows);
+  // coefficient offsets in lexicographic order, rows padded to kKB kz
+  std::vector<int> lcoef(K.nrows), lnb(K.nrows);
   int off = 0;
-  for (int i = 0; i < K.nrows; i++) { lcoef[i] = off; off += lz[i] + 1; }
-  K.ntrip = off;
+  K.ntrip = 0;
+  for (int i = 0; i < K.nrows; i++) {
+    lcoef[i] = off;
+    lnb[i] = (lz[i] + kKB) / kKB;
+    off += lnb[i] * kKB;
+    K.ntrip += lz[i] + 1 - (i == 0);
+  }
+  K.ncoef = off;
   // rows sorted by kz limit (descending), stable
   std::vector<int> ord(K.nrows);
   std::iota(ord.begin(), ord.end(), 0);
@@ -482,7 +512,7 @@ int kspace_setup(Ctx &c) {
     return a;
   };
   std::vector<int> kx_all = cat(K.row_kx, lkymax), ky_all = cat(K.row_ky, lrow0);
-  std::vector<int> kz_all = cat(K.row_kzmax, lz), c0_all = cat(K.row_coef0, lcoef);
+  std::vector<int> kz_all = cat(K.row_kzmax, lnb), c0_all = cat(K.row_coef0, lcoef);
   int rc;
   if ((rc = c.d_row_kx.alloc(kx_all.size())) || (rc = c.d_row_ky.alloc(ky_all.size())) ||
       (rc = c.d_row_kzmax.alloc(kz_all.size())) || (rc = c.d_row_coef0.alloc(c0_all.size())) ||
@@ -491,8 +521,14 @@ int kspace_setup(Ctx &c) {
       (rc = c.kpart.alloc((size_t)c.nsplit * K.nseg * 64 * 64)) ||
       (rc = c.tsum.alloc((size_t)K.nseg * 64 * 64)))
     return rc;
-  if (c.mode == MDX_MIXED) { if ((rc = c.coef32.alloc((size_t)K.ntrip * 16))) return rc; }
-  else if ((rc = c.coef64.alloc((size_t)K.ntrip * 16))) return rc;
+  // coefficient rows: the padding slots stay zero (k_kcoef writes kz<=kzmax)
+  if (c.mode == MDX_MIXED) {
+    if ((rc = c.coef32.alloc((size_t)K.ncoef * 16))) return rc;
+    MDX_HIP(hipMemset(c.coef32.p, 0, sizeof(float) * K.ncoef * 16));
+  } else {
+    if ((rc = c.coef64.alloc((size_t)K.ncoef * 16))) return rc;
+    MDX_HIP(hipMemset(c.coef64.p, 0, sizeof(double) * K.ncoef * 16));
+  }
   auto up = [&](int *dst, const std::vector<int> &v) {
     return hipMemcpy(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice) == hipSuccess;
   };

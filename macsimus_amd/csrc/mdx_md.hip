@@ -151,7 +151,170 @@ __global__ void k_berendsen(int n3, double *r1, const double *acc, double h, dou
   if (i == 0 && lambda_out) *lambda_out = lam;
 }
 
+// ------------------------------------------------------------------ Drude
+enum { SF_EEL = 0, SF_PV = 1, SF_SELF = 7, SF_ERR = 8, SF_BAD = 9, SF_N = 10 };
+
+// exacterud_sqrt_1(): -erf(alpha r)/r + 2 alpha/sqrt(pi) and its force/r,
+// Ewald correction of the centre-Drude pair of one site (sim/erexact.c:55-82)
+__device__ inline double exact_erud1(double rr, double alpha, double &erd) {
+  const double alphaq = alpha * alpha, A = alpha * (-2 / 1.772453850905516), B = A * alphaq * 2;
+  const double err = alphaq * rr;
+  if (err < 0.01) {
+    double su = 0, sd = 0, q = 1;
+    for (int n = 1;;) {
+      const double t = q / (2 * n + 1);
+      sd += t * n;
+      su += t;
+      if (fabs(q) < 1e-17) break;
+      q *= -err / ++n;
+    }
+    erd = B * sd;
+    return A * (-su * err);
+  }
+  const double r = sqrt(rr), y = -erf(alpha * r) / r;
+  erd = (y - A * exp(-err)) / rr;
+  return y - A;
+}
+
+// one selffield() iteration per Drude site (sim/scf.c:120-240), FP64:
+// r, f with 2ns rows (centres, Drude offsets / forces)
+__global__ __launch_bounds__(kBlock) void k_selffield(
+    int ns, double *__restrict__ r, double *__restrict__ f, const double *__restrict__ q,
+    const double *__restrict__ qpol, const double *__restrict__ alpha_qq, double alpha,
+    double omega, double minqq2, double *__restrict__ part) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[SF_N] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < ns && qpol[i] != 0) {
+    double *rp = r + 3 * (ns + i), *fp = f + 3 * (ns + i), *fc = f + 3 * i;
+    const double d[3] = {rp[0], rp[1], rp[2]};
+    const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (rr >= minqq2) v[SF_BAD] = 1;
+    double erd;
+    double ff = q[i] * qpol[i];
+    v[SF_EEL] = ff * exact_erud1(rr, alpha, erd);
+    ff *= erd;
+    double p[3] = {fp[0], fp[1], fp[2]}, c[3] = {fc[0], fc[1], fc[2]};
+    // MADDFORCES(fpol, fc, ff, d) with its tensor order
+    double fdr;
+    p[0] += fdr = ff * d[0]; c[0] -= fdr; v[SF_PV + 0] += d[0] * fdr; v[SF_PV + 5] += d[1] * fdr;
+    p[1] += fdr = ff * d[1]; c[1] -= fdr; v[SF_PV + 1] += d[1] * fdr; v[SF_PV + 3] += d[2] * fdr;
+    p[2] += fdr = ff * d[2]; c[2] -= fdr; v[SF_PV + 2] += d[2] * fdr; v[SF_PV + 4] += d[0] * fdr;
+    double nr[3], dd2 = 0;
+    for (int k = 0; k < 3; k++) {
+      nr[k] = (1 - omega) * d[k] + omega * alpha_qq[i] * p[k];
+      dd2 += (d[k] - nr[k]) * (d[k] - nr[k]);
+      c[k] += p[k];
+      rp[k] = nr[k];
+      fp[k] = p[k];
+      fc[k] = c[k];
+    }
+    v[SF_SELF] = nr[0] * p[0] + nr[1] * p[1] + nr[2] * p[2];
+    v[SF_PV + 0] -= nr[0] * p[0];
+    v[SF_PV + 1] -= nr[1] * p[1];
+    v[SF_PV + 2] -= nr[2] * p[2];
+    v[SF_PV + 3] -= nr[1] * p[2];
+    v[SF_PV + 4] -= nr[2] * p[0];
+    v[SF_PV + 5] -= nr[0] * p[1];
+    v[SF_ERR] = dd2 / (omega * omega) * qpol[i] * qpol[i];
+    if (nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2] >= minqq2) v[SF_BAD] = 1;
+  }
+  __shared__ double red[kBlock / 64][SF_N];
+  const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < SF_N; k++) {
+    const double s = k >= SF_ERR ? wave_max(v[k]) : wave_sum(v[k]);
+    if (ln == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < SF_N) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++)
+      s = threadIdx.x >= SF_ERR ? fmax(s, red[w][threadIdx.x]) : s + red[w][threadIdx.x];
+    part[(size_t)blockIdx.x * SF_N + threadIdx.x] = s;
+  }
+}
+
+__global__ void k_sf_reduce(const double *part, int nblocks, double *acc) {
+  __shared__ double red[kBlock];
+  for (int a = 0; a < SF_N; a++) {
+    const bool mx = a >= SF_ERR;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock)
+      s = mx ? fmax(s, part[(size_t)b * SF_N + a]) : s + part[(size_t)b * SF_N + a];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o)
+        red[threadIdx.x] = mx ? fmax(red[threadIdx.x], red[threadIdx.x + o])
+                              : red[threadIdx.x] + red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[a] = red[0];
+    __syncthreads();
+  }
+}
+
+// ASPC of order kk (sim/pred2.c:58-64): rpol <- sum_i B_i rpol(t-ih) over
+// the history, which shifts by one step.  hist[m] = rpol(t-(m+1)h) for
+// m = 0..k; the current offsets are the Drude rows of r.
+__global__ void k_aspc(int n3, double *__restrict__ rpol, double *__restrict__ hist, int k,
+                       int kk, double b0, double b1, double b2, double b3, double b4,
+                       double b5, double b6, double b7, double b8, double b9) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n3) return;
+  const double B[10] = {b0, b1, b2, b3, b4, b5, b6, b7, b8, b9};
+  double cur = rpol[j], pred = B[0] * cur;
+  for (int i = 1; i <= kk + 1 && i < 10; i++) pred += B[i] * hist[(size_t)(i - 1) * n3 + j];
+  for (int m = k; m >= 1; m--) hist[(size_t)m * n3 + j] = hist[(size_t)(m - 1) * n3 + j];
+  hist[j] = cur;
+  rpol[j] = pred;
+}
+
+int run_selffield(Ctx &c, double omega, double *sf, DBuf<double> &sfpart, DBuf<double> &sfacc) {
+  const int nb = nblk(c.ns);
+  int rc;
+  if ((rc = sfpart.alloc((size_t)nb * SF_N)) || (rc = sfacc.alloc(SF_N))) return rc;
+  k_selffield<<<nb, kBlock, 0, c.stream>>>(c.ns, c.r.p, c.f.p, c.charge.p, c.chargepol.p,
+                                          c.alpha_qq.p, c.alphar, omega, 1.0, sfpart.p);
+  k_sf_reduce<<<1, kBlock, 0, c.stream>>>(sfpart.p, nb, sfacc.p);
+  MDX_HIP(hipGetLastError());
+  if (sf) {
+    MDX_HIP(hipMemcpyAsync(sf, sfacc.p, sizeof(double) * SF_N, hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    if (sf[SF_BAD] > 0) return fail(MDX_E_RANGE, "selffield: Drude farther than el.minqq=1 A from its centre");
+  }
+  return 0;
+}
+
 }  // namespace
+
+extern "C" int mdx_selffield(mdx_ctx_t *h, double *r, double *f, mdx_out_t *out, double omega,
+                             double *Eself, double *maxerr) {
+  if (!h || !r || !f || !out) return fail(MDX_E_ARG, "mdx_selffield: NULL argument");
+  Ctx &c = h->c;
+  if (!c.polar) return fail(MDX_E_ARG, "mdx_selffield: system has no Drude charges");
+  if (!(omega > 0 && omega <= 1)) return fail(MDX_E_ARG, "mdx_selffield: omega in (0,1]");
+  MDX_HIP(hipSetDevice(c.device));
+  const size_t bytes = sizeof(double) * 6 * c.ns;
+  MDX_HIP(hipMemcpyAsync(c.r.p, r, bytes, hipMemcpyHostToDevice, c.stream));
+  MDX_HIP(hipMemcpyAsync(c.f.p, f, bytes, hipMemcpyHostToDevice, c.stream));
+  DBuf<double> sfpart, sfacc;
+  double sf[SF_N];
+  int rc = run_selffield(c, omega, sf, sfpart, sfacc);
+  if (!rc) {
+    if (hipMemcpyAsync(r, c.r.p, bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipMemcpyAsync(f, c.f.p, bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess)
+      rc = fail(MDX_E_GPU, "mdx_selffield: download failed");
+  }
+  sfpart.release();
+  sfacc.release();
+  if (rc) return rc;
+  out->En_el += sf[SF_EEL];
+  for (int k = 0; k < 6; k++) out->Pvir[k] += sf[SF_PV + k];
+  if (Eself) *Eself = sf[SF_SELF] / 2;
+  if (maxerr) *maxerr = sqrt(sf[SF_ERR]);
+  return 0;
+}
 
 extern "C" int mdx_upload_state(mdx_ctx_t *h, const double *r, const double *r1) {
   if (!h || !r || !r1) return fail(MDX_E_ARG, "mdx_upload_state: NULL argument");

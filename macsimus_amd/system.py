@@ -128,7 +128,9 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
             corr[i, j] = corr[j, i] = c
     bs.keep.append(sstab)
 
+    polar = ff.get("polar") or {}
     site_type, charge, mass, excl, mol_first = [], [], [], [], [0]
+    chargepol, alpha_qq = [], []
     cons_first, cons_ij, cons_bondq = [0], [], []
     for isp, n in enumerate(counts):
         sp = ff["species"][isp]
@@ -141,8 +143,15 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
         for m in range(n):
             base = mol_first[-1]
             for i in range(ns):
-                site_type.append(sp["site_type"][i])
-                charge.append(sp["site_charge_e"][i] * ELECTRON)
+                t = sp["site_type"][i]
+                site_type.append(t)
+                qtot = sp["site_charge_e"][i] * ELECTRON
+                # POLAR: centre charge = total - Drude (cook/simdef.c:715-735)
+                a_pol, qd_e = polar.get(ff["types"][t], (0.0, 0.0))
+                qd = qd_e * ELECTRON
+                chargepol.append(qd)
+                alpha_qq.append(a_pol / (qd * qd) if qd else 0.0)
+                charge.append(qtot - qd)
                 mass.append(ff["type_mass"][sp["site_type"][i]] / MUNIT)
                 excl.append(masks[i])
             for (a, b, bl) in sp["bonds"]:
@@ -153,8 +162,10 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     ns_tot = mol_first[-1]
     nmol = len(mol_first) - 1
     q = np.array(charge)
-    qmax = np.abs(q).max() if ns_tot else 0.0
-    sumq = float(np.sum(q * q))
+    qpol = np.array(chargepol)
+    qt = q + qpol                     # total site charges (Ewaldparm)
+    qmax = np.abs(qt).max() if ns_tot else 0.0
+    sumq = float(np.sum(qt * qt))
     if alpha is None or kappa is None:
         alpha, kappa = prov.ewald_autoset(qmax, sumq, cutoff, L, epsr, epsk)
     kappa, nk = prov.ewald_kvectors(kappa, L)
@@ -181,6 +192,9 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     S.erfc_n = nerf
     S.erfc_tab = C.cast(tab, C.POINTER(abi.Erfc))
     S.ncell = (C.c_int * 3)(*ncell)
+    if polar:
+        S.chargepol = bs.ptr(qpol, C.c_double)
+        S.alpha_qq = bs.ptr(np.array(alpha_qq), C.c_double)
     S.mass = bs.ptr(np.array(mass), C.c_double)
     S.ncons = len(cons_bondq)
     S.cons_first = bs.ptr(np.array(cons_first, np.int32), C.c_int)
@@ -193,7 +207,8 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
                    erfc_n=nerf, erfc_sgrid=sgrid, qmax=qmax, sumq=sumq,
                    ns=ns_tot, nmol=nmol, ncons=len(cons_bondq),
                    corr=corr, sstab=sstab, erfc_tab=tab,
-                   mass=np.array(mass), charge=q,
+                   mass=np.array(mass), charge=q, chargepol=qpol,
+                   alpha_qq=np.array(alpha_qq), polar=bool(polar),
                    site_type=np.array(site_type, np.int32),
                    mol_first=np.array(mol_first, np.int32),
                    excl=np.array(excl, np.uint32),

@@ -428,7 +428,7 @@ static void orc_ewald_pass(int pass, const orc_kset *K, const double *L,
    the charges to f[idx[i]]; energy/tensor bookkeeping of pass 2m */
 static double orc_ewald(const mdx_system_t *S, int nq, const int *idx,
                         const double *q, const double (*pos)[3], double (*f)[3],
-                        int measure, double Pvir[6], int nthreads)
+                        int measure, double Pvir[6], int nthreads, double sumq_self)
 {
   orc_kset K;
   const double *L = S->L;
@@ -550,6 +550,9 @@ static double orc_ewald(const mdx_system_t *S, int nq, const int *idx,
   free(ex); free(ey); free(ez); free(qt); free(F); free(Qk); free(Qth); free(ekk);
   orc_kset_free(&K);
   if (!measure) return 0;
+  /* POLAR: sum of squared total site charges; the centre-Drude pair of a
+     site is corrected in selffield() (sim/ewald.c:979-981, sim/scf.c:143) */
+  if (sumq_self >= 0) sumq = sumq_self;
   return Energy - S->alphar / sqrt(PI) * sumq;
 }
 
@@ -570,8 +573,9 @@ static double orc_minuseps(double x)
 
 typedef struct orc_site_s {
   double r[3], f[3];
+  double rpol[3], fpol[3];   /* Drude offset and force (POLAR) */
   int i, n, st;
-  double q;
+  double q, qpol;
   struct orc_site_s *next;
 } orc_site;
 
@@ -581,6 +585,126 @@ typedef struct {
   long long npair, nlj;
   int measure;
 } orc_acc;
+
+static void orc_addf(orc_acc *a, double *f1, double *f2, double f, const double *dr)
+{
+  double fdr;
+  f1[0] += fdr = f * dr[0]; f2[0] -= fdr;
+  if (a->measure) { a->Pvir[0] += dr[0] * fdr; a->Pvir[5] += dr[1] * fdr; }
+  f1[1] += fdr = f * dr[1]; f2[1] -= fdr;
+  if (a->measure) { a->Pvir[1] += dr[1] * fdr; a->Pvir[3] += dr[2] * fdr; }
+  f1[2] += fdr = f * dr[2]; f2[2] -= fdr;
+  if (a->measure) { a->Pvir[2] += dr[2] * fdr; a->Pvir[4] += dr[0] * fdr; }
+}
+
+static double orc_erud(const mdx_system_t *S, double rr, double *erd)
+{
+  const mdx_erfc_t *p = S->erfc_tab + (int)(rr * S->erfc_sgrid);
+  *erd = p->Bd / (p->Cd + rr) + p->Ad;
+  return p->Bu / (p->Cu + rr) + p->Au;
+}
+
+/* polarLJqq[m] for POLAR=0 (cook/ljqqpol.c): the pair is chosen by the
+   centre-centre distance; up to four charge-charge terms centre/Drude x
+   centre/Drude, Drudes at r+rpol, LJ between the centres only. */
+static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
+{
+  const mdx_system_t *S = a->S;
+  double cq = S->cutoff * S->cutoff;
+  for (; l; l = l->next) {
+    double dr[3], d[3], rr, r2, f = 0, U = 0, qq, erd, u;
+    const mdx_lj_t *ss;
+    int k;
+    for (k = 0; k < 3; k++) dr[k] = ls->r[k] - l->r[k];
+    rr = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    if (rr >= cq) continue;
+    a->npair++;
+    ss = S->sstab + ls->st * S->ntypes + l->st;
+    if (ls->n == l->n) {
+      int m0 = S->mol_first[ls->n];
+      int ii = ls->i - m0, jj = l->i - m0;
+      if ((S->excl[ls->i] >> jj & 1) || (S->excl[l->i] >> ii & 1)) {
+        /* EXCL: exact Ewald corrections of all four charge pairs */
+        if ((qq = ls->q * l->q) != 0) {
+          u = orc_exact(rr, S->alphar, &erd);
+          if (a->measure) a->Enel += u * qq;
+          orc_addf(a, ls->f, l->f, erd * qq, dr);
+        }
+        if ((qq = ls->qpol * l->q) != 0) {
+          for (k = 0; k < 3; k++) d[k] = dr[k] + ls->rpol[k];
+          r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          u = orc_exact(r2, S->alphar, &erd);
+          if (a->measure) a->Enel += u * qq;
+          orc_addf(a, ls->fpol, l->f, erd * qq, d);
+        }
+        if (l->qpol != 0) {
+          for (k = 0; k < 3; k++) d[k] = dr[k] - l->rpol[k];
+          if (ls->q != 0) {
+            qq = ls->q * l->qpol;
+            r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            u = orc_exact(r2, S->alphar, &erd);
+            if (a->measure) a->Enel += u * qq;
+            orc_addf(a, ls->f, l->fpol, erd * qq, d);
+          }
+          if (ls->qpol != 0) {
+            qq = ls->qpol * l->qpol;
+            for (k = 0; k < 3; k++) d[k] += ls->rpol[k];
+            r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            u = orc_exact(r2, S->alphar, &erd);
+            if (a->measure) a->Enel += u * qq;
+            orc_addf(a, ls->fpol, l->fpol, erd * qq, d);
+          }
+        }
+        continue;
+      }
+    }
+    if (rr < ss->C2q) {
+      if (rr < ss->C1q) {
+        double x = ss->Sq / rr, y = x * x * x, z = y * y;
+        U = ss->E4 * (z - y);
+        f = ss->E24 * (z + z - y) * x;
+      } else {
+        double x = rr - ss->C2q;
+        U = x * x * ss->A;
+        f = ss->A4 * x;
+      }
+      a->Envir += rr * f;
+      a->nlj += ss->A != 0;
+    }
+    a->U += U;
+    if ((qq = ls->q * l->q) != 0) {
+      u = orc_erud(S, rr, &erd);
+      if (a->measure) a->Enel += u * qq;
+      f += erd * qq;
+    }
+    orc_addf(a, ls->f, l->f, f, dr);
+    if ((qq = ls->qpol * l->q) != 0) {
+      for (k = 0; k < 3; k++) d[k] = dr[k] + ls->rpol[k];
+      r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      u = orc_erud(S, r2, &erd);
+      if (a->measure) a->Enel += u * qq;
+      orc_addf(a, ls->fpol, l->f, erd * qq, d);
+    }
+    if (l->qpol != 0) {
+      for (k = 0; k < 3; k++) d[k] = dr[k] - l->rpol[k];
+      if (ls->q != 0) {
+        qq = ls->q * l->qpol;
+        r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        u = orc_erud(S, r2, &erd);
+        if (a->measure) a->Enel += u * qq;
+        orc_addf(a, ls->f, l->fpol, erd * qq, d);
+      }
+      if (ls->qpol != 0) {
+        qq = ls->qpol * l->qpol;
+        for (k = 0; k < 3; k++) d[k] += ls->rpol[k];
+        r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        u = orc_erud(S, r2, &erd);
+        if (a->measure) a->Enel += u * qq;
+        orc_addf(a, ls->fpol, l->fpol, erd * qq, d);
+      }
+    }
+  }
+}
 
 /* LJqq[m] for the central site ls against the list l (cook/ljqqnp.c) */
 static void orc_ljqq(orc_acc *a, orc_site *l, orc_site *ls)
@@ -654,6 +778,7 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
   orc_site **list = calloc((size_t)nc[0] * nc[1] * nc[2], sizeof(orc_site *));
   int ixs, iys, izs, err = 0;
 #define LIST(X, Y, Z) list[((size_t)(X) * nc[1] + (Y)) * nc[2] + (Z)]
+#define PAIRS(L_, LS_) (S->chargepol ? orc_ljqq_pol(a, L_, LS_) : orc_ljqq(a, L_, LS_))
   for (k = 0; k < 3; k++) {
     double x;
     volatile double t = nc[k] / S->L[k];
@@ -674,6 +799,10 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
       if (err) break;
       if (cellid) cellid[i] = (ic[0] * nc[1] + ic[1]) * nc[2] + ic[2];
       l->i = i; l->n = n; l->st = S->site_type[i]; l->q = S->charge[i];
+      if (S->chargepol) {
+        l->qpol = S->chargepol[i];
+        for (k = 0; k < 3; k++) l->rpol[k] = r[ns + i][k];
+      }
       l->next = LIST(ic[0], ic[1], ic[2]);
       LIST(ic[0], ic[1], ic[2]) = l;
     }
@@ -716,8 +845,8 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
                 izm = iz0 = izs;
                 for (iz = iz0; iz <= iz1; iz++) {
                   if (izm == nc[2]) { lsloc.r[2] -= S->L[2]; izm -= nc[2]; }
-                  if (izm == izs) orc_ljqq(a, lsloc.next, &lsloc);
-                  else orc_ljqq(a, LIST(ixm, iym, izm), &lsloc);
+                  if (izm == izs) PAIRS(lsloc.next, &lsloc);
+                  else PAIRS(LIST(ixm, iym, izm), &lsloc);
                   izm++;
                 }
               } else {
@@ -725,7 +854,7 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
                 if (izm < 0) { lsloc.r[2] += S->L[2]; izm += nc[2]; }
                 for (iz = iz0; iz <= iz1; iz++) {
                   if (izm == nc[2]) { lsloc.r[2] -= S->L[2]; izm -= nc[2]; }
-                  orc_ljqq(a, LIST(ixm, iym, izm), &lsloc);
+                  PAIRS(LIST(ixm, iym, izm), &lsloc);
                   izm++;
                 }
               }
@@ -735,10 +864,13 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
             lsloc.r[1] = ls->r[1];
             ixm++;
           }
-          ls->f[0] = lsloc.f[0]; ls->f[1] = lsloc.f[1]; ls->f[2] = lsloc.f[2];
+          for (k = 0; k < 3; k++) { ls->f[k] = lsloc.f[k]; ls->fpol[k] = lsloc.fpol[k]; }
         }
       }
   for (i = 0; i < ns; i++) for (k = 0; k < 3; k++) f[i][k] += st[i].f[k];
+  if (S->chargepol)
+    for (i = 0; i < ns; i++) for (k = 0; k < 3; k++) f[ns + i][k] += st[i].fpol[k];
+#undef PAIRS
 #undef LIST
   free(st);
   free(list);
@@ -752,22 +884,35 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
 int oracle_forces(const mdx_system_t *S, const double *r, double *f,
                   mdx_out_t *out, int measure, int nthreads, int *cellid)
 {
-  int ns = S->ns, i, nq = 0, err;
-  int *idx = malloc(sizeof(int) * ns);
-  double *q = malloc(sizeof(double) * ns);
-  double (*pos)[3] = malloc(sizeof(double[3]) * ns);
-  double Pvir[6] = {0, 0, 0, 0, 0, 0}, Enel;
+  /* POLAR: r and f hold 2*ns rows, centres then Drude offsets/forces
+     (polar_off = No.s*24, sim/siminit.c:640) */
+  int ns = S->ns, i, nq = 0, err, npol = S->chargepol ? 2 : 1;
+  int *idx = malloc(sizeof(int) * ns * npol);
+  double *q = malloc(sizeof(double) * ns * npol);
+  double (*pos)[3] = malloc(sizeof(double[3]) * ns * npol);
+  double Pvir[6] = {0, 0, 0, 0, 0, 0}, Enel, sumq_self = -1;
   orc_acc a;
   if (nthreads < 1) nthreads = 1;
-  memset(f, 0, sizeof(double) * 3 * ns);
-  for (i = 0; i < ns; i++)
+  memset(f, 0, sizeof(double) * 3 * ns * npol);
+  for (i = 0; i < ns; i++) {
     if (S->charge[i] != 0) {
       idx[nq] = i; q[nq] = S->charge[i];
       pos[nq][0] = r[3 * i]; pos[nq][1] = r[3 * i + 1]; pos[nq][2] = r[3 * i + 2];
       nq++;
     }
+    if (S->chargepol && S->chargepol[i] != 0) {
+      int k;
+      idx[nq] = ns + i; q[nq] = S->chargepol[i];
+      for (k = 0; k < 3; k++) pos[nq][k] = r[3 * i + k] + r[3 * (ns + i) + k];
+      nq++;
+    }
+  }
+  if (S->chargepol) {
+    sumq_self = 0;
+    for (i = 0; i < ns; i++) sumq_self += sqr(S->charge[i] + S->chargepol[i]);
+  }
   Enel = nq ? orc_ewald(S, nq, idx, q, (const double (*)[3])pos, (double (*)[3])f,
-                        measure, Pvir, nthreads) : 0;
+                        measure, Pvir, nthreads, sumq_self) : 0;
   memset(&a, 0, sizeof(a));
   a.S = S;
   a.measure = measure;
@@ -784,23 +929,165 @@ int oracle_forces(const mdx_system_t *S, const double *r, double *f,
   return err;
 }
 
+/* exacterud_sqrt_1(): exacterud_sqrt() + 2 alpha/sqrt(pi), the Ewald
+   correction of the centre-Drude pair of one site (sim/erexact.c:55-82) */
+static double orc_exact1(double rr, double alpha, double *erd)
+{
+  double alphaq = alpha * alpha, A = alpha * (-2 / SQRTPI), B = A * alphaq * 2;
+  double err = alphaq * rr;
+  if (err < 0.01) {
+    int n = 1;
+    double su = 0, sd = 0, q = 1, r;
+    for (;;) {
+      r = q / (2 * n + 1);
+      sd += r * n;
+      su += r;
+      if (fabs(q) < 1e-17) break;
+      q *= -err / ++n;
+    }
+    *erd = B * sd;
+    return A * (-su * err);
+  } else {
+    double r = sqrt(rr), y = -erf(alpha * r) / r;
+    *erd = (y - A * exp(-err)) / rr;
+    return y - A;
+  }
+}
+
+/* One selffield() iteration (sim/scf.c:12-240, POLAR=0, no saturation or
+   axial polarizability): centre-Drude Ewald correction, Drude move
+   rpol <- (1-omega) rpol + omega alpha_qq f_pol, f_pol passed to the
+   centre, self energy and Drude virial.  r, f: 2*ns rows.  Adds to
+   out->En_el and out->Pvir; returns En.self in out->En_pot's place via
+   *Eself, the largest |drpol|^2/omega^2*qpol^2 in *maxerr. */
+int oracle_selffield(const mdx_system_t *S, double *r, double *f, mdx_out_t *out,
+                     double omega, double *Eself, double *maxerr)
+{
+  int ns = S->ns, i, k;
+  double self = 0, me = 0;
+  if (!S->chargepol) return MDX_E_ARG;
+  for (i = 0; i < ns; i++) {
+    double *rp = r + 3 * (ns + i), *fp = f + 3 * (ns + i), *fc = f + 3 * i;
+    double rr, ff, byerd, dd[3], e;
+    if (S->chargepol[i] == 0) continue;
+    rr = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+    if (rr >= 1.0) return MDX_E_RANGE;          /* el.minqq = 1 A */
+    ff = S->charge[i] * S->chargepol[i];
+    out->En_el += ff * orc_exact1(rr, S->alphar, &byerd);
+    ff *= byerd;
+    {
+      mdx_out_t tmp;
+      orc_acc a;
+      memset(&a, 0, sizeof(a));
+      a.measure = 1;
+      orc_addf(&a, fp, fc, ff, rp);
+      for (k = 0; k < 6; k++) out->Pvir[k] += a.Pvir[k];
+      (void)tmp;
+    }
+    for (k = 0; k < 3; k++) {
+      dd[k] = rp[k];
+      rp[k] = (1 - omega) * rp[k] + omega * S->alpha_qq[i] * fp[k];
+      dd[k] -= rp[k];
+      fc[k] += fp[k];
+    }
+    self += rp[0] * fp[0] + rp[1] * fp[1] + rp[2] * fp[2];
+    out->Pvir[0] -= rp[0] * fp[0];
+    out->Pvir[1] -= rp[1] * fp[1];
+    out->Pvir[2] -= rp[2] * fp[2];
+    out->Pvir[3] -= rp[1] * fp[2];
+    out->Pvir[4] -= rp[2] * fp[0];
+    out->Pvir[5] -= rp[0] * fp[1];
+    e = (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]) / (omega * omega) *
+        sqr(S->chargepol[i]);
+    if (e > me) me = e;
+  }
+  *Eself = self / 2;
+  *maxerr = sqrt(me);
+  return 0;
+}
+
 /* ------------------------------------------------------------- MD step */
 
 /* One Shake() step (sim/constrd.c:622-1081) for SHAKE=1, VERLET=3,
    thermostat none or Berendsen, no Nose/NPT.  r = cfg[0], r1 = cfg[1]
    are updated in place; f is scratch (cfg[2]).  Energies returned in out. */
+static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
+                          const mdx_step_t *p, int measure, mdx_step_out_t *out);
+
 int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
                    const mdx_step_t *p, int measure, mdx_step_out_t *out,
                    int nthreads)
 {
-  int ns = S->ns, n, i, k, al, err;
+  int err;
+  memset(out, 0, sizeof(*out));
+  err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
+  if (err) return err;
+  return orc_shake_step(S, r, r1, f, p, measure, out);
+}
+
+/* ASPC coefficients B_1..B_{kk+2} (sim/pred2.c:94-98) */
+static void orc_aspc_coef(int kk, double *B)
+{
+  double o = (double)(4 * kk + 6) / (double)(kk + 3);
+  int i;
+  for (i = 0; i < kk + 2; i++) {
+    B[i] = o * (i + 1);
+    o *= (double)(i - kk - 1) / (double)(kk + i + 4);
+  }
+}
+
+/* One Shake() step of a Drude system with ASPC (sim/pred2.c:25-99, option
+   -p{k}20) followed by scf_iter selffield() iterations and the SHAKE step.
+   r, r1, f have 2*ns rows.  As in cook, the Drude rows of r hold
+   aa[0] = rpol(t) and the Drude rows of r1 hold aa[1] (sim/pred2.c:54);
+   hist holds the older history aa[2..k+1] (k arrays of 3*ns, zero at the
+   start like sim/pred2.c:52).  *kk is the predictor length reached so far
+   (0 at the start; grows to k, sim/pred2.c:67-98).  scf_omega <= 0 takes
+   the stability value (kk+2)/(2kk+3) (sim/pred2.c:75-77).  scf_iter < 0
+   stops after the prediction (used to check the predictor alone).
+   Returns En.self in out->En_self. */
+int oracle_md_step_polar(const mdx_system_t *S, double *r, double *r1, double *f,
+                         double *hist, int *kk, const mdx_step_t *p, int measure,
+                         mdx_step_out_t *out, int nthreads)
+{
+  int ns = S->ns, n3 = 3 * ns, j, m, it, err, k = p->aspc_k;
+  double B[12], omega = p->scf_omega;
+  double *aa[12];
+  memset(out, 0, sizeof(*out));
+  if (k < 0 || k > 9 || *kk < 0 || *kk > k) return MDX_E_ARG;
+  aa[0] = r + n3;
+  aa[1] = r1 + n3;
+  for (m = 2; m <= k + 1; m++) aa[m] = hist + (size_t)(m - 2) * n3;
+  orc_aspc_coef(*kk, B);
+  for (j = 0; j < n3; j++) {
+    double pred = 0;
+    for (m = 0; m < *kk + 2; m++) pred += B[m] * aa[m][j];
+    for (m = k; m >= 0; m--) aa[m + 1][j] = aa[m][j];
+    aa[0][j] = pred;
+  }
+  if (*kk < k) (*kk)++;
+  if (omega <= 0) omega = (*kk + 2) / (double)(2 * *kk + 3);
+  if (p->scf_iter < 0) return 0;
+  for (it = 0; it < (p->scf_iter > 0 ? p->scf_iter : 1); it++) {
+    double es, me;
+    err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
+    if (err) return err;
+    err = oracle_selffield(S, r, f, &out->frc, omega, &es, &me);
+    if (err) return err;
+    out->En_self = es;
+    out->scf_maxerr = me;
+  }
+  return orc_shake_step(S, r, r1, f, p, measure, out);
+}
+
+static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
+                          const mdx_step_t *p, int measure, mdx_step_out_t *out)
+{
+  int ns = S->ns, n, i, k, al;
   double hh = p->h * p->h, hscale = 0.5 / hh;
   double Ekin = 0, Pkin[6] = {0, 0, 0, 0, 0, 0}, virc = 0, PTvirc[6] = {0, 0, 0, 0, 0, 0};
   double *pp = malloc(sizeof(double) * 3 * ns);
   int maxit0 = -16 * log(p->eps * 0.1), itmax = 0;
-  memset(out, 0, sizeof(*out));
-  err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
-  if (err) { free(pp); return err; }
   for (n = 0; n < S->nmol; n++) {
     int i0 = S->mol_first[n], i1 = S->mol_first[n + 1];
     int c0 = S->cons_first[n], c1 = S->cons_first[n + 1], nc = c1 - c0, it = 0;
@@ -855,6 +1142,7 @@ int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
     }
   }
   Ekin *= hscale;                          /* twice the kinetic energy */
+  (void)ns;
   for (k = 0; k < 6; k++) { Pkin[k] *= hscale; PTvirc[k] /= hh; }
   virc /= hh;
   out->lambda = 1;
@@ -915,7 +1203,7 @@ int oracle_time_sample(const mdx_system_t *S, const double *r, int nthreads,
   err = orc_pairs(&a, (const double (*)[3])r, (double (*)[3])f, NULL);
   t[0] = orc_now() - t0;
   t0 = orc_now();
-  if (nq) orc_ewald(S, nq, idx, q, (const double (*)[3])pos, (double (*)[3])f, 0, Pvir, nthreads);
+  if (nq) orc_ewald(S, nq, idx, q, (const double (*)[3])pos, (double (*)[3])f, 0, Pvir, nthreads, -1);
   t[1] = orc_now() - t0;
   t[2] = nq;
   free(idx); free(q); free(pos); free(f);

@@ -297,8 +297,75 @@ def traj_case(name, species, types, counts, L, cutoff, nsteps, thermostat="none"
     print(name, len(dumps), "dumps")
 
 
+def polar_case(name, species, types, counts, L, cutoff, polar):
+    """Drude (POLAR=0) force evaluations: Ewaldtest iterates forces() +
+    selffield() (sim/ewald.c:1247-1257); every forces() call is dumped with
+    the centre and Drude positions/forces (sim/simils.c:2285-2292)."""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types, polar)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "noint=1", "T=298", "scf.eps=-3", "scf.omega=0.7", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "t.get"), "w").write(
+            'init="crystal"\nel.test=-2\nno=0;\nel.test=-2;\nel.test=0;\n')
+        run([COOKPOL, "sys", "t", "-v128"], td)
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "t.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "t.%d.dump" % k), polar=True))
+            k += 1
+        meta = parse_prt(os.path.join(td, "t.prt"))
+        txt = open(os.path.join(td, "t.prt")).read()
+    out = dict(L=dumps[0]["L"], q=dumps[0]["q"], qpol=dumps[0]["qpol"], mol=dumps[0]["mol"],
+               site=dumps[0]["site"], r=dumps[0]["r"],
+               rpol=np.stack([d["rpol"] for d in dumps]), f=np.stack([d["f"] for d in dumps]),
+               fpol=np.stack([d["fpol"] for d in dumps]))
+    meta.update(dict(name=name, counts=counts, L=L, epsr=0.05, epsk=0.05, ndumps=len(dumps),
+                     scf_omega=0.7, forcefield=ff_arrays(species, types, polar)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps", {k: meta.get(k) for k in ("Epot", "Nk", "ncell", "scf_iterations")})
+
+
+def polar_traj_case(name, species, types, counts, L, cutoff, polar, nsteps):
+    """MD of the Drude system with ASPC k=4 (option -p420: ASPC, sim/pred2.c)
+    and one selffield() iteration per step (scf.eps=-1), SHAKE + leap-frog,
+    NVE; every force call is dumped (centres and Drude offsets)."""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types, polar)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.001", "noint=%d" % nsteps, "T=298", "epsc=1e-12", "scf.eps=-1", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="none"\nno=1;\n')
+        run([COOKPOL, "sys", "md", "-v128", "-p420"], td)
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "md.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "md.%d.dump" % k), polar=True))
+            k += 1
+        meta = parse_prt(os.path.join(td, "md.prt"))
+        txt = open(os.path.join(td, "md.prt")).read()
+        meta["aspc_lines"] = re.findall(r"@@@@@@@ initializing .*|scf.omega=\S+ \(stability.*|B\[\d\]=\S+", txt)
+    out = dict(L=dumps[0]["L"], q=dumps[0]["q"], qpol=dumps[0]["qpol"], mol=dumps[0]["mol"],
+               site=dumps[0]["site"], r=np.stack([d["r"] for d in dumps]),
+               rpol=np.stack([d["rpol"] for d in dumps]), f=np.stack([d["f"] for d in dumps]),
+               fpol=np.stack([d["fpol"] for d in dumps]))
+    meta.update(dict(name=name, counts=counts, L=L, epsr=0.05, epsk=0.05, h=0.001,
+                     ndumps=len(dumps), forcefield=ff_arrays(species, types, polar)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps")
+
+
 def main(argv):
-    subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8", "all"],
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8", "all", "polar"],
                    cwd=REPO, check=True)
     spce = spce_species()
     cases = {
@@ -313,6 +380,15 @@ def main(argv):
             "brine506", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 0.05, 0.05,
             md_steps=20),
+        # polarizable: Drude on O (1.44 A^3, -8 e) and on Cl (3.5 A^3, -8 e)
+        "polar306": lambda: polar_case(
+            "polar306", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 3, 300], 21.10, 9.0,
+            {"O": (1.44, -8.0), "CL": (3.5, -8.0)}),
+        "polar306_aspc": lambda: polar_traj_case(
+            "polar306_aspc", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 3, 300], 21.10, 9.0,
+            {"O": (1.44, -8.0), "CL": (3.5, -8.0)}, 8),
         # NVE trajectory of 216 SPC/E (SHAKE + leap-frog)
         "spce216_nve": lambda: traj_case(
             "spce216_nve", [spce], ["H", "O"], [216], 18.6206, 8.0, 8),

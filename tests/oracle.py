@@ -76,6 +76,42 @@ def forces(bs, r, measure=1, nthreads=None, cellid=False):
     return (f, out, cid) if cellid else (f, out)
 
 
+def selffield(bs, r, f, out, omega):
+    """one selffield() iteration on r (2ns rows, Drude offsets updated) and
+    f (2ns rows, Drude forces passed to the centres); adds to out."""
+    lib = load()
+    lib.oracle_selffield.argtypes = [C.POINTER(abi.System), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(abi.Out), C.c_double,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    es, me = C.c_double(0), C.c_double(0)
+    rc = lib.oracle_selffield(C.byref(bs.struct), r.ctypes.data_as(C.POINTER(C.c_double)),
+                              f.ctypes.data_as(C.POINTER(C.c_double)), C.byref(out), omega,
+                              C.byref(es), C.byref(me))
+    if rc:
+        raise RuntimeError("oracle_selffield failed (%d)" % rc)
+    return es.value, me.value
+
+
+def md_step_polar(bs, r, r1, hist, kk, p, measure=1, nthreads=None):
+    """one ASPC + SCF + SHAKE step; r, r1 with 2ns rows (Drude rows of r1
+    hold the history aa[1]); hist (k, ns, 3) = aa[2..k+1], updated in
+    place; returns (StepOut, new kk)"""
+    lib = load()
+    dp = C.POINTER(C.c_double)
+    lib.oracle_md_step_polar.argtypes = [C.POINTER(abi.System), dp, dp, dp, dp,
+                                         C.POINTER(C.c_int), C.POINTER(abi.Step), C.c_int,
+                                         C.POINTER(abi.StepOut), C.c_int]
+    f = np.zeros_like(r)
+    out = abi.StepOut()
+    k = C.c_int(kk)
+    rc = lib.oracle_md_step_polar(C.byref(bs.struct), r.ctypes.data_as(dp), r1.ctypes.data_as(dp),
+                                  f.ctypes.data_as(dp), hist.ctypes.data_as(dp), C.byref(k),
+                                  C.byref(p), measure, C.byref(out), nthreads or os.cpu_count() or 1)
+    if rc:
+        raise RuntimeError("oracle_md_step_polar failed (%d)" % rc)
+    return out, k.value
+
+
 def md_step(bs, r, r1, p, measure=1, nthreads=None):
     lib = load()
     f = np.zeros_like(r)

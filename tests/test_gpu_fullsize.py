@@ -1,0 +1,69 @@
+"""Full-size checks (194,304-site SPC/E + NaCl, 12 A cutoff, Ewald at
+0.05 K/A): the mixed engine against the FP64 GPU path (itself pinned to
+the reference at 1e-10 on the golden systems), and size-independent
+properties: momentum conservation (sum of forces), invariance of forces,
+energy and virial under a rigid translation of all sites."""
+import numpy as np
+import pytest
+
+from macsimus_amd import abi, workloads
+from macsimus_amd.engine import Engine
+from macsimus_amd.system import build_system
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2():
+    wl = workloads.brine_c2()
+    bs = build_system(wl["ff"], wl["counts"], [wl["L"]] * 3, wl["cutoff"], epsr=0.05, epsk=0.05)
+    res = {}
+    for mode in (abi.MDX_FP64, abi.MDX_MIXED):
+        eng = Engine(bs, mode)
+        f, out = eng.forces(wl["r"], measure=1)
+        shift = np.array([3.7, -11.2, 0.45])
+        f2, out2 = eng.forces(wl["r"] + shift, measure=1)
+        res[mode] = (f, out, f2, out2)
+        eng.close()
+    return wl, bs, res
+
+
+def rel(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def test_mixed_force_error_vs_fp64(c2):
+    wl, bs, res = c2
+    f64, o64 = res[abi.MDX_FP64][:2]
+    f32, o32 = res[abi.MDX_MIXED][:2]
+    e = rel(f32, f64)
+    print("C2 mixed vs FP64: rel RMS force err %.3e" % e, flush=True)
+    assert e < 1e-5
+    assert o32.npair == o64.npair
+    E64 = o64.En_pot + o64.En_el
+    E32 = o32.En_pot + o32.En_el
+    print("C2 Epot fp64 %.10e mixed %.10e rel %.2e" % (E64, E32, abs(E32 - E64) / abs(E64)), flush=True)
+    assert abs(E32 - E64) < 1e-6 * abs(E64)
+    for i in range(6):
+        scale = max(abs(o64.Pvir[k]) for k in range(3))
+        assert abs(o32.Pvir[i] - o64.Pvir[i]) < 1e-5 * scale
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_total_force_vanishes(c2, mode):
+    wl, bs, res = c2
+    f = res[mode][0]
+    fmax = np.abs(f).max()
+    tol = 1e-9 if mode == abi.MDX_FP64 else 2e-5
+    assert np.abs(f.sum(0)).max() < tol * fmax * np.sqrt(len(f))
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_translation_invariance(c2, mode):
+    wl, bs, res = c2
+    f, out, f2, out2 = res[mode]
+    tol = 1e-9 if mode == abi.MDX_FP64 else 1e-5
+    assert rel(f2, f) < tol
+    assert out2.npair == out.npair
+    E, E2 = out.En_pot + out.En_el, out2.En_pot + out2.En_el
+    assert abs(E2 - E) < tol * abs(E)
