@@ -116,11 +116,15 @@ const char *mdx_last_error(void);
 int mdx_create(mdx_ctx_t **ctx, const mdx_system_t *sys, int mode);
 int mdx_destroy(mdx_ctx_t *ctx);
 
-/* Replaces forces(B,A) (cook/forces.c:93-224) for nonpolar systems.
-   r: [ns*3] configuration A->rp;  f: [ns*3] forces B->rp (overwritten,
-   like sdszero(B) at cook/forces.c:183).  measure!=0 computes energies,
-   virial and pressure tensor (LJqqm/ewpass2m.c variants), otherwise
-   only forces (out->npair is still filled).  Host pointers. */
+/* Replaces forces(B,A) (cook/forces.c:93-224).
+   r: configuration A->rp;  f: forces B->rp (overwritten, like sdszero(B)
+   at cook/forces.c:183), both [ns*3] doubles, or for a Drude system
+   (POLAR, chargepol != NULL) [2*ns*3]: the centres, then the Drude
+   offsets rpol / Drude forces fpol as laid out by polarrof()
+   (sim/simglob.h).  measure!=0 computes energies, virial and pressure
+   tensor (LJqqm/ewpass2m.c variants), otherwise only forces (out->npair
+   is still filled).  Host pointers.  Overwrites the device positions, so
+   a following mdx_md_steps needs mdx_upload_state again. */
 int mdx_forces(mdx_ctx_t *ctx, const double *r, double *f, mdx_out_t *out,
                int measure);
 
@@ -170,11 +174,44 @@ typedef struct mdx_step_out_s {
 int mdx_selffield(mdx_ctx_t *ctx, double *r, double *f, mdx_out_t *out,
                   double omega, double *Eself, double *maxerr);
 
+/* State transfer of cfg[0]->rp (r) and cfg[1]->rp (r1), [ns*3] doubles,
+   for a Drude system [2*ns*3]: the Drude rows of r are rpol(t) (ASPC
+   aa[0]) and those of r1 the ASPC history aa[1] (sim/pred2.c:27-54). */
 int mdx_upload_state(mdx_ctx_t *ctx, const double *r, const double *r1);
 int mdx_download_state(mdx_ctx_t *ctx, double *r, double *r1);
-/* runs nsteps of Shake() (sim/constrd.c:622) with forces computed on the GPU */
+/* Runs nsteps of Shake() (sim/constrd.c:622) with forces computed on the
+   GPU.  Drude systems: each step is the ASPC prediction (sim/pred2.c:54-99,
+   the predictor length grows by one per step up to aspc_k), scf_iter
+   forces()+selffield() iterations, then the SHAKE step.  Errors:
+   MDX_E_NOCONV (SHAKE did not converge), MDX_E_RANGE (a Drude left
+   el.minqq), checked at every measured step and at the last one. */
 int mdx_md_steps(mdx_ctx_t *ctx, const mdx_step_t *p, int nsteps,
                  mdx_step_out_t *out);
+
+/* ASPC state of a Drude system: predictor length k (0..8), length
+   reached kk (0..k) and the older history aa[2..k+1] as k arrays of ns*3
+   doubles (hist = NULL: zeros, like the alloczero of sim/pred2.c:52).
+   cook keeps kk and aa[2..] in statics of sim/pred2.c:36-38; these calls
+   let a caller restart or inspect them.  mdx_md_steps with another
+   aspc_k than the context holds restarts the predictor (kk = 0). */
+int mdx_set_aspc(mdx_ctx_t *ctx, int k, int kk, const double *hist);
+int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
+
+/* ---- diagnostics (no counterpart in cook) -----------------------------
+   mdx_cook_cells: cook's linked-cell index of each site for the context's
+   cook grid (cook/linklist.c:304-337 with No.cell), to compare the cell
+   membership.  mdx_set_timing / mdx_get_timing: HIP-event timing of the
+   kernel families (bin, pair, excl, kpass1, kcoef, kpass2, shake) on the
+   context stream; ms[i], count[i] accumulate since the last get.
+   mdx_set_stencil: cell-stencil half width w (1..6) of the pair kernel
+   (a wider one is taken where the box needs it).
+   mdx_info: iv[0..n) = engine cells x,y,z, stencil w, charges, k rows,
+   kz blocks, k segments, k triples, k-sum splits, kx max, mode. */
+int mdx_cook_cells(mdx_ctx_t *ctx, const double *r, int *cellid);
+int mdx_set_timing(mdx_ctx_t *ctx, int on);
+int mdx_get_timing(mdx_ctx_t *ctx, double *ms, int *count, int n);
+int mdx_set_stencil(mdx_ctx_t *ctx, int w);
+int mdx_info(mdx_ctx_t *ctx, int *iv, int n);
 
 /* ---- set-up helpers restating cook's init code (product, not oracle) -- */
 

@@ -499,16 +499,24 @@ Grid make_grid(const Ctx &c, const int *n) {
 }
 
 // engine cell grid: cells at least cutoff/w wide, at least 2w+1 per side
-int choose_grid(Ctx &c, int w) {
-  for (; w >= 1; w--) {
-    bool ok = true;
-    for (int d = 0; d < 3; d++) {
-      c.nc[d] = (int)std::floor(c.L[d] * w / c.cutoff);
-      if (c.nc[d] < 2 * w + 1) ok = false;
-    }
-    if (ok) { c.wst = w; return 0; }
+// cells of edge >= cutoff/w and >= 2w+1 cells per side (no cell visited
+// twice by the (2w+1)^3 stencil); the requested w or, for a box that is
+// small against the cutoff (L < (2+1/w) cutoff), the smallest wider one
+constexpr int kMaxStencil = 6;
+
+bool grid_fits(Ctx &c, int w) {
+  for (int d = 0; d < 3; d++) {
+    c.nc[d] = (int)std::floor(c.L[d] * w / c.cutoff);
+    if (c.nc[d] < 2 * w + 1) return false;
   }
-  return fail(MDX_E_ARG, "box too small for the cutoff (need L >= 3*cutoff)");
+  c.wst = w;
+  return true;
+}
+
+int choose_grid(Ctx &c, int w) {
+  for (int v = w; v <= kMaxStencil; v++)
+    if (grid_fits(c, v)) return 0;
+  return fail(MDX_E_ARG, "box too small for the cutoff (need L >= (2+1/6)*cutoff)");
 }
 
 int alloc_grid(Ctx &c) {
@@ -575,7 +583,13 @@ int forces_device(Ctx &c, int measure) {
   if (c.timing) c.tm.start(T_BIN, c.stream);
   if ((rc = bin_sites(c, c.r.p))) return rc;
   if (c.timing) { c.tm.stop(T_BIN, c.stream); c.tm.start(T_PAIR, c.stream); }
-  rc = c.mode == MDX_FP64 ? launch_pair<double>(c, measure) : launch_pair<float>(c, measure);
+  // Drude systems: the four charge terms of a pair are each 10-30x the net
+  // pair energy (|q_D| of 1.5-2.5 e against the site charge), so FP32
+  // rounding of the terms leaves ~1e-5 of the total energy and virial;
+  // measured steps therefore take the FP64 kernel in both modes (forces of
+  // unmeasured steps stay FP32 in the mixed mode, well inside 1e-5)
+  const bool fp64 = c.mode == MDX_FP64 || (c.polar && measure);
+  rc = fp64 ? launch_pair<double>(c, measure) : launch_pair<float>(c, measure);
   if (rc) return rc;
   if (c.timing) { c.tm.stop(T_PAIR, c.stream); c.tm.start(T_EXCL, c.stream); }
   {
@@ -750,7 +764,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   // dynamics data (optional)
   if (s->mass) {
     c.ncons = s->ncons;
-    if ((rc = c.r1.alloc(3 * c.ns)) || (rc = c.pwork.alloc(3 * c.ns)) || (rc = c.mass.alloc(c.ns)) ||
+    if ((rc = c.r1.alloc(3 * nrow)) || (rc = c.pwork.alloc(3 * c.ns)) || (rc = c.mass.alloc(c.ns)) ||
         (rc = c.cons_first.alloc(c.nmol + 1)) || (rc = c.cons_ij.alloc(2 * std::max(1, c.ncons))) ||
         (rc = c.cons_bondq.alloc(std::max(1, c.ncons))) || (rc = c.mol_omega.alloc(c.nmol)))
       return bail(rc);
@@ -765,6 +779,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
         !up(c.mol_omega.p, om.data(), sizeof(double) * c.nmol))
       return bail(fail(MDX_E_GPU, "mdx_create: upload failed"));
   }
+  if ((rc = md_setup(c))) return bail(rc);
 
   if ((rc = choose_grid(c, 2)) || (rc = alloc_grid(c))) return bail(rc);
   if (c.nq && (rc = kspace_setup(c))) return bail(rc);
@@ -780,7 +795,8 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   DBuf<double> *dbl[] = {&c.r, &c.f, &c.charge, &c.lj64, &c.erf64, &c.spos, &c.part, &c.acc,
                          &c.kpart, &c.tsum, &c.coef64, &c.kfrc, &c.r1, &c.pwork, &c.mass,
                          &c.cons_bondq, &c.mol_omega, &c.chargepol, &c.alpha_qq, &c.spol,
-                         &c.qpos};
+                         &c.qpos, &c.mdpart, &c.mdacc, &c.sfpart, &c.sfacc, &c.mdflag,
+                         &c.aspc_hist};
   for (auto *b : dbl) b->release();
   DBuf<int> *ints[] = {&c.type, &c.mol, &c.molfirst, &c.q_site, &c.cellid, &c.cnt, &c.cstart,
                        &c.cursor, &c.sidx, &c.sinfo, &c.scan_tmp, &c.d_row_kx, &c.d_row_ky,
@@ -802,6 +818,7 @@ extern "C" int mdx_forces(mdx_ctx_t *h, const double *r, double *f, mdx_out_t *o
   Ctx &c = h->c;
   MDX_HIP(hipSetDevice(c.device));
   const size_t nrow = c.polar ? 2 * c.ns : c.ns;
+  c.state_on_device = false;           // the device positions are now the caller's r
   MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * nrow, hipMemcpyHostToDevice, c.stream));
   int rc = forces_device(c, measure);
   if (rc) return rc;
@@ -881,7 +898,7 @@ extern "C" int mdx_get_timing(mdx_ctx_t *h, double *ms, int *count, int n) {
 
 // engine cell grid (stencil half width w), for the autotuner
 extern "C" int mdx_set_stencil(mdx_ctx_t *h, int w) {
-  if (!h || w < 1 || w > 4) return fail(MDX_E_ARG, "mdx_set_stencil: w in 1..4");
+  if (!h || w < 1 || w > kMaxStencil) return fail(MDX_E_ARG, "mdx_set_stencil: w in 1..6");
   Ctx &c = h->c;
   int rc;
   if ((rc = choose_grid(c, w)) || (rc = alloc_grid(c))) return rc;

@@ -174,11 +174,20 @@ struct Ctx {
   DBuf<double> cons_bondq, mol_omega;
   int ncons = 0;
   bool state_on_device = false;
+  // reduction buffers of the MD step: SHAKE/kinetic partials and sums,
+  // selffield partials and sums, flags [lambda, SHAKE failed (sticky),
+  // max SHAKE iterations (sticky), Drude out of range (sticky)]
+  DBuf<double> mdpart, mdacc, sfpart, sfacc, mdflag;
+  // Drude ASPC history aa[2..k+1] (aa[0], aa[1] are the Drude rows of r, r1)
+  DBuf<double> aspc_hist;
+  int aspc_k = -1, aspc_kk = 0;
   // kernel timing with HIP events on the context stream (off by default)
   bool timing = false;
   Timer tm;
 };
 
+// reduction buffers of the MD step and the Drude selffield
+int md_setup(Ctx &c);
 // set-up of the k-space layout for the context box and kappa
 int kspace_setup(Ctx &c);
 // forces of the charges from the k-space sum (adds into c.f); energy and

@@ -20,6 +20,12 @@ using namespace mdx;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kAspcMax = 8;          // longest ASPC predictor (cook warns above 8)
+
+__global__ void k_fill(long n, double *__restrict__ x, double v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
 enum { MD_EKIN = 0, MD_PKIN = 1, MD_VIRC = 7, MD_PTV = 8, MD_ITMAX = 14, MD_FAIL = 15, MD_N = 16 };
 
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -116,7 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_shake(
   }
 }
 
-__global__ void k_md_reduce(const double *part, int nblocks, double *acc) {
+// flag[1] = SHAKE failed and flag[2] = max SHAKE iterations, both sticky
+// over the steps of one mdx_md_steps call
+__global__ void k_md_reduce(const double *part, int nblocks, double *acc, double *flag) {
   __shared__ double red[kBlock];
   for (int a = 0; a < MD_N; a++) {
     const bool mx = a >= MD_ITMAX;
@@ -131,7 +139,11 @@ __global__ void k_md_reduce(const double *part, int nblocks, double *acc) {
                               : red[threadIdx.x] + red[threadIdx.x + o];
       __syncthreads();
     }
-    if (threadIdx.x == 0) acc[a] = red[0];
+    if (threadIdx.x == 0) {
+      acc[a] = red[0];
+      if (a == MD_FAIL) flag[1] = fmax(flag[1], red[0]);
+      if (a == MD_ITMAX) flag[2] = fmax(flag[2], red[0]);
+    }
     __syncthreads();
   }
 }
@@ -233,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_selffield(
   }
 }
 
-__global__ void k_sf_reduce(const double *part, int nblocks, double *acc) {
+// flag[3] = a Drude left el.minqq (sticky over the steps of one call)
+__global__ void k_sf_reduce(const double *part, int nblocks, double *acc, double *flag) {
   __shared__ double red[kBlock];
   for (int a = 0; a < SF_N; a++) {
     const bool mx = a >= SF_ERR;
@@ -248,43 +261,99 @@ __global__ void k_sf_reduce(const double *part, int nblocks, double *acc) {
                               : red[threadIdx.x] + red[threadIdx.x + o];
       __syncthreads();
     }
-    if (threadIdx.x == 0) acc[a] = red[0];
+    if (threadIdx.x == 0) {
+      acc[a] = red[0];
+      if (a == SF_BAD) flag[3] = fmax(flag[3], red[0]);
+    }
     __syncthreads();
   }
 }
 
-// ASPC of order kk (sim/pred2.c:58-64): rpol <- sum_i B_i rpol(t-ih) over
-// the history, which shifts by one step.  hist[m] = rpol(t-(m+1)h) for
-// m = 0..k; the current offsets are the Drude rows of r.
-__global__ void k_aspc(int n3, double *__restrict__ rpol, double *__restrict__ hist, int k,
-                       int kk, double b0, double b1, double b2, double b3, double b4,
-                       double b5, double b6, double b7, double b8, double b9) {
+// ASPC predictor + history shift of sim/pred2.c:54-64: aa[0] = Drude rows
+// of r (rpol(t)), aa[1] = Drude rows of r1, aa[2..k+1] = hist; the
+// prediction sum_{i<kk+2} B_i aa[i] replaces aa[0] after aa[i+1] <- aa[i]
+struct AspcB {
+  double b[kAspcMax + 2];
+};
+
+__global__ void k_aspc(int n3, double *__restrict__ a0, double *__restrict__ a1,
+                       double *__restrict__ hist, int k, int kk, AspcB B) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n3) return;
-  const double B[10] = {b0, b1, b2, b3, b4, b5, b6, b7, b8, b9};
-  double cur = rpol[j], pred = B[0] * cur;
-  for (int i = 1; i <= kk + 1 && i < 10; i++) pred += B[i] * hist[(size_t)(i - 1) * n3 + j];
-  for (int m = k; m >= 1; m--) hist[(size_t)m * n3 + j] = hist[(size_t)(m - 1) * n3 + j];
-  hist[j] = cur;
-  rpol[j] = pred;
+  double aa[kAspcMax + 2];
+  aa[0] = a0[j];
+  aa[1] = a1[j];
+#pragma unroll
+  for (int m = 2; m < kAspcMax + 2; m++) aa[m] = m <= k + 1 ? hist[(size_t)(m - 2) * n3 + j] : 0.0;
+  double pred = 0;
+#pragma unroll
+  for (int m = 0; m < kAspcMax + 2; m++) pred += m < kk + 2 ? B.b[m] * aa[m] : 0.0;
+  // shift: aa[m+1] <- aa[m] for m = k..0 (aa[k+1] drops out)
+#pragma unroll
+  for (int m = 2; m < kAspcMax + 2; m++)
+    if (m <= k + 1) hist[(size_t)(m - 2) * n3 + j] = aa[m - 1];
+  a1[j] = aa[0];
+  a0[j] = pred;
 }
 
-int run_selffield(Ctx &c, double omega, double *sf, DBuf<double> &sfpart, DBuf<double> &sfacc) {
-  const int nb = nblk(c.ns);
-  int rc;
-  if ((rc = sfpart.alloc((size_t)nb * SF_N)) || (rc = sfacc.alloc(SF_N))) return rc;
-  k_selffield<<<nb, kBlock, 0, c.stream>>>(c.ns, c.r.p, c.f.p, c.charge.p, c.chargepol.p,
-                                          c.alpha_qq.p, c.alphar, omega, 1.0, sfpart.p);
-  k_sf_reduce<<<1, kBlock, 0, c.stream>>>(sfpart.p, nb, sfacc.p);
-  MDX_HIP(hipGetLastError());
-  if (sf) {
-    MDX_HIP(hipMemcpyAsync(sf, sfacc.p, sizeof(double) * SF_N, hipMemcpyDeviceToHost, c.stream));
-    MDX_HIP(hipStreamSynchronize(c.stream));
-    if (sf[SF_BAD] > 0) return fail(MDX_E_RANGE, "selffield: Drude farther than el.minqq=1 A from its centre");
+// B_1..B_{kk+2} of the ASPC predictor of length kk (sim/pred2.c:94-98;
+// kk = 0 gives the 2 a0 - a1 start of sim/pred2.c:48)
+AspcB aspc_coef(int kk) {
+  AspcB B{};
+  double o = (double)(4 * kk + 6) / (double)(kk + 3);
+  for (int i = 0; i < kk + 2; i++) {
+    B.b[i] = o * (i + 1);
+    o *= (double)(i - kk - 1) / (double)(kk + i + 4);
   }
+  return B;
+}
+
+void launch_selffield(Ctx &c, double omega) {
+  const int nb = nblk(c.ns);
+  k_selffield<<<nb, kBlock, 0, c.stream>>>(c.ns, c.r.p, c.f.p, c.charge.p, c.chargepol.p,
+                                          c.alpha_qq.p, c.alphar, omega, 1.0, c.sfpart.p);
+  k_sf_reduce<<<1, kBlock, 0, c.stream>>>(c.sfpart.p, nb, c.sfacc.p, c.mdflag.p);
+}
+
+int set_aspc(Ctx &c, int k, int kk, const double *hist) {
+  if (k < 0 || k > kAspcMax || kk < 0 || kk > k)
+    return fail(MDX_E_ARG, "ASPC: need 0 <= kk <= k <= 8");
+  const size_t n = (size_t)std::max(k, 1) * 3 * c.ns;
+  int rc;
+  if (c.aspc_k != k || !c.aspc_hist.p) {
+    c.aspc_hist.release();
+    if ((rc = c.aspc_hist.alloc(n))) return rc;
+  }
+  if (hist)
+    MDX_HIP(hipMemcpyAsync(c.aspc_hist.p, hist, sizeof(double) * 3 * c.ns * k,
+                           hipMemcpyHostToDevice, c.stream));
+  else
+    k_fill<<<nblk((long)n), kBlock, 0, c.stream>>>((long)n, c.aspc_hist.p, 0.0);
+  MDX_HIP(hipGetLastError());
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  c.aspc_k = k;
+  c.aspc_kk = kk;
   return 0;
 }
 
+}  // namespace
+
+int mdx::md_setup(Ctx &c) {
+  int rc;
+  if ((rc = c.mdflag.alloc(4))) return rc;
+  if (c.mass.p && ((rc = c.mdpart.alloc((size_t)nblk(c.nmol) * MD_N)) || (rc = c.mdacc.alloc(MD_N))))
+    return rc;
+  if (c.polar && ((rc = c.sfpart.alloc((size_t)nblk(c.ns) * SF_N)) || (rc = c.sfacc.alloc(SF_N))))
+    return rc;
+  return 0;
+}
+
+namespace {
+// flags: lambda = 1, the sticky ones 0 (a kernel, so that the step can be captured)
+void reset_flags(Ctx &c) {
+  k_fill<<<1, 64, 0, c.stream>>>(4, c.mdflag.p, 0.0);
+  k_fill<<<1, 64, 0, c.stream>>>(1, c.mdflag.p, 1.0);
+}
 }  // namespace
 
 extern "C" int mdx_selffield(mdx_ctx_t *h, double *r, double *f, mdx_out_t *out, double omega,
@@ -295,20 +364,19 @@ extern "C" int mdx_selffield(mdx_ctx_t *h, double *r, double *f, mdx_out_t *out,
   if (!(omega > 0 && omega <= 1)) return fail(MDX_E_ARG, "mdx_selffield: omega in (0,1]");
   MDX_HIP(hipSetDevice(c.device));
   const size_t bytes = sizeof(double) * 6 * c.ns;
+  double sf[SF_N], flag[4];
   MDX_HIP(hipMemcpyAsync(c.r.p, r, bytes, hipMemcpyHostToDevice, c.stream));
   MDX_HIP(hipMemcpyAsync(c.f.p, f, bytes, hipMemcpyHostToDevice, c.stream));
-  DBuf<double> sfpart, sfacc;
-  double sf[SF_N];
-  int rc = run_selffield(c, omega, sf, sfpart, sfacc);
-  if (!rc) {
-    if (hipMemcpyAsync(r, c.r.p, bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-        hipMemcpyAsync(f, c.f.p, bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-        hipStreamSynchronize(c.stream) != hipSuccess)
-      rc = fail(MDX_E_GPU, "mdx_selffield: download failed");
-  }
-  sfpart.release();
-  sfacc.release();
-  if (rc) return rc;
+  reset_flags(c);
+  launch_selffield(c, omega);
+  MDX_HIP(hipGetLastError());
+  MDX_HIP(hipMemcpyAsync(sf, c.sfacc.p, sizeof(sf), hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(flag, c.mdflag.p, sizeof(flag), hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(r, c.r.p, bytes, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(f, c.f.p, bytes, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  c.state_on_device = false;           // r now holds the caller's configuration
+  if (flag[3] > 0) return fail(MDX_E_RANGE, "selffield: Drude farther than el.minqq=1 A from its centre");
   out->En_el += sf[SF_EEL];
   for (int k = 0; k < 6; k++) out->Pvir[k] += sf[SF_PV + k];
   if (Eself) *Eself = sf[SF_SELF] / 2;
@@ -321,8 +389,9 @@ extern "C" int mdx_upload_state(mdx_ctx_t *h, const double *r, const double *r1)
   Ctx &c = h->c;
   if (!c.mass.p) return fail(MDX_E_ARG, "mdx_upload_state: system has no masses");
   MDX_HIP(hipSetDevice(c.device));
-  MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
-  MDX_HIP(hipMemcpyAsync(c.r1.p, r1, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
+  const size_t bytes = sizeof(double) * 3 * (c.polar ? 2 * c.ns : c.ns);
+  MDX_HIP(hipMemcpyAsync(c.r.p, r, bytes, hipMemcpyHostToDevice, c.stream));
+  MDX_HIP(hipMemcpyAsync(c.r1.p, r1, bytes, hipMemcpyHostToDevice, c.stream));
   MDX_HIP(hipStreamSynchronize(c.stream));
   c.state_on_device = true;
   return 0;
@@ -333,9 +402,33 @@ extern "C" int mdx_download_state(mdx_ctx_t *h, double *r, double *r1) {
   Ctx &c = h->c;
   if (!c.state_on_device) return fail(MDX_E_ARG, "mdx_download_state: no state uploaded");
   MDX_HIP(hipSetDevice(c.device));
-  MDX_HIP(hipMemcpyAsync(r, c.r.p, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
-  MDX_HIP(hipMemcpyAsync(r1, c.r1.p, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
+  const size_t bytes = sizeof(double) * 3 * (c.polar ? 2 * c.ns : c.ns);
+  MDX_HIP(hipMemcpyAsync(r, c.r.p, bytes, hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipMemcpyAsync(r1, c.r1.p, bytes, hipMemcpyDeviceToHost, c.stream));
   MDX_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+extern "C" int mdx_set_aspc(mdx_ctx_t *h, int k, int kk, const double *hist) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_aspc: NULL handle");
+  Ctx &c = h->c;
+  if (!c.polar) return fail(MDX_E_ARG, "mdx_set_aspc: system has no Drude charges");
+  MDX_HIP(hipSetDevice(c.device));
+  return set_aspc(c, k, kk, hist);
+}
+
+extern "C" int mdx_get_aspc(mdx_ctx_t *h, int *k, int *kk, double *hist) {
+  if (!h || !k || !kk) return fail(MDX_E_ARG, "mdx_get_aspc: NULL argument");
+  Ctx &c = h->c;
+  if (!c.polar) return fail(MDX_E_ARG, "mdx_get_aspc: system has no Drude charges");
+  *k = c.aspc_k;
+  *kk = c.aspc_kk;
+  if (hist && c.aspc_k > 0) {
+    MDX_HIP(hipSetDevice(c.device));
+    MDX_HIP(hipMemcpyAsync(hist, c.aspc_hist.p, sizeof(double) * 3 * c.ns * c.aspc_k,
+                           hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+  }
   return 0;
 }
 
@@ -346,38 +439,62 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
   if (!(p->h > 0) || !(p->eps > 0)) return fail(MDX_E_ARG, "mdx_md_steps: h and eps must be > 0");
   if (p->thermostat && !(p->tau > 0 && p->nf > 0 && p->T > 0))
     return fail(MDX_E_ARG, "mdx_md_steps: Berendsen needs T, tau, nf > 0");
+  if (c.polar && !(p->scf_omega <= 1))
+    return fail(MDX_E_ARG, "mdx_md_steps: scf_omega must be <= 1 (<= 0: stability value)");
   MDX_HIP(hipSetDevice(c.device));
-  const int nb = nblk(c.nmol);
-  DBuf<double> mdpart, mdacc, lam;
   int rc;
-  if ((rc = mdpart.alloc((size_t)nb * MD_N)) || (rc = mdacc.alloc(MD_N)) || (rc = lam.alloc(1)))
-    return rc;
+  if (c.polar && c.aspc_k != p->aspc_k && (rc = set_aspc(c, p->aspc_k, 0, nullptr))) return rc;
+  const int nb = nblk(c.nmol);
   std::memset(out, 0, sizeof(*out));
   out->lambda = 1;
   const int every = p->measure_every > 0 ? p->measure_every : nsteps;
-  double acc[ACC_N], md[MD_N];
-  for (int s = 1; s <= nsteps; s++) {
+  const int niter = p->scf_iter > 0 ? p->scf_iter : 1;
+  double acc[ACC_N], md[MD_N], sf[SF_N], flag[4];
+  reset_flags(c);
+  rc = 0;
+  for (int s = 1; s <= nsteps && !rc; s++) {
     const int measure = (s % every == 0) || s == nsteps;
-    if ((rc = forces_device(c, measure))) break;
+    if (c.polar) {
+      // ASPC prediction, then scf_iter selffield iterations (cook -p{k}20)
+      k_aspc<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r.p + 3 * c.ns,
+                                                       c.r1.p + 3 * c.ns, c.aspc_hist.p,
+                                                       c.aspc_k, c.aspc_kk, aspc_coef(c.aspc_kk));
+      if (c.aspc_kk < c.aspc_k) c.aspc_kk++;
+      const double omega = p->scf_omega > 0 ? p->scf_omega
+                                            : (c.aspc_kk + 2) / (double)(2 * c.aspc_kk + 3);
+      for (int it = 0; it < niter && !rc; it++) {
+        if ((rc = forces_device(c, measure && it == niter - 1))) break;
+        launch_selffield(c, omega);
+      }
+      if (rc) break;
+    } else if ((rc = forces_device(c, measure))) {
+      break;
+    }
     if (c.timing) c.tm.start(T_SHAKE, c.stream);
     k_shake<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.cons_first.p, c.cons_ij.p,
                                          c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
                                          p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
-                                         mdpart.p);
+                                         c.mdpart.p);
     if (c.timing) c.tm.stop(T_SHAKE, c.stream);
-    k_md_reduce<<<1, kBlock, 0, c.stream>>>(mdpart.p, nb, mdacc.p);
+    k_md_reduce<<<1, kBlock, 0, c.stream>>>(c.mdpart.p, nb, c.mdacc.p, c.mdflag.p);
     if (p->thermostat == 1)
-      k_berendsen<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r1.p, mdacc.p, p->h,
-                                                            p->T, p->tau, p->nf, lam.p);
-    if ((rc = hipGetLastError()) != hipSuccess) { rc = fail(MDX_E_GPU, "mdx_md_steps: launch"); break; }
+      k_berendsen<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r1.p, c.mdacc.p, p->h,
+                                                            p->T, p->tau, p->nf, c.mdflag.p);
+    if (hipGetLastError() != hipSuccess) { rc = fail(MDX_E_GPU, "mdx_md_steps: launch"); break; }
     if (measure) {
       if (hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-          hipMemcpyAsync(md, mdacc.p, sizeof(md), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+          hipMemcpyAsync(md, c.mdacc.p, sizeof(md), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+          hipMemcpyAsync(flag, c.mdflag.p, sizeof(flag), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+          (c.polar && hipMemcpyAsync(sf, c.sfacc.p, sizeof(sf), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
           hipStreamSynchronize(c.stream) != hipSuccess) {
         rc = fail(MDX_E_GPU, "mdx_md_steps: download");
         break;
       }
-      if (md[MD_FAIL] > 0) { rc = fail(MDX_E_NOCONV, "SHAKE: too many iterations"); break; }
+      if (flag[1] > 0) { rc = fail(MDX_E_NOCONV, "SHAKE: too many iterations"); break; }
+      if (flag[3] > 0) {
+        rc = fail(MDX_E_RANGE, "selffield: Drude farther than el.minqq=1 A from its centre");
+        break;
+      }
       const double hh = p->h * p->h, hscale = 0.5 / hh;
       fill_out(c, acc, 1, &out->frc);
       out->En_kin = md[MD_EKIN] * hscale / 2;
@@ -386,17 +503,16 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
         out->PTvirc[k] = md[MD_PTV + k] / hh;
       }
       out->virc = md[MD_VIRC] / hh;
-      out->shake_iter_max = (int)md[MD_ITMAX];
-      if (p->thermostat == 1) {
-        double l = 1;
-        (void)hipMemcpy(&l, lam.p, sizeof(double), hipMemcpyDeviceToHost);
-        out->lambda = l;
+      out->shake_iter_max = (int)flag[2];
+      out->lambda = flag[0];
+      if (c.polar) {
+        out->frc.En_el += sf[SF_EEL];
+        for (int k = 0; k < 6; k++) out->frc.Pvir[k] += sf[SF_PV + k];
+        out->En_self = sf[SF_SELF] / 2;
+        out->scf_maxerr = sqrt(sf[SF_ERR]);
       }
     }
   }
   (void)hipStreamSynchronize(c.stream);
-  mdpart.release();
-  mdacc.release();
-  lam.release();
   return rc;
 }

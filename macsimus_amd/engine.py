@@ -1,7 +1,9 @@
 """Python handle on the MI355X engine (libmdx.so) through its C ABI.
 
 Mirrors cook's force call: `Engine.forces(r, measure)` is forces(B,A) of
-cook/forces.c:93-224 with A->rp = r and B->rp = the returned forces.
+cook/forces.c:93-224 with A->rp = r and B->rp = the returned forces.  For
+a Drude (POLAR) system the configuration has 2*ns rows: the centres, then
+the Drude offsets (sim/simglob.h polarrof()).
 """
 import ctypes as C
 
@@ -15,6 +17,8 @@ class Engine:
         self.lib = lib or abi.load_engine()
         self.bs = built_system            # keeps the arrays alive
         self.ns = built_system.struct.ns
+        self.polar = bool(built_system.info.get("polar"))
+        self.nrow = 2 * self.ns if self.polar else self.ns
         self.ctx = C.c_void_p()
         rc = self.lib.mdx_create(C.byref(self.ctx), C.byref(built_system.struct), mode)
         abi.check(self.lib, rc, "mdx_create")
@@ -32,7 +36,7 @@ class Engine:
             pass
 
     def forces(self, r, measure=1):
-        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.ns, 3)
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.nrow, 3)
         f = np.empty_like(r)
         out = abi.Out()
         rc = self.lib.mdx_forces(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
@@ -40,6 +44,39 @@ class Engine:
                                  int(measure))
         abi.check(self.lib, rc, "mdx_forces")
         return f, out
+
+    def selffield(self, r, f, out, omega):
+        """one selffield() iteration in place on r, f (2*ns rows); returns
+        (En.self, max SCF error)"""
+        assert r.flags.c_contiguous and f.flags.c_contiguous and r.shape == (self.nrow, 3)
+        es = C.c_double(0)
+        me = C.c_double(0)
+        rc = self.lib.mdx_selffield(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                    f.ctypes.data_as(C.POINTER(C.c_double)), C.byref(out),
+                                    float(omega), C.byref(es), C.byref(me))
+        abi.check(self.lib, rc, "mdx_selffield")
+        return es.value, me.value
+
+    def set_aspc(self, k, kk=0, hist=None):
+        ptr = None
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, dtype=np.float64)
+            assert hist.shape == (k, self.ns, 3)
+            ptr = hist.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.mdx_set_aspc(self.ctx, int(k), int(kk), ptr)
+        abi.check(self.lib, rc, "mdx_set_aspc")
+
+    def get_aspc(self):
+        k = C.c_int(0)
+        kk = C.c_int(0)
+        rc = self.lib.mdx_get_aspc(self.ctx, C.byref(k), C.byref(kk), None)
+        abi.check(self.lib, rc, "mdx_get_aspc")
+        hist = np.zeros((max(k.value, 0), self.ns, 3))
+        if k.value > 0:
+            rc = self.lib.mdx_get_aspc(self.ctx, C.byref(k), C.byref(kk),
+                                       hist.ctypes.data_as(C.POINTER(C.c_double)))
+            abi.check(self.lib, rc, "mdx_get_aspc")
+        return k.value, kk.value, hist
 
     def cook_cells(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
@@ -77,13 +114,14 @@ class Engine:
     def upload_state(self, r, r1):
         r = np.ascontiguousarray(r, dtype=np.float64)
         r1 = np.ascontiguousarray(r1, dtype=np.float64)
+        assert r.size == 3 * self.nrow and r1.size == 3 * self.nrow
         rc = self.lib.mdx_upload_state(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
                                        r1.ctypes.data_as(C.POINTER(C.c_double)))
         abi.check(self.lib, rc, "mdx_upload_state")
 
     def download_state(self):
-        r = np.empty((self.ns, 3))
-        r1 = np.empty((self.ns, 3))
+        r = np.empty((self.nrow, 3))
+        r1 = np.empty((self.nrow, 3))
         rc = self.lib.mdx_download_state(self.ctx, r.ctypes.data_as(C.POINTER(C.c_double)),
                                          r1.ctypes.data_as(C.POINTER(C.c_double)))
         abi.check(self.lib, rc, "mdx_download_state")
@@ -102,3 +140,6 @@ def declare_extras(lib):
     lib.mdx_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lib.mdx_get_aspc.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_double)]

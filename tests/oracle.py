@@ -125,3 +125,37 @@ def md_step(bs, r, r1, p, measure=1, nthreads=None):
     if rc:
         raise RuntimeError("oracle_md_step failed (%d)" % rc)
     return out
+
+
+# ---- the cook run with ASPC (tests/golden/polar306_aspc): -p420, k = 4,
+# one SCF iteration per step; dump k holds r(t_k) and the predicted Drude
+# offsets of step k.
+ASPC_K = 4
+
+
+def aspc_omega(kk):
+    """stability value of scf.omega once the predictor reached length kk
+    (sim/pred2.c:75)"""
+    return (kk + 2) / (2 * kk + 3)
+
+
+def aspc_post_scf(bs, d):
+    """post-SCF Drude offsets of each dumped step: one selffield() from the
+    dump with the omega of that step"""
+    ns = bs.struct.ns
+    post = {}
+    for k in range(1, d["r"].shape[0]):
+        r = np.concatenate([d["r"][k], d["rpol"][k]])
+        f, out = forces(bs, r)
+        selffield(bs, r, f, out, aspc_omega(min(k, ASPC_K)))
+        post[k] = r[ns:].copy()
+    return post
+
+
+def aspc_state(d, post, k):
+    """(r, r1, hist) before step k in cook's layout: Drude rows of r =
+    aa[0], of r1 = aa[1], hist = aa[2..k+1]; needs k >= ASPC_K + 3"""
+    r = np.concatenate([d["r"][k], post[k - 1]])
+    r1 = np.concatenate([d["r"][k] - d["r"][k - 1], post[k - 2]])
+    hist = np.stack([post[k - 3 - m] for m in range(ASPC_K)])
+    return r, r1, hist

@@ -41,40 +41,18 @@ def test_selffield_moves_drudes_like_reference(pol):
         assert es > 0
 
 
-ASPC_K = 4
-
-
-def _omega(kk):
-    """stability value of scf.omega after the predictor reached length kk
-    (sim/pred2.c:75)"""
-    return (kk + 2) / (2 * kk + 3)
+ASPC_K = oracle.ASPC_K
 
 
 @pytest.fixture(scope="module")
 def aspc():
-    """cook run with -p420 (ASPC, k=4), one SCF iteration per step; dump k
-    holds r(t_k) and the predicted Drude positions of step k.  The post-SCF
-    Drude positions of step k are one selffield() from the dump."""
     d, meta = oracle.load_fixture("polar306_aspc")
     bs = oracle.fixture_system(meta)
-    ns = bs.struct.ns
-    post = {}
-    for k in range(1, d["r"].shape[0]):
-        r = np.concatenate([d["r"][k], d["rpol"][k]])
-        f, out = oracle.forces(bs, r)
-        oracle.selffield(bs, r, f, out, _omega(min(k, ASPC_K)))
-        post[k] = r[ns:].copy()
-    return d, meta, bs, post
+    return d, meta, bs, oracle.aspc_post_scf(bs, d)
 
 
 def _state(d, bs, post, k):
-    """oracle state before step k (k >= ASPC_K + 3 so that the history
-    aa[0..k+1] is the post-SCF values of steps k-1 .. k-6)"""
-    ns = bs.struct.ns
-    r = np.concatenate([d["r"][k], post[k - 1]])
-    r1 = np.concatenate([d["r"][k] - d["r"][k - 1], post[k - 2]])
-    hist = np.stack([post[k - 3 - m] for m in range(ASPC_K)])
-    return r, r1, hist
+    return oracle.aspc_state(d, post, k)
 
 
 def test_aspc_prediction_matches_reference(aspc):
@@ -120,3 +98,25 @@ def test_aspc_md_step_matches_reference_trajectory(aspc):
     assert np.abs(r[:ns] - d["r"][8]).max() < 1e-10
     assert np.abs(r[ns:] - post[7]).max() < 1e-10
     assert out.En_self > 0 and out.frc.npair > 0
+
+
+def test_energy_and_pressure_tensor_after_scf(pol):
+    """cook prints Epot = En.el + En.pot + En.self and the pressure tensor
+    after the last of scf_iterations selffield() calls (sim/scf.c:12-240,
+    cook/forces.c); dump k is the state before iteration k+1"""
+    from macsimus_amd.system import PUNIT
+    d, meta, bs = pol
+    k = meta["scf_iterations"] - 1
+    r = np.concatenate([d["r"], d["rpol"][k]])
+    f, out = oracle.forces(bs, r)
+    es, me = oracle.selffield(bs, r, f, out, meta["scf_omega"])
+    Epot = out.En_el + out.En_pot + es
+    assert abs(Epot - meta["Epot"]) < 1e-10 * abs(meta["Epot"])
+    V = meta["L"] ** 3
+    ref = meta["Ptens_printed_Pa"]
+    P = [out.Pvir[i] / (3 * V) * PUNIT for i in range(3)] + [out.Pvir[i] * PUNIT for i in range(3, 6)]
+    scale = max(abs(x) for x in ref[:3])
+    for a, b in zip(P[:3], ref[:3]):
+        assert abs(a - b) < 1e-9 * scale
+    for a, b in zip(P[3:], ref[3:]):
+        assert abs(a - b) < 1e-9 * scale * 3 * V
