@@ -73,6 +73,9 @@ def main():
     bs = build_system(wl["ff"], wl["counts"], L, wl["cutoff"], epsr=0.05, epsk=0.05)
     mode = abi.MDX_MIXED if a.mode == "mixed" else abi.MDX_FP64
     eng = Engine(bs, mode)
+    # the engine's autotuner picks the cell stencil and the k-space force
+    # kernel's kz chunk and work split on this configuration (HIP events)
+    tune = eng.autotune(wl["r"], reps=2)
     info = eng.info()
     h = 0.002
     nf = workloads.degrees_of_freedom(wl["ff"], wl["counts"])
@@ -135,7 +138,8 @@ def main():
         "config": {"workload": wl["name"], "sites": int(bs.struct.ns), "charges": nq,
                    "box_A": wl["L"], "cutoff_A": wl["cutoff"], "alpha": bs.info["alpha"],
                    "kappa": bs.info["kappa"], "k_vectors": nk, "pairs": npair,
-                   "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU"},
+                   "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU",
+                   "tuned": {"stencil_w": tune[0], "kz_chunk": tune[1], "k_parts": tune[2]}},
         "us_per_force": force_ms * 1e3,
         "kernel_ms": {k: v[0] for k, v in kern.items()},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,

@@ -793,7 +793,7 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   DBuf<double> *dbl[] = {&c.r, &c.f, &c.charge, &c.lj64, &c.erf64, &c.spos, &c.part, &c.acc,
-                         &c.kpart, &c.tsum, &c.coef64, &c.kfrc, &c.r1, &c.pwork, &c.mass,
+                         &c.kpart, &c.tsum, &c.coef64, &c.r1, &c.pwork, &c.mass,
                          &c.cons_bondq, &c.mol_omega, &c.chargepol, &c.alpha_qq, &c.spol,
                          &c.qpos, &c.mdpart, &c.mdacc, &c.sfpart, &c.sfacc, &c.mdflag,
                          &c.aspc_hist};
@@ -801,6 +801,7 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   DBuf<int> *ints[] = {&c.type, &c.mol, &c.molfirst, &c.q_site, &c.cellid, &c.cnt, &c.cstart,
                        &c.cursor, &c.sidx, &c.sinfo, &c.scan_tmp, &c.d_row_kx, &c.d_row_ky,
                        &c.d_row_kzmax, &c.d_row_coef0, &c.d_seg_blk, &c.d_seg_row0,
+                       &c.d_lex_row0, &c.d_lex_meta, &c.d_chunk_kymax, &c.d_kparts,
                        &c.d_seg_rowend, &c.cons_first, &c.cons_ij};
   for (auto *b : ints) b->release();
   c.excl.release();
@@ -905,11 +906,83 @@ extern "C" int mdx_set_stencil(mdx_ctx_t *h, int w) {
   return c.wst == w ? 0 : fail(MDX_E_ARG, "mdx_set_stencil: box too small for this w");
 }
 
+// kz chunk of the k-space force kernel (8..64, multiple of 8) and the
+// number of work parts it is split into (1..256)
+extern "C" int mdx_set_kchunk(mdx_ctx_t *h, int kzc, int parts) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_kchunk: NULL");
+  Ctx &c = h->c;
+  if (!c.nq) return 0;
+  MDX_HIP(hipSetDevice(c.device));
+  return kspace_chunks(c, kzc, parts);
+}
+
+// Times force calls on the positions r (or, r == NULL, on the uploaded MD
+// state) for each cell-stencil width w (pair kernel + binning time) and
+// each k-space kz chunk (pass-2 time), with HIP events on the context
+// stream, and keeps the fastest of each.  choice[0] = w, choice[1] = kz
+// chunk, choice[2] = k-space parts; ms[0], ms[1] = their times per call.
+extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice, double *ms) {
+  if (!h || reps < 1) return fail(MDX_E_ARG, "mdx_autotune: bad argument");
+  Ctx &c = h->c;
+  MDX_HIP(hipSetDevice(c.device));
+  if (r) {
+    c.state_on_device = false;
+    MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * (c.polar ? 2 * c.ns : c.ns),
+                           hipMemcpyHostToDevice, c.stream));
+  } else if (!c.state_on_device) {
+    return fail(MDX_E_ARG, "mdx_autotune: no positions");
+  }
+  const bool timing0 = c.timing;
+  double t[T_NFAM];
+  int cnt[T_NFAM];
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  c.tm.collect(t, cnt);                // drop what the caller had recorded
+  auto run = [&](double *fam) -> int {
+    int rc;
+    c.timing = false;
+    if ((rc = forces_device(c, 0))) return rc;       // warm-up
+    c.timing = true;
+    for (int i = 0; i < reps; i++)
+      if ((rc = forces_device(c, 0))) return rc;
+    c.timing = false;
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    c.tm.collect(t, cnt);
+    for (int f = 0; f < T_NFAM; f++) fam[f] = t[f] / reps;
+    return 0;
+  };
+  int rc = 0, best_w = c.wst, best_k = c.kzc, best_p = c.kparts;
+  double tw = 1e30, tk = 1e30, fam[T_NFAM];
+  for (int w = 1; w <= 3 && !rc; w++) {
+    if (!grid_fits(c, w)) continue;
+    if ((rc = alloc_grid(c)) || (rc = run(fam))) break;
+    const double tp = fam[T_BIN] + fam[T_PAIR];
+    if (tp < tw) { tw = tp; best_w = w; }
+  }
+  if (!rc && (rc = choose_grid(c, best_w)) == 0) rc = alloc_grid(c);
+  if (c.nq) {
+    const int cand[] = {8, 16, 24, 32, 48};
+    const int pcand[] = {4, 8, 16, 32, 64};
+    for (int kzc : cand)
+      for (int parts : pcand) {
+        if (rc) break;
+        if (kzc > 8 * c.K.nblk + 8) continue;
+        if ((rc = kspace_chunks(c, kzc, parts)) || (rc = run(fam))) break;
+        if (fam[T_KPASS2] < tk) { tk = fam[T_KPASS2]; best_k = kzc; best_p = parts; }
+      }
+    if (!rc) rc = kspace_chunks(c, best_k, best_p);
+  }
+  c.timing = timing0;
+  if (rc) return rc;
+  if (choice) { choice[0] = c.wst; choice[1] = c.kzc; choice[2] = c.kparts; }
+  if (ms) { ms[0] = tw; ms[1] = tk; }
+  return 0;
+}
+
 extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
   Ctx &c = h->c;
-  int v[12] = {c.nc[0], c.nc[1], c.nc[2], c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
-               c.K.ntrip, c.nsplit, c.K.kxmax, c.mode};
-  for (int i = 0; i < n && i < 12; i++) iv[i] = v[i];
+  int v[16] = {c.nc[0], c.nc[1], c.nc[2], c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+               c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart};
+  for (int i = 0; i < n && i < 16; i++) iv[i] = v[i];
   return 0;
 }

@@ -119,6 +119,10 @@ struct KLayout {
   int kzpad = 0;                       // 8*nblk
   std::vector<int> row_kx, row_ky, row_kzmax, row_coef0;
   std::vector<int> seg_blk, seg_row0, seg_rowend;
+  // pass 2: rows in lexicographic order (kx, ky); per row its number of
+  // 4-kz coefficient blocks and its first coefficient kz slot
+  std::vector<int> lex_row0, lex_meta;  // [kxmax+1], [2*nrows]
+  int lex_maxblk = 0;
 };
 
 struct Ctx {
@@ -162,11 +166,22 @@ struct Ctx {
   int nsplit = 1;
   DBuf<int> d_row_kx, d_row_ky, d_row_kzmax, d_row_coef0;
   DBuf<int> d_seg_blk, d_seg_row0, d_seg_rowend;
+  // pass 2: kz chunk of the force kernel (8..64, set by the tuner), the
+  // number of chunks and per chunk the largest ky of each kx whose row
+  // reaches into the chunk (-1: none); rows reaching a chunk form a
+  // ky-prefix for each kx since kzmax(kx,ky) falls with ky
+  int kzc = 16, nchunk = 0;
+  DBuf<int> d_lex_row0, d_lex_meta, d_chunk_kymax;
+  // pass 2 is split into parts (chunk, kx range) of about equal work so
+  // that the grid has several waves per SIMD; partial forces per part
+  // [nkpart][nq][3] are summed in a fixed order
+  int kparts = 16, nkpart = 0;
+  DBuf<int> d_kparts;
+  DBuf<double> kpf;
   DBuf<double> kpart;          // pass-1 partials [nsplit][nseg][64][64]
   DBuf<double> tsum;           // reduced T sums [nseg][64][64]
   DBuf<float> coef32;          // pass-2 coefficients [ntrip*16]
   DBuf<double> coef64;
-  DBuf<double> kfrc;           // [3nq] k-space forces per charge (FP64)
   // MD state
   DBuf<double> r1, pwork;
   DBuf<double> mass;
@@ -190,6 +205,9 @@ struct Ctx {
 int md_setup(Ctx &c);
 // set-up of the k-space layout for the context box and kappa
 int kspace_setup(Ctx &c);
+// pass-2 tables for kz chunk kzc (8..64, multiple of 8) and about
+// `parts` work parts
+int kspace_chunks(Ctx &c, int kzc, int parts);
 // forces of the charges from the k-space sum (adds into c.f); energy and
 // tensor sums into c.acc when measure
 int kspace_forces(Ctx &c, int measure);

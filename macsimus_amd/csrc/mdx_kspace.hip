@@ -300,59 +300,65 @@ __global__ void k_reduce_cols(const double *part, int nblocks, int a0, int a1, d
 }
 
 // ------------------------------------------------------------------ pass 2
-// Lane <-> charge.  Rows (kx,ky) in lexicographic order; cos/sin of c for
-// kz < KZ live in registers, of a from FP64 sincospi at each new kx, of b by
-// FP64 recursion along ky.  The 16 coefficients of a triple are uniform
-// across the wave (scalar loads).  G, Gz accumulate over kz in R, forces
-// in FP64.  Adds -q_j grad Psi (+ dipole field) to f.
-template <typename R, int KZ>
+// Lane <-> charge.  The kz range is taken in chunks of KZC: per chunk the
+// cos/sin of c for its kz live in registers (2 KZC of them, so the chunk
+// sets the register footprint and the occupancy), the rows (kx,ky) that
+// reach into the chunk are visited in lexicographic order, cos/sin of a
+// come from FP64 sincospi at each new kx and those of b from an FP64
+// recursion along ky.  The 16 coefficients of a triple are uniform across
+// the wave (scalar loads).  G, H accumulate over kz in R, forces in FP64.
+// Adds -q_j grad Psi (+ dipole field) to f.
+template <typename R, int KZC>
 __global__ __launch_bounds__(kBlock) void k_kforce(
-    int nq, const int *__restrict__ qsite, const double4 *__restrict__ qpos, double Lx,
-    double Ly, double Lz, int kxmax,
-    const int *__restrict__ lex_kymax, const int *__restrict__ lex_row0,
-    const int *__restrict__ row_nblk_lex, const int *__restrict__ row_coef0_lex,
-    const R *__restrict__ coef, const double *__restrict__ acc, double cdip,
-    double *__restrict__ f) {
+    int nq, const double4 *__restrict__ qpos, double iLx, double iLy, double iLz, int kxmax,
+    const int4 *__restrict__ parts, const int *__restrict__ chunk_kymax,
+    const int *__restrict__ lex_row0, const int2 *__restrict__ lex_meta,
+    const R *__restrict__ coef, double *__restrict__ kpf) {
+  constexpr int BPC = KZC / kKB;  // coefficient blocks per chunk
   const int j = blockIdx.x * kBlock + threadIdx.x;
+  const int4 part = parts[blockIdx.y];   // chunk, kx begin, kx end
+  const int ch = part.x;
   const double4 qp = qpos[min(j, nq - 1)];
-  const double sx = frac1(qp.x / Lx), sy = frac1(qp.y / Ly), sz = frac1(qp.z / Lz);
-  R zc[KZ], zs[KZ];
+  const double sx = frac1(qp.x * iLx), sy = frac1(qp.y * iLy), sz = frac1(qp.z * iLz);
+  const R kxs = (R)iLx, kys = (R)iLy;
+  double cb1, sb1;
+  cs2pi(sy, cb1, sb1);
+  R zc[KZC], zs[KZC];
   {
-    double c1, s1;
-    cs2pi(sz, c1, s1);
+    double cz1, sz1;
+    cs2pi(sz, cz1, sz1);
 #pragma unroll
-    for (int m = 0; m < KZ / 8; m++) {
+    for (int m = 0; m < KZC / 8; m++) {
       double c, s;
-      cs2pi(8.0 * m * sz, c, s);
+      cs2pi((double)(ch * KZC + 8 * m) * sz, c, s);
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         zc[8 * m + k] = (R)c;
         zs[8 * m + k] = (R)s;
-        const double nc = c * c1 - s * s1;
-        s = c * s1 + s * c1;
+        const double nc = c * cz1 - s * sz1;
+        s = c * sz1 + s * cz1;
         c = nc;
       }
     }
   }
-  double cb1, sb1;
-  cs2pi(sy, cb1, sb1);
+  const int *kym_c = chunk_kymax + ch * (kxmax + 1);
   double Fx = 0, Fy = 0, Fz = 0;
-  for (int kx = 0; kx <= kxmax; kx++) {
+  for (int kx = part.y; kx < part.z; kx++) {
+    const int kym = kym_c[kx];
     double ca, sa;
     cs2pi(kx * sx, ca, sa);
     const R rca = (R)ca, rsa = (R)sa;
-    const R kxl = (R)(kx / Lx);
-    const int kym = lex_kymax[kx], rbase = lex_row0[kx];
+    const R kxl = (R)kx * kxs;
+    const int rbase = lex_row0[kx];
     double cb = 1, sb = 0;
     for (int ky = 0; ky <= kym; ky++) {
-      const int row = rbase + ky;
-      // number of kKB-blocks of kz with coefficients (rows padded with zeros)
-      const int nb = __builtin_amdgcn_readfirstlane(row_nblk_lex[row]);
-      const R *cf = coef + (size_t)__builtin_amdgcn_readfirstlane(row_coef0_lex[row]) * 16;
+      const int2 mt = lex_meta[rbase + ky];
+      // 4-kz blocks of this row inside the chunk (>= 1), wave-uniform
+      const int nb = __builtin_amdgcn_readfirstlane(min(mt.x - ch * BPC, BPC));
+      const R *cf = coef + ((size_t)__builtin_amdgcn_readfirstlane(mt.y) + ch * KZC) * 16;
       R G0 = 0, G1 = 0, G2 = 0, G3 = 0, H0 = 0, H1 = 0, H2 = 0, H3 = 0;
-      // fully unrolled over the register arrays; nb is wave-uniform
 #pragma unroll
-      for (int b = 0; b < KZ / kKB; b++) {
+      for (int b = 0; b < BPC; b++) {
         if (b < nb) {
           const R *e = cf + b * (16 * kKB);
 #pragma unroll
@@ -371,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_kforce(
         }
       }
       const R rcb = (R)cb, rsb = (R)sb;
-      const R kyl = (R)(ky / Ly);
+      const R kyl = (R)ky * kys;
       Fx += (double)(kxl * (rca * (rcb * G2 + rsb * G3) - rsa * (rcb * G0 + rsb * G1)));
       Fy += (double)(kyl * (rca * (rcb * G1 - rsb * G0) + rsa * (rcb * G3 - rsb * G2)));
       Fz += (double)(rca * (rcb * H0 + rsb * H1) + rsa * (rcb * H2 + rsb * H3));
@@ -381,12 +387,31 @@ __global__ __launch_bounds__(kBlock) void k_kforce(
     }
   }
   if (j >= nq) return;
+  double *o = kpf + ((size_t)blockIdx.y * nq + j) * 3;
+  o[0] = Fx;
+  o[1] = Fy;
+  o[2] = Fz;
+}
+
+// sums the parts of pass 2 in a fixed order and adds -q grad Psi and the
+// dipole field to the force rows of the charges
+__global__ void k_kforce_sum(int nq, int nparts, const double *__restrict__ kpf,
+                             const int *__restrict__ qsite, const double4 *__restrict__ qpos,
+                             const double *__restrict__ acc, double cdip,
+                             double *__restrict__ f) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nq) return;
+  double F[3] = {0, 0, 0};
+  for (int p = 0; p < nparts; p++) {
+    const double *o = kpf + ((size_t)p * nq + j) * 3;
+    F[0] += o[0];
+    F[1] += o[1];
+    F[2] += o[2];
+  }
   const int site = qsite[j];   // row of f (Drudes after the centres)
-  const double q = qp.w;
+  const double q = qpos[j].w;
   const double *M = acc + mdx::ACC_M0;
-  f[3 * site] += -q * Fx - q * cdip * M[0];
-  f[3 * site + 1] += -q * Fy - q * cdip * M[1];
-  f[3 * site + 2] += -q * Fz - q * cdip * M[2];
+  for (int d = 0; d < 3; d++) f[3 * site + d] += -q * F[d] - q * cdip * M[d];
 }
 
 template <typename R>
@@ -423,19 +448,21 @@ int run_kspace(mdx::Ctx &c, int measure) {
   k_dipole<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, qpos, c.part.p);
   k_reduce_cols<<<1, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, ACC_M0, ACC_M0 + 3, c.acc.p);
   if (c.timing) { c.tm.stop(T_KCOEF, c.stream); c.tm.start(T_KPASS2, c.stream); }
-  const int *lk = c.d_row_kx.p + K.nrows;  // lexicographic tables stored after the sorted ones
-  const int *lr0 = c.d_row_ky.p + K.nrows;
-  const int *lkz = c.d_row_kzmax.p + K.nrows;  // kKB-blocks per lexicographic row
-  const int *lc0 = c.d_row_coef0.p + K.nrows;
-#define MDX_KF(KZV)                                                                          \
-  k_kforce<R, KZV><<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.q_site.p, qpos,              \
-                                                         c.L[0], c.L[1], c.L[2], K.kxmax, lk, \
-                                                         lr0, lkz, lc0, coef, c.acc.p, cdip, c.f.p)
-  if (KZT <= 16) MDX_KF(16);
-  else if (KZT <= 32) MDX_KF(32);
-  else if (KZT <= 48) MDX_KF(48);
-  else if (KZT <= 64) MDX_KF(64);
-  else return fail(MDX_E_ARG, "k-space: kz limit above 63 is not supported");
+#define MDX_KF(KZV)                                                                         \
+  k_kforce<R, KZV><<<dim3(nblk(c.nq), c.nkpart), kBlock, 0, c.stream>>>(                    \
+      c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.kxmax, (const int4 *)c.d_kparts.p,  \
+      c.d_chunk_kymax.p, c.d_lex_row0.p, (const int2 *)c.d_lex_meta.p, coef, c.kpf.p)
+  switch (c.kzc) {
+    case 8: MDX_KF(8); break;
+    case 16: MDX_KF(16); break;
+    case 24: MDX_KF(24); break;
+    case 32: MDX_KF(32); break;
+    case 48: MDX_KF(48); break;
+    case 64: MDX_KF(64); break;
+    default: return fail(MDX_E_ARG, "k-space: kz chunk must be 8, 16, 24, 32, 48 or 64");
+  }
+  k_kforce_sum<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.nkpart, c.kpf.p, c.q_site.p, qpos,
+                                                    c.acc.p, cdip, c.f.p);
 #undef MDX_KF
   MDX_HIP(hipGetLastError());
   if (c.timing) c.tm.stop(T_KPASS2, c.stream);
@@ -503,6 +530,14 @@ int kspace_setup(Ctx &c) {
     }
   }
   K.nseg = (int)K.seg_blk.size();
+  K.lex_row0 = lrow0;
+  K.lex_meta.resize(2 * K.nrows);
+  K.lex_maxblk = 0;
+  for (int i = 0; i < K.nrows; i++) {
+    K.lex_meta[2 * i] = lnb[i];
+    K.lex_meta[2 * i + 1] = lcoef[i];
+    K.lex_maxblk = std::max(K.lex_maxblk, lnb[i]);
+  }
   // charge splits: about 8 workgroups per CU, at least 256 charges each
   const int wg = (K.nseg + 3) / 4;
   c.nsplit = std::max(1, std::min(2048 / std::max(wg, 1), c.nq / 256));
@@ -511,8 +546,10 @@ int kspace_setup(Ctx &c) {
     a.insert(a.end(), b.begin(), b.end());
     return a;
   };
-  std::vector<int> kx_all = cat(K.row_kx, lkymax), ky_all = cat(K.row_ky, lrow0);
-  std::vector<int> kz_all = cat(K.row_kzmax, lnb), c0_all = cat(K.row_coef0, lcoef);
+  std::vector<int> kx_all = K.row_kx, ky_all = K.row_ky;
+  std::vector<int> kz_all = K.row_kzmax, c0_all = K.row_coef0;
+  (void)cat;
+  (void)lkymax;
   int rc;
   if ((rc = c.d_row_kx.alloc(kx_all.size())) || (rc = c.d_row_ky.alloc(ky_all.size())) ||
       (rc = c.d_row_kzmax.alloc(kz_all.size())) || (rc = c.d_row_coef0.alloc(c0_all.size())) ||
@@ -536,6 +573,83 @@ int kspace_setup(Ctx &c) {
       !up(c.d_row_coef0.p, c0_all) || !up(c.d_seg_blk.p, K.seg_blk) ||
       !up(c.d_seg_row0.p, K.seg_row0) || !up(c.d_seg_rowend.p, K.seg_rowend))
     return fail(MDX_E_GPU, "kspace_setup: upload failed");
+  if ((rc = c.d_lex_row0.alloc(K.lex_row0.size())) || (rc = c.d_lex_meta.alloc(K.lex_meta.size())))
+    return rc;
+  if (!up(c.d_lex_row0.p, K.lex_row0) || !up(c.d_lex_meta.p, K.lex_meta))
+    return fail(MDX_E_GPU, "kspace_setup: upload failed");
+  return kspace_chunks(c, c.kzc, c.kparts);
+}
+
+// Work split of pass 2 for kz chunk kzc into about `parts` parts: per
+// chunk the largest ky of each kx whose row reaches into the chunk, and
+// parts (chunk, kx begin, kx end) of about equal numbers of coefficient
+// blocks (a row costs its blocks plus about two blocks of phase and
+// force arithmetic).
+int kspace_chunks(Ctx &c, int kzc, int parts) {
+  const KLayout &K = c.K;
+  if (kzc < 8 || kzc > 64 || kzc % 8) return fail(MDX_E_ARG, "k-space: kz chunk in {8,16,...,64}");
+  if (parts < 1 || parts > 256) return fail(MDX_E_ARG, "k-space: parts in 1..256");
+  const int bpc = kzc / kKB, nkx = K.kxmax + 1;
+  const int nchunk = std::max(1, (K.lex_maxblk + bpc - 1) / bpc);
+  std::vector<int> kym((size_t)nchunk * nkx, -1);
+  std::vector<double> work((size_t)nchunk * nkx, 0);
+  double total = 0;
+  for (int ch = 0; ch < nchunk; ch++)
+    for (int kx = 0; kx < nkx; kx++) {
+      const int r0 = K.lex_row0[kx], r1 = kx + 1 < nkx ? K.lex_row0[kx + 1] : K.nrows;
+      for (int r = r0; r < r1; r++) {
+        const int nb = K.lex_meta[2 * r] - ch * bpc;
+        if (nb <= 0) continue;
+        kym[(size_t)ch * nkx + kx] = r - r0;
+        work[(size_t)ch * nkx + kx] += std::min(nb, bpc) + 2.0;
+      }
+      total += work[(size_t)ch * nkx + kx];
+    }
+  // the kernel walks each chunk's rows as a ky-prefix per kx: check it
+  for (int ch = 0; ch < nchunk; ch++)
+    for (int kx = 0; kx < nkx; kx++) {
+      const int v = kym[(size_t)ch * nkx + kx], r0 = K.lex_row0[kx];
+      for (int ky = 0; ky <= v; ky++)
+        if (K.lex_meta[2 * (r0 + ky)] <= ch * bpc)
+          return fail(MDX_E_ARG, "k-space: chunk rows are not a prefix in ky");
+    }
+  std::vector<int> pt;
+  const double target = total / parts;
+  for (int ch = 0; ch < nchunk; ch++) {
+    int kx0 = 0;
+    double w = 0;
+    for (int kx = 0; kx < nkx; kx++) {
+      w += work[(size_t)ch * nkx + kx];
+      const bool last = kx + 1 == nkx;
+      if (w > 0 && (w >= target || last)) {
+        pt.insert(pt.end(), {ch, kx0, kx + 1, 0});
+        kx0 = kx + 1;
+        w = 0;
+      }
+    }
+  }
+  // drop the tails of kx without rows in the chunk (kym < 0): kernels skip
+  // nothing inside a part, so trim each part to rows that exist
+  std::vector<int> ptrim;
+  for (size_t i = 0; i < pt.size(); i += 4) {
+    int ch = pt[i], a = pt[i + 1], b = pt[i + 2];
+    while (b > a && kym[(size_t)ch * nkx + b - 1] < 0) b--;
+    while (a < b && kym[(size_t)ch * nkx + a] < 0) a++;
+    for (int kx = a; kx < b; kx++)
+      if (kym[(size_t)ch * nkx + kx] < 0) return fail(MDX_E_ARG, "k-space: chunk rows are not a prefix in kx");
+    if (b > a) ptrim.insert(ptrim.end(), {ch, a, b, 0});
+  }
+  const int np = (int)ptrim.size() / 4;
+  int rc;
+  if ((rc = c.d_chunk_kymax.alloc(kym.size())) || (rc = c.d_kparts.alloc(ptrim.size())) ||
+      (rc = c.kpf.alloc((size_t)np * c.nq * 3)))
+    return rc;
+  MDX_HIP(hipMemcpy(c.d_chunk_kymax.p, kym.data(), sizeof(int) * kym.size(), hipMemcpyHostToDevice));
+  MDX_HIP(hipMemcpy(c.d_kparts.p, ptrim.data(), sizeof(int) * ptrim.size(), hipMemcpyHostToDevice));
+  c.kzc = kzc;
+  c.nchunk = nchunk;
+  c.kparts = parts;
+  c.nkpart = np;
   return 0;
 }
 

@@ -87,11 +87,29 @@ class Engine:
         return cid
 
     def info(self):
-        iv = (C.c_int * 12)()
-        self.lib.mdx_info(self.ctx, iv, 12)
+        iv = (C.c_int * 16)()
+        self.lib.mdx_info(self.ctx, iv, 16)
         keys = ("ncx", "ncy", "ncz", "stencil", "nq", "krows", "kblocks", "ksegments",
-                "ktriples", "ksplits", "kxmax", "mode")
+                "ktriples", "ksplits", "kxmax", "mode", "kchunk", "nkchunks", "kparts_asked",
+                "kparts")
         return dict(zip(keys, list(iv)))
+
+    def set_kchunk(self, kzc, parts=16):
+        rc = self.lib.mdx_set_kchunk(self.ctx, int(kzc), int(parts))
+        abi.check(self.lib, rc, "mdx_set_kchunk")
+
+    def autotune(self, r=None, reps=3):
+        """picks the fastest cell stencil and k-space kz chunk on r (None:
+        the uploaded MD state); returns (w, kz chunk, parts, ms pair, ms pass 2)"""
+        choice = (C.c_int * 3)()
+        ms = (C.c_double * 2)()
+        ptr = None
+        if r is not None:
+            r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.nrow, 3)
+            ptr = r.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.mdx_autotune(self.ctx, ptr, int(reps), choice, ms)
+        abi.check(self.lib, rc, "mdx_autotune")
+        return choice[0], choice[1], choice[2], ms[0], ms[1]
 
     def set_timing(self, on=True):
         self.lib.mdx_set_timing(self.ctx, int(on))
@@ -140,6 +158,9 @@ def declare_extras(lib):
     lib.mdx_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_kchunk.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
     lib.mdx_get_aspc.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_double)]
