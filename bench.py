@@ -139,7 +139,8 @@ def main():
                    "box_A": wl["L"], "cutoff_A": wl["cutoff"], "alpha": bs.info["alpha"],
                    "kappa": bs.info["kappa"], "k_vectors": nk, "pairs": npair,
                    "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU",
-                   "tuned": {"stencil_w": tune[0], "kz_chunk": tune[1], "k_parts": tune[2]}},
+                   "tuned": {"stencil_w": tune[0], "kz_chunk": tune[1], "k_parts": tune[2],
+                             "pair_cols_per_part": tune[3]}},
         "us_per_force": force_ms * 1e3,
         "kernel_ms": {k: v[0] for k, v in kern.items()},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,

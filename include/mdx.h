@@ -204,22 +204,24 @@ int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
    kernel families (bin, pair, excl, kpass1, kcoef, kpass2, shake) on the
    context stream; ms[i], count[i] accumulate since the last get.
    mdx_set_stencil: cell-stencil half width w (1..6) of the pair kernel
-   (a wider one is taken where the box needs it).
+   (a wider one is taken where the box needs it) and the split of its
+   (2w+1)^2 stencil columns into parts of pcols columns (0: one part),
+   which sets the waves in flight.
    mdx_set_kchunk: kz chunk (8..64, multiple of 8) of the k-space force
    kernel, which sets its register footprint, and the number of parts
    (1..256) its k-set is split into, which sets the waves in flight.
    mdx_autotune: times reps force calls per candidate on the positions r
    (NULL: the uploaded MD state) with HIP events and keeps the fastest
    stencil w (1..3, binning + pair time) and kz chunk and parts (pass-2
-   time); choice[3] = {w, kz chunk, parts}, ms[2] = their times per call
-   (either may be NULL).
+   time) and stencil columns per pair part; choice[4] = {w, kz chunk,
+   parts, columns}, ms[2] = their times per call (either may be NULL).
    mdx_info: iv[0..n) = engine cells x,y,z, stencil w, charges, k rows,
    kz blocks, k segments, k triples, k-sum splits, kx max, mode, kz chunk,
-   kz chunks, parts asked, parts made. */
+   kz chunks, parts asked, parts made, stencil columns per pair part. */
 int mdx_cook_cells(mdx_ctx_t *ctx, const double *r, int *cellid);
 int mdx_set_timing(mdx_ctx_t *ctx, int on);
 int mdx_get_timing(mdx_ctx_t *ctx, double *ms, int *count, int n);
-int mdx_set_stencil(mdx_ctx_t *ctx, int w);
+int mdx_set_stencil(mdx_ctx_t *ctx, int w, int pcols);
 int mdx_set_kchunk(mdx_ctx_t *ctx, int kzc, int parts);
 int mdx_autotune(mdx_ctx_t *ctx, const double *r, int reps, int *choice, double *ms);
 int mdx_info(mdx_ctx_t *ctx, int *iv, int n);

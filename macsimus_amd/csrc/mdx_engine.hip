@@ -44,7 +44,9 @@ __device__ inline double wave_sum(double v) {
 
 // sums NV per-thread values over the block into part[blockIdx.x*ACC_N+idx[k]]
 template <int NV>
-__device__ inline void block_store(double (&v)[NV], const int (&idx)[NV], double *part) {
+__device__ inline void block_store(double (&v)[NV], const int (&idx)[NV], double *part,
+                                   size_t row = (size_t)-1) {
+  if (row == (size_t)-1) row = blockIdx.x;
   __shared__ double red[kBlock / 64][NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -56,7 +58,7 @@ __device__ inline void block_store(double (&v)[NV], const int (&idx)[NV], double
   if (threadIdx.x < NV) {
     double s = 0;
     for (int w = 0; w < kBlock / 64; w++) s += red[w][threadIdx.x];
-    part[(size_t)blockIdx.x * ACC_N + idx[threadIdx.x]] += s;
+    part[row * ACC_N + idx[threadIdx.x]] += s;
   }
 }
 
@@ -269,11 +271,13 @@ __device__ inline void qq_term(const R *erf, double sgrid, R qq, const double (&
 // Drude (pc, pp); LJ acts between centres only.
 template <typename R, bool MEASURE, bool POLAR>
 __global__ __launch_bounds__(kBlock) void k_pair(
-    int ns, Grid g, int w, double cq, const double4 *__restrict__ spos,
+    int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
     const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
     double sgrid, double *__restrict__ f, double *__restrict__ part) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
+  // part blockIdx.y takes the stencil columns (dx,dy) [col0, col1)
+  const int span = 2 * w + 1, col0 = blockIdx.y * cpp, col1 = min(col0 + cpp, span * span);
   double fx = 0, fy = 0, fz = 0, px = 0, py = 0, pz = 0;
   double U = 0, Eel = 0, Vir = 0, pv[6] = {0, 0, 0, 0, 0, 0};
   double npair = 0, nlj = 0;
@@ -287,11 +291,12 @@ __global__ __launch_bounds__(kBlock) void k_pair(
     const int cz = c % g.nz, cy = (c / g.nz) % g.ny, cx = c / (g.nz * g.ny);
     const R qi = (R)pi.w, qpi = (R)ri.w;
     const R *ljrow = tab.lj + (size_t)ti * nt * 8;
-    for (int dx = -w; dx <= w; dx++) {
+    for (int col = col0; col < col1; col++) {
+      const int dx = col / span - w, dy = col % span - w;
       int nx = cx + dx;
       double sx = 0;
       if (nx < 0) { nx += g.nx; sx = -g.Lx; } else if (nx >= g.nx) { nx -= g.nx; sx = g.Lx; }
-      for (int dy = -w; dy <= w; dy++) {
+      {
         int ny = cy + dy;
         double sy = 0;
         if (ny < 0) { ny += g.ny; sy = -g.Ly; } else if (ny >= g.ny) { ny -= g.ny; sy = g.Ly; }
@@ -370,27 +375,41 @@ __global__ __launch_bounds__(kBlock) void k_pair(
         if (POLAR) { px += ap[0]; py += ap[1]; pz += ap[2]; }
       }
     }
+    // one part: the force rows themselves; several: partial rows
+    // [part][nrow][3], summed in order by k_pair_sum
     const int site = ii.w;
-    f[3 * site] = fx;
-    f[3 * site + 1] = fy;
-    f[3 * site + 2] = fz;
+    double *o = f + (size_t)blockIdx.y * (POLAR ? 2 : 1) * ns * 3;
+    o[3 * site] = fx;
+    o[3 * site + 1] = fy;
+    o[3 * site + 2] = fz;
     if (POLAR) {
-      f[3 * (ns + site)] = px;
-      f[3 * (ns + site) + 1] = py;
-      f[3 * (ns + site) + 2] = pz;
+      o[3 * (ns + site)] = px;
+      o[3 * (ns + site) + 1] = py;
+      o[3 * (ns + site) + 2] = pz;
     }
   }
+  const size_t row = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
   if (MEASURE) {
     double v[12] = {0.5 * U, 0.5 * Eel, 0.5 * Vir, 0.5 * pv[0], 0.5 * pv[1], 0.5 * pv[2],
                     0.5 * pv[3], 0.5 * pv[4], 0.5 * pv[5], 0.5 * npair, 0.5 * nlj, 0};
     const int idx[12] = {ACC_U, ACC_EEL, ACC_VIR, ACC_PV0, ACC_PV0 + 1, ACC_PV0 + 2,
                          ACC_PV0 + 3, ACC_PV0 + 4, ACC_PV0 + 5, ACC_NPAIR, ACC_NLJ, ACC_SUMQ};
-    block_store<12>(v, idx, part);
+    block_store<12>(v, idx, part, row);
   } else {
     double v[2] = {0.5 * npair, 0.5 * nlj};
     const int idx[2] = {ACC_NPAIR, ACC_NLJ};
-    block_store<2>(v, idx, part);
+    block_store<2>(v, idx, part, row);
   }
+}
+
+// force rows = sum of the stencil parts in a fixed order
+__global__ void k_pair_sum(long n3, int nparts, const double *__restrict__ pf,
+                           double *__restrict__ f) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  double s = 0;
+  for (int p = 0; p < nparts; p++) s += pf[(size_t)p * n3 + i];
+  f[i] = s;
 }
 
 // --------------------------------------------------- exclusion corrections
@@ -524,8 +543,12 @@ int alloc_grid(Ctx &c) {
   int rc;
   if ((rc = c.cnt.alloc(ncell)) || (rc = c.cstart.alloc(ncell + 1)) || (rc = c.cursor.alloc(ncell)))
     return rc;
-  c.part_blocks = std::max(nblk(c.ns), nblk(c.nmol));
+  const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
+  const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
+  const int np = (ncol + cpp - 1) / cpp;
+  c.part_blocks = std::max(nblk(c.ns) * np, nblk(c.nmol));
   c.part_blocks = std::max(c.part_blocks, 1024);
+  if (np > 1 && (rc = c.pf.alloc((size_t)np * 3 * (c.polar ? 2 : 1) * c.ns))) return rc;
   return c.part.alloc((size_t)c.part_blocks * ACC_N);
 }
 
@@ -552,16 +575,25 @@ int launch_pair_t(Ctx &c, int measure) {
   if constexpr (sizeof(R) == 4) { tab.lj = c.lj32.p; tab.erf = c.erf32.p; }
   else { tab.lj = (const R *)c.lj64.p; tab.erf = (const R *)c.erf64.p; }
   const double cq = c.cutoff * c.cutoff;
+  const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
+  const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
+  const int np = (ncol + cpp - 1) / cpp;
+  const dim3 grid(nblk(c.ns), np);
+  double *out = np > 1 ? c.pf.p : c.f.p;
   if (measure)
-    k_pair<R, true, POLAR><<<nblk(c.ns), kBlock, 0, c.stream>>>(
-        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
-        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p,
+    k_pair<R, true, POLAR><<<grid, kBlock, 0, c.stream>>>(
+        c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
         c.part.p);
   else
-    k_pair<R, false, POLAR><<<nblk(c.ns), kBlock, 0, c.stream>>>(
-        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
-        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, c.f.p,
+    k_pair<R, false, POLAR><<<grid, kBlock, 0, c.stream>>>(
+        c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
         c.part.p);
+  if (np > 1) {
+    const long n3 = 3L * (POLAR ? 2 : 1) * c.ns;
+    k_pair_sum<<<nblk(n3), kBlock, 0, c.stream>>>(n3, np, c.pf.p, c.f.p);
+  }
   MDX_HIP(hipGetLastError());
   return 0;
 }
@@ -897,11 +929,15 @@ extern "C" int mdx_get_timing(mdx_ctx_t *h, double *ms, int *count, int n) {
   return 0;
 }
 
-// engine cell grid (stencil half width w), for the autotuner
-extern "C" int mdx_set_stencil(mdx_ctx_t *h, int w) {
-  if (!h || w < 1 || w > kMaxStencil) return fail(MDX_E_ARG, "mdx_set_stencil: w in 1..6");
+// engine cell grid (stencil half width w) and the pair kernel's split of
+// the stencil into parts of pcols (dx,dy) columns (0: one part)
+extern "C" int mdx_set_stencil(mdx_ctx_t *h, int w, int pcols) {
+  if (!h || w < 1 || w > kMaxStencil || pcols < 0)
+    return fail(MDX_E_ARG, "mdx_set_stencil: w in 1..6, pcols >= 0");
   Ctx &c = h->c;
+  MDX_HIP(hipSetDevice(c.device));
   int rc;
+  c.pcols = pcols;
   if ((rc = choose_grid(c, w)) || (rc = alloc_grid(c))) return rc;
   return c.wst == w ? 0 : fail(MDX_E_ARG, "mdx_set_stencil: box too small for this w");
 }
@@ -920,7 +956,8 @@ extern "C" int mdx_set_kchunk(mdx_ctx_t *h, int kzc, int parts) {
 // state) for each cell-stencil width w (pair kernel + binning time) and
 // each k-space kz chunk (pass-2 time), with HIP events on the context
 // stream, and keeps the fastest of each.  choice[0] = w, choice[1] = kz
-// chunk, choice[2] = k-space parts; ms[0], ms[1] = their times per call.
+// chunk, choice[2] = k-space parts, choice[3] = stencil columns per pair
+// part; ms[0], ms[1] = their times per call.
 extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice, double *ms) {
   if (!h || reps < 1) return fail(MDX_E_ARG, "mdx_autotune: bad argument");
   Ctx &c = h->c;
@@ -950,14 +987,20 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
     for (int f = 0; f < T_NFAM; f++) fam[f] = t[f] / reps;
     return 0;
   };
-  int rc = 0, best_w = c.wst, best_k = c.kzc, best_p = c.kparts;
+  int rc = 0, best_w = c.wst, best_pc = c.pcols, best_k = c.kzc, best_p = c.kparts;
   double tw = 1e30, tk = 1e30, fam[T_NFAM];
   for (int w = 1; w <= 3 && !rc; w++) {
     if (!grid_fits(c, w)) continue;
-    if ((rc = alloc_grid(c)) || (rc = run(fam))) break;
-    const double tp = fam[T_BIN] + fam[T_PAIR];
-    if (tp < tw) { tw = tp; best_w = w; }
+    const int span = 2 * w + 1;
+    const int pcand[] = {0, span, (span + 1) / 2, 1};
+    for (int pc : pcand) {
+      c.pcols = pc;
+      if ((rc = alloc_grid(c)) || (rc = run(fam))) break;
+      const double tp = fam[T_BIN] + fam[T_PAIR];
+      if (tp < tw) { tw = tp; best_w = w; best_pc = pc; }
+    }
   }
+  c.pcols = best_pc;
   if (!rc && (rc = choose_grid(c, best_w)) == 0) rc = alloc_grid(c);
   if (c.nq) {
     const int cand[] = {8, 16, 24, 32, 48};
@@ -973,7 +1016,7 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
   }
   c.timing = timing0;
   if (rc) return rc;
-  if (choice) { choice[0] = c.wst; choice[1] = c.kzc; choice[2] = c.kparts; }
+  if (choice) { choice[0] = c.wst; choice[1] = c.kzc; choice[2] = c.kparts; choice[3] = c.pcols; }
   if (ms) { ms[0] = tw; ms[1] = tk; }
   return 0;
 }
@@ -981,8 +1024,9 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
 extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
   Ctx &c = h->c;
-  int v[16] = {c.nc[0], c.nc[1], c.nc[2], c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
-               c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart};
-  for (int i = 0; i < n && i < 16; i++) iv[i] = v[i];
+  int v[17] = {c.nc[0], c.nc[1], c.nc[2], c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+               c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart,
+               c.pcols};
+  for (int i = 0; i < n && i < 17; i++) iv[i] = v[i];
   return 0;
 }

@@ -153,6 +153,11 @@ struct Ctx {
   // engine cell grid (pair kernel)
   int nc[3] = {0, 0, 0};
   int wst = 2;                 // stencil half width in cells
+  // the pair kernel's stencil columns (dx,dy) are split into parts of
+  // pcols columns (0: one part) so that the grid has several waves per
+  // SIMD; partial force rows [part][nrow][3] are summed in a fixed order
+  int pcols = 0;
+  DBuf<double> pf;
   DBuf<int> cellid, cnt, cstart, cursor, sidx;
   DBuf<double> spos;           // [4ns] x,y,z wrapped, q  (cell order)
   DBuf<int> sinfo;             // [4ns] type, mol, excl, site

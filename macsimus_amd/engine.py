@@ -87,11 +87,11 @@ class Engine:
         return cid
 
     def info(self):
-        iv = (C.c_int * 16)()
-        self.lib.mdx_info(self.ctx, iv, 16)
+        iv = (C.c_int * 17)()
+        self.lib.mdx_info(self.ctx, iv, 17)
         keys = ("ncx", "ncy", "ncz", "stencil", "nq", "krows", "kblocks", "ksegments",
                 "ktriples", "ksplits", "kxmax", "mode", "kchunk", "nkchunks", "kparts_asked",
-                "kparts")
+                "kparts", "pair_cols")
         return dict(zip(keys, list(iv)))
 
     def set_kchunk(self, kzc, parts=16):
@@ -100,8 +100,9 @@ class Engine:
 
     def autotune(self, r=None, reps=3):
         """picks the fastest cell stencil and k-space kz chunk on r (None:
-        the uploaded MD state); returns (w, kz chunk, parts, ms pair, ms pass 2)"""
-        choice = (C.c_int * 3)()
+        the uploaded MD state); returns (w, kz chunk, k parts, pair columns
+        per part, ms pair, ms pass 2)"""
+        choice = (C.c_int * 4)()
         ms = (C.c_double * 2)()
         ptr = None
         if r is not None:
@@ -109,7 +110,7 @@ class Engine:
             ptr = r.ctypes.data_as(C.POINTER(C.c_double))
         rc = self.lib.mdx_autotune(self.ctx, ptr, int(reps), choice, ms)
         abi.check(self.lib, rc, "mdx_autotune")
-        return choice[0], choice[1], choice[2], ms[0], ms[1]
+        return choice[0], choice[1], choice[2], choice[3], ms[0], ms[1]
 
     def set_timing(self, on=True):
         self.lib.mdx_set_timing(self.ctx, int(on))
@@ -124,8 +125,8 @@ class Engine:
         abi.check(self.lib, rc, "mdx_get_timing")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.FAMILIES)}
 
-    def set_stencil(self, w):
-        rc = self.lib.mdx_set_stencil(self.ctx, int(w))
+    def set_stencil(self, w, pcols=0):
+        rc = self.lib.mdx_set_stencil(self.ctx, int(w), int(pcols))
         abi.check(self.lib, rc, "mdx_set_stencil")
 
     # ---- device-resident MD
@@ -157,7 +158,7 @@ def declare_extras(lib):
     lib.mdx_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     lib.mdx_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
-    lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_set_kchunk.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]

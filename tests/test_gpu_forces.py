@@ -62,3 +62,25 @@ def test_cook_cell_membership(name):
     _, _, ocid = oracle.forces(oracle.fixture_system(meta), d["r"], cellid=True)
     assert (cid == ocid).all()
     eng.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+@pytest.mark.parametrize("w,pcols", [(1, 0), (1, 1), (2, 5), (2, 3), (2, 1), (3, 7)])
+def test_stencil_and_part_split(mode, w, pcols):
+    """every cell stencil width and split of the stencil columns over
+    parts gives the reference forces and the same pair census"""
+    d, meta = oracle.load_fixture("spce1000_md")
+    bs = oracle.fixture_system(meta, provider_=engine_provider())
+    eng = Engine(bs, mode)
+    f0, out0 = eng.forces(d["r"], measure=1)
+    eng.set_stencil(w, pcols)
+    info = eng.info()
+    print("w", w, "pcols", pcols, info["stencil"], info["ncx"], flush=True)
+    f, out = eng.forces(d["r"], measure=1)
+    assert info["stencil"] == w and info["pair_cols"] == pcols
+    assert rel_rms(f, d["f"]) < TOL[mode]
+    assert out.npair == out0.npair and out.nlj == out0.nlj
+    assert abs(out.En_pot + out.En_el - meta["Epot"]) < (1e-10 if mode == abi.MDX_FP64 else 2e-6) * abs(meta["Epot"])
+    for a, b in zip(out.Pvir, out0.Pvir):
+        assert abs(a - b) < 1e-9 * max(abs(x) for x in out0.Pvir[:3]) * (1 if mode == abi.MDX_FP64 else 1e3)
+    eng.close()
