@@ -307,13 +307,20 @@ __global__ __launch_bounds__(kBlock) void k_pair(
           if (nz < 0) { nz += g.nz; sz = -g.Lz; } else if (nz >= g.nz) { nz -= g.nz; sz = g.Lz; }
           const int nc = (nx * g.ny + ny) * g.nz + nz;
           const int jb = cstart[nc], je = cstart[nc + 1];
+          // the loads of the next neighbour are issued before the current
+          // one is used, so one gather is always in flight
+          double4 pn = spos[jb < je ? jb : s];
+          int4 in = sinfo[jb < je ? jb : s];
           for (int j = jb; j < je; j++) {
-            const double4 pj = spos[j];
+            const double4 pj = pn;
+            const int4 jj = in;
+            const int jn = j + 1 < je ? j + 1 : j;
+            pn = spos[jn];
+            in = sinfo[jn];
             const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy), dzr = pi.z - (pj.z + sz);
             const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
             if (rr >= cq || j == s) continue;
             npair += 1;
-            const int4 jj = sinfo[j];
             if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) continue;  // EXCL
             const int tj = jj.x & 0xffff;
             const R *ss = ljrow + tj * 8;
