@@ -35,7 +35,52 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kstride", type=int, default=0,
                     help="CPU baseline: k-space over every n-th charge (0 = auto)")
+    ap.add_argument("--pin", default="",
+                    help="W,PCOLS,KZC,KPARTS: stencil, stencil columns per pair part, k-space kz "
+                         "chunk and parts instead of the autotuner")
+    ap.add_argument("--forces-only", type=int, default=0,
+                    help="run this many force calls (no MD, no JSON line), for counter passes")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="with --forces-only: end with a 1 GiB device copy, the reference for "
+                         "scaling FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes=1 << 30):
+    """a plain device-to-device copy of nbytes through the HIP runtime the
+    engine is linked against (the runtime's copy kernel reads and writes
+    nbytes), the reference for scaling FETCH_SIZE / WRITE_SIZE.  No torch
+    here: torch brings its own HIP runtime into the process."""
+    hip = C.CDLL("libamdhip64.so.7")        # already loaded with libmdx.so
+    src, dst = C.c_void_p(), C.c_void_p()
+    size = C.c_size_t(nbytes)
+    for call in (lambda: hip.hipMalloc(C.byref(src), size), lambda: hip.hipMalloc(C.byref(dst), size),
+                 lambda: hip.hipMemset(src, 1, size), lambda: hip.hipMemcpy(dst, src, size, 3),
+                 lambda: hip.hipDeviceSynchronize()):
+        rc = call()
+        if rc:
+            raise RuntimeError("copy calibration: HIP error %d" % rc)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def traffic_lookup(workload, mode, kernel, info):
+    """HBM bytes per launch measured with rocprofv3 --pmc for this kernel
+    and configuration (profiles/pmc_traffic.json), or None"""
+    path = os.path.join(REPO, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        doc = json.load(fh)
+    for e in doc["kernels"]:
+        if (e["workload"], e["mode"], e["kernel"]) != (workload, mode, kernel):
+            continue
+        if kernel.startswith("kpass") and (e["kz_chunk"], e["k_parts"]) != (info["kchunk"], info["kparts_asked"]):
+            continue
+        if kernel == "pair" and (e["stencil_w"], e["pair_cols_per_part"]) != (info["stencil"], info["pair_cols"]):
+            continue
+        return e["bytes"]
+    return None
 
 
 def cpu_baseline(wl, bs_oracle, kstride, threads):
@@ -73,10 +118,25 @@ def main():
     bs = build_system(wl["ff"], wl["counts"], L, wl["cutoff"], epsr=0.05, epsk=0.05)
     mode = abi.MDX_MIXED if a.mode == "mixed" else abi.MDX_FP64
     eng = Engine(bs, mode)
-    # the engine's autotuner picks the cell stencil and the k-space force
-    # kernel's kz chunk and work split on this configuration (HIP events)
-    tune = eng.autotune(wl["r"], reps=2)
+    if a.pin:
+        w, pcols, kzc, kparts = (int(x) for x in a.pin.split(","))
+        eng.set_stencil(w, pcols)
+        eng.set_kchunk(kzc, kparts)
+        tune = (w, kzc, kparts, pcols)
+    else:
+        # the engine's autotuner picks the cell stencil, its split over
+        # pair-kernel parts and the k-space force kernel's kz chunk and work
+        # split on this configuration (HIP events)
+        tune = eng.autotune(wl["r"], reps=2)[:4]
     info = eng.info()
+    if a.forces_only:
+        for _ in range(a.forces_only):
+            f, out = eng.forces(wl["r"], measure=0)
+        print("forces-only: %d calls, pairs %d, tuned %s" % (a.forces_only, out.npair, tune), flush=True)
+        if a.copy_calibration:
+            copy_calibration()
+            print("copy-calibration: 1 GiB device copy done", flush=True)
+        return
     h = 0.002
     nf = workloads.degrees_of_freedom(wl["ff"], wl["counts"])
     p = abi.Step(h=h, eps=1e-7, maxit=0, thermostat=1, T=298.0, tau=0.1, nf=float(nf),
@@ -144,7 +204,8 @@ def main():
         "us_per_force": force_ms * 1e3,
         "kernel_ms": {k: v[0] for k, v in kern.items()},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": traffic_lookup(wl["name"], a.mode, dom, info),
                      "work_per_launch": work[dom]},
         "cpu_baseline": cpu,
         "energy_K": {"En_pot": out.frc.En_pot + out.frc.En_el, "En_kin": out.En_kin},

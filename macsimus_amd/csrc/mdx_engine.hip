@@ -73,20 +73,20 @@ __global__ void k_zero_int(int *p, int n) {
 }
 
 // final reduction of per-block partials; one block, ACC_N columns
+// final sums of the per-block partials: one workgroup per accumulator
+// (launched with ACC_N workgroups), fixed summation order
 __global__ void k_reduce(const double *part, int nblocks, double *acc) {
   __shared__ double red[kBlock];
-  for (int a = 0; a < ACC_N; a++) {
-    double s = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * ACC_N + a];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) acc[a] = red[0];
+  const int a = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * ACC_N + a];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
+  if (threadIdx.x == 0) acc[a] = red[0];
 }
 
 // --------------------------------------------------------------- binning
@@ -152,15 +152,36 @@ __global__ void k_scatter(const int *__restrict__ cellid, int ns, const int *__r
 
 // makes the order inside each cell ascending in site index (deterministic
 // sums), then gathers cell-ordered coordinates and site records
-__global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__restrict__ sidx) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int b = start[c], e = start[c + 1];
-  for (int i = b + 1; i < e; i++) {
-    int v = sidx[i], k = i - 1;
-    while (k >= b && sidx[k] > v) { sidx[k + 1] = sidx[k]; k--; }
-    sidx[k + 1] = v;
+// sites of each cell in increasing index (the scatter's atomics leave
+// them in arrival order), so that the results do not depend on it.  One
+// wave per cell: up to 64 sites by a bitonic network across the lanes,
+// larger cells by rank (count of smaller indices) through tmp.
+__global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__restrict__ sidx,
+                           int *__restrict__ tmp) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int b = 0, n = 0;
+  if (c < ncell) { b = start[c]; n = start[c + 1] - b; }
+  if (n > 1 && n <= 64) {
+    int v = lane < n ? sidx[b + lane] : 0x7fffffff;
+    for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int o = __shfl_xor(v, j, 64);
+        const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+        v = lower == up ? min(v, o) : max(v, o);
+      }
+    if (lane < n) sidx[b + lane] = v;
+  } else if (n > 64) {
+    for (int i = lane; i < n; i += 64) {
+      const int v = sidx[b + i];
+      int rank = 0;
+      for (int k = 0; k < n; k++) rank += sidx[b + k] < v;
+      tmp[b + rank] = v;
+    }
   }
+  __syncthreads();
+  if (n > 64)
+    for (int i = lane; i < n; i += 64) sidx[b + i] = tmp[b + i];
 }
 
 __global__ void k_gather(int ns, Grid g, const int *__restrict__ sidx, const double *__restrict__ r,
@@ -566,7 +587,7 @@ int bin_sites(Ctx &c, const double *d_r) {
   k_cellid<<<nblk(c.ns), kBlock, 0, c.stream>>>(d_r, c.ns, g, c.cellid.p, c.cnt.p);
   k_scan<<<1, 1024, 0, c.stream>>>(c.cnt.p, ncell, c.cstart.p, c.cursor.p);
   k_scatter<<<nblk(c.ns), kBlock, 0, c.stream>>>(c.cellid.p, c.ns, c.cstart.p, c.cursor.p, c.sidx.p);
-  k_cellsort<<<nblk(ncell), kBlock, 0, c.stream>>>(ncell, c.cstart.p, c.sidx.p);
+  k_cellsort<<<nblk(64L * ncell), kBlock, 0, c.stream>>>(ncell, c.cstart.p, c.sidx.p, c.scan_tmp.p);
   k_gather<<<nblk(c.ns), kBlock, 0, c.stream>>>(
       c.ns, g, c.sidx.p, d_r, c.charge.p, c.polar ? c.chargepol.p : nullptr, c.type.p, c.mol.p,
       c.molfirst.p, c.excl.p, c.cellid.p, (double4 *)c.spos.p, (double4 *)c.spol.p,
@@ -645,7 +666,7 @@ int forces_device(Ctx &c, int measure) {
   MDX_HIP(hipGetLastError());
   if (c.timing) c.tm.stop(T_EXCL, c.stream);
   if (c.nq && (rc = kspace_forces(c, measure))) return rc;
-  k_reduce<<<1, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, c.acc.p);
+  k_reduce<<<ACC_N, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, c.acc.p);
   MDX_HIP(hipGetLastError());
   return 0;
 }

@@ -283,20 +283,20 @@ __global__ __launch_bounds__(kBlock) void k_dipole(int nq, const double4 *__rest
   }
 }
 
-__global__ void k_reduce_cols(const double *part, int nblocks, int a0, int a1, double *acc) {
+// sums of the accumulators a0 + blockIdx.x over the first nblocks partial
+// rows (one workgroup per accumulator, fixed order)
+__global__ void k_reduce_cols(const double *part, int nblocks, int a0, double *acc) {
   __shared__ double red[kBlock];
-  for (int a = a0; a < a1; a++) {
-    double s = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * mdx::ACC_N + a];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) acc[a] = red[0];
+  const int a = a0 + blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * mdx::ACC_N + a];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
+  if (threadIdx.x == 0) acc[a] = red[0];
 }
 
 // ------------------------------------------------------------------ pass 2
@@ -446,7 +446,8 @@ int run_kspace(mdx::Ctx &c, int measure) {
   const double V = c.L[0] * c.L[1] * c.L[2];
   const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
   k_dipole<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, qpos, c.part.p);
-  k_reduce_cols<<<1, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, ACC_M0, ACC_M0 + 3, c.acc.p);
+  // the dipole sums of k_dipole's workgroups
+  k_reduce_cols<<<3, kBlock, 0, c.stream>>>(c.part.p, nblk(c.nq), ACC_M0, c.acc.p);
   if (c.timing) { c.tm.stop(T_KCOEF, c.stream); c.tm.start(T_KPASS2, c.stream); }
 #define MDX_KF(KZV)                                                                         \
   k_kforce<R, KZV><<<dim3(nblk(c.nq), c.nkpart), kBlock, 0, c.stream>>>(                    \

@@ -84,3 +84,21 @@ def test_stencil_and_part_split(mode, w, pcols):
     for a, b in zip(out.Pvir, out0.Pvir):
         assert abs(a - b) < 1e-9 * max(abs(x) for x in out0.Pvir[:3]) * (1 if mode == abi.MDX_FP64 else 1e3)
     eng.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_forces_are_deterministic(mode):
+    """the cell sort orders each cell's sites by index and every sum has a
+    fixed order, so two calls give the same bits; the w=1 grid puts more
+    than 64 sites in a cell (the rank-sort branch)"""
+    d, meta = oracle.load_fixture("spce1000_md")
+    bs = oracle.fixture_system(meta, provider_=engine_provider())
+    eng = Engine(bs, mode)
+    for w in (2, 1):
+        eng.set_stencil(w, 0)
+        f1, o1 = eng.forces(d["r"], measure=1)
+        f2, o2 = eng.forces(d["r"], measure=1)
+        assert np.array_equal(f1, f2)
+        assert o1.En_el == o2.En_el and o1.En_pot == o2.En_pot and list(o1.Pvir) == list(o2.Pvir)
+        assert rel_rms(f1, d["f"]) < TOL[mode]
+    eng.close()
