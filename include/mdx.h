@@ -226,6 +226,29 @@ int mdx_set_kchunk(mdx_ctx_t *ctx, int kzc, int parts);
 int mdx_autotune(mdx_ctx_t *ctx, const double *r, int reps, int *choice, double *ms);
 int mdx_info(mdx_ctx_t *ctx, int *iv, int n);
 
+/* ---- pressure assembly and cutoff correction (host arithmetic) -------- */
+
+/* measurePconstraints() (sim/constrd.c:115-265) for PRESSURETENSOR=7: all
+   in program units (pressure x Punit = Pa; sim/units.h). */
+typedef struct mdx_pressure_s {
+  double vir;          /* En.vir: LJ virial + constraint virial + En.el
+                          (- 2 En.self with Drudes) */
+  double Pvir[6];      /* (Pvir + PTvirc)/V, xx yy zz yz zx xy */
+  double Pkin[6];      /* Pkin * Pkinq / V */
+  double Ptens[6];     /* Pvir + Pkin */
+  double Ptr, Ptr_c;   /* trace(Ptens)/3, + Ecorr/V^2 */
+  double Pevir, Pevir_c; /* (2 Ekin Pkinq + vir)/(3V), + Ecorr/V^2 */
+} mdx_pressure_t;
+/* so: outputs of a measured step (a force call alone: frc filled, the
+   rest zero); Pkinq: cook's No.Pkinq (1, or No.f0/No.f with corr&4);
+   Ecorr: from mdx_cutcorr; polar: the system has Drudes. */
+int mdx_pressure(const mdx_step_out_t *so, const double L[3], double Pkinq,
+                 double Ecorr, int polar, mdx_pressure_t *p);
+/* cutcorr() (sim/siminit.c:665-722): En.corr = sum over intermolecular
+   site pairs of corr(type_i,type_j), corr [ntypes*ntypes] from mdx_setss.
+   The energy correction is Ecorr/V, the pressure correction Ecorr/V^2. */
+int mdx_cutcorr(const mdx_system_t *sys, const double *corr, double *Ecorr);
+
 /* ---- set-up helpers restating cook's init code (product, not oracle) -- */
 
 /* combining rule (sim/lj/sitesite.c:66-112) + smooth cutoff (sim/setss.c:

@@ -5,6 +5,8 @@ macsimus_amd/csrc/Makefile (hipcc, gfx950).  There is no CPU fallback: if
 the library is missing, `load_engine()` raises.
 """
 import ctypes as C
+
+import numpy as np
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -79,6 +81,14 @@ class StepOut(C.Structure):
     ]
 
 
+class Pressure(C.Structure):
+    _fields_ = [
+        ("vir", C.c_double), ("Pvir", C.c_double * 6), ("Pkin", C.c_double * 6),
+        ("Ptens", C.c_double * 6), ("Ptr", C.c_double), ("Ptr_c", C.c_double),
+        ("Pevir", C.c_double), ("Pevir_c", C.c_double),
+    ]
+
+
 def declare_setup(lib, prefix):
     """Signatures of the set-up helpers (same in engine and oracle)."""
     f = getattr(lib, prefix + "setss")
@@ -120,6 +130,9 @@ def load_engine(path=None):
     lib.mdx_forces.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(Out), C.c_int]
     lib.mdx_set_box.argtypes = [C.c_void_p, C.c_double * 3]
     lib.mdx_selffield.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(Out), C.c_double, _dp, _dp]
+    lib.mdx_pressure.argtypes = [C.POINTER(StepOut), C.c_double * 3, C.c_double, C.c_double,
+                                 C.c_int, C.POINTER(Pressure)]
+    lib.mdx_cutcorr.argtypes = [C.POINTER(System), _dp, _dp]
     lib.mdx_upload_state.argtypes = [C.c_void_p, _dp, _dp]
     lib.mdx_download_state.argtypes = [C.c_void_p, _dp, _dp]
     lib.mdx_md_steps.argtypes = [C.c_void_p, C.POINTER(Step), C.c_int,
@@ -137,3 +150,22 @@ def check(lib, rc, what):
         msg = lib.mdx_last_error() if hasattr(lib, "mdx_last_error") else b""
         raise RuntimeError("%s failed: %s (%s)" % (
             what, ERRORS.get(rc, rc), (msg or b"").decode()))
+
+
+def pressure(lib, step_out, L, Pkinq=1.0, Ecorr=0.0, polar=False):
+    """measurePconstraints() of the reference on one measured step's
+    outputs (program units)"""
+    p = Pressure()
+    check(lib, lib.mdx_pressure(C.byref(step_out), (C.c_double * 3)(*L), Pkinq, Ecorr,
+                                int(polar), C.byref(p)), "mdx_pressure")
+    return p
+
+
+def cutcorr(lib, built_system):
+    """cutcorr(): En.corr of the system from the per-type-pair corrections
+    of mdx_setss"""
+    corr = np.ascontiguousarray(built_system.info["corr"], dtype=np.float64)
+    e = C.c_double(0)
+    check(lib, lib.mdx_cutcorr(C.byref(built_system.struct),
+                               corr.ctypes.data_as(C.POINTER(C.c_double)), C.byref(e)), "mdx_cutcorr")
+    return e.value

@@ -244,3 +244,60 @@ extern "C" int mdx_lcsetup(int ncell[3], int ns, double percell, const double L[
   }
   return 0;
 }
+
+// cutcorr() (sim/siminit.c:665-722): the homogeneous LJ cutoff correction
+// summed over all intermolecular site pairs, En.corr = sum_{molecule pairs
+// a<b} sum_{i in a, j in b} corr(type_i, type_j).  Written over site-type
+// counts: 1/2 (N.C.N - sum_a n_a.C.n_a) with N the counts of the system and
+// n_a those of molecule a; the same sum as the reference's species loop
+// with N(N-1)/2 pairs of equal species.  corr: [ntypes*ntypes] from
+// mdx_setss.  Energy correction Ecorr/V, pressure correction Ecorr/V^2.
+extern "C" int mdx_cutcorr(const mdx_system_t *s, const double *corr, double *Ecorr) {
+  if (!s || !corr || !Ecorr || !s->mol_first || !s->site_type || s->ntypes <= 0)
+    return mdx::fail(MDX_E_ARG, "mdx_cutcorr: NULL argument");
+  const int nt = s->ntypes;
+  std::vector<double> N(nt, 0.0), n(nt, 0.0);
+  for (int i = 0; i < s->ns; i++) {
+    const int t = s->site_type[i];
+    if (t < 0 || t >= nt) return mdx::fail(MDX_E_ARG, "mdx_cutcorr: site type out of range");
+    N[t] += 1;
+  }
+  auto quad = [&](const std::vector<double> &v) {
+    double q = 0;
+    for (int a = 0; a < nt; a++)
+      for (int b = 0; b < nt; b++) q += v[a] * corr[a * nt + b] * v[b];
+    return q;
+  };
+  double intra = 0;
+  for (int m = 0; m < s->nmol; m++) {
+    std::fill(n.begin(), n.end(), 0.0);
+    for (int i = s->mol_first[m]; i < s->mol_first[m + 1]; i++) n[s->site_type[i]] += 1;
+    intra += quad(n);
+  }
+  *Ecorr = 0.5 * (quad(N) - intra);
+  return 0;
+}
+
+// measurePconstraints() (sim/constrd.c:115-265, PRESSURETENSOR=7, no ECC,
+// no slab): the virial sum and the pressure tensors of one measured step.
+extern "C" int mdx_pressure(const mdx_step_out_t *so, const double L[3], double Pkinq,
+                            double Ecorr, int polar, mdx_pressure_t *p) {
+  if (!so || !L || !p) return mdx::fail(MDX_E_ARG, "mdx_pressure: NULL argument");
+  const double V = L[0] * L[1] * L[2];
+  if (!(V > 0)) return mdx::fail(MDX_E_ARG, "mdx_pressure: box volume must be > 0");
+  std::memset(p, 0, sizeof(*p));
+  // En.vir += En.virc + En.el (- 2 En.self with Drudes)
+  p->vir = so->frc.En_vir + so->virc + so->frc.En_el - (polar ? 2 * so->En_self : 0.0);
+  for (int i = 0; i < 6; i++) {
+    p->Pkin[i] = so->Pkin[i] * Pkinq / V;
+    p->Pvir[i] = (so->frc.Pvir[i] + so->PTvirc[i]) / V;
+    p->Ptens[i] = p->Pvir[i] + p->Pkin[i];
+  }
+  const double cP = Ecorr / (V * V);
+  p->Ptr = (p->Ptens[0] + p->Ptens[1] + p->Ptens[2]) / 3;
+  p->Ptr_c = p->Ptr + cP;
+  // En.kin is the doubled kinetic energy at this point of the reference
+  p->Pevir = (2 * so->En_kin * Pkinq + p->vir) / 3 / V;
+  p->Pevir_c = p->Pevir + cP;
+  return 0;
+}
