@@ -206,7 +206,9 @@ int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
    mdx_set_stencil: cell-stencil half width w (1..6) of the pair kernel
    (a wider one is taken where the box needs it) and the split of its
    (2w+1)^2 stencil columns into parts of pcols columns (0: one part),
-   which sets the waves in flight.
+   which sets the waves in flight.  w = 0 selects the all-pair path of
+   cook/pair.c (minimum image per site pair, for small systems and as a
+   cross-check) with pcols slices of the neighbour range (0: automatic).
    mdx_set_kchunk: kz chunk (8..64, multiple of 8) of the k-space force
    kernel, which sets its register footprint, and the number of parts
    (1..256) its k-set is split into, which sets the waves in flight.

@@ -290,6 +290,74 @@ __device__ inline void qq_term(const R *erf, double sgrid, R qq, const double (&
 // distance; the Drude of a site sits at r+rpol with charge qpol; the
 // central site collects the forces on its centre (cc, cp) and on its
 // Drude (pc, pp); LJ acts between centres only.
+// force terms of one pair inside the cutoff (separation d in FP64, rr <
+// cutoff^2, not the same site), added to the accumulators of the central
+// site: LJ with the smooth cutoff (cook/ljqqnp.c:147-206, sim/setss.c),
+// erfc Coulomb from the spline, and for Drudes the three extra charge
+// terms (cook/ljqqpol.c).  EXCL pairs of the same molecule are skipped
+// here and corrected in FP64 by k_excl (cook/ljqqnp.c:104-144).
+template <typename R, bool MEASURE, bool POLAR>
+__device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, const double4 ri,
+                                 const PairTab<R> &tab, double sgrid, const double4 pj,
+                                 const int4 jj, const double4 rj, double dxr, double dyr,
+                                 double dzr, double rr, R (&ac)[3], R (&ap)[3], double &U,
+                                 double &Eel, double &Vir, double (&pv)[6], double &nlj) {
+  if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) return;  // EXCL
+  const int tj = jj.x & 0xffff;
+  const R *ss = ljrow + tj * 8;
+  R fr = 0, u = 0;
+  if (rr < (double)ss[5]) {
+    if (rr < (double)ss[4]) {
+      const R x = ss[0] * rcp<R>((R)rr), y = x * x * x;
+      if (MEASURE) {
+        const R z = y * y;
+        u = ss[1] * (z - y);
+        fr = ss[2] * (z + z - y) * x;
+      } else {
+        fr = ss[3] * y * (y - (R)0.5) * x;
+      }
+    } else {
+      const R x = (R)(rr - (double)ss[5]);
+      if (MEASURE) u = x * x * ss[6];
+      fr = ss[7] * x;
+    }
+    if (MEASURE) Vir += rr * (double)fr;
+    nlj += ss[6] != 0;
+  }
+  const R qq = qi * (R)pj.w;
+  if (qq != 0) {
+    R eru = 0, erd;
+    spline<R>(tab.erf, rr, sgrid, eru, erd, MEASURE);
+    fr += qq * erd;
+    if (MEASURE) Eel += (double)(qq * eru);
+  }
+  const R ddx = (R)dxr, ddy = (R)dyr, ddz = (R)dzr;
+  const R fdx = fr * ddx, fdy = fr * ddy, fdz = fr * ddz;
+  ac[0] += fdx; ac[1] += fdy; ac[2] += fdz;
+  if (MEASURE) {
+    U += u;
+    pv[0] += (double)(ddx * fdx); pv[1] += (double)(ddy * fdy); pv[2] += (double)(ddz * fdz);
+    pv[3] += (double)(ddz * fdy); pv[4] += (double)(ddx * fdz); pv[5] += (double)(ddy * fdx);
+  }
+  if (POLAR) {
+    const R qpj = (R)rj.w, qj = (R)pj.w;
+    if (qpi != 0 && qj != 0) {        // Drude i - centre j
+      const double d[3] = {dxr + ri.x, dyr + ri.y, dzr + ri.z};
+      qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qj, d, ap, Eel, pv);
+    }
+    if (qpj != 0) {
+      if (qi != 0) {                  // centre i - Drude j
+        const double d[3] = {dxr - rj.x, dyr - rj.y, dzr - rj.z};
+        qq_term<R, MEASURE>(tab.erf, sgrid, qi * qpj, d, ac, Eel, pv);
+      }
+      if (qpi != 0) {                 // Drude i - Drude j
+        const double d[3] = {dxr - rj.x + ri.x, dyr - rj.y + ri.y, dzr - rj.z + ri.z};
+        qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qpj, d, ap, Eel, pv);
+      }
+    }
+  }
+}
+
 template <typename R, bool MEASURE, bool POLAR>
 __global__ __launch_bounds__(kBlock) void k_pair(
     int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
@@ -342,61 +410,10 @@ __global__ __launch_bounds__(kBlock) void k_pair(
             const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
             if (rr >= cq || j == s) continue;
             npair += 1;
-            if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) continue;  // EXCL
-            const int tj = jj.x & 0xffff;
-            const R *ss = ljrow + tj * 8;
-            R fr = 0, u = 0;
-            if (rr < (double)ss[5]) {
-              if (rr < (double)ss[4]) {
-                const R x = ss[0] * rcp<R>((R)rr), y = x * x * x;
-                if (MEASURE) {
-                  const R z = y * y;
-                  u = ss[1] * (z - y);
-                  fr = ss[2] * (z + z - y) * x;
-                } else {
-                  fr = ss[3] * y * (y - (R)0.5) * x;
-                }
-              } else {
-                const R x = (R)(rr - (double)ss[5]);
-                if (MEASURE) u = x * x * ss[6];
-                fr = ss[7] * x;
-              }
-              if (MEASURE) Vir += rr * (double)fr;
-              nlj += ss[6] != 0;
-            }
-            const R qq = qi * (R)pj.w;
-            if (qq != 0) {
-              R eru = 0, erd;
-              spline<R>(tab.erf, rr, sgrid, eru, erd, MEASURE);
-              fr += qq * erd;
-              if (MEASURE) Eel += (double)(qq * eru);
-            }
-            const R ddx = (R)dxr, ddy = (R)dyr, ddz = (R)dzr;
-            const R fdx = fr * ddx, fdy = fr * ddy, fdz = fr * ddz;
-            ac[0] += fdx; ac[1] += fdy; ac[2] += fdz;
-            if (MEASURE) {
-              U += u;
-              pv[0] += (double)(ddx * fdx); pv[1] += (double)(ddy * fdy); pv[2] += (double)(ddz * fdz);
-              pv[3] += (double)(ddz * fdy); pv[4] += (double)(ddx * fdz); pv[5] += (double)(ddy * fdx);
-            }
-            if (POLAR) {
-              const double4 rj = spol[j];
-              const R qpj = (R)rj.w, qj = (R)pj.w;
-              if (qpi != 0 && qj != 0) {        // Drude i - centre j
-                const double d[3] = {dxr + ri.x, dyr + ri.y, dzr + ri.z};
-                qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qj, d, ap, Eel, pv);
-              }
-              if (qpj != 0) {
-                if (qi != 0) {                  // centre i - Drude j
-                  const double d[3] = {dxr - rj.x, dyr - rj.y, dzr - rj.z};
-                  qq_term<R, MEASURE>(tab.erf, sgrid, qi * qpj, d, ac, Eel, pv);
-                }
-                if (qpi != 0) {                 // Drude i - Drude j
-                  const double d[3] = {dxr - rj.x + ri.x, dyr - rj.y + ri.y, dzr - rj.z + ri.z};
-                  qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qpj, d, ap, Eel, pv);
-                }
-              }
-            }
+            double4 rj = make_double4(0, 0, 0, 0);
+            if (POLAR) rj = spol[j];
+            pair_body<R, MEASURE, POLAR>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr, dyr,
+                                         dzr, rr, ac, ap, U, Eel, Vir, pv, nlj);
           }
         }
         fx += ac[0]; fy += ac[1]; fz += ac[2];
@@ -405,6 +422,78 @@ __global__ __launch_bounds__(kBlock) void k_pair(
     }
     // one part: the force rows themselves; several: partial rows
     // [part][nrow][3], summed in order by k_pair_sum
+    const int site = ii.w;
+    double *o = f + (size_t)blockIdx.y * (POLAR ? 2 : 1) * ns * 3;
+    o[3 * site] = fx;
+    o[3 * site + 1] = fy;
+    o[3 * site + 2] = fz;
+    if (POLAR) {
+      o[3 * (ns + site)] = px;
+      o[3 * (ns + site) + 1] = py;
+      o[3 * (ns + site) + 2] = pz;
+    }
+  }
+  const size_t row = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (MEASURE) {
+    double v[12] = {0.5 * U, 0.5 * Eel, 0.5 * Vir, 0.5 * pv[0], 0.5 * pv[1], 0.5 * pv[2],
+                    0.5 * pv[3], 0.5 * pv[4], 0.5 * pv[5], 0.5 * npair, 0.5 * nlj, 0};
+    const int idx[12] = {ACC_U, ACC_EEL, ACC_VIR, ACC_PV0, ACC_PV0 + 1, ACC_PV0 + 2,
+                         ACC_PV0 + 3, ACC_PV0 + 4, ACC_PV0 + 5, ACC_NPAIR, ACC_NLJ, ACC_SUMQ};
+    block_store<12>(v, idx, part, row);
+  } else {
+    double v[2] = {0.5 * npair, 0.5 * nlj};
+    const int idx[2] = {ACC_NPAIR, ACC_NLJ};
+    block_store<2>(v, idx, part, row);
+  }
+}
+
+// All-pair alternative (cook/pair.c:59-153 with the per-site minimum image
+// of the intermolecular X macros, cook/simpot.c:74-259): lane <-> site i,
+// the wave walks the sites j of its part [j0, j1) in step (scalar loads),
+// separation by the FP64 minimum image; same pair terms, census and
+// output layout as k_pair.  For small systems and as a cross-check of the
+// cell path.
+template <typename R, bool MEASURE, bool POLAR>
+__global__ __launch_bounds__(kBlock) void k_pair_all(
+    int ns, Grid g, int jper, double cq, const double4 *__restrict__ spos,
+    const double4 *__restrict__ spol, const int4 *__restrict__ sinfo, int nt, PairTab<R> tab,
+    double sgrid, double *__restrict__ f, double *__restrict__ part) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  const int j0 = blockIdx.y * jper, j1 = min(ns, j0 + jper);
+  double fx = 0, fy = 0, fz = 0, px = 0, py = 0, pz = 0;
+  double U = 0, Eel = 0, Vir = 0, pv[6] = {0, 0, 0, 0, 0, 0};
+  double npair = 0, nlj = 0;
+  const bool act = s < ns;
+  const int sl = act ? s : ns - 1;
+  const double4 pi = spos[sl];
+  double4 ri = make_double4(0, 0, 0, 0);
+  if (POLAR) ri = spol[sl];
+  const int4 ii = sinfo[sl];
+  const R qi = (R)pi.w, qpi = (R)ri.w;
+  const R *ljrow = tab.lj + (size_t)(ii.x & 0xffff) * nt * 8;
+  const double iLx = 1 / g.Lx, iLy = 1 / g.Ly, iLz = 1 / g.Lz;
+  for (int jb = j0; jb < j1; jb += 64) {
+    // R sums over 64 neighbours at most, then FP64
+    R ac[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
+    for (int j = jb; j < min(jb + 64, j1); j++) {
+      const double4 pj = spos[j];
+      double dxr = pi.x - pj.x, dyr = pi.y - pj.y, dzr = pi.z - pj.z;
+      dxr -= g.Lx * rint(dxr * iLx);
+      dyr -= g.Ly * rint(dyr * iLy);
+      dzr -= g.Lz * rint(dzr * iLz);
+      const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+      if (!act || rr >= cq || j == s) continue;
+      npair += 1;
+      const int4 jj = sinfo[j];
+      double4 rj = make_double4(0, 0, 0, 0);
+      if (POLAR) rj = spol[j];
+      pair_body<R, MEASURE, POLAR>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr, dyr,
+                                   dzr, rr, ac, ap, U, Eel, Vir, pv, nlj);
+    }
+    fx += ac[0]; fy += ac[1]; fz += ac[2];
+    if (POLAR) { px += ap[0]; py += ap[1]; pz += ap[2]; }
+  }
+  if (act) {
     const int site = ii.w;
     double *o = f + (size_t)blockIdx.y * (POLAR ? 2 : 1) * ns * 3;
     o[3 * site] = fx;
@@ -566,14 +655,22 @@ int choose_grid(Ctx &c, int w) {
   return fail(MDX_E_ARG, "box too small for the cutoff (need L >= (2+1/6)*cutoff)");
 }
 
+// parts of the pair kernel: groups of pcols stencil columns, or for the
+// all-pair path slices of the neighbour range (pcols, 0: about 256 sites each)
+int pair_parts(const Ctx &c) {
+  if (c.allpair)
+    return c.pcols > 0 ? std::min(c.pcols, std::max(c.ns, 1)) : std::max(1, std::min(64, c.ns / 256));
+  const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
+  const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
+  return (ncol + cpp - 1) / cpp;
+}
+
 int alloc_grid(Ctx &c) {
   const int ncell = c.nc[0] * c.nc[1] * c.nc[2];
   int rc;
   if ((rc = c.cnt.alloc(ncell)) || (rc = c.cstart.alloc(ncell + 1)) || (rc = c.cursor.alloc(ncell)))
     return rc;
-  const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
-  const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
-  const int np = (ncol + cpp - 1) / cpp;
+  const int np = pair_parts(c);
   c.part_blocks = std::max(nblk(c.ns) * np, nblk(c.nmol));
   c.part_blocks = std::max(c.part_blocks, 1024);
   if (np > 1 && (rc = c.pf.alloc((size_t)np * 3 * (c.polar ? 2 : 1) * c.ns))) return rc;
@@ -603,21 +700,33 @@ int launch_pair_t(Ctx &c, int measure) {
   if constexpr (sizeof(R) == 4) { tab.lj = c.lj32.p; tab.erf = c.erf32.p; }
   else { tab.lj = (const R *)c.lj64.p; tab.erf = (const R *)c.erf64.p; }
   const double cq = c.cutoff * c.cutoff;
-  const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
-  const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
-  const int np = (ncol + cpp - 1) / cpp;
+  const int np = pair_parts(c);
   const dim3 grid(nblk(c.ns), np);
   double *out = np > 1 ? c.pf.p : c.f.p;
-  if (measure)
-    k_pair<R, true, POLAR><<<grid, kBlock, 0, c.stream>>>(
-        c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
-        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-        c.part.p);
-  else
-    k_pair<R, false, POLAR><<<grid, kBlock, 0, c.stream>>>(
-        c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
-        (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-        c.part.p);
+  if (c.allpair) {
+    const int jper = (c.ns + np - 1) / np;
+    if (measure)
+      k_pair_all<R, true, POLAR><<<grid, kBlock, 0, c.stream>>>(
+          c.ns, g, jper, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+          (const int4 *)c.sinfo.p, c.ntypes, tab, c.erf_sgrid, out, c.part.p);
+    else
+      k_pair_all<R, false, POLAR><<<grid, kBlock, 0, c.stream>>>(
+          c.ns, g, jper, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+          (const int4 *)c.sinfo.p, c.ntypes, tab, c.erf_sgrid, out, c.part.p);
+  } else {
+    const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
+    const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
+    if (measure)
+      k_pair<R, true, POLAR><<<grid, kBlock, 0, c.stream>>>(
+          c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+          (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
+          c.part.p);
+    else
+      k_pair<R, false, POLAR><<<grid, kBlock, 0, c.stream>>>(
+          c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+          (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
+          c.part.p);
+  }
   if (np > 1) {
     const long n3 = 3L * (POLAR ? 2 : 1) * c.ns;
     k_pair_sum<<<nblk(n3), kBlock, 0, c.stream>>>(n3, np, c.pf.p, c.f.p);
@@ -960,12 +1069,14 @@ extern "C" int mdx_get_timing(mdx_ctx_t *h, double *ms, int *count, int n) {
 // engine cell grid (stencil half width w) and the pair kernel's split of
 // the stencil into parts of pcols (dx,dy) columns (0: one part)
 extern "C" int mdx_set_stencil(mdx_ctx_t *h, int w, int pcols) {
-  if (!h || w < 1 || w > kMaxStencil || pcols < 0)
-    return fail(MDX_E_ARG, "mdx_set_stencil: w in 1..6, pcols >= 0");
+  if (!h || w < 0 || w > kMaxStencil || pcols < 0)
+    return fail(MDX_E_ARG, "mdx_set_stencil: w in 0..6, pcols >= 0");
   Ctx &c = h->c;
   MDX_HIP(hipSetDevice(c.device));
   int rc;
   c.pcols = pcols;
+  c.allpair = w == 0;
+  if (c.allpair) return alloc_grid(c);   // the cell grid stays for the binning
   if ((rc = choose_grid(c, w)) || (rc = alloc_grid(c))) return rc;
   return c.wst == w ? 0 : fail(MDX_E_ARG, "mdx_set_stencil: box too small for this w");
 }
@@ -1015,6 +1126,7 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
     for (int f = 0; f < T_NFAM; f++) fam[f] = t[f] / reps;
     return 0;
   };
+  c.allpair = false;
   int rc = 0, best_w = c.wst, best_pc = c.pcols, best_k = c.kzc, best_p = c.kparts;
   double tw = 1e30, tk = 1e30, fam[T_NFAM];
   for (int w = 1; w <= 3 && !rc; w++) {
@@ -1052,7 +1164,7 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
 extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
   Ctx &c = h->c;
-  int v[17] = {c.nc[0], c.nc[1], c.nc[2], c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+  int v[17] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
                c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart,
                c.pcols};
   for (int i = 0; i < n && i < 17; i++) iv[i] = v[i];

@@ -158,6 +158,9 @@ struct Ctx {
   // SIMD; partial force rows [part][nrow][3] are summed in a fixed order
   int pcols = 0;
   DBuf<double> pf;
+  // all-pair path (cook/pair.c) instead of the cell stencil; its parts
+  // are slices of the neighbour range (pcols of them, 0: automatic)
+  bool allpair = false;
   DBuf<int> cellid, cnt, cstart, cursor, sidx;
   DBuf<double> spos;           // [4ns] x,y,z wrapped, q  (cell order)
   DBuf<int> sinfo;             // [4ns] type, mol, excl, site

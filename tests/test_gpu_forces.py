@@ -102,3 +102,24 @@ def test_forces_are_deterministic(mode):
         assert o1.En_el == o2.En_el and o1.En_pot == o2.En_pot and list(o1.Pvir) == list(o2.Pvir)
         assert rel_rms(f1, d["f"]) < TOL[mode]
     eng.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+@pytest.mark.parametrize("name,parts", [("spce1000_md", 0), ("brine506", 3), ("polar306", 0)])
+def test_all_pair_path(name, parts, mode):
+    """the all-pair path (cook/pair.c, minimum image per site pair) gives
+    the reference forces and the same census as the cell path"""
+    d, meta = oracle.load_fixture(name)
+    bs = oracle.fixture_system(meta, provider_=engine_provider())
+    eng = Engine(bs, mode)
+    r = d["r"] if name != "polar306" else np.concatenate([d["r"], d["rpol"][0]])
+    fr = d["f"] if name != "polar306" else np.concatenate([d["f"][0], d["fpol"][0]])
+    f0, o0 = eng.forces(r, measure=1)
+    eng.set_stencil(0, parts)
+    assert eng.info()["stencil"] == 0
+    f1, o1 = eng.forces(r, measure=1)
+    print(name, "all-pair rel err", rel_rms(f1, fr), "cells", rel_rms(f0, fr), flush=True)
+    assert rel_rms(f1, fr) < TOL[mode]
+    assert o1.npair == o0.npair and o1.nlj == o0.nlj
+    assert abs((o1.En_el + o1.En_pot) - (o0.En_el + o0.En_pot)) < 1e-9 * abs(o0.En_el + o0.En_pot) * (1 if mode == abi.MDX_FP64 else 1e3)
+    eng.close()
