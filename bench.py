@@ -31,7 +31,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c3"], default="c2",
+                    help="c2: 194,304-site SPC/E+NaCl; c1: 1000 SPC/E; c3: Drude-polarizable "
+                         "brine, 97,200 sites + 32,600 Drudes, ASPC k=4 with one SCF iteration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kstride", type=int, default=0,
                     help="CPU baseline: k-space over every n-th charge (0 = auto)")
@@ -92,13 +94,13 @@ def cpu_baseline(wl, bs_oracle, kstride, threads):
     lib = oracle.load()
     lib.oracle_time_sample.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_int,
                                        C.POINTER(C.c_double)]
-    r = np.ascontiguousarray(wl["r"])
+    r = np.ascontiguousarray(wl["r_full"])
     t = (C.c_double * 3)()
     rc = lib.oracle_time_sample(C.byref(bs_oracle.struct), r.ctypes.data_as(C.POINTER(C.c_double)),
                                 threads, kstride, t)
     if rc:
         raise RuntimeError("oracle_time_sample failed (%d)" % rc)
-    nq = int(np.count_nonzero(bs_oracle.info["charge"]))
+    nq = int(np.count_nonzero(bs_oracle.info["charge"]) + np.count_nonzero(bs_oracle.info["chargepol"]))
     t_force = t[0] + t[1] * nq / t[2]
     return t_force, dict(r_space_s=t[0], k_space_sample_s=t[1], k_charges=int(t[2]), nq=nq)
 
@@ -113,7 +115,10 @@ def main():
     from macsimus_amd.engine import Engine
     from macsimus_amd.system import build_system
 
-    wl = workloads.brine_c2() if a.workload == "c2" else workloads.water_c1()
+    wl = {"c2": workloads.brine_c2, "c1": workloads.water_c1, "c3": workloads.polar_c3}[a.workload]()
+    polar = bool(wl.get("polar"))
+    # Drude systems: the configuration has the Drude offsets after the sites
+    wl["r_full"] = np.concatenate([wl["r"], np.zeros_like(wl["r"])]) if polar else wl["r"]
     L = [wl["L"]] * 3
     bs = build_system(wl["ff"], wl["counts"], L, wl["cutoff"], epsr=0.05, epsk=0.05)
     mode = abi.MDX_MIXED if a.mode == "mixed" else abi.MDX_FP64
@@ -127,11 +132,11 @@ def main():
         # the engine's autotuner picks the cell stencil, its split over
         # pair-kernel parts and the k-space force kernel's kz chunk and work
         # split on this configuration (HIP events)
-        tune = eng.autotune(wl["r"], reps=2)[:4]
+        tune = eng.autotune(wl["r_full"], reps=2)[:4]
     info = eng.info()
     if a.forces_only:
         for _ in range(a.forces_only):
-            f, out = eng.forces(wl["r"], measure=0)
+            f, out = eng.forces(wl["r_full"], measure=0)
         print("forces-only: %d calls, pairs %d, tuned %s" % (a.forces_only, out.npair, tune), flush=True)
         if a.copy_calibration:
             copy_calibration()
@@ -140,12 +145,11 @@ def main():
     h = 0.002
     nf = workloads.degrees_of_freedom(wl["ff"], wl["counts"])
     p = abi.Step(h=h, eps=1e-7, maxit=0, thermostat=1, T=298.0, tau=0.1, nf=float(nf),
-                 measure_every=0)
-    r1 = np.zeros_like(wl["r"])
-    eng.upload_state(wl["r"], r1)
+                 measure_every=0, aspc_k=4 if polar else 0, scf_iter=1, scf_omega=0.0)
+    r1 = np.zeros_like(wl["r_full"])
     # census of the starting configuration (pairs inside the cutoff)
-    f0, out0 = eng.forces(wl["r"], measure=1)
-    eng.upload_state(wl["r"], r1)
+    f0, out0 = eng.forces(wl["r_full"], measure=1)
+    eng.upload_state(wl["r_full"], r1)
     if a.warmup:
         p.measure_every = a.warmup
         eng.md_steps(p, a.warmup)
@@ -196,6 +200,7 @@ def main():
         "vs_baseline": None, "dtype": "fp32" if mode == abi.MDX_MIXED else "f64",
         "data": "synthetic",
         "config": {"workload": wl["name"], "sites": int(bs.struct.ns), "charges": nq,
+                   "drude": polar,
                    "box_A": wl["L"], "cutoff_A": wl["cutoff"], "alpha": bs.info["alpha"],
                    "kappa": bs.info["kappa"], "k_vectors": nk, "pairs": npair,
                    "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU",

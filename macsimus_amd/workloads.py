@@ -90,6 +90,20 @@ def brine_c2():
     return dict(name="brine_194304", ff=ff, counts=counts, L=L, cutoff=12.0, r=r)
 
 
+def polar_c3():
+    """Polarizable brine: 32,000 water + 600 Na+/Cl- pairs with a Drude
+    charge q_D = -8 e on O (polarizability 1.44 A^3) and on Cl (3.5 A^3),
+    spring constant q_D^2/alpha; 97,200 sites, 129,800 charges with the
+    Drudes; 1.0 g/cm3, random placement on a 33^3 lattice (seed 3), 10 A
+    cutoff.  The Drudes start on their centres (offsets 0)."""
+    ff = brine_forcefield(True)
+    ff["polar"] = {"O": (1.44, -8.0), "CL": (3.5, -8.0)}
+    counts = [600, 600, 32000]
+    L = box_length([(600, M_NA), (600, M_CL), (32000, 2 * M_H + M_O)], 1.0)
+    r = lattice_config(ff, counts, L, 33, seed=3)
+    return dict(name="polar_brine_97200", ff=ff, counts=counts, L=L, cutoff=10.0, r=r, polar=True)
+
+
 def water_c1():
     """1000 SPC/E on a 10^3 lattice, L=31.04 A, 9 A cutoff (seed 1)."""
     ff = brine_forcefield(False)

@@ -1181,21 +1181,28 @@ static double orc_now(void)
 int oracle_time_sample(const mdx_system_t *S, const double *r, int nthreads,
                        int kstride, double t[3])
 {
-  int ns = S->ns, i, nq = 0, err;
-  int *idx = malloc(sizeof(int) * ns);
-  double *q = malloc(sizeof(double) * ns);
-  double (*pos)[3] = malloc(sizeof(double[3]) * ns);
-  double *f = calloc(3 * (size_t)ns, sizeof(double));
+  /* r: [ns] rows, or [2 ns] (centres, Drude offsets) with Drudes; the
+     k-space sample takes every kstride-th charge of the charge list
+     (centres, then Drudes at r + rpol) */
+  int ns = S->ns, i, nq = 0, err, polar = S->chargepol != NULL;
+  int nrow = polar ? 2 * ns : ns;
+  int *idx = malloc(sizeof(int) * nrow);
+  double *q = malloc(sizeof(double) * nrow);
+  double (*pos)[3] = malloc(sizeof(double[3]) * nrow);
+  double *f = calloc(3 * (size_t)nrow, sizeof(double));
   double Pvir[6] = {0, 0, 0, 0, 0, 0}, t0;
   orc_acc a;
   int k = 0;
   if (kstride < 1) kstride = 1;
-  for (i = 0; i < ns; i++)
-    if (S->charge[i] != 0 && (k++ % kstride) == 0) {
-      idx[nq] = i; q[nq] = S->charge[i];
-      pos[nq][0] = r[3 * i]; pos[nq][1] = r[3 * i + 1]; pos[nq][2] = r[3 * i + 2];
-      nq++;
-    }
+  for (i = 0; i < nrow; i++) {
+    int site = i < ns ? i : i - ns, d;
+    double qi = i < ns ? S->charge[i] : S->chargepol[site];
+    if (qi == 0 || (k++ % kstride) != 0) continue;
+    idx[nq] = i;
+    q[nq] = qi;
+    for (d = 0; d < 3; d++) pos[nq][d] = r[3 * site + d] + (i < ns ? 0 : r[3 * i + d]);
+    nq++;
+  }
   memset(&a, 0, sizeof(a));
   a.S = S;
   a.measure = 0;

@@ -67,3 +67,70 @@ def test_translation_invariance(c2, mode):
     assert out2.npair == out.npair
     E, E2 = out.En_pot + out.En_el, out2.En_pot + out2.En_el
     assert abs(E2 - E) < tol * abs(E)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    """the Drude-polarizable brine (97,200 sites + 32,600 Drudes, q_D = -8 e)
+    after 3 SCF iterations from Drudes on their centres, in FP64: a state
+    with nonzero Drude offsets for both precisions to evaluate"""
+    wl = workloads.polar_c3()
+    bs = build_system(wl["ff"], wl["counts"], [wl["L"]] * 3, wl["cutoff"], epsr=0.05, epsk=0.05)
+    eng = Engine(bs, abi.MDX_FP64)
+    r = np.concatenate([wl["r"], np.zeros_like(wl["r"])])
+    for _ in range(3):
+        f, out = eng.forces(r, measure=1)
+        eng.selffield(r, f, out, 0.7)
+    eng.close()
+    return wl, bs, r
+
+
+def test_polar_c3_mixed_vs_fp64(c3):
+    wl, bs, r = c3
+    ns = bs.struct.ns
+    assert np.abs(r[ns:]).max() > 1e-3          # the Drudes moved
+    res = {}
+    for mode in (abi.MDX_FP64, abi.MDX_MIXED):
+        eng = Engine(bs, mode)
+        res[mode] = [eng.forces(r, measure=m) for m in (0, 1)]
+        eng.close()
+    # measure=0: FP32 four-term pair kernel in the mixed mode
+    for m in (0, 1):
+        f64, o64 = res[abi.MDX_FP64][m]
+        f32, o32 = res[abi.MDX_MIXED][m]
+        e = rel(f32, f64)
+        ed = rel(f32[ns:], f64[ns:])
+        print("C3 measure=%d mixed vs FP64: rel RMS force err %.3e (Drude rows %.3e)" % (m, e, ed), flush=True)
+        assert e < 1e-5 and ed < 1e-5
+        assert o32.npair == o64.npair
+    # measured call: energy and tensor
+    o64, o32 = res[abi.MDX_FP64][1][1], res[abi.MDX_MIXED][1][1]
+    E64, E32 = o64.En_pot + o64.En_el, o32.En_pot + o32.En_el
+    print("C3 Epot fp64 %.10e mixed %.10e rel %.2e" % (E64, E32, abs(E32 - E64) / abs(E64)), flush=True)
+    assert abs(E32 - E64) < 1e-5 * abs(E64)
+    scale = max(abs(o64.Pvir[k]) for k in range(3))
+    for i in range(6):
+        assert abs(o32.Pvir[i] - o64.Pvir[i]) < 1e-5 * scale
+
+
+def test_polar_c3_md_steps(c3):
+    """a few device-resident ASPC + SCF + SHAKE steps in both precisions
+    stay together"""
+    wl, bs, r = c3
+    out = {}
+    for mode in (abi.MDX_FP64, abi.MDX_MIXED):
+        eng = Engine(bs, mode)
+        eng.upload_state(r, np.zeros_like(r))
+        p = abi.Step(h=0.001, eps=1e-7, maxit=0, thermostat=0, T=298.0, tau=0.1, nf=1.0,
+                     measure_every=0, aspc_k=4, scf_iter=1, scf_omega=0.0)
+        so = eng.md_steps(p, 5)
+        rr, r1 = eng.download_state()
+        out[mode] = (so, rr)
+        eng.close()
+    (s64, r64), (s32, r32) = out[abi.MDX_FP64], out[abi.MDX_MIXED]
+    ns = bs.struct.ns
+    d = np.abs(r32[:ns] - r64[:ns]).max()
+    print("C3 5 steps: max site deviation %.3e A, Eself %.6e / %.6e" % (d, s64.En_self, s32.En_self), flush=True)
+    assert d < 1e-6
+    assert abs(s32.En_self - s64.En_self) < 1e-5 * abs(s64.En_self)
+    assert s64.shake_iter_max > 0
