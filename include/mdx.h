@@ -187,6 +187,12 @@ int mdx_download_state(mdx_ctx_t *ctx, double *r, double *r1);
    el.minqq), checked at every measured step and at the last one. */
 int mdx_md_steps(mdx_ctx_t *ctx, const mdx_step_t *p, int nsteps,
                  mdx_step_out_t *out);
+/* Unmeasured steps in the steady state (for Drudes: once the predictor has
+   its full length) are replayed from a hipGraph of one step, captured on
+   first use and again after any change of step parameters or engine
+   configuration; on = 0 launches every step eagerly.  Kernel timing
+   (mdx_set_timing) also runs eagerly.  Default: on. */
+int mdx_set_graph(mdx_ctx_t *ctx, int on);
 
 /* ASPC state of a Drude system: predictor length k (0..8), length
    reached kk (0..k) and the older history aa[2..k+1] as k arrays of ns*3

@@ -671,6 +671,7 @@ int alloc_grid(Ctx &c) {
   if ((rc = c.cnt.alloc(ncell)) || (rc = c.cstart.alloc(ncell + 1)) || (rc = c.cursor.alloc(ncell)))
     return rc;
   const int np = pair_parts(c);
+  c.gen++;                             // launch shapes and buffers change
   c.part_blocks = std::max(nblk(c.ns) * np, nblk(c.nmol));
   c.part_blocks = std::max(c.part_blocks, 1024);
   if (np > 1 && (rc = c.pf.alloc((size_t)np * 3 * (c.polar ? 2 : 1) * c.ns))) return rc;
@@ -978,6 +979,8 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.erf32.release();
   c.coef32.release();
   c.tm.release();
+  if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+  if (c.graph) (void)hipGraphDestroy(c.graph);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return 0;

@@ -204,6 +204,16 @@ struct Ctx {
   // Drude ASPC history aa[2..k+1] (aa[0], aa[1] are the Drude rows of r, r1)
   DBuf<double> aspc_hist;
   int aspc_k = -1, aspc_kk = 0;
+  // configuration generation: bumped by every change of buffers, grid or
+  // k-space split, so that a captured step graph is rebuilt
+  unsigned gen = 0;
+  // the unmeasured MD step captured as a hipGraph (on by default)
+  bool use_graph = true, graph_ok = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned graph_gen = 0;
+  int graph_kk = -1;
+  mdx_step_t graph_p{};
   // kernel timing with HIP events on the context stream (off by default)
   bool timing = false;
   Timer tm;

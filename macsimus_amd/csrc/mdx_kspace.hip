@@ -651,6 +651,7 @@ int kspace_chunks(Ctx &c, int kzc, int parts) {
   c.nchunk = nchunk;
   c.kparts = parts;
   c.nkpart = np;
+  c.gen++;                             // launch shapes and buffers change
   return 0;
 }
 

@@ -323,6 +323,7 @@ int set_aspc(Ctx &c, int k, int kk, const double *hist) {
   if (c.aspc_k != k || !c.aspc_hist.p) {
     c.aspc_hist.release();
     if ((rc = c.aspc_hist.alloc(n))) return rc;
+    c.gen++;
   }
   if (hist)
     MDX_HIP(hipMemcpyAsync(c.aspc_hist.p, hist, sizeof(double) * 3 * c.ns * k,
@@ -432,6 +433,80 @@ extern "C" int mdx_get_aspc(mdx_ctx_t *h, int *k, int *kk, double *hist) {
   return 0;
 }
 
+namespace {
+
+// the launches of one MD step (no host synchronization): for Drudes the
+// ASPC prediction with predictor length kk and scf_iter forces()+selffield()
+// with mixing omega, then forces (others), SHAKE + leap-frog, the reduction
+// and the Berendsen scaling
+int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) {
+  const int nb = nblk(c.nmol);
+  int rc;
+  if (c.polar) {
+    k_aspc<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r.p + 3 * c.ns, c.r1.p + 3 * c.ns,
+                                                     c.aspc_hist.p, c.aspc_k, kk, aspc_coef(kk));
+    const int niter = p->scf_iter > 0 ? p->scf_iter : 1;
+    for (int it = 0; it < niter; it++) {
+      if ((rc = forces_device(c, measure && it == niter - 1))) return rc;
+      launch_selffield(c, omega);
+    }
+  } else if ((rc = forces_device(c, measure))) {
+    return rc;
+  }
+  if (c.timing) c.tm.start(T_SHAKE, c.stream);
+  k_shake<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.cons_first.p, c.cons_ij.p,
+                                       c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
+                                       p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
+                                       c.mdpart.p);
+  if (c.timing) c.tm.stop(T_SHAKE, c.stream);
+  k_md_reduce<<<1, kBlock, 0, c.stream>>>(c.mdpart.p, nb, c.mdacc.p, c.mdflag.p);
+  if (p->thermostat == 1)
+    k_berendsen<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r1.p, c.mdacc.p, p->h,
+                                                          p->T, p->tau, p->nf, c.mdflag.p);
+  if (hipGetLastError() != hipSuccess) return fail(MDX_E_GPU, "mdx_md_steps: launch");
+  return 0;
+}
+
+// the step parameters a captured graph depends on
+bool same_step(const mdx_step_t &a, const mdx_step_t &b) {
+  return a.h == b.h && a.eps == b.eps && a.maxit == b.maxit && a.thermostat == b.thermostat &&
+         a.T == b.T && a.tau == b.tau && a.nf == b.nf && a.aspc_k == b.aspc_k &&
+         a.scf_iter == b.scf_iter && a.scf_omega == b.scf_omega;
+}
+
+// an unmeasured step as a hipGraph, captured once per step parameters and
+// engine configuration (buffers, grid, k-space split: c.gen) and replayed
+int graph_step(Ctx &c, const mdx_step_t *p, int kk, double omega) {
+  if (!c.graph_ok || c.graph_gen != c.gen || !same_step(c.graph_p, *p) || c.graph_kk != kk) {
+    if (c.graph_exec) { (void)hipGraphExecDestroy(c.graph_exec); c.graph_exec = nullptr; }
+    if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
+    c.graph_ok = false;
+    MDX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(c, p, 0, kk, omega);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess || !g) return fail(MDX_E_GPU, "mdx_md_steps: graph capture failed");
+    c.graph = g;
+    if (hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0) != hipSuccess)
+      return fail(MDX_E_GPU, "mdx_md_steps: graph instantiation failed");
+    c.graph_ok = true;
+    c.graph_gen = c.gen;
+    c.graph_p = *p;
+    c.graph_kk = kk;
+  }
+  MDX_HIP(hipGraphLaunch(c.graph_exec, c.stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int mdx_set_graph(mdx_ctx_t *h, int on) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_graph: NULL");
+  h->c.use_graph = on != 0;
+  return 0;
+}
+
 extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_step_out_t *out) {
   if (!h || !p || !out || nsteps < 0) return fail(MDX_E_ARG, "mdx_md_steps: bad argument");
   Ctx &c = h->c;
@@ -444,43 +519,25 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
   MDX_HIP(hipSetDevice(c.device));
   int rc;
   if (c.polar && c.aspc_k != p->aspc_k && (rc = set_aspc(c, p->aspc_k, 0, nullptr))) return rc;
-  const int nb = nblk(c.nmol);
   std::memset(out, 0, sizeof(*out));
   out->lambda = 1;
   const int every = p->measure_every > 0 ? p->measure_every : nsteps;
-  const int niter = p->scf_iter > 0 ? p->scf_iter : 1;
   double acc[ACC_N], md[MD_N], sf[SF_N], flag[4];
   reset_flags(c);
   rc = 0;
   for (int s = 1; s <= nsteps && !rc; s++) {
     const int measure = (s % every == 0) || s == nsteps;
-    if (c.polar) {
-      // ASPC prediction, then scf_iter selffield iterations (cook -p{k}20)
-      k_aspc<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r.p + 3 * c.ns,
-                                                       c.r1.p + 3 * c.ns, c.aspc_hist.p,
-                                                       c.aspc_k, c.aspc_kk, aspc_coef(c.aspc_kk));
-      if (c.aspc_kk < c.aspc_k) c.aspc_kk++;
-      const double omega = p->scf_omega > 0 ? p->scf_omega
+    // predictor length of this step (grows to aspc_k, sim/pred2.c:67-77)
+    const int kk = c.aspc_kk;
+    if (c.polar && c.aspc_kk < c.aspc_k) c.aspc_kk++;
+    const double omega = !c.polar ? 0.0
+                         : p->scf_omega > 0 ? p->scf_omega
                                             : (c.aspc_kk + 2) / (double)(2 * c.aspc_kk + 3);
-      for (int it = 0; it < niter && !rc; it++) {
-        if ((rc = forces_device(c, measure && it == niter - 1))) break;
-        launch_selffield(c, omega);
-      }
-      if (rc) break;
-    } else if ((rc = forces_device(c, measure))) {
-      break;
-    }
-    if (c.timing) c.tm.start(T_SHAKE, c.stream);
-    k_shake<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.cons_first.p, c.cons_ij.p,
-                                         c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
-                                         p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
-                                         c.mdpart.p);
-    if (c.timing) c.tm.stop(T_SHAKE, c.stream);
-    k_md_reduce<<<1, kBlock, 0, c.stream>>>(c.mdpart.p, nb, c.mdacc.p, c.mdflag.p);
-    if (p->thermostat == 1)
-      k_berendsen<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r1.p, c.mdacc.p, p->h,
-                                                            p->T, p->tau, p->nf, c.mdflag.p);
-    if (hipGetLastError() != hipSuccess) { rc = fail(MDX_E_GPU, "mdx_md_steps: launch"); break; }
+    // unmeasured steps of the steady state replay a captured graph
+    const bool steady = !c.polar || kk == c.aspc_k;
+    if (!measure && steady && c.use_graph && !c.timing) rc = graph_step(c, p, kk, omega);
+    else rc = launch_step(c, p, measure, kk, omega);
+    if (rc) break;
     if (measure) {
       if (hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
           hipMemcpyAsync(md, c.mdacc.p, sizeof(md), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||

@@ -125,6 +125,10 @@ class Engine:
         abi.check(self.lib, rc, "mdx_get_timing")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.FAMILIES)}
 
+    def set_graph(self, on=True):
+        rc = self.lib.mdx_set_graph(self.ctx, int(on))
+        abi.check(self.lib, rc, "mdx_set_graph")
+
     def set_stencil(self, w, pcols=0):
         rc = self.lib.mdx_set_stencil(self.ctx, int(w), int(pcols))
         abi.check(self.lib, rc, "mdx_set_stencil")
@@ -160,6 +164,7 @@ def declare_extras(lib):
     lib.mdx_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_set_kchunk.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mdx_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
