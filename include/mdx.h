@@ -234,6 +234,22 @@ int mdx_set_kchunk(mdx_ctx_t *ctx, int kzc, int parts);
 int mdx_autotune(mdx_ctx_t *ctx, const double *r, int reps, int *choice, double *ms);
 int mdx_info(mdx_ctx_t *ctx, int *iv, int n);
 
+/* ---- force-field fitting (no counterpart in cook) ----------------------
+   mdx_set_params: new site-site table sstab [ntypes^2] (mdx_setss) and/or
+   charges (centre charges [ns], Drude charges [ns]) for the next calls;
+   NULL keeps the old ones.  alpha, kappa, the k-vectors and the set of
+   charged sites stay.
+   mdx_adjoint_lj: adjoint of the LJ terms at positions r (as mdx_forces)
+   for the weights w [ns*3] = dL/dF of a loss of the site forces; for each
+   type pair (a,b) (a = central site) gF[(a*nt+b)*4 + k] = sum_i w_i .
+   dF_i/dc_k with c = (Sq, E48, A4, C2q) and gE[(a*nt+b)*4 + k] = dEpot/dc_k
+   with c = (Sq, E4, A, C2q), each unordered pair split half and half
+   between (a,b) and (b,a).  The pair terms are recomputed in FP64. */
+int mdx_set_params(mdx_ctx_t *ctx, const mdx_lj_t *sstab, const double *charge,
+                   const double *chargepol);
+int mdx_adjoint_lj(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
+                   double *gE);
+
 /* ---- pressure assembly and cutoff correction (host arithmetic) -------- */
 
 /* measurePconstraints() (sim/constrd.c:115-265) for PRESSURETENSOR=7: all

@@ -781,6 +781,23 @@ int forces_device(Ctx &c, int measure) {
   return 0;
 }
 
+// binning of the device positions c.r (cell-ordered arrays for kernels
+// outside this file)
+int bin_device(Ctx &c) { return bin_sites(c, c.r.p); }
+
+// LJ tables of the kernels from the site-site table (A4 and the cutoffs of
+// LJ-free pairs are zero)
+std::vector<double> lj_table(const mdx_lj_t *sstab, int nt2) {
+  std::vector<double> lj(nt2 * 8);
+  for (int k = 0; k < nt2; k++) {
+    const mdx_lj_t &t = sstab[k];
+    double v[8] = {t.Sq, t.E4, t.E24, t.E48, t.C1q, t.C2q, t.A, t.A4};
+    if (!(t.C2q > 0)) for (int m = 4; m < 8; m++) v[m] = 0;
+    std::copy(v, v + 8, lj.begin() + 8 * k);
+  }
+  return lj;
+}
+
 void fill_out(const Ctx &c, const double *acc, int measure, mdx_out_t *out) {
   std::memset(out, 0, sizeof(*out));
   out->npair = (long long)std::llround(acc[ACC_NPAIR]);
@@ -880,13 +897,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
 
   // LJ table, FP64 and FP32 (A4 of LJ-free pairs is zero)
   const int nt2 = c.ntypes * c.ntypes;
-  std::vector<double> lj(nt2 * 8);
-  for (int k = 0; k < nt2; k++) {
-    const mdx_lj_t &t = s->sstab[k];
-    double v[8] = {t.Sq, t.E4, t.E24, t.E48, t.C1q, t.C2q, t.A, t.A4};
-    if (!(t.C2q > 0)) for (int m = 4; m < 8; m++) v[m] = 0;
-    std::copy(v, v + 8, lj.begin() + 8 * k);
-  }
+  std::vector<double> lj = lj_table(s->sstab, nt2);
   std::vector<float> lj32(lj.begin(), lj.end());
   // splines: FP64 cook form, FP32 stable form around the interval start
   c.nerf = s->erfc_n;

@@ -221,6 +221,10 @@ struct Ctx {
 
 // reduction buffers of the MD step and the Drude selffield
 int md_setup(Ctx &c);
+// binning of the device positions c.r into the cell-ordered arrays
+int bin_device(Ctx &c);
+// kernel LJ table [nt2][Sq E4 E24 E48 C1q C2q A A4]
+std::vector<double> lj_table(const mdx_lj_t *sstab, int nt2);
 // set-up of the k-space layout for the context box and kappa
 int kspace_setup(Ctx &c);
 // pass-2 tables for kz chunk kzc (8..64, multiple of 8) and about

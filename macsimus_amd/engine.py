@@ -125,6 +125,30 @@ class Engine:
         abi.check(self.lib, rc, "mdx_get_timing")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.FAMILIES)}
 
+    def set_params(self, sstab=None, charge=None, chargepol=None):
+        """new LJ table (ctypes array of abi.LJ, ntypes^2) and/or charges"""
+        dp = C.POINTER(C.c_double)
+        q = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+        qp = None if chargepol is None else np.ascontiguousarray(chargepol, dtype=np.float64)
+        rc = self.lib.mdx_set_params(self.ctx, None if sstab is None else C.cast(sstab, C.POINTER(abi.LJ)),
+                                     None if q is None else q.ctypes.data_as(dp),
+                                     None if qp is None else qp.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_set_params")
+
+    def adjoint_lj(self, r, w):
+        """(gF, gE) [nt, nt, 4]: sum_i w_i . dF_i/dc and dE/dc for the LJ
+        constants of each type pair (see include/mdx.h)"""
+        nt = self.bs.struct.ntypes
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.nrow, 3)
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.ns, 3)
+        gF = np.zeros((nt, nt, 4))
+        gE = np.zeros((nt, nt, 4))
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_adjoint_lj(self.ctx, r.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                     gF.ctypes.data_as(dp), gE.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_adjoint_lj")
+        return gF, gE
+
     def set_graph(self, on=True):
         rc = self.lib.mdx_set_graph(self.ctx, int(on))
         abi.check(self.lib, rc, "mdx_set_graph")
@@ -165,6 +189,9 @@ def declare_extras(lib):
     lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_set_kchunk.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
     lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]

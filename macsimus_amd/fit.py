@@ -1,0 +1,343 @@
+"""Force-field fitting on the engine: force matching (and energy gradients)
+for per-type-pair Lennard-Jones sigma/epsilon and per-type charges.
+
+Parameters theta = [sigma_p (A), epsilon_p (kJ/mol) for each fitted type
+pair p] + [charge parameters (e)].  A type pair (a,b) with its own sigma,
+epsilon enters cook's site-site table through setss() with both sites
+given the same (EvdW, RvdW), which the combining rule maps to themselves
+(sim/lj/sitesite.c:66-112); the type charges are an affine function of the
+charge parameters (e.g. water H = -q_O/2).
+
+Gradient of the loss L = sum_frames sum_i |F_i(theta) - F_i^target|^2:
+  * LJ: the engine's adjoint kernel (mdx_adjoint_lj) gives, for the
+    weights w_i = dL/dF_i = 2 (F_i - F_i^target), the derivatives with
+    respect to the table constants (Sq, E48, A4, C2q) of each type pair;
+    the chain to (sigma, epsilon) goes through setss() (mdx_setss, which
+    includes the smooth-cutoff fixed point) by central differences of the
+    constants on the host.
+  * charges: the site forces and the energy are quadratic forms in the
+    charges (r-space, exclusion, k-space and dipole terms alike), so the
+    central difference of the forward call, [F(q + h) - F(q - h)] / 2h,
+    is the exact derivative up to rounding, for any step h.
+
+`torch_loss(fm)` wraps this as a torch.autograd.Function on CPU tensors
+(torch is imported there only: its bundled HIP runtime must not share a
+process that computes on the GPU through torch).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .engine import Engine
+from .system import EUNIT_KCAL, ELECTRON, build_system
+
+KCAL_PER_KJ = 1 / 4.184
+TWO16 = 2 ** (1 / 6)
+
+
+class ForceMatch:
+    """frames: [nframe][ns][3] positions (for Drude systems [2 ns] rows);
+    targets: [nframe][ns][3] target forces on the sites (program units);
+    lj_pairs: list of type-name pairs (a, b) whose sigma, epsilon are
+    fitted; charge_map: (names of the types, base charges [e] per type,
+    matrix [ntypes][nq] so that q_type = base + M @ theta_q)."""
+
+    def __init__(self, ff, counts, L, cutoff, frames, targets, lj_pairs, charge_map,
+                 mode=abi.MDX_FP64, epsr=0.05, epsk=0.05, alpha=None, kappa=None):
+        self.ff = ff
+        self.counts = counts
+        self.L = [L] * 3 if np.isscalar(L) else list(L)
+        self.cutoff = cutoff
+        self.frames = [np.ascontiguousarray(f, dtype=np.float64) for f in frames]
+        self.targets = [np.ascontiguousarray(t, dtype=np.float64) for t in targets]
+        self.bs = build_system(ff, counts, self.L, cutoff, epsr=epsr, epsk=epsk,
+                               alpha=alpha, kappa=kappa)
+        self.eng = Engine(self.bs, mode)
+        self.lib = self.eng.lib
+        self.types = list(ff["types"])
+        self.nt = len(self.types)
+        self.pairs = [(self.types.index(a), self.types.index(b)) for a, b in lj_pairs]
+        names, base, M = charge_map
+        self.q_types = [self.types.index(n) for n in names]
+        self.q_base = np.asarray(base, dtype=np.float64)
+        self.q_M = np.asarray(M, dtype=np.float64)
+        self.npl = 2 * len(self.pairs)
+        self.npar = self.npl + self.q_M.shape[1]
+        self.site_type = self.bs.info["site_type"]
+        self.polar = bool(self.bs.info["polar"])
+        self.qpol = self.bs.info["chargepol"]
+        self.ns = self.bs.struct.ns
+        self.C2 = cutoff
+
+    # ---- parameters -> engine tables
+    def theta0(self):
+        """the parameters of the force field the system was built with"""
+        th = []
+        E = self.ff["type_EvdW_kcal"]
+        R = self.ff["type_RvdW"]
+        for a, b in self.pairs:
+            # Lorentz-Berthelot: sigma_ab = (sigma_a + sigma_b)/2, eps = sqrt(eps_a eps_b)
+            sa, sb = 2 * R[a] / TWO16, 2 * R[b] / TWO16
+            ea, eb = -E[a] / KCAL_PER_KJ, -E[b] / KCAL_PER_KJ
+            th += [(sa + sb) / 2, np.sqrt(ea * eb)]
+        q = self._type_charges_from_ff()
+        # least-squares inverse of the affine charge map
+        rhs = np.array([q[t] for t in self.q_types]) - self.q_base
+        th += list(np.linalg.lstsq(self.q_M, rhs, rcond=None)[0])
+        return np.array(th)
+
+    def _type_charges_from_ff(self):
+        q = np.zeros(self.nt)
+        for sp in self.ff["species"]:
+            for t, c in zip(sp["site_type"], sp["site_charge_e"]):
+                q[t] = c
+        return q
+
+    def _setss(self, sigma, eps_kj):
+        e_k = -eps_kj * KCAL_PER_KJ * EUNIT_KCAL
+        r = sigma * TWO16 / 2
+        ss = abi.LJ()
+        corr = C.c_double(0)
+        abi.check(self.lib, self.lib.mdx_setss(C.byref(ss), e_k, r, e_k, r, 1, self.C2, C.byref(corr)),
+                  "mdx_setss")
+        return ss
+
+    def _table(self, theta):
+        tab = (abi.LJ * (self.nt * self.nt))()
+        C.memmove(tab, self.bs.info["sstab"], C.sizeof(tab))
+        for k, (a, b) in enumerate(self.pairs):
+            ss = self._setss(theta[2 * k], theta[2 * k + 1])
+            tab[a * self.nt + b] = ss
+            tab[b * self.nt + a] = ss
+        return tab
+
+    def _charges(self, theta):
+        """centre charges [ns] (program units) for the charge parameters;
+        for Drude sites the centre carries total - Drude"""
+        qt = self._type_charges_from_ff()
+        qt[self.q_types] = self.q_base + self.q_M @ theta[self.npl:]
+        q = qt[self.site_type] * ELECTRON
+        return q - self.qpol if self.polar else q
+
+    def apply(self, theta):
+        self.eng.set_params(sstab=self._table(theta), charge=self._charges(theta))
+
+    # ---- loss and gradient
+    def forces(self, theta, measure=0):
+        self.apply(theta)
+        return [self.eng.forces(r, measure=measure) for r in self.frames]
+
+    def loss(self, theta):
+        L = 0.0
+        for (f, _), t in zip(self.forces(theta), self.targets):
+            L += float(((f[:self.ns] - t) ** 2).sum())
+        return L
+
+    def _dconst(self, sigma, eps):
+        """d(Sq, E48, A4, C2q, E4, A)/d(sigma, eps) by central differences
+        of setss"""
+        def vec(s, e):
+            ss = self._setss(s, e)
+            return np.array([ss.Sq, ss.E48, ss.A4, ss.C2q, ss.E4, ss.A])
+        out = np.zeros((6, 2))
+        for k, (x, h) in enumerate(((sigma, 1e-6 * sigma), (eps, 1e-6 * eps))):
+            if k == 0:
+                out[:, 0] = (vec(x + h, eps) - vec(x - h, eps)) / (2 * h)
+            else:
+                out[:, 1] = (vec(sigma, x + h) - vec(sigma, x - h)) / (2 * h)
+        return out
+
+    def loss_and_grad(self, theta, hq=1e-3):
+        """L(theta) and dL/dtheta (adjoint for LJ, exact quadratic-form
+        differences for the charges)"""
+        theta = np.asarray(theta, dtype=np.float64)
+        fr = self.forces(theta)
+        L = 0.0
+        gF = np.zeros((self.nt, self.nt, 4))
+        ws = []
+        for (f, _), t, r in zip(fr, self.targets, self.frames):
+            d = f[:self.ns] - t
+            L += float((d * d).sum())
+            w = 2 * d
+            ws.append(w)
+            gf, _ = self.eng.adjoint_lj(r, w)
+            gF += gf
+        g = np.zeros(self.npar)
+        for k, (a, b) in enumerate(self.pairs):
+            gc = gF[a, b] + (gF[b, a] if a != b else 0)          # d L / d(Sq, E48, A4, C2q)
+            J = self._dconst(theta[2 * k], theta[2 * k + 1])[:4]
+            g[2 * k:2 * k + 2] = gc @ J
+        for m in range(self.q_M.shape[1]):
+            e = np.zeros(self.npar)
+            e[self.npl + m] = hq
+            fp = self.forces(theta + e)
+            fm = self.forces(theta - e)
+            s = 0.0
+            for w, (a_, _), (b_, _) in zip(ws, fp, fm):
+                s += float((w * (a_[:self.ns] - b_[:self.ns])).sum()) / (2 * hq)
+            g[self.npl + m] = s
+        self.apply(theta)
+        return L, g
+
+    def energy_and_grad(self, theta, hq=1e-3):
+        """sum over frames of En.pot + En.el and its gradient (adjoint gE for
+        LJ, exact quadratic-form differences for the charges)"""
+        theta = np.asarray(theta, dtype=np.float64)
+        outs = self.forces(theta, measure=1)
+        E = sum(o.En_pot + o.En_el for _, o in outs)
+        gE = np.zeros((self.nt, self.nt, 4))
+        zero = np.zeros((self.ns, 3))
+        for r in self.frames:
+            gE += self.eng.adjoint_lj(r, zero)[1]
+        g = np.zeros(self.npar)
+        for k, (a, b) in enumerate(self.pairs):
+            gc = gE[a, b] + (gE[b, a] if a != b else 0)          # d E / d(Sq, E4, A, C2q)
+            J = self._dconst(theta[2 * k], theta[2 * k + 1])
+            g[2 * k:2 * k + 2] = gc @ J[[0, 4, 5, 3]]
+        for m in range(self.q_M.shape[1]):
+            e = np.zeros(self.npar)
+            e[self.npl + m] = hq
+            Ep = sum(o.En_pot + o.En_el for _, o in self.forces(theta + e, measure=1))
+            Em = sum(o.En_pot + o.En_el for _, o in self.forces(theta - e, measure=1))
+            g[self.npl + m] = (Ep - Em) / (2 * hq)
+        self.apply(theta)
+        return E, g
+
+    def close(self):
+        self.eng.close()
+
+
+def torch_loss(loss_and_grad):
+    """A torch function theta -> L(theta) whose backward returns the given
+    gradient: loss_and_grad(theta_numpy) -> (L, dL/dtheta).  CPU tensors."""
+    import torch
+
+    class _ForceMatchLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta):
+            L, g = loss_and_grad(theta.detach().cpu().numpy().astype(np.float64))
+            ctx.save_for_backward(torch.as_tensor(np.asarray(g), dtype=theta.dtype))
+            return theta.new_tensor(L)
+
+        @staticmethod
+        def backward(ctx, gout):
+            (g,) = ctx.saved_tensors
+            return gout * g
+
+    return _ForceMatchLoss.apply
+
+
+# ---- the force-matching workload: 64 frames of a 23,712-site box
+FIT_PAIRS = [("O", "O"), ("NA", "O"), ("CL", "O"), ("NA", "CL")]
+# q_H = -q_O/2 (neutral water), q_O, q_Na, q_Cl
+FIT_CHARGES = (["H", "O", "NA", "CL"], [0, 0, 0, 0],
+               [[-0.5, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+
+
+def make_frames(nframes=64, equil=1000, every=20, seed_shift=0):
+    """Frames of the 23,712-site SPC/E + 0.5 mol/kg NaCl box, equilibrated
+    by the engine (FP64, Berendsen 298 K, h = 2 fs, SHAKE) from the lattice:
+    equil steps, then one frame every `every` steps."""
+    from . import workloads
+    wl = workloads.brine_c5()
+    bs = build_system(wl["ff"], wl["counts"], [wl["L"]] * 3, wl["cutoff"], epsr=0.05, epsk=0.05)
+    eng = Engine(bs, abi.MDX_FP64)
+    eng.autotune(wl["r"], reps=1)
+    nf = workloads.degrees_of_freedom(wl["ff"], wl["counts"])
+    p = abi.Step(h=0.002, eps=1e-7, maxit=0, thermostat=1, T=298.0, tau=0.1, nf=float(nf),
+                 measure_every=0)
+    eng.upload_state(wl["r"], np.zeros_like(wl["r"]))
+    eng.md_steps(p, equil)
+    frames = []
+    for _ in range(nframes):
+        eng.md_steps(p, every)
+        frames.append(eng.download_state()[0])
+    info = dict(alpha=bs.info["alpha"], kappa=bs.info["kappa"])
+    eng.close()
+    return wl, frames, info
+
+
+def main():
+    import argparse
+    import json
+    import sys
+    import time
+    import torch   # before any GPU work in this process (CPU tensors only)
+
+    ap = argparse.ArgumentParser(description="force matching of LJ sigma/epsilon and charges")
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=8, help="L-BFGS iterations")
+    ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check")
+    a = ap.parse_args()
+    t0 = time.perf_counter()
+    wl, frames, info = make_frames(a.frames)
+    t_frames = time.perf_counter() - t0
+    kw = dict(alpha=info["alpha"], kappa=info["kappa"])
+    ref = ForceMatch(wl["ff"], wl["counts"], wl["L"], wl["cutoff"], frames,
+                     [np.zeros_like(f) for f in frames], FIT_PAIRS, FIT_CHARGES, **kw)
+    th0 = ref.theta0()
+    # targets: +5 % sigma and epsilon, charges -0.02 e (O), +0.02 e (Na), -0.02 e (Cl)
+    tht = th0 * np.array([1.05] * 8 + [1, 1, 1]) + np.array([0] * 8 + [-0.02, 0.02, -0.02])
+    targets = [f for f, _ in ref.forces(tht)]
+    ref.close()
+    res = {"workload": wl["name"], "frames": a.frames, "sites": int(sum(
+        n * len(sp["site_type"]) for n, sp in zip(wl["counts"], wl["ff"]["species"]))),
+        "frames_s": t_frames, "theta0": th0.tolist(), "theta_target": tht.tolist()}
+    fms = {m: ForceMatch(wl["ff"], wl["counts"], wl["L"], wl["cutoff"], frames, targets,
+                         FIT_PAIRS, FIT_CHARGES, mode=m, **kw) for m in (abi.MDX_FP64, abi.MDX_MIXED)}
+    fm64, fm32 = fms[abi.MDX_FP64], fms[abi.MDX_MIXED]
+    t0 = time.perf_counter()
+    L64, g64 = fm64.loss_and_grad(th0)
+    res["ms_loss_grad_fp64"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    L32, g32 = fm32.loss_and_grad(th0)
+    res["ms_loss_grad_mixed"] = (time.perf_counter() - t0) * 1e3
+    res["loss0"] = L64
+    res["grad_mixed_vs_fp64"] = float(np.abs(g32 - g64).max() / np.abs(g64).max())
+    if not a.no_fd:
+        gfd = np.zeros_like(g64)
+        for k in range(len(th0)):
+            h = 1e-4 * abs(th0[k])
+            e = np.zeros_like(th0)
+            e[k] = h
+            gfd[k] = (fm64.loss(th0 + e) - fm64.loss(th0 - e)) / (2 * h)
+        res["grad_vs_central_fd"] = float(np.abs(g64 - gfd).max() / np.abs(gfd).max())
+    print("gradients done: %s" % json.dumps({k: v for k, v in res.items() if k.startswith(("grad", "ms", "loss"))}),
+          file=sys.stderr, flush=True)
+
+    # fit with torch L-BFGS on the mixed engine, parameters relative to
+    # theta0; parameters for which setss has no smooth cutoff return a large
+    # loss (the line search then backtracks)
+    def scaled(x):
+        try:
+            L, g = fm32.loss_and_grad(x * th0)
+        except RuntimeError:
+            return 1e6, np.zeros_like(x)
+        return L / L64, g * th0 / L64
+    f = torch_loss(scaled)
+    x = torch.ones(len(th0), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([x], lr=0.1, max_iter=a.iters, line_search_fn="strong_wolfe")
+    hist = []
+
+    def closure():
+        opt.zero_grad()
+        L = f(x)
+        L.backward()
+        hist.append(float(L))
+        print("fit: evaluation %d, loss/loss0 %.6e" % (len(hist), float(L)), file=sys.stderr, flush=True)
+        return L
+    t0 = time.perf_counter()
+    opt.step(closure)
+    res["fit_s"] = time.perf_counter() - t0
+    th = x.detach().numpy() * th0
+    res["loss_history_rel"] = hist
+    res["theta_fit"] = th.tolist()
+    res["fit_param_rel_err_max"] = float(np.abs(th - tht).max() / np.abs(tht).max())
+    for fm in fms.values():
+        fm.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

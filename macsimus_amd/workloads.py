@@ -104,6 +104,17 @@ def polar_c3():
     return dict(name="polar_brine_97200", ff=ff, counts=counts, L=L, cutoff=10.0, r=r, polar=True)
 
 
+def brine_c5():
+    """Force-matching box: 7,856 SPC/E + 72 Na+/Cl- pairs (0.5 mol/kg,
+    23,712 sites), 1.0 g/cm3, a full 20^3 lattice in random order (seed 5),
+    10 A cutoff."""
+    ff = brine_forcefield(True)
+    counts = [72, 72, 7856]
+    L = box_length([(72, M_NA), (72, M_CL), (7856, 2 * M_H + M_O)], 1.0)
+    r = lattice_config(ff, counts, L, 20, seed=5)
+    return dict(name="brine_23712", ff=ff, counts=counts, L=L, cutoff=10.0, r=r)
+
+
 def water_c1():
     """1000 SPC/E on a 10^3 lattice, L=31.04 A, 9 A cutoff (seed 1)."""
     ff = brine_forcefield(False)

@@ -120,3 +120,19 @@ def test_erfc_spline_accuracy():
     p = tab[int(x * sg)]
     assert abs(p.Bu / (p.Cu + x) + p.Au) < 1e-12
     assert abs(p.Bd / (p.Cd + x) + p.Ad) < 1e-12
+
+
+def test_torch_autograd_wrapper_passes_the_gradient():
+    """the fitting harness's torch.autograd.Function returns the supplied
+    gradient (checked with torch.autograd.gradcheck on a quadratic whose
+    gradient is known in closed form)"""
+    torch = pytest.importorskip("torch")
+    from macsimus_amd.fit import torch_loss
+    A = np.array([[3.0, 1.0], [1.0, 2.0]])
+    b = np.array([0.5, -1.0])
+    f = torch_loss(lambda th: (float(th @ A @ th / 2 - b @ th), A @ th - b))
+    th = torch.tensor([0.3, -0.7], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(f, (th,), eps=1e-6, atol=1e-8)
+    L = f(th)
+    L.backward()
+    assert np.allclose(th.grad.numpy(), A @ th.detach().numpy() - b)
