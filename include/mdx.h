@@ -250,6 +250,47 @@ int mdx_set_params(mdx_ctx_t *ctx, const mdx_lj_t *sstab, const double *charge,
 int mdx_adjoint_lj(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
                    double *gE);
 
+/* ---- batched replicas ---------------------------------------------------
+   Many independent atomic systems (one site per molecule: LJ with cook's
+   smooth cutoff, point charges, Ewald; no exclusions, constraints or
+   Drudes) of uneven size in one batch: per system what one cook process
+   does with forces() (cook/forces.c:93-224) and the leap-frog + Berendsen
+   step of Shake() (sim/constrd.c:622-1081 for molecules without
+   constraints), each with its own box, cutoff, tables, k-vector set,
+   time step and thermostat, every kernel launched once for all systems.
+   Cook runs one system per process; these calls have no counterpart
+   beyond that per-system equivalence.
+   mdx_batch_create: sys[nsys] as for mdx_create (cubic boxes; nmol == ns,
+   ncons == 0, chargepol == NULL, masses required; ncell is ignored: each
+   system gets cells of at least cutoff/w, w = 1 until mdx_batch_set_stencil
+   or mdx_batch_autotune).  mode: the r-space precision as in mdx_create;
+   the batch k-space is FP64 in both modes.
+   Site arrays of the batch calls are the systems' [ns*3] arrays one after
+   another (system order).  out[nsys] per system. */
+typedef struct mdx_batch_s mdx_batch_t;
+int mdx_batch_create(mdx_batch_t **b, int nsys, const mdx_system_t *sys, int mode);
+int mdx_batch_destroy(mdx_batch_t *b);
+/* forces(B,A) of every system; measure as for mdx_forces */
+int mdx_batch_forces(mdx_batch_t *b, const double *r, double *f, mdx_out_t *out,
+                     int measure);
+/* MD state r, r1 of all systems (as mdx_upload_state per system) */
+int mdx_batch_upload_state(mdx_batch_t *b, const double *r, const double *r1);
+int mdx_batch_download_state(mdx_batch_t *b, double *r, double *r1);
+/* nsteps leap-frog steps of all systems; p[nsys] per system (h, T, tau,
+   nf, thermostat are used; measure_every of p[0] for all); out[nsys]
+   filled at the last measured step (frc, En_kin, Pkin, lambda).
+   Unmeasured steps are replayed from one hipGraph of the whole batch step
+   (mdx_batch_set_graph, default on), re-captured when p changes. */
+int mdx_batch_md_steps(mdx_batch_t *b, const mdx_step_t *p, int nsteps,
+                       mdx_step_out_t *out);
+int mdx_batch_set_graph(mdx_batch_t *b, int on);
+/* cell-stencil half width w (1..4) of all systems (cells >= cutoff/w) */
+int mdx_batch_set_stencil(mdx_batch_t *b, int w);
+/* times the force call on r for w = 1..3 and keeps the fastest: *w, *ms */
+int mdx_batch_autotune(mdx_batch_t *b, const double *r, int *w, double *ms);
+/* iv[0..n) = systems, sites, padded sites, cells, k items, mode, w */
+int mdx_batch_info(mdx_batch_t *b, int *iv, int n);
+
 /* ---- pressure assembly and cutoff correction (host arithmetic) -------- */
 
 /* measurePconstraints() (sim/constrd.c:115-265) for PRESSURETENSOR=7: all

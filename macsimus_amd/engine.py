@@ -181,7 +181,114 @@ class Engine:
         return out
 
 
+class Batch:
+    """Batched replicas (mdx_batch_*): independent atomic systems of uneven
+    size run by one launch per kernel.  Site arrays are lists of [ns_s, 3]
+    arrays or their concatenation."""
+
+    def __init__(self, built_systems, mode=abi.MDX_MIXED, lib=None):
+        self.lib = lib or abi.load_engine()
+        self.bss = list(built_systems)
+        self.nsys = len(self.bss)
+        self.ns = [b.struct.ns for b in self.bss]
+        self.off = np.concatenate([[0], np.cumsum(self.ns)]).astype(int)
+        arr = (abi.System * self.nsys)(*[b.struct for b in self.bss])
+        self._arr = arr
+        self.h = C.c_void_p()
+        rc = self.lib.mdx_batch_create(C.byref(self.h), self.nsys, arr, mode)
+        abi.check(self.lib, rc, "mdx_batch_create")
+        self.mode = mode
+
+    def close(self):
+        if self.h:
+            self.lib.mdx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _cat(self, rs):
+        if isinstance(rs, (list, tuple)):
+            rs = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in rs])
+        rs = np.ascontiguousarray(rs, dtype=np.float64).reshape(-1, 3)
+        assert rs.shape[0] == self.off[-1]
+        return rs
+
+    def split(self, a):
+        return [a[self.off[s]:self.off[s + 1]] for s in range(self.nsys)]
+
+    def forces(self, rs, measure=1):
+        """per-system forces (list) and mdx_out_t array"""
+        r = self._cat(rs)
+        f = np.empty_like(r)
+        out = (abi.Out * self.nsys)()
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_batch_forces(self.h, r.ctypes.data_as(dp), f.ctypes.data_as(dp), out,
+                                       int(measure))
+        abi.check(self.lib, rc, "mdx_batch_forces")
+        return self.split(f), list(out)
+
+    def upload_state(self, rs, r1s):
+        r, r1 = self._cat(rs), self._cat(r1s)
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_batch_upload_state(self.h, r.ctypes.data_as(dp), r1.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_batch_upload_state")
+
+    def download_state(self):
+        r = np.empty((self.off[-1], 3))
+        r1 = np.empty_like(r)
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_batch_download_state(self.h, r.ctypes.data_as(dp), r1.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_batch_download_state")
+        return self.split(r), self.split(r1)
+
+    def md_steps(self, params, nsteps):
+        """params: list of abi.Step (one per system) or one for all"""
+        if isinstance(params, abi.Step):
+            params = [params] * self.nsys
+        p = (abi.Step * self.nsys)(*params)
+        out = (abi.StepOut * self.nsys)()
+        rc = self.lib.mdx_batch_md_steps(self.h, p, int(nsteps), out)
+        abi.check(self.lib, rc, "mdx_batch_md_steps")
+        return list(out)
+
+    def set_graph(self, on=True):
+        abi.check(self.lib, self.lib.mdx_batch_set_graph(self.h, int(on)), "mdx_batch_set_graph")
+
+    def set_stencil(self, w):
+        abi.check(self.lib, self.lib.mdx_batch_set_stencil(self.h, int(w)), "mdx_batch_set_stencil")
+
+    def autotune(self, rs):
+        r = self._cat(rs)
+        w = C.c_int(0)
+        ms = C.c_double(0)
+        rc = self.lib.mdx_batch_autotune(self.h, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                         C.byref(w), C.byref(ms))
+        abi.check(self.lib, rc, "mdx_batch_autotune")
+        return w.value, ms.value
+
+    def info(self):
+        iv = (C.c_int * 7)()
+        self.lib.mdx_batch_info(self.h, iv, 7)
+        return dict(zip(("nsys", "sites", "padded", "cells", "kitems", "mode", "stencil"), list(iv)))
+
+
 def declare_extras(lib):
+    dp = C.POINTER(C.c_double)
+    lib.mdx_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(abi.System), C.c_int]
+    lib.mdx_batch_destroy.argtypes = [C.c_void_p]
+    lib.mdx_batch_forces.argtypes = [C.c_void_p, dp, dp, C.POINTER(abi.Out), C.c_int]
+    lib.mdx_batch_upload_state.argtypes = [C.c_void_p, dp, dp]
+    lib.mdx_batch_download_state.argtypes = [C.c_void_p, dp, dp]
+    lib.mdx_batch_md_steps.argtypes = [C.c_void_p, C.POINTER(abi.Step), C.c_int,
+                                       C.POINTER(abi.StepOut)]
+    lib.mdx_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_batch_set_stencil.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_batch_autotune.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int), dp]
+    lib.mdx_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     lib.mdx_cook_cells.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.mdx_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     lib.mdx_set_timing.argtypes = [C.c_void_p, C.c_int]

@@ -123,6 +123,82 @@ def water_c1():
     return dict(name="spce_3000", ff=ff, counts=counts, L=31.04, cutoff=9.0, r=r)
 
 
+# reduced LJ units of the replica batch: sigma = 1 A, epsilon = 1 K, m = 1
+# program unit, so that the LJ time unit is 1 ps, the reduced temperature
+# is T in K and a reduced charge q* is q* program units (q* / e in e)
+MUNIT_GMOL = 0.831446261815324
+EUNIT_KCAL = 4184.0 / 8.31446261815324
+
+
+def _lj_type(sigma, eps_K):
+    return sigma * TWO16 / 2, -eps_K / EUNIT_KCAL
+
+
+def replica_forcefield(kind):
+    """kind "salt": a restricted-primitive-like binary melt, both species
+    sigma = 1, epsilon = 1, reduced charges q* = +-2 (Bjerrum length
+    4/T* sigma); kind "ljmix": an 80:20
+    LJ mixture A (sigma 1, eps 1) + B (sigma 0.88, eps 0.5), no charges
+    (Lorentz-Berthelot)."""
+    from .system import ELECTRON
+    if kind == "salt":
+        types, par, q = ["P", "M"], [(1.0, 1.0), (1.0, 1.0)], [2.0 / ELECTRON, -2.0 / ELECTRON]
+    else:
+        types, par, q = ["A", "B"], [(1.0, 1.0), (0.88, 0.5)], [0.0, 0.0]
+    R = [_lj_type(*p)[0] for p in par]
+    E = [_lj_type(*p)[1] for p in par]
+    species = [dict(name=t, site_type=[i], site_charge_e=[q[i]], site_xyz=[[0, 0, 0]],
+                    site_excl=[[]], bonds=[]) for i, t in enumerate(types)]
+    return dict(types=types, type_mass=[MUNIT_GMOL] * 2, type_EvdW_kcal=E, type_RvdW=R,
+                species=species)
+
+
+def replicas_c4(nsys=256, seed=4):
+    """The replica batch: nsys independent atomic systems, even ones
+    binary salt melts, odd ones LJ mixtures; N ~ U[300, 6000] (even for
+    the melts), reduced density U[0.3, 0.9], reduced temperature
+    U[0.7, 2.0], cubic boxes, cutoff 2.5 sigma, Ewald at 1e-5 (absolute
+    force error in epsilon/sigma), random placement on a lattice and
+    Maxwell-Boltzmann velocities (zero total momentum), all from one
+    seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nsys):
+        kind = "salt" if s % 2 == 0 else "ljmix"
+        N = int(rng.integers(300, 6001))
+        rho = float(rng.uniform(0.3, 0.9))
+        T = float(rng.uniform(0.7, 2.0))
+        if kind == "salt":
+            N += N % 2
+            counts = [N // 2, N // 2]
+        else:
+            nb = int(round(0.2 * N))
+            counts = [N - nb, nb]
+        ff = replica_forcefield(kind)
+        L = (N / rho) ** (1 / 3)
+        nlat = int(np.ceil(N ** (1 / 3) - 1e-9))
+        r = lattice_config(ff, counts, L, nlat, seed=int(rng.integers(1 << 30)))
+        v = rng.normal(size=(N, 3)) * np.sqrt(T)      # m = 1, k_B = 1
+        v -= v.mean(0)
+        out.append(dict(name="replica%03d" % s, kind=kind, ff=ff, counts=counts, L=L,
+                        cutoff=2.5, r=r, v=v, T=T, rho=rho))
+    return out
+
+
+def build_replica(sysd, provider=None):
+    """mdx_system_t of one replica (charged: alpha, kappa from the error
+    estimate of Ewaldparm at 1e-5; neutral: the same formulas cannot be
+    applied, alpha = 3.5/cutoff and a kappa that gives no k-vectors are
+    passed, the k-space sum is empty)."""
+    from .system import build_system
+    L = [sysd["L"]] * 3
+    if sysd["kind"] == "salt":
+        return build_system(sysd["ff"], sysd["counts"], L, sysd["cutoff"], epsr=1e-5, epsk=1e-5,
+                            provider=provider)
+    return build_system(sysd["ff"], sysd["counts"], L, sysd["cutoff"], alpha=3.5 / sysd["cutoff"],
+                        kappa=0.5 / sysd["L"], provider=provider)
+
+
 def degrees_of_freedom(ff, counts):
     """No.f: 3 per site minus constraints minus 3 (total momentum)."""
     nf = 0
