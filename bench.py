@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--workload", choices=["c2", "c1", "c3"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4"], default="c2",
                     help="c2: 194,304-site SPC/E+NaCl; c1: 1000 SPC/E; c3: Drude-polarizable "
                          "brine, 97,200 sites + 32,600 Drudes, ASPC k=4 with one SCF iteration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,12 +108,113 @@ def cpu_baseline(wl, bs_oracle, kstride, threads):
     return t_force, dict(r_space_s=t[0], k_space_sample_s=t[1], k_charges=int(t[2]), nq=nq)
 
 
+def bench_replicas(a):
+    """The replica batch: 256 atomic systems (salt melts with Ewald, LJ
+    mixtures) of 300-6000 atoms, one batch step = every system's force call
+    and leap-frog + Berendsen step, h = 0.005 LJ time units (5 fs with
+    sigma = 1 A, epsilon = 1 K, m = 1 program unit).  value = aggregate
+    ns/day (all systems advance h per batch step)."""
+    from macsimus_amd import abi, workloads
+    from macsimus_amd.engine import Batch
+
+    sy = workloads.replicas_c4()
+    bss = [workloads.build_replica(sd) for sd in sy]
+    mode = abi.MDX_MIXED if a.mode == "mixed" else abi.MDX_FP64
+    b = Batch(bss, mode)
+    rs = [sd["r"] for sd in sy]
+    w, kzc, kparts, _, tune_ms = b.autotune(rs)
+    info = b.info()
+    h = 0.005
+    ps = [abi.Step(h=h, eps=1e-8, maxit=0, thermostat=1, T=sd["T"], tau=0.1,
+                   nf=3.0 * sum(sd["counts"]) - 3, measure_every=0) for sd in sy]
+    fs, outs0 = b.forces(rs, measure=1)
+    b.upload_state(rs, [sd["v"] * h for sd in sy])
+    if a.warmup:
+        for p in ps:
+            p.measure_every = a.warmup
+        b.md_steps(ps, a.warmup)
+    for p in ps:
+        p.measure_every = a.steps
+    t0 = time.perf_counter()
+    outs = b.md_steps(ps, a.steps)
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    ksteps = max(2, min(a.steps, 10))
+    b.set_timing(True)
+    for p in ps:
+        p.measure_every = ksteps
+    b.md_steps(ps, ksteps)
+    tm = b.timing()
+    b.set_timing(False)
+    kern = {k: (v[0] / max(v[1], 1), v[1]) for k, v in tm.items() if v[1]}
+    nsys = len(sy)
+    ns_day = nsys * 86400.0 / (ms_step * 1e-3) * h * 1e-3
+    npair = sum(o.npair for o in outs0)
+    nlj = sum(o.nlj for o in outs0)
+    qk = sum(int(np.count_nonzero(bs.info["charge"])) * bs.info["nk"] for bs in bss)
+    work = {"kpass1": 8.0 * qk, "kpass2": 12.0 * qk, "pair": 24.0 * npair + 7.0 * nlj}
+    dom = max(("kpass1", "kpass2", "pair"), key=lambda k: kern.get(k, (0, 0))[0])
+    peak = PEAK["fp64_valu"] if mode == abi.MDX_FP64 else PEAK["fp32_valu"]
+    achieved = work[dom] / (kern[dom][0] * 1e-3)
+    force_ms = sum(kern[k][0] for k in ("bin", "pair", "kpass1", "kcoef", "kpass2") if k in kern)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        # oracle force calls of every 16th system, one after another, scaled
+        # to the whole batch (the CPU runs the systems in turn)
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        prov = oracle.provider()
+        stride = 16
+        t_sample = 0.0
+        for k in range(0, nsys, stride):
+            bso = workloads.build_replica(sy[k], provider=prov)
+            tc = time.perf_counter()
+            oracle.forces(bso, sy[k]["r"], 0, nthreads=threads)
+            t_sample += time.perf_counter() - tc
+        t_batch = t_sample * stride
+        cpu = {"value": nsys * 86400.0 / t_batch * h * 1e-3, "unit": "ns/day (aggregate)",
+               "cores": threads, "kind": "port",
+               "sample": "oracle/ FP64 restatement, one force call of each of %d systems (every "
+                         "%dth, %.2f s), scaled x%d to the batch; leap-frog cost not included"
+                         % (len(range(0, nsys, stride)), stride, t_sample, stride),
+               "s_per_batch_force": t_batch}
+
+    line = {
+        "metric": "aggregate ns/day of 256 batched LJ + Ewald replicas",
+        "value": ns_day, "unit": "ns/day (aggregate)", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32" if mode == abi.MDX_MIXED else "f64",
+        "data": "synthetic",
+        "config": {"workload": "replicas_256", "systems": nsys, "sites": info["sites"],
+                   "padded_sites": info["padded"], "cells": info["cells"],
+                   "charges": info["charges"], "k_segments": info["kseg"], "pairs": npair,
+                   "timestep_ps": h, "precision": a.mode, "parallelism": "1 GPU",
+                   "tuned": {"stencil_w": w, "kz_chunk": kzc, "k_parts": kparts,
+                             "force_ms": tune_ms}},
+        "us_per_force": force_ms * 1e3,
+        "replica_steps_per_s": nsys / (ms_step * 1e-3),
+        "atom_steps_per_s": info["sites"] / (ms_step * 1e-3),
+        "kernel_ms": {k: v[0] for k, v in kern.items()},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "work_per_launch": work[dom]},
+        "cpu_baseline": cpu,
+        "energy_K": {"En_pot_sum": sum(o.frc.En_pot + o.frc.En_el for o in outs),
+                     "En_kin_sum": sum(o.En_kin for o in outs)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c4":
+        return bench_replicas(a)
     from macsimus_amd import abi, workloads
     from macsimus_amd.engine import Engine
     from macsimus_amd.system import build_system

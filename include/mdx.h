@@ -263,8 +263,8 @@ int mdx_adjoint_lj(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
    mdx_batch_create: sys[nsys] as for mdx_create (cubic boxes; nmol == ns,
    ncons == 0, chargepol == NULL, masses required; ncell is ignored: each
    system gets cells of at least cutoff/w, w = 1 until mdx_batch_set_stencil
-   or mdx_batch_autotune).  mode: the r-space precision as in mdx_create;
-   the batch k-space is FP64 in both modes.
+   or mdx_batch_autotune).  mode: as in mdx_create (the k-space passes
+   in FP32 with FP64 accumulation in MDX_MIXED).
    Site arrays of the batch calls are the systems' [ns*3] arrays one after
    another (system order).  out[nsys] per system. */
 typedef struct mdx_batch_s mdx_batch_t;
@@ -286,9 +286,21 @@ int mdx_batch_md_steps(mdx_batch_t *b, const mdx_step_t *p, int nsteps,
 int mdx_batch_set_graph(mdx_batch_t *b, int on);
 /* cell-stencil half width w (1..4) of all systems (cells >= cutoff/w) */
 int mdx_batch_set_stencil(mdx_batch_t *b, int w);
-/* times the force call on r for w = 1..3 and keeps the fastest: *w, *ms */
-int mdx_batch_autotune(mdx_batch_t *b, const double *r, int *w, double *ms);
-/* iv[0..n) = systems, sites, padded sites, cells, k items, mode, w */
+/* k-space work split: kz chunk kzc (8, 16, 24, 32) of the force pass and
+   about kparts parts (1..256) for the largest k-set, the others in
+   proportion to their k-sets */
+int mdx_batch_set_kspace(mdx_batch_t *b, int kzc, int kparts);
+/* times the force call on r (HIP events): the stencil w = 1..3, then kz
+   chunk x parts; keeps the fastest; choice[3] = {w, kz chunk, parts},
+   ms[2] = force-call times after each stage (either may be NULL) */
+int mdx_batch_autotune(mdx_batch_t *b, const double *r, int *choice, double *ms);
+/* HIP-event timing of the batch kernel families, in the slots of
+   mdx_get_timing (bin, pair, -, kpass1, -, kpass2, leap-frog); steps run
+   eagerly while timing is on */
+int mdx_batch_set_timing(mdx_batch_t *b, int on);
+int mdx_batch_get_timing(mdx_batch_t *b, double *ms, int *count, int n);
+/* iv[0..n) = systems, sites, padded sites, cells, charged systems, mode,
+   w, charges, k segments, pass-1 units, pass-2 units, kz chunk, parts */
 int mdx_batch_info(mdx_batch_t *b, int *iv, int n);
 
 /* ---- pressure assembly and cutoff correction (host arithmetic) -------- */

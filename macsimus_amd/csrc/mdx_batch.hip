@@ -7,19 +7,15 @@
 //
 // Layout.  Sites of system s occupy [pad_off_s, pad_off_s + ns_s) of the
 // padded arrays, pad_off a multiple of 64, so that a wave of the per-site
-// MD and k-space kernels never holds two systems (padding slots have type
-// -1 and charge 0).  The cells of all systems form one counting sort; the
-// sorted sites of a system are contiguous.  k-space: per system the
-// half-space k-vectors of cook's Ewald(0) set (sim/ewald.c:494-534),
-// (kx,ky) rows cut into items of up to 16 kz, items padded per system to a
-// multiple of 64.
+// MD kernels never holds two systems (padding slots have type -1 and
+// charge 0).  The cells of all systems form one counting sort; the sorted
+// sites of a system are contiguous.
 //
-// The physics is the single-system engine's (r-space: cook/ljqqnp.c with
-// the spline of sim/erfc.c; k-space: Ewald(2) of sim/ewald.c in the direct
-// form E = 1/(4 pi) sum_half e(k) |S(k)|^2, e(k) = 4 exp(-pi^2 k^2/alpha^2)
-// /(V k^2), F_j = sum_half e(k) q_j k Im(S(k)^* exp(2 pi i k.r_j)); leap-frog
-// of sim/shakev3.c and the Berendsen scaling of sim/thermo.c).  Sums per
-// system are segmented reductions in a fixed order.
+// The physics is the single-system engine's: r-space cook/ljqqnp.c with
+// the spline of sim/erfc.c, k-space Ewald(2) of sim/ewald.c in the T
+// formulation (mdx_bkspace.hip), leap-frog of sim/shakev3.c and the
+// Berendsen scaling of sim/thermo.c.  Sums per system are segmented
+// reductions in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,62 +23,23 @@
 #include <cstring>
 #include <vector>
 
-#include "mdx_internal.h"
+#include "mdx_batch.h"
 
 using namespace mdx;
+using namespace mdx::batch;
 
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWS = 64;      // padding of systems and k items
-constexpr int kKB = 16;      // kz per k item
 constexpr double kPi = 3.14159265358979323846;
-
-// per-system accumulators
-enum {
-  B_U = 0, B_EEL = 1, B_VIR = 2, B_PV = 3, B_NPAIR = 9, B_NLJ = 10,   // pair kernel (11)
-  B_EK = 11, B_PK = 12,                                                // k-space (7)
-  B_M = 18,                                                            // dipole (3)
-  B_KIN = 21, B_PKIN = 22,                                             // leap-frog (7)
-  B_N = 28
-};
 constexpr int kNE = 11;   // per-site pair sums
 
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
-
-struct DSys {
-  double L, iL, cq, sgrid, alpha, cdip;
-  int nc, w, cell_off, ncell, lj_off, nt, erf_off, pad_off, ns, item_off, nitem;
-};
-
-struct Batch {
-  int mode = MDX_MIXED, nsys = 0, npad = 0, nreal = 0, ncell = 0, nitem = 0, device = 0;
-  hipStream_t stream = nullptr;
-  std::vector<DSys> hs;
-  std::vector<double> sumq2;
-  DBuf<char> dsys;
-  DBuf<double> r, f, r1, q, mass;        // padded
-  DBuf<int> type, sysid;                 // padded
-  DBuf<int> cellid, cnt, cstart, cursor, sidx, scell, sinfo;
-  DBuf<double> spos;
-  DBuf<double> lj64, erf64;
-  DBuf<float> lj32, erf32;
-  DBuf<int> items, item_sys;             // [nitem*4] kx ky kz0 nkz, [nitem]
-  DBuf<double> C;                        // [nitem*kKB*2] e(k) S(k)
-  DBuf<double> epart, kpart, sacc, lam;  // per sorted site, per item, per system
-  DBuf<double> stepp;                    // per system h, T, tau, nf, thermostat
-  bool state_on_device = false, use_graph = true, graph_ok = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  std::vector<mdx_step_t> graph_p;
-};
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
-
-__device__ inline void cs2pi(double t, double &c, double &s) { sincospi(2 * (t - floor(t)), &s, &c); }
 
 __global__ void k_bfill(long n, double *__restrict__ x, double v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -335,135 +292,6 @@ __global__ void k_bsum_sites(const DSys *__restrict__ sy, const int *__restrict_
   }
 }
 
-// ---------------------------------------------------------------- k-space
-// pass 1: one lane per k item (a row (kx, ky) and up to 16 consecutive kz);
-// the wave's items belong to one system, whose charged sites it walks in
-// step.  Stores C = e(k) S(k) for pass 2 and, measuring, the item's energy
-// and tensor sums.
-template <bool MEASURE>
-__global__ __launch_bounds__(kBlock) void k_bsfac(
-    int nitem, const DSys *__restrict__ sy, const int4 *__restrict__ items,
-    const int *__restrict__ item_sys, const double *__restrict__ r, const double *__restrict__ q,
-    double *__restrict__ C, double *__restrict__ kpart) {
-  const int it = blockIdx.x * kBlock + threadIdx.x;
-  if (it >= nitem) return;
-  const int4 row = items[it];
-  const DSys S = sy[item_sys[it]];
-  double Sr[kKB], Si[kKB];
-#pragma unroll
-  for (int k = 0; k < kKB; k++) { Sr[k] = 0; Si[k] = 0; }
-  if (row.w > 0) {
-    for (int i = S.pad_off; i < S.pad_off + S.ns; i++) {
-      const double qi = q[i];
-      if (qi == 0) continue;
-      const double x = r[3 * i] * S.iL, y = r[3 * i + 1] * S.iL, z = r[3 * i + 2] * S.iL;
-      double c, sn, cz, sz;
-      cs2pi(row.x * x + row.y * y + row.z * z, c, sn);
-      cs2pi(z, cz, sz);
-#pragma unroll
-      for (int k = 0; k < kKB; k++) {
-        Sr[k] += qi * c;
-        Si[k] += qi * sn;
-        const double nc = c * cz - sn * sz;
-        sn = c * sz + sn * cz;
-        c = nc;
-      }
-    }
-  }
-  double E = 0, pk[6] = {0, 0, 0, 0, 0, 0};
-  const double V = S.L * S.L * S.L, pa = (kPi / S.alpha) * (kPi / S.alpha);
-  for (int k = 0; k < kKB; k++) {
-    double e = 0;
-    if (k < row.w) {
-      const double mx = row.x * S.iL, my = row.y * S.iL, mz = (row.z + k) * S.iL;
-      const double kk = mx * mx + my * my + mz * mz;
-      e = 4 / (kk * V) * exp(-pa * kk);
-      if (MEASURE) {
-        const double E1 = e * (Sr[k] * Sr[k] + Si[k] * Si[k]), ptf = 1 / kk + pa;
-        E += E1;
-        pk[0] += E1 * ptf * mx * mx; pk[1] += E1 * ptf * my * my; pk[2] += E1 * ptf * mz * mz;
-        pk[3] += E1 * ptf * my * mz; pk[4] += E1 * ptf * mx * mz; pk[5] += E1 * ptf * mx * my;
-      }
-    }
-    C[((size_t)it * kKB + k) * 2] = e * Sr[k];
-    C[((size_t)it * kKB + k) * 2 + 1] = e * Si[k];
-  }
-  if (MEASURE) {
-    double *o = kpart + (size_t)it * 7;
-    o[0] = E;
-    for (int k = 0; k < 6; k++) o[1 + k] = pk[k];
-  }
-}
-
-// per system: the k-space sums of its items and the dipole M = sum q r
-__global__ void k_bsum_k(const DSys *__restrict__ sy, const double *__restrict__ kpart,
-                         const double *__restrict__ r, const double *__restrict__ q, int measure,
-                         double *__restrict__ sacc) {
-  __shared__ double red[kBlock];
-  const DSys S = sy[blockIdx.x];
-  for (int a = 0; a < 10; a++) {
-    if (a < 7 && !measure) continue;
-    double v = 0;
-    if (a < 7) {
-      for (int it = S.item_off + threadIdx.x; it < S.item_off + S.nitem; it += kBlock)
-        v += kpart[(size_t)it * 7 + a];
-    } else {
-      for (int i = S.pad_off + threadIdx.x; i < S.pad_off + S.ns; i += kBlock)
-        v += q[i] * r[3 * i + (a - 7)];
-    }
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) sacc[(size_t)blockIdx.x * B_N + (a < 7 ? B_EK + a : B_M + a - 7)] = red[0];
-    __syncthreads();
-  }
-}
-
-// pass 2: one lane per padded site (a wave holds one system); walks the
-// system's items in step: F_j += q_j sum_k C(k)-weighted k Im(S^* e^{ik.r}),
-// then the dipole field -q_j cdip M
-__global__ __launch_bounds__(kBlock) void k_bkforce(
-    int npad, const DSys *__restrict__ sy, const int *__restrict__ sysid,
-    const int4 *__restrict__ items, const double *__restrict__ C, const double *__restrict__ r,
-    const double *__restrict__ q, const double *__restrict__ sacc, double *__restrict__ f) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= npad) return;
-  const DSys S = sy[sysid[i]];
-  const double qi = q[i];
-  if (S.nitem == 0 || qi == 0) return;
-  const double x = r[3 * i] * S.iL, y = r[3 * i + 1] * S.iL, z = r[3 * i + 2] * S.iL;
-  double cz, sz;
-  cs2pi(z, cz, sz);
-  double Fx = 0, Fy = 0, Fz = 0;
-  for (int it = S.item_off; it < S.item_off + S.nitem; it++) {
-    const int4 row = items[it];
-    if (row.w == 0) continue;
-    double c, sn;
-    cs2pi(row.x * x + row.y * y + row.z * z, c, sn);
-    double vs = 0, vz = 0;
-    const double *Ck = C + (size_t)it * kKB * 2;
-    for (int k = 0; k < row.w; k++) {
-      // Im(conj(e S) e^{i phi}) = Re(eS) sin - Im(eS) cos
-      const double v = Ck[2 * k] * sn - Ck[2 * k + 1] * c;
-      vs += v;
-      vz += v * (row.z + k);
-      const double nc = c * cz - sn * sz;
-      sn = c * sz + sn * cz;
-      c = nc;
-    }
-    Fx += vs * row.x;
-    Fy += vs * row.y;
-    Fz += vz;
-  }
-  const double *M = sacc + (size_t)sysid[i] * B_N + B_M;
-  f[3 * i] += qi * (Fx * S.iL - S.cdip * M[0]);
-  f[3 * i + 1] += qi * (Fy * S.iL - S.cdip * M[1]);
-  f[3 * i + 2] += qi * (Fz * S.iL - S.cdip * M[2]);
-}
-
 // ---------------------------------------------------------------- MD
 // leap-frog of one site (sim/shakev3.c for a molecule without constraints)
 // and per-wave kinetic sums; a wave holds one system
@@ -545,6 +373,7 @@ __global__ void k_bscale(int npad, const int *__restrict__ sysid, const double *
 // ---------------------------------------------------------------- host
 int forces_launch(Batch &b, int measure) {
   const DSys *sy = (const DSys *)b.dsys.p;
+  if (b.timing) b.tm.start(T_BIN, b.stream);
   k_bzero_int<<<nblk(b.ncell), kBlock, 0, b.stream>>>(b.cnt.p, b.ncell);
   k_bcell<<<nblk(b.npad), kBlock, 0, b.stream>>>(b.npad, b.r.p, b.type.p, b.sysid.p, sy, b.cellid.p,
                                                   b.cnt.p);
@@ -560,6 +389,7 @@ int forces_launch(Batch &b, int measure) {
   k_bfill<<<nblk(3L * b.npad), kBlock, 0, b.stream>>>(3L * b.npad, b.f.p, 0.0);
   const double4 *sp = (const double4 *)b.spos.p;
   const int4 *si = (const int4 *)b.sinfo.p;
+  if (b.timing) { b.tm.stop(T_BIN, b.stream); b.tm.start(T_PAIR, b.stream); }
   if (b.mode == MDX_FP64) {
     if (measure)
       k_bpair<double, true><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
@@ -576,20 +406,9 @@ int forces_launch(Batch &b, int measure) {
           b.nreal, sy, sp, si, b.scell.p, b.cstart.p, b.lj32.p, b.erf32.p, b.f.p, b.epart.p);
   }
   k_bsum_sites<<<b.nsys, kBlock, 0, b.stream>>>(sy, b.cstart.p, b.epart.p, b.sacc.p);
-  if (b.nitem) {
-    if (measure)
-      k_bsfac<true><<<nblk(b.nitem), kBlock, 0, b.stream>>>(b.nitem, sy, (const int4 *)b.items.p,
-                                                             b.item_sys.p, b.r.p, b.q.p, b.C.p,
-                                                             b.kpart.p);
-    else
-      k_bsfac<false><<<nblk(b.nitem), kBlock, 0, b.stream>>>(b.nitem, sy, (const int4 *)b.items.p,
-                                                              b.item_sys.p, b.r.p, b.q.p, b.C.p,
-                                                              b.kpart.p);
-  }
-  k_bsum_k<<<b.nsys, kBlock, 0, b.stream>>>(sy, b.kpart.p, b.r.p, b.q.p, measure, b.sacc.p);
-  if (b.nitem)
-    k_bkforce<<<nblk(b.npad), kBlock, 0, b.stream>>>(b.npad, sy, b.sysid.p, (const int4 *)b.items.p,
-                                                      b.C.p, b.r.p, b.q.p, b.sacc.p, b.f.p);
+  if (b.timing) b.tm.stop(T_PAIR, b.stream);
+  int rc;
+  if ((rc = bk_forces(b, measure))) return rc;
   if (hipGetLastError() != hipSuccess) return fail(MDX_E_GPU, "batch: launch failed");
   return 0;
 }
@@ -599,7 +418,6 @@ void fill_sys_out(const Batch &b, int s, const double *a, mdx_out_t *out) {
   out->npair = (long long)std::llround(a[B_NPAIR]);
   out->nlj = (long long)std::llround(a[B_NLJ]);
   const DSys &S = b.hs[s];
-  const double V = S.L * S.L * S.L;
   const double Ek = a[B_EK] / (4 * kPi);
   const double *M = a + B_M;
   const double Edip = S.cdip / 2 * (M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
@@ -608,12 +426,11 @@ void fill_sys_out(const Batch &b, int s, const double *a, mdx_out_t *out) {
   for (int i = 0; i < 3; i++) Pk[i] += Ek + Edip;
   Pk[0] -= S.cdip * M[0] * M[0]; Pk[1] -= S.cdip * M[1] * M[1]; Pk[2] -= S.cdip * M[2] * M[2];
   Pk[3] -= S.cdip * M[1] * M[2]; Pk[4] -= S.cdip * M[2] * M[0]; Pk[5] -= S.cdip * M[0] * M[1];
-  out->En_kspace = S.nitem ? Ek + Edip - S.alpha / std::sqrt(kPi) * b.sumq2[s] : 0.0;
+  out->En_kspace = S.ks >= 0 ? Ek + Edip - S.alpha / std::sqrt(kPi) * b.sumq2[s] : 0.0;
   out->En_el = out->En_kspace + a[B_EEL];
   out->En_pot = a[B_U];
   out->En_vir = a[B_VIR];
-  for (int i = 0; i < 6; i++) out->Pvir[i] = (S.nitem ? Pk[i] : 0.0) + a[B_PV + i];
-  (void)V;
+  for (int i = 0; i < 6; i++) out->Pvir[i] = (S.ks >= 0 ? Pk[i] : 0.0) + a[B_PV + i];
 }
 
 int choose_cells(DSys &S, int w) {
@@ -641,17 +458,19 @@ int layout_cells(Batch &b, int w) {
 }
 
 void release(Batch &b) {
-  DBuf<double> *d[] = {&b.r, &b.f, &b.r1, &b.q, &b.mass, &b.spos, &b.lj64, &b.erf64, &b.C,
-                       &b.epart, &b.kpart, &b.sacc, &b.lam, &b.stepp};
+  DBuf<double> *d[] = {&b.r, &b.f, &b.r1, &b.q, &b.mass, &b.spos, &b.lj64, &b.erf64,
+                       &b.epart, &b.sacc, &b.lam, &b.stepp};
   for (auto *x : d) x->release();
   DBuf<int> *n[] = {&b.type, &b.sysid, &b.cellid, &b.cnt, &b.cstart, &b.cursor, &b.sidx, &b.scell,
-                    &b.sinfo, &b.items, &b.item_sys};
+                    &b.sinfo};
   for (auto *x : n) x->release();
   b.lj32.release();
   b.erf32.release();
   b.dsys.release();
+  b.k.release();
   if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.tm.release();
   if (b.stream) (void)hipStreamDestroy(b.stream);
 }
 
@@ -679,11 +498,10 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   // host layout
   std::vector<double> lj, erf;
   std::vector<float> lj32, erf32;
-  std::vector<int4> items;
-  std::vector<int> item_sys;
-  int pad = 0, nit = 0;
+  int pad = 0;
   b.hs.resize(nsys);
   b.sumq2.assign(nsys, 0.0);
+  b.kappa.assign(nsys, 0.0);
   for (int s = 0; s < nsys; s++) {
     const mdx_system_t &y = sys[s];
     if (y.ns <= 0 || y.nmol != y.ns || y.chargepol || y.ncons != 0 || !y.mass || !y.charge ||
@@ -723,35 +541,12 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
     S.pad_off = pad;
     S.ns = y.ns;
     pad += (y.ns + kWS - 1) / kWS * kWS;
-    // k items: half-space set of Ewald(0) (sim/ewald.c:494-534), rows of up
-    // to 16 kz, padded per system to a multiple of 64
-    S.item_off = nit;
-    if (b.sumq2[s] > 0) {
-      if (!(y.kappa > 0)) return bail(fail(MDX_E_ARG, "mdx_batch_create: charged system needs kappa"));
-      const double k2 = y.kappa * y.kappa, L = S.L;
-      const int kxm = (int)(L * y.kappa);
-      for (int kx = 0; kx <= kxm; kx++) {
-        const double rx = k2 - (kx / L) * (kx / L);
-        if (rx < 0) continue;
-        const int kym = (int)(L * std::sqrt(rx));
-        for (int ky = kx ? -kym : 0; ky <= kym; ky++) {
-          const double ry = rx - (ky / L) * (ky / L);
-          if (ry < 0) continue;
-          const int kzm = (int)(L * std::sqrt(ry));
-          const int lo = (kx == 0 && ky == 0) ? 1 : -kzm;
-          for (int z0 = lo; z0 <= kzm; z0 += kKB) {
-            items.push_back(make_int4(kx, ky, z0, std::min(kKB, kzm - z0 + 1)));
-            item_sys.push_back(s);
-          }
-        }
-      }
-      while (items.size() % kWS) { items.push_back(make_int4(0, 0, 0, 0)); item_sys.push_back(s); }
-    }
-    S.nitem = (int)items.size() - nit;
-    nit = (int)items.size();
+    S.ks = -1;
+    if (b.sumq2[s] > 0 && !(y.kappa > 0))
+      return bail(fail(MDX_E_ARG, "mdx_batch_create: charged system needs kappa"));
+    b.kappa[s] = y.kappa;
   }
   b.npad = pad;
-  b.nitem = nit;
   lj32.assign(lj.begin(), lj.end());
   // per-site padded data
   std::vector<double> q(pad, 0.0), mass(pad, 1.0);
@@ -778,30 +573,25 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
       (rc = b.scell.alloc(std::max(nreal, pad))) || (rc = b.sinfo.alloc(4L * nreal)) ||
       (rc = b.spos.alloc(4L * nreal)) || (rc = b.lj64.alloc(lj.size())) || (rc = b.lj32.alloc(lj.size())) ||
       (rc = b.erf64.alloc(erf.size())) || (rc = b.erf32.alloc(erf32.size())) ||
-      (rc = b.items.alloc(4L * std::max(nit, 1))) || (rc = b.item_sys.alloc(std::max(nit, 1))) ||
-      (rc = b.C.alloc(2L * kKB * std::max(nit, 1))) || (rc = b.epart.alloc((size_t)kNE * nreal)) ||
-      (rc = b.kpart.alloc(7L * std::max(nit, 1))) || (rc = b.sacc.alloc((size_t)B_N * nsys)) ||
+      (rc = b.epart.alloc((size_t)kNE * nreal)) || (rc = b.sacc.alloc((size_t)B_N * nsys)) ||
       (rc = b.lam.alloc(nsys)) || (rc = b.stepp.alloc(5L * nsys)))
     return bail(rc);
   auto up = [&](void *d, const void *src, size_t n) {
-    return hipMemcpy(d, src, n, hipMemcpyHostToDevice) == hipSuccess;
+    return n == 0 || hipMemcpy(d, src, n, hipMemcpyHostToDevice) == hipSuccess;
   };
   if (!up(b.q.p, q.data(), sizeof(double) * pad) || !up(b.mass.p, mass.data(), sizeof(double) * pad) ||
       !up(b.type.p, type.data(), sizeof(int) * pad) || !up(b.sysid.p, sysid.data(), sizeof(int) * pad) ||
       !up(b.lj64.p, lj.data(), sizeof(double) * lj.size()) ||
       !up(b.lj32.p, lj32.data(), sizeof(float) * lj32.size()) ||
       !up(b.erf64.p, erf.data(), sizeof(double) * erf.size()) ||
-      !up(b.erf32.p, erf32.data(), sizeof(float) * erf32.size()) ||
-      (nit && !up(b.items.p, items.data(), sizeof(int4) * nit)) ||
-      (nit && !up(b.item_sys.p, item_sys.data(), sizeof(int) * nit)))
+      !up(b.erf32.p, erf32.data(), sizeof(float) * erf32.size()))
     return bail(fail(MDX_E_GPU, "mdx_batch_create: upload failed"));
   // zero state and sums (kernels: the step may be captured later)
   k_bfill<<<nblk(3L * pad), kBlock, 0, b.stream>>>(3L * pad, b.r.p, 0.0);
   k_bfill<<<nblk(3L * pad), kBlock, 0, b.stream>>>(3L * pad, b.r1.p, 0.0);
   k_bfill<<<nblk((long)B_N * nsys), kBlock, 0, b.stream>>>((long)B_N * nsys, b.sacc.p, 0.0);
-  k_bfill<<<nblk(7L * std::max(nit, 1)), kBlock, 0, b.stream>>>(7L * std::max(nit, 1), b.kpart.p, 0.0);
   if (hipStreamSynchronize(b.stream) != hipSuccess) return bail(fail(MDX_E_GPU, "mdx_batch_create: init"));
-  if ((rc = layout_cells(b, 1))) return bail(rc);
+  if ((rc = layout_cells(b, 1)) || (rc = bk_setup(b, 16, 8))) return bail(rc);
   *out = h;
   return 0;
 }
@@ -889,13 +679,13 @@ namespace {
 int step_launch(Batch &b, int measure) {
   int rc;
   if ((rc = forces_launch(b, measure))) return rc;
-  const int nw = b.npad / kWS;
+  if (b.timing) b.tm.start(T_SHAKE, b.stream);
   k_bleap<<<nblk(b.npad), kBlock, 0, b.stream>>>(b.npad, b.type.p, b.sysid.p, b.stepp.p, b.mass.p,
                                                   b.r.p, b.r1.p, b.f.p, b.epart.p);
-  (void)nw;
   k_bsum_kin<<<b.nsys, kBlock, 0, b.stream>>>((const DSys *)b.dsys.p, b.epart.p, b.stepp.p,
                                               b.sacc.p, b.lam.p);
   k_bscale<<<nblk(b.npad), kBlock, 0, b.stream>>>(b.npad, b.sysid.p, b.lam.p, b.r1.p);
+  if (b.timing) b.tm.stop(T_SHAKE, b.stream);
   if (hipGetLastError() != hipSuccess) return fail(MDX_E_GPU, "batch: launch failed");
   return 0;
 }
@@ -925,7 +715,7 @@ extern "C" int mdx_batch_md_steps(mdx_batch_t *h, const mdx_step_t *p, int nstep
   std::vector<double> a((size_t)B_N * b.nsys), lam(b.nsys);
   for (int s = 1; s <= nsteps && !rc; s++) {
     const int measure = (s % every == 0) || s == nsteps;
-    if (!measure && b.use_graph) {
+    if (!measure && b.use_graph && !b.timing) {
       bool same = b.graph_ok && b.graph_p.size() == (size_t)b.nsys;
       for (int k = 0; same && k < b.nsys; k++)
         same = b.graph_p[k].h == p[k].h && b.graph_p[k].T == p[k].T && b.graph_p[k].tau == p[k].tau &&
@@ -980,14 +770,55 @@ extern "C" int mdx_batch_set_graph(mdx_batch_t *h, int on) {
   return 0;
 }
 
+// HIP-event timing of the kernel families on the batch stream (bin, pair,
+// -, kpass1, -, kpass2, leap-frog in the slots of mdx_get_timing)
+extern "C" int mdx_batch_set_timing(mdx_batch_t *h, int on) {
+  if (!h) return fail(MDX_E_ARG, "mdx_batch_set_timing: NULL");
+  h->b.timing = on != 0;
+  return 0;
+}
+
+extern "C" int mdx_batch_get_timing(mdx_batch_t *h, double *ms, int *count, int n) {
+  if (!h || !ms || !count || n < T_NFAM) return fail(MDX_E_ARG, "mdx_batch_get_timing: bad argument");
+  MDX_HIP(hipStreamSynchronize(h->b.stream));
+  h->b.tm.collect(ms, count);
+  return 0;
+}
+
 extern "C" int mdx_batch_set_stencil(mdx_batch_t *h, int w) {
   if (!h || w < 1 || w > 4) return fail(MDX_E_ARG, "mdx_batch_set_stencil: w in 1..4");
   MDX_HIP(hipSetDevice(h->b.device));
   return layout_cells(h->b, w);
 }
 
-// the batch's stencil w (cells >= cutoff/w) with the fastest force call on r
-extern "C" int mdx_batch_autotune(mdx_batch_t *h, const double *r, int *w_out, double *ms) {
+extern "C" int mdx_batch_set_kspace(mdx_batch_t *h, int kzc, int kparts) {
+  if (!h) return fail(MDX_E_ARG, "mdx_batch_set_kspace: NULL");
+  MDX_HIP(hipSetDevice(h->b.device));
+  MDX_HIP(hipStreamSynchronize(h->b.stream));
+  return bk_setup(h->b, kzc, kparts);
+}
+
+namespace {
+// mean time of 3 force calls (after one untimed) on the device positions
+int time_forces(Batch &b, hipEvent_t e0, hipEvent_t e1, double *ms) {
+  int rc;
+  if ((rc = forces_launch(b, 0))) return rc;
+  MDX_HIP(hipEventRecord(e0, b.stream));
+  for (int k = 0; k < 3; k++)
+    if ((rc = forces_launch(b, 0))) return rc;
+  MDX_HIP(hipEventRecord(e1, b.stream));
+  MDX_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MDX_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / 3;
+  return 0;
+}
+}  // namespace
+
+// the stencil w (cells >= cutoff/w, 1..3), then the k-space kz chunk
+// (8..32) and parts (1..32) with the fastest force call on r;
+// choice = {w, kz chunk, parts}, ms = {after the stencil, after the k-space}
+extern "C" int mdx_batch_autotune(mdx_batch_t *h, const double *r, int *choice, double *ms) {
   if (!h || !r) return fail(MDX_E_ARG, "mdx_batch_autotune: NULL argument");
   Batch &b = h->b;
   MDX_HIP(hipSetDevice(b.device));
@@ -996,34 +827,40 @@ extern "C" int mdx_batch_autotune(mdx_batch_t *h, const double *r, int *w_out, d
   if ((rc = up_rows(b, b.r.p, r))) return rc;
   hipEvent_t e0, e1;
   MDX_HIP(hipEventCreate(&e0));
-  MDX_HIP(hipEventCreate(&e1));
-  double best = 1e30;
-  int bw = 1;
-  for (int w = 1; w <= 3 && !rc; w++) {
-    if ((rc = layout_cells(b, w))) break;
-    if ((rc = forces_launch(b, 0))) break;
-    (void)hipEventRecord(e0, b.stream);
-    for (int k = 0; k < 3 && !rc; k++) rc = forces_launch(b, 0);
-    (void)hipEventRecord(e1, b.stream);
-    (void)hipEventSynchronize(e1);
-    float t = 0;
-    (void)hipEventElapsedTime(&t, e0, e1);
-    if (t / 3 < best) { best = t / 3; bw = w; }
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(MDX_E_GPU, "mdx_batch_autotune: event");
   }
+  double best = 1e30, t = 0, best_k = 1e30;
+  int bw = 1, bz = b.k.kzc, bp = b.k.kparts;
+  for (int w = 1; w <= 3 && !rc; w++) {
+    if ((rc = layout_cells(b, w)) || (rc = time_forces(b, e0, e1, &t))) break;
+    if (t < best) { best = t; bw = w; }
+  }
+  if (!rc) rc = layout_cells(b, bw);
+  const int kzs[4] = {8, 16, 24, 32}, kps[6] = {1, 2, 4, 8, 16, 32};
+  for (int iz = 0; iz < 4 && !rc && b.k.nks; iz++)
+    for (int ip = 0; ip < 6 && !rc; ip++) {
+      if ((rc = bk_setup(b, kzs[iz], kps[ip])) || (rc = time_forces(b, e0, e1, &t))) break;
+      if (t < best_k) { best_k = t; bz = kzs[iz]; bp = kps[ip]; }
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if ((rc = layout_cells(b, bw))) return rc;
-  if (w_out) *w_out = bw;
-  if (ms) *ms = best;
+  if ((rc = bk_setup(b, bz, bp))) return rc;
+  if (choice) { choice[0] = bw; choice[1] = bz; choice[2] = bp; }
+  if (ms) { ms[0] = best; ms[1] = b.k.nks ? best_k : best; }
   return 0;
 }
 
-// iv: systems, sites, padded sites, cells, k items, mode, w
+// iv: systems, sites, padded sites, cells, charged systems, mode, w,
+// charges, k segments, pass-1 units, pass-2 units, kz chunk, parts asked
 extern "C" int mdx_batch_info(mdx_batch_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_batch_info: NULL");
   const Batch &b = h->b;
-  const int v[7] = {b.nsys, b.nreal, b.npad, b.ncell, b.nitem, b.mode, b.hs.empty() ? 0 : b.hs[0].w};
-  for (int i = 0; i < n && i < 7; i++) iv[i] = v[i];
+  const BK &k = b.k;
+  const int v[13] = {b.nsys, b.nreal, b.npad, b.ncell, k.nks, b.mode, b.hs.empty() ? 0 : b.hs[0].w,
+                     k.nqt, k.nseg, k.n1, k.n2, k.kzc, k.kparts};
+  for (int i = 0; i < n && i < 13; i++) iv[i] = v[i];
   return 0;
 }

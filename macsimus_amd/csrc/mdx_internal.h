@@ -225,6 +225,13 @@ int md_setup(Ctx &c);
 int bin_device(Ctx &c);
 // kernel LJ table [nt2][Sq E4 E24 E48 C1q C2q A A4]
 std::vector<double> lj_table(const mdx_lj_t *sstab, int nt2);
+// k-vector rows of Ewald(0) for box L and kappa (host tables only)
+void klayout_build(const double L[3], double kappa, KLayout &K);
+// pass-2 tables for kz chunk kzc and about `parts` parts (host): per chunk
+// the largest ky of each kx reaching it (kym [nchunk][kxmax+1]) and the
+// parts {chunk, kx begin, kx end, 0}
+int kchunks_build(const KLayout &K, int kzc, int parts, std::vector<int> &kym,
+                  std::vector<int> &parts_out, int &nchunk);
 // set-up of the k-space layout for the context box and kappa
 int kspace_setup(Ctx &c);
 // pass-2 tables for kz chunk kzc (8..64, multiple of 8) and about

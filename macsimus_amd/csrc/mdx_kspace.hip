@@ -478,22 +478,20 @@ namespace mdx {
 // rows (kx,ky), kx in [0,(int)(L0 kappa)], ky in [0,kymax(kx)], with kz
 // from !(kx+ky) to kzmax(kx,ky).  Rows are sorted by kz limit (pass 1
 // segments) and also kept in lexicographic order (pass 2).
-int kspace_setup(Ctx &c) {
-  KLayout &K = c.K;
+void klayout_build(const double L[3], double kappa, KLayout &K) {
   K = KLayout();
-  const double k2 = c.kappa * c.kappa;
-  K.kxmax = (int)(c.L[0] * c.kappa);
-  std::vector<int> lx, ly, lz, lkymax(K.kxmax + 1), lrow0(K.kxmax + 1);
+  const double k2 = kappa * kappa;
+  K.kxmax = (int)(L[0] * kappa);
+  std::vector<int> lx, ly, lz, lrow0(K.kxmax + 1);
   for (int kx = 0; kx <= K.kxmax; kx++) {
-    const double rx = k2 - (kx / c.L[0]) * (kx / c.L[0]);
-    const int kym = (int)(c.L[1] * std::sqrt(rx));
-    lkymax[kx] = kym;
+    const double rx = k2 - (kx / L[0]) * (kx / L[0]);
+    const int kym = (int)(L[1] * std::sqrt(rx));
     lrow0[kx] = (int)lx.size();
     for (int ky = 0; ky <= kym; ky++) {
-      const double ry = rx - (ky / c.L[1]) * (ky / c.L[1]);
+      const double ry = rx - (ky / L[1]) * (ky / L[1]);
       lx.push_back(kx);
       ly.push_back(ky);
-      lz.push_back((int)(c.L[2] * std::sqrt(ry)));
+      lz.push_back((int)(L[2] * std::sqrt(ry)));
     }
   }
   K.nrows = (int)lx.size();
@@ -539,6 +537,11 @@ int kspace_setup(Ctx &c) {
     K.lex_meta[2 * i + 1] = lcoef[i];
     K.lex_maxblk = std::max(K.lex_maxblk, lnb[i]);
   }
+}
+
+int kspace_setup(Ctx &c) {
+  KLayout &K = c.K;
+  klayout_build(c.L, c.kappa, K);
   // charge splits: about 8 workgroups per CU, at least 256 charges each
   const int wg = (K.nseg + 3) / 4;
   c.nsplit = std::max(1, std::min(2048 / std::max(wg, 1), c.nq / 256));
@@ -550,7 +553,6 @@ int kspace_setup(Ctx &c) {
   std::vector<int> kx_all = K.row_kx, ky_all = K.row_ky;
   std::vector<int> kz_all = K.row_kzmax, c0_all = K.row_coef0;
   (void)cat;
-  (void)lkymax;
   int rc;
   if ((rc = c.d_row_kx.alloc(kx_all.size())) || (rc = c.d_row_ky.alloc(ky_all.size())) ||
       (rc = c.d_row_kzmax.alloc(kz_all.size())) || (rc = c.d_row_coef0.alloc(c0_all.size())) ||
@@ -586,13 +588,13 @@ int kspace_setup(Ctx &c) {
 // parts (chunk, kx begin, kx end) of about equal numbers of coefficient
 // blocks (a row costs its blocks plus about two blocks of phase and
 // force arithmetic).
-int kspace_chunks(Ctx &c, int kzc, int parts) {
-  const KLayout &K = c.K;
+int kchunks_build(const KLayout &K, int kzc, int parts, std::vector<int> &kym,
+                  std::vector<int> &ptrim, int &nchunk_out) {
   if (kzc < 8 || kzc > 64 || kzc % 8) return fail(MDX_E_ARG, "k-space: kz chunk in {8,16,...,64}");
   if (parts < 1 || parts > 256) return fail(MDX_E_ARG, "k-space: parts in 1..256");
   const int bpc = kzc / kKB, nkx = K.kxmax + 1;
   const int nchunk = std::max(1, (K.lex_maxblk + bpc - 1) / bpc);
-  std::vector<int> kym((size_t)nchunk * nkx, -1);
+  kym.assign((size_t)nchunk * nkx, -1);
   std::vector<double> work((size_t)nchunk * nkx, 0);
   double total = 0;
   for (int ch = 0; ch < nchunk; ch++)
@@ -631,7 +633,7 @@ int kspace_chunks(Ctx &c, int kzc, int parts) {
   }
   // drop the tails of kx without rows in the chunk (kym < 0): kernels skip
   // nothing inside a part, so trim each part to rows that exist
-  std::vector<int> ptrim;
+  ptrim.clear();
   for (size_t i = 0; i < pt.size(); i += 4) {
     int ch = pt[i], a = pt[i + 1], b = pt[i + 2];
     while (b > a && kym[(size_t)ch * nkx + b - 1] < 0) b--;
@@ -640,8 +642,15 @@ int kspace_chunks(Ctx &c, int kzc, int parts) {
       if (kym[(size_t)ch * nkx + kx] < 0) return fail(MDX_E_ARG, "k-space: chunk rows are not a prefix in kx");
     if (b > a) ptrim.insert(ptrim.end(), {ch, a, b, 0});
   }
+  nchunk_out = nchunk;
+  return 0;
+}
+
+int kspace_chunks(Ctx &c, int kzc, int parts) {
+  std::vector<int> kym, ptrim;
+  int nchunk = 0, rc;
+  if ((rc = kchunks_build(c.K, kzc, parts, kym, ptrim, nchunk))) return rc;
   const int np = (int)ptrim.size() / 4;
-  int rc;
   if ((rc = c.d_chunk_kymax.alloc(kym.size())) || (rc = c.d_kparts.alloc(ptrim.size())) ||
       (rc = c.kpf.alloc((size_t)np * c.nq * 3)))
     return rc;

@@ -261,19 +261,36 @@ class Batch:
     def set_stencil(self, w):
         abi.check(self.lib, self.lib.mdx_batch_set_stencil(self.h, int(w)), "mdx_batch_set_stencil")
 
+    def set_kspace(self, kzc, kparts):
+        abi.check(self.lib, self.lib.mdx_batch_set_kspace(self.h, int(kzc), int(kparts)),
+                  "mdx_batch_set_kspace")
+
     def autotune(self, rs):
+        """fastest stencil, k-space kz chunk and parts on rs: (w, kzc,
+        kparts, ms after the stencil, ms after the k-space)"""
         r = self._cat(rs)
-        w = C.c_int(0)
-        ms = C.c_double(0)
-        rc = self.lib.mdx_batch_autotune(self.h, r.ctypes.data_as(C.POINTER(C.c_double)),
-                                         C.byref(w), C.byref(ms))
+        ch = (C.c_int * 3)()
+        ms = (C.c_double * 2)()
+        rc = self.lib.mdx_batch_autotune(self.h, r.ctypes.data_as(C.POINTER(C.c_double)), ch, ms)
         abi.check(self.lib, rc, "mdx_batch_autotune")
-        return w.value, ms.value
+        return ch[0], ch[1], ch[2], ms[0], ms[1]
+
+    def set_timing(self, on=True):
+        abi.check(self.lib, self.lib.mdx_batch_set_timing(self.h, int(on)), "mdx_batch_set_timing")
+
+    def timing(self):
+        """{family: (total ms, launches)} since the last call (HIP events)"""
+        ms = (C.c_double * 7)()
+        cnt = (C.c_int * 7)()
+        rc = self.lib.mdx_batch_get_timing(self.h, ms, cnt, 7)
+        abi.check(self.lib, rc, "mdx_batch_get_timing")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(Engine.FAMILIES)}
 
     def info(self):
-        iv = (C.c_int * 7)()
-        self.lib.mdx_batch_info(self.h, iv, 7)
-        return dict(zip(("nsys", "sites", "padded", "cells", "kitems", "mode", "stencil"), list(iv)))
+        iv = (C.c_int * 13)()
+        self.lib.mdx_batch_info(self.h, iv, 13)
+        return dict(zip(("nsys", "sites", "padded", "cells", "charged", "mode", "stencil",
+                         "charges", "kseg", "units1", "units2", "kchunk", "kparts"), list(iv)))
 
 
 def declare_extras(lib):
@@ -288,7 +305,11 @@ def declare_extras(lib):
     lib.mdx_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_batch_set_stencil.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_batch_autotune.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int), dp]
+    lib.mdx_batch_set_kspace.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
+    lib.mdx_batch_set_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_batch_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.c_int]
     lib.mdx_cook_cells.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.mdx_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     lib.mdx_set_timing.argtypes = [C.c_void_p, C.c_int]

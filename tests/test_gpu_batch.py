@@ -120,8 +120,32 @@ def test_batch_stencils_agree():
         for f, f0, o, o0 in zip(fs, ref[0], outs, ref[1]):
             assert o.npair == o0.npair and o.nlj == o0.nlj
             assert rel_rms(f, f0) < 1e-12
-    w, ms = b.autotune(rs)
-    assert 1 <= w <= 3 and ms > 0
+    w, kzc, kparts, ms1, ms2 = b.autotune(rs)
+    assert 1 <= w <= 3 and kzc in (8, 16, 24, 32) and 1 <= kparts <= 32 and 0 < ms2 <= ms1 * 1.5
+    info = b.info()
+    assert (info["stencil"], info["kchunk"], info["kparts"]) == (w, kzc, kparts)
+    fs, outs = b.forces(rs, 1)
+    for f, f0, o, o0 in zip(fs, ref[0], outs, ref[1]):
+        assert o.npair == o0.npair and rel_rms(f, f0) < 1e-12
+    b.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_batch_kspace_splits_agree(mode):
+    """every kz chunk and work split of the k-space gives the same forces
+    and energies (FP64: to rounding; mixed: to its bar)"""
+    sy = systems(6)
+    b = Batch([bs for _, bs in sy], mode)
+    rs = [sd["r"] for sd, _ in sy]
+    ref = b.forces(rs, 1)
+    tol = 1e-12 if mode == abi.MDX_FP64 else 1e-6
+    for kzc in (8, 16, 24, 32):
+        for kparts in (1, 3, 32):
+            b.set_kspace(kzc, kparts)
+            fs, outs = b.forces(rs, 1)
+            for f, f0, o, o0 in zip(fs, ref[0], outs, ref[1]):
+                assert rel_rms(f, f0) < tol, (kzc, kparts)
+                assert abs(o.En_el - o0.En_el) <= tol * abs(o0.En_el) + 1e-12
     b.close()
 
 
@@ -196,3 +220,44 @@ def test_batch_rejects_molecular_systems():
     bs = oracle.fixture_system(meta, provider_=engine_provider())
     with pytest.raises(RuntimeError):
         Batch([bs], abi.MDX_FP64)
+
+
+def test_batch_full_256_systems():
+    """the whole 256-system replica batch (828,442 atoms): mixed against
+    FP64 for every system (1e-5 relative RMS force error, equal census),
+    the three largest systems against the oracle, then 20 graph-replayed
+    MD steps with finite energies and Berendsen factors near 1"""
+    systems(0)
+    allsys = _SYS["all"]
+    prov = engine_provider()
+    bss = [workloads.build_replica(sd, provider=prov) for sd in allsys]
+    rs = [sd["r"] for sd in allsys]
+    b64 = Batch(bss, abi.MDX_FP64)
+    f64, o64 = b64.forces(rs, 1)
+    b64.close()
+    bm = Batch(bss, abi.MDX_MIXED)
+    fm, om = bm.forces(rs, 1)
+    worst = 0.0
+    for k in range(len(allsys)):
+        e = rel_rms(fm[k], f64[k])
+        worst = max(worst, e)
+        assert e < 1e-5, (k, e)
+        assert om[k].npair == o64[k].npair
+        assert rel(om[k].En_pot, o64[k].En_pot) < 1e-5
+    print("256 systems: worst mixed-vs-FP64 rel rms %.2e, pairs %d"
+          % (worst, sum(o.npair for o in o64)), flush=True)
+    big = sorted(range(len(allsys)), key=lambda k: -bss[k].struct.ns)[:3]
+    for k in big:
+        fo, oo = oracle.forces(bss[k], rs[k], 1)
+        e = rel_rms(f64[k], fo)
+        print("%s ns %d FP64 vs oracle %.2e" % (allsys[k]["name"], bss[k].struct.ns, e), flush=True)
+        assert e < 1e-10
+        check_out(o64[k], oo, 1e-10, allsys[k]["name"])
+    h = 0.005
+    ps = [make_step(sd) for sd in allsys]
+    bm.upload_state(rs, [sd["v"] * h for sd in allsys])
+    outs = bm.md_steps(ps, 20)
+    for k, o in enumerate(outs):
+        assert np.isfinite(o.frc.En_pot) and np.isfinite(o.En_kin)
+        assert 0.9 < o.lambda_ < 1.1, (k, o.lambda_)
+    bm.close()
