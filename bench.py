@@ -42,7 +42,7 @@ def parse():
                     help="CPU baseline: k-space over every n-th charge (0 = auto)")
     ap.add_argument("--pin", default="",
                     help="W,PCOLS,KZC,KPARTS: stencil, stencil columns per pair part, k-space kz "
-                         "chunk and parts instead of the autotuner")
+                         "chunk and parts instead of the autotuner (c4: W,KZC,KPARTS)")
     ap.add_argument("--forces-only", type=int, default=0,
                     help="run this many force calls (no MD, no JSON line), for counter passes")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -80,9 +80,11 @@ def traffic_lookup(workload, mode, kernel, info):
     for e in doc["kernels"]:
         if (e["workload"], e["mode"], e["kernel"]) != (workload, mode, kernel):
             continue
-        if kernel.startswith("kpass") and (e["kz_chunk"], e["k_parts"]) != (info["kchunk"], info["kparts_asked"]):
+        if kernel.startswith("kpass") and (e["kz_chunk"], e["k_parts"]) != (
+                info["kchunk"], info.get("kparts_asked", info.get("kparts"))):
             continue
-        if kernel == "pair" and (e["stencil_w"], e["pair_cols_per_part"]) != (info["stencil"], info["pair_cols"]):
+        if kernel == "pair" and (e["stencil_w"], e.get("pair_cols_per_part", 0)) != (
+                info["stencil"], info.get("pair_cols", 0)):
             continue
         return e["bytes"]
     return None
@@ -122,8 +124,24 @@ def bench_replicas(a):
     mode = abi.MDX_MIXED if a.mode == "mixed" else abi.MDX_FP64
     b = Batch(bss, mode)
     rs = [sd["r"] for sd in sy]
-    w, kzc, kparts, _, tune_ms = b.autotune(rs)
+    if a.pin:
+        w, kzc, kparts = (int(x) for x in a.pin.split(","))
+        b.set_stencil(w)
+        b.set_kspace(kzc, kparts)
+        tune_ms = None
+    else:
+        # the batch autotuner: stencil, then k-space kz chunk and parts
+        w, kzc, kparts, _, tune_ms = b.autotune(rs)
     info = b.info()
+    if a.forces_only:
+        for _ in range(a.forces_only):
+            fs, outs = b.forces(rs, measure=0)
+        print("forces-only: %d calls, pairs %d, config %s" % (a.forces_only, sum(o.npair for o in outs),
+                                                            (w, kzc, kparts)), flush=True)
+        if a.copy_calibration:
+            copy_calibration()
+            print("copy-calibration: 1 GiB device copy done", flush=True)
+        return
     h = 0.005
     ps = [abi.Step(h=h, eps=1e-8, maxit=0, thermostat=1, T=sd["T"], tau=0.1,
                    nf=3.0 * sum(sd["counts"]) - 3, measure_every=0) for sd in sy]
@@ -199,7 +217,8 @@ def bench_replicas(a):
         "kernel_ms": {k: v[0] for k, v in kern.items()},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "work_per_launch": work[dom]},
+                     "traffic": traffic_lookup("replicas_256", a.mode, dom, info),
+                     "work_per_launch": work[dom]},
         "cpu_baseline": cpu,
         "energy_K": {"En_pot_sum": sum(o.frc.En_pot + o.frc.En_el for o in outs),
                      "En_kin_sum": sum(o.En_kin for o in outs)},

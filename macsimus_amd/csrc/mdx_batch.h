@@ -14,6 +14,7 @@ constexpr int kWS = 64;      // sites of a system are padded to whole waves
 struct DSys {
   double L, iL, cq, sgrid, alpha, cdip;
   int nc, w, cell_off, ncell, lj_off, nt, erf_off, pad_off, ns;
+  int srt_off;               // first row of its sites in cell order
   int ks;                    // its k-space descriptor, -1: no charges
 };
 

@@ -74,28 +74,34 @@ __global__ void k_bcell(int npad, const double *__restrict__ r, const int *__res
   atomicAdd(&cnt[c], 1);
 }
 
-__global__ void k_bscan(const int *__restrict__ cnt, int n, int *__restrict__ start,
-                        int *__restrict__ cursor) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x, per = (n + 1023) / 1024;
-  const int b = min(t * per, n), e = min(b + per, n);
-  int s = 0;
-  for (int i = b; i < e; i++) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = t >= o ? part[t - o] : 0;
+// per system (one workgroup each): exclusive scan of its cells' counts,
+// starting at the first sorted row of the system (all sites of a system
+// fall into its own cells); cursors to zero
+__global__ void k_bscan_sys(const DSys *__restrict__ sy, int nsys, int nreal,
+                            const int *__restrict__ cnt, int *__restrict__ start,
+                            int *__restrict__ cursor) {
+  __shared__ int part[kBlock];
+  const DSys S = sy[blockIdx.x];
+  int base = S.srt_off;
+  for (int c0 = 0; c0 < S.ncell; c0 += kBlock) {
+    const int c = c0 + threadIdx.x;
+    const int v = c < S.ncell ? cnt[S.cell_off + c] : 0;
+    part[threadIdx.x] = v;
     __syncthreads();
-    part[t] += v;
+    for (int o = 1; o < kBlock; o <<= 1) {
+      const int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (c < S.ncell) {
+      start[S.cell_off + c] = base + part[threadIdx.x] - v;
+      cursor[S.cell_off + c] = 0;
+    }
+    base += part[kBlock - 1];
     __syncthreads();
   }
-  s = t ? part[t - 1] : 0;
-  for (int i = b; i < e; i++) {
-    start[i] = s;
-    cursor[i] = 0;
-    s += cnt[i];
-  }
-  if (t == 1023) start[n] = part[1023];
+  if (blockIdx.x == nsys - 1 && threadIdx.x == 0) start[S.cell_off + S.ncell] = nreal;
 }
 
 __global__ void k_bscatter(int npad, const int *__restrict__ type, const int *__restrict__ cellid,
@@ -273,22 +279,27 @@ __global__ __launch_bounds__(kBlock) void k_bpair(
 
 // per-system sums of the per-site pair sums (sorted sites of a system are
 // contiguous), one workgroup per system, fixed order
-__global__ void k_bsum_sites(const DSys *__restrict__ sy, const int *__restrict__ cstart,
-                             const double *__restrict__ epart, double *__restrict__ sacc) {
-  __shared__ double red[kBlock];
+__global__ void k_bsum_sites(const DSys *__restrict__ sy, const double *__restrict__ epart,
+                             double *__restrict__ sacc) {
+  __shared__ double red[kBlock / 64][kNE];
   const DSys S = sy[blockIdx.x];
-  const int b = cstart[S.cell_off], e = cstart[S.cell_off + S.ncell];
+  double v[kNE];
+#pragma unroll
+  for (int k = 0; k < kNE; k++) v[k] = 0;
+  for (int j = S.srt_off + threadIdx.x; j < S.srt_off + S.ns; j += kBlock)
+#pragma unroll
+    for (int k = 0; k < kNE; k++) v[k] += epart[(size_t)j * kNE + k];
+  const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
   for (int k = 0; k < kNE; k++) {
-    double v = 0;
-    for (int j = b + threadIdx.x; j < e; j += kBlock) v += epart[(size_t)j * kNE + k];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) sacc[(size_t)blockIdx.x * B_N + k] = red[0];
-    __syncthreads();
+    const double t = wave_sum(v[k]);
+    if (ln == 0) red[wv][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNE) {
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; w++) t += red[w][threadIdx.x];
+    sacc[(size_t)blockIdx.x * B_N + threadIdx.x] = t;
   }
 }
 
@@ -377,7 +388,7 @@ int forces_launch(Batch &b, int measure) {
   k_bzero_int<<<nblk(b.ncell), kBlock, 0, b.stream>>>(b.cnt.p, b.ncell);
   k_bcell<<<nblk(b.npad), kBlock, 0, b.stream>>>(b.npad, b.r.p, b.type.p, b.sysid.p, sy, b.cellid.p,
                                                   b.cnt.p);
-  k_bscan<<<1, 1024, 0, b.stream>>>(b.cnt.p, b.ncell, b.cstart.p, b.cursor.p);
+  k_bscan_sys<<<b.nsys, kBlock, 0, b.stream>>>(sy, b.nsys, b.nreal, b.cnt.p, b.cstart.p, b.cursor.p);
   k_bscatter<<<nblk(b.npad), kBlock, 0, b.stream>>>(b.npad, b.type.p, b.cellid.p, b.cstart.p,
                                                      b.cursor.p, b.sidx.p);
   k_bsort<<<nblk(64L * b.ncell), kBlock, 0, b.stream>>>(b.ncell, b.cstart.p, b.sidx.p, b.scell.p);
@@ -405,7 +416,7 @@ int forces_launch(Batch &b, int measure) {
       k_bpair<float, false><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
           b.nreal, sy, sp, si, b.scell.p, b.cstart.p, b.lj32.p, b.erf32.p, b.f.p, b.epart.p);
   }
-  k_bsum_sites<<<b.nsys, kBlock, 0, b.stream>>>(sy, b.cstart.p, b.epart.p, b.sacc.p);
+  k_bsum_sites<<<b.nsys, kBlock, 0, b.stream>>>(sy, b.epart.p, b.sacc.p);
   if (b.timing) b.tm.stop(T_PAIR, b.stream);
   int rc;
   if ((rc = bk_forces(b, measure))) return rc;
@@ -498,7 +509,7 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   // host layout
   std::vector<double> lj, erf;
   std::vector<float> lj32, erf32;
-  int pad = 0;
+  int pad = 0, srt = 0;
   b.hs.resize(nsys);
   b.sumq2.assign(nsys, 0.0);
   b.kappa.assign(nsys, 0.0);
@@ -540,6 +551,8 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
     }
     S.pad_off = pad;
     S.ns = y.ns;
+    S.srt_off = srt;
+    srt += y.ns;
     pad += (y.ns + kWS - 1) / kWS * kWS;
     S.ks = -1;
     if (b.sumq2[s] > 0 && !(y.kappa > 0))

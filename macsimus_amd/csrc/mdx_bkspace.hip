@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_bk_sfac(
     const int4 *__restrict__ units, const KSys *__restrict__ ks, const double4 *__restrict__ qpos,
     const int *__restrict__ seg_blk, const int *__restrict__ seg_row0,
     const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
-    const int *__restrict__ row_ky, double *__restrict__ kpart) {
+    const int *__restrict__ row_ky, double *__restrict__ kpart, double *__restrict__ tsum) {
   extern __shared__ char smem[];
   const int4 u = units[blockIdx.x];
   const KSys K = ks[u.x];
@@ -153,7 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_bk_sfac(
     }
   }
   if (!segok) return;
-  double *out = kpart + ((K.kp_off + (size_t)u.z * K.nseg + seg) * 64 + lane) * 64;
+  // one split: the T sums themselves
+  double *out = K.nsplit == 1 ? tsum + (((size_t)K.seg_off + seg) * 64 + lane) * 64
+                              : kpart + ((K.kp_off + (size_t)u.z * K.nseg + seg) * 64 + lane) * 64;
 #pragma unroll
   for (int k = 0; k < 8; k++)
 #pragma unroll
@@ -161,12 +163,14 @@ __global__ __launch_bounds__(kBlock) void k_bk_sfac(
 }
 
 // tsum of a segment = sum of its system's charge splits, in split order
+// (systems of one split had pass 1 write tsum)
 __global__ void k_bk_ksum(long n, const KSys *__restrict__ ks, const int *__restrict__ seg_ks,
                           const double *__restrict__ kpart, double *__restrict__ tsum) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int gs = (int)(i >> 12), e = (int)(i & 4095);
   const KSys &K = ks[seg_ks[gs]];
+  if (K.nsplit == 1) return;
   const int ls = gs - K.seg_off;
   double s = 0;
   for (int p = 0; p < K.nsplit; p++) s += kpart[((size_t)K.kp_off + (size_t)p * K.nseg + ls) * 4096 + e];
@@ -410,7 +414,7 @@ int run(Batch &b, int measure) {
     const size_t shmem = sizeof(C2<R>) * kJC * (size_t)k.shmem_max;
     k_bk_sfac<R><<<k.n1, kBlock, shmem, b.stream>>>((const int4 *)k.units1.p, ks, qpos, k.seg_blk.p,
                                                     k.seg_row0.p, k.seg_rowend.p, k.row_kx.p,
-                                                    k.row_ky.p, k.kpart.p);
+                                                    k.row_ky.p, k.kpart.p, k.tsum.p);
   }
   if (b.timing) { b.tm.stop(T_KPASS1, b.stream); b.tm.start(T_KCOEF, b.stream); }
   R *coef = nullptr;
@@ -514,7 +518,7 @@ int bk_setup(Batch &b, int kzc, int kparts) {
     x.nsplit = std::max(1, std::min(64, (x.nq + kJMin - 1) / kJMin));
     x.jper = (x.nq + x.nsplit - 1) / x.nsplit;
     x.kp_off = kp_off;
-    kp_off += (long)x.nsplit * x.nseg;
+    if (x.nsplit > 1) kp_off += (long)x.nsplit * x.nseg;
     for (int r = 0; r < K.nrows; r++) {
       row_kx.push_back(K.row_kx[r]);
       row_ky.push_back(K.row_ky[r]);
