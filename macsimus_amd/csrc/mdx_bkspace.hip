@@ -292,6 +292,8 @@ __global__ void k_bk_sums(const DSys *__restrict__ sy, const KSys *__restrict__ 
 // kz chunk's cos/sin of c live in registers, rows (kx,ky) reaching the
 // chunk are walked in lexicographic order, the 16 coefficients of a triple
 // are wave-uniform (scalar loads).  Partial forces per part.
+// Units are listed system by system, charge block by charge block, so
+// that the units in flight share the coefficients of few systems.
 template <typename R, int KZC>
 __global__ __launch_bounds__(kBlock) void k_bk_kforce(
     const int4 *__restrict__ units, const KSys *__restrict__ ks, const double4 *__restrict__ qpos,
@@ -474,7 +476,7 @@ int bk_setup(Batch &b, int kzc, int kparts) {
   std::vector<KSys> hks;
   std::vector<int> qrow, qks, row_kx, row_ky, row_kzmax, row_coef0, seg_blk, seg_row0, seg_rowend,
       seg_ks, units1, units2, parts, lex_row0, lex_meta, chunk_kymax;
-  std::vector<double> qh;
+  std::vector<double> cost1;
   std::vector<KLayout> lay;
   // the k-sets first: the largest sets the parts of the others
   for (int s = 0; s < b.nsys; s++) {
@@ -487,7 +489,7 @@ int bk_setup(Batch &b, int kzc, int kparts) {
   }
   long ntrip_max = 1;
   for (const KLayout &K : lay) ntrip_max = std::max<long>(ntrip_max, K.ntrip);
-  int nrows = 0, nseg = 0, nunits2 = 0, maxlds = 0;
+  int nrows = 0, nseg = 0, maxlds = 0;
   long ncoef = 0, kp_off = 0, kpf_off = 0;
   std::vector<double> qcharge(b.npad);
   // charges of the padded rows (host copy)
@@ -532,7 +534,10 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       seg_ks.push_back((int)hks.size());
     }
     for (int g = 0; g < (K.nseg + 3) / 4; g++)
-      for (int p = 0; p < x.nsplit; p++) units1.insert(units1.end(), {(int)hks.size(), g, p, 0});
+      for (int p = 0; p < x.nsplit; p++) {
+        units1.insert(units1.end(), {(int)hks.size(), g, p, 0});
+        cost1.push_back((double)x.jper);
+      }
     // pass 2: lexicographic rows and chunk tables
     x.lexr_off = (int)lex_row0.size();
     for (int kx = 0; kx <= K.kxmax; kx++) lex_row0.push_back(K.lex_row0[kx] + nrows);
@@ -553,7 +558,9 @@ int bk_setup(Batch &b, int kzc, int kparts) {
     for (int p = 0; p < x.nkpart; p++)
       parts.insert(parts.end(), {pt[4 * p], pt[4 * p + 1], pt[4 * p + 2], p});
     for (int cb = 0; cb < (x.nq + kBlock - 1) / kBlock; cb++)
-      for (int p = 0; p < x.nkpart; p++) units2.insert(units2.end(), {(int)hks.size(), cb, p0 + p, 0});
+      for (int p = 0; p < x.nkpart; p++) {
+        units2.insert(units2.end(), {(int)hks.size(), cb, p0 + p, 0});
+      }
     x.kcb0 = 2 * nseg;
     x.kcb1 = 2 * (nseg + K.nseg);
     nrows += K.nrows;
@@ -562,7 +569,18 @@ int bk_setup(Batch &b, int kzc, int kparts) {
     S.ks = (int)hks.size();
     hks.push_back(x);
   }
-  (void)nunits2;
+  // pass-1 units longest first (stable: ties keep the system order), so
+  // that the short units of small systems fill the end of the launch
+  auto by_cost = [](std::vector<int> &u, const std::vector<double> &c) {
+    std::vector<int> ord(c.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return c[a] > c[b]; });
+    std::vector<int> out;
+    out.reserve(u.size());
+    for (int i : ord) out.insert(out.end(), u.begin() + 4 * i, u.begin() + 4 * i + 4);
+    u.swap(out);
+  };
+  by_cost(units1, cost1);
   k.release();
   k.hks = hks;
   k.nks = (int)hks.size();
