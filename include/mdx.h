@@ -284,7 +284,8 @@ int mdx_batch_download_state(mdx_batch_t *b, double *r, double *r1);
 int mdx_batch_md_steps(mdx_batch_t *b, const mdx_step_t *p, int nsteps,
                        mdx_step_out_t *out);
 int mdx_batch_set_graph(mdx_batch_t *b, int on);
-/* cell-stencil half width w (1..4) of all systems (cells >= cutoff/w) */
+/* cell-stencil half width w (1..4) of all systems (cells >= cutoff/w; a
+   system whose box is under (2w+1)/w cutoffs takes a wider one, up to 8) */
 int mdx_batch_set_stencil(mdx_batch_t *b, int w);
 /* k-space work split: kz chunk kzc (8, 16, 24, 32) of the force pass and
    about kparts parts (1..256) for the largest k-set, the others in

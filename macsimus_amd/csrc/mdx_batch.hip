@@ -444,8 +444,10 @@ void fill_sys_out(const Batch &b, int s, const double *a, mdx_out_t *out) {
   for (int i = 0; i < 6; i++) out->Pvir[i] = (S.ks >= 0 ? Pk[i] : 0.0) + a[B_PV + i];
 }
 
+// cells of edge >= cutoff/v with at least 2v+1 per side (no cell is met
+// twice by the stencil); boxes under 3 cutoffs take v above the asked w
 int choose_cells(DSys &S, int w) {
-  for (int v = w; v <= 4; v++) {
+  for (int v = w; v <= 8; v++) {
     const int nc = (int)std::floor(S.L * v / std::sqrt(S.cq));
     if (nc >= 2 * v + 1) { S.nc = nc; S.w = v; S.ncell = nc * nc * nc; return 0; }
   }
