@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--kstride", type=int, default=0,
                     help="CPU baseline: k-space over every n-th charge (0 = auto)")
     ap.add_argument("--pin", default="",
-                    help="W,PCOLS,KZC,KPARTS: stencil, stencil columns per pair part, k-space kz "
-                         "chunk and parts instead of the autotuner (c4: W,KZC,KPARTS)")
+                    help="W,PCOLS,KZC,KPARTS[,MFMA]: stencil, stencil columns per pair part, "
+                         "k-space kz chunk and parts, pass 2 on the matrix cores (mixed) instead "
+                         "of the autotuner (c4: W,KZC,KPARTS)")
     ap.add_argument("--forces-only", type=int, default=0,
                     help="run this many force calls (no MD, no JSON line), for counter passes")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -247,9 +248,12 @@ def main():
     mode = abi.MDX_MIXED if a.mode == "mixed" else abi.MDX_FP64
     eng = Engine(bs, mode)
     if a.pin:
-        w, pcols, kzc, kparts = (int(x) for x in a.pin.split(","))
+        pin = [int(x) for x in a.pin.split(",")]
+        w, pcols, kzc, kparts = pin[:4]
         eng.set_stencil(w, pcols)
         eng.set_kchunk(kzc, kparts)
+        if mode == abi.MDX_MIXED:
+            eng.set_kspace_mfma(len(pin) > 4 and pin[4] != 0)
         tune = (w, kzc, kparts, pcols)
     else:
         # the engine's autotuner picks the cell stencil, its split over
@@ -335,7 +339,7 @@ def main():
                    "kappa": bs.info["kappa"], "k_vectors": nk, "pairs": npair,
                    "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU",
                    "tuned": {"stencil_w": tune[0], "kz_chunk": tune[1], "k_parts": tune[2],
-                             "pair_cols_per_part": tune[3]}},
+                             "pair_cols_per_part": tune[3], "kspace_pass2_mfma": bool(info["kmfma"])}},
         "us_per_force": force_ms * 1e3,
         "kernel_ms": {k: v[0] for k, v in kern.items()},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,

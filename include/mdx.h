@@ -193,6 +193,12 @@ int mdx_md_steps(mdx_ctx_t *ctx, const mdx_step_t *p, int nsteps,
    configuration; on = 0 launches every step eagerly.  Kernel timing
    (mdx_set_timing) also runs eagerly.  Default: on. */
 int mdx_set_graph(mdx_ctx_t *ctx, int on);
+/* The k-space passes of a force call run on a second stream of the
+   context, beside binning, pair and exclusion kernels, and join before
+   their forces are added (fork/join inside a captured step too); on = 0
+   runs everything on one stream.  Results are the same bits either way.
+   Kernel timing (mdx_set_timing) runs on one stream.  Default: on. */
+int mdx_set_overlap(mdx_ctx_t *ctx, int on);
 
 /* ASPC state of a Drude system: predictor length k (0..8), length
    reached kk (0..k) and the older history aa[2..k+1] as k arrays of ns*3
@@ -225,7 +231,12 @@ int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
    parts, columns}, ms[2] = their times per call (either may be NULL).
    mdx_info: iv[0..n) = engine cells x,y,z, stencil w, charges, k rows,
    kz blocks, k segments, k triples, k-sum splits, kx max, mode, kz chunk,
-   kz chunks, parts asked, parts made, stencil columns per pair part. */
+   kz chunks, parts asked, parts made, stencil columns per pair part,
+   matrix-core pass 2 (0/1).
+   mdx_set_kspace_mfma (MDX_MIXED): k-space pass 2 on the matrix cores
+   (FP32 MFMA, FP32 products and sums as the FMA variant) or as FP32 FMA
+   chains; mdx_autotune tries both.  Default: on in MDX_MIXED. */
+int mdx_set_kspace_mfma(mdx_ctx_t *ctx, int on);
 int mdx_cook_cells(mdx_ctx_t *ctx, const double *r, int *cellid);
 int mdx_set_timing(mdx_ctx_t *ctx, int on);
 int mdx_get_timing(mdx_ctx_t *ctx, double *ms, int *count, int n);

@@ -672,8 +672,10 @@ int alloc_grid(Ctx &c) {
     return rc;
   const int np = pair_parts(c);
   c.gen++;                             // launch shapes and buffers change
+  // partial rows: one per workgroup of the pair, exclusion, k-space
+  // coefficient (2 per segment) and dipole kernels
   c.part_blocks = std::max(nblk(c.ns) * np, nblk(c.nmol));
-  c.part_blocks = std::max(c.part_blocks, 1024);
+  c.part_blocks = std::max({c.part_blocks, 1024, nblk(c.nq), 2 * c.K.nseg});
   if (np > 1 && (rc = c.pf.alloc((size_t)np * 3 * (c.polar ? 2 : 1) * c.ns))) return rc;
   return c.part.alloc((size_t)c.part_blocks * ACC_N);
 }
@@ -750,6 +752,16 @@ int forces_device(Ctx &c, int measure) {
   int rc;
   const size_t npart = (size_t)c.part_blocks * ACC_N;
   k_zero<<<nblk(npart), kBlock, 0, c.stream>>>(c.part.p, npart);
+  // k-space on its own stream, beside binning, pair and exclusions: it
+  // reads only the positions and charges, writes other columns of the
+  // partial rows and adds its forces after the join
+  const bool fork = c.nq && c.overlap && !c.timing;
+  if (fork) {
+    MDX_HIP(hipEventRecord(c.ev_fork, c.stream));
+    MDX_HIP(hipStreamWaitEvent(c.kstream, c.ev_fork, 0));
+    if ((rc = kspace_pass(c, measure, c.kstream))) return rc;
+    MDX_HIP(hipEventRecord(c.ev_join, c.kstream));
+  }
   if (c.timing) c.tm.start(T_BIN, c.stream);
   if ((rc = bin_sites(c, c.r.p))) return rc;
   if (c.timing) { c.tm.stop(T_BIN, c.stream); c.tm.start(T_PAIR, c.stream); }
@@ -775,7 +787,12 @@ int forces_device(Ctx &c, int measure) {
   }
   MDX_HIP(hipGetLastError());
   if (c.timing) c.tm.stop(T_EXCL, c.stream);
-  if (c.nq && (rc = kspace_forces(c, measure))) return rc;
+  if (fork) {
+    MDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    if ((rc = kspace_add(c, c.stream))) return rc;
+  } else if (c.nq) {
+    if ((rc = kspace_pass(c, measure, c.stream)) || (rc = kspace_add(c, c.stream))) return rc;
+  }
   k_reduce<<<ACC_N, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, c.acc.p);
   MDX_HIP(hipGetLastError());
   return 0;
@@ -863,8 +880,12 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     return code;
   };
   c.mode = mode;
+  c.kmfma = mode == MDX_MIXED;       // pass 2 on the matrix cores by default
   (void)hipGetDevice(&c.device);
-  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c.kstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) != hipSuccess)
     return bail(fail(MDX_E_GPU, "hipStreamCreate failed"));
   c.ns = s->ns;
   c.nmol = s->nmol;
@@ -989,9 +1010,16 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.lj32.release();
   c.erf32.release();
   c.coef32.release();
+  c.coefg.release();
+  c.d_grp_start.release();
+  c.d_grp_base.release();
+  c.d_grp_nb.release();
   c.tm.release();
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+  if (c.kstream) (void)hipStreamDestroy(c.kstream);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return 0;
@@ -1142,6 +1170,7 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
   };
   c.allpair = false;
   int rc = 0, best_w = c.wst, best_pc = c.pcols, best_k = c.kzc, best_p = c.kparts;
+  bool best_m = c.kmfma;
   double tw = 1e30, tk = 1e30, fam[T_NFAM];
   for (int w = 1; w <= 3 && !rc; w++) {
     if (!grid_fits(c, w)) continue;
@@ -1159,13 +1188,18 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
   if (c.nq) {
     const int cand[] = {8, 16, 24, 32, 48};
     const int pcand[] = {4, 8, 16, 32, 64};
-    for (int kzc : cand)
-      for (int parts : pcand) {
-        if (rc) break;
-        if (kzc > 8 * c.K.nblk + 8) continue;
-        if ((rc = kspace_chunks(c, kzc, parts)) || (rc = run(fam))) break;
-        if (fam[T_KPASS2] < tk) { tk = fam[T_KPASS2]; best_k = kzc; best_p = parts; }
-      }
+    // MDX_MIXED: the FP32 FMA and the matrix-core variants of pass 2
+    for (int mf = 0; mf < (c.mode == MDX_MIXED ? 2 : 1) && !rc; mf++) {
+      c.kmfma = mf != 0;
+      for (int kzc : cand)
+        for (int parts : pcand) {
+          if (rc) break;
+          if (kzc > 8 * c.K.nblk + 8) continue;
+          if ((rc = kspace_chunks(c, kzc, parts)) || (rc = run(fam))) break;
+          if (fam[T_KPASS2] < tk) { tk = fam[T_KPASS2]; best_k = kzc; best_p = parts; best_m = c.kmfma; }
+        }
+    }
+    c.kmfma = best_m;
     if (!rc) rc = kspace_chunks(c, best_k, best_p);
   }
   c.timing = timing0;
@@ -1175,12 +1209,20 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
   return 0;
 }
 
+extern "C" int mdx_set_kspace_mfma(mdx_ctx_t *h, int on) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_mfma: NULL");
+  if (on && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_mfma: MDX_MIXED only");
+  h->c.kmfma = on != 0;
+  h->c.gen++;
+  return 0;
+}
+
 extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
   Ctx &c = h->c;
-  int v[17] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+  int v[18] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
                c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart,
-               c.pcols};
-  for (int i = 0; i < n && i < 17; i++) iv[i] = v[i];
+               c.pcols, c.kmfma ? 1 : 0};
+  for (int i = 0; i < n && i < 18; i++) iv[i] = v[i];
   return 0;
 }

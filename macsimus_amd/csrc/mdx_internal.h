@@ -123,12 +123,23 @@ struct KLayout {
   // 4-kz coefficient blocks and its first coefficient kz slot
   std::vector<int> lex_row0, lex_meta;  // [kxmax+1], [2*nrows]
   int lex_maxblk = 0;
+  // groups of 4 rows (kx, 4g..4g+3) for the matrix-core pass 2: per kx the
+  // first group, per group the offset of its coefficient block (floats)
+  // and its 4-kz blocks (those of its first, longest row)
+  std::vector<int> grp_start, grp_base, grp_nb;  // [kxmax+2], [ngrp], [ngrp]
+  long grp_floats = 0;
 };
 
 struct Ctx {
   int mode = MDX_MIXED;
   int device = 0;
   hipStream_t stream = nullptr;
+  // k-space runs on kstream beside binning, pair and exclusion kernels on
+  // stream (fork and join by events), unless overlap is off or kernels
+  // are being timed; both streams have the default priority
+  hipStream_t kstream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;
   // system
   int ns = 0, nmol = 0, ntypes = 0, nq = 0;
   double L[3] = {0, 0, 0};
@@ -184,6 +195,12 @@ struct Ctx {
   // that the grid has several waves per SIMD; partial forces per part
   // [nkpart][nq][3] are summed in a fixed order
   int kparts = 16, nkpart = 0;
+  // MDX_MIXED: pass 2 on the matrix cores (k_kforce_mfma) instead of FP32
+  // FMA chains, from the coefficients also written per row group in the
+  // operand order of the matrix instruction (coefg, zero padded)
+  bool kmfma = false;
+  DBuf<int> d_grp_start, d_grp_base, d_grp_nb;
+  DBuf<float> coefg;
   DBuf<int> d_kparts;
   DBuf<double> kpf;
   DBuf<double> kpart;          // pass-1 partials [nsplit][nseg][64][64]
@@ -237,9 +254,11 @@ int kspace_setup(Ctx &c);
 // pass-2 tables for kz chunk kzc (8..64, multiple of 8) and about
 // `parts` work parts
 int kspace_chunks(Ctx &c, int kzc, int parts);
-// forces of the charges from the k-space sum (adds into c.f); energy and
-// tensor sums into c.acc when measure
-int kspace_forces(Ctx &c, int measure);
+// k-space on stream s up to the per-part forces of the charges (energy
+// and tensor sums into the partial rows when measure, dipole into c.acc)
+int kspace_pass(Ctx &c, int measure, hipStream_t s);
+// adds the k-space forces and the dipole field to c.f on stream s
+int kspace_add(Ctx &c, hipStream_t s);
 
 #endif  // __HIPCC__
 

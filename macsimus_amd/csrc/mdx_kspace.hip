@@ -193,7 +193,9 @@ __global__ __launch_bounds__(kBlock) void k_kcoef(
     const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
     const int *__restrict__ row_ky, const int *__restrict__ row_kzmax,
     const int *__restrict__ row_coef0, const double *__restrict__ tsum, double Lx, double Ly,
-    double Lz, double alpha, R *__restrict__ coef, double *__restrict__ part, int measure) {
+    double Lz, double alpha, R *__restrict__ coef, double *__restrict__ part, int measure,
+    const int *__restrict__ grp_start, const int *__restrict__ grp_base,
+    float *__restrict__ coefg) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int seg = t >> 9, lane = (t >> 3) & 63, k = t & 7;
   double E = 0, pk[6] = {0, 0, 0, 0, 0, 0};
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_kcoef(
       const int kx = row_kx[row], ky = row_ky[row];
       R *cf = coef + (size_t)(row_coef0[row] + kz) * 16;
       if (kx == 0 && ky == 0 && kz == 0) {
-        for (int m = 0; m < 16; m++) cf[m] = 0;
+        for (int m = 0; m < 16; m++) cf[m] = 0;   // (the group block stays zero)
       } else {
         const double *T = tsum + ((size_t)seg * 64 + lane) * 64 + k * 8;
         const int nz = (kx > 0) + (ky > 0) + (kz > 0);
@@ -219,6 +221,18 @@ __global__ __launch_bounds__(kBlock) void k_kcoef(
           cf[2 * m + 1] = (R)(we4 * T[2 * m + 1]);
           cf[8 + 2 * m] = (R)(kzl * we4 * T[2 * m + 1]);
           cf[8 + 2 * m + 1] = (R)(-kzl * we4 * T[2 * m]);
+        }
+        if (coefg) {
+          // the same 16 values in the operand order of k_kforce_mfma: row
+          // q = ky % 4 of group ky / 4, output p, cos/sin kk -> lane
+          // 16(q/2) + 8(p/4) + 4(q%2) + p%4 + 32 kk of kz slot kz
+          const int q = ky & 3;
+          float *g = coefg + grp_base[grp_start[kx] + (ky >> 2)] + (size_t)kz * 64;
+          const int ib = 16 * (q >> 1) + 4 * (q & 1);
+          for (int e = 0; e < 16; e++) {
+            const int pp = e >> 1, kk = e & 1;
+            g[ib + 8 * (pp >> 2) + (pp & 3) + 32 * kk] = (float)cf[e];
+          }
         }
         if (measure) {
           double s2 = 0;
@@ -393,6 +407,121 @@ __global__ __launch_bounds__(kBlock) void k_kforce(
   o[2] = Fz;
 }
 
+// Pass 2 on the matrix cores (MDX_MIXED).  For a group of 4 rows (kx,
+// ky0..ky0+3) of the chunk and 32 charges, the 8 sums G0..3, H0..3 of each
+// row and charge are one product D[32 x 32] = E[32 x 2 KZC] Z[2 KZC x 32]
+// with E = the rows' coefficients and Z = the charges' cos/sin of c along
+// kz: a chain of v_mfma_f32_32x32x2_f32, one per kz (K = cos, sin).
+// Layout of that instruction on gfx950 (checked on the device): lane l
+// holds A[l%32][l/32], B[l/32][l%32] and, in register r, D[4(l/32) + r%4 +
+// 8(r/4)][l%32].  The M index i of D is mapped to row 2(i/16) + (i/4)%2
+// and output 4((i/8)%2) + i%4, so that lane l (charge l%32, half h = l/32)
+// receives all 8 outputs of rows h and 2+h of the group in registers 0-7
+// and 8-15, and forms their x, y and z forces itself; it supplies the cos
+// (h = 0) or sin (h = 1) of its charge along kz and one coefficient of one
+// row.  Rows past the chunk's last ky of a kx and kz past a row's last
+// block contribute zeros.  Row phases: FP64 anchors per kx, FP64 steps of
+// two rows along ky.  The products and sums are FP32 as in k_kforce (on
+// gfx950 the results are the same bits).
+typedef float mfma_f32x16 __attribute__((ext_vector_type(16)));
+
+template <int KZC>
+__global__ __launch_bounds__(kBlock, 4) void k_kforce_mfma(
+    int nq, const double4 *__restrict__ qpos, double iLx, double iLy, double iLz, int kxmax,
+    const int4 *__restrict__ parts, const int *__restrict__ chunk_kymax,
+    const int *__restrict__ grp_start, const int *__restrict__ grp_base,
+    const int *__restrict__ grp_nb, const float *__restrict__ coefg, double *__restrict__ kpf) {
+  constexpr int BPC = KZC / kKB;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = lane & 31, h = lane >> 5;
+  const int j = (blockIdx.x * (kBlock / 64) + wv) * 32 + c;
+  const int4 part = parts[blockIdx.y];
+  const int ch = part.x;
+  const double4 qp = qpos[min(j, nq - 1)];
+  const double sx = frac1(qp.x * iLx), sy = frac1(qp.y * iLy), sz = frac1(qp.z * iLz);
+  const float kxs = (float)iLx, kys = (float)iLy;
+  // e^{2 pi i sy} to the powers 1, 2, 4
+  double cb1, sb1;
+  cs2pi(sy, cb1, sb1);
+  const double cb2 = cb1 * cb1 - sb1 * sb1, sb2 = 2 * cb1 * sb1;
+  const double cb4 = cb2 * cb2 - sb2 * sb2, sb4 = 2 * cb2 * sb2;
+  // this lane's half of the kz table: cos (h = 0) or sin (h = 1)
+  float zt[KZC];
+  {
+    double cz1, sz1;
+    cs2pi(sz, cz1, sz1);
+#pragma unroll
+    for (int m = 0; m < KZC / 8; m++) {
+      double cc, ss;
+      cs2pi((double)(ch * KZC + 8 * m) * sz, cc, ss);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        zt[8 * m + k] = (float)(h ? ss : cc);
+        const double nc = cc * cz1 - ss * sz1;
+        ss = cc * sz1 + ss * cz1;
+        cc = nc;
+      }
+    }
+  }
+  const int *kym_c = chunk_kymax + ch * (kxmax + 1);
+  const float *cfl = coefg + lane + (size_t)ch * KZC * 64;
+  double Fx = 0, Fy = 0, Fz = 0;
+  for (int kx = part.y; kx < part.z; kx++) {
+    const int kym = kym_c[kx];
+    double ca, sa;
+    cs2pi(kx * sx, ca, sa);
+    const float rca = (float)ca, rsa = (float)sa;
+    const float kxl = (float)kx * kxs;
+    const int g0 = grp_start[kx];
+    // phase of row ky0 + h
+    double cb = h ? cb1 : 1.0, sb = h ? sb1 : 0.0;
+    for (int ky0 = 0; ky0 <= kym; ky0 += 4) {
+      const int gi = g0 + (ky0 >> 2);
+      const int nb = min(grp_nb[gi] - ch * BPC, BPC);   // wave-uniform, >= 1
+      const float *ga = cfl + grp_base[gi];
+      mfma_f32x16 acc;
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+      for (int b = 0; b < BPC; b++) {
+        if (b < nb) {
+          float av[kKB];
+#pragma unroll
+          for (int k = 0; k < kKB; k++) av[k] = ga[(b * kKB + k) * 64];
+#pragma unroll
+          for (int k = 0; k < kKB; k++)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k], zt[b * kKB + k], acc, 0, 0, 0);
+        }
+      }
+      // rows ky0 + h (registers 0-7) and ky0 + 2 + h (8-15); rows past kym
+      // hold zeros
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const float G0 = acc[8 * q], G1 = acc[8 * q + 1], G2 = acc[8 * q + 2], G3 = acc[8 * q + 3];
+        const float H0 = acc[8 * q + 4], H1 = acc[8 * q + 5], H2 = acc[8 * q + 6], H3 = acc[8 * q + 7];
+        const double cq = q ? cb * cb2 - sb * sb2 : cb, sq = q ? cb * sb2 + sb * cb2 : sb;
+        const float rcb = (float)cq, rsb = (float)sq;
+        const float kyl = (float)(ky0 + 2 * q + h) * kys;
+        Fx += (double)(kxl * (rca * (rcb * G2 + rsb * G3) - rsa * (rcb * G0 + rsb * G1)));
+        Fy += (double)(kyl * (rca * (rcb * G1 - rsb * G0) + rsa * (rcb * G3 - rsb * G2)));
+        Fz += (double)(rca * (rcb * H0 + rsb * H1) + rsa * (rcb * H2 + rsb * H3));
+      }
+      const double nc = cb * cb4 - sb * sb4;
+      sb = cb * sb4 + sb * cb4;
+      cb = nc;
+    }
+  }
+  // the two halves hold the even and odd rows of the same charges
+  Fx += __shfl_xor(Fx, 32, 64);
+  Fy += __shfl_xor(Fy, 32, 64);
+  Fz += __shfl_xor(Fz, 32, 64);
+  if (h || j >= nq) return;
+  double *o = kpf + ((size_t)blockIdx.y * nq + j) * 3;
+  o[0] = Fx;
+  o[1] = Fy;
+  o[2] = Fz;
+}
+
 // sums the parts of pass 2 in a fixed order and adds -q grad Psi and the
 // dipole field to the force rows of the charges
 __global__ void k_kforce_sum(int nq, int nparts, const double *__restrict__ kpf,
@@ -415,7 +544,7 @@ __global__ void k_kforce_sum(int nq, int nparts, const double *__restrict__ kpf,
 }
 
 template <typename R>
-int run_kspace(mdx::Ctx &c, int measure) {
+int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   using namespace mdx;
   const KLayout &K = c.K;
   const int KXT = K.kxmax + 1, KYT = K.row_ky.empty() ? 1 : *std::max_element(K.row_ky.begin(), K.row_ky.end()) + 1;
@@ -424,33 +553,55 @@ int run_kspace(mdx::Ctx &c, int measure) {
   const int jper = (c.nq + c.nsplit - 1) / c.nsplit;
   dim3 g1((K.nseg + 3) / 4, c.nsplit);
   const double4 *qpos = (const double4 *)c.qpos.p;
-  k_qgather<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.ns, c.q_site.p, c.r.p, c.charge.p,
+  k_qgather<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, c.ns, c.q_site.p, c.r.p, c.charge.p,
                                                   c.polar ? c.chargepol.p : nullptr,
                                                   (double4 *)c.qpos.p);
-  if (c.timing) c.tm.start(T_KPASS1, c.stream);
-  k_sfac<R><<<g1, kBlock, shmem, c.stream>>>(
+  if (c.timing) c.tm.start(T_KPASS1, st);
+  k_sfac<R><<<g1, kBlock, shmem, st>>>(
       c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
       c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p, KXT, KYT, KZT,
       jper, c.kpart.p);
   MDX_HIP(hipGetLastError());
-  if (c.timing) { c.tm.stop(T_KPASS1, c.stream); c.tm.start(T_KCOEF, c.stream); }
+  if (c.timing) { c.tm.stop(T_KPASS1, st); c.tm.start(T_KCOEF, st); }
   const size_t nt = (size_t)K.nseg * 64 * 64;
-  k_ksum<<<nblk(nt), kBlock, 0, c.stream>>>(c.kpart.p, c.nsplit, nt, c.tsum.p);
+  k_ksum<<<nblk(nt), kBlock, 0, st>>>(c.kpart.p, c.nsplit, nt, c.tsum.p);
   R *coef = nullptr;
   if constexpr (sizeof(R) == 4) coef = c.coef32.p; else coef = (R *)c.coef64.p;
-  k_kcoef<R><<<nblk((long)K.nseg * 512), kBlock, 0, c.stream>>>(
+  float *coefg = nullptr;
+  if constexpr (sizeof(R) == 4) coefg = c.kmfma ? c.coefg.p : nullptr;
+  k_kcoef<R><<<nblk((long)K.nseg * 512), kBlock, 0, st>>>(
       K.nseg, c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p,
       c.d_row_kzmax.p, c.d_row_coef0.p, c.tsum.p, c.L[0], c.L[1], c.L[2], c.alpha, coef, c.part.p,
-      measure);
+      measure, c.d_grp_start.p, c.d_grp_base.p, coefg);
   MDX_HIP(hipGetLastError());
   const double V = c.L[0] * c.L[1] * c.L[2];
   const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
-  k_dipole<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, qpos, c.part.p);
+  k_dipole<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, qpos, c.part.p);
   // the dipole sums of k_dipole's workgroups
-  k_reduce_cols<<<3, kBlock, 0, c.stream>>>(c.part.p, nblk(c.nq), ACC_M0, c.acc.p);
-  if (c.timing) { c.tm.stop(T_KCOEF, c.stream); c.tm.start(T_KPASS2, c.stream); }
+  k_reduce_cols<<<3, kBlock, 0, st>>>(c.part.p, nblk(c.nq), ACC_M0, c.acc.p);
+  if (c.timing) { c.tm.stop(T_KCOEF, st); c.tm.start(T_KPASS2, st); }
+  if constexpr (sizeof(R) == 4) {
+    if (c.kmfma) {
+#define MDX_KM(KZV)                                                                         \
+  k_kforce_mfma<KZV><<<dim3(nblk(c.nq, 128), c.nkpart), kBlock, 0, st>>>(                   \
+      c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.kxmax, (const int4 *)c.d_kparts.p,  \
+      c.d_chunk_kymax.p, c.d_grp_start.p, c.d_grp_base.p, c.d_grp_nb.p, c.coefg.p, c.kpf.p)
+      switch (c.kzc) {
+        case 8: MDX_KM(8); break;
+        case 16: MDX_KM(16); break;
+        case 24: MDX_KM(24); break;
+        case 32: MDX_KM(32); break;
+        case 48: MDX_KM(48); break;
+        case 64: MDX_KM(64); break;
+        default: return fail(MDX_E_ARG, "k-space: kz chunk must be 8, 16, 24, 32, 48 or 64");
+      }
+#undef MDX_KM
+      MDX_HIP(hipGetLastError());
+      return 0;
+    }
+  }
 #define MDX_KF(KZV)                                                                         \
-  k_kforce<R, KZV><<<dim3(nblk(c.nq), c.nkpart), kBlock, 0, c.stream>>>(                    \
+  k_kforce<R, KZV><<<dim3(nblk(c.nq), c.nkpart), kBlock, 0, st>>>(                    \
       c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.kxmax, (const int4 *)c.d_kparts.p,  \
       c.d_chunk_kymax.p, c.d_lex_row0.p, (const int2 *)c.d_lex_meta.p, coef, c.kpf.p)
   switch (c.kzc) {
@@ -462,11 +613,21 @@ int run_kspace(mdx::Ctx &c, int measure) {
     case 64: MDX_KF(64); break;
     default: return fail(MDX_E_ARG, "k-space: kz chunk must be 8, 16, 24, 32, 48 or 64");
   }
-  k_kforce_sum<<<nblk(c.nq), kBlock, 0, c.stream>>>(c.nq, c.nkpart, c.kpf.p, c.q_site.p, qpos,
-                                                    c.acc.p, cdip, c.f.p);
 #undef MDX_KF
   MDX_HIP(hipGetLastError());
-  if (c.timing) c.tm.stop(T_KPASS2, c.stream);
+  (void)cdip;
+  return 0;
+}
+
+// the parts of pass 2 and the dipole field into the force rows
+int add_kspace(mdx::Ctx &c, hipStream_t st) {
+  using namespace mdx;
+  const double V = c.L[0] * c.L[1] * c.L[2];
+  const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
+  k_kforce_sum<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, c.nkpart, c.kpf.p, c.q_site.p,
+                                             (const double4 *)c.qpos.p, c.acc.p, cdip, c.f.p);
+  MDX_HIP(hipGetLastError());
+  if (c.timing) c.tm.stop(T_KPASS2, st);
   return 0;
 }
 
@@ -537,11 +698,34 @@ void klayout_build(const double L[3], double kappa, KLayout &K) {
     K.lex_meta[2 * i + 1] = lcoef[i];
     K.lex_maxblk = std::max(K.lex_maxblk, lnb[i]);
   }
+  // row groups: for each kx the rows ky = 4g..4g+3; a group's block holds
+  // 4 lnb(first row) kz slots of 64 floats (32 M entries x cos/sin)
+  K.grp_start.assign(K.kxmax + 2, 0);
+  K.grp_floats = 0;
+  for (int kx = 0; kx <= K.kxmax; kx++) {
+    K.grp_start[kx] = (int)K.grp_base.size();
+    const int r0 = K.lex_row0[kx], r1 = kx < K.kxmax ? K.lex_row0[kx + 1] : K.nrows;
+    for (int r = r0; r < r1; r += 4) {
+      K.grp_base.push_back((int)K.grp_floats);
+      K.grp_nb.push_back(lnb[r]);
+      K.grp_floats += (long)lnb[r] * kKB * 64;
+    }
+  }
+  K.grp_start[K.kxmax + 1] = (int)K.grp_base.size();
 }
 
 int kspace_setup(Ctx &c) {
   KLayout &K = c.K;
   klayout_build(c.L, c.kappa, K);
+  // the coefficient kernel writes 2 partial rows per segment, the dipole
+  // kernel one per 256 charges
+  const int need = std::max(2 * K.nseg, (c.nq + kBlock - 1) / kBlock);
+  if (need > c.part_blocks) {
+    c.part_blocks = need;
+    int rc0;
+    if ((rc0 = c.part.alloc((size_t)c.part_blocks * ACC_N))) return rc0;
+    c.gen++;
+  }
   // charge splits: about 8 workgroups per CU, at least 256 charges each
   const int wg = (K.nseg + 3) / 4;
   c.nsplit = std::max(1, std::min(2048 / std::max(wg, 1), c.nq / 256));
@@ -565,6 +749,14 @@ int kspace_setup(Ctx &c) {
   if (c.mode == MDX_MIXED) {
     if ((rc = c.coef32.alloc((size_t)K.ncoef * 16))) return rc;
     MDX_HIP(hipMemset(c.coef32.p, 0, sizeof(float) * K.ncoef * 16));
+    // row-group blocks of the matrix-core pass 2 (padding stays zero)
+    if ((rc = c.coefg.alloc((size_t)K.grp_floats)) || (rc = c.d_grp_start.alloc(K.grp_start.size())) ||
+        (rc = c.d_grp_base.alloc(K.grp_base.size())) || (rc = c.d_grp_nb.alloc(K.grp_nb.size())))
+      return rc;
+    MDX_HIP(hipMemset(c.coefg.p, 0, sizeof(float) * K.grp_floats));
+    MDX_HIP(hipMemcpy(c.d_grp_start.p, K.grp_start.data(), sizeof(int) * K.grp_start.size(), hipMemcpyHostToDevice));
+    MDX_HIP(hipMemcpy(c.d_grp_base.p, K.grp_base.data(), sizeof(int) * K.grp_base.size(), hipMemcpyHostToDevice));
+    MDX_HIP(hipMemcpy(c.d_grp_nb.p, K.grp_nb.data(), sizeof(int) * K.grp_nb.size(), hipMemcpyHostToDevice));
   } else {
     if ((rc = c.coef64.alloc((size_t)K.ncoef * 16))) return rc;
     MDX_HIP(hipMemset(c.coef64.p, 0, sizeof(double) * K.ncoef * 16));
@@ -664,8 +856,10 @@ int kspace_chunks(Ctx &c, int kzc, int parts) {
   return 0;
 }
 
-int kspace_forces(Ctx &c, int measure) {
-  return c.mode == MDX_FP64 ? run_kspace<double>(c, measure) : run_kspace<float>(c, measure);
+int kspace_pass(Ctx &c, int measure, hipStream_t s) {
+  return c.mode == MDX_FP64 ? run_kspace<double>(c, measure, s) : run_kspace<float>(c, measure, s);
 }
+
+int kspace_add(Ctx &c, hipStream_t s) { return add_kspace(c, s); }
 
 }  // namespace mdx

@@ -507,6 +507,15 @@ extern "C" int mdx_set_graph(mdx_ctx_t *h, int on) {
   return 0;
 }
 
+extern "C" int mdx_set_overlap(mdx_ctx_t *h, int on) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_overlap: NULL");
+  MDX_HIP(hipSetDevice(h->c.device));
+  MDX_HIP(hipStreamSynchronize(h->c.stream));
+  h->c.overlap = on != 0;
+  h->c.gen++;                          // a captured step holds the old shape
+  return 0;
+}
+
 extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_step_out_t *out) {
   if (!h || !p || !out || nsteps < 0) return fail(MDX_E_ARG, "mdx_md_steps: bad argument");
   Ctx &c = h->c;

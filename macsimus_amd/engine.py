@@ -87,11 +87,11 @@ class Engine:
         return cid
 
     def info(self):
-        iv = (C.c_int * 17)()
-        self.lib.mdx_info(self.ctx, iv, 17)
+        iv = (C.c_int * 18)()
+        self.lib.mdx_info(self.ctx, iv, 18)
         keys = ("ncx", "ncy", "ncz", "stencil", "nq", "krows", "kblocks", "ksegments",
                 "ktriples", "ksplits", "kxmax", "mode", "kchunk", "nkchunks", "kparts_asked",
-                "kparts", "pair_cols")
+                "kparts", "pair_cols", "kmfma")
         return dict(zip(keys, list(iv)))
 
     def set_kchunk(self, kzc, parts=16):
@@ -152,6 +152,14 @@ class Engine:
     def set_graph(self, on=True):
         rc = self.lib.mdx_set_graph(self.ctx, int(on))
         abi.check(self.lib, rc, "mdx_set_graph")
+
+    def set_kspace_mfma(self, on=True):
+        rc = self.lib.mdx_set_kspace_mfma(self.ctx, int(on))
+        abi.check(self.lib, rc, "mdx_set_kspace_mfma")
+
+    def set_overlap(self, on=True):
+        rc = self.lib.mdx_set_overlap(self.ctx, int(on))
+        abi.check(self.lib, rc, "mdx_set_overlap")
 
     def set_stencil(self, w, pcols=0):
         rc = self.lib.mdx_set_stencil(self.ctx, int(w), int(pcols))
@@ -317,6 +325,8 @@ def declare_extras(lib):
     lib.mdx_set_stencil.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_set_kchunk.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mdx_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_overlap.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_kspace_mfma.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4

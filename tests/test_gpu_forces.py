@@ -123,3 +123,30 @@ def test_all_pair_path(name, parts, mode):
     assert o1.npair == o0.npair and o1.nlj == o0.nlj
     assert abs((o1.En_el + o1.En_pot) - (o0.En_el + o0.En_pot)) < 1e-9 * abs(o0.En_el + o0.En_pot) * (1 if mode == abi.MDX_FP64 else 1e3)
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["spce1000_lattice", "brine506"])
+def test_kspace_pass2_on_matrix_cores(name):
+    """the mixed engine's pass 2 on the matrix cores gives cook's forces
+    within the mixed bar and agrees with the FMA variant to FP32 rounding,
+    for every kz chunk and part split (energies do not depend on pass 2)"""
+    d, meta = oracle.load_fixture(name)
+    bs = oracle.fixture_system(meta, provider_=engine_provider())
+    eng = Engine(bs, abi.MDX_MIXED)
+    for kzc in (8, 16, 24, 32):
+        for parts in (1, 5, 16):
+            eng.set_kchunk(kzc, parts)
+            eng.set_kspace_mfma(False)
+            f0, o0 = eng.forces(d["r"], 1)
+            eng.set_kspace_mfma(True)
+            assert eng.info()["kmfma"] == 1
+            f1, o1 = eng.forces(d["r"], 1)
+            e, d01 = rel_rms(f1, d["f"]), rel_rms(f1, f0)
+            print(name, kzc, parts, "mfma vs cook %.2e, vs fma %.2e" % (e, d01), flush=True)
+            assert e < TOL[abi.MDX_MIXED] and d01 < 2e-6
+            assert o1.En_el == o0.En_el and o1.npair == o0.npair
+    e64 = Engine(bs, abi.MDX_FP64)
+    with pytest.raises(RuntimeError):
+        e64.set_kspace_mfma(True)
+    eng.close()
+    e64.close()
