@@ -132,8 +132,11 @@ def bench_replicas(a):
     b = Batch(bss, mode)
     rs = [sd["r"] for sd in sy]
     if a.pin:
-        w, kzc, kparts = (int(x) for x in a.pin.split(","))
+        pin = [int(x) for x in a.pin.split(",")]
+        w, kzc, kparts = pin[:3]
         b.set_stencil(w)
+        if mode == abi.MDX_MIXED:
+            b.set_kspace_mfma(len(pin) > 3 and pin[3] != 0)
         b.set_kspace(kzc, kparts)
         tune_ms = None
     else:
@@ -217,12 +220,13 @@ def bench_replicas(a):
                    "charges": info["charges"], "k_segments": info["kseg"], "pairs": npair,
                    "timestep_ps": h, "precision": a.mode, "parallelism": "1 GPU",
                    "tuned": {"stencil_w": w, "kz_chunk": kzc, "k_parts": kparts,
-                             "force_ms": tune_ms}},
+                             "kspace_pass2_mfma": bool(info["kmfma"]), "force_ms": tune_ms}},
         "us_per_force": force_ms * 1e3,
         "replica_steps_per_s": nsys / (ms_step * 1e-3),
         "atom_steps_per_s": info["sites"] / (ms_step * 1e-3),
         "kernel_ms": {k: v[0] for k, v in kern.items()},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
+        "roofline": {"bound": "mfma" if (dom == "kpass2" and info["kmfma"]) else "valu",
+                     "kernel": dom, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_lookup("replicas_256", a.mode, dom, info),
                      "work_per_launch": work[dom]},

@@ -302,6 +302,9 @@ int mdx_batch_set_stencil(mdx_batch_t *b, int w);
    about kparts parts (1..256) for the largest k-set, the others in
    proportion to their k-sets */
 int mdx_batch_set_kspace(mdx_batch_t *b, int kzc, int kparts);
+/* MDX_MIXED: k-space pass 2 on the matrix cores (as mdx_set_kspace_mfma);
+   mdx_batch_autotune tries both.  Default: on in MDX_MIXED. */
+int mdx_batch_set_kspace_mfma(mdx_batch_t *b, int on);
 /* times the force call on r (HIP events): the stencil w = 1..3, then kz
    chunk x parts; keeps the fastest; choice[3] = {w, kz chunk, parts},
    ms[2] = force-call times after each stage (either may be NULL) */
@@ -312,7 +315,8 @@ int mdx_batch_autotune(mdx_batch_t *b, const double *r, int *choice, double *ms)
 int mdx_batch_set_timing(mdx_batch_t *b, int on);
 int mdx_batch_get_timing(mdx_batch_t *b, double *ms, int *count, int n);
 /* iv[0..n) = systems, sites, padded sites, cells, charged systems, mode,
-   w, charges, k segments, pass-1 units, pass-2 units, kz chunk, parts */
+   w, charges, k segments, pass-1 units, pass-2 units, kz chunk, parts,
+   matrix-core pass 2 (0/1) */
 int mdx_batch_info(mdx_batch_t *b, int *iv, int n);
 
 /* ---- pressure assembly and cutoff correction (host arithmetic) -------- */

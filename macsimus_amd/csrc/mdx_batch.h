@@ -30,13 +30,16 @@ struct KSys {
   int chunk_off, lexr_off;   // pass-2 chunk table and lexicographic row starts
   int nkpart;                // pass-2 parts
   long kpf_off;              // its pass-2 partial force rows
+  int grps_off;              // its row-group starts per kx (matrix-core pass 2)
   int kcb0, kcb1;            // its coefficient-kernel workgroups
 };
 
 // k-space of the batch: the single-system T formulation laid out per
 // system, every kernel launched once for all charged systems
 struct BK {
-  int nks = 0, nqt = 0, nseg = 0, n1 = 0, n2 = 0, nkcb = 0, kzc = 16, kparts = 8;
+  int nks = 0, nqt = 0, nseg = 0, n1 = 0, n2 = 0, n2m = 0, nkcb = 0, kzc = 16, kparts = 8;
+  // MDX_MIXED: pass 2 on the matrix cores (units of 128 charges, units2m)
+  bool kmfma = false;
   int shmem_max = 0;         // pass-1 LDS tables of the largest system
   long ncoef = 0;
   std::vector<KSys> hks;
@@ -45,7 +48,9 @@ struct BK {
   DBuf<double> qpos;                         // [nqt] double4
   DBuf<int> row_kx, row_ky, row_kzmax, row_coef0;
   DBuf<int> seg_blk, seg_row0, seg_rowend, seg_ks;
-  DBuf<int> units1, units2, parts;           // int4 each
+  DBuf<int> units1, units2, units2m, parts;  // int4 each
+  DBuf<int> grp_start, grp_base, grp_nb;     // row groups of all systems
+  DBuf<float> coefg;
   DBuf<int> lex_row0, lex_meta, chunk_kymax;
   DBuf<double> kpart, tsum, kpf, kcpart;
   DBuf<double> coef64;

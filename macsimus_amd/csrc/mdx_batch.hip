@@ -503,6 +503,7 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   auto *h = new mdx_batch_s;
   Batch &b = h->b;
   b.mode = mode;
+  b.k.kmfma = mode == MDX_MIXED;      // pass 2 on the matrix cores by default
   b.nsys = nsys;
   auto bail = [&](int rc) { release(b); delete h; return rc; };
   (void)hipGetDevice(&b.device);
@@ -806,6 +807,16 @@ extern "C" int mdx_batch_set_stencil(mdx_batch_t *h, int w) {
   return layout_cells(h->b, w);
 }
 
+extern "C" int mdx_batch_set_kspace_mfma(mdx_batch_t *h, int on) {
+  if (!h) return fail(MDX_E_ARG, "mdx_batch_set_kspace_mfma: NULL");
+  if (on && h->b.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_batch_set_kspace_mfma: MDX_MIXED only");
+  MDX_HIP(hipSetDevice(h->b.device));
+  MDX_HIP(hipStreamSynchronize(h->b.stream));
+  h->b.k.kmfma = on != 0;
+  h->b.graph_ok = false;
+  return 0;
+}
+
 extern "C" int mdx_batch_set_kspace(mdx_batch_t *h, int kzc, int kparts) {
   if (!h) return fail(MDX_E_ARG, "mdx_batch_set_kspace: NULL");
   MDX_HIP(hipSetDevice(h->b.device));
@@ -854,11 +865,17 @@ extern "C" int mdx_batch_autotune(mdx_batch_t *h, const double *r, int *choice, 
   }
   if (!rc) rc = layout_cells(b, bw);
   const int kzs[4] = {8, 16, 24, 32}, kps[6] = {1, 2, 4, 8, 16, 32};
-  for (int iz = 0; iz < 4 && !rc && b.k.nks; iz++)
-    for (int ip = 0; ip < 6 && !rc; ip++) {
-      if ((rc = bk_setup(b, kzs[iz], kps[ip])) || (rc = time_forces(b, e0, e1, &t))) break;
-      if (t < best_k) { best_k = t; bz = kzs[iz]; bp = kps[ip]; }
-    }
+  bool bm = b.k.kmfma;
+  // MDX_MIXED: the FP32 FMA and the matrix-core variants of pass 2
+  for (int mf = 0; mf < (b.mode == MDX_MIXED ? 2 : 1) && !rc && b.k.nks; mf++) {
+    b.k.kmfma = mf != 0;
+    for (int iz = 0; iz < 4 && !rc; iz++)
+      for (int ip = 0; ip < 6 && !rc; ip++) {
+        if ((rc = bk_setup(b, kzs[iz], kps[ip])) || (rc = time_forces(b, e0, e1, &t))) break;
+        if (t < best_k) { best_k = t; bz = kzs[iz]; bp = kps[ip]; bm = b.k.kmfma; }
+      }
+  }
+  b.k.kmfma = bm;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
@@ -869,13 +886,14 @@ extern "C" int mdx_batch_autotune(mdx_batch_t *h, const double *r, int *choice, 
 }
 
 // iv: systems, sites, padded sites, cells, charged systems, mode, w,
-// charges, k segments, pass-1 units, pass-2 units, kz chunk, parts asked
+// charges, k segments, pass-1 units, pass-2 units, kz chunk, parts asked,
+// matrix-core pass 2
 extern "C" int mdx_batch_info(mdx_batch_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_batch_info: NULL");
   const Batch &b = h->b;
   const BK &k = b.k;
-  const int v[13] = {b.nsys, b.nreal, b.npad, b.ncell, k.nks, b.mode, b.hs.empty() ? 0 : b.hs[0].w,
-                     k.nqt, k.nseg, k.n1, k.n2, k.kzc, k.kparts};
-  for (int i = 0; i < n && i < 13; i++) iv[i] = v[i];
+  const int v[14] = {b.nsys, b.nreal, b.npad, b.ncell, k.nks, b.mode, b.hs.empty() ? 0 : b.hs[0].w,
+                     k.nqt, k.nseg, k.n1, k.kmfma ? k.n2m : k.n2, k.kzc, k.kparts, k.kmfma ? 1 : 0};
+  for (int i = 0; i < n && i < 14; i++) iv[i] = v[i];
   return 0;
 }

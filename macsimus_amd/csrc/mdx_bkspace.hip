@@ -188,7 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_bk_kcoef(
     const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
     const int *__restrict__ row_ky, const int *__restrict__ row_kzmax,
     const int *__restrict__ row_coef0, const double *__restrict__ tsum, R *__restrict__ coef,
-    double *__restrict__ kcpart, int measure) {
+    double *__restrict__ kcpart, int measure, const int *__restrict__ grp_start,
+    const int *__restrict__ grp_base, float *__restrict__ coefg) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int seg = t >> 9, lane = (t >> 3) & 63, k = t & 7;
   double E = 0, pk[6] = {0, 0, 0, 0, 0, 0};
@@ -215,6 +216,18 @@ __global__ __launch_bounds__(kBlock) void k_bk_kcoef(
           cf[2 * m + 1] = (R)(we4 * T[2 * m + 1]);
           cf[8 + 2 * m] = (R)(kzl * we4 * T[2 * m + 1]);
           cf[8 + 2 * m + 1] = (R)(-kzl * we4 * T[2 * m]);
+        }
+        if (coefg) {
+          // the same 16 values in the operand order of the matrix-core pass
+          // 2 (row q = ky % 4 of group ky / 4, output p, cos/sin kk -> lane
+          // 16(q/2) + 8(p/4) + 4(q%2) + p%4 + 32 kk of kz slot kz)
+          const int q = ky & 3;
+          float *g = coefg + grp_base[grp_start[K.grps_off + kx] + (ky >> 2)] + (size_t)kz * 64;
+          const int ib = 16 * (q >> 1) + 4 * (q & 1);
+          for (int e2 = 0; e2 < 16; e2++) {
+            const int pp = e2 >> 1, kk2 = e2 & 1;
+            g[ib + 8 * (pp >> 2) + (pp & 3) + 32 * kk2] = (float)cf[e2];
+          }
         }
         if (measure) {
           double s2 = 0;
@@ -381,6 +394,106 @@ __global__ __launch_bounds__(kBlock) void k_bk_kforce(
   o[2] = Fz;
 }
 
+// Pass 2 on the matrix cores (MDX_MIXED): the kernel k_kforce_mfma of the
+// single-system engine with the unit's system descriptor; unit = (system,
+// block of 128 of its charges, part), 32 charges per wave.  See that
+// kernel for the instruction layout and the row-group operand order.
+typedef float mfma_f32x16 __attribute__((ext_vector_type(16)));
+
+template <int KZC>
+__global__ __launch_bounds__(kBlock, 4) void k_bk_kforce_mfma(
+    const int4 *__restrict__ units, const KSys *__restrict__ ks, const double4 *__restrict__ qpos,
+    const int4 *__restrict__ parts, const int *__restrict__ chunk_kymax,
+    const int *__restrict__ grp_start, const int *__restrict__ grp_base,
+    const int *__restrict__ grp_nb, const float *__restrict__ coefg, double *__restrict__ kpf) {
+  constexpr int BPC = KZC / kKB;
+  const int4 u = units[blockIdx.x];
+  const KSys K = ks[u.x];
+  const int4 part = parts[u.z];
+  const int ch = part.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = lane & 31, h = lane >> 5;
+  const int j = (u.y * (kBlock / 64) + wv) * 32 + c;
+  const double4 qp = qpos[K.q_off + min(j, K.nq - 1)];
+  const double sx = frac1(qp.x * K.iL), sy = frac1(qp.y * K.iL), sz = frac1(qp.z * K.iL);
+  const float kls = (float)K.iL;
+  double cb1, sb1;
+  cs2pi(sy, cb1, sb1);
+  const double cb2 = cb1 * cb1 - sb1 * sb1, sb2 = 2 * cb1 * sb1;
+  const double cb4 = cb2 * cb2 - sb2 * sb2, sb4 = 2 * cb2 * sb2;
+  float zt[KZC];
+  {
+    double cz1, sz1;
+    cs2pi(sz, cz1, sz1);
+#pragma unroll
+    for (int m = 0; m < KZC / 8; m++) {
+      double cc, ss;
+      cs2pi((double)(ch * KZC + 8 * m) * sz, cc, ss);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        zt[8 * m + k] = (float)(h ? ss : cc);
+        const double nc = cc * cz1 - ss * sz1;
+        ss = cc * sz1 + ss * cz1;
+        cc = nc;
+      }
+    }
+  }
+  const int *kym_c = chunk_kymax + K.chunk_off + ch * (K.kxmax + 1);
+  const int *gs = grp_start + K.grps_off;
+  const float *cfl = coefg + lane + (size_t)ch * KZC * 64;
+  double Fx = 0, Fy = 0, Fz = 0;
+  for (int kx = part.y; kx < part.z; kx++) {
+    const int kym = kym_c[kx];
+    double ca, sa;
+    cs2pi(kx * sx, ca, sa);
+    const float rca = (float)ca, rsa = (float)sa;
+    const float kxl = (float)kx * kls;
+    const int g0 = gs[kx];
+    double cb = h ? cb1 : 1.0, sb = h ? sb1 : 0.0;
+    for (int ky0 = 0; ky0 <= kym; ky0 += 4) {
+      const int gi = g0 + (ky0 >> 2);
+      const int nb = min(grp_nb[gi] - ch * BPC, BPC);
+      const float *ga = cfl + grp_base[gi];
+      mfma_f32x16 acc;
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+      for (int b = 0; b < BPC; b++) {
+        if (b < nb) {
+          float av[kKB];
+#pragma unroll
+          for (int k = 0; k < kKB; k++) av[k] = ga[(b * kKB + k) * 64];
+#pragma unroll
+          for (int k = 0; k < kKB; k++)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k], zt[b * kKB + k], acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const float G0 = acc[8 * q], G1 = acc[8 * q + 1], G2 = acc[8 * q + 2], G3 = acc[8 * q + 3];
+        const float H0 = acc[8 * q + 4], H1 = acc[8 * q + 5], H2 = acc[8 * q + 6], H3 = acc[8 * q + 7];
+        const double cq = q ? cb * cb2 - sb * sb2 : cb, sq = q ? cb * sb2 + sb * cb2 : sb;
+        const float rcb = (float)cq, rsb = (float)sq;
+        const float kyl = (float)(ky0 + 2 * q + h) * kls;
+        Fx += (double)(kxl * (rca * (rcb * G2 + rsb * G3) - rsa * (rcb * G0 + rsb * G1)));
+        Fy += (double)(kyl * (rca * (rcb * G1 - rsb * G0) + rsa * (rcb * G3 - rsb * G2)));
+        Fz += (double)(rca * (rcb * H0 + rsb * H1) + rsa * (rcb * H2 + rsb * H3));
+      }
+      const double nc = cb * cb4 - sb * sb4;
+      sb = cb * sb4 + sb * cb4;
+      cb = nc;
+    }
+  }
+  Fx += __shfl_xor(Fx, 32, 64);
+  Fy += __shfl_xor(Fy, 32, 64);
+  Fz += __shfl_xor(Fz, 32, 64);
+  if (h || j >= K.nq) return;
+  double *o = kpf + ((size_t)K.kpf_off + (size_t)part.w * K.nq + j) * 3;
+  o[0] = Fx;
+  o[1] = Fy;
+  o[2] = Fz;
+}
+
 // sums a charge's parts in order and adds -q grad Psi and the dipole field
 __global__ void k_bk_kforce_sum(int nqt, const KSys *__restrict__ ks, const int *__restrict__ qks,
                                 const int *__restrict__ qrow, const double4 *__restrict__ qpos,
@@ -424,13 +537,38 @@ int run(Batch &b, int measure) {
   if (k.nks) {
     const long nt = (long)k.nseg * 4096;
     k_bk_ksum<<<nblk(nt), kBlock, 0, b.stream>>>(nt, ks, k.seg_ks.p, k.kpart.p, k.tsum.p);
+    float *coefg = nullptr;
+    if constexpr (sizeof(R) == 4) coefg = k.kmfma ? k.coefg.p : nullptr;
     k_bk_kcoef<R><<<k.nkcb, kBlock, 0, b.stream>>>(
         k.nseg, ks, k.seg_ks.p, k.seg_blk.p, k.seg_row0.p, k.seg_rowend.p, k.row_kx.p, k.row_ky.p,
-        k.row_kzmax.p, k.row_coef0.p, k.tsum.p, coef, k.kcpart.p, measure);
+        k.row_kzmax.p, k.row_coef0.p, k.tsum.p, coef, k.kcpart.p, measure, k.grp_start.p,
+        k.grp_base.p, coefg);
   }
   k_bk_sums<<<b.nsys, kBlock, 0, b.stream>>>(sy, ks, k.kcpart.p, qpos, measure, b.sacc.p);
   if (b.timing) { b.tm.stop(T_KCOEF, b.stream); b.tm.start(T_KPASS2, b.stream); }
-  if (k.nks) {
+  bool done2 = false;
+  if constexpr (sizeof(R) == 4) {
+    if (k.nks && k.kmfma) {
+#define MDX_BKM(KZV)                                                                          \
+  k_bk_kforce_mfma<KZV><<<k.n2m, kBlock, 0, b.stream>>>((const int4 *)k.units2m.p, ks, qpos,  \
+                                                       (const int4 *)k.parts.p,              \
+                                                       k.chunk_kymax.p, k.grp_start.p,       \
+                                                       k.grp_base.p, k.grp_nb.p, k.coefg.p,  \
+                                                       k.kpf.p)
+      switch (k.kzc) {
+        case 8: MDX_BKM(8); break;
+        case 16: MDX_BKM(16); break;
+        case 24: MDX_BKM(24); break;
+        case 32: MDX_BKM(32); break;
+        default: return fail(MDX_E_ARG, "batch k-space: kz chunk must be 8, 16, 24 or 32");
+      }
+#undef MDX_BKM
+      k_bk_kforce_sum<<<nblk(k.nqt), kBlock, 0, b.stream>>>(k.nqt, ks, k.qks.p, k.qrow.p, qpos,
+                                                             k.kpf.p, b.sacc.p, b.f.p);
+      done2 = true;
+    }
+  }
+  if (k.nks && !done2) {
 #define MDX_BKF(KZV)                                                                          \
   k_bk_kforce<R, KZV><<<k.n2, kBlock, 0, b.stream>>>((const int4 *)k.units2.p, ks, qpos,      \
                                                      (const int4 *)k.parts.p, k.chunk_kymax.p, \
@@ -460,12 +598,13 @@ namespace batch {
 void BK::release() {
   ks.release();
   DBuf<int> *n[] = {&qrow, &qks, &row_kx, &row_ky, &row_kzmax, &row_coef0, &seg_blk, &seg_row0,
-                    &seg_rowend, &seg_ks, &units1, &units2, &parts, &lex_row0, &lex_meta,
-                    &chunk_kymax};
+                    &seg_rowend, &seg_ks, &units1, &units2, &units2m, &parts, &lex_row0, &lex_meta,
+                    &chunk_kymax, &grp_start, &grp_base, &grp_nb};
   for (auto *x : n) x->release();
   DBuf<double> *d[] = {&qpos, &kpart, &tsum, &kpf, &kcpart, &coef64};
   for (auto *x : d) x->release();
   coef32.release();
+  coefg.release();
 }
 
 int bk_setup(Batch &b, int kzc, int kparts) {
@@ -475,7 +614,9 @@ int bk_setup(Batch &b, int kzc, int kparts) {
   BK &k = b.k;
   std::vector<KSys> hks;
   std::vector<int> qrow, qks, row_kx, row_ky, row_kzmax, row_coef0, seg_blk, seg_row0, seg_rowend,
-      seg_ks, units1, units2, parts, lex_row0, lex_meta, chunk_kymax;
+      seg_ks, units1, units2, units2m, parts, lex_row0, lex_meta, chunk_kymax, grp_start, grp_base,
+      grp_nb;
+  long grp_floats = 0;
   std::vector<double> cost1;
   std::vector<KLayout> lay;
   // the k-sets first: the largest sets the parts of the others
@@ -561,6 +702,15 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       for (int p = 0; p < x.nkpart; p++) {
         units2.insert(units2.end(), {(int)hks.size(), cb, p0 + p, 0});
       }
+    // matrix-core pass 2: units of 128 charges; row groups of the system
+    for (int cb = 0; cb < (x.nq + 127) / 128; cb++)
+      for (int p = 0; p < x.nkpart; p++) units2m.insert(units2m.end(), {(int)hks.size(), cb, p0 + p, 0});
+    x.grps_off = (int)grp_start.size();
+    const int g0 = (int)grp_base.size();
+    for (int v : K.grp_start) grp_start.push_back(v + g0);
+    for (int v : K.grp_base) grp_base.push_back((int)(v + grp_floats));
+    grp_nb.insert(grp_nb.end(), K.grp_nb.begin(), K.grp_nb.end());
+    grp_floats += K.grp_floats;
     x.kcb0 = 2 * nseg;
     x.kcb1 = 2 * (nseg + K.nseg);
     nrows += K.nrows;
@@ -588,6 +738,7 @@ int bk_setup(Batch &b, int kzc, int kparts) {
   k.nseg = nseg;
   k.n1 = (int)units1.size() / 4;
   k.n2 = (int)units2.size() / 4;
+  k.n2m = (int)units2m.size() / 4;
   k.nkcb = 2 * nseg;
   k.kzc = kzc;
   k.kparts = kparts;
@@ -605,6 +756,8 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       (rc = k.seg_row0.alloc(one(seg_row0.size()))) ||
       (rc = k.seg_rowend.alloc(one(seg_rowend.size()))) || (rc = k.seg_ks.alloc(one(seg_ks.size()))) ||
       (rc = k.units1.alloc(one(units1.size()))) || (rc = k.units2.alloc(one(units2.size()))) ||
+      (rc = k.units2m.alloc(one(units2m.size()))) || (rc = k.grp_start.alloc(one(grp_start.size()))) ||
+      (rc = k.grp_base.alloc(one(grp_base.size()))) || (rc = k.grp_nb.alloc(one(grp_nb.size()))) ||
       (rc = k.parts.alloc(one(parts.size()))) || (rc = k.lex_row0.alloc(one(lex_row0.size()))) ||
       (rc = k.lex_meta.alloc(one(lex_meta.size()))) ||
       (rc = k.chunk_kymax.alloc(one(chunk_kymax.size()))) ||
@@ -612,8 +765,10 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       (rc = k.kpf.alloc(one((size_t)kpf_off * 3))) || (rc = k.kcpart.alloc(one((size_t)k.nkcb * 7))))
     return rc;
   if (b.mode == MDX_MIXED) {
-    if ((rc = k.coef32.alloc(one((size_t)ncoef * 16)))) return rc;
+    if ((rc = k.coef32.alloc(one((size_t)ncoef * 16))) || (rc = k.coefg.alloc(one((size_t)grp_floats))))
+      return rc;
     MDX_HIP(hipMemset(k.coef32.p, 0, sizeof(float) * one((size_t)ncoef * 16)));
+    MDX_HIP(hipMemset(k.coefg.p, 0, sizeof(float) * one((size_t)grp_floats)));
   } else {
     if ((rc = k.coef64.alloc(one((size_t)ncoef * 16)))) return rc;
     MDX_HIP(hipMemset(k.coef64.p, 0, sizeof(double) * one((size_t)ncoef * 16)));
@@ -627,7 +782,9 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       !upi(k.row_kx, row_kx) || !upi(k.row_ky, row_ky) || !upi(k.row_kzmax, row_kzmax) ||
       !upi(k.row_coef0, row_coef0) || !upi(k.seg_blk, seg_blk) || !upi(k.seg_row0, seg_row0) ||
       !upi(k.seg_rowend, seg_rowend) || !upi(k.seg_ks, seg_ks) || !upi(k.units1, units1) ||
-      !upi(k.units2, units2) || !upi(k.parts, parts) || !upi(k.lex_row0, lex_row0) ||
+      !upi(k.units2, units2) || !upi(k.units2m, units2m) || !upi(k.grp_start, grp_start) ||
+      !upi(k.grp_base, grp_base) || !upi(k.grp_nb, grp_nb) ||
+      !upi(k.parts, parts) || !upi(k.lex_row0, lex_row0) ||
       !upi(k.lex_meta, lex_meta) || !upi(k.chunk_kymax, chunk_kymax))
     return fail(MDX_E_GPU, "batch k-space: upload failed");
   // the systems' k-space indices

@@ -294,11 +294,16 @@ class Batch:
         abi.check(self.lib, rc, "mdx_batch_get_timing")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(Engine.FAMILIES)}
 
+    def set_kspace_mfma(self, on=True):
+        abi.check(self.lib, self.lib.mdx_batch_set_kspace_mfma(self.h, int(on)),
+                  "mdx_batch_set_kspace_mfma")
+
     def info(self):
-        iv = (C.c_int * 13)()
-        self.lib.mdx_batch_info(self.h, iv, 13)
+        iv = (C.c_int * 14)()
+        self.lib.mdx_batch_info(self.h, iv, 14)
         return dict(zip(("nsys", "sites", "padded", "cells", "charged", "mode", "stencil",
-                         "charges", "kseg", "units1", "units2", "kchunk", "kparts"), list(iv)))
+                         "charges", "kseg", "units1", "units2", "kchunk", "kparts", "kmfma"),
+                        list(iv)))
 
 
 def declare_extras(lib):
@@ -314,6 +319,7 @@ def declare_extras(lib):
     lib.mdx_batch_set_stencil.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_batch_autotune.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int), dp]
     lib.mdx_batch_set_kspace.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mdx_batch_set_kspace_mfma.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     lib.mdx_batch_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_batch_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int),
