@@ -937,7 +937,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   }
 
   if ((rc = c.r.alloc(3 * nrow)) || (rc = c.f.alloc(3 * nrow)) || (rc = c.type.alloc(c.ns)) ||
-      (rc = c.qpos.alloc(4 * std::max(c.nq, 1))) ||
+      (rc = c.qpos.alloc(4 * std::max(c.nq, 1))) || (rc = c.qtrig.alloc(12 * std::max(c.nq, 1))) ||
       (c.polar && ((rc = c.chargepol.alloc(c.ns)) || (rc = c.alpha_qq.alloc(c.ns)) ||
                    (rc = c.spol.alloc(4 * c.ns)))) ||
       (rc = c.mol.alloc(c.ns)) || (rc = c.molfirst.alloc(c.nmol + 1)) ||
@@ -995,6 +995,7 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   DBuf<double> *dbl[] = {&c.r, &c.f, &c.charge, &c.lj64, &c.erf64, &c.spos, &c.part, &c.acc,
+                         &c.qtrig,
                          &c.kpart, &c.tsum, &c.coef64, &c.r1, &c.pwork, &c.mass,
                          &c.cons_bondq, &c.mol_omega, &c.chargepol, &c.alpha_qq, &c.spol,
                          &c.qpos, &c.mdpart, &c.mdacc, &c.sfpart, &c.sfacc, &c.mdflag,

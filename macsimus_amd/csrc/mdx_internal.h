@@ -154,6 +154,7 @@ struct Ctx {
   DBuf<double> chargepol, alpha_qq;  // [ns]
   DBuf<double> spol;           // [4ns] Drude offset, qpol (cell order)
   DBuf<double> qpos;           // [4nq] charge positions and charges
+  DBuf<double> qtrig;          // [12nq] per direction cos, sin of 2 pi s and 16 pi s
   DBuf<unsigned> excl;        // [ns]
   DBuf<double> lj64;          // [ntypes^2*8]
   DBuf<float> lj32;

@@ -53,19 +53,33 @@ __device__ inline double wave_sum(double v) {
 }
 
 // positions and charges of the charge list: index i < ns is the centre of
-// site i, ns+i its Drude at r_i + rpol_i (POLAR)
+// site i, ns+i its Drude at r_i + rpol_i (POLAR); with them, per direction
+// d the phases e^{2 pi i s} and e^{2 pi i 8 s} of s = frac(x_d / L_d) in
+// FP64 (qtrig[j][d] = cos s, sin s, cos 8s, sin 8s), from which pass 1
+// builds its tables
 __global__ void k_qgather(int nq, int ns, const int *__restrict__ qsite,
                           const double *__restrict__ r, const double *__restrict__ charge,
-                          const double *__restrict__ chargepol, double4 *__restrict__ qpos) {
+                          const double *__restrict__ chargepol, double iLx, double iLy,
+                          double iLz, double4 *__restrict__ qpos, double4 *__restrict__ qtrig) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nq) return;
   const int k = qsite[j];
+  double4 p;
   if (k < ns) {
-    qpos[j] = make_double4(r[3 * k], r[3 * k + 1], r[3 * k + 2], charge[k]);
+    p = make_double4(r[3 * k], r[3 * k + 1], r[3 * k + 2], charge[k]);
   } else {
     const int i = k - ns;
-    qpos[j] = make_double4(r[3 * i] + r[3 * k], r[3 * i + 1] + r[3 * k + 1],
-                           r[3 * i + 2] + r[3 * k + 2], chargepol[i]);
+    p = make_double4(r[3 * i] + r[3 * k], r[3 * i + 1] + r[3 * k + 1],
+                     r[3 * i + 2] + r[3 * k + 2], chargepol[i]);
+  }
+  qpos[j] = p;
+  const double x[3] = {p.x * iLx, p.y * iLy, p.z * iLz};
+  for (int d = 0; d < 3; d++) {
+    const double sd = frac1(x[d]);
+    double c1, s1, c8, s8;
+    cs2pi(sd, c1, s1);
+    cs2pi(8.0 * sd, c8, s8);
+    qtrig[3 * (size_t)j + d] = make_double4(c1, s1, c8, s8);
   }
 }
 
@@ -79,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
     const int *__restrict__ seg_blk, const int *__restrict__ seg_row0,
     const int *__restrict__ seg_rowend, const int *__restrict__ row_kx,
     const int *__restrict__ row_ky, int KXT, int KYT, int KZT, int jper,
-    double *__restrict__ kpart) {
+    double *__restrict__ kpart, const double4 *__restrict__ qtrig) {
   extern __shared__ char smem[];
   C2<R> *Xs = (C2<R> *)smem;            // [kJC][KXT], q folded in
   C2<R> *Ys = Xs + kJC * KXT;           // [kJC][KYT]
@@ -107,8 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
     for (int m = 0; m < 8; m++) { acc[k][m] = 0; mid[k][m] = 0; }
 
   for (int jc = j0; jc < j1; jc += kJC) {
-    // cos/sin tables of this chunk: anchors e^{2 pi i 8m s} and the base
-    // e^{2 pi i s} in FP64, 7 recursion steps in R
+    // cos/sin tables of this chunk: anchors e^{2 pi i 8m s} = (e^{2 pi i
+    // 8s})^m in FP64 from the gathered phases, 7 recursion steps in R with
+    // the base e^{2 pi i s}
     for (int t = threadIdx.x; t < kJC * ntask; t += kBlock) {
       const int jj = t / ntask, task = t % ntask;
       int d, m, KT;
@@ -118,14 +133,15 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
       else { d = 2; m = task - ntx - nty; KT = KZT; dst = Zs + jj * KZT; }
       const int j = jc + jj;
       if (j < j1) {
-        const double4 qp = qpos[j];
-        const double iL = d == 0 ? iLx : (d == 1 ? iLy : iLz);
-        const double x = d == 0 ? qp.x : (d == 1 ? qp.y : qp.z);
-        const double sd = frac1(x * iL);
-        const double qf = d == 0 ? qp.w : 1.0;
-        double c0, s0, c1, s1;
-        cs2pi(8.0 * m * sd, c0, s0);
-        cs2pi(sd, c1, s1);
+        const double4 tr = qtrig[3 * (size_t)j + d];
+        const double qf = d == 0 ? qpos[j].w : 1.0;
+        double c0 = 1, s0 = 0;
+        for (int i = 0; i < m; i++) {
+          const double nc = c0 * tr.z - s0 * tr.w;
+          s0 = c0 * tr.w + s0 * tr.z;
+          c0 = nc;
+        }
+        const double c1 = tr.x, s1 = tr.y;
         c0 *= qf; s0 *= qf;
         R ec = (R)c0, es = (R)s0;
         const R bc = (R)c1, bs = (R)s1;
@@ -554,13 +570,14 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   dim3 g1((K.nseg + 3) / 4, c.nsplit);
   const double4 *qpos = (const double4 *)c.qpos.p;
   k_qgather<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, c.ns, c.q_site.p, c.r.p, c.charge.p,
-                                                  c.polar ? c.chargepol.p : nullptr,
-                                                  (double4 *)c.qpos.p);
+                                          c.polar ? c.chargepol.p : nullptr, 1 / c.L[0],
+                                          1 / c.L[1], 1 / c.L[2], (double4 *)c.qpos.p,
+                                          (double4 *)c.qtrig.p);
   if (c.timing) c.tm.start(T_KPASS1, st);
   k_sfac<R><<<g1, kBlock, shmem, st>>>(
       c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
       c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p, KXT, KYT, KZT,
-      jper, c.kpart.p);
+      jper, c.kpart.p, (const double4 *)c.qtrig.p);
   MDX_HIP(hipGetLastError());
   if (c.timing) { c.tm.stop(T_KPASS1, st); c.tm.start(T_KCOEF, st); }
   const size_t nt = (size_t)K.nseg * 64 * 64;
