@@ -117,12 +117,24 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     C2 = cutoff if lj_cutoff is None else lj_cutoff
     sstab = (abi.LJ * (nt * nt))()
     corr = np.zeros((nt, nt))
+    # combining rule of a pair (sim/lj/sitesite.c:33-58, 66-112): the
+    # file's comb_rule, with the square-root energy rule for sites of
+    # alpha <= 0, or-ed over the pair; the Kirkwood rule (bit 1 clear) is
+    # not restated
+    rule = [1] * nt
+    if "comb_rule" in ff:
+        alpha = ff.get("type_alpha", [0.0] * nt)
+        rule = [int(ff["comb_rule"]) | (1 if alpha[i] <= 0 else 0) for i in range(nt)]
     for i in range(nt):
         for j in range(i + 1):
             if EvdW[i] == 0 or EvdW[j] == 0:
                 ss, c = abi.LJ(), 0.0
             else:
-                ss, c = prov.setss(EvdW[i], RvdW[i], EvdW[j], RvdW[j], 1, C2)
+                cr = rule[i] | rule[j]
+                if not cr & 1:
+                    raise ValueError("types %d-%d: the Kirkwood combining rule is not supported"
+                                     % (i, j))
+                ss, c = prov.setss(EvdW[i], RvdW[i], EvdW[j], RvdW[j], cr, C2)
             sstab[i * nt + j] = ss
             sstab[j * nt + i] = ss
             corr[i, j] = corr[j, i] = c

@@ -150,3 +150,21 @@ def test_kspace_pass2_on_matrix_cores(name):
         e64.set_kspace_mfma(True)
     eng.close()
     e64.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_engine_from_cook_input_files(mode, tmp_path):
+    """the engine built from the .ble force field and .def deck the brine
+    fixture came from (read by macsimus_amd.cookio) gives cook's forces"""
+    import test_cookio
+    from macsimus_amd import cookio
+    d, meta = oracle.load_fixture("brine506")
+    species, types, _ = test_cookio.CASES["brine506"]()
+    ble, dfn = test_cookio.write_deck(str(tmp_path), meta, species, types, None)
+    bs, _ = cookio.system_from_cook(ble, dfn, provider=engine_provider())
+    eng = Engine(bs, mode)
+    f, out = eng.forces(d["r"], 1)
+    e = rel_rms(f, d["f"])
+    assert e < TOL[mode], e
+    assert abs(out.En_pot + out.En_el - meta["Epot"]) < (1e-10 if mode == abi.MDX_FP64 else 2e-6) * abs(meta["Epot"])
+    eng.close()
