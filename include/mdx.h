@@ -209,6 +209,28 @@ int mdx_set_overlap(mdx_ctx_t *ctx, int on);
 int mdx_set_aspc(mdx_ctx_t *ctx, int k, int kk, const double *hist);
 int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
 
+/* RDF histograms of the measured force calls, as cook records them for
+   rdf.grid and rdf.cutoff: initrdf() (sim/simmeas.c:655-767), the counts
+   of cook/ljqqnp.c:218-225 (for Drude sites, of the centres,
+   cook/ljqqpol.c:477-485) and advancerdf() (sim/simmeas.c:777-787).
+   mdx_set_rdf starts them from zero: grid > 0 one record per pair of site
+   types (i, j<=i) in the order i = 0.., j = 0..i; grid < 0 one record of
+   all pairs; grid = 0 off (the default).  cook truncates rdf.grid to an
+   integer (abs() of a double); cutoff in [0, 1e4] gives
+   nhist = (int)(cutoff*|grid| + 1.0000001) bins (at most 100000).  A pair
+   is counted when it is not excluded and closer than the r-space cutoff,
+   in bin (int)(r*|grid|) when that is below nhist.  Every measured force
+   call (mdx_forces with measure, measured steps of mdx_md_steps) adds its
+   pairs, one measurement and the box volume; for Drude systems only the
+   last SCF iteration of a step is measured, where cook counts every
+   iteration's forces() (the same g(r), counts divided by the iterations).
+   mdx_get_rdf: the records x bins counts (hist NULL: only the sizes), the
+   measurements and the summed volume (cook's nmeas and V).  cook keeps
+   unsigned 32-bit counts; these are 64-bit. */
+int mdx_set_rdf(mdx_ctx_t *ctx, int grid, double cutoff);
+int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int *nhist,
+                int *nmeas, double *V);
+
 /* ---- diagnostics (no counterpart in cook) -----------------------------
    mdx_cook_cells: cook's linked-cell index of each site for the context's
    cook grid (cook/linklist.c:304-337 with No.cell), to compare the cell

@@ -21,10 +21,17 @@ evaluates them).
 `system_from_cook(ble, def)` builds the engine's system from the two, as
 cook's set-up does (initss, initrspace, Ewald parameters from el.epsr and
 el.epsk, LJcutoff).
+
+`write_rdf` / `read_rdf` write and read cook's binary RDF histograms
+(`.rdf`, saverdf() of sim/simmeas.c:813-825) so that cook's own tools
+(util/rdfg.c) take the engine's histograms.
 """
 import ast
 import operator
 import re
+import struct
+
+import numpy as np
 
 
 class CookInputError(ValueError):
@@ -278,3 +285,123 @@ def system_from_cook(ble_path, def_path, provider=None, **kw):
         args["epsinf"] = float(d["el.epsinf"])
     args.update(kw)
     return build_system(ff, counts, L, cutoff, provider=provider, **args), d
+
+
+# ---------------------------------------------------------------- .rdf
+def _var_record(data):
+    """one record of cook's variable-length files (gen/varfile.c:61-74):
+    int4 size, the bytes, int4 checksum 0x4A4B + the sum of the bytes"""
+    return struct.pack("<i", len(data)) + data + struct.pack("<I", (0x4A4B + sum(data)) & 0xffffffff)
+
+
+def _var_records(data):
+    recs, o = [], 0
+    while True:
+        if o + 4 > len(data):
+            raise CookInputError("varfile: no end record")
+        (n,) = struct.unpack_from("<i", data, o)
+        o += 4
+        if n == 0:
+            return recs
+        rec = data[o:o + n]
+        if len(rec) != n or o + n + 4 > len(data):
+            raise CookInputError("varfile: truncated record")
+        (chk,) = struct.unpack_from("<I", data, o + n)
+        if (0x4A4B + sum(rec)) & 0xffffffff != chk:
+            raise CookInputError("varfile: checksum")
+        recs.append(rec)
+        o += n + 4
+
+
+def _rdf_index(ti, tj):
+    hi, lo = max(ti, tj), min(ti, tj)
+    return hi * (hi + 1) // 2 + lo
+
+
+def rdf_npair(ff, counts, merged=False):
+    """npair of each RDF record as cutcorr() counts it (sim/siminit.c:
+    665-720): for every species pair and every site pair of the two
+    species, N(N-1)/2 (same species) or N*M pairs of the sites' type pair;
+    then N per intramolecular pair that is not excluded"""
+    nt = len(ff["types"])
+    out = np.zeros(1 if merged else nt * (nt + 1) // 2)
+    sp = ff["species"]
+    for n in range(len(sp)):
+        for m in range(n + 1):
+            npr = counts[n] * (counts[n] - 1) / 2 if m == n else counts[n] * counts[m]
+            for ti in sp[n]["site_type"]:
+                for tj in sp[m]["site_type"]:
+                    out[0 if merged else _rdf_index(ti, tj)] += npr
+        ns = len(sp[n]["site_type"])
+        ex = set()
+        for i, lst in enumerate(sp[n]["site_excl"]):
+            for j in lst:
+                ex.add((max(i, j), min(i, j)))
+        for i in range(ns):
+            for j in range(i):
+                if (i, j) not in ex:
+                    t = (sp[n]["site_type"][i], sp[n]["site_type"][j])
+                    out[0 if merged else _rdf_index(*t)] += counts[n]
+    return out
+
+
+def write_rdf(path, rdf, ff, counts):
+    """cook's .rdf file of the histograms rdf = Engine.rdf() (hist
+    [records, bins], nmeas, V, grid as given to set_rdf) for the force
+    field ff with counts molecules per species: the grid as given, the
+    number of site types, then per record the rdf_t of sim/rdf.h without
+    its size field (names, type indices, site counts, npair, grid, V,
+    nmeas, nhist, 32-bit counts) and the end record.  Counts above
+    2^32-1 do not fit cook's record and raise ValueError."""
+    grid = rdf["grid"]
+    g = int(grid)
+    hist = np.asarray(rdf["hist"])
+    nt = len(ff["types"])
+    merged = g < 0
+    nrec = 1 if merged else nt * (nt + 1) // 2
+    if hist.ndim != 2 or hist.shape[0] != nrec:
+        raise ValueError("hist must have %d records" % nrec)
+    if hist.size and int(hist.max()) > 0xffffffff:
+        raise ValueError("a histogram count exceeds cook's unsigned 32-bit field")
+    sitecount = [0] * nt
+    for sp, N in zip(ff["species"], counts):
+        for t in sp["site_type"]:
+            sitecount[t] += N
+    npair = rdf_npair(ff, counts, merged)
+    name8 = lambda s: s.encode()[:8].ljust(8, b"\0")
+    out = [_var_record(struct.pack("<d", float(grid))), _var_record(struct.pack("<i", nt))]
+    pairs = [(0, 0)] if merged else [(i, j) for i in range(nt) for j in range(i + 1)]
+    for k, (i, j) in enumerate(pairs):
+        if merged:
+            names, ns = (name8("*"), name8("*")), (sum(sitecount),) * 2
+        else:
+            names, ns = (name8(ff["types"][i]), name8(ff["types"][j])), (sitecount[i], sitecount[j])
+        rec = (names[0] + names[1] + struct.pack("<2i2i", i, j, *ns) + b"\0" * 4 +
+               struct.pack("<3d2i", npair[k], float(abs(g)), rdf["V"], rdf["nmeas"],
+                           hist.shape[1]) +
+               hist[k].astype("<u4").tobytes())
+        out.append(_var_record(rec))
+    with open(path, "wb") as f:
+        f.write(b"".join(out) + struct.pack("<i", 0))
+
+
+def read_rdf(path):
+    """(grid, number of site types, records) of a cook .rdf file; each
+    record a dict of names, indx, ns, npair, grid, V, nmeas, hist"""
+    recs = _var_records(open(path, "rb").read())
+    if len(recs) < 2:
+        raise CookInputError("%s: no header" % path)
+    (grid,) = struct.unpack("<d", recs[0])
+    (nsites,) = struct.unpack("<i", recs[1])
+    out = []
+    for rec in recs[2:]:
+        nhist = struct.unpack_from("<i", rec, 64)[0]
+        if len(rec) != 68 + 4 * nhist:
+            raise CookInputError("%s: record length %d for %d bins" % (path, len(rec), nhist))
+        npair, grd, V = struct.unpack_from("<3d", rec, 36)
+        out.append(dict(names=[rec[0:8].split(b"\0")[0].decode(), rec[8:16].split(b"\0")[0].decode()],
+                        indx=list(struct.unpack_from("<2i", rec, 16)),
+                        ns=list(struct.unpack_from("<2i", rec, 24)), npair=npair, grid=grd, V=V,
+                        nmeas=struct.unpack_from("<i", rec, 60)[0],
+                        hist=np.frombuffer(rec, "<u4", nhist, 68).astype(np.uint64)))
+    return grid, nsites, out

@@ -519,6 +519,82 @@ __global__ __launch_bounds__(kBlock) void k_pair_all(
   }
 }
 
+// RDF histograms of a measured call (cook/ljqqnp.c:218-225; for Drude
+// sites the centres, cook/ljqqpol.c:477-485): every non-excluded pair with
+// rr below the cutoff once, from its lower cell-order index, in bin
+// (int)(sqrt(rr)*grid) of the record of its type pair (ti, tj<=ti) at
+// ti(ti+1)/2+tj, or of record 0 when merged; bins from nh on are dropped.
+// Separations as in k_pair (cell stencil) or k_pair_all (minimum image,
+// ALL).  A workgroup counts into LDS (nlds 32-bit entries) when the records
+// fit and flushes its nonzero bins with 64-bit atomics, else it adds to the
+// records in memory; it strides over the sites so that it flushes once.
+template <bool ALL>
+__global__ __launch_bounds__(kBlock) void k_rdf(int ns, Grid g, int w, double cq,
+                                                const double4 *__restrict__ spos,
+                                                const int4 *__restrict__ sinfo,
+                                                const int *__restrict__ scell,
+                                                const int *__restrict__ cstart, int merged,
+                                                double grid, int nh, int nlds,
+                                                unsigned long long *__restrict__ hist) {
+  extern __shared__ unsigned lh[];
+  for (int k = threadIdx.x; k < nlds; k += kBlock) lh[k] = 0;
+  __syncthreads();
+  const double iLx = 1 / g.Lx, iLy = 1 / g.Ly, iLz = 1 / g.Lz;
+  for (int s = blockIdx.x * kBlock + threadIdx.x; s < ns; s += gridDim.x * kBlock) {
+    const double4 pi = spos[s];
+    const int4 ii = sinfo[s];
+    const int ti = ii.x & 0xffff;
+    auto count = [&](const int4 jj, double rr) {
+      if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) return;  // EXCL
+      const int b = (int)(sqrt(rr) * grid);
+      if (b >= nh) return;
+      const int tj = jj.x & 0xffff, hi = max(ti, tj), lo = min(ti, tj);
+      const int k = (merged ? 0 : hi * (hi + 1) / 2 + lo) * nh + b;
+      if (nlds) atomicAdd(&lh[k], 1u);
+      else atomicAdd(&hist[k], 1ull);
+    };
+    if (ALL) {
+      for (int j = s + 1; j < ns; j++) {
+        const double4 pj = spos[j];
+        double dxr = pi.x - pj.x, dyr = pi.y - pj.y, dzr = pi.z - pj.z;
+        dxr -= g.Lx * rint(dxr * iLx);
+        dyr -= g.Ly * rint(dyr * iLy);
+        dzr -= g.Lz * rint(dzr * iLz);
+        const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+        if (rr < cq) count(sinfo[j], rr);
+      }
+      continue;
+    }
+    const int c = scell[s];
+    const int cz = c % g.nz, cy = (c / g.nz) % g.ny, cx = c / (g.nz * g.ny);
+    for (int dx = -w; dx <= w; dx++) {
+      int nx = cx + dx;
+      double sx = 0;
+      if (nx < 0) { nx += g.nx; sx = -g.Lx; } else if (nx >= g.nx) { nx -= g.nx; sx = g.Lx; }
+      for (int dy = -w; dy <= w; dy++) {
+        int ny = cy + dy;
+        double sy = 0;
+        if (ny < 0) { ny += g.ny; sy = -g.Ly; } else if (ny >= g.ny) { ny -= g.ny; sy = g.Ly; }
+        for (int dz = -w; dz <= w; dz++) {
+          int nz = cz + dz;
+          double sz = 0;
+          if (nz < 0) { nz += g.nz; sz = -g.Lz; } else if (nz >= g.nz) { nz -= g.nz; sz = g.Lz; }
+          const int nc = (nx * g.ny + ny) * g.nz + nz;
+          for (int j = max(cstart[nc], s + 1); j < cstart[nc + 1]; j++) {
+            const double4 pj = spos[j];
+            const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy), dzr = pi.z - (pj.z + sz);
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            if (rr < cq) count(sinfo[j], rr);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nlds; k += kBlock)
+    if (lh[k]) atomicAdd(&hist[k], (unsigned long long)lh[k]);
+}
+
 // force rows = sum of the stencil parts in a fixed order
 __global__ void k_pair_sum(long n3, int nparts, const double *__restrict__ pf,
                            double *__restrict__ f) {
@@ -743,6 +819,32 @@ int launch_pair(Ctx &c, int measure) {
   return c.polar ? launch_pair_t<R, true>(c, measure) : launch_pair_t<R, false>(c, measure);
 }
 
+// records that fit a workgroup's LDS histogram (64 KiB of 32-bit counts)
+constexpr int kRdfLds = 16384;
+
+// RDF records of a measured call on the context stream, and advancerdf()
+// (sim/simmeas.c:777-787): one more measurement of the current volume
+int launch_rdf(Ctx &c) {
+  const Grid g = make_grid(c, c.nc);
+  const int nbins = c.rdf_nrec * c.rdf_nh;
+  const int nlds = nbins <= kRdfLds ? nbins : 0;
+  const int nb = std::min(nblk(c.ns), 1024);
+  const double cq = c.cutoff * c.cutoff;
+  const size_t lds = sizeof(unsigned) * nlds;
+  if (c.allpair)
+    k_rdf<true><<<nb, kBlock, lds, c.stream>>>(
+        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
+        c.cstart.p, c.rdf_grid < 0, std::abs(c.rdf_grid), c.rdf_nh, nlds, c.rdf_hist.p);
+  else
+    k_rdf<false><<<nb, kBlock, lds, c.stream>>>(
+        c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
+        c.cstart.p, c.rdf_grid < 0, std::abs(c.rdf_grid), c.rdf_nh, nlds, c.rdf_hist.p);
+  MDX_HIP(hipGetLastError());
+  c.rdf_nmeas++;
+  c.rdf_V += c.L[0] * c.L[1] * c.L[2];
+  return 0;
+}
+
 }  // namespace
 
 namespace mdx {
@@ -787,6 +889,7 @@ int forces_device(Ctx &c, int measure) {
   }
   MDX_HIP(hipGetLastError());
   if (c.timing) c.tm.stop(T_EXCL, c.stream);
+  if (measure && c.rdf_grid && (rc = launch_rdf(c))) return rc;
   if (fork) {
     MDX_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     if ((rc = kspace_add(c, c.stream))) return rc;
@@ -915,6 +1018,16 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   }
   c.nq = (int)qsite.size();
   const int nrow = c.polar ? 2 * c.ns : c.ns;
+  // exclusion masks made symmetric: cook finds an exclusion in the table
+  // of the later site of the pair (cook/ljqqnp.c:103-118), a caller may set
+  // either bit, and the kernels test the mask of one site
+  std::vector<unsigned> excl(s->excl, s->excl + c.ns);
+  for (int n = 0; n < c.nmol; n++) {
+    const int m0 = s->mol_first[n], m1 = s->mol_first[n + 1];
+    for (int i = m0; i < m1; i++)
+      for (int j = m0; j < m1; j++)
+        if ((s->excl[i] >> (j - m0)) & 1) excl[j] |= 1u << (i - m0);
+  }
 
   // LJ table, FP64 and FP32 (A4 of LJ-free pairs is zero)
   const int nt2 = c.ntypes * c.ntypes;
@@ -953,7 +1066,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   if (!up(c.type.p, s->site_type, sizeof(int) * c.ns) || !up(c.mol.p, mol.data(), sizeof(int) * c.ns) ||
       !up(c.molfirst.p, s->mol_first, sizeof(int) * (c.nmol + 1)) ||
       !up(c.charge.p, s->charge, sizeof(double) * c.ns) ||
-      !up(c.excl.p, s->excl, sizeof(unsigned) * c.ns) ||
+      !up(c.excl.p, excl.data(), sizeof(unsigned) * c.ns) ||
       (c.nq && !up(c.q_site.p, qsite.data(), sizeof(int) * c.nq)) ||
       !up(c.lj64.p, lj.data(), sizeof(double) * lj.size()) ||
       !up(c.lj32.p, lj32.data(), sizeof(float) * lj32.size()) ||
@@ -1015,6 +1128,7 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.d_grp_start.release();
   c.d_grp_base.release();
   c.d_grp_nb.release();
+  c.rdf_hist.release();
   c.tm.release();
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);
@@ -1215,6 +1329,53 @@ extern "C" int mdx_set_kspace_mfma(mdx_ctx_t *h, int on) {
   if (on && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_mfma: MDX_MIXED only");
   h->c.kmfma = on != 0;
   h->c.gen++;
+  return 0;
+}
+
+extern "C" int mdx_set_rdf(mdx_ctx_t *h, int grid, double cutoff) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_rdf: NULL");
+  Ctx &c = h->c;
+  (void)hipSetDevice(c.device);
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  c.rdf_grid = c.rdf_nmeas = 0;
+  c.rdf_V = 0;
+  if (!grid) {
+    c.rdf_hist.release();
+    c.rdf_nh = c.rdf_nrec = 0;
+    return 0;
+  }
+  // initrdf() (sim/simmeas.c:655-684): range of the cutoff, nhist bins
+  if (!(cutoff >= 0 && cutoff <= 1e4)) return fail(MDX_E_ARG, "mdx_set_rdf: cutoff out of [0, 1e4]");
+  const double ag = std::fabs((double)grid), nhd = cutoff * ag + 1.0000001;
+  if (nhd > 100000) return fail(MDX_E_ARG, "mdx_set_rdf: more than 100000 histogram bins");
+  const int nh = (int)nhd;
+  const long nrec = grid < 0 ? 1 : (long)c.ntypes * (c.ntypes + 1) / 2;
+  if (nrec * nh > 0x7fffffffL) return fail(MDX_E_ARG, "mdx_set_rdf: too many records x bins");
+  int rc;
+  if ((rc = c.rdf_hist.alloc((size_t)nrec * nh))) return rc;
+  k_zero<<<nblk(nrec * nh), kBlock, 0, c.stream>>>((double *)c.rdf_hist.p, (size_t)nrec * nh);
+  MDX_HIP(hipGetLastError());
+  c.rdf_grid = grid;
+  c.rdf_nh = nh;
+  c.rdf_nrec = (int)nrec;
+  return 0;
+}
+
+extern "C" int mdx_get_rdf(mdx_ctx_t *h, unsigned long long *hist, long n, int *nrec,
+                           int *nhist, int *nmeas, double *V) {
+  if (!h) return fail(MDX_E_ARG, "mdx_get_rdf: NULL");
+  Ctx &c = h->c;
+  if (!c.rdf_grid) return fail(MDX_E_ARG, "mdx_get_rdf: RDF histograms are off (mdx_set_rdf)");
+  if (nrec) *nrec = c.rdf_nrec;
+  if (nhist) *nhist = c.rdf_nh;
+  if (nmeas) *nmeas = c.rdf_nmeas;
+  if (V) *V = c.rdf_V;
+  if (!hist) return 0;
+  if (n != (long)c.rdf_nrec * c.rdf_nh)
+    return fail(MDX_E_ARG, "mdx_get_rdf: n is not records x bins");
+  (void)hipSetDevice(c.device);
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  MDX_HIP(hipMemcpy(hist, c.rdf_hist.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -23,6 +23,7 @@ class Engine:
         rc = self.lib.mdx_create(C.byref(self.ctx), C.byref(built_system.struct), mode)
         abi.check(self.lib, rc, "mdx_create")
         self.mode = mode
+        self._rdf_grid = 0
 
     def close(self):
         if self.ctx:
@@ -164,6 +165,27 @@ class Engine:
     def set_stencil(self, w, pcols=0):
         rc = self.lib.mdx_set_stencil(self.ctx, int(w), int(pcols))
         abi.check(self.lib, rc, "mdx_set_stencil")
+
+    # ---- RDF histograms of the measured calls
+    def set_rdf(self, grid, cutoff):
+        """start the histograms from zero (cook's rdf.grid, truncated to an
+        integer as initrdf() does, and rdf.cutoff); grid 0 turns them off"""
+        rc = self.lib.mdx_set_rdf(self.ctx, int(grid), float(cutoff))
+        abi.check(self.lib, rc, "mdx_set_rdf")
+        self._rdf_grid = grid
+
+    def rdf(self):
+        """dict(hist [records, bins] uint64, nmeas, V, grid): records of
+        the site-type pairs (i, j<=i) in cook's order, or the one merged
+        record for grid < 0"""
+        nrec, nh, nm, V = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        rc = self.lib.mdx_get_rdf(self.ctx, None, 0, C.byref(nrec), C.byref(nh), None, None, None)
+        abi.check(self.lib, rc, "mdx_get_rdf")
+        hist = np.zeros((nrec.value, nh.value), np.uint64)
+        rc = self.lib.mdx_get_rdf(self.ctx, hist.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                  hist.size, None, None, C.byref(nm), C.byref(V))
+        abi.check(self.lib, rc, "mdx_get_rdf")
+        return dict(hist=hist, nmeas=nm.value, V=V.value, grid=self._rdf_grid)
 
     # ---- device-resident MD
     def upload_state(self, r, r1):
@@ -341,3 +363,7 @@ def declare_extras(lib):
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
     lib.mdx_get_aspc.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_double)]
+    lib.mdx_set_rdf.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    lib.mdx_get_rdf.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_long,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_double)]

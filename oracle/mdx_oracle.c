@@ -584,7 +584,26 @@ typedef struct {
   double U, Enel, Envir, Pvir[6];
   long long npair, nlj;
   int measure;
+  /* RDF histograms of the measure path: records [nrec][nh] */
+  unsigned long long *rdf;
+  double rdf_grid;
+  int rdf_nh, rdf_merged;
 } orc_acc;
+
+/* one pair of site types ti, tj at squared distance rr into the RDF
+   records (cook/ljqqnp.c:218-225): bin (int)(r*grid), dropped from nhist
+   on; the record of the type pair (i, j<=i) in the order initrdf() makes
+   them (sim/simmeas.c:737-767), or the one merged record for grid<0 */
+static void orc_rdf_count(orc_acc *a, int ti, int tj, double rr)
+{
+  int b = (int)(sqrt(rr) * a->rdf_grid), p = 0;
+  if (b >= a->rdf_nh) return;
+  if (!a->rdf_merged) {
+    if (ti < tj) { int t = ti; ti = tj; tj = t; }
+    p = ti * (ti + 1) / 2 + tj;
+  }
+  a->rdf[(size_t)p * a->rdf_nh + b]++;
+}
 
 static void orc_addf(orc_acc *a, double *f1, double *f2, double f, const double *dr)
 {
@@ -658,6 +677,8 @@ static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
         continue;
       }
     }
+    /* RDF of the centres (cook/ljqqpol.c:477-485) */
+    if (a->rdf) orc_rdf_count(a, ls->st, l->st, rr);
     if (rr < ss->C2q) {
       if (rr < ss->C1q) {
         double x = ss->Sq / rr, y = x * x * x, z = y * y;
@@ -754,6 +775,7 @@ static void orc_ljqq(orc_acc *a, orc_site *l, orc_site *ls)
       if (a->measure) a->Enel += eru * qq;
       f += byerd * qq;
     }
+    if (a->rdf) orc_rdf_count(a, ls->st, l->st, rr);
   addUf:
     a->U += U;
     ls->f[0] += fdr = f * dr[0]; l->f[0] -= fdr;
@@ -881,8 +903,38 @@ static int orc_pairs(orc_acc *a, const double (*r)[3], double (*f)[3], int *cell
 
 /* forces(B,A) of cook (cook/forces.c:180-203, serial): f is zeroed,
    Ewald(1), En.el=Ewald(2), pairforces() with En.el saved/restored. */
+static int orc_forces(const mdx_system_t *S, const double *r, double *f, mdx_out_t *out,
+                      int measure, int nthreads, int *cellid, orc_acc *rdf);
+
 int oracle_forces(const mdx_system_t *S, const double *r, double *f,
                   mdx_out_t *out, int measure, int nthreads, int *cellid)
+{
+  return orc_forces(S, r, f, out, measure, nthreads, cellid, NULL);
+}
+
+/* a measured forces() call with the RDF histograms of its pairs added to
+   hist (initrdf() records, sim/simmeas.c:655-767): grid is truncated to an
+   integer as cook's abs() does, grid>0 one record per type pair (i, j<=i),
+   grid<0 one record of all pairs, nh = (int)(cutoff*|grid|+1.0000001)
+   bins each.  Returns MDX_E_ARG for grid 0 or nh out of [1, 100000]. */
+int oracle_forces_rdf(const mdx_system_t *S, const double *r, double *f, mdx_out_t *out,
+                      int nthreads, int *cellid, int grid, double cutoff,
+                      unsigned long long *hist)
+{
+  orc_acc h;
+  int ag = grid < 0 ? -grid : grid;
+  memset(&h, 0, sizeof(h));
+  if (!ag || !(cutoff >= 0 && cutoff <= 1e4)) return MDX_E_ARG;
+  h.rdf_nh = (int)(cutoff * ag + 1.0000001);
+  if (h.rdf_nh > 100000) return MDX_E_ARG;
+  h.rdf = hist;
+  h.rdf_grid = ag;
+  h.rdf_merged = grid < 0;
+  return orc_forces(S, r, f, out, 1, nthreads, cellid, &h);
+}
+
+static int orc_forces(const mdx_system_t *S, const double *r, double *f, mdx_out_t *out,
+                      int measure, int nthreads, int *cellid, orc_acc *rdf)
 {
   /* POLAR: r and f hold 2*ns rows, centres then Drude offsets/forces
      (polar_off = No.s*24, sim/siminit.c:640) */
@@ -914,6 +966,7 @@ int oracle_forces(const mdx_system_t *S, const double *r, double *f,
   Enel = nq ? orc_ewald(S, nq, idx, q, (const double (*)[3])pos, (double (*)[3])f,
                         measure, Pvir, nthreads, sumq_self) : 0;
   memset(&a, 0, sizeof(a));
+  if (rdf) a = *rdf;
   a.S = S;
   a.measure = measure;
   err = orc_pairs(&a, (const double (*)[3])r, (double (*)[3])f, cellid);

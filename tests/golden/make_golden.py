@@ -297,6 +297,83 @@ def traj_case(name, species, types, counts, L, cutoff, nsteps, thermostat="none"
     print(name, len(dumps), "dumps")
 
 
+def parse_rdf(path):
+    """cook's binary .rdf file (saverdf(), sim/simmeas.c:812-825): records
+    <int4 size><data><int4 checksum> (gen/varfile.c); the grid, the number
+    of site types, then per type pair (i, j<=i) the rdf_t of sim/rdf.h
+    without its leading size field: name[2][8], indx[2], ns[2] (int4),
+    4 bytes of alignment, npair, grid, V (double), nmeas, nhist (int4),
+    hist[nhist] (unsigned int4)."""
+    import struct
+    data = open(path, "rb").read()
+    recs, o = [], 0
+    while o + 4 <= len(data):
+        (n,) = struct.unpack_from("<i", data, o)
+        o += 4
+        if n == 0:
+            break
+        rec = data[o:o + n]
+        o += n
+        (chk,) = struct.unpack_from("<i", data, o)
+        o += 4
+        if (0x4A4B + sum(rec)) & 0xffffffff != chk & 0xffffffff:
+            raise RuntimeError("%s: checksum" % path)
+        recs.append(rec)
+    grid = struct.unpack("<d", recs[0])[0]
+    nsites = struct.unpack("<i", recs[1])[0]
+    pairs = []
+    for rec in recs[2:]:
+        names = (rec[0:8].split(b"\0")[0].decode(), rec[8:16].split(b"\0")[0].decode())
+        indx = struct.unpack_from("<2i", rec, 16)
+        ns = struct.unpack_from("<2i", rec, 24)
+        npair, grd, V = struct.unpack_from("<3d", rec, 36)
+        nmeas, nhist = struct.unpack_from("<2i", rec, 60)
+        hist = np.frombuffer(rec, dtype="<u4", count=nhist, offset=68).copy()
+        pairs.append(dict(names=names, indx=list(indx), ns=list(ns), npair=npair, grid=grd, V=V,
+                          nmeas=nmeas, nhist=nhist, hist=hist))
+    return grid, nsites, pairs
+
+
+def rdf_case(name, species, types, counts, L, cutoff, nsteps, grid):
+    """site-site RDF histograms of cook's measure path (cook/ljqqnp.c:
+    218-225): a short MD run with rdf.grid, every force call dumped by
+    -v128 (dump 0: initial evaluation, dump k: step k) and the binary
+    histograms saved by cook at the end, parsed and as the file's bytes."""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "noint=%d" % nsteps, "T=298", "epsc=1e-12", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="none"\nrdf.grid=%g\nrdf.cutoff=%g\nno=1;\n'
+            % (grid, cutoff))
+        run([COOK, "sys", "md", "-v128"], td)
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "md.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "md.%d.dump" % k)))
+            k += 1
+        fn = [f for f in os.listdir(td) if f.endswith(".rdf")]
+        g, nsites, pairs = parse_rdf(os.path.join(td, fn[0]))
+        raw = open(os.path.join(td, fn[0]), "rb").read()
+        meta = parse_prt(os.path.join(td, "md.prt"))
+    out = dict(L=dumps[0]["L"], q=dumps[0]["q"], mol=dumps[0]["mol"], site=dumps[0]["site"],
+               r=np.stack([d["r"] for d in dumps]))
+    for p in pairs:
+        out["hist_%d_%d" % tuple(p["indx"])] = p["hist"]
+    out["rdf_file"] = np.frombuffer(raw, np.uint8)
+    meta.update(dict(name=name, counts=counts, L=L, epsr=0.05, epsk=0.05, h=0.002,
+                     ndumps=len(dumps), rdf_grid=g, rdf_nsites=nsites, rdf_cutoff=cutoff,
+                     rdf_pairs=[{k: v for k, v in p.items() if k != "hist"} for p in pairs],
+                     forcefield=ff_arrays(species, types)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps", [(p["indx"], p["nmeas"], int(p["hist"].sum())) for p in pairs])
+
+
 def polar_case(name, species, types, counts, L, cutoff, polar):
     """Drude (POLAR=0) force evaluations: Ewaldtest iterates forces() +
     selffield() (sim/ewald.c:1247-1257); every forces() call is dumped with
@@ -389,6 +466,10 @@ def main(argv):
             "polar306_aspc", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 300], 21.10, 9.0,
             {"O": (1.44, -8.0), "CL": (3.5, -8.0)}, 8),
+        # site-site RDF histograms of the small brine over 3 MD steps
+        "brine506_rdf": lambda: rdf_case(
+            "brine506_rdf", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 3, 20.0),
         # NVE trajectory of 216 SPC/E (SHAKE + leap-frog)
         "spce216_nve": lambda: traj_case(
             "spce216_nve", [spce], ["H", "O"], [216], 18.6206, 8.0, 8),

@@ -76,6 +76,37 @@ def forces(bs, r, measure=1, nthreads=None, cellid=False):
     return (f, out, cid) if cellid else (f, out)
 
 
+def rdf_shape(ntypes, grid, cutoff):
+    """(records, bins) of cook's RDF histograms (sim/simmeas.c:655-767)"""
+    g = int(grid)
+    nrec = 1 if g < 0 else ntypes * (ntypes + 1) // 2
+    return nrec, int(cutoff * abs(g) + 1.0000001)
+
+
+def forces_rdf(bs, r, grid, cutoff, hist=None, nthreads=None):
+    """a measured force call adding its RDF histograms to hist (records of
+    type pairs (i, j<=i), uint64); returns (f, out, hist)"""
+    lib = load()
+    dp = C.POINTER(C.c_double)
+    lib.oracle_forces_rdf.argtypes = [C.POINTER(abi.System), dp, dp, C.POINTER(abi.Out),
+                                      C.c_int, C.POINTER(C.c_int), C.c_int, C.c_double,
+                                      C.POINTER(C.c_ulonglong)]
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    f = np.zeros_like(r)
+    out = abi.Out()
+    shape = rdf_shape(bs.struct.ntypes, grid, cutoff)
+    if hist is None:
+        hist = np.zeros(shape, np.uint64)
+    if hist.shape != shape or hist.dtype != np.uint64 or not hist.flags.c_contiguous:
+        raise ValueError("hist must be a C-contiguous uint64 array of shape %s" % (shape,))
+    rc = lib.oracle_forces_rdf(C.byref(bs.struct), r.ctypes.data_as(dp), f.ctypes.data_as(dp),
+                               C.byref(out), nthreads or os.cpu_count() or 1, None, int(grid),
+                               cutoff, hist.ctypes.data_as(C.POINTER(C.c_ulonglong)))
+    if rc:
+        raise RuntimeError("oracle_forces_rdf failed (%d)" % rc)
+    return f, out, hist
+
+
 def selffield(bs, r, f, out, omega):
     """one selffield() iteration on r (2ns rows, Drude offsets updated) and
     f (2ns rows, Drude forces passed to the centres); adds to out."""
