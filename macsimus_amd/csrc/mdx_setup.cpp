@@ -88,8 +88,8 @@ extern "C" int mdx_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej,
   ss->C2q = C2 * C2;
   if (corr) {
     // homogeneous tail: analytic quartic part + 2-point Gauss in 1/r on
-    // [C1,inf) with 100 intervals (sim/setss.c:470-499)
-    const int ng = 100;
+    // [C1,inf) with NGauss4 = 64 intervals (sim/setss.c:49,470-499)
+    const int ng = 64;
     const double g = 0.2113248654051871;
     double c0 = 4 * kPi * ss->A *
                 (C1 * (p6(C1) / 7 - 0.4 * std::pow(C1, 4) * C2 * C2 + C1 * C1 * std::pow(C2, 4) / 3) -

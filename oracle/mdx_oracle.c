@@ -94,8 +94,9 @@ int oracle_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej, double Rj,
   ss->C1q = C1 * C1;
   if (corr) {
     /* 4 pi int_C1^C2 A(r^2-C2^2)^2 r^2 analytically (with minus sign) +
-       4 pi int_C1^inf U r^2 by 2-point Gauss in x=1/r, NGauss4 intervals */
-    const int NG = 100;
+       4 pi int_C1^inf U r^2 by 2-point Gauss in x=1/r, NGauss4 = 64
+       intervals (sim/setss.c:49,470-499) */
+    const int NG = 64;
     const double g4 = 0.2113248654051871;
     double c0 = 4 * PI * ss->A *
       (C1 * (pow6(C1) / 7 - 0.4 * sqr(C1) * sqr(C1) * sqr(C2) + sqr(C1) * sqr(C2) * sqr(C2) / 3)
