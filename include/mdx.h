@@ -49,6 +49,18 @@ typedef struct mdx_lj_s {
   double C1q, C2q, A, A4;
 } mdx_lj_t;
 
+/* The same for cook built with the exp-6 form (sim/exp6/sitesite.h:59-75,
+   constants of initssaux(), sim/exp6/sitesite.c:78-85):
+   u(r) = eu*exp(tau*r) - ru/r^6                for r^2 <  C1q
+        = A*(r^2-C2q)^2                          for C1q <= r^2 < C2q
+   f(r)/r = ef*exp(tau*r)/r - 6*ru/r^8  resp.  A4*(r^2-C2q)
+   with tau = -a/sig, eu = 6 eps e^a/(a-6), ef = eu a/sig,
+   ru = eps a sig^6/(a-6) (eps = |E_min| at r = sig, a the stiffness). */
+typedef struct mdx_exp6_s {
+  double tau, eu, ef, ru;
+  double C1q, C2q, A, A4;
+} mdx_exp6_t;
+
 /* one interval of cook's hyperbolic r-space splines, ertab_t
    (sim/elst.h:62-68): eru(x)=Bu/(Cu+x)+Au, erd(x)=Bd/(Cd+x)+Ad,
    x=r^2, interval index (int)(x*sgrid) (sim/elst.h:122-130) */
@@ -92,6 +104,10 @@ typedef struct mdx_system_s {
   const int *cons_ij;        /* [2*ncons] global site indices i,j */
   const double *cons_bondq;  /* [ncons] squared bond lengths */
   const double *mol_omega;   /* [nmol] SHAKE relaxation omega (= omegac/2) */
+  /* site-site form: NULL = Lennard-Jones (sstab); else the exp-6 table
+     [ntypes*ntypes] (sstab is then not read).  mdx_create only; the
+     batch and the adjoint take Lennard-Jones systems. */
+  const mdx_exp6_t *sstab_exp6;
 } mdx_system_t;
 
 /* what forces() leaves in the globals of sim/simglob.h:436-577 */
@@ -372,6 +388,14 @@ int mdx_cutcorr(const mdx_system_t *sys, const double *corr, double *Ecorr);
    correction corr (sim/setss.c:455-501) in *corr if corr != NULL. */
 int mdx_setss(mdx_lj_t *ss, double EvdW_i, double RvdW_i, double EvdW_j,
               double RvdW_j, int comb_rule, double C2, double *corr);
+/* The same for the exp-6 form: combrule() of sim/exp6/sitesite.c:60-76
+   (sig = RvdW_i+RvdW_j, or sqrt(4 RvdW_i RvdW_j) for comb_rule&2;
+   eps = sqrt(EvdW_i EvdW_j); a = sqrt(a_i a_j); comb_rule&1 must be set,
+   cook has no Kirkwood rule for exp-6) and the smooth cutoff and tail
+   correction of setss() on its u and f.  EvdW in K (negative). */
+int mdx_setss_exp6(mdx_exp6_t *ss, double EvdW_i, double RvdW_i, double a_i,
+                   double EvdW_j, double RvdW_j, double a_j, int comb_rule, double C2,
+                   double *corr);
 
 /* initerfc()+makesplines() with SPLINE=-2, SUBGRID=1 (sim/erfc.c:119-191,
    sim/splines.c:23-158): fills tab[0..*n) (caller provides room for

@@ -34,6 +34,12 @@ class Erfc(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("Au", "Bu", "Cu", "Ad", "Bd", "Cd")]
 
 
+class Exp6(C.Structure):
+    """the exp-6 form's table entry (sim/exp6/sitesite.h:59-75)."""
+    _fields_ = [(n, C.c_double) for n in
+                ("tau", "eu", "ef", "ru", "C1q", "C2q", "A", "A4")]
+
+
 class System(C.Structure):
     _fields_ = [
         ("ns", C.c_int), ("nmol", C.c_int), ("ntypes", C.c_int),
@@ -47,6 +53,7 @@ class System(C.Structure):
         ("chargepol", _dp), ("alpha_qq", _dp),
         ("mass", _dp), ("ncons", C.c_int), ("cons_first", _ip),
         ("cons_ij", _ip), ("cons_bondq", _dp), ("mol_omega", _dp),
+        ("sstab_exp6", C.POINTER(Exp6)),
     ]
 
 
@@ -94,6 +101,9 @@ def declare_setup(lib, prefix):
     f = getattr(lib, prefix + "setss")
     f.argtypes = [C.POINTER(LJ), C.c_double, C.c_double, C.c_double,
                   C.c_double, C.c_int, C.c_double, _dp]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + "setss_exp6")
+    f.argtypes = [C.POINTER(Exp6)] + [C.c_double] * 6 + [C.c_int, C.c_double, _dp]
     f.restype = C.c_int
     f = getattr(lib, prefix + "erfc_table")
     f.argtypes = [C.POINTER(Erfc), _ip, _dp, C.c_int, C.c_double, C.c_double,

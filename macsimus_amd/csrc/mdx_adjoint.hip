@@ -124,6 +124,7 @@ extern "C" int mdx_set_params(mdx_ctx_t *h, const mdx_lj_t *sstab, const double 
   MDX_HIP(hipSetDevice(c.device));
   MDX_HIP(hipStreamSynchronize(c.stream));
   if (sstab) {
+    if (c.exp6) return fail(MDX_E_ARG, "mdx_set_params: the context has the exp-6 form, not LJ");
     const int nt2 = c.ntypes * c.ntypes;
     std::vector<double> lj = lj_table(sstab, nt2);
     std::vector<float> lj32(lj.begin(), lj.end());
@@ -159,6 +160,7 @@ extern "C" int mdx_adjoint_lj(mdx_ctx_t *h, const double *r, const double *w, do
                               double *gE) {
   if (!h || !r || !w || !gF || !gE) return fail(MDX_E_ARG, "mdx_adjoint_lj: NULL argument");
   Ctx &c = h->c;
+  if (c.exp6) return fail(MDX_E_ARG, "mdx_adjoint_lj: the context has the exp-6 form, not LJ");
   MDX_HIP(hipSetDevice(c.device));
   const int nt = c.ntypes, nt2 = nt * nt;
   const size_t nrow = c.polar ? 2 * (size_t)c.ns : c.ns;

@@ -518,6 +518,9 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   b.kappa.assign(nsys, 0.0);
   for (int s = 0; s < nsys; s++) {
     const mdx_system_t &y = sys[s];
+    if (y.sstab_exp6)
+      return bail(fail(MDX_E_ARG, "mdx_batch_create: the batch takes the LJ form only (system " +
+                                      std::to_string(s) + " has the exp-6 table)"));
     if (y.ns <= 0 || y.nmol != y.ns || y.chargepol || y.ncons != 0 || !y.mass || !y.charge ||
         !y.site_type || !y.sstab || !y.erfc_tab || y.erfc_n <= 0)
       return bail(fail(MDX_E_ARG, "mdx_batch_create: systems must be atomic (one site per molecule, "

@@ -292,11 +292,11 @@ __device__ inline void qq_term(const R *erf, double sgrid, R qq, const double (&
 // Drude (pc, pp); LJ acts between centres only.
 // force terms of one pair inside the cutoff (separation d in FP64, rr <
 // cutoff^2, not the same site), added to the accumulators of the central
-// site: LJ with the smooth cutoff (cook/ljqqnp.c:147-206, sim/setss.c),
-// erfc Coulomb from the spline, and for Drudes the three extra charge
+// site: LJ (or exp-6, E6) with the smooth cutoff (cook/ljqqnp.c:147-206,
+// sim/setss.c), erfc Coulomb from the spline, and for Drudes the three extra charge
 // terms (cook/ljqqpol.c).  EXCL pairs of the same molecule are skipped
 // here and corrected in FP64 by k_excl (cook/ljqqnp.c:104-144).
-template <typename R, bool MEASURE, bool POLAR>
+template <typename R, bool MEASURE, bool POLAR, bool E6>
 __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, const double4 ri,
                                  const PairTab<R> &tab, double sgrid, const double4 pj,
                                  const int4 jj, const double4 rj, double dxr, double dyr,
@@ -308,13 +308,21 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
   R fr = 0, u = 0;
   if (rr < (double)ss[5]) {
     if (rr < (double)ss[4]) {
-      const R x = ss[0] * rcp<R>((R)rr), y = x * x * x;
-      if (MEASURE) {
-        const R z = y * y;
-        u = ss[1] * (z - y);
-        fr = ss[2] * (z + z - y) * x;
+      if (E6) {
+        // exp-6 (sim/exp6/sitesite.h:59-66): [tau eu ef ru], x = r,
+        // y = e^(tau r), u = eu y - ru/r^6, f/r = ef y/r - 6 ru/r^8
+        const R q = (R)rr, x = sqrt(q), y = exp(ss[0] * x), iz = rcp<R>(q * q * q);
+        if (MEASURE) u = ss[1] * y - ss[3] * iz;
+        fr = ss[2] * y * rcp<R>(x) - (R)6 * ss[3] * iz * rcp<R>(q);
       } else {
-        fr = ss[3] * y * (y - (R)0.5) * x;
+        const R x = ss[0] * rcp<R>((R)rr), y = x * x * x;
+        if (MEASURE) {
+          const R z = y * y;
+          u = ss[1] * (z - y);
+          fr = ss[2] * (z + z - y) * x;
+        } else {
+          fr = ss[3] * y * (y - (R)0.5) * x;
+        }
       }
     } else {
       const R x = (R)(rr - (double)ss[5]);
@@ -358,7 +366,7 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
   }
 }
 
-template <typename R, bool MEASURE, bool POLAR>
+template <typename R, bool MEASURE, bool POLAR, bool E6>
 __global__ __launch_bounds__(kBlock) void k_pair(
     int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
@@ -412,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(
             npair += 1;
             double4 rj = make_double4(0, 0, 0, 0);
             if (POLAR) rj = spol[j];
-            pair_body<R, MEASURE, POLAR>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr, dyr,
+            pair_body<R, MEASURE, POLAR, E6>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr, dyr,
                                          dzr, rr, ac, ap, U, Eel, Vir, pv, nlj);
           }
         }
@@ -453,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(
 // separation by the FP64 minimum image; same pair terms, census and
 // output layout as k_pair.  For small systems and as a cross-check of the
 // cell path.
-template <typename R, bool MEASURE, bool POLAR>
+template <typename R, bool MEASURE, bool POLAR, bool E6>
 __global__ __launch_bounds__(kBlock) void k_pair_all(
     int ns, Grid g, int jper, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo, int nt, PairTab<R> tab,
@@ -487,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_all(
       const int4 jj = sinfo[j];
       double4 rj = make_double4(0, 0, 0, 0);
       if (POLAR) rj = spol[j];
-      pair_body<R, MEASURE, POLAR>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr, dyr,
+      pair_body<R, MEASURE, POLAR, E6>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr, dyr,
                                    dzr, rr, ac, ap, U, Eel, Vir, pv, nlj);
     }
     fx += ac[0]; fy += ac[1]; fz += ac[2];
@@ -772,7 +780,7 @@ int bin_sites(Ctx &c, const double *d_r) {
   return 0;
 }
 
-template <typename R, bool POLAR>
+template <typename R, bool POLAR, bool E6>
 int launch_pair_t(Ctx &c, int measure) {
   const Grid g = make_grid(c, c.nc);
   PairTab<R> tab;
@@ -785,23 +793,23 @@ int launch_pair_t(Ctx &c, int measure) {
   if (c.allpair) {
     const int jper = (c.ns + np - 1) / np;
     if (measure)
-      k_pair_all<R, true, POLAR><<<grid, kBlock, 0, c.stream>>>(
+      k_pair_all<R, true, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, jper, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.ntypes, tab, c.erf_sgrid, out, c.part.p);
     else
-      k_pair_all<R, false, POLAR><<<grid, kBlock, 0, c.stream>>>(
+      k_pair_all<R, false, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, jper, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.ntypes, tab, c.erf_sgrid, out, c.part.p);
   } else {
     const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
     const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
     if (measure)
-      k_pair<R, true, POLAR><<<grid, kBlock, 0, c.stream>>>(
+      k_pair<R, true, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
           c.part.p);
     else
-      k_pair<R, false, POLAR><<<grid, kBlock, 0, c.stream>>>(
+      k_pair<R, false, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
           c.part.p);
@@ -816,7 +824,9 @@ int launch_pair_t(Ctx &c, int measure) {
 
 template <typename R>
 int launch_pair(Ctx &c, int measure) {
-  return c.polar ? launch_pair_t<R, true>(c, measure) : launch_pair_t<R, false>(c, measure);
+  if (c.exp6)
+    return c.polar ? launch_pair_t<R, true, true>(c, measure) : launch_pair_t<R, false, true>(c, measure);
+  return c.polar ? launch_pair_t<R, true, false>(c, measure) : launch_pair_t<R, false, false>(c, measure);
 }
 
 // records that fit a workgroup's LDS histogram (64 KiB of 32-bit counts)
@@ -918,6 +928,17 @@ std::vector<double> lj_table(const mdx_lj_t *sstab, int nt2) {
   return lj;
 }
 
+std::vector<double> exp6_table(const mdx_exp6_t *sstab, int nt2) {
+  std::vector<double> t6(nt2 * 8);
+  for (int k = 0; k < nt2; k++) {
+    const mdx_exp6_t &t = sstab[k];
+    double v[8] = {t.tau, t.eu, t.ef, t.ru, t.C1q, t.C2q, t.A, t.A4};
+    if (!(t.C2q > 0)) for (int m = 4; m < 8; m++) v[m] = 0;
+    std::copy(v, v + 8, t6.begin() + 8 * k);
+  }
+  return t6;
+}
+
 void fill_out(const Ctx &c, const double *acc, int measure, mdx_out_t *out) {
   std::memset(out, 0, sizeof(*out));
   out->npair = (long long)std::llround(acc[ACC_NPAIR]);
@@ -957,7 +978,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   *out = nullptr;
   if (mode != MDX_FP64 && mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_create: bad mode");
   if (s->ns <= 0 || s->nmol <= 0 || s->ntypes <= 0 || !s->mol_first || !s->site_type ||
-      !s->charge || !s->excl || !s->sstab || !s->erfc_tab || s->erfc_n <= 0)
+      !s->charge || !s->excl || !(s->sstab || s->sstab_exp6) || !s->erfc_tab || s->erfc_n <= 0)
     return fail(MDX_E_ARG, "mdx_create: incomplete system");
   if (s->ntypes > 0xffff) return fail(MDX_E_ARG, "mdx_create: too many site types");
   int ndev = 0;
@@ -1029,9 +1050,10 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
         if ((s->excl[i] >> (j - m0)) & 1) excl[j] |= 1u << (i - m0);
   }
 
-  // LJ table, FP64 and FP32 (A4 of LJ-free pairs is zero)
+  // LJ or exp-6 table, FP64 and FP32 (A4 of pairs without the term is zero)
   const int nt2 = c.ntypes * c.ntypes;
-  std::vector<double> lj = lj_table(s->sstab, nt2);
+  c.exp6 = s->sstab_exp6 != nullptr;
+  std::vector<double> lj = c.exp6 ? exp6_table(s->sstab_exp6, nt2) : lj_table(s->sstab, nt2);
   std::vector<float> lj32(lj.begin(), lj.end());
   // splines: FP64 cook form, FP32 stable form around the interval start
   c.nerf = s->erfc_n;

@@ -156,7 +156,8 @@ struct Ctx {
   DBuf<double> qpos;           // [4nq] charge positions and charges
   DBuf<double> qtrig;          // [12nq] per direction cos, sin of 2 pi s and 16 pi s
   DBuf<unsigned> excl;        // [ns]
-  DBuf<double> lj64;          // [ntypes^2*8]
+  bool exp6 = false;           // site-site form: exp-6 table instead of LJ
+  DBuf<double> lj64;          // [ntypes^2*8] LJ or exp-6 constants, C1q C2q A A4
   DBuf<float> lj32;
   DBuf<double> erf64;         // [nerf*8] stable spline form
   DBuf<float> erf32;
@@ -248,6 +249,8 @@ int md_setup(Ctx &c);
 int bin_device(Ctx &c);
 // kernel LJ table [nt2][Sq E4 E24 E48 C1q C2q A A4]
 std::vector<double> lj_table(const mdx_lj_t *sstab, int nt2);
+// kernel exp-6 table [nt2][tau eu ef ru C1q C2q A A4]
+std::vector<double> exp6_table(const mdx_exp6_t *sstab, int nt2);
 // k-vector rows of Ewald(0) for box L and kappa (host tables only)
 void klayout_build(const double L[3], double kappa, KLayout &K);
 // pass-2 tables for kz chunk kzc and about `parts` parts (host): per chunk

@@ -5,6 +5,7 @@
 // project's own.
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/mdx.h"
@@ -50,6 +51,73 @@ bool almost_integer(double x) {  // sim/ewald.c:240-250, machineprec=1e-9
   return std::fabs(0.5 - std::fabs(x - 0.5)) < 1e-9;
 }
 
+// exp-6 of sim/exp6/sitesite.h:59-75 with the constants of initssaux()
+// (sim/exp6/sitesite.c:78-85): u = eu e^(tau r) - ru/r^6,
+// f(r)/r = ef e^(tau r)/r - 6 ru/r^8; eps > 0 is the well depth |E_min|
+// at r = sig, a the stiffness
+struct Exp6 {
+  double tau, eu, ef, ru, rf;
+  Exp6(double sig, double eps, double a) {
+    tau = -(a / sig);
+    eu = 6 * eps / (a - 6) * std::exp(a);
+    ef = eu * (a / sig);
+    ru = eps * a * p6(sig) / (a - 6);
+    rf = 6 * ru;
+  }
+  // ss_u() and ss_f() of sim/setss.c:105-134: the macro evaluated from
+  // rr = r^2, U and f*r (the force, f being force/r)
+  double u(double r) const {
+    double rr = r * r, x = std::sqrt(rr);
+    return eu * std::exp(tau * x) - ru / (rr * rr * rr);
+  }
+  double f(double r) const {
+    double rr = r * r, x = std::sqrt(rr), z = rr * rr * rr;
+    return (ef * std::exp(tau * x) / x - rf / z / rr) * r;
+  }
+};
+
+// smooth cutoff of setss() (sim/setss.c:244-306), potential-independent:
+// the fixed point for C1 such that the quartic A (r^2 - C2^2)^2 on
+// [C1,C2] matches u and f at C1; then the homogeneous tail correction
+// (sim/setss.c:455-499): the analytic quartic part + 2-point Gauss in 1/r
+// on [C1,inf) with NGauss4 = 64 intervals (sim/setss.c:49)
+template <class Form>
+int smooth_cutoff(const Form &v, double C2, const char *who, double &C1q, double &C2q,
+                  double &A, double &A4, double *corr) {
+  double C1 = 0.775 * C2, prev;
+  int n = 0;
+  do {
+    double fc = v.f(C1);
+    if (std::fabs(fc) < 1e-50) return mdx::fail(MDX_E_ARG, std::string(who) + ": zero force at C1");
+    prev = C1;
+    double t = 1 + 4 * v.u(C1) / C1 / fc;
+    if (n++ > 2000 || t < 0)
+      return mdx::fail(MDX_E_NOCONV, std::string(who) +
+                                         ": no consistent smooth cutoff (LJcutoff too short)");
+    C1 = C2 / std::sqrt(t);
+  } while (std::fabs(1 - C1 / prev) > kMachEps * n);
+  if (C1 > 0.999 * C2 || C1 < 0.5 * C2) return mdx::fail(MDX_E_ARG, std::string(who) + ": bad cutoff");
+  double d = C2 * C2 - C1 * C1;
+  A = v.u(C1) / (d * d);
+  A4 = -v.f(C1) / d / C1;
+  C1q = C1 * C1;
+  C2q = C2 * C2;
+  if (corr) {
+    const int ng = 64;
+    const double g = 0.2113248654051871;
+    double c0 = 4 * kPi * A *
+                (C1 * (p6(C1) / 7 - 0.4 * std::pow(C1, 4) * C2 * C2 + C1 * C1 * std::pow(C2, 4) / 3) -
+                 8.0 / 105 * std::pow(C2, 7));
+    double h = 1.0 / ng / C1, c1 = 0;
+    for (int k = 0; k < ng; k++) {
+      double r1 = 1 / ((k + g) * h), r2 = 1 / ((k + 1 - g) * h);
+      c1 += std::pow(r1, 4) * v.u(r1) + std::pow(r2, 4) * v.u(r2);
+    }
+    *corr = c0 + 2 * kPi * h * c1;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int mdx_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej,
@@ -67,41 +135,30 @@ extern "C" int mdx_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej,
     ss->E48 = -48 * v.eps / ss->Sq;
   }
   if (v.eps == 0 || C2 == 0) return 0;  // no LJ term: C1=C2=A=A4=0
-  // smooth cutoff: fixed point for C1 such that the quartic
-  // A (r^2 - C2^2)^2 on [C1,C2] matches u and f at C1 (sim/setss.c:244-296)
-  double C1 = 0.775 * C2, prev;
-  int n = 0;
-  do {
-    double fc = v.f(C1);
-    if (std::fabs(fc) < 1e-50) return mdx::fail(MDX_E_ARG, "mdx_setss: zero force at C1");
-    prev = C1;
-    double t = 1 + 4 * v.u(C1) / C1 / fc;
-    if (n++ > 2000 || t < 0)
-      return mdx::fail(MDX_E_NOCONV, "mdx_setss: no consistent smooth cutoff (LJcutoff too short)");
-    C1 = C2 / std::sqrt(t);
-  } while (std::fabs(1 - C1 / prev) > kMachEps * n);
-  if (C1 > 0.999 * C2 || C1 < 0.5 * C2) return mdx::fail(MDX_E_ARG, "mdx_setss: bad cutoff");
-  double d = C2 * C2 - C1 * C1;
-  ss->A = v.u(C1) / (d * d);
-  ss->A4 = -v.f(C1) / d / C1;
-  ss->C1q = C1 * C1;
-  ss->C2q = C2 * C2;
-  if (corr) {
-    // homogeneous tail: analytic quartic part + 2-point Gauss in 1/r on
-    // [C1,inf) with NGauss4 = 64 intervals (sim/setss.c:49,470-499)
-    const int ng = 64;
-    const double g = 0.2113248654051871;
-    double c0 = 4 * kPi * ss->A *
-                (C1 * (p6(C1) / 7 - 0.4 * std::pow(C1, 4) * C2 * C2 + C1 * C1 * std::pow(C2, 4) / 3) -
-                 8.0 / 105 * std::pow(C2, 7));
-    double h = 1.0 / ng / C1, c1 = 0;
-    for (int k = 0; k < ng; k++) {
-      double r1 = 1 / ((k + g) * h), r2 = 1 / ((k + 1 - g) * h);
-      c1 += std::pow(r1, 4) * v.u(r1) + std::pow(r2, 4) * v.u(r2);
-    }
-    *corr = c0 + 2 * kPi * h * c1;
-  }
-  return 0;
+  return smooth_cutoff(v, C2, "mdx_setss", ss->C1q, ss->C2q, ss->A, ss->A4, corr);
+}
+
+extern "C" int mdx_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej,
+                              double Rj, double aj, int comb_rule, double C2, double *corr) {
+  if (!ss) return mdx::fail(MDX_E_ARG, "mdx_setss_exp6: ss is NULL");
+  std::memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (Ei > 0 || Ej > 0) return mdx::fail(MDX_E_ARG, "mdx_setss_exp6: EvdW must be <= 0");
+  if (!(comb_rule & 1))
+    return mdx::fail(MDX_E_ARG, "mdx_setss_exp6: comb_rule&1 = 0 (Kirkwood) is not implemented");
+  // combrule(), sim/exp6/sitesite.c:60-76: geometric (2) or additive
+  // diameters, geometric mean of the (negative) minima and of a
+  const double sig = (comb_rule & 2) ? std::sqrt(Ri * Rj * 4) : Ri + Rj;
+  const double eps = std::sqrt(Ei * Ej), a = std::sqrt(ai * aj);
+  if (eps == 0 || C2 == 0) return 0;    // no exp-6 term: the table is zero
+  if (!(sig > 0) || !(a > 6))
+    return mdx::fail(MDX_E_ARG, "mdx_setss_exp6: need sigma > 0 and a > 6");
+  Exp6 v(sig, eps, a);
+  ss->tau = v.tau;
+  ss->eu = v.eu;
+  ss->ef = v.ef;
+  ss->ru = v.ru;
+  return smooth_cutoff(v, C2, "mdx_setss_exp6", ss->C1q, ss->C2q, ss->A, ss->A4, corr);
 }
 
 extern "C" int mdx_erfc_size(int grid, double maxr, double alpha) {

@@ -47,6 +47,15 @@ class Provider:
             raise RuntimeError("setss failed (%d)" % rc)
         return ss, corr.value
 
+    def setss_exp6(self, Ei, Ri, ai, Ej, Rj, aj, comb_rule, C2):
+        ss = abi.Exp6()
+        corr = C.c_double(0)
+        rc = self._f("setss_exp6")(C.byref(ss), Ei, Ri, ai, Ej, Rj, aj, comb_rule, C2,
+                                   C.byref(corr))
+        if rc:
+            raise RuntimeError("setss_exp6 failed (%d)" % rc)
+        return ss, corr.value
+
     def erfc_table(self, grid, maxr, cutoff, alpha, shift):
         nmax = int((maxr * alpha) ** 2 * grid) + 2
         tab = (abi.Erfc * nmax)()
@@ -102,20 +111,29 @@ class BuiltSystem:
 def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
                  rplus=2.5, rshift=3, epsinf=3e33, ncell=(0, 0, 0),
                  alpha=None, kappa=None, lj_cutoff=None, provider=None,
-                 omega=0.6):
+                 omega=0.6, exp6_evdw_scale=1.0):
     """ff: force-field dict as stored in tests/golden/*.json['forcefield']:
          types, type_mass, type_EvdW_kcal, type_RvdW,
          species[{site_type, site_charge_e, site_excl, bonds}]
        counts: molecules per species (cook's N[i]); molecules are ordered
-       by species like cook's molec[] (sim/siminit.c:464-647)."""
+       by species like cook's molec[] (sim/siminit.c:464-647).
+       ff["form"] = "exp6" selects the exp-6 form (sim/exp6) with the
+       stiffness a of each type in ff["type_parm"]; exp6_evdw_scale
+       multiplies its EvdW after the kcal conversion (cook's exp-6
+       initcombrule() converts kcal to K once more, sim/exp6/sitesite.c:
+       52-54, which system_from_cook reproduces)."""
     prov = provider or engine_provider()
     bs = BuiltSystem()
     S = bs.struct
     nt = len(ff["types"])
-    EvdW = [e * EUNIT_KCAL for e in ff["type_EvdW_kcal"]]
+    form = ff.get("form", "lj")
+    if form not in ("lj", "exp6"):
+        raise ValueError("site-site form %r: lj or exp6" % form)
+    exp6 = form == "exp6"
+    EvdW = [e * EUNIT_KCAL * (exp6_evdw_scale if exp6 else 1.0) for e in ff["type_EvdW_kcal"]]
     RvdW = list(ff["type_RvdW"])
     C2 = cutoff if lj_cutoff is None else lj_cutoff
-    sstab = (abi.LJ * (nt * nt))()
+    sstab = ((abi.Exp6 if exp6 else abi.LJ) * (nt * nt))()
     corr = np.zeros((nt, nt))
     # combining rule of a pair (sim/lj/sitesite.c:33-58, 66-112): the
     # file's comb_rule, with the square-root energy rule for sites of
@@ -123,18 +141,22 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     # not restated
     rule = [1] * nt
     if "comb_rule" in ff:
-        alpha = ff.get("type_alpha", [0.0] * nt)
-        rule = [int(ff["comb_rule"]) | (1 if alpha[i] <= 0 else 0) for i in range(nt)]
+        t_alpha = ff.get("type_alpha", [0.0] * nt)
+        rule = [int(ff["comb_rule"]) | (1 if t_alpha[i] <= 0 else 0) for i in range(nt)]
     for i in range(nt):
         for j in range(i + 1):
             if EvdW[i] == 0 or EvdW[j] == 0:
-                ss, c = abi.LJ(), 0.0
+                ss, c = (abi.Exp6 if exp6 else abi.LJ)(), 0.0
             else:
                 cr = rule[i] | rule[j]
                 if not cr & 1:
                     raise ValueError("types %d-%d: the Kirkwood combining rule is not supported"
                                      % (i, j))
-                ss, c = prov.setss(EvdW[i], RvdW[i], EvdW[j], RvdW[j], cr, C2)
+                if exp6:
+                    a = ff["type_parm"]
+                    ss, c = prov.setss_exp6(EvdW[i], RvdW[i], a[i], EvdW[j], RvdW[j], a[j], cr, C2)
+                else:
+                    ss, c = prov.setss(EvdW[i], RvdW[i], EvdW[j], RvdW[j], cr, C2)
             sstab[i * nt + j] = ss
             sstab[j * nt + i] = ss
             corr[i, j] = corr[j, i] = c
@@ -193,7 +215,10 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     S.site_type = bs.ptr(np.array(site_type, np.int32), C.c_int)
     S.charge = bs.ptr(q, C.c_double)
     S.excl = bs.ptr(np.array(excl, np.uint32), C.c_uint)
-    S.sstab = C.cast(sstab, C.POINTER(abi.LJ))
+    if exp6:
+        S.sstab_exp6 = C.cast(sstab, C.POINTER(abi.Exp6))
+    else:
+        S.sstab = C.cast(sstab, C.POINTER(abi.LJ))
     S.L = (C.c_double * 3)(*L)
     S.cutoff = cutoff
     S.alpha = alpha
@@ -218,7 +243,7 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     bs.info = dict(alpha=alpha, kappa=kappa, nk=nk, ncell=list(ncell),
                    erfc_n=nerf, erfc_sgrid=sgrid, qmax=qmax, sumq=sumq,
                    ns=ns_tot, nmol=nmol, ncons=len(cons_bondq),
-                   corr=corr, sstab=sstab, erfc_tab=tab,
+                   corr=corr, sstab=sstab, form=form, erfc_tab=tab,
                    mass=np.array(mass), charge=q, chargepol=qpol,
                    alpha_qq=np.array(alpha_qq), polar=bool(polar),
                    site_type=np.array(site_type, np.int32),

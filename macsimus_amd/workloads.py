@@ -42,6 +42,28 @@ def brine_forcefield(with_ions=True):
                 type_RvdW=[tab[t][2] for t in types], species=species)
 
 
+# exp-6 stiffness a of each brine site type (H carries no site-site term)
+EXP6_A = {"H": 14.0, "O": 14.0, "NA": 13.0, "CL": 13.5}
+
+
+def brine_forcefield_exp6(with_ions=True):
+    """the brine with the exp-6 form (sim/exp6) in place of LJ: the same
+    well depths and minimum distances (EvdW, RvdW), stiffness a from
+    EXP6_A (a near 14 follows LJ around the minimum)"""
+    ff = brine_forcefield(with_ions)
+    ff["form"] = "exp6"
+    ff["type_parm"] = [EXP6_A[t] for t in ff["types"]]
+    return ff
+
+
+def brine_c2_exp6():
+    """the 194,304-site brine of brine_c2 (same configuration) with the
+    exp-6 form"""
+    wl = brine_c2()
+    wl.update(name="brine_194304_exp6", ff=brine_forcefield_exp6(True))
+    return wl
+
+
 def box_length(counts_masses, density_g_cm3):
     mass_g = sum(n * m for n, m in counts_masses) / AVOGADRO
     return (mass_g / density_g_cm3 * 1e24) ** (1 / 3)

@@ -112,6 +112,76 @@ int oracle_setss(mdx_lj_t *ss, double Ei, double Ri, double Ej, double Rj,
   return 0;
 }
 
+/* --------------------------------------------------------------- exp-6 */
+
+/* the exp-6 macros of sim/exp6/sitesite.h:59-75 evaluated from rr = r^2
+   with the constants of initssaux() (sim/exp6/sitesite.c:78-85): U, and
+   f*r, the force, as ss_u() and ss_f() return them (sim/setss.c:105-134) */
+typedef struct { double tau, eu, ef, ru, rf; } orc_e6;
+
+static double orc_e6U(const orc_e6 *p, double r)
+{ double rr = r * r, x = sqrt(rr); return p->eu * exp(p->tau * x) - p->ru / (rr * rr * rr); }
+static double orc_e6F(const orc_e6 *p, double r)
+{
+  double rr = r * r, x = sqrt(rr), z = rr * rr * rr;
+  return (p->ef * exp(p->tau * x) / x - p->rf / z / rr) * r;
+}
+
+/* combrule() of sim/exp6/sitesite.c:60-76 and the potential-independent
+   smooth cutoff and tail correction of setss() (sim/setss.c:244-306,
+   455-499) on the exp-6 u and f */
+int oracle_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej,
+                      double Rj, double aj, int comb_rule, double C2, double *corr)
+{
+  const double neps = 1.1102230246251565e-16;
+  double sig, eps, a, C1, oldC1;
+  orc_e6 p;
+  int n;
+  memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (!(comb_rule & 1) || Ei > 0 || Ej > 0) return MDX_E_ARG;
+  sig = comb_rule & 2 ? sqrt(Ri * Rj * 4) : Ri + Rj;
+  eps = sqrt(Ei * Ej);
+  a = sqrt(ai * aj);
+  if (eps == 0 || C2 == 0) return 0;
+  p.tau = -(a / sig);
+  p.eu = 6 * eps / (a - 6) * exp(a);
+  p.ef = p.eu * (a / sig);
+  p.ru = eps * a * pow6(sig) / (a - 6);
+  p.rf = 6 * p.ru;
+  ss->tau = p.tau; ss->eu = p.eu; ss->ef = p.ef; ss->ru = p.ru;
+  C1 = 0.775 * C2;
+  n = 0;
+  do {
+    double den = orc_e6F(&p, C1);
+    if (fabs(den) < 1e-50) return MDX_E_ARG;
+    oldC1 = C1;
+    C1 = 1 + 4 * orc_e6U(&p, C1) / C1 / den;
+    if (n++ > 2000 || C1 < 0) return MDX_E_ARG;
+    C1 = C2 / sqrt(C1);
+  } while (fabs(1 - C1 / oldC1) > neps * n);
+  if (C1 > 0.999 * C2 || C1 < 0.5 * C2) return MDX_E_ARG;
+  ss->A = orc_e6U(&p, C1) / sqr(C2 * C2 - C1 * C1);
+  ss->A4 = -orc_e6F(&p, C1) / (C2 * C2 - C1 * C1) / C1;
+  ss->C2q = C2 * C2;
+  ss->C1q = C1 * C1;
+  if (corr) {
+    const int NG = 64;
+    const double g4 = 0.2113248654051871;
+    double c0 = 4 * PI * ss->A *
+      (C1 * (pow6(C1) / 7 - 0.4 * sqr(C1) * sqr(C1) * sqr(C2) + sqr(C1) * sqr(C2) * sqr(C2) / 3)
+       - 8.0 / 105 * pow(C2, 7));
+    double h = 1.0 / NG / C1, c1 = 0, r;
+    int k;
+    for (k = 0; k < NG; k++) {
+      r = 1 / ((k + g4) * h); c1 += sqr(r * r) * orc_e6U(&p, r);
+      r = 1 / ((k + (1 - g4)) * h); c1 += sqr(r * r) * orc_e6U(&p, r);
+    }
+    *corr = c0 + c1 * 2 * PI * h;
+  }
+  return 0;
+}
+
 /* --------------------------------------------------------- erfc splines */
 
 int oracle_erfc_table(mdx_erfc_t *tab, int *nout, double *sgridout, int grid,
@@ -624,6 +694,42 @@ static double orc_erud(const mdx_system_t *S, double rr, double *erd)
   return p->Bu / (p->Cu + rr) + p->Au;
 }
 
+/* site-site term of a non-excluded pair below C2: SS_MEASURE of the LJ
+   form (sim/lj/sitesite.h:60-68) or of exp-6 (sim/exp6/sitesite.h:59-66)
+   below C1, the quartic tail A (rr-C2q)^2 above (cook/ljqqnp.c:206-213);
+   adds U and f(r)/r, the LJ virial and the LJ pair count */
+static void orc_sitesite(orc_acc *a, int ti, int tj, double rr, double *U, double *f)
+{
+  const mdx_system_t *S = a->S;
+  double C1q, C2q, A, A4;
+  if (S->sstab_exp6) {
+    const mdx_exp6_t *e = S->sstab_exp6 + ti * S->ntypes + tj;
+    C1q = e->C1q; C2q = e->C2q; A = e->A; A4 = e->A4;
+    if (rr >= C2q) return;
+    if (rr < C1q) {
+      double x = sqrt(rr), y = exp(e->tau * x), z = rr * rr * rr;
+      *U = e->eu * y - e->ru / z;
+      *f = e->ef * y / x - 6 * e->ru / z / rr;
+    }
+  } else {
+    const mdx_lj_t *ss = S->sstab + ti * S->ntypes + tj;
+    C1q = ss->C1q; C2q = ss->C2q; A = ss->A; A4 = ss->A4;
+    if (rr >= C2q) return;
+    if (rr < C1q) {
+      double x = ss->Sq / rr, y = x * x * x, z = y * y;
+      *U = ss->E4 * (z - y);
+      *f = ss->E24 * (z + z - y) * x;
+    }
+  }
+  if (rr >= C1q) {
+    double x = rr - C2q;
+    *U = x * x * A;
+    *f = A4 * x;
+  }
+  a->Envir += rr * *f;
+  a->nlj += A != 0;
+}
+
 /* polarLJqq[m] for POLAR=0 (cook/ljqqpol.c): the pair is chosen by the
    centre-centre distance; up to four charge-charge terms centre/Drude x
    centre/Drude, Drudes at r+rpol, LJ between the centres only. */
@@ -633,13 +739,11 @@ static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
   double cq = S->cutoff * S->cutoff;
   for (; l; l = l->next) {
     double dr[3], d[3], rr, r2, f = 0, U = 0, qq, erd, u;
-    const mdx_lj_t *ss;
     int k;
     for (k = 0; k < 3; k++) dr[k] = ls->r[k] - l->r[k];
     rr = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     if (rr >= cq) continue;
     a->npair++;
-    ss = S->sstab + ls->st * S->ntypes + l->st;
     if (ls->n == l->n) {
       int m0 = S->mol_first[ls->n];
       int ii = ls->i - m0, jj = l->i - m0;
@@ -680,19 +784,7 @@ static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
     }
     /* RDF of the centres (cook/ljqqpol.c:477-485) */
     if (a->rdf) orc_rdf_count(a, ls->st, l->st, rr);
-    if (rr < ss->C2q) {
-      if (rr < ss->C1q) {
-        double x = ss->Sq / rr, y = x * x * x, z = y * y;
-        U = ss->E4 * (z - y);
-        f = ss->E24 * (z + z - y) * x;
-      } else {
-        double x = rr - ss->C2q;
-        U = x * x * ss->A;
-        f = ss->A4 * x;
-      }
-      a->Envir += rr * f;
-      a->nlj += ss->A != 0;
-    }
+    orc_sitesite(a, ls->st, l->st, rr, &U, &f);
     a->U += U;
     if ((qq = ls->q * l->q) != 0) {
       u = orc_erud(S, rr, &erd);
@@ -735,14 +827,12 @@ static void orc_ljqq(orc_acc *a, orc_site *l, orc_site *ls)
   double cq = S->cutoff * S->cutoff;
   for (; l; l = l->next) {
     double dr[3], rr, f = 0, U = 0, qq, fdr;
-    const mdx_lj_t *ss;
     dr[0] = ls->r[0] - l->r[0];
     dr[1] = ls->r[1] - l->r[1];
     dr[2] = ls->r[2] - l->r[2];
     rr = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     if (rr >= cq) continue;
     a->npair++;
-    ss = S->sstab + ls->st * S->ntypes + l->st;
     if (ls->n == l->n) {
       int m0 = S->mol_first[ls->n];
       int ii = ls->i - m0, jj = l->i - m0;
@@ -756,19 +846,7 @@ static void orc_ljqq(orc_acc *a, orc_site *l, orc_site *ls)
         goto addUf;
       }
     }
-    if (rr < ss->C2q) {
-      if (rr < ss->C1q) {
-        double x = ss->Sq / rr, y = x * x * x, z = y * y;
-        U = ss->E4 * (z - y);
-        f = ss->E24 * (z + z - y) * x;
-      } else {
-        double x = rr - ss->C2q;
-        U = x * x * ss->A;
-        f = ss->A4 * x;
-      }
-      a->Envir += rr * f;
-      a->nlj += ss->A != 0;
-    }
+    orc_sitesite(a, ls->st, l->st, rr, &U, &f);
     if ((qq = ls->q * l->q) != 0) {
       const mdx_erfc_t *p = S->erfc_tab + (int)(rr * S->erfc_sgrid);
       double byerd = p->Bd / (p->Cd + rr) + p->Ad;

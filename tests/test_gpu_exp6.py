@@ -1,0 +1,114 @@
+"""The exp-6 form (sim/exp6) on the GPU against the oracle, through the
+C ABI: forces, energies, virial, tensor and census of a liquid brine (the
+positions of cook's brine506 run, exp-6 parameters of the same minima),
+both precisions, cell and all-pair paths; MD steps; the full-size brine
+mixed against FP64; the LJ-only entry points refuse exp-6."""
+import numpy as np
+import pytest
+
+import oracle
+from macsimus_amd import abi, workloads
+from macsimus_amd.engine import Batch, Engine
+from macsimus_amd.system import build_system, engine_provider
+
+pytestmark = pytest.mark.gpu
+
+TOL = {abi.MDX_FP64: 1e-10, abi.MDX_MIXED: 1e-5}
+ETOL = {abi.MDX_FP64: 1e-10, abi.MDX_MIXED: 2e-6}
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+@pytest.fixture(scope="module")
+def brine():
+    d, meta = oracle.load_fixture("brine506")
+    ff = dict(meta["forcefield"])
+    ff["form"] = "exp6"
+    ff["type_parm"] = [workloads.EXP6_A[t] for t in ff["types"]]
+    bs = oracle.fixture_system(dict(meta, forcefield=ff), provider_=engine_provider())
+    bso = oracle.fixture_system(dict(meta, forcefield=ff))
+    f_ref, o_ref = oracle.forces(bso, d["r"], 1)
+    return d, meta, bs, bso, f_ref, o_ref
+
+
+@pytest.mark.parametrize("w", [2, 0])
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_exp6_forces_vs_oracle(brine, mode, w):
+    d, meta, bs, bso, f_ref, o_ref = brine
+    print("exp-6 brine506 mode %d stencil %d" % (mode, w), flush=True)
+    e = Engine(bs, mode)
+    e.set_stencil(w)
+    f, out = e.forces(d["r"], 1)
+    f0, _ = e.forces(d["r"], 0)
+    e.close()
+    err = rel_rms(f, f_ref)
+    print("  rel RMS force err %.3e" % err, flush=True)
+    assert err < TOL[mode]
+    assert rel_rms(f0, f_ref) < TOL[mode]
+    assert out.npair == o_ref.npair and out.nlj == o_ref.nlj
+    # energies against the potential energy En.pot + En.el (the site-site
+    # part alone is a small net sum of large terms of both signs)
+    Etot = o_ref.En_pot + o_ref.En_el
+    for k in ("En_pot", "En_vir", "En_el"):
+        dk = getattr(out, k) - getattr(o_ref, k)
+        print("  %s %.10e diff %.3e (%.2e of Epot)" % (k, getattr(o_ref, k), dk, dk / Etot), flush=True)
+        assert abs(dk) < ETOL[mode] * abs(Etot), k
+    scale = max(abs(o_ref.Pvir[i]) for i in range(3))
+    for i in range(6):
+        assert abs(out.Pvir[i] - o_ref.Pvir[i]) < 10 * ETOL[mode] * scale
+
+
+@pytest.mark.parametrize("mode,tol", [(abi.MDX_FP64, 1e-9), (abi.MDX_MIXED, 1e-5)])
+def test_exp6_md_steps_vs_oracle(brine, mode, tol):
+    d, meta, bs, bso, _, _ = brine
+    r = d["r"].copy()
+    r1 = np.zeros_like(r)
+    p = abi.Step(h=0.002, eps=1e-10, maxit=0, thermostat=0, T=298.0, tau=0.1, nf=1.0,
+                 measure_every=0)
+    e = Engine(bs, mode)
+    e.upload_state(r, r1)
+    so = e.md_steps(p, 5)
+    rg, _ = e.download_state()
+    e.close()
+    ro, r1o = r.copy(), r1.copy()
+    for k in range(5):
+        oo = oracle.md_step(bso, ro, r1o, p, measure=int(k == 4))
+    dev = np.abs(rg - ro).max()
+    print("exp-6 5 steps mode %d: max deviation %.3e A" % (mode, dev), flush=True)
+    assert dev < tol
+    Etot = oo.frc.En_pot + oo.frc.En_el
+    assert abs(so.frc.En_pot + so.frc.En_el - Etot) < ETOL[mode] * abs(Etot)
+    assert abs(so.frc.En_pot - oo.frc.En_pot) < ETOL[mode] * abs(Etot)
+
+
+def test_exp6_full_size_mixed_vs_fp64():
+    wl = workloads.brine_c2_exp6()
+    bs = build_system(wl["ff"], wl["counts"], [wl["L"]] * 3, wl["cutoff"], epsr=0.05, epsk=0.05)
+    res = {}
+    for mode in (abi.MDX_FP64, abi.MDX_MIXED):
+        print("C2 exp-6 mode %d" % mode, flush=True)
+        e = Engine(bs, mode)
+        res[mode] = e.forces(wl["r"], 1)
+        e.close()
+    (f64, o64), (f32, o32) = res[abi.MDX_FP64], res[abi.MDX_MIXED]
+    err = rel_rms(f32, f64)
+    print("  rel RMS force err mixed vs FP64 %.3e" % err, flush=True)
+    assert err < 1e-5
+    assert o32.npair == o64.npair and o32.nlj == o64.nlj
+    Etot = o64.En_pot + o64.En_el
+    print("  En_pot %.10e diff %.3e" % (o64.En_pot, o32.En_pot - o64.En_pot), flush=True)
+    assert abs(o32.En_pot - o64.En_pot) < 2e-6 * abs(Etot)
+    fmax = np.abs(f64).max()
+    assert np.abs(f64.sum(0)).max() < 1e-9 * fmax * np.sqrt(len(f64))
+
+
+def test_lj_only_entry_points_refuse_exp6(brine):
+    d, meta, bs, _, _, _ = brine
+    e = Engine(bs, abi.MDX_FP64)
+    with pytest.raises(RuntimeError, match="exp-6"):
+        e.adjoint_lj(d["r"], np.ones_like(d["r"]))
+    e.close()
+    with pytest.raises(RuntimeError, match="exp-6"):
+        Batch([bs], abi.MDX_FP64)
