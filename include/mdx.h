@@ -293,11 +293,18 @@ int mdx_info(mdx_ctx_t *ctx, int *iv, int n);
    type pair (a,b) (a = central site) gF[(a*nt+b)*4 + k] = sum_i w_i .
    dF_i/dc_k with c = (Sq, E48, A4, C2q) and gE[(a*nt+b)*4 + k] = dEpot/dc_k
    with c = (Sq, E4, A, C2q), each unordered pair split half and half
-   between (a,b) and (b,a).  The pair terms are recomputed in FP64. */
+   between (a,b) and (b,a).  The pair terms are recomputed in FP64.
+   mdx_set_params_exp6 / mdx_adjoint_exp6: the same for a context of the
+   exp-6 form (mdx_set_params then takes charges only), with
+   gF[(a*nt+b)*5 + k] for c = (tau, ef, ru, A4, C2q) and
+   gE[(a*nt+b)*5 + k] for c = (tau, eu, ru, A, C2q) of mdx_exp6_t. */
 int mdx_set_params(mdx_ctx_t *ctx, const mdx_lj_t *sstab, const double *charge,
                    const double *chargepol);
 int mdx_adjoint_lj(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
                    double *gE);
+int mdx_set_params_exp6(mdx_ctx_t *ctx, const mdx_exp6_t *sstab);
+int mdx_adjoint_exp6(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
+                     double *gE);
 
 /* ---- batched replicas ---------------------------------------------------
    Many independent atomic systems (one site per molecule: LJ with cook's

@@ -1,5 +1,5 @@
-// Parameter updates and the adjoint of the LJ pair terms, for fitting the
-// force field to forces or energies.
+// Parameter updates and the adjoint of the LJ and exp-6 pair terms, for
+// fitting the force field to forces or energies.
 //
 // The LJ force of a pair (cook/ljqqnp.c:147-206 with the smooth cutoff of
 // sim/setss.c:137-306) depends on the constants of its type pair:
@@ -28,27 +28,38 @@ using namespace mdx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kG = 8;   // per site and partner type: 4 force, 4 energy derivatives
 
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-// gsite[(tj * ns + s) * 8 + k], s in cell order:
-//   k = 0..3: sum_j w_s . d  df/dc for c = Sq, E48, A4, C2q
-//   k = 4..7: 1/2 sum_j du/dc       for c = Sq, E4, A, C2q
-__global__ __launch_bounds__(kBlock) void k_adjoint_lj(
+// derivatives kept per site and partner type: LJ 4 force + 4 energy,
+// exp-6 5 force + 5 energy
+template <bool E6>
+constexpr int ng_of() { return E6 ? 10 : 8; }
+
+// gsite[(tj * ns + s) * NG + k], s in cell order, for the constants c of
+// the pair (ti, tj):
+//   LJ:    k = 0..3: sum_j w_s . d  df/dc for c = Sq, E48, A4, C2q
+//          k = 4..7: 1/2 sum_j du/dc       for c = Sq, E4, A, C2q
+//   exp-6: k = 0..4: sum_j w_s . d  df/dc for c = tau, ef, ru, A4, C2q
+//          k = 5..9: 1/2 sum_j du/dc       for c = tau, eu, ru, A, C2q
+//   (exp-6 inner: f/r = ef y/r - 6 ru/r^8, u = eu y - ru/r^6, y = e^(tau r))
+template <bool E6>
+__global__ __launch_bounds__(kBlock) void k_adjoint(
     int ns, int nx, int ny, int nz, double Lx, double Ly, double Lz, int w, double cq,
     const double4 *__restrict__ spos, const int4 *__restrict__ sinfo,
     const int *__restrict__ scell, const int *__restrict__ cstart, int nt,
     const double *__restrict__ lj, const double *__restrict__ wgt, double *__restrict__ gsite) {
+  constexpr int NG = ng_of<E6>();
   const int s = blockIdx.x * kBlock + threadIdx.x;
   const int tj_want = blockIdx.y;
   if (s >= ns) return;
-  double g[kG] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double g[NG];
+  for (int k = 0; k < NG; k++) g[k] = 0;
   const double4 pi = spos[s];
   const int4 ii = sinfo[s];
   const int ti = ii.x & 0xffff;
   const double *ss = lj + ((size_t)ti * nt + tj_want) * 8;
-  const double Sq = ss[0], E4 = ss[1], E48 = ss[3], C1q = ss[4], C2q = ss[5], A = ss[6], A4 = ss[7];
+  const double C1q = ss[4], C2q = ss[5], A = ss[6], A4 = ss[7];
   const double wx = wgt[3 * ii.w], wy = wgt[3 * ii.w + 1], wz = wgt[3 * ii.w + 2];
   if (C2q > 0) {
     const int c = scell[s];
@@ -76,36 +87,48 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_lj(
             if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) continue;  // EXCL
             const double wd = wx * ddx + wy * ddy + wz * ddz;
             if (rr < C1q) {
-              const double x = Sq / rr, x3 = x * x * x, x4 = x3 * x, x6 = x3 * x3;
-              g[0] += wd * E48 * (7 * x6 - 2 * x3) / rr;
-              g[1] += wd * (x6 * x - 0.5 * x4);
-              g[4] += 0.5 * E4 * (6 * x6 / x - 3 * x * x) / rr;
-              g[5] += 0.5 * (x6 - x3);
+              if (E6) {
+                const double x = sqrt(rr), y = exp(ss[0] * x), i6 = 1 / (rr * rr * rr);
+                g[0] += wd * ss[2] * y;
+                g[1] += wd * y / x;
+                g[2] += -wd * 6 * i6 / rr;
+                g[5] += 0.5 * ss[1] * x * y;
+                g[6] += 0.5 * y;
+                g[7] += -0.5 * i6;
+              } else {
+                const double Sq = ss[0], E4 = ss[1], E48 = ss[3];
+                const double x = Sq / rr, x3 = x * x * x, x4 = x3 * x, x6 = x3 * x3;
+                g[0] += wd * E48 * (7 * x6 - 2 * x3) / rr;
+                g[1] += wd * (x6 * x - 0.5 * x4);
+                g[4] += 0.5 * E4 * (6 * x6 / x - 3 * x * x) / rr;
+                g[5] += 0.5 * (x6 - x3);
+              }
             } else {
               const double t = rr - C2q;
-              g[2] += wd * t;
-              g[3] += -wd * A4;
-              g[6] += 0.5 * t * t;
-              g[7] += -A * t;
+              constexpr int kf = E6 ? 3 : 2, ke = E6 ? 8 : 6;
+              g[kf] += wd * t;
+              g[kf + 1] += -wd * A4;
+              g[ke] += 0.5 * t * t;
+              g[ke + 1] += -A * t;
             }
           }
         }
       }
     }
   }
-  double *o = gsite + ((size_t)tj_want * ns + s) * kG;
-  for (int k = 0; k < kG; k++) o[k] = g[k];
+  double *o = gsite + ((size_t)tj_want * ns + s) * NG;
+  for (int k = 0; k < NG; k++) o[k] = g[k];
 }
 
-// out[(ti * nt + tj) * 8 + k] = sum over sites s of type ti of gsite[tj][s][k],
+// out[(ti * nt + tj) * ng + k] = sum over sites s of type ti of gsite[tj][s][k],
 // one workgroup per (ti, tj, k), fixed order
-__global__ void k_adjoint_sum(int ns, int nt, const int4 *__restrict__ sinfo,
+__global__ void k_adjoint_sum(int ns, int nt, int ng, const int4 *__restrict__ sinfo,
                               const double *__restrict__ gsite, double *__restrict__ out) {
   __shared__ double red[kBlock];
-  const int id = blockIdx.x, k = id % kG, tj = (id / kG) % nt, ti = id / (kG * nt);
+  const int id = blockIdx.x, k = id % ng, tj = (id / ng) % nt, ti = id / (ng * nt);
   double sum = 0;
   for (int s = threadIdx.x; s < ns; s += kBlock)
-    if ((sinfo[s].x & 0xffff) == ti) sum += gsite[((size_t)tj * ns + s) * kG + k];
+    if ((sinfo[s].x & 0xffff) == ti) sum += gsite[((size_t)tj * ns + s) * ng + k];
   red[threadIdx.x] = sum;
   __syncthreads();
   for (int o = kBlock / 2; o > 0; o >>= 1) {
@@ -156,18 +179,20 @@ extern "C" int mdx_set_params(mdx_ctx_t *h, const mdx_lj_t *sstab, const double 
   return 0;
 }
 
-extern "C" int mdx_adjoint_lj(mdx_ctx_t *h, const double *r, const double *w, double *gF,
-                              double *gE) {
-  if (!h || !r || !w || !gF || !gE) return fail(MDX_E_ARG, "mdx_adjoint_lj: NULL argument");
-  Ctx &c = h->c;
-  if (c.exp6) return fail(MDX_E_ARG, "mdx_adjoint_lj: the context has the exp-6 form, not LJ");
+namespace {
+
+// the adjoint of the context's site-site form at positions r for weights
+// w: res[(a*nt+b)*ng + k] as gsite summed over the sites of type a
+template <bool E6>
+int adjoint_run(Ctx &c, const double *r, const double *w, std::vector<double> &res) {
+  constexpr int ng = ng_of<E6>();
   MDX_HIP(hipSetDevice(c.device));
   const int nt = c.ntypes, nt2 = nt * nt;
   const size_t nrow = c.polar ? 2 * (size_t)c.ns : c.ns;
   DBuf<double> wd, gs, out;
   int rc;
-  if ((rc = wd.alloc(3 * (size_t)c.ns)) || (rc = gs.alloc((size_t)nt * c.ns * kG)) ||
-      (rc = out.alloc((size_t)nt2 * kG))) {
+  if ((rc = wd.alloc(3 * (size_t)c.ns)) || (rc = gs.alloc((size_t)nt * c.ns * ng)) ||
+      (rc = out.alloc((size_t)nt2 * ng))) {
     wd.release(); gs.release();
     return rc;
   }
@@ -176,14 +201,15 @@ extern "C" int mdx_adjoint_lj(mdx_ctx_t *h, const double *r, const double *w, do
   if (e == hipSuccess)
     e = hipMemcpyAsync(wd.p, w, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream);
   if (e == hipSuccess && (rc = bin_device(c)) == 0) {
-    k_adjoint_lj<<<dim3(nblk(c.ns), nt), kBlock, 0, c.stream>>>(
+    k_adjoint<E6><<<dim3(nblk(c.ns), nt), kBlock, 0, c.stream>>>(
         c.ns, c.nc[0], c.nc[1], c.nc[2], c.L[0], c.L[1], c.L[2], c.wst, c.cutoff * c.cutoff,
         (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, nt, c.lj64.p,
         wd.p, gs.p);
-    k_adjoint_sum<<<nt2 * kG, kBlock, 0, c.stream>>>(c.ns, nt, (const int4 *)c.sinfo.p, gs.p, out.p);
+    k_adjoint_sum<<<nt2 * ng, kBlock, 0, c.stream>>>(c.ns, nt, ng, (const int4 *)c.sinfo.p, gs.p,
+                                                     out.p);
     e = hipGetLastError();
   }
-  std::vector<double> res((size_t)nt2 * kG);
+  res.assign((size_t)nt2 * ng, 0.0);
   if (!rc && e == hipSuccess)
     e = hipMemcpyAsync(res.data(), out.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c.stream);
   if (!rc && e == hipSuccess) e = hipStreamSynchronize(c.stream);
@@ -191,11 +217,55 @@ extern "C" int mdx_adjoint_lj(mdx_ctx_t *h, const double *r, const double *w, do
   gs.release();
   out.release();
   if (rc) return rc;
-  if (e != hipSuccess) return fail(MDX_E_GPU, "mdx_adjoint_lj: HIP error");
+  if (e != hipSuccess) return fail(MDX_E_GPU, "adjoint: HIP error");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int mdx_set_params_exp6(mdx_ctx_t *h, const mdx_exp6_t *sstab) {
+  if (!h || !sstab) return fail(MDX_E_ARG, "mdx_set_params_exp6: NULL argument");
+  Ctx &c = h->c;
+  if (!c.exp6) return fail(MDX_E_ARG, "mdx_set_params_exp6: the context has the LJ form");
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> t6 = exp6_table(sstab, c.ntypes * c.ntypes);
+  std::vector<float> t32(t6.begin(), t6.end());
+  MDX_HIP(hipMemcpy(c.lj64.p, t6.data(), sizeof(double) * t6.size(), hipMemcpyHostToDevice));
+  MDX_HIP(hipMemcpy(c.lj32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int mdx_adjoint_lj(mdx_ctx_t *h, const double *r, const double *w, double *gF,
+                              double *gE) {
+  if (!h || !r || !w || !gF || !gE) return fail(MDX_E_ARG, "mdx_adjoint_lj: NULL argument");
+  Ctx &c = h->c;
+  if (c.exp6) return fail(MDX_E_ARG, "mdx_adjoint_lj: the context has the exp-6 form, not LJ");
+  std::vector<double> res;
+  int rc = adjoint_run<false>(c, r, w, res);
+  if (rc) return rc;
+  const int nt2 = c.ntypes * c.ntypes;
   for (int p = 0; p < nt2; p++)
     for (int k = 0; k < 4; k++) {
-      gF[p * 4 + k] = res[p * kG + k];
-      gE[p * 4 + k] = res[p * kG + 4 + k];
+      gF[p * 4 + k] = res[p * 8 + k];
+      gE[p * 4 + k] = res[p * 8 + 4 + k];
+    }
+  return 0;
+}
+
+extern "C" int mdx_adjoint_exp6(mdx_ctx_t *h, const double *r, const double *w, double *gF,
+                                double *gE) {
+  if (!h || !r || !w || !gF || !gE) return fail(MDX_E_ARG, "mdx_adjoint_exp6: NULL argument");
+  Ctx &c = h->c;
+  if (!c.exp6) return fail(MDX_E_ARG, "mdx_adjoint_exp6: the context has the LJ form");
+  std::vector<double> res;
+  int rc = adjoint_run<true>(c, r, w, res);
+  if (rc) return rc;
+  const int nt2 = c.ntypes * c.ntypes;
+  for (int p = 0; p < nt2; p++)
+    for (int k = 0; k < 5; k++) {
+      gF[p * 5 + k] = res[p * 10 + k];
+      gE[p * 5 + k] = res[p * 10 + 5 + k];
     }
   return 0;
 }

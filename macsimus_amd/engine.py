@@ -127,14 +127,36 @@ class Engine:
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.FAMILIES)}
 
     def set_params(self, sstab=None, charge=None, chargepol=None):
-        """new LJ table (ctypes array of abi.LJ, ntypes^2) and/or charges"""
+        """new site-site table (ctypes array of abi.LJ, or of abi.Exp6 for
+        an exp-6 context, ntypes^2) and/or charges"""
         dp = C.POINTER(C.c_double)
+        if sstab is not None and getattr(sstab, "_type_", None) is abi.Exp6:
+            rc = self.lib.mdx_set_params_exp6(self.ctx, C.cast(sstab, C.POINTER(abi.Exp6)))
+            abi.check(self.lib, rc, "mdx_set_params_exp6")
+            sstab = None
+            if charge is None and chargepol is None:
+                return
         q = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
         qp = None if chargepol is None else np.ascontiguousarray(chargepol, dtype=np.float64)
         rc = self.lib.mdx_set_params(self.ctx, None if sstab is None else C.cast(sstab, C.POINTER(abi.LJ)),
                                      None if q is None else q.ctypes.data_as(dp),
                                      None if qp is None else qp.ctypes.data_as(dp))
         abi.check(self.lib, rc, "mdx_set_params")
+
+    def adjoint_exp6(self, r, w):
+        """(gF, gE) [nt, nt, 5]: sum_i w_i . dF_i/dc for c = (tau, ef, ru,
+        A4, C2q) and dE/dc for c = (tau, eu, ru, A, C2q) of each type pair
+        of an exp-6 context (see include/mdx.h)"""
+        nt = self.bs.struct.ntypes
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.nrow, 3)
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.ns, 3)
+        gF = np.zeros((nt, nt, 5))
+        gE = np.zeros((nt, nt, 5))
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_adjoint_exp6(self.ctx, r.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                       gF.ctypes.data_as(dp), gE.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_adjoint_exp6")
+        return gF, gE
 
     def adjoint_lj(self, r, w):
         """(gF, gE) [nt, nt, 4]: sum_i w_i . dF_i/dc and dE/dc for the LJ
@@ -358,6 +380,8 @@ def declare_extras(lib):
     lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
+    lib.mdx_set_params_exp6.argtypes = [C.c_void_p, C.POINTER(abi.Exp6)]
+    lib.mdx_adjoint_exp6.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
     lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]

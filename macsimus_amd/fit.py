@@ -1,20 +1,23 @@
 """Force-field fitting on the engine: force matching (and energy gradients)
-for per-type-pair Lennard-Jones sigma/epsilon and per-type charges.
+for per-type-pair Lennard-Jones sigma/epsilon, or exp-6 sigma/epsilon/a,
+and per-type charges.
 
-Parameters theta = [sigma_p (A), epsilon_p (kJ/mol) for each fitted type
-pair p] + [charge parameters (e)].  A type pair (a,b) with its own sigma,
-epsilon enters cook's site-site table through setss() with both sites
-given the same (EvdW, RvdW), which the combining rule maps to themselves
-(sim/lj/sitesite.c:66-112); the type charges are an affine function of the
-charge parameters (e.g. water H = -q_O/2).
+Parameters theta = [sigma_p (A), epsilon_p (kJ/mol) (, a_p for exp-6) for
+each fitted type pair p] + [charge parameters (e)].  A type pair (a,b)
+with its own parameters enters cook's site-site table through setss()
+with both sites given the same (EvdW, RvdW(, a)), which the combining rule
+maps to themselves (sim/lj/sitesite.c:66-112, sim/exp6/sitesite.c:60-76);
+for exp-6, sigma is the distance of the minimum.  The type charges are an
+affine function of the charge parameters (e.g. water H = -q_O/2).
 
 Gradient of the loss L = sum_frames sum_i |F_i(theta) - F_i^target|^2:
-  * LJ: the engine's adjoint kernel (mdx_adjoint_lj) gives, for the
-    weights w_i = dL/dF_i = 2 (F_i - F_i^target), the derivatives with
-    respect to the table constants (Sq, E48, A4, C2q) of each type pair;
-    the chain to (sigma, epsilon) goes through setss() (mdx_setss, which
-    includes the smooth-cutoff fixed point) by central differences of the
-    constants on the host.
+  * LJ / exp-6: the engine's adjoint kernel (mdx_adjoint_lj,
+    mdx_adjoint_exp6) gives, for the weights w_i = dL/dF_i = 2 (F_i -
+    F_i^target), the derivatives with respect to the table constants of
+    each type pair ((Sq, E48, A4, C2q), resp. (tau, ef, ru, A4, C2q)); the
+    chain to the parameters goes through setss() (mdx_setss,
+    mdx_setss_exp6, which include the smooth-cutoff fixed point) by
+    central differences of the constants on the host.
   * charges: the site forces and the energy are quadratic forms in the
     charges (r-space, exclusion, k-space and dipole terms alike), so the
     central difference of the forward call, [F(q + h) - F(q - h)] / 2h,
@@ -62,7 +65,9 @@ class ForceMatch:
         self.q_types = [self.types.index(n) for n in names]
         self.q_base = np.asarray(base, dtype=np.float64)
         self.q_M = np.asarray(M, dtype=np.float64)
-        self.npl = 2 * len(self.pairs)
+        self.exp6 = ff.get("form", "lj") == "exp6"
+        self.pp = 3 if self.exp6 else 2          # parameters per fitted pair
+        self.npl = self.pp * len(self.pairs)
         self.npar = self.npl + self.q_M.shape[1]
         self.site_type = self.bs.info["site_type"]
         self.polar = bool(self.bs.info["polar"])
@@ -77,9 +82,14 @@ class ForceMatch:
         E = self.ff["type_EvdW_kcal"]
         R = self.ff["type_RvdW"]
         for a, b in self.pairs:
+            ea, eb = -E[a] / KCAL_PER_KJ, -E[b] / KCAL_PER_KJ
+            if self.exp6:
+                # exp-6 rule: sigma_ab = R_a + R_b, eps, a geometric means
+                A = self.ff["type_parm"]
+                th += [R[a] + R[b], np.sqrt(ea * eb), np.sqrt(A[a] * A[b])]
+                continue
             # Lorentz-Berthelot: sigma_ab = (sigma_a + sigma_b)/2, eps = sqrt(eps_a eps_b)
             sa, sb = 2 * R[a] / TWO16, 2 * R[b] / TWO16
-            ea, eb = -E[a] / KCAL_PER_KJ, -E[b] / KCAL_PER_KJ
             th += [(sa + sb) / 2, np.sqrt(ea * eb)]
         q = self._type_charges_from_ff()
         # least-squares inverse of the affine charge map
@@ -103,11 +113,22 @@ class ForceMatch:
                   "mdx_setss")
         return ss
 
+    def _setss6(self, sigma, eps_kj, a):
+        e_k = -eps_kj * KCAL_PER_KJ * EUNIT_KCAL
+        ss = abi.Exp6()
+        corr = C.c_double(0)
+        abi.check(self.lib, self.lib.mdx_setss_exp6(C.byref(ss), e_k, sigma / 2, a, e_k, sigma / 2, a,
+                                                    1, self.C2, C.byref(corr)), "mdx_setss_exp6")
+        return ss
+
+    def _pair_table(self, th):
+        return self._setss6(*th) if self.exp6 else self._setss(*th)
+
     def _table(self, theta):
-        tab = (abi.LJ * (self.nt * self.nt))()
+        tab = ((abi.Exp6 if self.exp6 else abi.LJ) * (self.nt * self.nt))()
         C.memmove(tab, self.bs.info["sstab"], C.sizeof(tab))
         for k, (a, b) in enumerate(self.pairs):
-            ss = self._setss(theta[2 * k], theta[2 * k + 1])
+            ss = self._pair_table(theta[self.pp * k:self.pp * (k + 1)])
             tab[a * self.nt + b] = ss
             tab[b * self.nt + a] = ss
         return tab
@@ -134,19 +155,32 @@ class ForceMatch:
             L += float(((f[:self.ns] - t) ** 2).sum())
         return L
 
-    def _dconst(self, sigma, eps):
-        """d(Sq, E48, A4, C2q, E4, A)/d(sigma, eps) by central differences
-        of setss"""
-        def vec(s, e):
-            ss = self._setss(s, e)
-            return np.array([ss.Sq, ss.E48, ss.A4, ss.C2q, ss.E4, ss.A])
-        out = np.zeros((6, 2))
-        for k, (x, h) in enumerate(((sigma, 1e-6 * sigma), (eps, 1e-6 * eps))):
-            if k == 0:
-                out[:, 0] = (vec(x + h, eps) - vec(x - h, eps)) / (2 * h)
-            else:
-                out[:, 1] = (vec(sigma, x + h) - vec(sigma, x - h)) / (2 * h)
+    # table constants of the adjoint: force derivatives, energy derivatives
+    # (positions in the vector of _dconst)
+    CONSTS = {False: ("Sq", "E48", "A4", "C2q", "E4", "A"),
+              True: ("tau", "ef", "ru", "A4", "C2q", "eu", "A")}
+    FROWS = {False: [0, 1, 2, 3], True: [0, 1, 2, 3, 4]}
+    EROWS = {False: [0, 4, 5, 3], True: [0, 5, 2, 6, 4]}
+
+    def _dconst(self, th):
+        """d(table constants)/d(pair parameters) by central differences of
+        setss (rows as CONSTS)"""
+        names = self.CONSTS[self.exp6]
+
+        def vec(t):
+            ss = self._pair_table(t)
+            return np.array([getattr(ss, n) for n in names])
+        th = np.asarray(th, dtype=np.float64)
+        out = np.zeros((len(names), len(th)))
+        for k in range(len(th)):
+            h = 1e-6 * th[k]
+            e = np.zeros(len(th))
+            e[k] = h
+            out[:, k] = (vec(th + e) - vec(th - e)) / (2 * h)
         return out
+
+    def _adjoint(self, r, w):
+        return self.eng.adjoint_exp6(r, w) if self.exp6 else self.eng.adjoint_lj(r, w)
 
     def loss_and_grad(self, theta, hq=1e-3):
         """L(theta) and dL/dtheta (adjoint for LJ, exact quadratic-form
@@ -154,20 +188,20 @@ class ForceMatch:
         theta = np.asarray(theta, dtype=np.float64)
         fr = self.forces(theta)
         L = 0.0
-        gF = np.zeros((self.nt, self.nt, 4))
+        gF = 0.0
         ws = []
         for (f, _), t, r in zip(fr, self.targets, self.frames):
             d = f[:self.ns] - t
             L += float((d * d).sum())
             w = 2 * d
             ws.append(w)
-            gf, _ = self.eng.adjoint_lj(r, w)
-            gF += gf
+            gF = gF + self._adjoint(r, w)[0]
         g = np.zeros(self.npar)
+        pp = self.pp
         for k, (a, b) in enumerate(self.pairs):
-            gc = gF[a, b] + (gF[b, a] if a != b else 0)          # d L / d(Sq, E48, A4, C2q)
-            J = self._dconst(theta[2 * k], theta[2 * k + 1])[:4]
-            g[2 * k:2 * k + 2] = gc @ J
+            gc = gF[a, b] + (gF[b, a] if a != b else 0)          # d L / d(force constants)
+            J = self._dconst(theta[pp * k:pp * (k + 1)])[self.FROWS[self.exp6]]
+            g[pp * k:pp * (k + 1)] = gc @ J
         for m in range(self.q_M.shape[1]):
             e = np.zeros(self.npar)
             e[self.npl + m] = hq
@@ -186,15 +220,16 @@ class ForceMatch:
         theta = np.asarray(theta, dtype=np.float64)
         outs = self.forces(theta, measure=1)
         E = sum(o.En_pot + o.En_el for _, o in outs)
-        gE = np.zeros((self.nt, self.nt, 4))
+        gE = 0.0
         zero = np.zeros((self.ns, 3))
         for r in self.frames:
-            gE += self.eng.adjoint_lj(r, zero)[1]
+            gE = gE + self._adjoint(r, zero)[1]
         g = np.zeros(self.npar)
+        pp = self.pp
         for k, (a, b) in enumerate(self.pairs):
-            gc = gE[a, b] + (gE[b, a] if a != b else 0)          # d E / d(Sq, E4, A, C2q)
-            J = self._dconst(theta[2 * k], theta[2 * k + 1])
-            g[2 * k:2 * k + 2] = gc @ J[[0, 4, 5, 3]]
+            gc = gE[a, b] + (gE[b, a] if a != b else 0)          # d E / d(energy constants)
+            J = self._dconst(theta[pp * k:pp * (k + 1)])
+            g[pp * k:pp * (k + 1)] = gc @ J[self.EROWS[self.exp6]]
         for m in range(self.q_M.shape[1]):
             e = np.zeros(self.npar)
             e[self.npl + m] = hq
