@@ -170,9 +170,17 @@ class _Lines:
         return self.toks.pop(0)
 
 
-def read_ble(path):
+def read_ble(path, form=None):
     """the force field of a .ble file in the engine's description
-    (types, type_mass, type_EvdW_kcal, type_RvdW, species, polar)"""
+    (types, type_mass, type_EvdW_kcal, type_RvdW, species, polar; for the
+    exp-6 form also form="exp6" and type_parm, the stiffness a).
+
+    The site-site form is cook's compile-time choice; the file has one
+    parameter column per nparms.  nparms=0 is Lennard-Jones; nparms=1 is
+    read as exp-6 when the file's table is the `exp-6` one (files with a
+    polar key) or when form="exp6" is given (the Busing and Buckingham
+    forms of one parameter are not carried)."""
+    from .system import EUNIT_KCAL
     text = open(path).read()
     # header: getdata up to ';' after skipping comments
     head, _ = parse_getdata(text)
@@ -182,15 +190,32 @@ def read_ble(path):
     distance14 = int(head.get("distance14", 4))
     polar = int(head.get("polar", -1))
     nparms = int(head.get("nparms", 0))
-    eunit = float(head.get("eunit", 1.0))
-    if nparms != 0:
-        raise CookInputError("nparms=%d: only the Lennard-Jones form (SS_PARMS=0) is supported" % nparms)
+    # energies of the file times eunit are K (cook/simdef.c:43,531,626;
+    # default kcal/mol); the engine's description keeps kcal/mol
+    escale = float(head.get("eunit", EUNIT_KCAL)) / EUNIT_KCAL
+    comb_rule = int(head.get("comb_rule", 0))
+    if nparms == 1:
+        if form is None and polar >= 0 and re.search(r"^exp-6(\s|$)", text, re.M):
+            form = "exp6"
+        if form != "exp6":
+            raise CookInputError("nparms=1: name the site-site form (form='exp6'); the Busing and "
+                                 "Buckingham forms are not supported")
+        if not comb_rule & 1:
+            raise CookInputError("exp-6: comb_rule&1 = 0 (Kirkwood) is not implemented "
+                                 "(sim/exp6/sitesite.c:52)")
+    elif nparms == 0:
+        if form not in (None, "lj"):
+            raise CookInputError("form %r needs nparms=1 in the file" % form)
+        form = "lj"
+    else:
+        raise CookInputError("nparms=%d: only the Lennard-Jones (0) and exp-6 (1) forms are "
+                             "supported" % nparms)
     if nnbfixes:
         raise CookInputError("nbfixes are not supported by the engine")
     n14 = 2 if distance14 else 1
     L = _Lines(text)
-    L.find("site-types" if polar < 0 else "Lennard-Jones")
-    types, mass, evdw, rvdw, alphas, pol = [], [], [], [], [], {}
+    L.find("site-types" if polar < 0 else ("exp-6" if form == "exp6" else "Lennard-Jones"))
+    types, mass, evdw, rvdw, alphas, pol, parm = [], [], [], [], [], {}, []
     for i in range(nsites):
         L.tok()                                   # site number
         name = L.tok()
@@ -200,8 +225,10 @@ def read_ble(path):
             t = L.tok(required=(j == 0))
             if t is None or t == "=":
                 break
-            groups.append((float(t), float(L.tok()) * eunit, float(L.tok())))
-        alpha, E, R = groups[0]
+            g = (float(t), float(L.tok()) * escale, float(L.tok()))
+            groups.append(g + tuple(float(L.tok()) for _ in range(nparms)))
+        alpha, E, R = groups[0][:3]
+        parm.append(groups[0][3:])
         if E > 0:
             E = -E                                # initcombrule(): EvdW is a minimum
         alphas.append(alpha)
@@ -263,17 +290,28 @@ def read_ble(path):
         species.append(dict(name=sname, site_type=site_type, site_charge_e=site_q,
                             site_xyz=site_xyz, site_excl=site_excl, bonds=bonds))
     ff = dict(types=types, type_mass=mass, type_EvdW_kcal=evdw, type_RvdW=rvdw,
-              type_alpha=alphas, species=species, comb_rule=int(head.get("comb_rule", 0)))
+              type_alpha=alphas, species=species, comb_rule=comb_rule)
+    if form == "exp6":
+        ff["form"] = "exp6"
+        ff["type_parm"] = [p[0] for p in parm]
     if pol:
         ff["polar"] = pol
     return ff
 
 
-def system_from_cook(ble_path, def_path, provider=None, **kw):
+def system_from_cook(ble_path, def_path, provider=None, form=None, exp6_cook_units=True, **kw):
     """(BuiltSystem, parameters) from a .ble force field and a .def deck:
-    N[i], L[0..2], cutoff, LJcutoff, el.epsr, el.epsk, el.epsinf"""
-    from .system import build_system
-    ff = read_ble(ble_path)
+    N[i], L[0..2], cutoff, LJcutoff, el.epsr, el.epsk, el.epsinf.
+
+    exp-6: cook's exp-6 initcombrule() multiplies EvdW by kcal/Eunit
+    (sim/exp6/sitesite.c:52-54) after readblend() has converted it with
+    eunit (cook/simdef.c:626), so cook's exp-6 runs with EvdW scaled twice;
+    exp6_cook_units=True reproduces that, False converts once as the other
+    forms do."""
+    from .system import EUNIT_KCAL, build_system
+    ff = read_ble(ble_path, form)
+    if ff.get("form") == "exp6" and exp6_cook_units:
+        kw.setdefault("exp6_evdw_scale", EUNIT_KCAL)
     d = read_get(def_path)
     counts = [int(d.get("N[%d]" % i, 0)) for i in range(len(ff["species"]))]
     L = [float(d["L[%d]" % k]) for k in range(3)]

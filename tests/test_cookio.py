@@ -98,7 +98,8 @@ def test_getdata_expressions_and_strings():
 
 @pytest.mark.parametrize("change,msg", [
     (("nnbfixes=0", "nnbfixes=1"), "nbfixes"),
-    (("nparms=0", "nparms=1"), "Lennard-Jones form"),
+    (("nparms=0", "nparms=1"), "site-site form"),
+    (("nparms=0", "nparms=2"), "Lennard-Jones"),
 ])
 def test_unsupported_input_is_rejected(tmp_path, change, msg):
     species, types = [mg.spce_species()], ["H", "O"]
@@ -132,3 +133,71 @@ def test_one_four_and_vibrating_bonds_are_rejected(tmp_path):
     open(ble, "w").write("\n".join(lines2) + "\n")
     with pytest.raises(cookio.CookInputError, match="vibrating"):
         cookio.read_ble(ble)
+
+
+def exp6_ble(tmp_path, a_of, table="exp-6", comb_rule=1):
+    """the SPC/E .ble of the fixtures turned into an exp-6 file: nparms=1,
+    the exp-6 table, a stiffness column after RvdW"""
+    species, types = [mg.spce_species()], ["H", "O"]
+    ble = str(tmp_path / "w6.ble")
+    mg.write_ble(ble, species, types)
+    lines = open(ble).read().replace("nparms=0", "nparms=1").replace(
+        "comb_rule=1", "comb_rule=%d" % comb_rule).splitlines()
+    k = lines.index("Lennard-Jones")
+    lines[k] = table
+    for i in range(k + 2, k + 2 + len(types)):
+        name = lines[i].split()[1]
+        lines[i] += "  %.3f" % a_of[name]
+    open(ble, "w").write("\n".join(lines) + "\n")
+    return ble, species, types
+
+
+def test_exp6_force_field_file(tmp_path):
+    from macsimus_amd import abi
+    from macsimus_amd.system import EUNIT_KCAL
+    a_of = {"H": 14.0, "O": 13.5}
+    ble, species, types = exp6_ble(tmp_path, a_of)
+    ff = cookio.read_ble(ble)
+    assert ff["form"] == "exp6" and ff["type_parm"] == [14.0, 13.5]
+    assert cookio.read_ble(ble, form="exp6") == ff
+    # a deck: cook's double kcal conversion by default, single on request
+    open(tmp_path / "w.def", "w").write("N[0]=64 L[0]=12.4 L[1]=12.4 L[2]=12.4 cutoff=6 "
+                                        "el.epsr=0.05 el.epsk=0.05;\n")
+    prov = oracle.provider()
+    bs2, _ = cookio.system_from_cook(ble, str(tmp_path / "w.def"), provider=prov)
+    bs1, _ = cookio.system_from_cook(ble, str(tmp_path / "w.def"), provider=prov,
+                                     exp6_cook_units=False)
+    t2, t1 = bs2.info["sstab"][3], bs1.info["sstab"][3]          # O-O
+    assert isinstance(t1, abi.Exp6) and bs1.struct.sstab_exp6
+    E_O = -ff["type_EvdW_kcal"][1] * EUNIT_KCAL
+    assert t1.eu == pytest.approx(6 * E_O / (13.5 - 6) * np.exp(13.5), rel=1e-12)
+    assert t2.eu == pytest.approx(EUNIT_KCAL * t1.eu, rel=1e-12)
+    # without a polar key the table is site-types: the form must be named
+    ble2, _, _ = exp6_ble(tmp_path, a_of, table="site-types")
+    txt = open(ble2).read().replace("polar=0", "")
+    open(ble2, "w").write(txt)
+    with pytest.raises(cookio.CookInputError, match="site-site form"):
+        cookio.read_ble(ble2)
+    assert cookio.read_ble(ble2, form="exp6")["type_parm"] == [14.0, 13.5]
+    ble3, _, _ = exp6_ble(tmp_path, a_of, comb_rule=0)
+    with pytest.raises(cookio.CookInputError, match="Kirkwood"):
+        cookio.read_ble(ble3)
+
+
+def test_eunit_of_the_file(tmp_path):
+    """energies given in K with eunit=1 read as the same force field as
+    the kcal/mol file (cook/simdef.c:43,531,626)"""
+    from macsimus_amd.system import EUNIT_KCAL
+    species, types = [mg.spce_species()], ["H", "O"]
+    ble = str(tmp_path / "k.ble")
+    mg.write_ble(ble, species, types)
+    ref = cookio.read_ble(ble)
+    lines = open(ble).read().replace("nparms=0;", "nparms=0 eunit=1;").splitlines()
+    k = lines.index("Lennard-Jones")
+    for i in range(k + 2, k + 2 + len(types)):
+        f = lines[i].split()
+        f[4] = "%.15g" % (float(f[4]) * EUNIT_KCAL)
+        lines[i] = "  " + "  ".join(f)
+    open(ble, "w").write("\n".join(lines) + "\n")
+    got = cookio.read_ble(ble)
+    np.testing.assert_allclose(got["type_EvdW_kcal"], ref["type_EvdW_kcal"], rtol=1e-14)
