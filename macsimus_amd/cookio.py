@@ -443,3 +443,63 @@ def read_rdf(path):
                         nmeas=struct.unpack_from("<i", rec, 60)[0],
                         hist=np.frombuffer(rec, "<u4", nhist, 68).astype(np.uint64)))
     return grid, nsites, out
+
+
+# ---------------------------------------------------------------- .plb
+class PlaybackWriter:
+    """cook's playback file (openplayback() and writeplayback(),
+    sim/simils.c:1550-1680): a header of two floats (ns, -3; -6 for a
+    centred file), then per frame the box L and the ns site vectors, all as
+    little-endian float32.  write(L, r) takes the positions r(t) (not
+    wrapped, as cook keeps them); for cook's -n2 pass r - r1/2 (r(t-h/2)),
+    for a .vlb file r1/h (velocities v(t-h/2)).  With centered=True the
+    sites are shifted by -L/2 (cook's -n4).  append=True continues a file
+    whose header matches ns."""
+
+    def __init__(self, path, ns, centered=False, append=False):
+        self.ns = int(ns)
+        self.centered = bool(centered)
+        if append:
+            with open(path, "rb") as f:
+                head = np.frombuffer(f.read(8), "<f4")
+            if len(head) != 2 or int(head[0]) != self.ns:
+                raise CookInputError("%s: not a playback file of %d sites" % (path, self.ns))
+            self.f = open(path, "ab")
+        else:
+            self.f = open(path, "wb")
+            self.f.write(np.array([self.ns, -6 if centered else -3], "<f4").tobytes())
+
+    def write(self, L, r):
+        r = np.asarray(r, dtype=np.float64)
+        if r.shape != (self.ns, 3):
+            raise ValueError("r must be [%d, 3]" % self.ns)
+        L = np.asarray(L, dtype=np.float64).reshape(3)
+        if self.centered:
+            r = r - L / 2
+        self.f.write(L.astype("<f4").tobytes() + r.astype("<f4").tobytes())
+        self.f.flush()
+
+    def close(self):
+        self.f.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def read_plb(path):
+    """(ns, header flag, L [frames, 3], r [frames, ns, 3]) of a playback
+    file (float32 as stored)"""
+    data = open(path, "rb").read()
+    if len(data) < 8:
+        raise CookInputError("%s: too short" % path)
+    ns, flag = np.frombuffer(data[:8], "<f4")
+    ns = int(ns)
+    fb = 12 * (ns + 1)
+    if (len(data) - 8) % fb:
+        raise CookInputError("%s: %d bytes after the header is not whole frames of %d sites"
+                             % (path, len(data) - 8, ns))
+    a = np.frombuffer(data[8:], "<f4").reshape(-1, ns + 1, 3)
+    return ns, float(flag), a[:, 0].copy(), a[:, 1:].copy()

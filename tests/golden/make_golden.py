@@ -297,6 +297,36 @@ def traj_case(name, species, types, counts, L, cutoff, nsteps, thermostat="none"
     print(name, len(dumps), "dumps")
 
 
+def plb_case(name, species, types, counts, L, cutoff, nsteps):
+    """cook's playback file (openplayback() and writeplayback() of
+    sim/simils.c:1550-1700) of a short SHAKE MD run with -y and dt.plb=h:
+    its bytes, and the positions of every force call (-v128)"""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "noint=%d" % nsteps, "T=298", "epsc=1e-12", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="none"\ndt.plb=0.002\nno=1;\n')
+        run([COOK, "sys", "md", "-v128", "-y%d" % sum(counts)], td)
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "md.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "md.%d.dump" % k)))
+            k += 1
+        raw = open(os.path.join(td, "md.plb"), "rb").read()
+    out = dict(L=dumps[0]["L"], r=np.stack([d["r"] for d in dumps]),
+               plb=np.frombuffer(raw, np.uint8))
+    meta = dict(name=name, counts=counts, L=L, cutoff=cutoff, h=0.002, ndumps=len(dumps),
+                plb_bytes=len(raw), forcefield=ff_arrays(species, types))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps", len(raw), "plb bytes")
+
+
 def parse_rdf(path):
     """cook's binary .rdf file (saverdf(), sim/simmeas.c:812-825): records
     <int4 size><data><int4 checksum> (gen/varfile.c); the grid, the number
@@ -470,6 +500,9 @@ def main(argv):
         "brine506_rdf": lambda: rdf_case(
             "brine506_rdf", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 3, 20.0),
+        # playback file of 216 SPC/E over 4 steps
+        "spce216_plb": lambda: plb_case(
+            "spce216_plb", [spce], ["H", "O"], [216], 18.6206, 8.0, 4),
         # NVE trajectory of 216 SPC/E (SHAKE + leap-frog)
         "spce216_nve": lambda: traj_case(
             "spce216_nve", [spce], ["H", "O"], [216], 18.6206, 8.0, 8),
