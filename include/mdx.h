@@ -106,7 +106,7 @@ typedef struct mdx_system_s {
   const double *mol_omega;   /* [nmol] SHAKE relaxation omega (= omegac/2) */
   /* site-site form: NULL = Lennard-Jones (sstab); else the exp-6 table
      [ntypes*ntypes] (sstab is then not read).  mdx_create only; the
-     batch and the adjoint take Lennard-Jones systems. */
+     batch takes Lennard-Jones systems. */
   const mdx_exp6_t *sstab_exp6;
 } mdx_system_t;
 
@@ -154,9 +154,12 @@ typedef struct mdx_step_s {
   double h;            /* time step [ps] */
   double eps;          /* SHAKE relative tolerance (epsc) */
   int maxit;           /* max SHAKE sweeps per molecule */
-  int thermostat;      /* 0 = none (NVE), 1 = Berendsen (sim/thermo.c:141-160) */
-  double T;            /* Berendsen target temperature [K] */
-  double tau;          /* Berendsen time constant [ps] */
+  int thermostat;      /* 0 = none (NVE), 1 = Berendsen (sim/thermo.c:141-160),
+                          2 = Nose-Hoover with the time-reversible velocity
+                          predictor (sim/trvpscale.c, sim/verlet.c:22-36,
+                          sim/thermo.c:180-200); not with Drudes */
+  double T;            /* target temperature [K] */
+  double tau;          /* thermostat time constant tau.T [ps] */
   double nf;           /* No.f degrees of freedom */
   int measure_every;   /* energies/virial on every n-th step (cook: noint) */
   /* POLAR: always-stable predictor of the Drude offsets (sim/pred2.c) of
@@ -166,6 +169,9 @@ typedef struct mdx_step_s {
   int aspc_k;
   int scf_iter;
   double scf_omega;
+  /* Nose-Hoover: length of the velocity predictor (cook -p option % 10,
+     0..9); the predictor grows to it over the first steps (sim/trvpscale.c) */
+  int trvp_k;
 } mdx_step_t;
 
 typedef struct mdx_step_out_s {
@@ -179,6 +185,9 @@ typedef struct mdx_step_out_s {
   double PTvirc[6];    /* constraint virial tensor */
   double lambda;       /* last Berendsen scaling factor */
   int shake_iter_max;  /* max SHAKE sweeps used by a molecule */
+  /* Nose-Hoover, last measured step: En.ext (VERLET=3 form), logs after
+     the step and En.dlogs (sim/thermo.c:180-200) */
+  double En_ext, logs, dlogs;
 } mdx_step_out_t;
 
 /* Replaces one selffield() iteration (sim/scf.c:12-240, POLAR=0):
@@ -224,6 +233,16 @@ int mdx_set_overlap(mdx_ctx_t *ctx, int on);
    aspc_k than the context holds restarts the predictor (kk = 0). */
 int mdx_set_aspc(mdx_ctx_t *ctx, int k, int kk, const double *hist);
 int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
+
+/* Nose-Hoover state: logs = cfg[0]->logs, logs1 = cfg[1]->logs
+   (logs(t) - logs(t-h)), the velocity predictor's length k (0..9), the
+   length reached kk (0..k) and its history hist [k][1 + 3 ns]: per level
+   logs1, then r1 of the sites (hist = NULL: zeros).  cook keeps k, kk and
+   the history in statics of sim/constrd.c:673-684 and logs in the cfg;
+   mdx_md_steps with another trvp_k than the context holds restarts the
+   predictor (kk = 0).  Both logs start at 0 in mdx_create. */
+int mdx_set_nose(mdx_ctx_t *ctx, double logs, double logs1, int k, int kk, const double *hist);
+int mdx_get_nose(mdx_ctx_t *ctx, double *logs, double *logs1, int *k, int *kk, double *hist);
 
 /* RDF histograms of the measured force calls, as cook records them for
    rdf.grid and rdf.cutoff: initrdf() (sim/simmeas.c:655-767), the counts

@@ -503,3 +503,81 @@ def read_plb(path):
                              % (path, len(data) - 8, ns))
     a = np.frombuffer(data[8:], "<f4").reshape(-1, ns + 1, 3)
     return ns, float(flag), a[:, 0].copy(), a[:, 1:].copy()
+
+
+# ---------------------------------------------------------------- .cfg
+def read_cfg(path):
+    """cook's configuration file (savecfg(), sim/simils.c:1445-1517): the
+    varfile records cfgkey, optionlist[32], nspec, spec[nspec] (N, ns),
+    three key records (1: N, t, h, En.tot; 2: ns, L; 4: thermostat,
+    sigvdW, tau.T, tau.P; 17 for Drude systems: tau.dip), then cfg[0],
+    cfg[1], .. as ToInt blocks (sim/simglob.h:306-315: size, dep, logs,
+    lambda[3], offdiag[3], rp[nrow][3]), closed by KeyClose(): an int4 0
+    and the int4 time key.  Returns a dict with those fields; cfgs is a
+    list of dict(dep, logs, lambda, offdiag, rp)."""
+    data = open(path, "rb").read()
+    recs, o = [], 0
+    while True:
+        if o + 4 > len(data):
+            raise CookInputError("%s: no end record" % path)
+        (n,) = struct.unpack_from("<i", data, o)
+        o += 4
+        if n == 0:
+            break
+        rec = data[o:o + n]
+        if len(rec) != n or o + n + 4 > len(data):
+            raise CookInputError("%s: truncated record" % path)
+        (chk,) = struct.unpack_from("<I", data, o + n)
+        if (0x4A4B + sum(rec)) & 0xffffffff != chk:
+            raise CookInputError("%s: checksum" % path)
+        recs.append(rec)
+        o += n + 4
+    if o + 4 != len(data):
+        raise CookInputError("%s: %d bytes after the end record" % (path, len(data) - o))
+    (timekey,) = struct.unpack_from("<i", data, o)
+    out = dict(cfgkey=struct.unpack("<i", recs[0])[0],
+               options=list(struct.unpack("<32i", recs[1])),
+               nspec=struct.unpack("<i", recs[2])[0], timekey=timekey, keys={})
+    out["spec"] = [struct.unpack_from("<2i", recs[3], 8 * i) for i in range(out["nspec"])]
+    k = 4
+    while k < len(recs) and len(recs[k]) == 32:
+        key, iv = struct.unpack_from("<2i", recs[k])
+        out["keys"][key] = (iv, list(struct.unpack_from("<3d", recs[k], 8)))
+        k += 1
+    cfgs = []
+    for rec in recs[k:]:
+        size, dep = struct.unpack_from("<2i", rec)
+        if size != len(rec) or (size - 64) % 24:
+            raise CookInputError("%s: bad configuration record" % path)
+        v = np.frombuffer(rec, "<f8", offset=8)
+        cfgs.append(dict(dep=dep, logs=float(v[0]), **{"lambda": v[1:4].copy()},
+                         offdiag=v[4:7].copy(), rp=v[7:].reshape(-1, 3).copy()))
+    out["cfgs"] = cfgs
+    if 1 in out["keys"]:
+        N, (t, h, Etot) = out["keys"][1]
+        out.update(N=N, t=t, h=h, Etot=Etot)
+    if 2 in out["keys"]:
+        out.update(ns=out["keys"][2][0], L=out["keys"][2][1])
+    if 4 in out["keys"]:
+        th, (sig, tauT, tauP) = out["keys"][4]
+        out.update(thermostat=th, sigvdW=sig, tauT=tauT, tauP=tauP)
+    return out
+
+
+def write_cfg(path, cfg):
+    """the configuration file of a dict as read_cfg returns it (records in
+    the same order; a cfg level's ToInt is rebuilt from dep, logs, lambda,
+    offdiag and rp), e.g. a cook file with the engine's downloaded state
+    put into cfgs[0]["rp"] (r) and cfgs[1]["rp"] (r1 = r(t)-r(t-h))"""
+    out = [_var_record(struct.pack("<i", cfg["cfgkey"])),
+           _var_record(struct.pack("<32i", *cfg["options"])),
+           _var_record(struct.pack("<i", len(cfg["spec"]))),
+           _var_record(b"".join(struct.pack("<2i", *s) for s in cfg["spec"]))]
+    for key, (iv, vec) in cfg["keys"].items():
+        out.append(_var_record(struct.pack("<2i3d", key, iv, *vec)))
+    for c in cfg["cfgs"]:
+        rp = np.ascontiguousarray(c["rp"], dtype="<f8").reshape(-1, 3)
+        body = np.concatenate([[c["logs"]], c["lambda"], c["offdiag"], rp.ravel()]).astype("<f8")
+        out.append(_var_record(struct.pack("<2i", 8 + body.nbytes, c["dep"]) + body.tobytes()))
+    with open(path, "wb") as f:
+        f.write(b"".join(out) + struct.pack("<2i", 0, cfg["timekey"]))

@@ -1144,7 +1144,8 @@ int oracle_selffield(const mdx_system_t *S, double *r, double *f, mdx_out_t *out
    thermostat none or Berendsen, no Nose/NPT.  r = cfg[0], r1 = cfg[1]
    are updated in place; f is scratch (cfg[2]).  Energies returned in out. */
 static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
-                          const mdx_step_t *p, int measure, mdx_step_out_t *out);
+                          const mdx_step_t *p, int measure, mdx_step_out_t *out,
+                          const double *vpred, double vfac);
 
 int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
                    const mdx_step_t *p, int measure, mdx_step_out_t *out,
@@ -1154,7 +1155,7 @@ int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
   memset(out, 0, sizeof(*out));
   err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
   if (err) return err;
-  return orc_shake_step(S, r, r1, f, p, measure, out);
+  return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0);
 }
 
 /* ASPC coefficients B_1..B_{kk+2} (sim/pred2.c:94-98) */
@@ -1209,11 +1210,12 @@ int oracle_md_step_polar(const mdx_system_t *S, double *r, double *r1, double *f
     out->En_self = es;
     out->scf_maxerr = me;
   }
-  return orc_shake_step(S, r, r1, f, p, measure, out);
+  return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0);
 }
 
 static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
-                          const mdx_step_t *p, int measure, mdx_step_out_t *out)
+                          const mdx_step_t *p, int measure, mdx_step_out_t *out,
+                          const double *vpred, double vfac)
 {
   int ns = S->ns, n, i, k, al;
   double hh = p->h * p->h, hscale = 0.5 / hh;
@@ -1225,9 +1227,15 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
     int c0 = S->cons_first[n], c1 = S->cons_first[n + 1], nc = c1 - c0, it = 0;
     int maxit = p->maxit > 0 ? p->maxit : maxit0 * (nc + 4);
     double omega = S->mol_omega ? S->mol_omega[n] : 0.6, maxsq;
-    for (i = i0; i < i1; i++)        /* verlet.c, no thermostat branch */
-      for (k = 0; k < 3; k++)
-        pp[3 * i + k] = hh / S->mass[i] * f[3 * i + k] + r[3 * i + k] + r1[3 * i + k];
+    if (vpred)                         /* verlet.c, Nose with predicted velocities */
+      for (i = i0; i < i1; i++)
+        for (k = 0; k < 3; k++)
+          pp[3 * i + k] = hh / S->mass[i] * f[3 * i + k] - vfac * vpred[3 * i + k]
+                          + r[3 * i + k] + r1[3 * i + k];
+    else                               /* verlet.c, no thermostat branch */
+      for (i = i0; i < i1; i++)
+        for (k = 0; k < 3; k++)
+          pp[3 * i + k] = hh / S->mass[i] * f[3 * i + k] + r[3 * i + k] + r1[3 * i + k];
     if (nc) do {
       it++;
       maxsq = 0;
@@ -1292,6 +1300,58 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
   out->virc = virc;
   out->shake_iter_max = itmax;
   free(pp);
+  return 0;
+}
+
+/* One Shake() step with the Nose-Hoover thermostat (thermostat == 2):
+   the time-reversible velocity predictor of sim/trvpscale.c (length kk
+   known so far, growing to p->trvp_k; hist[j-1] = level j of the history,
+   [1 + 3 ns] doubles: logs velocity h dlogs/dt, then the h v of the
+   sites), the Verlet step of sim/verlet.c with the friction -Vfactor
+   vpred, SHAKE, and the leap-frog of logs of sim/thermo.c:180-200 (VERLET
+   = 3 extended energy).  nose[0] = logs, nose[1] = logs(t) - logs(t-h)
+   (cfg[1]->logs), updated; nose[2] = En.ext, nose[3] = En.dlogs out. */
+int oracle_md_step_nose(const mdx_system_t *S, double *r, double *r1, double *f,
+                        const mdx_step_t *p, double nose[4], int *kk, double *hist,
+                        int measure, mdx_step_out_t *out, int nthreads)
+{
+  int ns = S->ns, stride = 1 + 3 * ns, k = *kk, j, i, err, knew;
+  double b[10], vfac, hh = p->h * p->h, *vpred, ld, logs0, Tk;
+  if (p->thermostat != 2 || p->trvp_k < 0 || p->trvp_k > 9 || k > p->trvp_k) return MDX_E_ARG;
+  err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
+  if (err) return err;
+  /* TRVP coefficients for k (sim/trvpscale.c:28-33) */
+  b[0] = (double)(2 * k + 1) / (k + 1);
+  for (j = 0; j < k; j++) b[j + 1] = -b[j] * (double)(k - j) / (k + j + 2);
+  vpred = malloc(sizeof(double) * 3 * ns);
+  vfac = b[0] * nose[1];
+  for (j = 1; j <= k; j++) vfac += b[j] * hist[(size_t)(j - 1) * stride];
+  for (i = 0; i < 3 * ns; i++) {
+    double v = b[0] * r1[i];
+    for (j = 1; j <= k; j++) v += b[j] * hist[(size_t)(j - 1) * stride + 1 + i];
+    vpred[i] = v;
+  }
+  /* the history grows to trvp_k and shifts: level 1 gets (logs1, r1) */
+  knew = k + 1 < p->trvp_k ? k + 1 : p->trvp_k;
+  for (j = knew; j >= 2; j--)
+    memcpy(hist + (size_t)(j - 1) * stride, hist + (size_t)(j - 2) * stride, sizeof(double) * stride);
+  if (knew >= 1) {
+    hist[0] = nose[1];
+    memcpy(hist + 1, r1, sizeof(double) * 3 * ns);
+  }
+  *kk = knew;
+  err = orc_shake_step(S, r, r1, f, p, measure, out, vpred, vfac);
+  free(vpred);
+  if (err) return err;
+  /* logs by leap-frog with the doubled kinetic energy (thermo.c) */
+  Tk = 2 * out->En_kin / p->nf;
+  ld = nose[1];
+  logs0 = nose[0];
+  nose[1] += hh * (Tk / p->T - 1) / (p->tau * p->tau);
+  nose[0] += nose[1];
+  nose[3] = (ld + nose[1]) / (2 * p->h);
+  nose[2] = (logs0 + (p->tau / p->h) * (p->tau / p->h) * (ld * ld + nose[1] * nose[1]) / 4)
+            * p->nf * p->T;
   return 0;
 }
 

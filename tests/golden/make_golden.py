@@ -327,6 +327,46 @@ def plb_case(name, species, types, counts, L, cutoff, nsteps):
     print(name, len(dumps), "dumps", len(raw), "plb bytes")
 
 
+def nose_case(name, species, types, counts, L, cutoff, nsteps, tauT, pred):
+    """Nose-Hoover with the time-reversible velocity predictor in cook's
+    Shake() (sim/trvpscale.c, sim/verlet.c, sim/thermo.c): run 1 (crystal,
+    Berendsen, 5 steps) saves md.cfg; run 2 starts from it (init="start")
+    with thermostat="Nose", tau.T and -p<pred>, every force call dumped by
+    -v128.  Stored: both .cfg files as bytes (the start state r, r1, logs
+    and the final one) and the dumped positions and forces."""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "T=298", "epsc=1e-12", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="Berendsen"\ntau.T=0.1\nnoint=5\nno=1;\n')
+        run([COOK, "sys", "md"], td)
+        cfg0 = open(os.path.join(td, "md.cfg"), "rb").read()
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="start"\nthermostat="Nose"\ntau.T=%g\nnoint=%d\nno=1;\n' % (tauT, nsteps))
+        run([COOK, "sys", "md", "-v128", "-p%d" % pred], td)
+        cfg1 = open(os.path.join(td, "md.cfg"), "rb").read()
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "md.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "md.%d.dump" % k)))
+            k += 1
+        meta = parse_prt(os.path.join(td, "md.prt"))
+    out = dict(L=dumps[0]["L"], r=np.stack([d["r"] for d in dumps]),
+               f=np.stack([d["f"] for d in dumps]),
+               cfg0=np.frombuffer(cfg0, np.uint8), cfg1=np.frombuffer(cfg1, np.uint8))
+    meta.update(dict(name=name, counts=counts, L=L, cutoff=cutoff, epsr=0.05, epsk=0.05,
+                     h=0.002, T=298.0, tauT=tauT, pred=pred, nsteps=nsteps, ndumps=len(dumps),
+                     forcefield=ff_arrays(species, types)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps", len(cfg0), len(cfg1), "cfg bytes")
+
+
 def parse_rdf(path):
     """cook's binary .rdf file (saverdf(), sim/simmeas.c:812-825): records
     <int4 size><data><int4 checksum> (gen/varfile.c); the grid, the number
@@ -500,6 +540,9 @@ def main(argv):
         "brine506_rdf": lambda: rdf_case(
             "brine506_rdf", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 3, 20.0),
+        # Nose-Hoover (TRVP k=3) on 216 SPC/E from a saved configuration
+        "spce216_nose": lambda: nose_case(
+            "spce216_nose", [spce], ["H", "O"], [216], 18.6206, 8.0, 8, 0.05, 3),
         # playback file of 216 SPC/E over 4 steps
         "spce216_plb": lambda: plb_case(
             "spce216_plb", [spce], ["H", "O"], [216], 18.6206, 8.0, 4),

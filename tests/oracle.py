@@ -190,3 +190,27 @@ def aspc_state(d, post, k):
     r1 = np.concatenate([d["r"][k] - d["r"][k - 1], post[k - 2]])
     hist = np.stack([post[k - 3 - m] for m in range(ASPC_K)])
     return r, r1, hist
+
+
+def md_step_nose(bs, r, r1, nose, kk, hist, p, measure=1, nthreads=None):
+    """one Shake() step with Nose-Hoover + TRVP; r, r1, nose [4] (logs,
+    logs1, En.ext, dlogs) and hist [k][1+3ns] updated in place; returns
+    (StepOut, new kk)"""
+    lib = load()
+    dp = C.POINTER(C.c_double)
+    lib.oracle_md_step_nose.argtypes = [C.POINTER(abi.System), dp, dp, dp, C.POINTER(abi.Step),
+                                        dp, C.POINTER(C.c_int), dp, C.c_int,
+                                        C.POINTER(abi.StepOut), C.c_int]
+    for a in (r, r1, nose, hist):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    assert hist.shape[0] >= max(p.trvp_k, 1) and hist.shape[1] == 1 + r.size
+    f = np.zeros_like(r)
+    out = abi.StepOut()
+    k = C.c_int(kk)
+    rc = lib.oracle_md_step_nose(C.byref(bs.struct), r.ctypes.data_as(dp), r1.ctypes.data_as(dp),
+                                 f.ctypes.data_as(dp), C.byref(p), nose.ctypes.data_as(dp),
+                                 C.byref(k), hist.ctypes.data_as(dp), measure, C.byref(out),
+                                 nthreads or os.cpu_count() or 1)
+    if rc:
+        raise RuntimeError("oracle_md_step_nose failed (%d)" % rc)
+    return out, k.value
