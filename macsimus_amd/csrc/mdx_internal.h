@@ -223,6 +223,10 @@ struct Ctx {
   // Drude ASPC history aa[2..k+1] (aa[0], aa[1] are the Drude rows of r, r1)
   DBuf<double> aspc_hist;
   int aspc_k = -1, aspc_kk = 0;
+  // Nose-Hoover: nose = [logs, logs(t)-logs(t-h), En.ext, En.dlogs] and the
+  // velocity predictor's history [trvp_k][1 + 3 ns] of length trvp_kk
+  DBuf<double> nose, trvp_hist;
+  int trvp_k = -1, trvp_kk = 0;
   // configuration generation: bumped by every change of buffers, grid or
   // k-space split, so that a captured step graph is rebuilt
   unsigned gen = 0;

@@ -39,23 +39,62 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+// the time-reversible velocity predictor of one step (sim/trvpscale.c):
+// coefficients b[0..kk] for the history known so far, the length knew
+// after this step
+struct Trvp {
+  int kk, knew;
+  double b[10];
+};
+
+Trvp trvp_coef(int kk, int kmax) {
+  Trvp t{};
+  t.kk = kk;
+  t.knew = std::min(kk + 1, kmax);
+  t.b[0] = (double)(2 * kk + 1) / (kk + 1);
+  for (int j = 0; j < kk; j++) t.b[j + 1] = -t.b[j] * (double)(kk - j) / (kk + j + 2);
+  return t;
+}
+
 // one thread per molecule: p = h^2 f/m + r + r1, SHAKE sweeps until the
-// largest relative violation is <= eps, then r1 = p - r, r = p
+// largest relative violation is <= eps, then r1 = p - r, r = p.
+// Nose-Hoover (nose != NULL, sim/verlet.c:22-36): p -= Vfactor vpred with
+// the predicted h v of each site and h dlogs/dt from b and the history
+// (level 0 = the current r1 and logs1, level j = hist[j-1], stride 1+3ns:
+// logs1 then the sites); each thread then shifts its sites' history (the
+// logs slots are shifted by k_nose after all threads have read them)
 __global__ __launch_bounds__(kBlock) void k_shake(
     int nmol, const int *__restrict__ molfirst, const int *__restrict__ cons_first,
     const int *__restrict__ cons_ij, const double *__restrict__ bondq,
     const double *__restrict__ omega, const double *__restrict__ mass, double h, double eps,
     int maxit, int measure, double *__restrict__ r, double *__restrict__ r1,
-    const double *__restrict__ f, double *__restrict__ p, double *__restrict__ part) {
+    const double *__restrict__ f, double *__restrict__ p, double *__restrict__ part,
+    const double *__restrict__ nose, double *__restrict__ hist, long stride, Trvp tv) {
   const int m = blockIdx.x * kBlock + threadIdx.x;
   double ek = 0, pk[6] = {0, 0, 0, 0, 0, 0}, virc = 0, ptv[6] = {0, 0, 0, 0, 0, 0};
   double its = 0, bad = 0;
   if (m < nmol) {
     const double hh = h * h;
     const int b = molfirst[m], e = molfirst[m + 1];
-    for (int i = b; i < e; i++) {
-      const double im = hh / mass[i];
-      for (int k = 0; k < 3; k++) p[3 * i + k] = im * f[3 * i + k] + r[3 * i + k] + r1[3 * i + k];
+    if (nose) {
+      double vfac = tv.b[0] * nose[1];
+      for (int j = 1; j <= tv.kk; j++) vfac += tv.b[j] * hist[(j - 1) * stride];
+      for (int i = b; i < e; i++) {
+        const double im = hh / mass[i];
+        for (int k = 0; k < 3; k++) {
+          const long x = 3L * i + k;
+          double v = tv.b[0] * r1[x];
+          for (int j = 1; j <= tv.kk; j++) v += tv.b[j] * hist[(j - 1) * stride + 1 + x];
+          p[x] = im * f[x] - vfac * v + r[x] + r1[x];
+          for (int j = tv.knew; j >= 2; j--) hist[(j - 1) * stride + 1 + x] = hist[(j - 2) * stride + 1 + x];
+          if (tv.knew >= 1) hist[1 + x] = r1[x];
+        }
+      }
+    } else {
+      for (int i = b; i < e; i++) {
+        const double im = hh / mass[i];
+        for (int k = 0; k < 3; k++) p[3 * i + k] = im * f[3 * i + k] + r[3 * i + k] + r1[3 * i + k];
+      }
     }
     const int c0 = cons_first[m], c1 = cons_first[m + 1], nc = c1 - c0;
     const double om = omega[m];
@@ -161,6 +200,24 @@ __global__ void k_berendsen(int n3, double *r1, const double *acc, double h, dou
   const double lam = exp(b * -h / tau);
   if (i < n3) r1[i] *= lam;
   if (i == 0 && lambda_out) *lambda_out = lam;
+}
+
+// Nose-Hoover, after SHAKE (sim/thermo.c:180-200): shifts the logs slots
+// of the predictor history, then the leap-frog of logs with the doubled
+// kinetic energy T_kin = 2 E_kin / No.f and the VERLET=3 extended energy
+// En.ext = No.f T (logs(t) + (tau/h)^2 (dlogs_old^2 + dlogs_new^2)/4)
+__global__ void k_nose(const double *acc, double *nose, double *hist, long stride, int knew,
+                       double h, double T, double tau, double nf) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double hh = h * h;
+  const double Tk = acc[MD_EKIN] * 0.5 / hh / nf;
+  const double ld = nose[1], logs0 = nose[0];
+  for (int j = knew; j >= 2; j--) hist[(j - 1) * stride] = hist[(j - 2) * stride];
+  if (knew >= 1) hist[0] = ld;
+  nose[1] = ld + hh * (Tk / T - 1) / (tau * tau);
+  nose[0] = logs0 + nose[1];
+  nose[3] = (ld + nose[1]) / (2 * h);
+  nose[2] = (logs0 + (tau / h) * (tau / h) * (ld * ld + nose[1] * nose[1]) / 4) * nf * T;
 }
 
 // ------------------------------------------------------------------ Drude
@@ -337,13 +394,40 @@ int set_aspc(Ctx &c, int k, int kk, const double *hist) {
   return 0;
 }
 
+// the predictor of the Nose-Hoover step: length k (0..9), reached kk,
+// history [k][1+3ns] (NULL: zeros)
+int set_trvp(Ctx &c, int k, int kk, const double *hist) {
+  if (k < 0 || k > 9 || kk < 0 || kk > k) return fail(MDX_E_ARG, "Nose: need 0 <= kk <= k <= 9");
+  const size_t n = (size_t)std::max(k, 1) * (1 + 3 * (size_t)c.ns);
+  int rc;
+  if (c.trvp_k != k || !c.trvp_hist.p) {
+    c.trvp_hist.release();
+    if ((rc = c.trvp_hist.alloc(n))) return rc;
+    c.gen++;
+  }
+  if (hist && k > 0)
+    MDX_HIP(hipMemcpyAsync(c.trvp_hist.p, hist, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+  else
+    k_fill<<<nblk((long)n), kBlock, 0, c.stream>>>((long)n, c.trvp_hist.p, 0.0);
+  MDX_HIP(hipGetLastError());
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  c.trvp_k = k;
+  c.trvp_kk = kk;
+  return 0;
+}
+
 }  // namespace
 
 int mdx::md_setup(Ctx &c) {
   int rc;
   if ((rc = c.mdflag.alloc(4))) return rc;
-  if (c.mass.p && ((rc = c.mdpart.alloc((size_t)nblk(c.nmol) * MD_N)) || (rc = c.mdacc.alloc(MD_N))))
+  if (c.mass.p && ((rc = c.mdpart.alloc((size_t)nblk(c.nmol) * MD_N)) || (rc = c.mdacc.alloc(MD_N)) ||
+                    (rc = c.nose.alloc(4))))
     return rc;
+  if (c.mass.p) {
+    k_fill<<<1, 64, 0, c.stream>>>(4, c.nose.p, 0.0);
+    MDX_HIP(hipGetLastError());
+  }
   if (c.polar && ((rc = c.sfpart.alloc((size_t)nblk(c.ns) * SF_N)) || (rc = c.sfacc.alloc(SF_N))))
     return rc;
   return 0;
@@ -433,6 +517,39 @@ extern "C" int mdx_get_aspc(mdx_ctx_t *h, int *k, int *kk, double *hist) {
   return 0;
 }
 
+extern "C" int mdx_set_nose(mdx_ctx_t *h, double logs, double logs1, int k, int kk,
+                            const double *hist) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_nose: NULL handle");
+  Ctx &c = h->c;
+  if (!c.mass.p) return fail(MDX_E_ARG, "mdx_set_nose: system has no masses");
+  MDX_HIP(hipSetDevice(c.device));
+  int rc;
+  if ((rc = set_trvp(c, k, kk, hist))) return rc;
+  const double v[4] = {logs, logs1, 0, 0};
+  MDX_HIP(hipMemcpyAsync(c.nose.p, v, sizeof(v), hipMemcpyHostToDevice, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+extern "C" int mdx_get_nose(mdx_ctx_t *h, double *logs, double *logs1, int *k, int *kk,
+                            double *hist) {
+  if (!h) return fail(MDX_E_ARG, "mdx_get_nose: NULL handle");
+  Ctx &c = h->c;
+  if (!c.mass.p) return fail(MDX_E_ARG, "mdx_get_nose: system has no masses");
+  MDX_HIP(hipSetDevice(c.device));
+  double v[4];
+  MDX_HIP(hipMemcpyAsync(v, c.nose.p, sizeof(v), hipMemcpyDeviceToHost, c.stream));
+  if (hist && c.trvp_k > 0)
+    MDX_HIP(hipMemcpyAsync(hist, c.trvp_hist.p, sizeof(double) * c.trvp_k * (1 + 3 * (size_t)c.ns),
+                           hipMemcpyDeviceToHost, c.stream));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  if (logs) *logs = v[0];
+  if (logs1) *logs1 = v[1];
+  if (k) *k = std::max(c.trvp_k, 0);
+  if (kk) *kk = c.trvp_kk;
+  return 0;
+}
+
 namespace {
 
 // the launches of one MD step (no host synchronization): for Drudes the
@@ -454,15 +571,22 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) 
     return rc;
   }
   if (c.timing) c.tm.start(T_SHAKE, c.stream);
+  const bool nose = p->thermostat == 2;
+  const long stride = 1 + 3L * c.ns;
+  const Trvp tv = trvp_coef(nose ? kk : 0, nose ? c.trvp_k : 0);
   k_shake<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.cons_first.p, c.cons_ij.p,
                                        c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
                                        p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
-                                       c.mdpart.p);
+                                       c.mdpart.p, nose ? c.nose.p : nullptr, c.trvp_hist.p,
+                                       stride, tv);
   if (c.timing) c.tm.stop(T_SHAKE, c.stream);
   k_md_reduce<<<1, kBlock, 0, c.stream>>>(c.mdpart.p, nb, c.mdacc.p, c.mdflag.p);
   if (p->thermostat == 1)
     k_berendsen<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r1.p, c.mdacc.p, p->h,
                                                           p->T, p->tau, p->nf, c.mdflag.p);
+  if (nose)
+    k_nose<<<1, 64, 0, c.stream>>>(c.mdacc.p, c.nose.p, c.trvp_hist.p, stride, tv.knew, p->h,
+                                   p->T, p->tau, p->nf);
   if (hipGetLastError() != hipSuccess) return fail(MDX_E_GPU, "mdx_md_steps: launch");
   return 0;
 }
@@ -471,7 +595,7 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) 
 bool same_step(const mdx_step_t &a, const mdx_step_t &b) {
   return a.h == b.h && a.eps == b.eps && a.maxit == b.maxit && a.thermostat == b.thermostat &&
          a.T == b.T && a.tau == b.tau && a.nf == b.nf && a.aspc_k == b.aspc_k &&
-         a.scf_iter == b.scf_iter && a.scf_omega == b.scf_omega;
+         a.scf_iter == b.scf_iter && a.scf_omega == b.scf_omega && a.trvp_k == b.trvp_k;
 }
 
 // an unmeasured step as a hipGraph, captured once per step parameters and
@@ -521,29 +645,40 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
   Ctx &c = h->c;
   if (!c.state_on_device) return fail(MDX_E_ARG, "mdx_md_steps: upload the state first");
   if (!(p->h > 0) || !(p->eps > 0)) return fail(MDX_E_ARG, "mdx_md_steps: h and eps must be > 0");
+  if (p->thermostat < 0 || p->thermostat > 2)
+    return fail(MDX_E_ARG, "mdx_md_steps: thermostat 0 (none), 1 (Berendsen) or 2 (Nose-Hoover)");
   if (p->thermostat && !(p->tau > 0 && p->nf > 0 && p->T > 0))
-    return fail(MDX_E_ARG, "mdx_md_steps: Berendsen needs T, tau, nf > 0");
+    return fail(MDX_E_ARG, "mdx_md_steps: the thermostat needs T, tau, nf > 0");
+  if (p->thermostat == 2 && c.polar)
+    return fail(MDX_E_ARG, "mdx_md_steps: Nose-Hoover is not carried for Drude systems");
+  if (p->thermostat == 2 && (p->trvp_k < 0 || p->trvp_k > 9))
+    return fail(MDX_E_ARG, "mdx_md_steps: trvp_k in 0..9");
   if (c.polar && !(p->scf_omega <= 1))
     return fail(MDX_E_ARG, "mdx_md_steps: scf_omega must be <= 1 (<= 0: stability value)");
   MDX_HIP(hipSetDevice(c.device));
   int rc;
   if (c.polar && c.aspc_k != p->aspc_k && (rc = set_aspc(c, p->aspc_k, 0, nullptr))) return rc;
+  if (p->thermostat == 2 && c.trvp_k != p->trvp_k && (rc = set_trvp(c, p->trvp_k, 0, nullptr)))
+    return rc;
   std::memset(out, 0, sizeof(*out));
   out->lambda = 1;
   const int every = p->measure_every > 0 ? p->measure_every : nsteps;
-  double acc[ACC_N], md[MD_N], sf[SF_N], flag[4];
+  double acc[ACC_N], md[MD_N], sf[SF_N], flag[4], nv[4];
   reset_flags(c);
   rc = 0;
   for (int s = 1; s <= nsteps && !rc; s++) {
     const int measure = (s % every == 0) || s == nsteps;
-    // predictor length of this step (grows to aspc_k, sim/pred2.c:67-77)
-    const int kk = c.aspc_kk;
+    // predictor length of this step (ASPC: grows to aspc_k, sim/pred2.c:
+    // 67-77; Nose-Hoover: grows to trvp_k, sim/trvpscale.c:95)
+    const bool nose = p->thermostat == 2;
+    const int kk = nose ? c.trvp_kk : c.aspc_kk;
     if (c.polar && c.aspc_kk < c.aspc_k) c.aspc_kk++;
+    if (nose && c.trvp_kk < c.trvp_k) c.trvp_kk++;
     const double omega = !c.polar ? 0.0
                          : p->scf_omega > 0 ? p->scf_omega
                                             : (c.aspc_kk + 2) / (double)(2 * c.aspc_kk + 3);
     // unmeasured steps of the steady state replay a captured graph
-    const bool steady = !c.polar || kk == c.aspc_k;
+    const bool steady = nose ? kk == c.trvp_k : (!c.polar || kk == c.aspc_k);
     if (!measure && steady && c.use_graph && !c.timing) rc = graph_step(c, p, kk, omega);
     else rc = launch_step(c, p, measure, kk, omega);
     if (rc) break;
@@ -551,6 +686,7 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
       if (hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
           hipMemcpyAsync(md, c.mdacc.p, sizeof(md), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
           hipMemcpyAsync(flag, c.mdflag.p, sizeof(flag), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+          (nose && hipMemcpyAsync(nv, c.nose.p, sizeof(nv), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
           (c.polar && hipMemcpyAsync(sf, c.sfacc.p, sizeof(sf), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
           hipStreamSynchronize(c.stream) != hipSuccess) {
         rc = fail(MDX_E_GPU, "mdx_md_steps: download");
@@ -571,6 +707,11 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
       out->virc = md[MD_VIRC] / hh;
       out->shake_iter_max = (int)flag[2];
       out->lambda = flag[0];
+      if (nose) {
+        out->logs = nv[0];
+        out->En_ext = nv[2];
+        out->dlogs = nv[3];
+      }
       if (c.polar) {
         out->frc.En_el += sf[SF_EEL];
         for (int k = 0; k < 6; k++) out->frc.Pvir[k] += sf[SF_PV + k];

@@ -79,6 +79,29 @@ class Engine:
             abi.check(self.lib, rc, "mdx_get_aspc")
         return k.value, kk.value, hist
 
+    def set_nose(self, logs=0.0, logs1=0.0, k=0, kk=0, hist=None):
+        """Nose-Hoover state: logs, logs1 = logs(t)-logs(t-h), predictor
+        length k, length reached kk, history [k][1+3ns] (None: zeros)"""
+        ptr = None
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, dtype=np.float64)
+            assert hist.shape == (k, 1 + 3 * self.ns)
+            ptr = hist.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.mdx_set_nose(self.ctx, float(logs), float(logs1), int(k), int(kk), ptr)
+        abi.check(self.lib, rc, "mdx_set_nose")
+
+    def get_nose(self):
+        """(logs, logs1, k, kk, hist [k][1+3ns])"""
+        a, b, k, kk = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        rc = self.lib.mdx_get_nose(self.ctx, C.byref(a), C.byref(b), C.byref(k), C.byref(kk), None)
+        abi.check(self.lib, rc, "mdx_get_nose")
+        hist = np.zeros((k.value, 1 + 3 * self.ns))
+        if k.value:
+            rc = self.lib.mdx_get_nose(self.ctx, None, None, None, None,
+                                       hist.ctypes.data_as(C.POINTER(C.c_double)))
+            abi.check(self.lib, rc, "mdx_get_nose")
+        return a.value, b.value, k.value, kk.value, hist
+
     def cook_cells(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         cid = np.empty(self.ns, np.int32)
@@ -387,6 +410,10 @@ def declare_extras(lib):
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
     lib.mdx_get_aspc.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_double)]
+    lib.mdx_set_nose.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.POINTER(C.c_double)]
+    lib.mdx_get_nose.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mdx_set_rdf.argtypes = [C.c_void_p, C.c_int, C.c_double]
     lib.mdx_get_rdf.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_long,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
