@@ -422,6 +422,17 @@ int mdx_setss(mdx_lj_t *ss, double EvdW_i, double RvdW_i, double EvdW_j,
 int mdx_setss_exp6(mdx_exp6_t *ss, double EvdW_i, double RvdW_i, double a_i,
                    double EvdW_j, double RvdW_j, double a_j, int comb_rule, double C2,
                    double *corr);
+/* The Buckingham form (cook built with sim/buck, POW = 0),
+   u = A_ij exp(-B_ij r) - C_ij/r^6, into the exp-6 table (tau = -B_ij,
+   eu = A_ij, ef = A_ij B_ij, ru = C_ij; initssaux() of
+   sim/buck/sitesite.c:178-188) with the combining rules of combrule()
+   (sim/buck/sitesite.c:113-170): comb_rule&3 for B (0 Waldman-Hagler,
+   1 geometric, 2 arithmetic, 3 additive 1/B), comb_rule&12 for A (0
+   Waldman-Hagler, 4 geometric, 8 Kong, 12 Busing), C geometric.  A in K,
+   B in 1/Angstrom, C in K Angstrom^6 (the .ble columns EvdW, RvdW,
+   parm[0] after the eunit conversion). */
+int mdx_setss_buck(mdx_exp6_t *ss, double A_i, double B_i, double C_i, double A_j,
+                   double B_j, double C_j, int comb_rule, double C2, double *corr);
 
 /* initerfc()+makesplines() with SPLINE=-2, SUBGRID=1 (sim/erfc.c:119-191,
    sim/splines.c:23-158): fills tab[0..*n) (caller provides room for

@@ -195,12 +195,15 @@ def read_ble(path, form=None):
     escale = float(head.get("eunit", EUNIT_KCAL)) / EUNIT_KCAL
     comb_rule = int(head.get("comb_rule", 0))
     if nparms == 1:
-        if form is None and polar >= 0 and re.search(r"^exp-6(\s|$)", text, re.M):
-            form = "exp6"
-        if form != "exp6":
-            raise CookInputError("nparms=1: name the site-site form (form='exp6'); the Busing and "
-                                 "Buckingham forms are not supported")
-        if not comb_rule & 1:
+        if form is None and polar >= 0:
+            if re.search(r"^exp-6(\s|$)", text, re.M):
+                form = "exp6"
+            elif re.search(r"^Buckingham(\s|$)", text, re.M):
+                form = "buck"
+        if form not in ("exp6", "buck"):
+            raise CookInputError("nparms=1: name the site-site form (form='exp6' or 'buck'); the "
+                                 "Busing form is not supported")
+        if form == "exp6" and not comb_rule & 1:
             raise CookInputError("exp-6: comb_rule&1 = 0 (Kirkwood) is not implemented "
                                  "(sim/exp6/sitesite.c:52)")
     elif nparms == 0:
@@ -214,7 +217,8 @@ def read_ble(path, form=None):
         raise CookInputError("nbfixes are not supported by the engine")
     n14 = 2 if distance14 else 1
     L = _Lines(text)
-    L.find("site-types" if polar < 0 else ("exp-6" if form == "exp6" else "Lennard-Jones"))
+    L.find("site-types" if polar < 0 else
+           {"exp6": "exp-6", "buck": "Buckingham"}.get(form, "Lennard-Jones"))
     types, mass, evdw, rvdw, alphas, pol, parm = [], [], [], [], [], {}, []
     for i in range(nsites):
         L.tok()                                   # site number
@@ -226,10 +230,12 @@ def read_ble(path, form=None):
             if t is None or t == "=":
                 break
             g = (float(t), float(L.tok()) * escale, float(L.tok()))
-            groups.append(g + tuple(float(L.tok()) for _ in range(nparms)))
+            # Buckingham: C_ii is an energy times A^6 (Epow 1, sim/buck/sitesite.c:16-24)
+            ps = escale if form == "buck" else 1.0
+            groups.append(g + tuple(float(L.tok()) * ps for _ in range(nparms)))
         alpha, E, R = groups[0][:3]
         parm.append(groups[0][3:])
-        if E > 0:
+        if E > 0 and form != "buck":
             E = -E                                # initcombrule(): EvdW is a minimum
         alphas.append(alpha)
         if polar == 1:
@@ -291,8 +297,8 @@ def read_ble(path, form=None):
                             site_xyz=site_xyz, site_excl=site_excl, bonds=bonds))
     ff = dict(types=types, type_mass=mass, type_EvdW_kcal=evdw, type_RvdW=rvdw,
               type_alpha=alphas, species=species, comb_rule=comb_rule)
-    if form == "exp6":
-        ff["form"] = "exp6"
+    if form in ("exp6", "buck"):
+        ff["form"] = form
         ff["type_parm"] = [p[0] for p in parm]
     if pol:
         ff["polar"] = pol

@@ -57,6 +57,7 @@ bool almost_integer(double x) {  // sim/ewald.c:240-250, machineprec=1e-9
 // at r = sig, a the stiffness
 struct Exp6 {
   double tau, eu, ef, ru, rf;
+  Exp6() = default;
   Exp6(double sig, double eps, double a) {
     tau = -(a / sig);
     eu = 6 * eps / (a - 6) * std::exp(a);
@@ -159,6 +160,55 @@ extern "C" int mdx_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, d
   ss->ef = v.ef;
   ss->ru = v.ru;
   return smooth_cutoff(v, C2, "mdx_setss_exp6", ss->C1q, ss->C2q, ss->A, ss->A4, corr);
+}
+
+extern "C" int mdx_setss_buck(mdx_exp6_t *ss, double Ai, double Bi, double Ci, double Aj,
+                              double Bj, double Cj, int comb_rule, double C2, double *corr) {
+  if (!ss) return mdx::fail(MDX_E_ARG, "mdx_setss_buck: ss is NULL");
+  std::memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (((comb_rule & 12) == 0) != ((comb_rule & 3) == 0))
+    return mdx::fail(MDX_E_ARG, "mdx_setss_buck: Waldman-Hagler must be set for both A and B or for "
+                                "none (sim/buck/sitesite.c:85-87)");
+  if (Ai < 0 || Aj < 0 || Bi < 0 || Bj < 0 || Ci < 0 || Cj < 0)
+    return mdx::fail(MDX_E_ARG, "mdx_setss_buck: A, B, C must be >= 0");
+  // combrule() of sim/buck/sitesite.c:113-170 (POW = 0)
+  double B = 0, A = 0, aux = 0;
+  const double Bi6 = 0.5 / p6(Bi), Bj6 = 0.5 / p6(Bj);
+  if ((comb_rule & 3) == 2 || Bi * Bj != 0) switch (comb_rule & 3) {
+    case 0: aux = Bi6 + Bj6; B = std::pow(aux, -1. / 6); break;
+    case 1: B = std::sqrt(Bi) * std::sqrt(Bj); break;
+    case 2: B = (Bi + Bj) / 2; break;
+    case 3: B = 1 / (0.5 / Bi + 0.5 / Bj); break;
+  }
+  if (Ai * Aj != 0) switch (comb_rule & 12) {
+    case 0: A = std::sqrt(Ai) / (Bi * Bi * Bi) * (std::sqrt(Aj) / (Bj * Bj * Bj)) / aux; break;
+    case 4: A = std::sqrt(Ai) * std::sqrt(Aj); break;
+    case 8: {                                     // Kong
+      const double qi = Bi / (Bi + Bj), x = (Aj * Bj) / (Ai * Bi);
+      A = (Ai * std::pow(x, qi) + Aj * std::pow(x, qi - 1)) / 2;
+      break;
+    }
+    case 12: {                                    // Busing
+      const double qi = (0.5 / Bi) / (0.5 / Bi + 0.5 / Bj);
+      A = std::pow(Ai * Bi, qi) * std::pow(Aj * Bj, 1 - qi) / B;
+      break;
+    }
+  }
+  const double C = std::sqrt(Ci) * std::sqrt(Cj);
+  if (A == 0 || C2 == 0) return 0;                // no site-site term: the table is zero
+  // initssaux(), sim/buck/sitesite.c:178-188: the exp-6 form's constants
+  Exp6 v;
+  v.tau = -B;
+  v.eu = A;
+  v.ef = A * B;
+  v.ru = C;
+  v.rf = 6 * C;
+  ss->tau = v.tau;
+  ss->eu = v.eu;
+  ss->ef = v.ef;
+  ss->ru = v.ru;
+  return smooth_cutoff(v, C2, "mdx_setss_buck", ss->C1q, ss->C2q, ss->A, ss->A4, corr);
 }
 
 extern "C" int mdx_erfc_size(int grid, double maxr, double alpha) {

@@ -56,6 +56,15 @@ class Provider:
             raise RuntimeError("setss_exp6 failed (%d)" % rc)
         return ss, corr.value
 
+    def setss_buck(self, Ai, Bi, Ci, Aj, Bj, Cj, comb_rule, C2):
+        ss = abi.Exp6()
+        corr = C.c_double(0)
+        rc = self._f("setss_buck")(C.byref(ss), Ai, Bi, Ci, Aj, Bj, Cj, comb_rule, C2,
+                                   C.byref(corr))
+        if rc:
+            raise RuntimeError("setss_buck failed (%d)" % rc)
+        return ss, corr.value
+
     def erfc_table(self, grid, maxr, cutoff, alpha, shift):
         nmax = int((maxr * alpha) ** 2 * grid) + 2
         tab = (abi.Erfc * nmax)()
@@ -121,15 +130,20 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
        stiffness a of each type in ff["type_parm"]; exp6_evdw_scale
        multiplies its EvdW after the kcal conversion (cook's exp-6
        initcombrule() converts kcal to K once more, sim/exp6/sitesite.c:
-       52-54, which system_from_cook reproduces)."""
+       52-54, which system_from_cook reproduces).  ff["form"] = "buck" is
+       cook's Buckingham form (sim/buck, POW = 0) in the exp-6 table: A_ii
+       [kcal/mol] in type_EvdW_kcal, B_ii [1/A] in type_RvdW, C_ii
+       [kcal/mol A^6] in type_parm, ff["comb_rule"] as in sim/buck
+       (default 5: geometric A and B)."""
     prov = provider or engine_provider()
     bs = BuiltSystem()
     S = bs.struct
     nt = len(ff["types"])
     form = ff.get("form", "lj")
-    if form not in ("lj", "exp6"):
-        raise ValueError("site-site form %r: lj or exp6" % form)
-    exp6 = form == "exp6"
+    if form not in ("lj", "exp6", "buck"):
+        raise ValueError("site-site form %r: lj, exp6 or buck" % form)
+    buck = form == "buck"
+    exp6 = form in ("exp6", "buck")       # buck fills the exp-6 table
     EvdW = [e * EUNIT_KCAL * (exp6_evdw_scale if exp6 else 1.0) for e in ff["type_EvdW_kcal"]]
     RvdW = list(ff["type_RvdW"])
     C2 = cutoff if lj_cutoff is None else lj_cutoff
@@ -145,7 +159,14 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
         rule = [int(ff["comb_rule"]) | (1 if t_alpha[i] <= 0 else 0) for i in range(nt)]
     for i in range(nt):
         for j in range(i + 1):
-            if EvdW[i] == 0 or EvdW[j] == 0:
+            if buck:
+                # A in the EvdW column, B in RvdW, C in type_parm (kcal/mol,
+                # 1/A, kcal/mol A^6; sim/buck/sitesite.c:75-81); one rule
+                cr = int(ff.get("comb_rule", 5))
+                Cp = [c * EUNIT_KCAL for c in ff["type_parm"]]
+                Ab = [e * EUNIT_KCAL for e in ff["type_EvdW_kcal"]]
+                ss, c = prov.setss_buck(Ab[i], RvdW[i], Cp[i], Ab[j], RvdW[j], Cp[j], cr, C2)
+            elif EvdW[i] == 0 or EvdW[j] == 0:
                 ss, c = (abi.Exp6 if exp6 else abi.LJ)(), 0.0
             else:
                 cr = rule[i] | rule[j]

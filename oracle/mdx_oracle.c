@@ -182,6 +182,77 @@ int oracle_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej
   return 0;
 }
 
+/* Buckingham (sim/buck, POW = 0): u = A_ij exp(-B_ij r) - C_ij/r^6, the
+   exp-6 macros with tau = -B_ij, eu = A_ij, ef = A_ij B_ij, ru = C_ij
+   (initssaux(), sim/buck/sitesite.c:178-188); the combining rules of
+   combrule() (sim/buck/sitesite.c:113-170); setss() as for exp-6 */
+int oracle_setss_buck(mdx_exp6_t *ss, double Ai, double Bi, double Ci, double Aj,
+                      double Bj, double Cj, int comb_rule, double C2, double *corr)
+{
+  const double neps = 1.1102230246251565e-16;
+  double A = 0, B = 0, aux = 0, C, C1, oldC1;
+  orc_e6 p;
+  int n;
+  memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (((comb_rule & 12) == 0) != ((comb_rule & 3) == 0)) return MDX_E_ARG;
+  if (Ai < 0 || Aj < 0 || Bi < 0 || Bj < 0 || Ci < 0 || Cj < 0) return MDX_E_ARG;
+  if ((comb_rule & 3) == 2 || Bi * Bj != 0) switch (comb_rule & 3) {
+    case 0: aux = 0.5 / pow6(Bi) + 0.5 / pow6(Bj); B = pow(aux, -1. / 6); break;
+    case 1: B = sqrt(Bi) * sqrt(Bj); break;
+    case 2: B = (Bi + Bj) / 2; break;
+    case 3: B = 1 / (0.5 / Bi + 0.5 / Bj); break;
+  }
+  if (Ai * Aj != 0) switch (comb_rule & 12) {
+    case 0: A = sqrt(Ai) / (Bi * Bi * Bi) * (sqrt(Aj) / (Bj * Bj * Bj)) / aux; break;
+    case 4: A = sqrt(Ai) * sqrt(Aj); break;
+    case 8: {
+      double qi = Bi / (Bi + Bj), x = Aj * Bj / (Ai * Bi);
+      A = (Ai * pow(x, qi) + Aj * pow(x, qi - 1)) / 2;
+      break;
+    }
+    case 12: {
+      double qi = (0.5 / Bi) / (0.5 / Bi + 0.5 / Bj);
+      A = pow(Ai * Bi, qi) * pow(Aj * Bj, 1 - qi) / B;
+      break;
+    }
+  }
+  C = sqrt(Ci) * sqrt(Cj);
+  if (A == 0 || C2 == 0) return 0;
+  p.tau = -B; p.eu = A; p.ef = A * B; p.ru = C; p.rf = 6 * C;
+  ss->tau = p.tau; ss->eu = p.eu; ss->ef = p.ef; ss->ru = p.ru;
+  C1 = 0.775 * C2;
+  n = 0;
+  do {
+    double den = orc_e6F(&p, C1);
+    if (fabs(den) < 1e-50) return MDX_E_ARG;
+    oldC1 = C1;
+    C1 = 1 + 4 * orc_e6U(&p, C1) / C1 / den;
+    if (n++ > 2000 || C1 < 0) return MDX_E_ARG;
+    C1 = C2 / sqrt(C1);
+  } while (fabs(1 - C1 / oldC1) > neps * n);
+  if (C1 > 0.999 * C2 || C1 < 0.5 * C2) return MDX_E_ARG;
+  ss->A = orc_e6U(&p, C1) / sqr(C2 * C2 - C1 * C1);
+  ss->A4 = -orc_e6F(&p, C1) / (C2 * C2 - C1 * C1) / C1;
+  ss->C2q = C2 * C2;
+  ss->C1q = C1 * C1;
+  if (corr) {
+    const int NG = 64;
+    const double g4 = 0.2113248654051871;
+    double c0 = 4 * PI * ss->A *
+      (C1 * (pow6(C1) / 7 - 0.4 * sqr(C1) * sqr(C1) * sqr(C2) + sqr(C1) * sqr(C2) * sqr(C2) / 3)
+       - 8.0 / 105 * pow(C2, 7));
+    double h = 1.0 / NG / C1, c1 = 0, r;
+    int k;
+    for (k = 0; k < NG; k++) {
+      r = 1 / ((k + g4) * h); c1 += sqr(r * r) * orc_e6U(&p, r);
+      r = 1 / ((k + (1 - g4)) * h); c1 += sqr(r * r) * orc_e6U(&p, r);
+    }
+    *corr = c0 + c1 * 2 * PI * h;
+  }
+  return 0;
+}
+
 /* --------------------------------------------------------- erfc splines */
 
 int oracle_erfc_table(mdx_erfc_t *tab, int *nout, double *sgridout, int grid,

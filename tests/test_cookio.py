@@ -201,3 +201,23 @@ def test_eunit_of_the_file(tmp_path):
     open(ble, "w").write("\n".join(lines) + "\n")
     got = cookio.read_ble(ble)
     np.testing.assert_allclose(got["type_EvdW_kcal"], ref["type_EvdW_kcal"], rtol=1e-14)
+
+
+def test_buckingham_force_field_file(tmp_path):
+    """the Buckingham table: A in the EvdW column (positive, not flipped),
+    B in RvdW, C in the parameter column; energies scaled by eunit"""
+    a_of = {"H": 0.0, "O": 1.2e6}
+    ble, species, types = exp6_ble(tmp_path, a_of, table="Buckingham", comb_rule=5)
+    txt = open(ble).read()
+    lines = txt.splitlines()
+    k = lines.index("Buckingham")
+    for i in range(k + 2, k + 4):
+        f = lines[i].split()
+        f[4] = "%.6f" % abs(float(f[4]) * 1e5)           # A_ii > 0
+        f[5] = "3.5"                                      # B_ii
+        lines[i] = "  " + "  ".join(f)
+    open(ble, "w").write("\n".join(lines) + "\n")
+    ff = cookio.read_ble(ble)
+    assert ff["form"] == "buck" and ff["comb_rule"] == 5
+    assert ff["type_EvdW_kcal"][1] > 0 and ff["type_RvdW"][1] == 3.5
+    assert ff["type_parm"] == [0.0, 1.2e6]

@@ -112,3 +112,32 @@ def test_lj_only_entry_points_refuse_exp6(brine):
     e.close()
     with pytest.raises(RuntimeError, match="exp-6"):
         Batch([bs], abi.MDX_FP64)
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_buckingham_forces_vs_oracle(brine, mode):
+    """the Buckingham form on the exp-6 kernels: each brine site as the
+    Buckingham site of its exp-6 parameters (A, B, C of the identical-site
+    mapping), geometric rules (comb_rule 5)"""
+    import math
+    d, meta, _, _, _, _ = brine
+    ff = dict(meta["forcefield"], form="buck", comb_rule=5)
+    A, B, Cc = [], [], []
+    for t, E, R in zip(ff["types"], ff["type_EvdW_kcal"], ff["type_RvdW"]):
+        a, sig, eps = workloads.EXP6_A[t], 2 * R, -E
+        A.append(6 * eps * math.exp(a) / (a - 6))
+        B.append(a / sig)
+        Cc.append(eps * a * sig ** 6 / (a - 6))
+    ff.update(type_EvdW_kcal=A, type_RvdW=B, type_parm=Cc)
+    bs = oracle.fixture_system(dict(meta, forcefield=ff), provider_=engine_provider())
+    bso = oracle.fixture_system(dict(meta, forcefield=ff))
+    f_ref, o_ref = oracle.forces(bso, d["r"], 1)
+    e = Engine(bs, mode)
+    f, out = e.forces(d["r"], 1)
+    e.close()
+    err = rel_rms(f, f_ref)
+    print("Buckingham brine506 mode %d: rel RMS force err %.3e" % (mode, err), flush=True)
+    assert err < TOL[mode]
+    assert out.npair == o_ref.npair and out.nlj == o_ref.nlj > 0
+    Etot = o_ref.En_pot + o_ref.En_el
+    assert abs(out.En_pot - o_ref.En_pot) < ETOL[mode] * abs(Etot)
