@@ -40,6 +40,11 @@ def parse():
                     help="c2: 194,304-site SPC/E+NaCl; c1: 1000 SPC/E; c3: Drude-polarizable "
                          "brine, 97,200 sites + 32,600 Drudes, ASPC k=4 with one SCF iteration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", choices=["lj", "exp6"], default="lj",
+                    help="site-site form of the c2 brine (exp6: the exp-6 brine of the same minima)")
+    ap.add_argument("--rdf", type=int, default=0,
+                    help="RDF grid: histograms on, and the cost of a measured force call with "
+                         "and without them reported")
     ap.add_argument("--kstride", type=int, default=0,
                     help="CPU baseline: k-space over every n-th charge (0 = auto)")
     ap.add_argument("--pin", default="",
@@ -250,6 +255,10 @@ def main():
     from macsimus_amd.system import build_system
 
     wl = {"c2": workloads.brine_c2, "c1": workloads.water_c1, "c3": workloads.polar_c3}[a.workload]()
+    if a.form == "exp6":
+        if a.workload != "c2":
+            sys.exit("bench.py: --form exp6 is for the c2 brine")
+        wl = workloads.brine_c2_exp6()
     polar = bool(wl.get("polar"))
     # Drude systems: the configuration has the Drude offsets after the sites
     wl["r_full"] = np.concatenate([wl["r"], np.zeros_like(wl["r"])]) if polar else wl["r"]
@@ -291,6 +300,23 @@ def main():
     r1 = np.zeros_like(wl["r_full"])
     # census of the starting configuration (pairs inside the cutoff)
     f0, out0 = eng.forces(wl["r_full"], measure=1)
+    rdf_cost = None
+    if a.rdf:
+        # a measured force call without and with the RDF pass (host clock
+        # around synchronous calls, 3 each after one warm call)
+        def timed(n=3):
+            eng.forces(wl["r_full"], measure=1)
+            t = time.perf_counter()
+            for _ in range(n):
+                eng.forces(wl["r_full"], measure=1)
+            return (time.perf_counter() - t) * 1e3 / n
+        t_off = timed()
+        eng.set_rdf(a.rdf, wl["cutoff"])
+        t_on = timed()
+        h_ = eng.rdf()
+        rdf_cost = {"grid": a.rdf, "records": int(h_["hist"].shape[0]), "bins": int(h_["hist"].shape[1]),
+                    "ms_measured_call": t_off, "ms_measured_call_with_rdf": t_on,
+                    "pairs_counted_per_call": int(h_["hist"].sum()) // max(h_["nmeas"], 1)}
     eng.upload_state(wl["r_full"], r1)
     if a.warmup:
         p.measure_every = a.warmup
@@ -365,6 +391,10 @@ def main():
         "cpu_baseline": cpu,
         "energy_K": {"En_pot": out.frc.En_pot + out.frc.En_el, "En_kin": out.En_kin},
     }
+    if a.form != "lj":
+        line["config"]["site_site_form"] = a.form
+    if rdf_cost:
+        line["rdf"] = rdf_cost
     print(json.dumps(line), flush=True)
 
 
