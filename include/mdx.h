@@ -105,8 +105,8 @@ typedef struct mdx_system_s {
   const double *cons_bondq;  /* [ncons] squared bond lengths */
   const double *mol_omega;   /* [nmol] SHAKE relaxation omega (= omegac/2) */
   /* site-site form: NULL = Lennard-Jones (sstab); else the exp-6 table
-     [ntypes*ntypes] (sstab is then not read).  mdx_create only; the
-     batch takes Lennard-Jones systems. */
+     [ntypes*ntypes] (sstab is then not read).  A batch takes one form
+     for all its systems. */
   const mdx_exp6_t *sstab_exp6;
 } mdx_system_t;
 

@@ -60,6 +60,7 @@ struct BK {
 
 struct Batch {
   int mode = MDX_MIXED, nsys = 0, npad = 0, nreal = 0, ncell = 0, device = 0;
+  bool exp6 = false;                     // site-site form of every system
   hipStream_t stream = nullptr;
   std::vector<DSys> hs;
   std::vector<double> sumq2, kappa;

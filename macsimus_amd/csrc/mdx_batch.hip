@@ -1,5 +1,6 @@
 // Batched replicas: many independent atomic systems (one site per
-// molecule: Lennard-Jones with cook's smooth cutoff, point charges, Ewald,
+// molecule: Lennard-Jones or exp-6 with cook's smooth cutoff, one form for
+// the batch, point charges, Ewald,
 // leap-frog with Berendsen) of uneven size, run by one launch per kernel
 // for all systems, each with its own box, cell grid, spline table, LJ table,
 // k-vector set and thermostat; the unmeasured step is captured as one
@@ -189,8 +190,9 @@ template <>
 __device__ inline double brcp<double>(double x) { return 1.0 / x; }
 
 // one lane per sorted site over the (2w+1)^3 cells of its system's grid;
-// forces to the site's own row (no atomics), per-site sums halved later
-template <typename R, bool MEASURE>
+// forces to the site's own row (no atomics), per-site sums halved later;
+// E6: the exp-6 table [tau eu ef ru C1q C2q A A4] (sim/exp6/sitesite.h:59-66)
+template <typename R, bool MEASURE, bool E6>
 __global__ __launch_bounds__(kBlock) void k_bpair(
     int nreal, const DSys *__restrict__ sy, const double4 *__restrict__ spos,
     const int4 *__restrict__ sinfo, const int *__restrict__ scell, const int *__restrict__ cstart,
@@ -232,13 +234,19 @@ __global__ __launch_bounds__(kBlock) void k_bpair(
           R fr = 0, u = 0;
           if (rr < (double)ss[5]) {
             if (rr < (double)ss[4]) {
-              const R x = ss[0] * brcp<R>((R)rr), y = x * x * x;
-              if (MEASURE) {
-                const R z = y * y;
-                u = ss[1] * (z - y);
-                fr = ss[2] * (z + z - y) * x;
+              if (E6) {
+                const R q = (R)rr, x = sqrt(q), y = exp(ss[0] * x), iz = brcp<R>(q * q * q);
+                if (MEASURE) u = ss[1] * y - ss[3] * iz;
+                fr = ss[2] * y * brcp<R>(x) - (R)6 * ss[3] * iz * brcp<R>(q);
               } else {
-                fr = ss[3] * y * (y - (R)0.5) * x;
+                const R x = ss[0] * brcp<R>((R)rr), y = x * x * x;
+                if (MEASURE) {
+                  const R z = y * y;
+                  u = ss[1] * (z - y);
+                  fr = ss[2] * (z + z - y) * x;
+                } else {
+                  fr = ss[3] * y * (y - (R)0.5) * x;
+                }
               }
             } else {
               const R x = (R)(rr - (double)ss[5]);
@@ -381,6 +389,17 @@ __global__ void k_bscale(int npad, const int *__restrict__ sysid, const double *
   for (int d = 0; d < 3; d++) r1[3 * i + d] *= l;
 }
 
+template <typename R, bool E6>
+void launch_bpair(Batch &b, int measure, const DSys *sy, const double4 *sp, const int4 *si,
+                  const R *lj, const R *erf) {
+  if (measure)
+    k_bpair<R, true, E6><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
+        b.nreal, sy, sp, si, b.scell.p, b.cstart.p, lj, erf, b.f.p, b.epart.p);
+  else
+    k_bpair<R, false, E6><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
+        b.nreal, sy, sp, si, b.scell.p, b.cstart.p, lj, erf, b.f.p, b.epart.p);
+}
+
 // ---------------------------------------------------------------- host
 int forces_launch(Batch &b, int measure) {
   const DSys *sy = (const DSys *)b.dsys.p;
@@ -402,19 +421,11 @@ int forces_launch(Batch &b, int measure) {
   const int4 *si = (const int4 *)b.sinfo.p;
   if (b.timing) { b.tm.stop(T_BIN, b.stream); b.tm.start(T_PAIR, b.stream); }
   if (b.mode == MDX_FP64) {
-    if (measure)
-      k_bpair<double, true><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
-          b.nreal, sy, sp, si, b.scell.p, b.cstart.p, b.lj64.p, b.erf64.p, b.f.p, b.epart.p);
-    else
-      k_bpair<double, false><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
-          b.nreal, sy, sp, si, b.scell.p, b.cstart.p, b.lj64.p, b.erf64.p, b.f.p, b.epart.p);
+    if (b.exp6) launch_bpair<double, true>(b, measure, sy, sp, si, b.lj64.p, b.erf64.p);
+    else launch_bpair<double, false>(b, measure, sy, sp, si, b.lj64.p, b.erf64.p);
   } else {
-    if (measure)
-      k_bpair<float, true><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
-          b.nreal, sy, sp, si, b.scell.p, b.cstart.p, b.lj32.p, b.erf32.p, b.f.p, b.epart.p);
-    else
-      k_bpair<float, false><<<nblk(b.nreal), kBlock, 0, b.stream>>>(
-          b.nreal, sy, sp, si, b.scell.p, b.cstart.p, b.lj32.p, b.erf32.p, b.f.p, b.epart.p);
+    if (b.exp6) launch_bpair<float, true>(b, measure, sy, sp, si, b.lj32.p, b.erf32.p);
+    else launch_bpair<float, false>(b, measure, sy, sp, si, b.lj32.p, b.erf32.p);
   }
   k_bsum_sites<<<b.nsys, kBlock, 0, b.stream>>>(sy, b.epart.p, b.sacc.p);
   if (b.timing) b.tm.stop(T_PAIR, b.stream);
@@ -518,11 +529,11 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   b.kappa.assign(nsys, 0.0);
   for (int s = 0; s < nsys; s++) {
     const mdx_system_t &y = sys[s];
-    if (y.sstab_exp6)
-      return bail(fail(MDX_E_ARG, "mdx_batch_create: the batch takes the LJ form only (system " +
-                                      std::to_string(s) + " has the exp-6 table)"));
+    if ((y.sstab_exp6 != nullptr) != (sys[0].sstab_exp6 != nullptr))
+      return bail(fail(MDX_E_ARG, "mdx_batch_create: all systems need the same site-site form "
+                                  "(system " + std::to_string(s) + " differs from system 0)"));
     if (y.ns <= 0 || y.nmol != y.ns || y.chargepol || y.ncons != 0 || !y.mass || !y.charge ||
-        !y.site_type || !y.sstab || !y.erfc_tab || y.erfc_n <= 0)
+        !y.site_type || !(y.sstab || y.sstab_exp6) || !y.erfc_tab || y.erfc_n <= 0)
       return bail(fail(MDX_E_ARG, "mdx_batch_create: systems must be atomic (one site per molecule, "
                                   "no constraints, no Drudes) with masses, charges and tables"));
     if (y.L[0] != y.L[1] || y.L[0] != y.L[2]) return bail(fail(MDX_E_ARG, "mdx_batch_create: cubic boxes only"));
@@ -537,7 +548,9 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
     S.cdip = 4 * kPi / (2 * y.epsinf + 1) / (S.L * S.L * S.L);
     S.nt = y.ntypes;
     S.lj_off = (int)lj.size();
-    std::vector<double> t = lj_table(y.sstab, y.ntypes * y.ntypes);
+    b.exp6 = y.sstab_exp6 != nullptr;
+    std::vector<double> t = b.exp6 ? exp6_table(y.sstab_exp6, y.ntypes * y.ntypes)
+                                   : lj_table(y.sstab, y.ntypes * y.ntypes);
     lj.insert(lj.end(), t.begin(), t.end());
     // spline intervals up to the cutoff (index (int)(r^2 sgrid), r^2 <
     // cutoff^2); none for a system without charges

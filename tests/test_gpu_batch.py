@@ -268,3 +268,28 @@ def test_batch_full_256_systems():
         assert np.isfinite(o.frc.En_pot) and np.isfinite(o.En_kin)
         assert 0.9 < o.lambda_ < 1.1, (k, o.lambda_)
     bm.close()
+
+
+@pytest.mark.parametrize("mode,tol", [(abi.MDX_FP64, 1e-10), (abi.MDX_MIXED, 1e-5)])
+def test_batch_exp6_matches_oracle(mode, tol):
+    """replicas with the exp-6 form (a = 14 for every type, the LJ minima)
+    against the oracle; a batch mixing forms is refused"""
+    base = systems(6)
+    prov = engine_provider()
+    sy = []
+    for sd, _ in base:
+        ff = dict(sd["ff"], form="exp6", type_parm=[14.0] * len(sd["ff"]["types"]))
+        sd6 = dict(sd, ff=ff)
+        sy.append((sd6, workloads.build_replica(sd6, provider=prov),
+                   workloads.build_replica(sd6, provider=oracle.provider())))
+    b = Batch([bs for _, bs, _ in sy], mode)
+    fs, outs = b.forces([sd["r"] for sd, _, _ in sy], measure=1)
+    for (sd, bs, bso), f, o in zip(sy, fs, outs):
+        fo, oo = oracle.forces(bso, sd["r"], 1)
+        err = rel_rms(f, fo)
+        print("exp-6 %s %s rel rms %.2e" % (sd["name"], sd["kind"], err), flush=True)
+        assert err < tol, (sd["name"], err)
+        check_out(o, oo, tol, sd["name"], exact_nlj=(mode == abi.MDX_FP64))
+    b.close()
+    with pytest.raises(RuntimeError, match="same site-site form"):
+        Batch([sy[0][1], base[1][1]], mode)
