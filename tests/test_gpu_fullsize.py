@@ -134,3 +134,50 @@ def test_polar_c3_md_steps(c3):
     assert d < 1e-6
     assert abs(s32.En_self - s64.En_self) < 1e-5 * abs(s64.En_self)
     assert s64.shake_iter_max > 0
+
+
+def _vs_oracle(bs, bso, r, label, polar=False):
+    import oracle
+    fo, oo = oracle.forces(bso, r, 1, nthreads=16)
+    for mode, tol, etol in ((abi.MDX_FP64, 1e-10, 1e-10), (abi.MDX_MIXED, 1e-5, 2e-6)):
+        e = Engine(bs, mode)
+        f, out = e.forces(r, 1)
+        e.close()
+        err = rel(f, fo)
+        Etot = oo.En_pot + oo.En_el
+        dE = out.En_pot + out.En_el - Etot
+        print("%s mode %d vs oracle: rel RMS force err %.3e, Epot rel %.2e, pairs %d" % (
+            label, mode, err, dE / Etot, out.npair), flush=True)
+        assert err < tol
+        assert out.npair == oo.npair
+        if mode == abi.MDX_FP64 and not polar:
+            assert out.nlj == oo.nlj
+        assert abs(dE) < etol * abs(Etot)
+        scale = max(abs(oo.Pvir[k]) for k in range(3))
+        for i in range(6):
+            assert abs(out.Pvir[i] - oo.Pvir[i]) < 10 * etol * scale
+
+
+def test_full_size_brine_against_the_oracle(c2):
+    """one force call of the 194,304-site brine, both precisions, against
+    the oracle (its FP64 CPU restatement, 16 threads) on the same input:
+    forces, census, potential energy and tensor"""
+    import oracle
+    wl, bs, _ = c2
+    r = np.mod(wl["r"], wl["L"])
+    bso = build_system(wl["ff"], wl["counts"], [wl["L"]] * 3, wl["cutoff"], epsr=0.05, epsk=0.05,
+                       provider=oracle.provider())
+    _vs_oracle(bs, bso, r, "C2")
+
+
+def test_full_size_drude_brine_against_the_oracle(c3):
+    """the same for the Drude brine (97,200 sites + 32,600 Drudes) in its
+    SCF state: the four charge terms of every pair at full size"""
+    import oracle
+    wl, bs, r = c3
+    ns = bs.struct.ns
+    r = r.copy()
+    r[:ns] = np.mod(r[:ns], wl["L"])
+    bso = build_system(wl["ff"], wl["counts"], [wl["L"]] * 3, wl["cutoff"], epsr=0.05, epsk=0.05,
+                       provider=oracle.provider())
+    _vs_oracle(bs, bso, r, "C3", polar=True)
