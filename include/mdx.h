@@ -157,7 +157,10 @@ typedef struct mdx_step_s {
   int thermostat;      /* 0 = none (NVE), 1 = Berendsen (sim/thermo.c:141-160),
                           2 = Nose-Hoover with the time-reversible velocity
                           predictor (sim/trvpscale.c, sim/verlet.c:22-36,
-                          sim/thermo.c:180-200); not with Drudes */
+                          sim/thermo.c:180-200); not with Drudes;
+                          11, 12, 13 = the decoupled friction thermostats
+                          of molecular translations, of the intramolecular
+                          motion, of both (sim/thermo.c:107-135) */
   double T;            /* target temperature [K] */
   double tau;          /* thermostat time constant tau.T [ps] */
   double nf;           /* No.f degrees of freedom */
@@ -172,6 +175,10 @@ typedef struct mdx_step_s {
   /* Nose-Hoover: length of the velocity predictor (cook -p option % 10,
      0..9); the predictor grows to it over the first steps (sim/trvpscale.c) */
   int trvp_k;
+  /* friction thermostats 11-13: No.f_tr, the translational degrees of
+     freedom (No.f_in = nf - nf_tr), and T_tr_in = T_tr/T_in of 13 */
+  double nf_tr;
+  double T_tr_in;
 } mdx_step_t;
 
 typedef struct mdx_step_out_s {
@@ -188,6 +195,7 @@ typedef struct mdx_step_out_s {
   /* Nose-Hoover, last measured step: En.ext (VERLET=3 form), logs after
      the step and En.dlogs (sim/thermo.c:180-200) */
   double En_ext, logs, dlogs;
+  double En_kin_tr;    /* kinetic energy of the molecular translations (En.kin_tr/2) */
 } mdx_step_out_t;
 
 /* Replaces one selffield() iteration (sim/scf.c:12-240, POLAR=0):

@@ -76,7 +76,7 @@ class Step(C.Structure):
         ("thermostat", C.c_int), ("T", C.c_double), ("tau", C.c_double),
         ("nf", C.c_double), ("measure_every", C.c_int),
         ("aspc_k", C.c_int), ("scf_iter", C.c_int), ("scf_omega", C.c_double),
-        ("trvp_k", C.c_int),
+        ("trvp_k", C.c_int), ("nf_tr", C.c_double), ("T_tr_in", C.c_double),
     ]
 
 
@@ -87,6 +87,7 @@ class StepOut(C.Structure):
         ("virc", C.c_double), ("PTvirc", C.c_double * 6),
         ("lambda_", C.c_double), ("shake_iter_max", C.c_int),
         ("En_ext", C.c_double), ("logs", C.c_double), ("dlogs", C.c_double),
+        ("En_kin_tr", C.c_double),
     ]
 
 

@@ -26,7 +26,8 @@ __global__ void k_fill(long n, double *__restrict__ x, double v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;
 }
-enum { MD_EKIN = 0, MD_PKIN = 1, MD_VIRC = 7, MD_PTV = 8, MD_ITMAX = 14, MD_FAIL = 15, MD_N = 16 };
+enum { MD_EKIN = 0, MD_PKIN = 1, MD_VIRC = 7, MD_PTV = 8, MD_KTR = 14, MD_ITMAX = 15, MD_FAIL = 16,
+       MD_N = 17 };
 
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_shake(
     const double *__restrict__ nose, double *__restrict__ hist, long stride, Trvp tv) {
   const int m = blockIdx.x * kBlock + threadIdx.x;
   double ek = 0, pk[6] = {0, 0, 0, 0, 0, 0}, virc = 0, ptv[6] = {0, 0, 0, 0, 0, 0};
-  double its = 0, bad = 0;
+  double its = 0, bad = 0, ktr = 0;
   if (m < nmol) {
     const double hh = h * h;
     const int b = molfirst[m], e = molfirst[m + 1];
@@ -130,10 +131,13 @@ __global__ __launch_bounds__(kBlock) void k_shake(
       if (maxsq > eps) bad = 1;
     }
     its = it;
+    double vm1[3] = {0, 0, 0}, vm2[3] = {0, 0, 0}, mm = 0;
     for (int i = b; i < e; i++) {
       double V1[3], V2[3];
       for (int k = 0; k < 3; k++) { V1[k] = r1[3 * i + k]; V2[k] = p[3 * i + k] - r[3 * i + k]; }
       const double mi = mass[i];
+      for (int k = 0; k < 3; k++) { vm1[k] += mi * V1[k]; vm2[k] += mi * V2[k]; }
+      mm += mi;
       ek += (V1[0] * V1[0] + V1[1] * V1[1] + V1[2] * V1[2] + V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]) * mi;
       pk[0] += (V1[0] * V1[0] + V2[0] * V2[0]) * mi;
       pk[1] += (V1[1] * V1[1] + V2[1] * V2[1]) * mi;
@@ -143,11 +147,14 @@ __global__ __launch_bounds__(kBlock) void k_shake(
       pk[5] += (V1[0] * V1[1] + V2[0] * V2[1]) * mi;
       for (int k = 0; k < 3; k++) { r1[3 * i + k] = V2[k]; r[3 * i + k] = p[3 * i + k]; }
     }
+    // translational kinetic energy of the molecule (sim/shakev3.c:35)
+    ktr = (vm1[0] * vm1[0] + vm1[1] * vm1[1] + vm1[2] * vm1[2] +
+           vm2[0] * vm2[0] + vm2[1] * vm2[1] + vm2[2] * vm2[2]) / mm;
   }
   __shared__ double red[kBlock / 64][MD_N];
   const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double v[MD_N] = {ek, pk[0], pk[1], pk[2], pk[3], pk[4], pk[5], virc,
-                    ptv[0], ptv[1], ptv[2], ptv[3], ptv[4], ptv[5], its, bad};
+                    ptv[0], ptv[1], ptv[2], ptv[3], ptv[4], ptv[5], ktr, its, bad};
   for (int k = 0; k < MD_N; k++) {
     const double s = k >= MD_ITMAX ? wave_max(v[k]) : wave_sum(v[k]);
     if (ln == 0) red[wv][k] = s;
@@ -218,6 +225,46 @@ __global__ void k_nose(const double *acc, double *nose, double *hist, long strid
   nose[0] = logs0 + nose[1];
   nose[3] = (ld + nose[1]) / (2 * h);
   nose[2] = (logs0 + (tau / h) * (tau / h) * (ld * ld + nose[1] * nose[1]) / 4) * nf * T;
+}
+
+// the decoupled friction thermostats after SHAKE (sim/thermo.c:107-135),
+// one thread per molecule: ze acts on the molecule's translation (kind
+// 11), z on the intramolecular motion (12), both with their own
+// temperatures from T_tr_in (13); r1 -= z r1 + (ze - z) v_mol with the
+// linear factors blogT(T_kin/T) h/tau of the doubled kinetic energies
+__device__ inline double blogT(double E, double T) {
+  const double e2 = 7.38905609893065022723;
+  return E > T * e2 ? 1 : (E * e2 < T ? -1 : log(E / T) / 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_friction(int nmol, const int *__restrict__ molfirst,
+                                                     const double *__restrict__ mass,
+                                                     double *__restrict__ r1, const double *acc,
+                                                     double h, double T, double tau, double nf,
+                                                     double nf_tr, double T_tr_in, int kind) {
+  const int m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= nmol) return;
+  const double hs = 0.5 / (h * h);
+  const double kin = acc[MD_EKIN] * hs, ktr = acc[MD_KTR] * hs, f_in = nf - nf_tr;
+  double z = 0, ze = 0;
+  if (kind == 11) ze = blogT(ktr / nf_tr, T) * (h / tau);
+  if (kind == 12) z = blogT((kin - ktr) / f_in, T) * (h / tau);
+  if (kind == 13) {
+    const double T_tr = (f_in + nf_tr) * T / (T_tr_in * f_in + nf_tr);
+    const double T_in = (f_in + nf_tr) * T / (f_in + nf_tr / T_tr_in);
+    z = blogT((kin - ktr) / f_in, T_in) * (h / tau);
+    ze = blogT(ktr / nf_tr, T_tr) * (h / tau);
+  }
+  ze -= z;
+  const int b = molfirst[m], e = molfirst[m + 1];
+  double vm[3] = {0, 0, 0}, mm = 0;
+  for (int i = b; i < e; i++) {
+    for (int k = 0; k < 3; k++) vm[k] += mass[i] * r1[3 * i + k];
+    mm += mass[i];
+  }
+  for (int k = 0; k < 3; k++) vm[k] /= mm;
+  for (int i = b; i < e; i++)
+    for (int k = 0; k < 3; k++) r1[3 * i + k] -= z * r1[3 * i + k] + ze * vm[k];
 }
 
 // ------------------------------------------------------------------ Drude
@@ -587,6 +634,10 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) 
   if (nose)
     k_nose<<<1, 64, 0, c.stream>>>(c.mdacc.p, c.nose.p, c.trvp_hist.p, stride, tv.knew, p->h,
                                    p->T, p->tau, p->nf);
+  if (p->thermostat >= 11)
+    k_friction<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.mass.p, c.r1.p, c.mdacc.p,
+                                            p->h, p->T, p->tau, p->nf, p->nf_tr, p->T_tr_in,
+                                            p->thermostat);
   if (hipGetLastError() != hipSuccess) return fail(MDX_E_GPU, "mdx_md_steps: launch");
   return 0;
 }
@@ -595,7 +646,8 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) 
 bool same_step(const mdx_step_t &a, const mdx_step_t &b) {
   return a.h == b.h && a.eps == b.eps && a.maxit == b.maxit && a.thermostat == b.thermostat &&
          a.T == b.T && a.tau == b.tau && a.nf == b.nf && a.aspc_k == b.aspc_k &&
-         a.scf_iter == b.scf_iter && a.scf_omega == b.scf_omega && a.trvp_k == b.trvp_k;
+         a.scf_iter == b.scf_iter && a.scf_omega == b.scf_omega && a.trvp_k == b.trvp_k &&
+         a.nf_tr == b.nf_tr && a.T_tr_in == b.T_tr_in;
 }
 
 // an unmeasured step as a hipGraph, captured once per step parameters and
@@ -645,8 +697,14 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
   Ctx &c = h->c;
   if (!c.state_on_device) return fail(MDX_E_ARG, "mdx_md_steps: upload the state first");
   if (!(p->h > 0) || !(p->eps > 0)) return fail(MDX_E_ARG, "mdx_md_steps: h and eps must be > 0");
-  if (p->thermostat < 0 || p->thermostat > 2)
-    return fail(MDX_E_ARG, "mdx_md_steps: thermostat 0 (none), 1 (Berendsen) or 2 (Nose-Hoover)");
+  const int th = p->thermostat;
+  if (!(th >= 0 && th <= 2) && !(th >= 11 && th <= 13))
+    return fail(MDX_E_ARG, "mdx_md_steps: thermostat 0 (none), 1 (Berendsen), 2 (Nose-Hoover) "
+                           "or 11-13 (friction on translations, intramolecular, both)");
+  if (th >= 11 && (c.polar || !(p->nf_tr > 0 && p->nf - p->nf_tr > 0) ||
+                   (th == 13 && !(p->T_tr_in > 0))))
+    return fail(MDX_E_ARG, "mdx_md_steps: the friction thermostats need 0 < nf_tr < nf (and "
+                           "T_tr_in > 0 for 13), not for Drude systems");
   if (p->thermostat && !(p->tau > 0 && p->nf > 0 && p->T > 0))
     return fail(MDX_E_ARG, "mdx_md_steps: the thermostat needs T, tau, nf > 0");
   if (p->thermostat == 2 && c.polar)
@@ -700,6 +758,7 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
       const double hh = p->h * p->h, hscale = 0.5 / hh;
       fill_out(c, acc, 1, &out->frc);
       out->En_kin = md[MD_EKIN] * hscale / 2;
+      out->En_kin_tr = md[MD_KTR] * hscale / 2;
       for (int k = 0; k < 6; k++) {
         out->Pkin[k] = md[MD_PKIN + k] * hscale;
         out->PTvirc[k] = md[MD_PTV + k] / hh;

@@ -1284,6 +1284,15 @@ int oracle_md_step_polar(const mdx_system_t *S, double *r, double *r1, double *f
   return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0);
 }
 
+/* blogT(): ln(E/T)/2 clamped to [-1, 1] (sim/constrd.c:98-107) */
+static double orc_blogT(double E, double T)
+{
+  const double exp2 = 7.38905609893065022723;
+  if (E > T * exp2) return 1;
+  if (E * exp2 < T) return -1;
+  return log(E / T) / 2;
+}
+
 static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
                           const mdx_step_t *p, int measure, mdx_step_out_t *out,
                           const double *vpred, double vfac)
@@ -1291,6 +1300,7 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
   int ns = S->ns, n, i, k, al;
   double hh = p->h * p->h, hscale = 0.5 / hh;
   double Ekin = 0, Pkin[6] = {0, 0, 0, 0, 0, 0}, virc = 0, PTvirc[6] = {0, 0, 0, 0, 0, 0};
+  double Ktr = 0;
   double *pp = malloc(sizeof(double) * 3 * ns);
   int maxit0 = -16 * log(p->eps * 0.1), itmax = 0;
   for (n = 0; n < S->nmol; n++) {
@@ -1338,6 +1348,7 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
     } while (maxsq > p->eps && it < maxit);
     if (nc && it == maxit && maxsq > p->eps) { free(pp); return MDX_E_NOCONV; }
     if (it > itmax) itmax = it;
+    double vm1[3] = {0, 0, 0}, vm2[3] = {0, 0, 0}, mm = 0;
     for (i = i0; i < i1; i++) {        /* shakev3.c */
       double V1[3], V2[3], m = S->mass[i];
       for (k = 0; k < 3; k++) { V1[k] = r1[3 * i + k]; V2[k] = pp[3 * i + k] - r[3 * i + k]; }
@@ -1349,10 +1360,20 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
       Pkin[3] += (V1[1] * V1[2] + V2[1] * V2[2]) * m;
       Pkin[4] += (V1[2] * V1[0] + V2[2] * V2[0]) * m;
       Pkin[5] += (V1[0] * V1[1] + V2[0] * V2[1]) * m;
-      for (k = 0; k < 3; k++) { r1[3 * i + k] = V2[k]; r[3 * i + k] = pp[3 * i + k]; }
+      for (k = 0; k < 3; k++) {
+        vm1[k] += m * V1[k];
+        vm2[k] += m * V2[k];
+        r1[3 * i + k] = V2[k];
+        r[3 * i + k] = pp[3 * i + k];
+      }
+      mm += m;
     }
+    /* translational kinetic energy of the molecule (sim/shakev3.c:35) */
+    Ktr += (vm1[0] * vm1[0] + vm1[1] * vm1[1] + vm1[2] * vm1[2] +
+            vm2[0] * vm2[0] + vm2[1] * vm2[1] + vm2[2] * vm2[2]) / mm;
   }
   Ekin *= hscale;                          /* twice the kinetic energy */
+  Ktr *= hscale;
   (void)ns;
   for (k = 0; k < 6; k++) { Pkin[k] *= hscale; PTvirc[k] /= hh; }
   virc /= hh;
@@ -1366,6 +1387,33 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
     out->lambda = exp(b * -p->h / p->tau);
     for (i = 0; i < 3 * ns; i++) r1[i] *= out->lambda;
   }
+  if (p->thermostat >= 11 && p->thermostat <= 13) {
+    /* decoupled friction thermostats (sim/thermo.c:107-135): ze on the
+       molecular translations (T_TR = 11), z on the intramolecular motion
+       (T_IN = 12), both with their own temperatures (T_FRICTIONS = 13) */
+    double z = 0, ze = 0, f_tr = p->nf_tr, f_in = p->nf - p->nf_tr;
+    if (p->thermostat == 11) ze = orc_blogT(Ktr / f_tr, p->T) * (p->h / p->tau);
+    if (p->thermostat == 12) z = orc_blogT((Ekin - Ktr) / f_in, p->T) * (p->h / p->tau);
+    if (p->thermostat == 13) {
+      double T_tr = (f_in + f_tr) * p->T / (p->T_tr_in * f_in + f_tr);
+      double T_in = (f_in + f_tr) * p->T / (f_in + f_tr / p->T_tr_in);
+      z = orc_blogT((Ekin - Ktr) / f_in, T_in) * (p->h / p->tau);
+      ze = orc_blogT(Ktr / f_tr, T_tr) * (p->h / p->tau);
+    }
+    ze -= z;
+    for (n = 0; n < S->nmol; n++) {
+      int i0 = S->mol_first[n], i1 = S->mol_first[n + 1];
+      double vm[3] = {0, 0, 0}, mm = 0;
+      for (i = i0; i < i1; i++) {
+        for (k = 0; k < 3; k++) vm[k] += S->mass[i] * r1[3 * i + k];
+        mm += S->mass[i];
+      }
+      for (k = 0; k < 3; k++) vm[k] /= mm;
+      for (i = i0; i < i1; i++)
+        for (k = 0; k < 3; k++) r1[3 * i + k] -= z * r1[3 * i + k] + ze * vm[k];
+    }
+  }
+  out->En_kin_tr = Ktr / 2;
   out->En_kin = Ekin / 2;
   for (k = 0; k < 6; k++) { out->Pkin[k] = Pkin[k]; out->PTvirc[k] = PTvirc[k]; }
   out->virc = virc;

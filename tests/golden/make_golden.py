@@ -327,6 +327,42 @@ def plb_case(name, species, types, counts, L, cutoff, nsteps):
     print(name, len(dumps), "dumps", len(raw), "plb bytes")
 
 
+def friction_case(name, species, types, counts, L, cutoff, nsteps):
+    """the decoupled friction thermostats of sim/thermo.c:107-135: from the
+    configuration of 5 Berendsen steps (cfg0), three runs of nsteps each
+    with thermostat="BerendsenCM" (translations), "BerendsenIN" (the
+    intramolecular motion) and "frictions" (both, T_tr_in=1.3), tau.T=0.05;
+    stored: cfg0 and each run's final configuration"""
+    runs = [("tr", "BerendsenCM", ""), ("in", "BerendsenIN", ""), ("fr", "frictions", "T_tr_in=1.3\n")]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "T=298", "epsc=1e-12", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="Berendsen"\ntau.T=0.1\nnoint=5\nno=1;\n')
+        run([COOK, "sys", "md"], td)
+        cfg0 = open(os.path.join(td, "md.cfg"), "rb").read()
+        out["cfg0"] = np.frombuffer(cfg0, np.uint8)
+        for tag, th, extra in runs:
+            open(os.path.join(td, "md.cfg"), "wb").write(cfg0)
+            open(os.path.join(td, "md.get"), "w").write(
+                'init="start"\nthermostat="%s"\n%stau.T=0.05\nnoint=%d\nno=1;\n' % (th, extra, nsteps))
+            run([COOK, "sys", "md"], td)
+            out["cfg_" + tag] = np.frombuffer(open(os.path.join(td, "md.cfg"), "rb").read(), np.uint8)
+        meta = parse_prt(os.path.join(td, "md.prt"))
+    meta.update(dict(name=name, counts=counts, L=L, cutoff=cutoff, epsr=0.05, epsk=0.05,
+                     h=0.002, T=298.0, tauT=0.05, T_tr_in=1.3, nsteps=nsteps,
+                     runs=[r[:2] for r in runs], forcefield=ff_arrays(species, types)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, [(k, len(v)) for k, v in out.items()])
+
+
 def nose_case(name, species, types, counts, L, cutoff, nsteps, tauT, pred):
     """Nose-Hoover with the time-reversible velocity predictor in cook's
     Shake() (sim/trvpscale.c, sim/verlet.c, sim/thermo.c): run 1 (crystal,
@@ -540,6 +576,9 @@ def main(argv):
         "brine506_rdf": lambda: rdf_case(
             "brine506_rdf", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 3, 20.0),
+        # decoupled friction thermostats on 216 SPC/E from a saved configuration
+        "spce216_friction": lambda: friction_case(
+            "spce216_friction", [spce], ["H", "O"], [216], 18.6206, 8.0, 6),
         # Nose-Hoover (TRVP k=3) on 216 SPC/E from a saved configuration
         "spce216_nose": lambda: nose_case(
             "spce216_nose", [spce], ["H", "O"], [216], 18.6206, 8.0, 8, 0.05, 3),

@@ -61,7 +61,9 @@ def test_rdf_forms_and_md_step(brine, mode):
     d, meta, ref, bs = brine
     bso = oracle.fixture_system(meta)
     e = Engine(bs, mode)
-    for grid, cut in ((-20, 9.0), (20, 4.0), (33, 12.0)):
+    # records x bins: 10 x 1594 = 15,940 (LDS histograms at nearly their
+    # 64 KiB budget) and 10 x 18,001 (past it: global 64-bit atomics)
+    for grid, cut in ((-20, 9.0), (20, 4.0), (33, 12.0), (177, 9.0), (2000, 9.0)):
         print("rdf grid %d cutoff %g" % (grid, cut), flush=True)
         e.set_rdf(grid, cut)
         e.forces(d["r"][3], 1)
