@@ -148,6 +148,16 @@ int mdx_forces(mdx_ctx_t *ctx, const double *r, double *f, mdx_out_t *out,
    sim/ewald.c:570-613 (ekk recomputed, same ktab). */
 int mdx_set_box(mdx_ctx_t *ctx, const double L[3]);
 
+/* A new system in the same context (No.s, molecules, types, tables or box
+   changed: cook re-allocates linklist()'s and Ewald()'s scratch on such a
+   change, cook/linklist.c:88-125, sim/ewald.c:367-377).  Everything
+   mdx_create builds is built again; the precision and the settings made by
+   mdx_set_stencil, mdx_set_kchunk, mdx_set_kspace_mfma, mdx_set_overlap,
+   mdx_set_graph, mdx_set_timing and mdx_set_rdf carry over where the new
+   system allows them (the RDF records restart from zero); the MD state
+   must be uploaded again.  On failure the context keeps the old system. */
+int mdx_resize(mdx_ctx_t *ctx, const mdx_system_t *sys);
+
 /* ---- device-resident MD (the captured step) ---------------------------
    State lives on the GPU: r = cfg[0] (r(t)), r1 = cfg[1] (r(t)-r(t-h)). */
 typedef struct mdx_step_s {

@@ -1190,6 +1190,35 @@ extern "C" int mdx_set_box(mdx_ctx_t *h, const double L[3]) {
   return 0;  // k-space uses c.L at every call (ekk recomputed, same k-set)
 }
 
+// a new system in the same context: everything mdx_create builds is built
+// again for it (a complete new context is made and swapped in, the old one
+// destroyed; on failure the old system stays); the settings carry over
+// where the new system allows them
+extern "C" int mdx_resize(mdx_ctx_t *h, const mdx_system_t *s) {
+  if (!h || !s) return fail(MDX_E_ARG, "mdx_resize: NULL argument");
+  Ctx &c = h->c;
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  mdx_ctx_t *t = nullptr;
+  int rc = mdx_create(&t, s, c.mode);
+  if (rc) return rc;
+  const bool allpair = c.allpair, kmfma = c.kmfma, overlap = c.overlap, graph = c.use_graph;
+  const bool timing = c.timing;
+  const int w = c.wst, pcols = c.pcols, kzc = c.kzc, kparts = c.kparts, rdf_grid = c.rdf_grid;
+  const double rdf_cutoff = c.rdf_cutoff;
+  std::swap(h->c, t->c);
+  mdx_destroy(t);                      // the old system's buffers, streams, graph
+  Ctx &n = h->c;
+  n.kmfma = kmfma;
+  n.overlap = overlap;
+  n.use_graph = graph;
+  n.timing = timing;
+  if (mdx_set_stencil(h, allpair ? 0 : w, pcols)) (void)mdx_set_stencil(h, 2, 0);
+  if (n.nq && (rc = mdx_set_kchunk(h, kzc, kparts))) return rc;
+  if (rdf_grid && (rc = mdx_set_rdf(h, rdf_grid, rdf_cutoff))) return rc;
+  return 0;
+}
+
 // ------------------------------------------------- debugging / test hooks
 
 // cook's cell id of every site for cook's No.cell (membership check)
@@ -1380,6 +1409,7 @@ extern "C" int mdx_set_rdf(mdx_ctx_t *h, int grid, double cutoff) {
   c.rdf_grid = grid;
   c.rdf_nh = nh;
   c.rdf_nrec = (int)nrec;
+  c.rdf_cutoff = cutoff;
   return 0;
 }
 

@@ -243,7 +243,7 @@ struct Ctx {
   // RDF histograms of the measured calls (off: rdf_grid 0): records
   // [rdf_nrec][rdf_nh], measured calls and their summed box volume
   int rdf_grid = 0, rdf_nh = 0, rdf_nrec = 0, rdf_nmeas = 0;
-  double rdf_V = 0;
+  double rdf_V = 0, rdf_cutoff = 0;
   DBuf<unsigned long long> rdf_hist;
 };
 

@@ -25,6 +25,16 @@ class Engine:
         self.mode = mode
         self._rdf_grid = 0
 
+    def resize(self, built_system):
+        """another system in this context (mdx_resize): the settings carry
+        over, the MD state must be uploaded again"""
+        rc = self.lib.mdx_resize(self.ctx, C.byref(built_system.struct))
+        abi.check(self.lib, rc, "mdx_resize")
+        self.bs = built_system
+        self.ns = built_system.struct.ns
+        self.polar = bool(built_system.info.get("polar"))
+        self.nrow = 2 * self.ns if self.polar else self.ns
+
     def close(self):
         if self.ctx:
             self.lib.mdx_destroy(self.ctx)
@@ -414,6 +424,7 @@ def declare_extras(lib):
                                  C.POINTER(C.c_double)]
     lib.mdx_get_nose.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.mdx_resize.argtypes = [C.c_void_p, C.POINTER(abi.System)]
     lib.mdx_set_rdf.argtypes = [C.c_void_p, C.c_int, C.c_double]
     lib.mdx_get_rdf.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_long,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -168,3 +168,32 @@ def test_engine_from_cook_input_files(mode, tmp_path):
     assert e < TOL[mode], e
     assert abs(out.En_pot + out.En_el - meta["Epot"]) < (1e-10 if mode == abi.MDX_FP64 else 2e-6) * abs(meta["Epot"])
     eng.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_resize_to_another_system_and_back(mode):
+    """mdx_resize: the context takes another system (other numbers of
+    sites, types and box) and gives a fresh context's bits for it with the
+    same settings; back to the first system, its bits again"""
+    da, ma = oracle.load_fixture("spce1000_md")
+    db, mb = oracle.load_fixture("brine506")
+    bsa = oracle.fixture_system(ma, provider_=engine_provider())
+    bsb = oracle.fixture_system(mb, provider_=engine_provider())
+    e = Engine(bsa, mode)
+    e.set_stencil(2, 5)
+    e.set_kchunk(16, 8)
+    fa, oa = e.forces(da["r"], 1)
+    e.resize(bsb)
+    info = e.info()
+    assert info["pair_cols"] == 5 and info["kchunk"] == 16
+    fb, ob = e.forces(db["r"], 1)
+    fresh = Engine(bsb, mode)
+    fresh.set_stencil(2, 5)
+    fresh.set_kchunk(16, 8)
+    fb2, ob2 = fresh.forces(db["r"], 1)
+    fresh.close()
+    assert np.array_equal(fb, fb2) and ob.En_el == ob2.En_el and ob.npair == ob2.npair
+    e.resize(bsa)
+    fa2, oa2 = e.forces(da["r"], 1)
+    e.close()
+    assert np.array_equal(fa, fa2) and oa.En_pot == oa2.En_pot
