@@ -245,8 +245,9 @@ def bench_replicas(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (several GPUs "
+              "would each own a slab of the box with its halo and split the k-vectors)" % a.gpus,
+              file=sys.stderr)
         sys.exit(2)
     if a.workload == "c4":
         return bench_replicas(a)
