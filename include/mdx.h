@@ -451,6 +451,16 @@ int mdx_setss_exp6(mdx_exp6_t *ss, double EvdW_i, double RvdW_i, double a_i,
    parm[0] after the eunit conversion). */
 int mdx_setss_buck(mdx_exp6_t *ss, double A_i, double B_i, double C_i, double A_j,
                    double B_j, double C_j, int comb_rule, double C2, double *corr);
+/* The Busing form (cook built with sim/busing, POW = 0),
+   u = f rho_ij exp((R_ij - r)/rho_ij) - C_ij/r^6 with f = 0.05 e^2/A^2,
+   into the exp-6 table (tau = -1/rho_ij, ef = f exp(R_ij/rho_ij),
+   eu = rho_ij ef, ru = C_ij; initssaux() of sim/busing/sitesite.c:87-108)
+   with combrule() (sim/busing/sitesite.c:63-80): rho_ij f = EvdW_i +
+   EvdW_j (0, the pair switched off, if either is negative), R_ij =
+   RvdW_i + RvdW_j, C_ij = C_i C_j; comb_rule must be 0.  EvdW = f rho_i in
+   K, RvdW = R_i in Angstrom, C_i in sqrt(K) Angstrom^3. */
+int mdx_setss_busing(mdx_exp6_t *ss, double EvdW_i, double RvdW_i, double C_i, double EvdW_j,
+                     double RvdW_j, double C_j, int comb_rule, double C2, double *corr);
 
 /* initerfc()+makesplines() with SPLINE=-2, SUBGRID=1 (sim/erfc.c:119-191,
    sim/splines.c:23-158): fills tab[0..*n) (caller provides room for

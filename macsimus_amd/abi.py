@@ -111,6 +111,9 @@ def declare_setup(lib, prefix):
     f = getattr(lib, prefix + "setss_buck")
     f.argtypes = [C.POINTER(Exp6)] + [C.c_double] * 6 + [C.c_int, C.c_double, _dp]
     f.restype = C.c_int
+    f = getattr(lib, prefix + "setss_busing")
+    f.argtypes = [C.POINTER(Exp6)] + [C.c_double] * 6 + [C.c_int, C.c_double, _dp]
+    f.restype = C.c_int
     f = getattr(lib, prefix + "erfc_table")
     f.argtypes = [C.POINTER(Erfc), _ip, _dp, C.c_int, C.c_double, C.c_double,
                   C.c_double, C.c_int]

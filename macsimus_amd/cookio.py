@@ -178,8 +178,9 @@ def read_ble(path, form=None):
     The site-site form is cook's compile-time choice; the file has one
     parameter column per nparms.  nparms=0 is Lennard-Jones; nparms=1 is
     read as exp-6 when the file's table is the `exp-6` one (files with a
-    polar key) or when form="exp6" is given (the Busing and Buckingham
-    forms of one parameter are not carried)."""
+    polar key) or when form="exp6" is given; form="buck" and
+    form="busing" read the Buckingham and Busing tables (POW = 0) of the
+    same layout."""
     from .system import EUNIT_KCAL
     text = open(path).read()
     # header: getdata up to ';' after skipping comments
@@ -200,9 +201,11 @@ def read_ble(path, form=None):
                 form = "exp6"
             elif re.search(r"^Buckingham(\s|$)", text, re.M):
                 form = "buck"
-        if form not in ("exp6", "buck"):
-            raise CookInputError("nparms=1: name the site-site form (form='exp6' or 'buck'); the "
-                                 "Busing form is not supported")
+            elif re.search(r"^Busing(\s|$)", text, re.M):
+                form = "busing"
+        if form not in ("exp6", "buck", "busing"):
+            raise CookInputError("nparms=1: name the site-site form (form='exp6', 'buck' or "
+                                 "'busing')")
         if form == "exp6" and not comb_rule & 1:
             raise CookInputError("exp-6: comb_rule&1 = 0 (Kirkwood) is not implemented "
                                  "(sim/exp6/sitesite.c:52)")
@@ -218,7 +221,8 @@ def read_ble(path, form=None):
     n14 = 2 if distance14 else 1
     L = _Lines(text)
     L.find("site-types" if polar < 0 else
-           {"exp6": "exp-6", "buck": "Buckingham"}.get(form, "Lennard-Jones"))
+           {"exp6": "exp-6", "buck": "Buckingham", "busing": "Busing"}.get(form,
+                                                                           "Lennard-Jones"))
     types, mass, evdw, rvdw, alphas, pol, parm = [], [], [], [], [], {}, []
     for i in range(nsites):
         L.tok()                                   # site number
@@ -230,12 +234,13 @@ def read_ble(path, form=None):
             if t is None or t == "=":
                 break
             g = (float(t), float(L.tok()) * escale, float(L.tok()))
-            # Buckingham: C_ii is an energy times A^6 (Epow 1, sim/buck/sitesite.c:16-24)
-            ps = escale if form == "buck" else 1.0
+            # Buckingham: C_ii is an energy times A^6 (Epow 1, sim/buck/sitesite.c:16-24);
+            # Busing: C_i is the square root of one (Epow 1/2, sim/busing/sitesite.c:5-16)
+            ps = {"buck": escale, "busing": escale ** 0.5}.get(form, 1.0)
             groups.append(g + tuple(float(L.tok()) * ps for _ in range(nparms)))
         alpha, E, R = groups[0][:3]
         parm.append(groups[0][3:])
-        if E > 0 and form != "buck":
+        if E > 0 and form not in ("buck", "busing"):
             E = -E                                # initcombrule(): EvdW is a minimum
         alphas.append(alpha)
         if polar == 1:
@@ -297,7 +302,7 @@ def read_ble(path, form=None):
                             site_xyz=site_xyz, site_excl=site_excl, bonds=bonds))
     ff = dict(types=types, type_mass=mass, type_EvdW_kcal=evdw, type_RvdW=rvdw,
               type_alpha=alphas, species=species, comb_rule=comb_rule)
-    if form in ("exp6", "buck"):
+    if form in ("exp6", "buck", "busing"):
         ff["form"] = form
         ff["type_parm"] = [p[0] for p in parm]
     if pol:

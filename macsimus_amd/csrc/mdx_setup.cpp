@@ -15,6 +15,8 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kSqrtPi = 1.772453850905516;
+// the electron charge in program units, sqrt(K A) (sim/units.h:108)
+constexpr double kElectron = 408.779826922048;
 // No.eps = machprec() (sim/siminit.c:64-66), the convergence scale of setss
 constexpr double kMachEps = 1.1102230246251565e-16;
 
@@ -209,6 +211,35 @@ extern "C" int mdx_setss_buck(mdx_exp6_t *ss, double Ai, double Bi, double Ci, d
   ss->ef = v.ef;
   ss->ru = v.ru;
   return smooth_cutoff(v, C2, "mdx_setss_buck", ss->C1q, ss->C2q, ss->A, ss->A4, corr);
+}
+
+extern "C" int mdx_setss_busing(mdx_exp6_t *ss, double Ei, double Ri, double Ci, double Ej,
+                                double Rj, double Cj, int comb_rule, double C2, double *corr) {
+  if (!ss) return mdx::fail(MDX_E_ARG, "mdx_setss_busing: ss is NULL");
+  std::memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (comb_rule != 0)
+    return mdx::fail(MDX_E_ARG, "mdx_setss_busing: the Busing combining rule is comb_rule=0 "
+                                "(sim/busing/sitesite.c:37)");
+  // combrule() of sim/busing/sitesite.c:63-80: rho and R additive, C a
+  // product; a site of EvdW < 0 switches every pair of it off
+  const double eps = (Ei < 0 || Ej < 0) ? 0.0 : Ei + Ej;
+  const double R = Ri + Rj, C = Ci * Cj;
+  if (eps == 0 || C2 == 0) return 0;    // SS_MEASURE skips eps = 0 whole: the table is zero
+  // initssaux(), sim/busing/sitesite.c:87-108: rho = eps/f,
+  // u_rep = rho A e^(-r/rho) with A = f e^(R/rho), f = 0.05 e^2/A^2
+  const double f = 0.05 * kElectron * kElectron, rho = eps / f;
+  Exp6 v;
+  v.tau = -f / eps;
+  v.ef = f * std::exp(R / rho);
+  v.eu = rho * v.ef;
+  v.ru = C;
+  v.rf = 6 * C;
+  ss->tau = v.tau;
+  ss->eu = v.eu;
+  ss->ef = v.ef;
+  ss->ru = v.ru;
+  return smooth_cutoff(v, C2, "mdx_setss_busing", ss->C1q, ss->C2q, ss->A, ss->A4, corr);
 }
 
 extern "C" int mdx_erfc_size(int grid, double maxr, double alpha) {

@@ -48,6 +48,9 @@ class ForceMatch:
 
     def __init__(self, ff, counts, L, cutoff, frames, targets, lj_pairs, charge_map,
                  mode=abi.MDX_FP64, epsr=0.05, epsk=0.05, alpha=None, kappa=None):
+        form = ff.get("form", "lj")
+        if form not in ("lj", "exp6"):
+            raise ValueError("ForceMatch fits the Lennard-Jones and exp-6 forms, not %r" % form)
         self.ff = ff
         self.counts = counts
         self.L = [L] * 3 if np.isscalar(L) else list(L)
@@ -65,7 +68,7 @@ class ForceMatch:
         self.q_types = [self.types.index(n) for n in names]
         self.q_base = np.asarray(base, dtype=np.float64)
         self.q_M = np.asarray(M, dtype=np.float64)
-        self.exp6 = ff.get("form", "lj") == "exp6"
+        self.exp6 = form == "exp6"
         self.pp = 3 if self.exp6 else 2          # parameters per fitted pair
         self.npl = self.pp * len(self.pairs)
         self.npar = self.npl + self.q_M.shape[1]

@@ -65,6 +65,15 @@ class Provider:
             raise RuntimeError("setss_buck failed (%d)" % rc)
         return ss, corr.value
 
+    def setss_busing(self, Ei, Ri, Ci, Ej, Rj, Cj, comb_rule, C2):
+        ss = abi.Exp6()
+        corr = C.c_double(0)
+        rc = self._f("setss_busing")(C.byref(ss), Ei, Ri, Ci, Ej, Rj, Cj, comb_rule, C2,
+                                     C.byref(corr))
+        if rc:
+            raise RuntimeError("setss_busing failed (%d)" % rc)
+        return ss, corr.value
+
     def erfc_table(self, grid, maxr, cutoff, alpha, shift):
         nmax = int((maxr * alpha) ** 2 * grid) + 2
         tab = (abi.Erfc * nmax)()
@@ -134,16 +143,21 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
        cook's Buckingham form (sim/buck, POW = 0) in the exp-6 table: A_ii
        [kcal/mol] in type_EvdW_kcal, B_ii [1/A] in type_RvdW, C_ii
        [kcal/mol A^6] in type_parm, ff["comb_rule"] as in sim/buck
-       (default 5: geometric A and B)."""
+       (default 5: geometric A and B).  ff["form"] = "busing" is cook's
+       Busing form (sim/busing, POW = 0), also in the exp-6 table: f rho_i
+       [kcal/mol] in type_EvdW_kcal (negative: the site interacts with
+       nothing), R_i [A] in type_RvdW, C_i [sqrt(kcal/mol) A^3] in
+       type_parm (sim/busing/sitesite.c:5-16), comb_rule 0."""
     prov = provider or engine_provider()
     bs = BuiltSystem()
     S = bs.struct
     nt = len(ff["types"])
     form = ff.get("form", "lj")
-    if form not in ("lj", "exp6", "buck"):
-        raise ValueError("site-site form %r: lj, exp6 or buck" % form)
+    if form not in ("lj", "exp6", "buck", "busing"):
+        raise ValueError("site-site form %r: lj, exp6, buck or busing" % form)
     buck = form == "buck"
-    exp6 = form in ("exp6", "buck")       # buck fills the exp-6 table
+    busing = form == "busing"
+    exp6 = form in ("exp6", "buck", "busing")   # buck and busing fill the exp-6 table
     EvdW = [e * EUNIT_KCAL * (exp6_evdw_scale if exp6 else 1.0) for e in ff["type_EvdW_kcal"]]
     RvdW = list(ff["type_RvdW"])
     C2 = cutoff if lj_cutoff is None else lj_cutoff
@@ -166,6 +180,13 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
                 Cp = [c * EUNIT_KCAL for c in ff["type_parm"]]
                 Ab = [e * EUNIT_KCAL for e in ff["type_EvdW_kcal"]]
                 ss, c = prov.setss_buck(Ab[i], RvdW[i], Cp[i], Ab[j], RvdW[j], Cp[j], cr, C2)
+            elif busing:
+                # f rho_i [K] and C_i [sqrt(K) A^3] after the eunit powers
+                # of Epow (1 and 1/2, sim/busing/sitesite.c:5-16)
+                Eb = [e * EUNIT_KCAL for e in ff["type_EvdW_kcal"]]
+                Cb = [c * EUNIT_KCAL ** 0.5 for c in ff["type_parm"]]
+                ss, c = prov.setss_busing(Eb[i], RvdW[i], Cb[i], Eb[j], RvdW[j], Cb[j],
+                                          int(ff.get("comb_rule", 0)), C2)
             elif EvdW[i] == 0 or EvdW[j] == 0:
                 ss, c = (abi.Exp6 if exp6 else abi.LJ)(), 0.0
             else:

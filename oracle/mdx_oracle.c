@@ -127,28 +127,14 @@ static double orc_e6F(const orc_e6 *p, double r)
   return (p->ef * exp(p->tau * x) / x - p->rf / z / rr) * r;
 }
 
-/* combrule() of sim/exp6/sitesite.c:60-76 and the potential-independent
-   smooth cutoff and tail correction of setss() (sim/setss.c:244-306,
-   455-499) on the exp-6 u and f */
-int oracle_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej,
-                      double Rj, double aj, int comb_rule, double C2, double *corr)
+/* the potential-independent smooth cutoff and tail correction of
+   setss() (sim/setss.c:244-306, 455-499) on the exp-6 u and f of p */
+static int orc_e6_cut(const orc_e6 *q, mdx_exp6_t *ss, double C2, double *corr)
 {
   const double neps = 1.1102230246251565e-16;
-  double sig, eps, a, C1, oldC1;
-  orc_e6 p;
+  orc_e6 p = *q;
+  double C1, oldC1;
   int n;
-  memset(ss, 0, sizeof(*ss));
-  if (corr) *corr = 0;
-  if (!(comb_rule & 1) || Ei > 0 || Ej > 0) return MDX_E_ARG;
-  sig = comb_rule & 2 ? sqrt(Ri * Rj * 4) : Ri + Rj;
-  eps = sqrt(Ei * Ej);
-  a = sqrt(ai * aj);
-  if (eps == 0 || C2 == 0) return 0;
-  p.tau = -(a / sig);
-  p.eu = 6 * eps / (a - 6) * exp(a);
-  p.ef = p.eu * (a / sig);
-  p.ru = eps * a * pow6(sig) / (a - 6);
-  p.rf = 6 * p.ru;
   ss->tau = p.tau; ss->eu = p.eu; ss->ef = p.ef; ss->ru = p.ru;
   C1 = 0.775 * C2;
   n = 0;
@@ -182,6 +168,29 @@ int oracle_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej
   return 0;
 }
 
+/* combrule() of sim/exp6/sitesite.c:60-76 and the potential-independent
+   smooth cutoff and tail correction of setss() (sim/setss.c:244-306,
+   455-499) on the exp-6 u and f */
+int oracle_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej,
+                      double Rj, double aj, int comb_rule, double C2, double *corr)
+{
+  double sig, eps, a;
+  orc_e6 p;
+  memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (!(comb_rule & 1) || Ei > 0 || Ej > 0) return MDX_E_ARG;
+  sig = comb_rule & 2 ? sqrt(Ri * Rj * 4) : Ri + Rj;
+  eps = sqrt(Ei * Ej);
+  a = sqrt(ai * aj);
+  if (eps == 0 || C2 == 0) return 0;
+  p.tau = -(a / sig);
+  p.eu = 6 * eps / (a - 6) * exp(a);
+  p.ef = p.eu * (a / sig);
+  p.ru = eps * a * pow6(sig) / (a - 6);
+  p.rf = 6 * p.ru;
+  return orc_e6_cut(&p, ss, C2, corr);
+}
+
 /* Buckingham (sim/buck, POW = 0): u = A_ij exp(-B_ij r) - C_ij/r^6, the
    exp-6 macros with tau = -B_ij, eu = A_ij, ef = A_ij B_ij, ru = C_ij
    (initssaux(), sim/buck/sitesite.c:178-188); the combining rules of
@@ -189,10 +198,8 @@ int oracle_setss_exp6(mdx_exp6_t *ss, double Ei, double Ri, double ai, double Ej
 int oracle_setss_buck(mdx_exp6_t *ss, double Ai, double Bi, double Ci, double Aj,
                       double Bj, double Cj, int comb_rule, double C2, double *corr)
 {
-  const double neps = 1.1102230246251565e-16;
-  double A = 0, B = 0, aux = 0, C, C1, oldC1;
+  double A = 0, B = 0, aux = 0, C;
   orc_e6 p;
-  int n;
   memset(ss, 0, sizeof(*ss));
   if (corr) *corr = 0;
   if (((comb_rule & 12) == 0) != ((comb_rule & 3) == 0)) return MDX_E_ARG;
@@ -220,37 +227,35 @@ int oracle_setss_buck(mdx_exp6_t *ss, double Ai, double Bi, double Ci, double Aj
   C = sqrt(Ci) * sqrt(Cj);
   if (A == 0 || C2 == 0) return 0;
   p.tau = -B; p.eu = A; p.ef = A * B; p.ru = C; p.rf = 6 * C;
-  ss->tau = p.tau; ss->eu = p.eu; ss->ef = p.ef; ss->ru = p.ru;
-  C1 = 0.775 * C2;
-  n = 0;
-  do {
-    double den = orc_e6F(&p, C1);
-    if (fabs(den) < 1e-50) return MDX_E_ARG;
-    oldC1 = C1;
-    C1 = 1 + 4 * orc_e6U(&p, C1) / C1 / den;
-    if (n++ > 2000 || C1 < 0) return MDX_E_ARG;
-    C1 = C2 / sqrt(C1);
-  } while (fabs(1 - C1 / oldC1) > neps * n);
-  if (C1 > 0.999 * C2 || C1 < 0.5 * C2) return MDX_E_ARG;
-  ss->A = orc_e6U(&p, C1) / sqr(C2 * C2 - C1 * C1);
-  ss->A4 = -orc_e6F(&p, C1) / (C2 * C2 - C1 * C1) / C1;
-  ss->C2q = C2 * C2;
-  ss->C1q = C1 * C1;
-  if (corr) {
-    const int NG = 64;
-    const double g4 = 0.2113248654051871;
-    double c0 = 4 * PI * ss->A *
-      (C1 * (pow6(C1) / 7 - 0.4 * sqr(C1) * sqr(C1) * sqr(C2) + sqr(C1) * sqr(C2) * sqr(C2) / 3)
-       - 8.0 / 105 * pow(C2, 7));
-    double h = 1.0 / NG / C1, c1 = 0, r;
-    int k;
-    for (k = 0; k < NG; k++) {
-      r = 1 / ((k + g4) * h); c1 += sqr(r * r) * orc_e6U(&p, r);
-      r = 1 / ((k + (1 - g4)) * h); c1 += sqr(r * r) * orc_e6U(&p, r);
-    }
-    *corr = c0 + c1 * 2 * PI * h;
-  }
-  return 0;
+  return orc_e6_cut(&p, ss, C2, corr);
+}
+
+/* Busing (sim/busing, POW = 0): u = f rho_ij exp((R_ij - r)/rho_ij) -
+   C_ij/r^6, f = 0.05 e^2/A^2 = 0.05 electron^2 (sim/busing/sitesite.h:
+   1-14, 112-115), the exp-6 macros with tau = iBij = -f/eps,
+   ef = Aij = f exp(R_ij/rho_ij), eu = rhorho Aij, ru = C
+   (initssaux(), sim/busing/sitesite.c:87-108); combrule()
+   (sim/busing/sitesite.c:63-80): eps = EvdW_i + EvdW_j = f rho_ij, 0 when
+   either is negative, which switches the pair off whole (SS_MEASURE tests
+   eps, sim/busing/sitesite.h:248-256); R additive, C a product */
+int oracle_setss_busing(mdx_exp6_t *ss, double Ei, double Ri, double Ci, double Ej,
+                        double Rj, double Cj, int comb_rule, double C2, double *corr)
+{
+  const double fb = 0.05 * 408.779826922048 * 408.779826922048;
+  double eps, rhorho;
+  orc_e6 p;
+  memset(ss, 0, sizeof(*ss));
+  if (corr) *corr = 0;
+  if (comb_rule) return MDX_E_ARG;
+  eps = (Ei < 0 || Ej < 0) ? 0 : Ei + Ej;
+  if (eps == 0 || C2 == 0) return 0;
+  rhorho = eps / fb;
+  p.ef = fb * exp((Ri + Rj) / rhorho);
+  p.tau = -fb / eps;
+  p.eu = rhorho * p.ef;
+  p.ru = Ci * Cj;
+  p.rf = 6 * p.ru;
+  return orc_e6_cut(&p, ss, C2, corr);
 }
 
 /* --------------------------------------------------------- erfc splines */

@@ -221,3 +221,36 @@ def test_buckingham_force_field_file(tmp_path):
     assert ff["form"] == "buck" and ff["comb_rule"] == 5
     assert ff["type_EvdW_kcal"][1] > 0 and ff["type_RvdW"][1] == 3.5
     assert ff["type_parm"] == [0.0, 1.2e6]
+
+
+def test_busing_force_field_file(tmp_path):
+    """the Busing table: f rho_i in the EvdW column (kept as written, a
+    negative one switches the site off), R_i in RvdW, C_i in the parameter
+    column; EvdW scaled by eunit, C_i by its square root (Epow 1 and 1/2,
+    sim/busing/sitesite.c:5-16); the deck builds the Busing table"""
+    from macsimus_amd.system import EUNIT_KCAL
+    a_of = {"H": 5.0, "O": 40.0}
+    ble, species, types = exp6_ble(tmp_path, a_of, table="Busing", comb_rule=0)
+    lines = open(ble).read().splitlines()
+    k = lines.index("Busing")
+    for i, (E, R) in zip(range(k + 2, k + 4), ((-1.0, 0.0), (6.2, 1.8))):
+        f = lines[i].split()
+        f[4], f[5] = "%.3f" % E, "%.3f" % R
+        lines[i] = "  " + "  ".join(f)
+    open(ble, "w").write("\n".join(lines) + "\n")
+    ff = cookio.read_ble(ble)
+    assert ff["form"] == "busing" and ff["comb_rule"] == 0
+    assert ff["type_EvdW_kcal"] == [-1.0, 6.2] and ff["type_RvdW"] == [0.0, 1.8]
+    assert ff["type_parm"] == [5.0, 40.0]
+    txt = open(ble).read().replace("comb_rule=0", "comb_rule=0 eunit=%.15g" % (2 * EUNIT_KCAL))
+    open(ble, "w").write(txt)
+    ff2 = cookio.read_ble(ble)
+    assert ff2["type_EvdW_kcal"] == pytest.approx([-2.0, 12.4], rel=1e-14)
+    assert ff2["type_parm"] == pytest.approx([5.0 * 2 ** 0.5, 40.0 * 2 ** 0.5], rel=1e-14)
+    open(tmp_path / "w.def", "w").write("N[0]=64 L[0]=12.4 L[1]=12.4 L[2]=12.4 cutoff=6 "
+                                        "el.epsr=0.05 el.epsk=0.05;\n")
+    bs, _ = cookio.system_from_cook(ble, str(tmp_path / "w.def"), provider=oracle.provider())
+    t = bs.info["sstab"]
+    assert bs.struct.sstab_exp6 and t[0].eu == 0 and t[1].eu == 0
+    assert t[3].eu * np.exp(t[3].tau * 3.6) == pytest.approx(2 * 12.4 * EUNIT_KCAL, rel=1e-13)
+    assert t[3].ru == pytest.approx((40.0 * 2 ** 0.5) ** 2 * EUNIT_KCAL, rel=1e-13)

@@ -105,12 +105,15 @@ def test_exp6_full_size_mixed_vs_fp64():
 
 
 def test_lj_only_entry_points_refuse_exp6(brine):
+    """the LJ adjoint refuses an exp-6 context; the replica batch takes
+    exp-6 systems (tests/test_gpu_batch.py) but, as for LJ, atomic ones
+    only: the molecular brine is refused for that"""
     d, meta, bs, _, _, _ = brine
     e = Engine(bs, abi.MDX_FP64)
     with pytest.raises(RuntimeError, match="exp-6"):
         e.adjoint_lj(d["r"], np.ones_like(d["r"]))
     e.close()
-    with pytest.raises(RuntimeError, match="exp-6"):
+    with pytest.raises(RuntimeError, match="atomic"):
         Batch([bs], abi.MDX_FP64)
 
 
@@ -141,3 +144,49 @@ def test_buckingham_forces_vs_oracle(brine, mode):
     assert out.npair == o_ref.npair and out.nlj == o_ref.nlj > 0
     Etot = o_ref.En_pot + o_ref.En_el
     assert abs(out.En_pot - o_ref.En_pot) < ETOL[mode] * abs(Etot)
+
+
+@pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
+def test_busing_forces_vs_oracle(brine, mode):
+    """the Busing form on the exp-6 kernels: the Buckingham sites of the
+    brine's exp-6 parameters converted to Busing sites (rho = 1/2B,
+    R = rho ln(A/2 f rho), C_i = sqrt(C); a site without repulsion gets
+    EvdW < 0); against the oracle and against the Buckingham system of
+    additive 1/B and the Busing rule for A (comb_rule 15) on the GPU"""
+    import math
+    from macsimus_amd.system import EUNIT_KCAL
+    d, meta, _, _, _, _ = brine
+    fb = 0.05 * 408.779826922048 ** 2 / EUNIT_KCAL           # f in kcal/mol/A
+    base = meta["forcefield"]
+    A, B, Cc, E, R, Ci = [], [], [], [], [], []
+    for t, Ev, Rv in zip(base["types"], base["type_EvdW_kcal"], base["type_RvdW"]):
+        a, sig, eps = workloads.EXP6_A[t], 2 * Rv, -Ev
+        A.append(6 * eps * math.exp(a) / (a - 6))
+        B.append(a / sig)
+        Cc.append(eps * a * sig ** 6 / (a - 6))
+        rho = 0.5 / B[-1]
+        E.append(fb * rho if A[-1] > 0 else -1.0)
+        R.append(rho * math.log(A[-1] / (2 * fb * rho)) if A[-1] > 0 else 0.0)
+        Ci.append(math.sqrt(Cc[-1]))
+    ffb = dict(base, form="busing", comb_rule=0, type_EvdW_kcal=E, type_RvdW=R, type_parm=Ci)
+    ffk = dict(base, form="buck", comb_rule=15, type_EvdW_kcal=A, type_RvdW=B, type_parm=Cc)
+    bso = oracle.fixture_system(dict(meta, forcefield=ffb))
+    f_ref, o_ref = oracle.forces(bso, d["r"], 1)
+    res = []
+    for ff in (ffb, ffk):
+        print("%s brine506 mode %d" % (ff["form"], mode), flush=True)
+        e = Engine(oracle.fixture_system(dict(meta, forcefield=ff), provider_=engine_provider()),
+                   mode)
+        res.append(e.forces(d["r"], 1))
+        e.close()
+    (f, out), (fk, outk) = res
+    err = rel_rms(f, f_ref)
+    print("Busing brine506 mode %d: rel RMS force err %.3e" % (mode, err), flush=True)
+    assert err < TOL[mode]
+    assert out.npair == o_ref.npair and out.nlj == o_ref.nlj > 0
+    Etot = o_ref.En_pot + o_ref.En_el
+    assert abs(out.En_pot - o_ref.En_pot) < ETOL[mode] * abs(Etot)
+    assert abs(out.En_vir - o_ref.En_vir) < ETOL[mode] * abs(o_ref.En_vir)
+    # the same tables up to rounding: the same forces
+    assert rel_rms(f, fk) < TOL[mode]
+    assert out.En_pot == pytest.approx(outk.En_pot, rel=ETOL[mode])
