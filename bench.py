@@ -381,10 +381,12 @@ def main():
                    "kappa": bs.info["kappa"], "k_vectors": nk, "pairs": npair,
                    "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU",
                    "tuned": {"stencil_w": tune[0], "kz_chunk": tune[1], "k_parts": tune[2],
-                             "pair_cols_per_part": tune[3], "kspace_pass2_mfma": bool(info["kmfma"])}},
+                             "pair_cols_per_part": tune[3], "kspace_pass2_mfma": bool(info["kmfma"]),
+                             "kspace_pass1_tiles": info["kpass1_tiles"]}},
         "us_per_force": force_ms * 1e3,
         "kernel_ms": {k: v[0] for k, v in kern.items()},
-        "roofline": {"bound": "mfma" if (dom == "kpass2" and info["kmfma"]) else "valu",
+        "roofline": {"bound": "mfma" if ((dom == "kpass2" and info["kmfma"]) or
+                                         (dom == "kpass1" and info["kpass1_tiles"])) else "valu",
                      "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_lookup(wl["name"], a.mode, dom, info),

@@ -307,11 +307,16 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    mdx_info: iv[0..n) = engine cells x,y,z, stencil w, charges, k rows,
    kz blocks, k segments, k triples, k-sum splits, kx max, mode, kz chunk,
    kz chunks, parts asked, parts made, stencil columns per pair part,
-   matrix-core pass 2 (0/1).
+   matrix-core pass 2 (0/1), pass-1 tiles (0 = FMA; n may be 19).
    mdx_set_kspace_mfma (MDX_MIXED): k-space pass 2 on the matrix cores
    (FP32 MFMA, FP32 products and sums as the FMA variant) or as FP32 FMA
-   chains; mdx_autotune tries both.  Default: on in MDX_MIXED. */
+   chains; mdx_autotune tries both.  Default: on in MDX_MIXED.
+   mdx_set_kspace_pass1 (MDX_MIXED): k-space pass 1 as FP32 FMA chains
+   (tiles = 0, the default) or on the matrix cores (FP32 MFMA
+   16x16x4, 16 or 8 tiles of 4 k rows per wave); the same sums, flushed
+   to FP64 every 32 charges.  mdx_autotune keeps the fastest. */
 int mdx_set_kspace_mfma(mdx_ctx_t *ctx, int on);
+int mdx_set_kspace_pass1(mdx_ctx_t *ctx, int tiles);
 int mdx_cook_cells(mdx_ctx_t *ctx, const double *r, int *cellid);
 int mdx_set_timing(mdx_ctx_t *ctx, int on);
 int mdx_get_timing(mdx_ctx_t *ctx, double *ms, int *count, int n);

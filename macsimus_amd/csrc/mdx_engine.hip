@@ -1367,11 +1367,33 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
     }
     c.kmfma = best_m;
     if (!rc) rc = kspace_chunks(c, best_k, best_p);
+    if (!rc && c.mode == MDX_MIXED) {
+      // pass 1: FP32 FMA (0) or matrix cores with 16 or 8 tiles per wave
+      int best_nt = c.p1nt;
+      double t1 = 1e30;
+      for (int nt : {0, 16, 8}) {
+        c.p1nt = nt;
+        if ((rc = run(fam))) break;
+        if (fam[T_KPASS1] < t1) { t1 = fam[T_KPASS1]; best_nt = nt; }
+      }
+      c.p1nt = best_nt;
+      c.gen++;                         // captured graphs hold the launch shape
+    }
   }
   c.timing = timing0;
   if (rc) return rc;
   if (choice) { choice[0] = c.wst; choice[1] = c.kzc; choice[2] = c.kparts; choice[3] = c.pcols; }
   if (ms) { ms[0] = tw; ms[1] = tk; }
+  return 0;
+}
+
+extern "C" int mdx_set_kspace_pass1(mdx_ctx_t *h, int tiles) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: NULL");
+  if (tiles != 0 && tiles != 8 && tiles != 16)
+    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0, 8 or 16");
+  if (tiles && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: MDX_MIXED only");
+  h->c.p1nt = tiles;
+  h->c.gen++;
   return 0;
 }
 
@@ -1434,9 +1456,9 @@ extern "C" int mdx_get_rdf(mdx_ctx_t *h, unsigned long long *hist, long n, int *
 extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
   Ctx &c = h->c;
-  int v[18] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+  int v[19] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
                c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart,
-               c.pcols, c.kmfma ? 1 : 0};
-  for (int i = 0; i < n && i < 18; i++) iv[i] = v[i];
+               c.pcols, c.kmfma ? 1 : 0, c.p1nt};
+  for (int i = 0; i < n && i < 19; i++) iv[i] = v[i];
   return 0;
 }

@@ -201,6 +201,7 @@ struct Ctx {
   // FMA chains, from the coefficients also written per row group in the
   // operand order of the matrix instruction (coefg, zero padded)
   bool kmfma = false;
+  int p1nt = 0;                 // MDX_MIXED pass 1: FP32 FMA (0) or matrix cores, 16 or 8 tiles per wave
   DBuf<int> d_grp_start, d_grp_base, d_grp_nb;
   DBuf<float> coefg;
   DBuf<int> d_kparts;

@@ -19,8 +19,9 @@
 // zero components.  Pass 1 is then a real GEMM: rows (kx,ky) x 4 cos/sin
 // products of a and b, columns kz x {cos,sin} of c, summed over charges.
 // Pass 2 contracts per charge the coefficients 4 w e T over kz and takes
-// the gradient analytically.  Both are FP32 VALU work in MDX_MIXED with
-// FP64 accumulation across charge chunks, and FP64 throughout in MDX_FP64.
+// the gradient analytically.  In MDX_MIXED both run in FP32 with FP64
+// accumulation across charge chunks, each as FMA chains or on the matrix
+// cores (k_sfac_mfma, k_kforce_mfma); FP64 throughout in MDX_FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -187,6 +188,119 @@ __global__ __launch_bounds__(kBlock) void k_sfac(
   for (int k = 0; k < 8; k++)
 #pragma unroll
     for (int m = 0; m < 8; m++) out[k * 8 + m] = (double)mid[k][m] + (double)acc[k][m];
+}
+
+// Pass 1 on the matrix cores (mixed mode): the same sums as k_sfac<float>
+// as a chain of v_mfma_f32_16x16x4_f32 per chunk, with the same LDS tables,
+// FP64 flush per chunk and kpart layout.  A segment (64 rows x one 8-wide
+// kz block) is taken by 64/(4 NT) waves of NT tiles of 4 rows each (NT = 16:
+// one wave, 4 segments per workgroup; NT = 8: two waves, fewer registers
+// and 2 segments per workgroup); a tile's product
+// is D[i][n] = sum_k A[i][k] B[k][n] over 4 charges k, with M index i =
+// (row 4t + i/4, product pq = i%4 of cos/sin a and b) and N index n = (kz
+// 8 blk + n/2, cos/sin c = n%2).  Operand layout of the instruction: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and receives D[4(l/16) + r][l%16]
+// in register r.
+typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_sfac_mfma(
+    int nq, const double4 *__restrict__ qpos, int nseg, const int *__restrict__ seg_blk,
+    const int *__restrict__ seg_row0, const int *__restrict__ seg_rowend,
+    const int *__restrict__ row_kx, const int *__restrict__ row_ky, int KXT, int KYT, int KZT,
+    int jper, double *__restrict__ kpart, const double4 *__restrict__ qtrig) {
+  extern __shared__ char smem[];
+  C2<float> *Xs = (C2<float> *)smem;    // [kJC][KXT], q folded in
+  C2<float> *Ys = Xs + kJC * KXT;       // [kJC][KYT]
+  C2<float> *Zs = Ys + kJC * KYT;       // [kJC][KZT]
+  const float *Xf = (const float *)Xs, *Yf = (const float *)Ys, *Zf = (const float *)Zs;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  constexpr int WPS = 16 / NT;          // waves per segment
+  const int seg = blockIdx.x * (4 / WPS) + wv / WPS;
+  const int rbase = (wv % WPS) * 4 * NT;  // first row of this wave in the segment
+  const bool segok = seg < nseg;
+  int blk = 0, row0 = 0, rowend = 0;
+  if (segok) { blk = seg_blk[seg]; row0 = seg_row0[seg]; rowend = seg_rowend[seg]; }
+  blk = __builtin_amdgcn_readfirstlane(blk);
+  // A operand of tile t: row row0 + rbase + 4t + i16/4, X factor cos/sin by
+  // bit 1 of i16, Y factor by bit 0; float offsets within a charge's table row
+  int xoff[NT], yoff[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int row = row0 + rbase + 4 * t + (i16 >> 2);
+    int kx = 0, ky = 0;
+    if (row < rowend) { kx = row_kx[row]; ky = row_ky[row]; }
+    xoff[t] = 2 * kx + ((i16 >> 1) & 1);
+    yoff[t] = 2 * ky + (i16 & 1);
+  }
+  const int zoff = 2 * (8 * blk + (i16 >> 1)) + (i16 & 1);
+  const int j0 = blockIdx.y * jper, j1 = min(nq, j0 + jper);
+  const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
+
+  mfma_f32x4 acc[NT];
+  double mid[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) { acc[t][r] = 0.0f; mid[t][r] = 0; }
+
+  for (int jc = j0; jc < j1; jc += kJC) {
+    // the tables of k_sfac (zero for charges past the split)
+    for (int t = threadIdx.x; t < kJC * ntask; t += kBlock) {
+      const int jj = t / ntask, task = t % ntask;
+      int d, m, KT;
+      C2<float> *dst;
+      if (task < ntx) { d = 0; m = task; KT = KXT; dst = Xs + jj * KXT; }
+      else if (task < ntx + nty) { d = 1; m = task - ntx; KT = KYT; dst = Ys + jj * KYT; }
+      else { d = 2; m = task - ntx - nty; KT = KZT; dst = Zs + jj * KZT; }
+      const int j = jc + jj;
+      if (j < j1) {
+        const double4 tr = qtrig[3 * (size_t)j + d];
+        const double qf = d == 0 ? qpos[j].w : 1.0;
+        double c0 = 1, s0 = 0;
+        for (int i = 0; i < m; i++) {
+          const double nc = c0 * tr.z - s0 * tr.w;
+          s0 = c0 * tr.w + s0 * tr.z;
+          c0 = nc;
+        }
+        c0 *= qf; s0 *= qf;
+        float ec = (float)c0, es = (float)s0;
+        const float bc = (float)tr.x, bs = (float)tr.y;
+        for (int k = 8 * m; k < min(8 * m + 8, KT); k++) {
+          dst[k].x = ec;
+          dst[k].y = es;
+          const float nc = ec * bc - es * bs;
+          es = ec * bs + es * bc;
+          ec = nc;
+        }
+      } else {
+        for (int k = 8 * m; k < min(8 * m + 8, KT); k++) { dst[k].x = 0; dst[k].y = 0; }
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < kJC; s += 4) {
+      const int jj = s + kq;
+      const float b = Zf[jj * 2 * KZT + zoff];
+      const float *xr = Xf + jj * 2 * KXT, *yr = Yf + jj * 2 * KYT;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[xoff[t]] * yr[yoff[t]], b, acc[t], 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) { mid[t][r] += (double)acc[t][r]; acc[t][r] = 0.0f; }
+  }
+  if (!segok) return;
+  // lane l holds rows rbase + 4t + l/16, kz 8 blk + i16/2, T index 2r + i16%2
+  double *out = kpart + ((size_t)blockIdx.y * nseg + seg) * 64 * 64 + (i16 >> 1) * 8 + (i16 & 1);
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[(rbase + 4 * t + kq) * 64 + 2 * r] = mid[t][r];
 }
 
 // sum of the charge splits: tsum[i] = sum_s kpart[s*n + i]
@@ -574,10 +688,21 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
                                           1 / c.L[1], 1 / c.L[2], (double4 *)c.qpos.p,
                                           (double4 *)c.qtrig.p);
   if (c.timing) c.tm.start(T_KPASS1, st);
-  k_sfac<R><<<g1, kBlock, shmem, st>>>(
-      c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
-      c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p, KXT, KYT, KZT,
-      jper, c.kpart.p, (const double4 *)c.qtrig.p);
+  bool pass1_mfma = false;
+  if constexpr (sizeof(R) == 4) pass1_mfma = c.p1nt != 0;
+  if (pass1_mfma && c.p1nt == 8)
+    k_sfac_mfma<8><<<dim3((K.nseg + 1) / 2, c.nsplit), kBlock, shmem, st>>>(
+        c.nq, qpos, K.nseg, c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p,
+        c.d_row_ky.p, KXT, KYT, KZT, jper, c.kpart.p, (const double4 *)c.qtrig.p);
+  else if (pass1_mfma)
+    k_sfac_mfma<16><<<g1, kBlock, shmem, st>>>(
+        c.nq, qpos, K.nseg, c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p,
+        c.d_row_ky.p, KXT, KYT, KZT, jper, c.kpart.p, (const double4 *)c.qtrig.p);
+  else
+    k_sfac<R><<<g1, kBlock, shmem, st>>>(
+        c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.nseg,
+        c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p, c.d_row_ky.p, KXT, KYT,
+        KZT, jper, c.kpart.p, (const double4 *)c.qtrig.p);
   MDX_HIP(hipGetLastError());
   if (c.timing) { c.tm.stop(T_KPASS1, st); c.tm.start(T_KCOEF, st); }
   const size_t nt = (size_t)K.nseg * 64 * 64;

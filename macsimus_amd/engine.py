@@ -121,11 +121,11 @@ class Engine:
         return cid
 
     def info(self):
-        iv = (C.c_int * 18)()
-        self.lib.mdx_info(self.ctx, iv, 18)
+        iv = (C.c_int * 19)()
+        self.lib.mdx_info(self.ctx, iv, 19)
         keys = ("ncx", "ncy", "ncz", "stencil", "nq", "krows", "kblocks", "ksegments",
                 "ktriples", "ksplits", "kxmax", "mode", "kchunk", "nkchunks", "kparts_asked",
-                "kparts", "pair_cols", "kmfma")
+                "kparts", "pair_cols", "kmfma", "kpass1_tiles")
         return dict(zip(keys, list(iv)))
 
     def set_kchunk(self, kzc, parts=16):
@@ -212,6 +212,10 @@ class Engine:
     def set_kspace_mfma(self, on=True):
         rc = self.lib.mdx_set_kspace_mfma(self.ctx, int(on))
         abi.check(self.lib, rc, "mdx_set_kspace_mfma")
+
+    def set_kspace_pass1(self, tiles=0):
+        rc = self.lib.mdx_set_kspace_pass1(self.ctx, int(tiles))
+        abi.check(self.lib, rc, "mdx_set_kspace_pass1")
 
     def set_overlap(self, on=True):
         rc = self.lib.mdx_set_overlap(self.ctx, int(on))
@@ -410,6 +414,7 @@ def declare_extras(lib):
     lib.mdx_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_overlap.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_kspace_mfma.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_kspace_pass1.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
