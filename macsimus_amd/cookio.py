@@ -10,7 +10,7 @@ header data, the site-type table (`Lennard-Jones`, POLAR columns), the
 found by its keyword as the first token of a line (find(),
 cook/simdef.c:179-202); tokens are whitespace separated (tok(),
 cook/simdef.c:205-218).  What the engine does not carry is rejected with
-a clear error: nbfixes, 1-4 pairs (`*k` exclusions), flexible angles,
+a clear error: nbfixes of the non-LJ forms, 1-4 pairs (`*k` exclusions), flexible angles,
 torsions and vibrating bonds.
 
 `read_get` reads the getdata syntax of `.def` and `.get` files
@@ -216,8 +216,8 @@ def read_ble(path, form=None):
     else:
         raise CookInputError("nparms=%d: only the Lennard-Jones (0) and exp-6 (1) forms are "
                              "supported" % nparms)
-    if nnbfixes:
-        raise CookInputError("nbfixes are not supported by the engine")
+    if nnbfixes and form != "lj":
+        raise CookInputError("nbfixes are supported for the Lennard-Jones form only")
     n14 = 2 if distance14 else 1
     L = _Lines(text)
     L.find("site-types" if polar < 0 else
@@ -253,6 +253,23 @@ def read_ble(path, form=None):
         evdw.append(E)
         rvdw.append(R)
         L.getline()
+    # nbfixes (cook/simdef.c:772-798): type, type, then EvdW RvdW of the
+    # pair itself (Epow 1 and 0: scaled by eunit, not combined, not
+    # sign-flipped) and optionally a 1-4 group, which the engine does not use
+    nbfix = []
+    if nnbfixes:
+        L.find("NBFIX" if polar < 0 else "nbfixes")
+        for _ in range(nnbfixes):
+            names = []
+            for _k in range(2):
+                t = L.tok()
+                if t not in types and t.isdigit() and int(t) < len(types):
+                    t = types[int(t)]
+                if t not in types:
+                    raise CookInputError("nbfix: unknown site type %s" % t)
+                names.append(t)
+            nbfix.append([names[0], names[1], float(L.tok()) * escale, float(L.tok())])
+            L.getline()
     species = []
     for sp in range(nspec):
         L.find("species")
@@ -302,6 +319,8 @@ def read_ble(path, form=None):
                             site_xyz=site_xyz, site_excl=site_excl, bonds=bonds))
     ff = dict(types=types, type_mass=mass, type_EvdW_kcal=evdw, type_RvdW=rvdw,
               type_alpha=alphas, species=species, comb_rule=comb_rule)
+    if nbfix:
+        ff["nbfix"] = nbfix
     if form in ("exp6", "buck", "busing"):
         ff["form"] = form
         ff["type_parm"] = [p[0] for p in parm]

@@ -202,6 +202,18 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
             sstab[i * nt + j] = ss
             sstab[j * nt + i] = ss
             corr[i, j] = corr[j, i] = c
+    # nbfixes (sim/setss.c:190-199): the pair's own Emin and RvdW in place of
+    # the combining rule; the identical-site case of the same helper, whose
+    # rule gives Emin and 2 (RvdW/2) = RvdW back
+    for a_, b_, E_, R_ in ff.get("nbfix", []):
+        if form != "lj":
+            raise ValueError("nbfixes: Lennard-Jones form only")
+        i, j = ff["types"].index(a_), ff["types"].index(b_)
+        Ek = E_ * EUNIT_KCAL
+        ss, c = (abi.LJ(), 0.0) if Ek == 0 else prov.setss(Ek, R_ / 2, Ek, R_ / 2, 1, C2)
+        sstab[i * nt + j] = ss
+        sstab[j * nt + i] = ss
+        corr[i, j] = corr[j, i] = c
     bs.keep.append(sstab)
 
     polar = ff.get("polar") or {}

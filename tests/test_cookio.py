@@ -97,7 +97,7 @@ def test_getdata_expressions_and_strings():
 
 
 @pytest.mark.parametrize("change,msg", [
-    (("nnbfixes=0", "nnbfixes=1"), "nbfixes"),
+    (("nnbfixes=0", "nnbfixes=1"), "missing data"),        # an nbfix row announced, none given
     (("nparms=0", "nparms=1"), "site-site form"),
     (("nparms=0", "nparms=2"), "Lennard-Jones"),
 ])
@@ -254,3 +254,49 @@ def test_busing_force_field_file(tmp_path):
     assert bs.struct.sstab_exp6 and t[0].eu == 0 and t[1].eu == 0
     assert t[3].eu * np.exp(t[3].tau * 3.6) == pytest.approx(2 * 12.4 * EUNIT_KCAL, rel=1e-13)
     assert t[3].ru == pytest.approx((40.0 * 2 ** 0.5) ** 2 * EUNIT_KCAL, rel=1e-13)
+
+
+def test_nbfix_table(tmp_path):
+    """nbfixes (cook/simdef.c:772-798, sim/setss.c:190-199): the pair's own
+    Emin and RvdW replace the combining rule for that type pair.  An O-O
+    nbfix of the values the rule gives (Emin_O, 2 RvdW_O) leaves the table
+    and the oracle's forces as they were; other values give cook's LJ
+    constants of initssaux() (sim/lj/sitesite.c:96-104): E4 = -4 Emin,
+    Sq = RvdW^2 2^(-1/3); other pairs keep the rule."""
+    from macsimus_amd.system import EUNIT_KCAL
+    species, types = [mg.spce_species()], ["H", "O"]
+    ble = str(tmp_path / "fix.ble")
+    mg.write_ble(ble, species, types)
+    base = cookio.read_ble(ble)
+    E_O, R_O = base["type_EvdW_kcal"][1], base["type_RvdW"][1]
+    open(tmp_path / "w.def", "w").write("N[0]=64 L[0]=12.4 L[1]=12.4 L[2]=12.4 cutoff=6 "
+                                        "el.epsr=0.05 el.epsk=0.05;\n")
+    prov = oracle.provider()
+    bs0, _ = cookio.system_from_cook(ble, str(tmp_path / "w.def"), provider=prov)
+    txt = open(ble).read().replace("nnbfixes=0", "nnbfixes=1")
+
+    def with_row(row):
+        open(ble, "w").write(txt.replace("! atom atom Eref R\n", "! atom atom Eref R\n%s\n" % row))
+        return cookio.system_from_cook(ble, str(tmp_path / "w.def"), provider=prov)[0]
+
+    bs1 = with_row("O O %.15g %.15g" % (E_O, 2 * R_O))
+    assert cookio.read_ble(ble)["nbfix"] == [["O", "O", E_O, 2 * R_O]]
+    t0, t1 = bs0.info["sstab"][3], bs1.info["sstab"][3]
+    for k in ("Sq", "E4", "E24", "E48", "C1q", "C2q", "A", "A4"):
+        assert getattr(t1, k) == pytest.approx(getattr(t0, k), rel=1e-13), k
+    rng = np.random.default_rng(3)
+    r = rng.uniform(0, 12.4, (192, 3))
+    f0, o0 = oracle.forces(bs0, r, 1)
+    f1, o1 = oracle.forces(bs1, r, 1)
+    assert np.abs(f1 - f0).max() < 1e-9 * np.abs(f0).max()
+    bs2 = with_row("1 1 -0.2 3.4")                  # type numbers, other values
+    t2 = bs2.info["sstab"][3]
+    assert t2.E4 == pytest.approx(0.8 * EUNIT_KCAL, rel=1e-13)
+    assert t2.Sq == pytest.approx(3.4 ** 2 * 2 ** (-1 / 3), rel=1e-13)
+    for k in (0, 1, 2):
+        assert bs2.info["sstab"][k].E4 == bs0.info["sstab"][k].E4
+    # the exp-6 form refuses nbfixes
+    ble6, _, _ = exp6_ble(tmp_path, {"H": 14.0, "O": 13.5})
+    open(ble6, "w").write(open(ble6).read().replace("nnbfixes=0", "nnbfixes=1"))
+    with pytest.raises(cookio.CookInputError):
+        cookio.read_ble(ble6)
