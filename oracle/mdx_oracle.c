@@ -22,6 +22,10 @@
  *   LJ + Coulomb pair kernel        cook/ljqqnp.c:43-259, sim/intermac.h:47-64
  *   SHAKE + leap-frog + Berendsen   sim/constrd.c:622-1081, sim/verlet.c, sim/virc.c,
  *                                   sim/shakev3.c, sim/thermo.c:141-160, sim/constrd.c:98-107
+ *   other site-site forms           exp-6 sim/exp6/sitesite.{c,h}, Buckingham sim/buck,
+ *                                   Busing sim/busing (POW = 0), all in the exp-6 table
+ *   RDF, Nose-Hoover, frictions     cook/ljqqnp.c:218-225, sim/trvpscale.c, sim/thermo.c:107-135
+ * (these later functions state their own citations and how they are pinned)
  * Everything is double precision, serial except the k-space sums, which
  * split the charges over OpenMP threads as cook's PARALLEL=1 does
  * (sim/ewald.c:197-244).
