@@ -51,6 +51,9 @@ def parse():
                     help="W,PCOLS,KZC,KPARTS[,MFMA]: stencil, stencil columns per pair part, "
                          "k-space kz chunk and parts, pass 2 on the matrix cores (mixed) instead "
                          "of the autotuner (c4: W,KZC,KPARTS)")
+    ap.add_argument("--kpass1", type=int, default=-1, choices=[-1, 0, 8, 16],
+                    help="mixed mode: k-space pass 1 as FMA chains (0) or on the matrix cores "
+                         "with 16 or 8 tiles per wave (-1: as pinned or tuned)")
     ap.add_argument("--tune", action="store_true",
                     help="c2: run the autotuner instead of the profiled configuration")
     ap.add_argument("--forces-only", type=int, default=0,
@@ -285,6 +288,8 @@ def main():
         # pair-kernel parts and the k-space force kernel's kz chunk and work
         # split on this configuration (HIP events)
         tune = eng.autotune(wl["r_full"], reps=2)[:4]
+    if a.kpass1 >= 0 and mode == abi.MDX_MIXED:
+        eng.set_kspace_pass1(a.kpass1)
     info = eng.info()
     if a.forces_only:
         for _ in range(a.forces_only):
