@@ -108,6 +108,26 @@ typedef struct mdx_system_s {
      [ntypes*ntypes] (sstab is then not read).  A batch takes one form
      for all its systems. */
   const mdx_exp6_t *sstab_exp6;
+  /* Ewald corrections beyond the self and dipole terms (sim/ewald.c:
+     477-492, 865-997; sim/ewaldprt.c:13).  A system with a net charge Q
+     (Q/sqrt(sum q^2) > 1e-10; cook's el.epsq is 0) gets cook's background
+     energy -pi/2 (Q/alpha)^2 / V in En_el (not in Pvir); it needs el_bg
+     and a tin-foil epsinf (|epsinf| >= 1e16), else MDX_E_ARG, as cook's
+     ERROR (sim/siminit.c:152-160, sim/ewaldprt.c:16-25).
+     el_corr: el.corr, the Yeh-Berkowitz slab correction in place of the
+       epsinf dipole term: bit 1, 2, 4 = x, y, z; +8 adds only its energy
+       and pressure, not its force; needs |epsinf| >= 1e15.
+     el_bg: el.bg (0, 1, 2); nonzero allows a net charge and, with
+       el_corr, adds the force q 4 pi Q r_k / V of the charged background
+       on each charge in the el_corr directions (and its virial),
+       Ballenegger et al.'s term of sim/ewald.c:901-931.  cook's el.bg = 2
+       neutralizes the charges at set-up (sim/siminit.c:170-199); a caller
+       passes the shifted charges.
+     el_diag: el.diag; 1 adds Ediag, the systematic k-space cutoff term
+       of the energy computed for the box at mdx_create. */
+  int el_corr;
+  int el_bg;
+  int el_diag;
 } mdx_system_t;
 
 /* what forces() leaves in the globals of sim/simglob.h:436-577 */
@@ -152,10 +172,11 @@ int mdx_set_box(mdx_ctx_t *ctx, const double L[3]);
    changed: cook re-allocates linklist()'s and Ewald()'s scratch on such a
    change, cook/linklist.c:88-125, sim/ewald.c:367-377).  Everything
    mdx_create builds is built again; the precision and the settings made by
-   mdx_set_stencil, mdx_set_kchunk, mdx_set_kspace_mfma, mdx_set_overlap,
-   mdx_set_graph, mdx_set_timing and mdx_set_rdf carry over where the new
-   system allows them (the RDF records restart from zero); the MD state
-   must be uploaded again.  On failure the context keeps the old system. */
+   mdx_set_stencil, mdx_set_kchunk, mdx_set_kspace_mfma,
+   mdx_set_kspace_pass1, mdx_set_overlap, mdx_set_graph, mdx_set_timing and
+   mdx_set_rdf carry over where the new system allows them (the RDF records
+   restart from zero); the MD state must be uploaded again.  On failure
+   the context keeps the old system. */
 int mdx_resize(mdx_ctx_t *ctx, const mdx_system_t *sys);
 
 /* ---- device-resident MD (the captured step) ---------------------------

@@ -54,6 +54,7 @@ class System(C.Structure):
         ("mass", _dp), ("ncons", C.c_int), ("cons_first", _ip),
         ("cons_ij", _ip), ("cons_bondq", _dp), ("mol_omega", _dp),
         ("sstab_exp6", C.POINTER(Exp6)),
+        ("el_corr", C.c_int), ("el_bg", C.c_int), ("el_diag", C.c_int),
     ]
 
 

@@ -64,6 +64,9 @@ struct Batch {
   hipStream_t stream = nullptr;
   std::vector<DSys> hs;
   std::vector<double> sumq2, kappa;
+  // Ewald energy terms of each system: background of a net charge
+  // -pi/2 (Q/alpha)^2 / V and el.diag's Ediag (sim/ewald.c:983-997)
+  std::vector<double> ebg;
   DBuf<char> dsys;
   DBuf<double> r, f, r1, q, mass;        // padded rows
   DBuf<int> type, sysid;                 // padded rows

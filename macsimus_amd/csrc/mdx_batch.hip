@@ -448,7 +448,7 @@ void fill_sys_out(const Batch &b, int s, const double *a, mdx_out_t *out) {
   for (int i = 0; i < 3; i++) Pk[i] += Ek + Edip;
   Pk[0] -= S.cdip * M[0] * M[0]; Pk[1] -= S.cdip * M[1] * M[1]; Pk[2] -= S.cdip * M[2] * M[2];
   Pk[3] -= S.cdip * M[1] * M[2]; Pk[4] -= S.cdip * M[2] * M[0]; Pk[5] -= S.cdip * M[0] * M[1];
-  out->En_kspace = S.ks >= 0 ? Ek + Edip - S.alpha / std::sqrt(kPi) * b.sumq2[s] : 0.0;
+  out->En_kspace = S.ks >= 0 ? Ek + Edip - S.alpha / std::sqrt(kPi) * b.sumq2[s] + b.ebg[s] : 0.0;
   out->En_el = out->En_kspace + a[B_EEL];
   out->En_pot = a[B_U];
   out->En_vir = a[B_VIR];
@@ -526,6 +526,7 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   int pad = 0, srt = 0;
   b.hs.resize(nsys);
   b.sumq2.assign(nsys, 0.0);
+  b.ebg.assign(nsys, 0.0);
   b.kappa.assign(nsys, 0.0);
   for (int s = 0; s < nsys; s++) {
     const mdx_system_t &y = sys[s];
@@ -554,7 +555,21 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
     lj.insert(lj.end(), t.begin(), t.end());
     // spline intervals up to the cutoff (index (int)(r^2 sgrid), r^2 <
     // cutoff^2); none for a system without charges
-    for (int i = 0; i < y.ns; i++) b.sumq2[s] += y.charge[i] * y.charge[i];
+    double qnet = 0;
+    for (int i = 0; i < y.ns; i++) {
+      b.sumq2[s] += y.charge[i] * y.charge[i];
+      qnet += y.charge[i];
+    }
+    // Ewald corrections: the slab form is not carried by the batch; a net
+    // charge adds the background energy, el_diag the diagonal term
+    if (y.el_corr)
+      return bail(fail(MDX_E_ARG, "mdx_batch_create: the Yeh-Berkowitz correction (el_corr) is not "
+                                  "supported in a batch"));
+    if (int rc0 = ewald_check(0, y.el_bg, y.epsinf, qnet, b.sumq2[s], "mdx_batch_create")) return bail(rc0);
+    if (b.sumq2[s] > 0) {
+      b.ebg[s] = -kPi / 2 * (qnet / y.alpha) * (qnet / y.alpha) / (S.L * S.L * S.L);
+      if (y.el_diag == 1) b.ebg[s] += ewald_ediag(y.L, y.kappa, y.alpha, b.sumq2[s]);
+    }
     const int nerf = b.sumq2[s] > 0 ? (int)(S.cq * y.erfc_sgrid) + 1 : 0;
     if (nerf > y.erfc_n) return bail(fail(MDX_E_ARG, "mdx_batch_create: spline table shorter than the cutoff"));
     S.erf_off = (int)erf.size();

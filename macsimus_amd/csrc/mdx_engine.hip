@@ -946,23 +946,25 @@ void fill_out(const Ctx &c, const double *acc, int measure, mdx_out_t *out) {
   if (!measure) return;
   const double V = c.L[0] * c.L[1] * c.L[2];
   // k-space: Energy = sum/(4 pi); diag += Energy; all -= 2/(4 pi) PVIR
-  // (sim/ewald.c:853-854); dipole term for epsinf (sim/ewald.c:885-897)
+  // (sim/ewald.c:853-854); cell-dipole term (epsinf or Yeh-Berkowitz,
+  // with the charged-background field of el.bg, sim/ewald.c:865-931):
+  // Edip on the diagonal, then sum_i q_i ekx_a r_i,b with ekx = -cm M +
+  // cb r, i.e. -cm_a M_a M_b + cb_a S_ab
   const double Ek = acc[ACC_EK] / (4 * kPi);
-  const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
-  const double M2 = acc[ACC_M0] * acc[ACC_M0] + acc[ACC_M0 + 1] * acc[ACC_M0 + 1] +
-                    acc[ACC_M0 + 2] * acc[ACC_M0 + 2];
-  const double Edip = cdip / 2 * M2;
+  double cm[3], ce[3], cb[3];
+  ewald_dipole_coefs(c.el_corr, c.el_bg, c.epsinf, c.qnet, V, cm, ce, cb);
+  const double *M = acc + ACC_M0, *S = acc + ACC_S0;
+  const double Edip = ce[0] / 2 * M[0] * M[0] + ce[1] / 2 * M[1] * M[1] + ce[2] / 2 * M[2] * M[2];
   double Pk[6];
   for (int i = 0; i < 6; i++) Pk[i] = -2 / (4 * kPi) * acc[ACC_PK0 + i];
-  for (int i = 0; i < 3; i++) Pk[i] += Ek + Edip;
-  // dipole-field virial sum_i q_i ek r_i with ek=-c M: -c M M^T
-  Pk[0] -= cdip * acc[ACC_M0] * acc[ACC_M0];
-  Pk[1] -= cdip * acc[ACC_M0 + 1] * acc[ACC_M0 + 1];
-  Pk[2] -= cdip * acc[ACC_M0 + 2] * acc[ACC_M0 + 2];
-  Pk[3] -= cdip * acc[ACC_M0 + 1] * acc[ACC_M0 + 2];
-  Pk[4] -= cdip * acc[ACC_M0 + 2] * acc[ACC_M0];
-  Pk[5] -= cdip * acc[ACC_M0] * acc[ACC_M0 + 1];
-  out->En_kspace = Ek + Edip - c.alphar / std::sqrt(kPi) * c.sumq2;
+  for (int i = 0; i < 3; i++) Pk[i] += Ek + Edip - cm[i] * M[i] * M[i] + cb[i] * S[i];
+  Pk[3] += -cm[1] * M[1] * M[2] + cb[1] * S[3];
+  Pk[4] += -cm[2] * M[2] * M[0] + cb[2] * S[4];
+  Pk[5] += -cm[0] * M[0] * M[1] + cb[0] * S[5];
+  // the energy Ewald(2) returns (sim/ewald.c:983-997): diagonal term,
+  // k-space, background of a net charge, self term, dipole term
+  out->En_kspace = (c.el_diag == 1 ? c.Ediag : 0.0) + Ek + c.bgE / V -
+                   c.alphar / std::sqrt(kPi) * c.sumq2 + Edip;
   out->En_el = out->En_kspace + acc[ACC_EEL];
   out->En_pot = acc[ACC_U];
   out->En_vir = acc[ACC_VIR];
@@ -981,9 +983,20 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
       !s->charge || !s->excl || !(s->sstab || s->sstab_exp6) || !s->erfc_tab || s->erfc_n <= 0)
     return fail(MDX_E_ARG, "mdx_create: incomplete system");
   if (s->ntypes > 0xffff) return fail(MDX_E_ARG, "mdx_create: too many site types");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-    return fail(MDX_E_NODEV, "mdx_create: no HIP device");
+  {
+    // host checks of the site data before any device work: the type
+    // indexes the site-site table on the device; cook's charge and slab
+    // checks of the Ewald set-up
+    double q2 = 0, qn = 0;
+    for (int i = 0; i < s->ns; i++) {
+      if (s->site_type[i] < 0 || s->site_type[i] >= s->ntypes)
+        return fail(MDX_E_ARG, "mdx_create: site type out of range (site " + std::to_string(i) + ")");
+      const double qt = s->charge[i] + (s->chargepol ? s->chargepol[i] : 0);
+      q2 += qt * qt;
+      qn += qt;
+    }
+    if (int rc0 = ewald_check(s->el_corr, s->el_bg, s->epsinf, qn, q2, "mdx_create")) return rc0;
+  }
   for (int n = 0; n < s->nmol; n++) {
     if (s->mol_first[n + 1] - s->mol_first[n] > 32)
       return fail(MDX_E_ARG, "mdx_create: molecules with more than 32 sites are not supported");
@@ -995,6 +1008,9 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     if (!(s->L[d] > 2 * s->cutoff)) return fail(MDX_E_ARG, "mdx_create: cutoff >= L/2");
   if ((int)(s->cutoff * s->cutoff * s->erfc_sgrid) >= s->erfc_n)
     return fail(MDX_E_ARG, "mdx_create: erfc table shorter than the cutoff");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(MDX_E_NODEV, "mdx_create: no HIP device");
 
   mdx_ctx_s *h = new mdx_ctx_s;
   Ctx &c = h->c;
@@ -1030,14 +1046,23 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   for (int n = 0; n < c.nmol; n++)
     for (int i = s->mol_first[n]; i < s->mol_first[n + 1]; i++) mol[i] = n;
   c.sumq2 = 0;
+  c.qnet = 0;
   for (int i = 0; i < c.ns; i++) {
     const double qp = c.polar ? s->chargepol[i] : 0;
     if (s->charge[i] != 0) qsite.push_back(i);
     if (qp != 0) qsite.push_back(c.ns + i);
     // self term: squared total site charge (sim/ewald.c:402-424)
     c.sumq2 += (s->charge[i] + qp) * (s->charge[i] + qp);
+    c.qnet += s->charge[i] + qp;
   }
   c.nq = (int)qsite.size();
+  // Ewald corrections: the background energy of a net charge
+  // (sim/ewaldprt.c:13) and Ediag of this box (sim/ewald.c:477-492)
+  c.el_corr = s->el_corr;
+  c.el_bg = s->el_bg != 0;
+  c.el_diag = s->el_diag;
+  c.bgE = -kPi / 2 * (c.qnet / c.alpha) * (c.qnet / c.alpha);
+  c.Ediag = c.nq ? ewald_ediag(c.L, c.kappa, c.alpha, c.sumq2) : 0;
   const int nrow = c.polar ? 2 * c.ns : c.ns;
   // exclusion masks made symmetric: cook finds an exclusion in the table
   // of the later site of the pair (cook/ljqqnp.c:103-118), a caller may set
@@ -1202,20 +1227,23 @@ extern "C" int mdx_resize(mdx_ctx_t *h, const mdx_system_t *s) {
   mdx_ctx_t *t = nullptr;
   int rc = mdx_create(&t, s, c.mode);
   if (rc) return rc;
-  const bool allpair = c.allpair, kmfma = c.kmfma, overlap = c.overlap, graph = c.use_graph;
-  const bool timing = c.timing;
-  const int w = c.wst, pcols = c.pcols, kzc = c.kzc, kparts = c.kparts, rdf_grid = c.rdf_grid;
-  const double rdf_cutoff = c.rdf_cutoff;
+  // the settings go to the new context before it replaces the old one, so
+  // that a failing setter leaves the old system in place
+  Ctx &n = t->c;
+  n.kmfma = c.kmfma;
+  n.p1nt = c.p1nt;                     // same precision, so still valid
+  n.overlap = c.overlap;
+  n.use_graph = c.use_graph;
+  n.timing = c.timing;
+  if (mdx_set_stencil(t, c.allpair ? 0 : c.wst, c.pcols)) (void)mdx_set_stencil(t, 2, 0);
+  if ((n.nq && (rc = mdx_set_kchunk(t, c.kzc, c.kparts))) ||
+      (c.rdf_grid && (rc = mdx_set_rdf(t, c.rdf_grid, c.rdf_cutoff)))) {
+    const std::string msg = g_last_error;
+    mdx_destroy(t);
+    return fail(rc, msg);
+  }
   std::swap(h->c, t->c);
   mdx_destroy(t);                      // the old system's buffers, streams, graph
-  Ctx &n = h->c;
-  n.kmfma = kmfma;
-  n.overlap = overlap;
-  n.use_graph = graph;
-  n.timing = timing;
-  if (mdx_set_stencil(h, allpair ? 0 : w, pcols)) (void)mdx_set_stencil(h, 2, 0);
-  if (n.nq && (rc = mdx_set_kchunk(h, kzc, kparts))) return rc;
-  if (rdf_grid && (rc = mdx_set_rdf(h, rdf_grid, rdf_cutoff))) return rc;
   return 0;
 }
 

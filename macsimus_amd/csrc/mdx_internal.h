@@ -16,6 +16,23 @@ namespace mdx {
 // records the message of the last failure of this thread, returns code
 int fail(int code, const std::string &msg);
 
+// Ediag of Ewald(0) (sim/ewald.c:477-492): the k-space terms between
+// kappa and sqrt(kappa^2+alpha^2), times sumq/(2 pi V), for the box L
+double ewald_ediag(const double L[3], double kappa, double alpha, double sumq);
+
+// coefficients of the cell-dipole terms (sim/ewald.c:865-931) per
+// direction d: the field on a charge is -cm[d] M_d + cb[d] r_d, the
+// energy sum_d ce[d]/2 M_d^2; el_corr selects the Yeh-Berkowitz form,
+// otherwise the epsinf form; cb is the charged-background term of el_bg
+void ewald_dipole_coefs(int el_corr, int el_bg, double epsinf, double qnet, double V,
+                        double cm[3], double ce[3], double cb[3]);
+
+// the checks cook's set-up makes on these settings (sim/siminit.c:
+// 152-160, sim/ewaldprt.c:16-25, sim/main.c:719-721): a charged system
+// needs el.bg and a tin-foil epsinf, the slab correction an infinite one
+int ewald_check(int el_corr, int el_bg, double epsinf, double qnet, double sumq2,
+                const char *who);
+
 #ifdef __HIPCC__
 
 #define MDX_HIP(call)                                                     \
@@ -60,7 +77,8 @@ enum Acc {
   ACC_EK,         // k-space energy sum_k w e 4 sum T^2
   ACC_PK0,        // 6 k-space tensor sums (before the 2/(4 pi) factor)
   ACC_M0 = ACC_PK0 + 6,  // dipole moment M (3)
-  ACC_SUMQ = ACC_M0 + 3, // sum q^2 (constant, kept for checking)
+  ACC_S0 = ACC_M0 + 3,   // second moments sum q r_a r_b (6: xx yy zz yz zx xy)
+  ACC_SUMQ = ACC_S0 + 6, // sum q^2 (constant, kept for checking)
   ACC_N
 };
 
@@ -145,6 +163,11 @@ struct Ctx {
   double L[3] = {0, 0, 0};
   double cutoff = 0, alpha = 0, alphar = 0, kappa = 0, epsinf = 3e33;
   double sumq2 = 0;
+  // Ewald corrections (sim/ewald.c:477-492, 865-997): el.corr, el.bg,
+  // el.diag, the net charge, the background energy -pi/2 (Q/alpha)^2
+  // (divided by V at each call) and Ediag of the box at set-up
+  int el_corr = 0, el_bg = 0, el_diag = 0;
+  double qnet = 0, bgE = 0, Ediag = 0;
   int cook_ncell[3] = {0, 0, 0};
   // site data
   DBuf<double> r, f;          // [3ns] input positions, output forces

@@ -404,23 +404,27 @@ __global__ __launch_bounds__(kBlock) void k_kcoef(
   }
 }
 
-// dipole moment M = sum q r of the unwrapped sites (sim/ewald.c:644)
+// dipole moment M = sum q r of the unwrapped sites (sim/ewald.c:644) and
+// the second moments S_ab = sum q r_a r_b (xx yy zz yz zx xy) that the
+// virial of the charged-background field needs (sim/ewald.c:909-931)
 __global__ __launch_bounds__(kBlock) void k_dipole(int nq, const double4 *__restrict__ qpos,
                                                    double *__restrict__ part) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
-  double m[3] = {0, 0, 0};
+  double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (j < nq) {
     const double4 p = qpos[j];
     m[0] = p.w * p.x; m[1] = p.w * p.y; m[2] = p.w * p.z;
+    m[3] = m[0] * p.x; m[4] = m[1] * p.y; m[5] = m[2] * p.z;
+    m[6] = m[1] * p.z; m[7] = m[2] * p.x; m[8] = m[0] * p.y;
   }
-  __shared__ double red[kBlock / 64][3];
+  __shared__ double red[kBlock / 64][9];
   const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < 9; d++) {
     const double s = wave_sum(m[d]);
     if (ln == 0) red[wv][d] = s;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < 9) {
     double s = 0;
     for (int w = 0; w < kBlock / 64; w++) s += red[w][threadIdx.x];
     part[(size_t)blockIdx.x * mdx::ACC_N + mdx::ACC_M0 + threadIdx.x] += s;
@@ -652,11 +656,16 @@ __global__ __launch_bounds__(kBlock, 4) void k_kforce_mfma(
   o[2] = Fz;
 }
 
+// the cell-dipole field on a charge at r: ek_d = -cm_d M_d + cb_d r_d
+struct DipField {
+  double cm[3], cb[3];
+};
+
 // sums the parts of pass 2 in a fixed order and adds -q grad Psi and the
-// dipole field to the force rows of the charges
+// cell-dipole field to the force rows of the charges
 __global__ void k_kforce_sum(int nq, int nparts, const double *__restrict__ kpf,
                              const int *__restrict__ qsite, const double4 *__restrict__ qpos,
-                             const double *__restrict__ acc, double cdip,
+                             const double *__restrict__ acc, DipField df,
                              double *__restrict__ f) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nq) return;
@@ -668,9 +677,11 @@ __global__ void k_kforce_sum(int nq, int nparts, const double *__restrict__ kpf,
     F[2] += o[2];
   }
   const int site = qsite[j];   // row of f (Drudes after the centres)
-  const double q = qpos[j].w;
+  const double4 p = qpos[j];
+  const double q = p.w, R[3] = {p.x, p.y, p.z};
   const double *M = acc + mdx::ACC_M0;
-  for (int d = 0; d < 3; d++) f[3 * site + d] += -q * F[d] - q * cdip * M[d];
+  for (int d = 0; d < 3; d++)
+    f[3 * site + d] += -q * F[d] + q * (-df.cm[d] * M[d] + df.cb[d] * R[d]);
 }
 
 template <typename R>
@@ -716,11 +727,9 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
       c.d_row_kzmax.p, c.d_row_coef0.p, c.tsum.p, c.L[0], c.L[1], c.L[2], c.alpha, coef, c.part.p,
       measure, c.d_grp_start.p, c.d_grp_base.p, coefg);
   MDX_HIP(hipGetLastError());
-  const double V = c.L[0] * c.L[1] * c.L[2];
-  const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
   k_dipole<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, qpos, c.part.p);
-  // the dipole sums of k_dipole's workgroups
-  k_reduce_cols<<<3, kBlock, 0, st>>>(c.part.p, nblk(c.nq), ACC_M0, c.acc.p);
+  // the dipole and second-moment sums of k_dipole's workgroups
+  k_reduce_cols<<<9, kBlock, 0, st>>>(c.part.p, nblk(c.nq), ACC_M0, c.acc.p);
   if (c.timing) { c.tm.stop(T_KCOEF, st); c.tm.start(T_KPASS2, st); }
   if constexpr (sizeof(R) == 4) {
     if (c.kmfma) {
@@ -757,7 +766,6 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   }
 #undef MDX_KF
   MDX_HIP(hipGetLastError());
-  (void)cdip;
   return 0;
 }
 
@@ -765,9 +773,11 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
 int add_kspace(mdx::Ctx &c, hipStream_t st) {
   using namespace mdx;
   const double V = c.L[0] * c.L[1] * c.L[2];
-  const double cdip = 4 * kPi / (2 * c.epsinf + 1) / V;
+  DipField df;
+  double ce[3];
+  ewald_dipole_coefs(c.el_corr, c.el_bg, c.epsinf, c.qnet, V, df.cm, ce, df.cb);
   k_kforce_sum<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, c.nkpart, c.kpf.p, c.q_site.p,
-                                             (const double4 *)c.qpos.p, c.acc.p, cdip, c.f.p);
+                                             (const double4 *)c.qpos.p, c.acc.p, df, c.f.p);
   MDX_HIP(hipGetLastError());
   if (c.timing) c.tm.stop(T_KPASS2, st);
   return 0;

@@ -439,3 +439,69 @@ extern "C" int mdx_pressure(const mdx_step_out_t *so, const double L[3], double 
   p->Pevir_c = p->Pevir + cP;
   return 0;
 }
+
+namespace mdx {
+
+// Ediag, sim/ewald.c:477-492: the omitted k-space terms with kappa^2 <
+// k^2 < kappa^2 + alpha^2 over the full k lattice, each exp(-pi^2 k^2 /
+// alpha^2)/k^2, times sumq/(2 pi V); cook truncates K = (int)(kappax L)
+double ewald_ediag(const double L[3], double kappa, double alpha, double sumq) {
+  const double pi = 3.14159265358979323846;
+  const double qkx = kappa * kappa + alpha * alpha, kx_ = std::sqrt(qkx);
+  const double q = -(pi / alpha) * (pi / alpha), kq = kappa * kappa;
+  const int K[3] = {(int)(kx_ * L[0]), (int)(kx_ * L[1]), (int)(kx_ * L[2])};
+  double E = 0;
+  for (int kx = -K[0]; kx <= K[0]; kx++)
+    for (int ky = -K[1]; ky <= K[1]; ky++)
+      for (int kz = -K[2]; kz <= K[2]; kz++) {
+        const double a = kx / L[0], b = ky / L[1], c = kz / L[2];
+        const double kk = a * a + b * b + c * c;
+        if (kk > kq && kk < qkx) E += std::exp(kk * q) / kk;
+      }
+  return E * sumq / (2 * pi * L[0] * L[1] * L[2]);
+}
+
+void ewald_dipole_coefs(int el_corr, int el_bg, double epsinf, double qnet, double V,
+                        double cm[3], double ce[3], double cb[3]) {
+  const double pi = 3.14159265358979323846;
+  for (int d = 0; d < 3; d++) cm[d] = ce[d] = cb[d] = 0;
+  if (el_corr) {
+    // Yeh-Berkowitz in the directions of el.corr (sim/ewald.c:869-882);
+    // +8: energy and pressure only; with el.bg the background field
+    // 4 pi Q r_d / V in the same directions (sim/ewald.c:901-931)
+    const double c = 4 * pi / V;
+    for (int d = 0; d < 3; d++)
+      if (el_corr & 1 << d) {
+        ce[d] = c;
+        if (!(el_corr & 8)) cm[d] = c;
+        if (el_bg) cb[d] = 4 * pi * qnet / V;
+      }
+  } else {
+    // dielectric boundary epsinf (sim/ewald.c:884-887)
+    const double c = 4 * pi / (2 * epsinf + 1) / V;
+    for (int d = 0; d < 3; d++) cm[d] = ce[d] = c;
+  }
+}
+
+int ewald_check(int el_corr, int el_bg, double epsinf, double qnet, double sumq2,
+                const char *who) {
+  if (el_corr < 0 || el_corr > 15)
+    return fail(MDX_E_ARG, std::string(who) + ": el_corr must be in 0..15");
+  if (el_bg < 0 || el_bg > 2) return fail(MDX_E_ARG, std::string(who) + ": el_bg must be 0, 1 or 2");
+  if (el_corr && std::fabs(epsinf) < 1e15)
+    return fail(MDX_E_ARG, std::string(who) +
+                               ": the Yeh-Berkowitz correction (el_corr) needs epsinf = infinity");
+  // cook's neutrality test Q/sqrt(sumq) > el.epsq (sim/siminit.c:152-160,
+  // sim/ewaldprt.c:16): el.epsq = 0 there, a relative charge below 1e-10
+  // counts as rounding here
+  if (qnet * qnet > 1e-20 * sumq2) {
+    if (!el_bg)
+      return fail(MDX_E_ARG, std::string(who) + ": the system is charged and el_bg = 0");
+    if (std::fabs(epsinf) < 1e16)
+      return fail(MDX_E_ARG, std::string(who) +
+                                 ": a charged system needs a tin-foil epsinf (|epsinf| >= 1e16)");
+  }
+  return 0;
+}
+
+}  // namespace mdx

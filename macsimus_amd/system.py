@@ -129,7 +129,7 @@ class BuiltSystem:
 def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
                  rplus=2.5, rshift=3, epsinf=3e33, ncell=(0, 0, 0),
                  alpha=None, kappa=None, lj_cutoff=None, provider=None,
-                 omega=0.6, exp6_evdw_scale=1.0):
+                 omega=0.6, exp6_evdw_scale=1.0, el_corr=0, el_bg=0, el_diag=0):
     """ff: force-field dict as stored in tests/golden/*.json['forcefield']:
          types, type_mass, type_EvdW_kcal, type_RvdW,
          species[{site_type, site_charge_e, site_excl, bonds}]
@@ -147,7 +147,11 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
        Busing form (sim/busing, POW = 0), also in the exp-6 table: f rho_i
        [kcal/mol] in type_EvdW_kcal (negative: the site interacts with
        nothing), R_i [A] in type_RvdW, C_i [sqrt(kcal/mol) A^3] in
-       type_parm (sim/busing/sitesite.c:5-16), comb_rule 0."""
+       type_parm (sim/busing/sitesite.c:5-16), comb_rule 0.
+       el_corr, el_bg, el_diag: cook's el.corr (Yeh-Berkowitz slab
+       correction, bits x=1 y=2 z=4, +8 energy and pressure only), el.bg
+       (its charged-background force) and el.diag (1: Ediag in the
+       energy), sim/ewald.c:477-492, 865-997."""
     prov = provider or engine_provider()
     bs = BuiltSystem()
     S = bs.struct
@@ -292,6 +296,9 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     S.cons_ij = bs.ptr(np.array(cons_ij if cons_ij else [0, 0], np.int32), C.c_int)
     S.cons_bondq = bs.ptr(np.array(cons_bondq if cons_bondq else [1.0]), C.c_double)
     S.mol_omega = bs.ptr(np.full(max(nmol, 1), omega), C.c_double)
+    S.el_corr = int(el_corr)
+    S.el_bg = int(el_bg)
+    S.el_diag = int(el_diag)
     # No.f: 3 per site minus constraints (cook subtracts 3 for the total
     # momentum only when it is conserved; callers can override)
     bs.info = dict(alpha=alpha, kappa=kappa, nk=nk, ncell=list(ncell),

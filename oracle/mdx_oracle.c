@@ -577,6 +577,24 @@ static void orc_ewald_pass(int pass, const orc_kset *K, const double *L,
 
 /* k-space part: returns En.el of Ewald(2) (measure) and adds forces of
    the charges to f[idx[i]]; energy/tensor bookkeeping of pass 2m */
+/* Ediag of Ewald(0), sim/ewald.c:477-492: the k-vectors of the full
+   lattice with kappa^2 < k^2 < kappa^2 + alpha^2, exp(-pi^2 k^2/alpha^2)/k^2
+   each, times charges.sumq/(2 pi V) */
+static double orc_ediag(const mdx_system_t *S, double sumq)
+{
+  const double *L = S->L;
+  double qkappax = sqr(S->kappa) + sqr(S->alpha), kappax = sqrt(qkappax), kk;
+  double q = -sqr(PI / S->alpha), E = 0;
+  int Kx = kappax * L[0], Ky = kappax * L[1], Kz = kappax * L[2], kx, ky, kz;
+  for (kx = -Kx; kx <= Kx; kx++)
+    for (ky = -Ky; ky <= Ky; ky++)
+      for (kz = -Kz; kz <= Kz; kz++) {
+        kk = sqr(kx / L[0]) + sqr(ky / L[1]) + sqr(kz / L[2]);
+        if (kk > sqr(S->kappa) && kk < qkappax) E += exp(kk * q) / kk;
+      }
+  return E * sumq / (2 * PI * L[0] * L[1] * L[2]);
+}
+
 static double orc_ewald(const mdx_system_t *S, int nq, const int *idx,
                         const double *q, const double (*pos)[3], double (*f)[3],
                         int measure, double Pvir[6], int nthreads, double sumq_self)
@@ -676,35 +694,60 @@ static double orc_ewald(const mdx_system_t *S, int nq, const int *idx,
     for (i = 0; i < 3; i++) Pvir[i] += Energy;
     for (i = 0; i < 6; i++) Pvir[i] -= 2 / (4 * PI) * PVIR[i];
   }
-  /* dipole term, tin-foil limit for epsinf -> inf (sim/ewald.c:865-897) */
+  /* cell-dipole term (sim/ewald.c:865-897): the Yeh-Berkowitz slab form
+     in the el.corr directions (+8: no force) or the epsinf form; with el.bg
+     and el.corr the charged-background field in those directions, added
+     per charge with its virial as ADDFORCES of sim/ewald.c:901-931 does */
   {
-    double c = 4 * PI / (2 * S->epsinf + 1) / V, ek[3], Edip;
-    ek[0] = -c * M[0]; ek[1] = -c * M[1]; ek[2] = -c * M[2];
-    Edip = c / 2 * (sqr(M[0]) + sqr(M[1]) + sqr(M[2]));
+    double c, ek[3] = {0, 0, 0}, cbg = 0, Edip = 0, qsum = 0;
+    int k;
+    for (i = 0; i < nq; i++) qsum += q[i];
+    if (S->el_corr) {
+      c = 4 * PI / V;
+      for (k = 0; k < 3; k++)
+        if (S->el_corr & 1 << k) {
+          if (!(S->el_corr & 8)) ek[k] = -c * M[k];
+          Edip += c / 2 * sqr(M[k]);
+        }
+    } else {
+      c = 4 * PI / (2 * S->epsinf + 1) / V;
+      ek[0] = -c * M[0]; ek[1] = -c * M[1]; ek[2] = -c * M[2];
+      Edip = c / 2 * (sqr(M[0]) + sqr(M[1]) + sqr(M[2]));
+    }
     if (measure) for (i = 0; i < 3; i++) Pvir[i] += Edip;
+    if (S->el_bg && S->el_corr) cbg = 4 * PI * qsum / V;
     for (i = 0; i < nq; i++) {
-      double *fi = f[idx[i]];
-      fi[0] += F[i][0] + q[i] * ek[0];
-      fi[1] += F[i][1] + q[i] * ek[1];
-      fi[2] += F[i][2] + q[i] * ek[2];
+      double *fi = f[idx[i]], ekx[3];
+      for (k = 0; k < 3; k++) {
+        ekx[k] = ek[k];
+        if (cbg != 0 && (S->el_corr & 1 << k)) ekx[k] += cbg * pos[i][k];
+      }
+      fi[0] += F[i][0] + q[i] * ekx[0];
+      fi[1] += F[i][1] + q[i] * ekx[1];
+      fi[2] += F[i][2] + q[i] * ekx[2];
       if (measure) {
-        Pvir[0] += q[i] * ek[0] * pos[i][0];
-        Pvir[1] += q[i] * ek[1] * pos[i][1];
-        Pvir[2] += q[i] * ek[2] * pos[i][2];
-        Pvir[3] += q[i] * ek[1] * pos[i][2];
-        Pvir[4] += q[i] * ek[2] * pos[i][0];
-        Pvir[5] += q[i] * ek[0] * pos[i][1];
+        Pvir[0] += q[i] * ekx[0] * pos[i][0];
+        Pvir[1] += q[i] * ekx[1] * pos[i][1];
+        Pvir[2] += q[i] * ekx[2] * pos[i][2];
+        Pvir[3] += q[i] * ekx[1] * pos[i][2];
+        Pvir[4] += q[i] * ekx[2] * pos[i][0];
+        Pvir[5] += q[i] * ekx[0] * pos[i][1];
       }
     }
-    Energy += Edip;
+    free(ex); free(ey); free(ez); free(qt); free(F); free(Qk); free(Qth); free(ekk);
+    orc_kset_free(&K);
+    if (!measure) return 0;
+    /* POLAR: sum of squared total site charges; the centre-Drude pair of a
+       site is corrected in selffield() (sim/ewald.c:979-981, sim/scf.c:143) */
+    if (sumq_self >= 0) sumq = sumq_self;
+    /* the return of Ewald(2), sim/ewald.c:983-997: Ediag for el.diag = 1,
+       the k-space energy, the background term of a net charge
+       charges.bgE/V (bgE = -pi/2 (Q/alpha)^2, sim/ewaldprt.c:13), the self
+       term and the dipole term */
+    return (S->el_diag == 1 ? orc_ediag(S, sumq) : 0.0) + Energy
+           - PI / 2 * sqr(qsum / S->alpha) / V
+           - S->alphar / sqrt(PI) * sumq + Edip;
   }
-  free(ex); free(ey); free(ez); free(qt); free(F); free(Qk); free(Qth); free(ekk);
-  orc_kset_free(&K);
-  if (!measure) return 0;
-  /* POLAR: sum of squared total site charges; the centre-Drude pair of a
-     site is corrected in selffield() (sim/ewald.c:979-981, sim/scf.c:143) */
-  if (sumq_self >= 0) sumq = sumq_self;
-  return Energy - S->alphar / sqrt(PI) * sumq;
 }
 
 /* -------------------------------------------------------- linked cells */

@@ -219,12 +219,16 @@ def ff_arrays(species, types, polar=None):
 
 
 def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
-               extra_def="", polar=None, scf=None, md_steps=0):
+               extra_def="", polar=None, scf=None, md_steps=0, ewald=None):
     """One Ewaldtest force evaluation on cook's own initial configuration.
 
     md_steps>0 first runs that many MD steps (SHAKE, leap-frog, Berendsen)
-    from the lattice start and then evaluates the forces on the result."""
+    from the lattice start and then evaluates the forces on the result.
+    ewald: cook's el.corr, el.bg, el.diag ({"el_corr": 4, ...}), set in the
+    .def deck and stored with the fixture."""
     exe = COOKPOL if polar else COOK
+    if ewald:
+        extra_def = " ".join([extra_def] + ["el.%s=%d" % (k[3:], v) for k, v in ewald.items()])
     with tempfile.TemporaryDirectory() as td:
         write_ble(os.path.join(td, "sys.ble"), species, types, polar)
         lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
@@ -253,7 +257,7 @@ def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
         dump = parse_dump(os.path.join(td, "t.0.dump"), polar=bool(polar))
         meta = parse_prt(os.path.join(td, "t.prt"))
     meta.update(dict(name=name, counts=counts, L=L, epsr=epsr, epsk=epsk,
-                     md_steps=md_steps,
+                     md_steps=md_steps, ewald=ewald or {},
                      forcefield=ff_arrays(species, types, polar)))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **dump)
     with open(os.path.join(HERE, name + ".json"), "w") as f:
@@ -572,6 +576,22 @@ def main(argv):
             "polar306_aspc", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 300], 21.10, 9.0,
             {"O": (1.44, -8.0), "CL": (3.5, -8.0)}, 8),
+        # Ewald corrections of Ewald(2) (sim/ewald.c:865-997): a net charge
+        # of +1 e (3 Na, 2 Cl, el.bg=1) gets the background term; the slab
+        # (Yeh-Berkowitz) correction in z; a charged box with the slab
+        # correction in x and z, its background force and the diagonal term
+        "brine505_charged": lambda: force_case(
+            "brine505_charged", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 2, 500], 24.86, 9.0, 0.05, 0.05, md_steps=10,
+            ewald={"el_bg": 1}),
+        "brine506_slab": lambda: force_case(
+            "brine506_slab", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 0.05, 0.05, md_steps=10,
+            ewald={"el_corr": 4}),
+        "brine505_slabbg": lambda: force_case(
+            "brine505_slabbg", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 2, 500], 24.86, 9.0, 0.05, 0.05, md_steps=10,
+            ewald={"el_corr": 5, "el_bg": 1, "el_diag": 1}),
         # site-site RDF histograms of the small brine over 3 MD steps
         "brine506_rdf": lambda: rdf_case(
             "brine506_rdf", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],

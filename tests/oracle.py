@@ -55,6 +55,7 @@ def fixture_system(meta, provider_=None, **kw):
     args = dict(counts=meta["counts"], L=L, cutoff=meta["cutoff"],
                 epsr=meta["epsr"], epsk=meta["epsk"],
                 ncell=tuple(meta.get("ncell", (0, 0, 0))))
+    args.update(meta.get("ewald", {}))
     args.update(kw)
     return build_system(meta["forcefield"], provider=provider_ or provider(), **args)
 

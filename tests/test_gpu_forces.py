@@ -11,6 +11,9 @@ from macsimus_amd.system import PUNIT, engine_provider
 pytestmark = pytest.mark.gpu
 
 CASES = ["spce1000_lattice", "spce1000_md", "brine506"]
+# Ewald corrections: background energy of a charged box, slab correction,
+# slab with background force and diagonal term (cook fixtures)
+ECASES = ["brine505_charged", "brine506_slab", "brine505_slabbg"]
 # relative RMS force error bound vs the FP64 reference: 1e-5 for the mixed
 # engine, 1e-10 for the plain FP64 path
 TOL = {abi.MDX_FP64: 1e-10, abi.MDX_MIXED: 1e-5}
@@ -21,7 +24,7 @@ def rel_rms(a, b):
 
 
 @pytest.mark.parametrize("mode", [abi.MDX_FP64, abi.MDX_MIXED])
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + ECASES)
 def test_forces_vs_golden(name, mode):
     d, meta = oracle.load_fixture(name)
     bs = oracle.fixture_system(meta, provider_=engine_provider())
