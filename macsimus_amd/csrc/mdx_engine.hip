@@ -203,21 +203,126 @@ __global__ void k_gather(int ns, Grid g, const int *__restrict__ sidx, const dou
   scell[s] = cellid[i];
 }
 
-// cook's own cell id of every site: WORM wrap by subtraction, C truncation,
-// rL shaved so that (int)(L*rL) < No.cell (cook/linklist.c:81-86,155-176).
-// Used to check cell membership against the reference bit for bit.
-__global__ void k_cook_cellid(const double *__restrict__ r, int ns, int nx, int ny, int nz,
-                              double Lx, double Ly, double Lz, double rLx, double rLy,
-                              double rLz, int *__restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// ------------------------------------------- cook's census, band cases
+// The engine's grid, wrap and image arithmetic differ from cook's, so the
+// squared separation it computes for a pair can differ from cook's in the
+// last bits.  Only a pair whose engine rr lies within `band` of cutoff^2
+// can be decided differently; for those the pair is decided by cook's own
+// arithmetic, restated below from the raw positions: cook's wrap by
+// subtraction (negatives stay, cook/linklist.c:155-176), its cell of each
+// site with the shaved rL (cook/linklist.c:81-86), and the traversal of
+// cook/lc.c:217-420 from each site of the pair as the central one (which
+// shifts the central site's local copy by +-L, cook/lc.c:292-399), with
+// dr = central - neighbour and rr = dx*dx + dy*dy + dz*dz (cook/ljqqnp.c:
+// 74, 85) in IEEE double without contraction.  `band` bounds the
+// difference of the two rr with a wide margin (rounding of coordinates of
+// magnitude up to a few L), so outside it both agree.
+
+struct CookGrid {
+  int n[3];                 // No.cell
+  double L[3], rLs[3];      // box, shaved rL of the binning
+  double rL[3], cell[3];    // No.cell/L and L/No.cell of the traversal
+  double cutoff, cq, band;
+};
+
+// cook's wrapped coordinate and cell index along k (-1: cook rejects it)
+__device__ inline int cook_cell1(double &x, const CookGrid &g, int k) {
+  while (x >= g.L[k]) x -= g.L[k];
+  const int ic = (int)(x * g.rLs[k]);
+  return ic >= 0 && ic < g.n[k] ? ic : -1;
+}
+
+// times cook meets site b (cell cb, wrapped rb, index ib) from central site
+// a (ca, ra, ia) with rr < cq; *rr the rr of that meeting
+__device__ int cook_meet(const CookGrid &g, const double (&ra)[3], const int (&ca)[3], int ia,
+                         const double (&rb)[3], const int (&cb)[3], int ib, double *rrout) {
+#pragma clang fp contract(off)
+  double lr[3] = {ra[0], ra[1], ra[2]}, cellr[3];
+  for (int k = 0; k < 3; k++) cellr[k] = g.rL[k] * ra[k];
+  const int ix1 = (int)(cellr[0] + g.cutoff * g.rL[0]);
+  int hits = 0, ixm = ca[0];
+  for (int ix = ca[0]; ix <= ix1; ix++) {
+    const double DX = ix == ca[0] ? 0.0 : ix - cellr[0];
+    if (ixm == g.n[0]) { lr[0] -= g.L[0]; ixm -= g.n[0]; }
+    const double dxc = DX * g.cell[0];
+    double dyq = g.cq - dxc * dxc;
+    if (dyq < 0) dyq = 0;
+    const double dy = __dsqrt_rn(dyq) * g.rL[1];
+    int iym, iy0;
+    if (ixm == ca[0]) {
+      iy0 = iym = ca[1];
+    } else {
+      iym = iy0 = (int)(cellr[1] + g.n[1] - dy) - g.n[1];
+      if (iym < 0) { lr[1] += g.L[1]; iym += g.n[1]; }
+    }
+    const int iy1 = (int)(cellr[1] + dy);
+    for (int iy = iy0; iy <= iy1; iy++) {
+      if (iym == g.n[1]) { lr[1] -= g.L[1]; iym -= g.n[1]; }
+      const double DY = iy > ca[1] ? iy - cellr[1] : iy < ca[1] ? iy + 1 - cellr[1] : 0.0;
+      const double dyc = DY * g.cell[1];
+      double dz = dyq - dyc * dyc;
+      if (dz < 0) dz = 0;
+      dz = __dsqrt_rn(dz) * g.rL[2];
+      const int iz1 = (int)(cellr[2] + dz);
+      const bool samexy = ixm == ca[0] && iym == ca[1];
+      int izm, iz0;
+      if (samexy) {
+        izm = iz0 = ca[2];
+      } else {
+        izm = iz0 = (int)(cellr[2] + g.n[2] - dz) - g.n[2];
+        if (izm < 0) { lr[2] += g.L[2]; izm += g.n[2]; }
+      }
+      for (int iz = iz0; iz <= iz1; iz++) {
+        if (izm == g.n[2]) { lr[2] -= g.L[2]; izm -= g.n[2]; }
+        // the central cell itself holds the sites after a in cook's list,
+        // which is built by head insertion in site order: lower indices
+        if (ixm == cb[0] && iym == cb[1] && izm == cb[2] &&
+            (!(samexy && izm == ca[2]) || ib < ia)) {
+          const double d0 = lr[0] - rb[0], d1 = lr[1] - rb[1], d2 = lr[2] - rb[2];
+          const double rr = d0 * d0 + d1 * d1 + d2 * d2;
+          if (rr < g.cq) { hits++; *rrout = rr; }
+        }
+        izm++;
+      }
+      lr[2] = ra[2];
+      iym++;
+    }
+    lr[1] = ra[1];
+    ixm++;
+  }
+  return hits;
+}
+
+// cook's decision for the centre pair of sites a, b (raw positions r in
+// cook's rp layout): the times it counts the pair (0: not inside the
+// cutoff) and its rr; *ok = false where cook would reject a position, in
+// which case the caller keeps its own decision
+__device__ int cook_pair(const CookGrid &g, const double *__restrict__ r, int a, int b,
+                         double *rr, bool *ok) {
+  double ra[3], rb[3];
+  int ca[3], cb[3];
+  *ok = true;
+  for (int k = 0; k < 3; k++) {
+    ra[k] = r[3 * a + k];
+    rb[k] = r[3 * b + k];
+    ca[k] = cook_cell1(ra[k], g, k);
+    cb[k] = cook_cell1(rb[k], g, k);
+    if (ca[k] < 0 || cb[k] < 0) *ok = false;
+  }
+  if (!*ok) return 0;
+  return cook_meet(g, ra, ca, a, rb, cb, b, rr) + cook_meet(g, rb, cb, b, ra, ca, a, rr);
+}
+
+// cook's cell id of every site (the rule of cook_cell1), -1 where cook
+// rejects the position: the membership check of mdx_cook_cells
+__global__ void k_cook_cellid(const double *__restrict__ r, int ns, CookGrid g,
+                              int *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
-  double x = r[3 * i], y = r[3 * i + 1], z = r[3 * i + 2];
-  while (x >= Lx) x -= Lx;
-  while (y >= Ly) y -= Ly;
-  while (z >= Lz) z -= Lz;
-  int cx = (int)(x * rLx), cy = (int)(y * rLy), cz = (int)(z * rLz);
-  bool ok = cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz;
-  out[i] = ok ? (cx * ny + cy) * nz + cz : -1;
+  double x[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+  int ic[3];
+  for (int k = 0; k < 3; k++) ic[k] = cook_cell1(x[k], g, k);
+  out[i] = ic[0] < 0 || ic[1] < 0 || ic[2] < 0 ? -1 : (ic[0] * g.n[1] + ic[1]) * g.n[2] + ic[2];
 }
 
 // ------------------------------------------------------------- pair kernel
@@ -371,7 +476,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(
     int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
     const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
-    double sgrid, double *__restrict__ f, double *__restrict__ part) {
+    double sgrid, double *__restrict__ f, double *__restrict__ part, CookGrid cg,
+    const double *__restrict__ rraw) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   // part blockIdx.y takes the stencil columns (dx,dy) [col0, col1)
   const int span = 2 * w + 1, col0 = blockIdx.y * cpp, col1 = min(col0 + cpp, span * span);
@@ -416,7 +522,14 @@ __global__ __launch_bounds__(kBlock) void k_pair(
             in = sinfo[jn];
             const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy), dzr = pi.z - (pj.z + sz);
             const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
-            if (rr >= cq || j == s) continue;
+            if (rr >= cq + cg.band || j == s) continue;
+            if (rr > cq - cg.band) {
+              // within rounding of the cutoff: cook's arithmetic decides
+              double rrc;
+              bool ok;
+              const int m = cook_pair(cg, rraw, ii.w, jj.w, &rrc, &ok);
+              if (ok ? m == 0 : rr >= cq) continue;
+            }
             npair += 1;
             double4 rj = make_double4(0, 0, 0, 0);
             if (POLAR) rj = spol[j];
@@ -543,7 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_rdf(int ns, Grid g, int w, double cq
                                                 const int *__restrict__ scell,
                                                 const int *__restrict__ cstart, int merged,
                                                 double grid, int nh, int nlds,
-                                                unsigned long long *__restrict__ hist) {
+                                                unsigned long long *__restrict__ hist,
+                                                CookGrid cg, const double *__restrict__ rraw) {
   extern __shared__ unsigned lh[];
   for (int k = threadIdx.x; k < nlds; k += kBlock) lh[k] = 0;
   __syncthreads();
@@ -592,7 +706,21 @@ __global__ __launch_bounds__(kBlock) void k_rdf(int ns, Grid g, int w, double cq
             const double4 pj = spos[j];
             const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy), dzr = pi.z - (pj.z + sz);
             const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
-            if (rr < cq) count(sinfo[j], rr);
+            if (rr >= cq + cg.band) continue;
+            const int4 jj = sinfo[j];
+            // within rounding of the cutoff or of a bin edge: the pair and
+            // its bin are decided by cook's own arithmetic
+            const double r1 = sqrt(rr), t = r1 * grid;
+            if (rr > cq - cg.band || fabs(t - rint(t)) * 2 * r1 <= grid * cg.band) {
+              double rrc = rr;
+              bool ok;
+              const int m = cook_pair(cg, rraw, ii.w, jj.w, &rrc, &ok);
+              if (ok) {
+                if (m) count(jj, rrc);
+                continue;
+              }
+            }
+            if (rr < cq) count(jj, rr);
           }
         }
       }
@@ -710,6 +838,44 @@ __global__ __launch_bounds__(kBlock) void k_excl(int nmol, int ns, const int *__
 
 inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
+// minuseps() of cook/linklist.c:15-29: a number a few ulps below x
+double minuseps(double x) {
+  double y = x * 0.999999999999999667;
+  for (int i = 0; i < 5; i++) {
+    volatile double z = (x + y) / 2;
+    if (z >= x) return y;
+    y = z;
+  }
+  return y;
+}
+
+// cook's grid for the context box: No.cell, rL shaved until
+// (int)(minuseps(L) rL) < No.cell (cook/linklist.c:81-86), the traversal's
+// rL and cell (cook/lc.c:262-263), and the band of rr around cutoff^2
+// inside which cook's arithmetic decides a pair
+CookGrid make_cook_grid(const Ctx &c) {
+  CookGrid g;
+  double Lmax = 0;
+  for (int k = 0; k < 3; k++) {
+    g.n[k] = c.cook_ncell[k];
+    g.L[k] = c.L[k];
+    volatile double t = g.n[k] / c.L[k];
+    g.rLs[k] = t;
+    const double x = minuseps(c.L[k]);
+    while ((int)(x * g.rLs[k]) >= g.n[k]) g.rLs[k] = minuseps(g.rLs[k]);
+    g.rL[k] = g.n[k] / c.L[k];
+    g.cell[k] = c.L[k] / g.n[k];
+    Lmax = std::max(Lmax, c.L[k]);
+  }
+  g.cutoff = c.cutoff;
+  g.cq = c.cutoff * c.cutoff;
+  // cook's and the engine's rr differ by a few roundings of coordinates
+  // of magnitude up to a few L (~1e-15 L^2 relative); 1e-11 (cq + L^2)
+  // leaves a margin of about 1e3
+  g.band = 1e-11 * (g.cq + Lmax * Lmax);
+  return g;
+}
+
 Grid make_grid(const Ctx &c, const int *n) {
   Grid g;
   g.nx = n[0]; g.ny = n[1]; g.nz = n[2];
@@ -807,12 +973,12 @@ int launch_pair_t(Ctx &c, int measure) {
       k_pair<R, true, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-          c.part.p);
+          c.part.p, make_cook_grid(c), c.r.p);
     else
       k_pair<R, false, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-          c.part.p);
+          c.part.p, make_cook_grid(c), c.r.p);
   }
   if (np > 1) {
     const long n3 = 3L * (POLAR ? 2 : 1) * c.ns;
@@ -844,11 +1010,13 @@ int launch_rdf(Ctx &c) {
   if (c.allpair)
     k_rdf<true><<<nb, kBlock, lds, c.stream>>>(
         c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
-        c.cstart.p, c.rdf_grid < 0, std::abs(c.rdf_grid), c.rdf_nh, nlds, c.rdf_hist.p);
+        c.cstart.p, c.rdf_grid < 0, std::abs(c.rdf_grid), c.rdf_nh, nlds, c.rdf_hist.p,
+        make_cook_grid(c), c.r.p);
   else
     k_rdf<false><<<nb, kBlock, lds, c.stream>>>(
         c.ns, g, c.wst, cq, (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p,
-        c.cstart.p, c.rdf_grid < 0, std::abs(c.rdf_grid), c.rdf_nh, nlds, c.rdf_hist.p);
+        c.cstart.p, c.rdf_grid < 0, std::abs(c.rdf_grid), c.rdf_nh, nlds, c.rdf_hist.p,
+        make_cook_grid(c), c.r.p);
   MDX_HIP(hipGetLastError());
   c.rdf_nmeas++;
   c.rdf_V += c.L[0] * c.L[1] * c.L[2];
@@ -1031,6 +1199,11 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   c.nmol = s->nmol;
   c.ntypes = s->ntypes;
   for (int d = 0; d < 3; d++) { c.L[d] = s->L[d]; c.cook_ncell[d] = s->ncell[d]; }
+  // cook's No.cell (its census is matched for it); lcsetup()'s choice
+  // where the caller leaves it 0 (cook/linklist.c:304-337)
+  if ((rc = mdx_lcsetup(c.cook_ncell, c.ns, 0, c.L))) return bail(rc);
+  for (int d = 0; d < 3; d++)
+    if (c.cook_ncell[d] < 2) return bail(fail(MDX_E_ARG, "mdx_create: No.cell below 2"));
   c.cutoff = s->cutoff;
   c.alpha = s->alpha;
   c.alphar = s->alphar > 0 ? s->alphar : s->alpha;
@@ -1253,29 +1426,10 @@ extern "C" int mdx_resize(mdx_ctx_t *h, const mdx_system_t *s) {
 extern "C" int mdx_cook_cells(mdx_ctx_t *h, const double *r, int *cellid) {
   if (!h || !r || !cellid) return fail(MDX_E_ARG, "mdx_cook_cells: NULL argument");
   Ctx &c = h->c;
-  const int *n = c.cook_ncell;
-  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(MDX_E_ARG, "mdx_cook_cells: ncell not set");
-  double rL[3];
-  for (int k = 0; k < 3; k++) {
-    // rL shaved until (int)(minuseps(L)*rL) < No.cell (cook/linklist.c:15-29,81-86)
-    auto minuseps = [](double x) {
-      double y = x * 0.999999999999999667;
-      for (int i = 0; i < 5; i++) {
-        volatile double z = (x + y) / 2;
-        if (z >= x) return y;
-        y = z;
-      }
-      return y;
-    };
-    volatile double t = n[k] / c.L[k];
-    rL[k] = t;
-    const double x = minuseps(c.L[k]);
-    while ((int)(x * rL[k]) >= n[k]) rL[k] = minuseps(rL[k]);
-  }
   MDX_HIP(hipSetDevice(c.device));
+  c.state_on_device = false;           // the device positions are now r
   MDX_HIP(hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream));
-  k_cook_cellid<<<nblk(c.ns), kBlock, 0, c.stream>>>(c.r.p, c.ns, n[0], n[1], n[2], c.L[0], c.L[1],
-                                                     c.L[2], rL[0], rL[1], rL[2], c.cellid.p);
+  k_cook_cellid<<<nblk(c.ns), kBlock, 0, c.stream>>>(c.r.p, c.ns, make_cook_grid(c), c.cellid.p);
   MDX_HIP(hipGetLastError());
   MDX_HIP(hipMemcpyAsync(cellid, c.cellid.p, sizeof(int) * c.ns, hipMemcpyDeviceToHost, c.stream));
   MDX_HIP(hipStreamSynchronize(c.stream));
