@@ -366,6 +366,18 @@ int mdx_set_params(mdx_ctx_t *ctx, const mdx_lj_t *sstab, const double *charge,
 int mdx_adjoint_lj(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
                    double *gE);
 int mdx_set_params_exp6(mdx_ctx_t *ctx, const mdx_exp6_t *sstab);
+/* No counterpart in cook.  For callers that cannot allocate device memory
+   (code under HIP graph capture, tight force-matching loops, harnesses
+   that forbid hipMalloc): mdx_adjoint_lj with the same arguments and
+   results, its scratch (staged weights, per-site derivative rows, summed
+   result: 3 ns + 8 ntypes ns + 8 ntypes^2 doubles) in a static device
+   array of MDX_ADJ_SCRATCH_BYTES per device instead of buffers allocated
+   per call.  Larger systems get MDX_E_ARG before any device work.  The
+   array is shared by every call on a device and keeps the last call's
+   contents; calls that overlap on one device race. */
+#define MDX_ADJ_SCRATCH_BYTES (8u << 20)
+int mdx_adjoint_lj_noalloc(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
+                           double *gE);
 int mdx_adjoint_exp6(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
                      double *gE);
 

@@ -19,11 +19,49 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "mdx_internal.h"
 
 using namespace mdx;
+
+// Scratch of mdx_adjoint_lj_noalloc: the staged weights, the per-site
+// derivative rows and the summed result, in a fixed array of the code
+// object instead of buffers allocated per call.  Each device that loads
+// the library holds its own copy until the process ends; host code uses
+// only the device address that hipGetSymbolAddress gives for the current
+// device, never the array's name.
+constexpr size_t kAdjScratchBytes = (size_t)MDX_ADJ_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kAdjScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace mdx {
+
+// the device address of the scratch array on device dev (current), looked
+// up once per device
+int adjoint_scratch_setup(int dev, double **ptr) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(MDX_E_ARG, "adjoint scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MDX_E_GPU, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  if (ptr) *ptr = (double *)g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace mdx
 
 namespace {
 
@@ -181,41 +219,60 @@ extern "C" int mdx_set_params(mdx_ctx_t *h, const mdx_lj_t *sstab, const double 
 
 namespace {
 
+// doubles of scratch the adjoint needs: staged weights [3 ns], derivative
+// rows [nt][ns][ng], result [nt^2][ng]
+size_t adjoint_scratch_doubles(const Ctx &c, int ng) {
+  const size_t nt = c.ntypes;
+  return 3 * (size_t)c.ns + nt * c.ns * ng + nt * nt * ng;
+}
+
 // the adjoint of the context's site-site form at positions r for weights
-// w: res[(a*nt+b)*ng + k] as gsite summed over the sites of type a
+// w: res[(a*nt+b)*ng + k] as gsite summed over the sites of type a.  The
+// scratch is allocated for the call, or (fixed != NULL) is the caller's
+// device array of at least adjoint_scratch_doubles; every entry of it
+// that is read is written first in the call.
 template <bool E6>
-int adjoint_run(Ctx &c, const double *r, const double *w, std::vector<double> &res) {
+int adjoint_run(Ctx &c, const double *r, const double *w, std::vector<double> &res,
+                double *fixed = nullptr) {
   constexpr int ng = ng_of<E6>();
   MDX_HIP(hipSetDevice(c.device));
   const int nt = c.ntypes, nt2 = nt * nt;
   const size_t nrow = c.polar ? 2 * (size_t)c.ns : c.ns;
-  DBuf<double> wd, gs, out;
-  int rc;
-  if ((rc = wd.alloc(3 * (size_t)c.ns)) || (rc = gs.alloc((size_t)nt * c.ns * ng)) ||
-      (rc = out.alloc((size_t)nt2 * ng))) {
-    wd.release(); gs.release();
-    return rc;
+  DBuf<double> wb, gb, ob;
+  double *wd, *gs, *out;
+  int rc = 0;
+  if (fixed) {
+    wd = fixed;
+    gs = wd + 3 * (size_t)c.ns;
+    out = gs + (size_t)nt * c.ns * ng;
+  } else {
+    if ((rc = wb.alloc(3 * (size_t)c.ns)) || (rc = gb.alloc((size_t)nt * c.ns * ng)) ||
+        (rc = ob.alloc((size_t)nt2 * ng))) {
+      wb.release(); gb.release(); ob.release();
+      return rc;
+    }
+    wd = wb.p; gs = gb.p; out = ob.p;
   }
   c.state_on_device = false;           // the device positions are now r
   hipError_t e = hipMemcpyAsync(c.r.p, r, sizeof(double) * 3 * nrow, hipMemcpyHostToDevice, c.stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(wd.p, w, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream);
+    e = hipMemcpyAsync(wd, w, sizeof(double) * 3 * c.ns, hipMemcpyHostToDevice, c.stream);
   if (e == hipSuccess && (rc = bin_device(c)) == 0) {
     k_adjoint<E6><<<dim3(nblk(c.ns), nt), kBlock, 0, c.stream>>>(
         c.ns, c.nc[0], c.nc[1], c.nc[2], c.L[0], c.L[1], c.L[2], c.wst, c.cutoff * c.cutoff,
         (const double4 *)c.spos.p, (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, nt, c.lj64.p,
-        wd.p, gs.p);
-    k_adjoint_sum<<<nt2 * ng, kBlock, 0, c.stream>>>(c.ns, nt, ng, (const int4 *)c.sinfo.p, gs.p,
-                                                     out.p);
+        wd, gs);
+    k_adjoint_sum<<<nt2 * ng, kBlock, 0, c.stream>>>(c.ns, nt, ng, (const int4 *)c.sinfo.p, gs,
+                                                     out);
     e = hipGetLastError();
   }
   res.assign((size_t)nt2 * ng, 0.0);
   if (!rc && e == hipSuccess)
-    e = hipMemcpyAsync(res.data(), out.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c.stream);
+    e = hipMemcpyAsync(res.data(), out, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c.stream);
   if (!rc && e == hipSuccess) e = hipStreamSynchronize(c.stream);
-  wd.release();
-  gs.release();
-  out.release();
+  wb.release();
+  gb.release();
+  ob.release();
   if (rc) return rc;
   if (e != hipSuccess) return fail(MDX_E_GPU, "adjoint: HIP error");
   return 0;
@@ -244,6 +301,35 @@ extern "C" int mdx_adjoint_lj(mdx_ctx_t *h, const double *r, const double *w, do
   std::vector<double> res;
   int rc = adjoint_run<false>(c, r, w, res);
   if (rc) return rc;
+  const int nt2 = c.ntypes * c.ntypes;
+  for (int p = 0; p < nt2; p++)
+    for (int k = 0; k < 4; k++) {
+      gF[p * 4 + k] = res[p * 8 + k];
+      gE[p * 4 + k] = res[p * 8 + 4 + k];
+    }
+  return 0;
+}
+
+// mdx_adjoint_lj with its scratch in the static array (no device memory
+// is allocated or freed): the device address comes from the per-device
+// look-up made at mdx_create; sizes beyond the array are refused before
+// any device work
+extern "C" int mdx_adjoint_lj_noalloc(mdx_ctx_t *h, const double *r, const double *w,
+                                      double *gF, double *gE) {
+  if (!h || !r || !w || !gF || !gE) return fail(MDX_E_ARG, "mdx_adjoint_lj_noalloc: NULL argument");
+  Ctx &c = h->c;
+  if (c.exp6) return fail(MDX_E_ARG, "mdx_adjoint_lj_noalloc: the context has the exp-6 form, not LJ");
+  const size_t need = adjoint_scratch_doubles(c, ng_of<false>()) * sizeof(double);
+  if (need > kAdjScratchBytes)
+    return fail(MDX_E_ARG, "mdx_adjoint_lj_noalloc: needs " + std::to_string(need) +
+                               " bytes of scratch (3 ns + 8 ntypes ns + 8 ntypes^2 doubles), more "
+                               "than MDX_ADJ_SCRATCH_BYTES = " + std::to_string(kAdjScratchBytes));
+  MDX_HIP(hipSetDevice(c.device));
+  double *ptr = nullptr;
+  int rc = adjoint_scratch_setup(c.device, &ptr);
+  if (rc) return rc;
+  std::vector<double> res;
+  if ((rc = adjoint_run<false>(c, r, w, res, ptr))) return rc;
   const int nt2 = c.ntypes * c.ntypes;
   for (int p = 0; p < nt2; p++)
     for (int k = 0; k < 4; k++) {

@@ -1190,6 +1190,8 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   c.mode = mode;
   c.kmfma = mode == MDX_MIXED;       // pass 2 on the matrix cores by default
   (void)hipGetDevice(&c.device);
+  // set-up of this device: the address of the no-allocation adjoint's scratch
+  if ((rc = adjoint_scratch_setup(c.device, nullptr))) return bail(rc);
   if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c.kstream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) != hipSuccess ||

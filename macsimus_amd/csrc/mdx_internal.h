@@ -27,6 +27,10 @@ double ewald_ediag(const double L[3], double kappa, double alpha, double sumq);
 void ewald_dipole_coefs(int el_corr, int el_bg, double epsinf, double qnet, double V,
                         double cm[3], double ce[3], double cb[3]);
 
+// device address of mdx_adjoint_lj_noalloc's static scratch array on
+// device dev (the current one), looked up once per device
+int adjoint_scratch_setup(int dev, double **ptr);
+
 // the checks cook's set-up makes on these settings (sim/siminit.c:
 // 152-160, sim/ewaldprt.c:16-25, sim/main.c:719-721): a charged system
 // needs el.bg and a tin-foil epsinf, the slab correction an infinite one

@@ -205,6 +205,21 @@ class Engine:
         abi.check(self.lib, rc, "mdx_adjoint_lj")
         return gF, gE
 
+    def adjoint_lj_noalloc(self, r, w):
+        """adjoint_lj through mdx_adjoint_lj_noalloc: the same results with
+        the scratch in the library's static device array (no device
+        allocation per call; systems beyond MDX_ADJ_SCRATCH_BYTES raise)"""
+        nt = self.bs.struct.ntypes
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.nrow, 3)
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.ns, 3)
+        gF = np.zeros((nt, nt, 4))
+        gE = np.zeros((nt, nt, 4))
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_adjoint_lj_noalloc(self.ctx, r.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                             gF.ctypes.data_as(dp), gE.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_adjoint_lj_noalloc")
+        return gF, gE
+
     def set_graph(self, on=True):
         rc = self.lib.mdx_set_graph(self.ctx, int(on))
         abi.check(self.lib, rc, "mdx_set_graph")
@@ -418,6 +433,7 @@ def declare_extras(lib):
     lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
+    lib.mdx_adjoint_lj_noalloc.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
     lib.mdx_set_params_exp6.argtypes = [C.c_void_p, C.POINTER(abi.Exp6)]
     lib.mdx_adjoint_exp6.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
     lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
