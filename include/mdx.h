@@ -137,7 +137,10 @@ typedef struct mdx_out_s {
   double En_vir;       /* En.vir += sum rr*f over LJ terms */
   double Pvir[6];      /* En.Pvir[0..5] += virial tensor (not /V) */
   double En_kspace;    /* k-space energy incl. self and dipole terms */
-  long long npair;     /* census: site pairs with rr < cutoff^2 */
+  long long npair;     /* census: site pairs with rr < cutoff^2, cook's
+                          (pairs within rounding of the cutoff decided by
+                          cook's arithmetic); -1 if more than
+                          max(65536, 2 ns) pairs lie that close to it */
   long long nlj;       /* of those, pairs inside C2q with nonzero LJ */
 } mdx_out_t;
 
@@ -333,9 +336,12 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    (FP32 MFMA, FP32 products and sums as the FMA variant) or as FP32 FMA
    chains; mdx_autotune tries both.  Default: on in MDX_MIXED.
    mdx_set_kspace_pass1 (MDX_MIXED): k-space pass 1 as FP32 FMA chains
-   (tiles = 0, the default) or on the matrix cores (FP32 MFMA
-   16x16x4, 16 or 8 tiles of 4 k rows per wave); the same sums, flushed
-   to FP64 every 32 charges.  mdx_autotune keeps the fastest. */
+   (tiles = 0), on the matrix cores as units of 8 k rows with all their
+   kz per wave (tiles = 1, the default where kz stays below 64: FP32 MFMA
+   16x16x4, sums added to FP32 totals every 128 charges and to FP64 per
+   charge split), or on the matrix cores with 16 or 8 tiles of 4 k rows
+   and one 8-kz block per wave (FP64 flush every 32 charges).
+   mdx_autotune keeps the fastest. */
 int mdx_set_kspace_mfma(mdx_ctx_t *ctx, int on);
 int mdx_set_kspace_pass1(mdx_ctx_t *ctx, int tiles);
 int mdx_cook_cells(mdx_ctx_t *ctx, const double *r, int *cellid);

@@ -313,6 +313,34 @@ __device__ int cook_pair(const CookGrid &g, const double *__restrict__ r, int a,
   return cook_meet(g, ra, ca, a, rb, cb, b, rr) + cook_meet(g, rb, cb, b, ra, ca, a, rr);
 }
 
+// the census of the pairs k_pair listed within rounding of the cutoff,
+// each once per order with k_pair's own decision d (counted 1/2 per
+// order): the pair count gets cook's count m in place of them, 1/2 m - 1/2
+// d per entry (an exact sum in any order); a list that overflowed makes
+// the count -1
+__global__ __launch_bounds__(kBlock) void k_census_fix(const int4 *__restrict__ blist,
+                                                      const int *__restrict__ bcount, int bcap,
+                                                      CookGrid g, const double *__restrict__ r,
+                                                      double *__restrict__ acc) {
+  __shared__ double red[kBlock];
+  const int n = *bcount;
+  double s = 0;
+  for (int i = threadIdx.x; i < min(n, bcap); i += kBlock) {
+    const int4 e = blist[i];
+    double rr;
+    bool ok;
+    const int m = cook_pair(g, r, e.x, e.y, &rr, &ok);
+    s += ok ? 0.5 * m - 0.5 * e.z : 0.0;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[ACC_NPAIR] = n > bcap ? -1.0 : acc[ACC_NPAIR] + red[0];
+}
+
 // cook's cell id of every site (the rule of cook_cell1), -1 where cook
 // rejects the position: the membership check of mdx_cook_cells
 __global__ void k_cook_cellid(const double *__restrict__ r, int ns, CookGrid g,
@@ -476,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(
     int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
     const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
-    double sgrid, double *__restrict__ f, double *__restrict__ part, CookGrid cg,
-    const double *__restrict__ rraw) {
+    double sgrid, double *__restrict__ f, double *__restrict__ part, double band,
+    int4 *__restrict__ blist, int *__restrict__ bcount, int bcap) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   // part blockIdx.y takes the stencil columns (dx,dy) [col0, col1)
   const int span = 2 * w + 1, col0 = blockIdx.y * cpp, col1 = min(col0 + cpp, span * span);
@@ -522,13 +550,13 @@ __global__ __launch_bounds__(kBlock) void k_pair(
             in = sinfo[jn];
             const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy), dzr = pi.z - (pj.z + sz);
             const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
-            if (rr >= cq + cg.band || j == s) continue;
-            if (rr > cq - cg.band) {
-              // within rounding of the cutoff: cook's arithmetic decides
-              double rrc;
-              bool ok;
-              const int m = cook_pair(cg, rraw, ii.w, jj.w, &rrc, &ok);
-              if (ok ? m == 0 : rr >= cq) continue;
+            if (rr >= cq + band || j == s) continue;
+            if (rr > cq - band) {
+              // within rounding of the cutoff: listed for k_census_fix,
+              // which puts cook's decision in place of this one
+              const int slot = atomicAdd(bcount, 1);
+              if (slot < bcap) blist[slot] = make_int4(ii.w, jj.w, rr < cq, 0);
+              if (rr >= cq) continue;
             }
             npair += 1;
             double4 rj = make_double4(0, 0, 0, 0);
@@ -973,12 +1001,12 @@ int launch_pair_t(Ctx &c, int measure) {
       k_pair<R, true, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-          c.part.p, make_cook_grid(c), c.r.p);
+          c.part.p, make_cook_grid(c).band, (int4 *)c.blist.p, c.bcount.p, c.bcap);
     else
       k_pair<R, false, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-          c.part.p, make_cook_grid(c), c.r.p);
+          c.part.p, make_cook_grid(c).band, (int4 *)c.blist.p, c.bcount.p, c.bcap);
   }
   if (np > 1) {
     const long n3 = 3L * (POLAR ? 2 : 1) * c.ns;
@@ -1032,6 +1060,7 @@ int forces_device(Ctx &c, int measure) {
   int rc;
   const size_t npart = (size_t)c.part_blocks * ACC_N;
   k_zero<<<nblk(npart), kBlock, 0, c.stream>>>(c.part.p, npart);
+  k_zero_int<<<1, kBlock, 0, c.stream>>>(c.bcount.p, 1);
   // k-space on its own stream, beside binning, pair and exclusions: it
   // reads only the positions and charges, writes other columns of the
   // partial rows and adds its forces after the join
@@ -1075,6 +1104,9 @@ int forces_device(Ctx &c, int measure) {
     if ((rc = kspace_pass(c, measure, c.stream)) || (rc = kspace_add(c, c.stream))) return rc;
   }
   k_reduce<<<ACC_N, kBlock, 0, c.stream>>>(c.part.p, c.part_blocks, c.acc.p);
+  if (!c.allpair)
+    k_census_fix<<<1, kBlock, 0, c.stream>>>((const int4 *)c.blist.p, c.bcount.p, c.bcap,
+                                             make_cook_grid(c), c.r.p, c.acc.p);
   MDX_HIP(hipGetLastError());
   return 0;
 }
@@ -1199,6 +1231,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     return bail(fail(MDX_E_GPU, "hipStreamCreate failed"));
   c.ns = s->ns;
   c.nmol = s->nmol;
+  c.bcap = std::max(65536, 2 * c.ns);
   c.ntypes = s->ntypes;
   for (int d = 0; d < 3; d++) { c.L[d] = s->L[d]; c.cook_ncell[d] = s->ncell[d]; }
   // cook's No.cell (its census is matched for it); lcsetup()'s choice
@@ -1280,7 +1313,8 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
       (rc = c.lj64.alloc(lj.size())) || (rc = c.lj32.alloc(lj32.size())) ||
       (rc = c.erf64.alloc(e64.size())) || (rc = c.erf32.alloc(e32.size())) ||
       (rc = c.cellid.alloc(c.ns)) || (rc = c.sidx.alloc(c.ns)) || (rc = c.spos.alloc(4 * c.ns)) ||
-      (rc = c.sinfo.alloc(4 * c.ns)) || (rc = c.scan_tmp.alloc(c.ns)) || (rc = c.acc.alloc(ACC_N)))
+      (rc = c.sinfo.alloc(4 * c.ns)) || (rc = c.scan_tmp.alloc(c.ns)) || (rc = c.acc.alloc(ACC_N)) ||
+      (rc = c.blist.alloc(4 * (size_t)c.bcap)) || (rc = c.bcount.alloc(1)))
     return bail(rc);
   auto up = [&](void *dst, const void *src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -1320,6 +1354,9 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
 
   if ((rc = choose_grid(c, 2)) || (rc = alloc_grid(c))) return bail(rc);
   if (c.nq && (rc = kspace_setup(c))) return bail(rc);
+  // mixed mode: pass 1 on the matrix cores as row units where the k-set
+  // allows it (kz below 64)
+  if (mode == MDX_MIXED && c.nq && c.K.nblk <= 8) c.p1nt = 1;
   *out = h;
   return 0;
 }
@@ -1340,7 +1377,8 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
                        &c.cursor, &c.sidx, &c.sinfo, &c.scan_tmp, &c.d_row_kx, &c.d_row_ky,
                        &c.d_row_kzmax, &c.d_row_coef0, &c.d_seg_blk, &c.d_seg_row0,
                        &c.d_lex_row0, &c.d_lex_meta, &c.d_chunk_kymax, &c.d_kparts,
-                       &c.d_seg_rowend, &c.cons_first, &c.cons_ij};
+                       &c.d_seg_rowend, &c.cons_first, &c.cons_ij, &c.d_blk_rows, &c.d_blk_seg0,
+                       &c.blist, &c.bcount};
   for (auto *b : ints) b->release();
   c.excl.release();
   c.lj32.release();
@@ -1351,6 +1389,7 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.d_grp_base.release();
   c.d_grp_nb.release();
   c.rdf_hist.release();
+  c.qanc.release();
   c.tm.release();
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);
@@ -1555,7 +1594,8 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
       // pass 1: FP32 FMA (0) or matrix cores with 16 or 8 tiles per wave
       int best_nt = c.p1nt;
       double t1 = 1e30;
-      for (int nt : {0, 16, 8}) {
+      for (int nt : {0, 1, 16, 8}) {
+        if (nt == 1 && c.K.nblk > 8) continue;
         c.p1nt = nt;
         if ((rc = run(fam))) break;
         if (fam[T_KPASS1] < t1) { t1 = fam[T_KPASS1]; best_nt = nt; }
@@ -1573,8 +1613,8 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
 
 extern "C" int mdx_set_kspace_pass1(mdx_ctx_t *h, int tiles) {
   if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: NULL");
-  if (tiles != 0 && tiles != 8 && tiles != 16)
-    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0, 8 or 16");
+  if (tiles != 0 && tiles != 1 && tiles != 8 && tiles != 16)
+    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0, 1, 8 or 16");
   if (tiles && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: MDX_MIXED only");
   h->c.p1nt = tiles;
   h->c.gen++;

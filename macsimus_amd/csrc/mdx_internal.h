@@ -182,6 +182,9 @@ struct Ctx {
   DBuf<double> spol;           // [4ns] Drude offset, qpol (cell order)
   DBuf<double> qpos;           // [4nq] charge positions and charges
   DBuf<double> qtrig;          // [12nq] per direction cos, sin of 2 pi s and 16 pi s
+  // matrix-core pass 1: FP32 anchors e^{2 pi i 8m s} per block of 8 [task][nq]
+  // (q folded in along x), then the bases e^{2 pi i s} [3][nq]
+  DBuf<float2> qanc;
   DBuf<unsigned> excl;        // [ns]
   bool exp6 = false;           // site-site form: exp-6 table instead of LJ
   DBuf<double> lj64;          // [ntypes^2*8] LJ or exp-6 constants, C1q C2q A A4
@@ -203,6 +206,11 @@ struct Ctx {
   bool allpair = false;
   DBuf<int> cellid, cnt, cstart, cursor, sidx;
   DBuf<double> spos;           // [4ns] x,y,z wrapped, q  (cell order)
+  // pairs the pair kernel met within rounding of the cutoff: [bcap] of
+  // (site a, site b, its own decision), their count; cook's decision
+  // replaces them in the census
+  DBuf<int> blist, bcount;
+  int bcap = 0;
   DBuf<int> sinfo;             // [4ns] type, mol, excl, site
   DBuf<int> scan_tmp;
   // reductions
@@ -213,6 +221,7 @@ struct Ctx {
   KLayout K;
   int nsplit = 1;
   DBuf<int> d_row_kx, d_row_ky, d_row_kzmax, d_row_coef0;
+  DBuf<int> d_blk_rows, d_blk_seg0;   // per 8-kz block: rows reaching it, first segment
   DBuf<int> d_seg_blk, d_seg_row0, d_seg_rowend;
   // pass 2: kz chunk of the force kernel (8..64, set by the tuner), the
   // number of chunks and per chunk the largest ky of each kx whose row

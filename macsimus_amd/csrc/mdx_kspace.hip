@@ -61,7 +61,9 @@ __device__ inline double wave_sum(double v) {
 __global__ void k_qgather(int nq, int ns, const int *__restrict__ qsite,
                           const double *__restrict__ r, const double *__restrict__ charge,
                           const double *__restrict__ chargepol, double iLx, double iLy,
-                          double iLz, double4 *__restrict__ qpos, double4 *__restrict__ qtrig) {
+                          double iLz, double4 *__restrict__ qpos, double4 *__restrict__ qtrig,
+                          float2 *__restrict__ qanc, float2 *__restrict__ qbase, int3 nb3,
+                          int3 boff3) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nq) return;
   const int k = qsite[j];
@@ -75,12 +77,27 @@ __global__ void k_qgather(int nq, int ns, const int *__restrict__ qsite,
   }
   qpos[j] = p;
   const double x[3] = {p.x * iLx, p.y * iLy, p.z * iLz};
+  const int nb[3] = {nb3.x, nb3.y, nb3.z}, boff[3] = {boff3.x, boff3.y, boff3.z};
   for (int d = 0; d < 3; d++) {
     const double sd = frac1(x[d]);
     double c1, s1, c8, s8;
     cs2pi(sd, c1, s1);
     cs2pi(8.0 * sd, c8, s8);
     qtrig[3 * (size_t)j + d] = make_double4(c1, s1, c8, s8);
+    if (!qanc) continue;
+    // the FP32 table anchors of the matrix-core pass 1: e^{2 pi i 8m s}
+    // (times q along x) for the nb[d] blocks of 8, by the same FP64
+    // products as the tables of k_sfac, and the FP32 base e^{2 pi i s};
+    // stored [task][charge]
+    const double qf = d == 0 ? p.w : 1.0;
+    double c0 = 1, s0 = 0;
+    for (int m = 0; m < nb[d]; m++) {
+      qanc[(size_t)(boff[d] + m) * nq + j] = make_float2((float)(c0 * qf), (float)(s0 * qf));
+      const double nc = c0 * c8 - s0 * s8;
+      s0 = c0 * s8 + s0 * c8;
+      c0 = nc;
+    }
+    qbase[(size_t)d * nq + j] = make_float2((float)c1, (float)s1);
   }
 }
 
@@ -301,6 +318,174 @@ __global__ __launch_bounds__(kBlock) void k_sfac_mfma(
   for (int t = 0; t < NT; t++)
 #pragma unroll
     for (int r = 0; r < 4; r++) out[(rbase + 4 * t + kq) * 64 + 2 * r] = mid[t][r];
+}
+
+// Pass 1 as a GEMM on the matrix cores (mixed mode), T[(row,pq),(kz,r)] =
+// sum_j A[(row,pq),j] B[j,(kz,r)] with A = q X_p(kx) Y_q(ky), B = Z_r(kz):
+// a wave owns a unit of 8 consecutive rows of the kz-sorted row list (two
+// M tiles of 4 rows x 4 products) and every 8-kz block its first (longest)
+// row reaches (NA <= 8 N tiles of 8 kz x cos/sin), so each A operand is
+// used by all N tiles and each B operand by both M tiles; one
+// v_mfma_f32_16x16x4_f32 takes 4 charges.  Layout of the instruction:
+// lane l supplies A[l%16][l/16] and B[l/16][l%16] and receives D[4(l/16) +
+// r][l%16] in register r, i.e. row l/16 of the tile, kz l%16/2, cos/sin
+// l%2, product pq = r: T index 2r + l%2 of that row and kz.
+// The cos/sin tables of a chunk of JC charges are built in LDS as in
+// k_sfac (FP64 anchors every 8 entries, FP32 recursion), laid out [k][j]
+// (charge fastest) with a row stride of 2 JC + 8 floats: the lanes of a
+// wave build consecutive charges, so the table writes are conflict-free,
+// and the B operands of a tile (8 kz x 2 x 4 charges) fall in 64 distinct
+// banks.  The matrix accumulators are added to FP64 sums after each chunk.
+// Output: the kpart layout of k_sfac (segments of 64 rows of one 8-kz
+// block; rows of a unit past a block's rows are dropped).
+template <int JC>
+struct SfTab {
+  static constexpr int S = 2 * JC + 8;   // floats per table row (one k)
+};
+
+// the cos/sin tables of charges [jc, jc+JC) of the split ending at j1,
+// from the anchors of k_qgather: 7 FP32 recursion steps per block of 8
+template <int JC>
+__device__ inline void sf_tables(float *Xf, float *Yf, float *Zf, int KXT, int KYT, int KZT,
+                                 int jc, int j1, int nq, const float2 *__restrict__ qanc,
+                                 const float2 *__restrict__ qbase) {
+  constexpr int S = SfTab<JC>::S;
+  const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
+  for (int t = threadIdx.x; t < JC * ntask; t += kBlock) {
+    const int jj = t % JC, task = t / JC;
+    int d, m, KT;
+    float *dst;
+    if (task < ntx) { d = 0; m = task; KT = KXT; dst = Xf; }
+    else if (task < ntx + nty) { d = 1; m = task - ntx; KT = KYT; dst = Yf; }
+    else { d = 2; m = task - ntx - nty; KT = KZT; dst = Zf; }
+    dst += 2 * jj;
+    const int j = jc + jj;
+    const int k1 = min(8 * m + 8, KT);
+    if (j < j1) {
+      const float2 a = qanc[(size_t)task * nq + j], b = qbase[(size_t)d * nq + j];
+      float ec = a.x, es = a.y;
+      for (int k = 8 * m; k < k1; k++) {
+        *(float2 *)(dst + k * S) = make_float2(ec, es);
+        const float nc = ec * b.x - es * b.y;
+        es = ec * b.y + es * b.x;
+        ec = nc;
+      }
+    } else {
+      for (int k = 8 * m; k < k1; k++) *(float2 *)(dst + k * S) = make_float2(0.0f, 0.0f);
+    }
+  }
+}
+
+// one wave's unit of NA N tiles over the charges [j0, j1): every wave of
+// the workgroup builds tables and meets the barriers alike
+template <int NA, int JC>
+__device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT, int KZT,
+                               int j0, int j1, int r0, int nrows, const int *__restrict__ row_kx,
+                               const int *__restrict__ row_ky, const int *__restrict__ blk_rows,
+                               const int *__restrict__ blk_seg0, int nseg,
+                               double *__restrict__ kpart, int nq,
+                               const float2 *__restrict__ qanc,
+                               const float2 *__restrict__ qbase) {
+  constexpr int S = SfTab<JC>::S;
+  const int lane = threadIdx.x & 63, i16 = lane & 15, kq = lane >> 4;
+  // operand offsets (floats) for charge 0 of a chunk's first K step:
+  // A of M tile m: row r0 + 4m + i16/4, X factor by bit 1 of i16, Y by bit 0;
+  // B of N tile t: kz 8t + i16/2, cos/sin by bit 0
+  int xo[2], yo[2];
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const int row = r0 + 4 * m + (i16 >> 2);
+    int kx = 0, ky = 0;
+    if (row < nrows) { kx = row_kx[row]; ky = row_ky[row]; }
+    xo[m] = kx * S + 2 * kq + ((i16 >> 1) & 1);
+    yo[m] = ky * S + 2 * kq + (i16 & 1);
+  }
+  const int zo = (i16 >> 1) * S + 2 * kq + (i16 & 1);
+  mfma_f32x4 acc[2][NA > 0 ? NA : 1];
+  float mid[2][NA > 0 ? NA : 1][4];
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int t = 0; t < NA; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) { acc[m][t][r] = 0.0f; mid[m][t][r] = 0; }
+  int nchunk = 0;
+  for (int jc = j0; jc < j1; jc += JC) {
+    sf_tables<JC>(Xf, Yf, Zf, KXT, KYT, KZT, jc, j1, nq, qanc, qbase);
+    __syncthreads();
+    if (NA > 0) {
+#pragma unroll 4
+      for (int s = 0; s < JC; s += 4) {
+        const float a0 = Xf[xo[0] + 2 * s] * Yf[yo[0] + 2 * s];
+        const float a1 = Xf[xo[1] + 2 * s] * Yf[yo[1] + 2 * s];
+        float b[NA > 0 ? NA : 1];
+#pragma unroll
+        for (int t = 0; t < NA; t++) b[t] = Zf[zo + 8 * t * S + 2 * s];
+#pragma unroll
+        for (int t = 0; t < NA; t++) {
+          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b[t], acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b[t], acc[1][t], 0, 0, 0);
+        }
+      }
+      // the matrix sums of 2 JC charges go to the second-level sums
+      if (++nchunk % 2 == 0 || jc + JC >= j1) {
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+          for (int t = 0; t < NA; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) { mid[m][t][r] += acc[m][t][r]; acc[m][t][r] = 0.0f; }
+      }
+    }
+    __syncthreads();
+  }
+  // lane l: row r0 + 4m + l/16, kz 8t + i16/2, T index 2r + i16%2
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const int row = r0 + 4 * m + kq;
+#pragma unroll
+    for (int t = 0; t < NA; t++) {
+      if (row >= blk_rows[t]) continue;
+      const int seg = blk_seg0[t] + row / 64;
+      double *out = kpart + (((size_t)blockIdx.y * nseg + seg) * 64 + row % 64) * 64 +
+                    (i16 >> 1) * 8 + (i16 & 1);
+#pragma unroll
+      for (int r = 0; r < 4; r++) out[2 * r] = (double)mid[m][t][r];
+    }
+  }
+}
+
+template <int NTMAX, int JC>
+__global__ __launch_bounds__(kBlock) void k_sfac_mf(
+    int nq, const double4 *__restrict__ qpos, int nrows, const int *__restrict__ row_kx,
+    const int *__restrict__ row_ky, const int *__restrict__ row_kzmax,
+    const int *__restrict__ blk_rows, const int *__restrict__ blk_seg0, int nseg, int KXT,
+    int KYT, int KZT, int jper, double *__restrict__ kpart, const float2 *__restrict__ qanc,
+    const float2 *__restrict__ qbase) {
+  extern __shared__ char smem[];
+  constexpr int S = SfTab<JC>::S;
+  float *Xf = (float *)smem, *Yf = Xf + KXT * S, *Zf = Yf + KYT * S;
+  const int r0 = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 8;
+  // N tiles of the unit (wave-uniform), 0 for a wave past the rows
+  int nt = 0;
+  if (r0 < nrows) nt = min((row_kzmax[r0] + 8) / 8, NTMAX);
+  nt = __builtin_amdgcn_readfirstlane(nt);
+  const int j0 = blockIdx.y * jper, j1 = min(nq, j0 + jper);
+#define MDX_U(NA)                                                                               \
+  sf_unit<NA, JC>(Xf, Yf, Zf, KXT, KYT, KZT, j0, j1, r0, nrows, row_kx, row_ky, blk_rows,       \
+                  blk_seg0, nseg, kpart, nq, qanc, qbase)
+  switch (nt) {
+    case 0: MDX_U(0); break;
+    case 1: MDX_U(1); break;
+    case 2: if constexpr (NTMAX >= 2) MDX_U(2); break;
+    case 3: if constexpr (NTMAX >= 3) MDX_U(3); break;
+    case 4: if constexpr (NTMAX >= 4) MDX_U(4); break;
+    case 5: if constexpr (NTMAX >= 5) MDX_U(5); break;
+    case 6: if constexpr (NTMAX >= 6) MDX_U(6); break;
+    case 7: if constexpr (NTMAX >= 7) MDX_U(7); break;
+    default: if constexpr (NTMAX >= 8) MDX_U(8); break;
+  }
+#undef MDX_U
 }
 
 // sum of the charge splits: tsum[i] = sum_s kpart[s*n + i]
@@ -694,14 +879,44 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   const int jper = (c.nq + c.nsplit - 1) / c.nsplit;
   dim3 g1((K.nseg + 3) / 4, c.nsplit);
   const double4 *qpos = (const double4 *)c.qpos.p;
+  // anchors of the matrix-core pass 1 (mixed mode, row units)
+  const bool anc = sizeof(R) == 4 && c.p1nt == 1;
+  const int3 nb = make_int3((KXT + 7) / 8, (KYT + 7) / 8, (KZT + 7) / 8);
+  const int3 boff = make_int3(0, nb.x, nb.x + nb.y);
+  if (anc && (size_t)(nb.x + nb.y + nb.z + 3) * c.nq > c.qanc.n)
+    return fail(MDX_E_ARG, "k-space: anchor buffer not set up");
   k_qgather<<<nblk(c.nq), kBlock, 0, st>>>(c.nq, c.ns, c.q_site.p, c.r.p, c.charge.p,
                                           c.polar ? c.chargepol.p : nullptr, 1 / c.L[0],
                                           1 / c.L[1], 1 / c.L[2], (double4 *)c.qpos.p,
-                                          (double4 *)c.qtrig.p);
+                                          (double4 *)c.qtrig.p, anc ? c.qanc.p : nullptr,
+                                          anc ? c.qanc.p + (size_t)(nb.x + nb.y + nb.z) * c.nq
+                                              : nullptr,
+                                          nb, boff);
   if (c.timing) c.tm.start(T_KPASS1, st);
   bool pass1_mfma = false;
+  int nsplit1 = c.nsplit;              // charge splits written to kpart
   if constexpr (sizeof(R) == 4) pass1_mfma = c.p1nt != 0;
-  if (pass1_mfma && c.p1nt == 8)
+  if (pass1_mfma && c.p1nt == 1) {
+    // row units of 8 rows, 4 waves per workgroup; splits for about 1024
+    // workgroups
+    const int wgx = ((K.nrows + 7) / 8 + 3) / 4;
+    nsplit1 = std::max(1, std::min(c.nsplit, 1024 / wgx));
+    const int jp = (c.nq + nsplit1 - 1) / nsplit1;
+    const dim3 gm(wgx, nsplit1);
+    constexpr int JC = 64;
+    const size_t shm = sizeof(float) * SfTab<JC>::S * (KXT + KYT + KZT);
+#define MDX_SF(NT)                                                                            \
+  k_sfac_mf<NT, JC><<<gm, kBlock, shm, st>>>(c.nq, qpos, K.nrows, c.d_row_kx.p, c.d_row_ky.p, \
+                                           c.d_row_kzmax.p, c.d_blk_rows.p, c.d_blk_seg0.p,  \
+                                           K.nseg, KXT, KYT, KZT, jp, c.kpart.p, c.qanc.p,   \
+                                           c.qanc.p + (size_t)(nb.x + nb.y + nb.z) * c.nq)
+    if (K.nblk <= 2) MDX_SF(2);
+    else if (K.nblk <= 4) MDX_SF(4);
+    else if (K.nblk <= 6) MDX_SF(6);
+    else if (K.nblk <= 8) MDX_SF(8);
+    else return fail(MDX_E_ARG, "k-space pass 1 on row units: at most 8 kz blocks (kz < 64)");
+#undef MDX_SF
+  } else if (pass1_mfma && c.p1nt == 8)
     k_sfac_mfma<8><<<dim3((K.nseg + 1) / 2, c.nsplit), kBlock, shmem, st>>>(
         c.nq, qpos, K.nseg, c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p,
         c.d_row_ky.p, KXT, KYT, KZT, jper, c.kpart.p, (const double4 *)c.qtrig.p);
@@ -717,7 +932,7 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   MDX_HIP(hipGetLastError());
   if (c.timing) { c.tm.stop(T_KPASS1, st); c.tm.start(T_KCOEF, st); }
   const size_t nt = (size_t)K.nseg * 64 * 64;
-  k_ksum<<<nblk(nt), kBlock, 0, st>>>(c.kpart.p, c.nsplit, nt, c.tsum.p);
+  k_ksum<<<nblk(nt), kBlock, 0, st>>>(c.kpart.p, nsplit1, nt, c.tsum.p);
   R *coef = nullptr;
   if constexpr (sizeof(R) == 4) coef = c.coef32.p; else coef = (R *)c.coef64.p;
   float *coefg = nullptr;
@@ -895,10 +1110,27 @@ int kspace_setup(Ctx &c) {
       (rc = c.d_seg_blk.alloc(K.nseg)) || (rc = c.d_seg_row0.alloc(K.nseg)) ||
       (rc = c.d_seg_rowend.alloc(K.nseg)) ||
       (rc = c.kpart.alloc((size_t)c.nsplit * K.nseg * 64 * 64)) ||
-      (rc = c.tsum.alloc((size_t)K.nseg * 64 * 64)))
+      (rc = c.tsum.alloc((size_t)K.nseg * 64 * 64)) || (rc = c.d_blk_rows.alloc(K.nblk)) ||
+      (rc = c.d_blk_seg0.alloc(K.nblk)))
     return rc;
+  {
+    // per 8-kz block: rows reaching it and its first segment
+    std::vector<int> br(K.nblk, 0), bs(K.nblk, -1);
+    for (int sg = 0; sg < K.nseg; sg++) {
+      const int b = K.seg_blk[sg];
+      if (bs[b] < 0) bs[b] = sg;
+      br[b] = std::max(br[b], K.seg_rowend[sg]);
+    }
+    MDX_HIP(hipMemcpy(c.d_blk_rows.p, br.data(), sizeof(int) * K.nblk, hipMemcpyHostToDevice));
+    MDX_HIP(hipMemcpy(c.d_blk_seg0.p, bs.data(), sizeof(int) * K.nblk, hipMemcpyHostToDevice));
+  }
   // coefficient rows: the padding slots stay zero (k_kcoef writes kz<=kzmax)
   if (c.mode == MDX_MIXED) {
+    // anchors and bases of the row-unit pass 1 (k_qgather), made here so
+    // that no call allocates (a captured step cannot)
+    const int KYT = K.row_ky.empty() ? 1 : *std::max_element(K.row_ky.begin(), K.row_ky.end()) + 1;
+    const size_t nanc = (size_t)((K.kxmax + 8) / 8 + (KYT + 7) / 8 + (K.kzpad + 7) / 8 + 3) * c.nq;
+    if ((rc = c.qanc.alloc(nanc))) return rc;
     if ((rc = c.coef32.alloc((size_t)K.ncoef * 16))) return rc;
     MDX_HIP(hipMemset(c.coef32.p, 0, sizeof(float) * K.ncoef * 16));
     // row-group blocks of the matrix-core pass 2 (padding stays zero)
