@@ -43,11 +43,11 @@ __device__ inline double wave_sum(double v) {
 }
 
 // sums NV per-thread values over the block into part[blockIdx.x*ACC_N+idx[k]]
-template <int NV>
+template <int NV, int NT = kBlock>
 __device__ inline void block_store(double (&v)[NV], const int (&idx)[NV], double *part,
                                    size_t row = (size_t)-1) {
   if (row == (size_t)-1) row = blockIdx.x;
-  __shared__ double red[kBlock / 64][NV];
+  __shared__ double red[NT / 64][NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
@@ -57,7 +57,7 @@ __device__ inline void block_store(double (&v)[NV], const int (&idx)[NV], double
   __syncthreads();
   if (threadIdx.x < NV) {
     double s = 0;
-    for (int w = 0; w < kBlock / 64; w++) s += red[w][threadIdx.x];
+    for (int w = 0; w < NT / 64; w++) s += red[w][threadIdx.x];
     part[row * ACC_N + idx[threadIdx.x]] += s;
   }
 }
@@ -366,6 +366,12 @@ template <typename R>
 struct PairTab {
   const R *lj;    // [nt*nt*8] Sq E4 E24 E48 C1q C2q A A4
   const R *erf;   // [nerf*8] f0u bu Du f0d bd Dd (stable form) or Au Bu Cu Ad Bd Cd
+  // FP32 force splines staged in LDS (structure of arrays f0d, bd, Dd of
+  // the first nsl intervals), read in place of erf when only forces are
+  // wanted; NULL: erf only
+  const float *sl = nullptr;
+  int nsl = 0;
+  double isgrid = 0;   // 1/sgrid
 };
 
 // eru/erd of the hyperbolic spline.  FP64: cook's form B/(C+x)+A
@@ -373,41 +379,52 @@ struct PairTab {
 // interval start x0 = i/sgrid, f0 - b t/(D+t) with D=C+x0, b=B/D,
 // f0=b+A, t=x-x0, which avoids the cancellation of B/(C+x) against A.
 template <typename R>
-__device__ inline void spline(const R *e, double rr, double sgrid, R &eru, R &erd, bool want_u);
+__device__ inline void spline(const PairTab<R> &tab, double rr, double sgrid, R &eru, R &erd,
+                              bool want_u);
 
 template <>
-__device__ inline void spline<double>(const double *e, double rr, double sgrid, double &eru,
-                                      double &erd, bool want_u) {
-  const double *p = e + 8 * (int)(rr * sgrid);
+__device__ inline void spline<double>(const PairTab<double> &tab, double rr, double sgrid,
+                                      double &eru, double &erd, bool want_u) {
+  const double *p = tab.erf + 8 * (int)(rr * sgrid);
   erd = p[4] / (p[5] + rr) + p[3];
   if (want_u) eru = p[1] / (p[2] + rr) + p[0];
 }
 
+// FP32 reciprocal of the hardware (1 ulp), for the FP32 terms
+__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <>
-__device__ inline void spline<float>(const float *e, double rr, double sgrid, float &eru,
-                                     float &erd, bool want_u) {
+__device__ inline void spline<float>(const PairTab<float> &tab, double rr, double sgrid,
+                                     float &eru, float &erd, bool want_u) {
   const int i = (int)(rr * sgrid);
-  const float *p = e + 8 * i;
-  const float t = (float)(rr - i / sgrid);
-  erd = p[3] - p[4] * t * __frcp_rn(p[5] + t);
-  if (want_u) eru = p[0] - p[1] * t * __frcp_rn(p[2] + t);
+  // interval start i/sgrid by the inverse grid (within an ulp of the
+  // quotient; t only enters the FP32 form)
+  const float t = (float)(rr - i * tab.isgrid);
+  if (!want_u && tab.sl && i < tab.nsl) {
+    const float f0 = tab.sl[i], b = tab.sl[tab.nsl + i], D = tab.sl[2 * tab.nsl + i];
+    erd = f0 - b * t * rcp32(D + t);
+    return;
+  }
+  const float *p = tab.erf + 8 * i;
+  erd = p[3] - p[4] * t * rcp32(p[5] + t);
+  if (want_u) eru = p[0] - p[1] * t * rcp32(p[2] + t);
 }
 
 template <typename R>
 __device__ inline R rcp(R x);
 template <>
-__device__ inline float rcp<float>(float x) { return __frcp_rn(x); }
+__device__ inline float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <>
 __device__ inline double rcp<double>(double x) { return 1.0 / x; }
 
 // one charge-charge term of the pair kernel: force/r from the spline at
 // the FP64 separation d, scattered to the accumulator of the central site
 template <typename R, bool MEASURE>
-__device__ inline void qq_term(const R *erf, double sgrid, R qq, const double (&d)[3],
+__device__ inline void qq_term(const PairTab<R> &tab, double sgrid, R qq, const double (&d)[3],
                                R (&acc)[3], double &Eel, double (&pv)[6]) {
   const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   R eru = 0, erd;
-  spline<R>(erf, rr, sgrid, eru, erd, MEASURE);
+  spline<R>(tab, rr, sgrid, eru, erd, MEASURE);
   const R fr = qq * erd;
   const R dx = (R)d[0], dy = (R)d[1], dz = (R)d[2];
   const R fx = fr * dx, fy = fr * dy, fz = fr * dz;
@@ -468,7 +485,7 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
   const R qq = qi * (R)pj.w;
   if (qq != 0) {
     R eru = 0, erd;
-    spline<R>(tab.erf, rr, sgrid, eru, erd, MEASURE);
+    spline<R>(tab, rr, sgrid, eru, erd, MEASURE);
     fr += qq * erd;
     if (MEASURE) Eel += (double)(qq * eru);
   }
@@ -484,16 +501,16 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
     const R qpj = (R)rj.w, qj = (R)pj.w;
     if (qpi != 0 && qj != 0) {        // Drude i - centre j
       const double d[3] = {dxr + ri.x, dyr + ri.y, dzr + ri.z};
-      qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qj, d, ap, Eel, pv);
+      qq_term<R, MEASURE>(tab, sgrid, qpi * qj, d, ap, Eel, pv);
     }
     if (qpj != 0) {
       if (qi != 0) {                  // centre i - Drude j
         const double d[3] = {dxr - rj.x, dyr - rj.y, dzr - rj.z};
-        qq_term<R, MEASURE>(tab.erf, sgrid, qi * qpj, d, ac, Eel, pv);
+        qq_term<R, MEASURE>(tab, sgrid, qi * qpj, d, ac, Eel, pv);
       }
       if (qpi != 0) {                 // Drude i - Drude j
         const double d[3] = {dxr - rj.x + ri.x, dyr - rj.y + ri.y, dzr - rj.z + ri.z};
-        qq_term<R, MEASURE>(tab.erf, sgrid, qpi * qpj, d, ap, Eel, pv);
+        qq_term<R, MEASURE>(tab, sgrid, qpi * qpj, d, ap, Eel, pv);
       }
     }
   }
@@ -593,6 +610,212 @@ __global__ __launch_bounds__(kBlock) void k_pair(
     double v[2] = {0.5 * npair, 0.5 * nlj};
     const int idx[2] = {ACC_NPAIR, ACC_NLJ};
     block_store<2>(v, idx, part, row);
+  }
+}
+
+// Cell-column variant of k_pair: the same pairs, terms and summation
+// order, with the neighbour sites staged in LDS.  A workgroup takes the
+// sites of one (x, y) column of cells, kPairBlock at a time (a round); for
+// each stencil column (dx, dy) of its part it copies the sites of the cells
+// z-w .. z+w around the round's cells into LDS, the image shift already
+// added (FP64 positions, so the separations are those of k_pair), then
+// each lane walks the contiguous range of its own window: 32 candidates at
+// a time are tested (FP64 rr < cutoff^2, not the site itself) into a bit
+// mask, and the pair terms run only for the set bits, so the lanes of a
+// wave spend little time on candidates another lane accepts.  Pairs within
+// rounding of the cutoff are listed for k_census_fix as in k_pair.  A
+// round whose window holds more than kMaxJ sites reads them from global
+// memory instead (same arithmetic, cell by cell).
+constexpr int kPairBlock = 512;
+constexpr int kMaxJ = 768;
+constexpr int kMaxSplineLds = 48 * 1024;   // bytes of force splines staged in LDS
+constexpr int kMaxZ = 64;      // buffer cells of a round (z cells + 2w)
+
+template <typename R, bool MEASURE, bool POLAR, bool E6>
+__global__ __launch_bounds__(kPairBlock) void k_pairc(
+    int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
+    const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
+    const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
+    double sgrid, double *__restrict__ f, double *__restrict__ part, double band,
+    int4 *__restrict__ blist, int *__restrict__ bcount, int bcap) {
+  // kMaxJ staged sites and 32 far-away fillers after the last, so that a
+  // batch of 32 candidates never reads past the buffer
+  __shared__ double4 jpos[kMaxJ + 32];
+  __shared__ int4 jinf[kMaxJ];
+  __shared__ double4 jpol[POLAR ? kMaxJ : 1];
+  __shared__ int joff[kMaxZ + 1];
+  __shared__ int jsrc[kMaxZ];
+  __shared__ double jsz[kMaxZ];
+  extern __shared__ float lsp[];       // force splines (nsl intervals x 3), see PairTab
+  if (tab.nsl) {
+    for (int i = threadIdx.x; i < tab.nsl; i += kPairBlock) {
+      const float *e = (const float *)tab.erf + 8 * i;
+      lsp[i] = e[3];
+      lsp[tab.nsl + i] = e[4];
+      lsp[2 * tab.nsl + i] = e[5];
+    }
+    tab.sl = lsp;
+  }
+  const int cxy = blockIdx.x, cx = cxy / g.ny, cy = cxy % g.ny;
+  const int c0 = cxy * g.nz;
+  const int span = 2 * w + 1, col0 = blockIdx.y * cpp, col1 = min(col0 + cpp, span * span);
+  const int ib0 = cstart[c0], ib1 = cstart[c0 + g.nz];
+  const int lane = threadIdx.x & 63;
+  double U = 0, Eel = 0, Vir = 0, pv[6] = {0, 0, 0, 0, 0, 0};
+  double npair = 0, nlj = 0;
+  for (int rb = ib0; rb < ib1; rb += kPairBlock) {
+    const int s = rb + threadIdx.x;
+    const bool act = s < ib1;
+    const int sl = act ? s : rb;
+    const double4 pi = spos[sl];
+    double4 ri = make_double4(0, 0, 0, 0);
+    if (POLAR) ri = spol[sl];
+    const int4 ii = sinfo[sl];
+    const R qi = (R)pi.w, qpi = (R)ri.w;
+    const R *ljrow = tab.lj + (size_t)(ii.x & 0xffff) * nt * 8;
+    const int czi = scell[sl] - c0;
+    // z cells of the round and its buffer cells z0-w .. z1+w
+    const int z0 = scell[rb] - c0, z1 = scell[min(rb + kPairBlock, ib1) - 1] - c0;
+    const int nzb = z1 - z0 + 2 * w + 1;
+    double fx = 0, fy = 0, fz = 0, px = 0, py = 0, pz = 0;
+    for (int col = col0; col < col1; col++) {
+      const int dx = col / span - w, dy = col % span - w;
+      int nx = cx + dx, ny = cy + dy;
+      double sx = 0, sy = 0;
+      if (nx < 0) { nx += g.nx; sx = -g.Lx; } else if (nx >= g.nx) { nx -= g.nx; sx = g.Lx; }
+      if (ny < 0) { ny += g.ny; sy = -g.Ly; } else if (ny >= g.ny) { ny -= g.ny; sy = g.Ly; }
+      const int nb0 = (nx * g.ny + ny) * g.nz;
+      __syncthreads();                 // the previous column's buffer is done
+      if (threadIdx.x < 64) {
+        // buffer cell k = lane: its sites, shift and offset (a wave scan)
+        int n = 0;
+        if (lane < nzb) {
+          int z = z0 - w + lane;
+          double sz = 0;
+          if (z < 0) { z += g.nz; sz = -g.Lz; } else if (z >= g.nz) { z -= g.nz; sz = g.Lz; }
+          const int a = cstart[nb0 + z];
+          n = cstart[nb0 + z + 1] - a;
+          jsrc[lane] = a;
+          jsz[lane] = sz;
+        }
+        int incl = n;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += v;
+        }
+        if (lane < nzb) joff[lane + 1] = incl;
+        if (lane == 0) joff[0] = 0;
+      }
+      __syncthreads();
+      const bool lds = joff[nzb] <= kMaxJ;
+      if (lds) {
+        if (threadIdx.x < 32) jpos[joff[nzb] + threadIdx.x] = make_double4(1e30, 1e30, 1e30, 0);
+        for (int k = 0; k < nzb; k++) {
+          const int n = joff[k + 1] - joff[k], src = jsrc[k], dst = joff[k];
+          const double sz = jsz[k];
+          for (int m = threadIdx.x; m < n; m += kPairBlock) {
+            const double4 p = spos[src + m];
+            jpos[dst + m] = make_double4(p.x + sx, p.y + sy, p.z + sz, p.w);
+            jinf[dst + m] = sinfo[src + m];
+            if (POLAR) jpol[dst + m] = spol[src + m];
+          }
+        }
+      }
+      __syncthreads();
+      if (!act) continue;
+      R ac[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
+      // this site's window: buffer cells k0 .. k0+2w
+      const int k0 = czi - z0;
+      if (lds) {
+        // one contiguous range; the site itself sits in its own cell of the
+        // (0, 0) column
+        const int jlo = joff[k0], jhi = joff[k0 + 2 * w + 1];
+        const int self = (dx == 0 && dy == 0) ? joff[k0 + w] + (s - jsrc[k0 + w]) : -1;
+        const double cqb = cq + band;
+        for (int jb = jlo; jb < jhi; jb += 32) {
+          // candidates inside the cutoff or within rounding of it, as bits
+          unsigned mask = 0;
+#pragma unroll 8
+          for (int t = 0; t < 32; t++) {
+            const double4 pj = jpos[jb + t];
+            const double dxr = pi.x - pj.x, dyr = pi.y - pj.y, dzr = pi.z - pj.z;
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            mask |= (unsigned)(rr < cqb) << t;
+          }
+          const int nb = jhi - jb;
+          if (nb < 32) mask &= (1u << nb) - 1;
+          if ((unsigned)(self - jb) < 32u) mask &= ~(1u << (self - jb));
+          while (mask) {
+            const int j = jb + __ffs(mask) - 1;
+            mask &= mask - 1;
+            const double4 pj = jpos[j];
+            const int4 jj = jinf[j];
+            const double dxr = pi.x - pj.x, dyr = pi.y - pj.y, dzr = pi.z - pj.z;
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            if (rr > cq - band) {
+              const int slot = atomicAdd(bcount, 1);
+              if (slot < bcap) blist[slot] = make_int4(ii.w, jj.w, rr < cq, 0);
+              if (rr >= cq) continue;
+            }
+            npair += 1;
+            double4 rj = make_double4(0, 0, 0, 0);
+            if (POLAR) rj = jpol[j];
+            pair_body<R, MEASURE, POLAR, E6>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr,
+                                             dyr, dzr, rr, ac, ap, U, Eel, Vir, pv, nlj);
+          }
+        }
+      } else {
+        for (int k = k0; k <= k0 + 2 * w; k++) {
+          const int src = jsrc[k], n = joff[k + 1] - joff[k];
+          const double sz = jsz[k];
+          for (int m = 0; m < n; m++) {
+            if (dx == 0 && dy == 0 && k == k0 + w && src + m == s) continue;
+            double4 pj = spos[src + m];
+            pj.x += sx; pj.y += sy; pj.z += sz;
+            const double dxr = pi.x - pj.x, dyr = pi.y - pj.y, dzr = pi.z - pj.z;
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            if (rr >= cq + band) continue;
+            const int4 jj = sinfo[src + m];
+            if (rr > cq - band) {
+              const int slot = atomicAdd(bcount, 1);
+              if (slot < bcap) blist[slot] = make_int4(ii.w, jj.w, rr < cq, 0);
+              if (rr >= cq) continue;
+            }
+            npair += 1;
+            double4 rj = make_double4(0, 0, 0, 0);
+            if (POLAR) rj = spol[src + m];
+            pair_body<R, MEASURE, POLAR, E6>(ii, ljrow, qi, qpi, ri, tab, sgrid, pj, jj, rj, dxr,
+                                             dyr, dzr, rr, ac, ap, U, Eel, Vir, pv, nlj);
+          }
+        }
+      }
+      fx += ac[0]; fy += ac[1]; fz += ac[2];
+      if (POLAR) { px += ap[0]; py += ap[1]; pz += ap[2]; }
+    }
+    if (act) {
+      const int site = ii.w;
+      double *o = f + (size_t)blockIdx.y * (POLAR ? 2 : 1) * ns * 3;
+      o[3 * site] = fx;
+      o[3 * site + 1] = fy;
+      o[3 * site + 2] = fz;
+      if (POLAR) {
+        o[3 * (ns + site)] = px;
+        o[3 * (ns + site) + 1] = py;
+        o[3 * (ns + site) + 2] = pz;
+      }
+    }
+  }
+  const size_t row = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (MEASURE) {
+    double v[12] = {0.5 * U, 0.5 * Eel, 0.5 * Vir, 0.5 * pv[0], 0.5 * pv[1], 0.5 * pv[2],
+                    0.5 * pv[3], 0.5 * pv[4], 0.5 * pv[5], 0.5 * npair, 0.5 * nlj, 0};
+    const int idx[12] = {ACC_U, ACC_EEL, ACC_VIR, ACC_PV0, ACC_PV0 + 1, ACC_PV0 + 2,
+                         ACC_PV0 + 3, ACC_PV0 + 4, ACC_PV0 + 5, ACC_NPAIR, ACC_NLJ, ACC_SUMQ};
+    block_store<12, kPairBlock>(v, idx, part, row);
+  } else {
+    double v[2] = {0.5 * npair, 0.5 * nlj};
+    const int idx[2] = {ACC_NPAIR, ACC_NLJ};
+    block_store<2, kPairBlock>(v, idx, part, row);
   }
 }
 
@@ -952,7 +1175,7 @@ int alloc_grid(Ctx &c) {
   c.gen++;                             // launch shapes and buffers change
   // partial rows: one per workgroup of the pair, exclusion, k-space
   // coefficient (2 per segment) and dipole kernels
-  c.part_blocks = std::max(nblk(c.ns) * np, nblk(c.nmol));
+  c.part_blocks = std::max({nblk(c.ns) * np, nblk(c.nmol), c.nc[0] * c.nc[1] * np});
   c.part_blocks = std::max({c.part_blocks, 1024, nblk(c.nq), 2 * c.K.nseg});
   if (np > 1 && (rc = c.pf.alloc((size_t)np * 3 * (c.polar ? 2 : 1) * c.ns))) return rc;
   return c.part.alloc((size_t)c.part_blocks * ACC_N);
@@ -980,6 +1203,7 @@ int launch_pair_t(Ctx &c, int measure) {
   PairTab<R> tab;
   if constexpr (sizeof(R) == 4) { tab.lj = c.lj32.p; tab.erf = c.erf32.p; }
   else { tab.lj = (const R *)c.lj64.p; tab.erf = (const R *)c.erf64.p; }
+  tab.isgrid = 1 / c.erf_sgrid;
   const double cq = c.cutoff * c.cutoff;
   const int np = pair_parts(c);
   const dim3 grid(nblk(c.ns), np);
@@ -997,7 +1221,31 @@ int launch_pair_t(Ctx &c, int measure) {
   } else {
     const int ncol = (2 * c.wst + 1) * (2 * c.wst + 1);
     const int cpp = c.pcols > 0 ? std::min(c.pcols, ncol) : ncol;
-    if (measure)
+    if (c.nc[2] + 2 * c.wst <= kMaxZ) {
+      // cell columns with the neighbours staged in LDS; the FP32 force
+      // splines up to the cutoff too (unmeasured calls without Drudes,
+      // whose terms reach past the cutoff)
+      const dim3 gc(c.nc[0] * c.nc[1], np);
+      const double band = make_cook_grid(c).band;
+      size_t shm = 0;
+      if constexpr (sizeof(R) == 4) {
+        const int nsl = std::min(c.nerf, (int)((cq + band) * c.erf_sgrid) + 2);
+        if (!measure && !POLAR && nsl * 12 <= kMaxSplineLds) {
+          tab.nsl = nsl;
+          shm = (size_t)nsl * 12;
+        }
+      }
+      if (measure)
+        k_pairc<R, true, POLAR, E6><<<gc, kPairBlock, shm, c.stream>>>(
+            c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+            (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
+            c.part.p, band, (int4 *)c.blist.p, c.bcount.p, c.bcap);
+      else
+        k_pairc<R, false, POLAR, E6><<<gc, kPairBlock, shm, c.stream>>>(
+            c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
+            (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
+            c.part.p, band, (int4 *)c.blist.p, c.bcount.p, c.bcap);
+    } else if (measure)
       k_pair<R, true, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
           (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
