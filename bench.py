@@ -25,7 +25,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-PEAK = {"fp32_valu": 157.3e12, "fp64_valu": 78.6e12}
+PEAK = {"fp32_valu": 157.3e12, "fp64_valu": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
 # stencil w, stencil columns per pair part, kz chunk, k parts, matrix-core pass 2
 PROFILED_PIN = {"mixed": "2,3,24,16,1", "fp64": "2,3,8,64,0"}
 
@@ -56,6 +56,11 @@ def parse():
                          "with 16 or 8 tiles per wave (-1: as pinned or tuned)")
     ap.add_argument("--tune", action="store_true",
                     help="c2: run the autotuner instead of the profiled configuration")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="k-space on the same stream as the pair kernel (each kernel's own "
+                         "duration under a profiler)")
+    ap.add_argument("--no-fp64-ref", action="store_true",
+                    help="mixed mode: skip the FP64 path's rate and the mixed-vs-FP64 force error")
     ap.add_argument("--forces-only", type=int, default=0,
                     help="run this many force calls (no MD, no JSON line), for counter passes")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -103,6 +108,18 @@ def traffic_lookup(workload, mode, kernel, info):
             continue
         return e["bytes"]
     return None
+
+
+def cpu_model():
+    """the host CPU's model name (/proc/cpuinfo)"""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(wl, bs_oracle, kstride, threads):
@@ -290,6 +307,8 @@ def main():
         tune = eng.autotune(wl["r_full"], reps=2)[:4]
     if a.kpass1 >= 0 and mode == abi.MDX_MIXED:
         eng.set_kspace_pass1(a.kpass1)
+    if a.no_overlap:
+        eng.set_overlap(False)
     info = eng.info()
     if a.forces_only:
         for _ in range(a.forces_only):
@@ -356,8 +375,62 @@ def main():
     achieved = work[dom] / t_dom
     force_ms = sum(kern[k][0] for k in ("bin", "pair", "excl", "kpass1", "kcoef", "kpass2") if k in kern)
 
+    # every hot kernel against its own roof: FLOP/s for the pair kernel
+    # and both k-space passes (matrix cores where they run there), HBM
+    # bytes/s for binning (44 B per site + 8 per cell) and the SHAKE step
+    # (96 B per site: r, p, r1 read, r, r1 written)
+    ncell = info["ncx"] * info["ncy"] * info["ncz"]
+    mfma = {"kpass1": bool(info["kpass1_tiles"]), "kpass2": bool(info["kmfma"])}
+    rooflines = {}
+    for k in ("pair", "kpass1", "kpass2"):
+        if k not in kern:
+            continue
+        pk = PEAK["fp32_mfma"] if (mode == abi.MDX_MIXED and mfma.get(k)) else peak
+        ach = work[k] / (kern[k][0] * 1e-3)
+        rooflines[k] = {"bound": "mfma" if (mode == abi.MDX_MIXED and mfma.get(k)) else "valu",
+                        "ms": kern[k][0], "achieved": ach / 1e12, "peak": pk / 1e12,
+                        "unit": "TFLOP/s", "frac": ach / pk, "work_per_launch": work[k]}
+    bytes_ = {"bin": 44.0 * bs.struct.ns + 8.0 * ncell, "shake": 96.0 * bs.struct.ns}
+    for k, b_ in bytes_.items():
+        if k in kern:
+            ach = b_ / (kern[k][0] * 1e-3)
+            rooflines[k] = {"bound": "hbm", "ms": kern[k][0], "achieved": ach / 1e9,
+                            "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": ach / PEAK["hbm"],
+                            "bytes_per_launch": b_}
+
+    # the plain FP64 GPU path on the same system and steps, and the
+    # mixed-vs-FP64 error of one force call on the starting configuration
+    fp64_ref = None
+    if mode == abi.MDX_MIXED and not a.no_fp64_ref:
+        eng64 = Engine(bs, abi.MDX_FP64)
+        pin64 = [int(x) for x in PROFILED_PIN["fp64"].split(",")] if a.workload == "c2" else None
+        if pin64:
+            eng64.set_stencil(pin64[0], pin64[1])
+            eng64.set_kchunk(pin64[2], pin64[3])
+        else:
+            eng64.autotune(wl["r_full"], reps=2)
+        f64, o64 = eng64.forces(wl["r_full"], measure=1)
+        eng64.upload_state(wl["r_full"], r1)
+        if a.warmup:
+            p.measure_every = a.warmup
+            eng64.md_steps(p, a.warmup)
+        p.measure_every = a.steps
+        t0 = time.perf_counter()
+        eng64.md_steps(p, a.steps)
+        ms64 = (time.perf_counter() - t0) * 1e3 / a.steps
+        eng64.close()
+        E0, E64 = out0.En_pot + out0.En_el, o64.En_pot + o64.En_el
+        pscale = max(abs(x) for x in o64.Pvir[:3])
+        fp64_ref = {"ns_per_day": 86400.0 / (ms64 * 1e-3) * h * 1e-3, "ms_per_step": ms64,
+                    "mixed_over_fp64": ms64 / ms_step,
+                    "rel_rms_force_err": float(np.sqrt(((f0 - f64) ** 2).sum() / (f64 ** 2).sum())),
+                    "rel_energy_err": abs(E0 - E64) / abs(E64),
+                    "rel_virial_tensor_err": max(abs(x - y) for x, y in zip(out0.Pvir, o64.Pvir)) / pscale,
+                    "same_census": out0.npair == o64.npair}
+
     cpu = None
     if not a.no_cpu_baseline:
+        # the host's CPU share of one GPU on the box (16 threads)
         threads = min(16, os.cpu_count() or 1)
         kstride = a.kstride or max(1, nq // 48000)
         sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -366,7 +439,7 @@ def main():
                            provider=oracle.provider())
         t_force, detail = cpu_baseline(wl, bso, kstride, threads)
         cpu = {"value": 86400.0 / t_force * h * 1e-3, "unit": "ns/day", "cores": threads,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": "oracle/ FP64 restatement, one force call of the same system: full "
                          "r-space pass (%.2f s) + k-space over %d of %d charges (%.2f s) scaled "
                          "to all charges; SHAKE step cost not included" % (
@@ -396,9 +469,13 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_lookup(wl["name"], a.mode, dom, info),
                      "work_per_launch": work[dom]},
+        "rooflines": rooflines,
+        "fp64_path": fp64_ref,
         "cpu_baseline": cpu,
         "energy_K": {"En_pot": out.frc.En_pot + out.frc.En_el, "En_kin": out.En_kin},
     }
+    if a.no_overlap:
+        line["config"]["kspace_overlap"] = False
     if a.form != "lj":
         line["config"]["site_site_form"] = a.form
     if rdf_cost:
