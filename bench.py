@@ -27,7 +27,8 @@ sys.path.insert(0, REPO)
 
 PEAK = {"fp32_valu": 157.3e12, "fp64_valu": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
 # stencil w, stencil columns per pair part, kz chunk, k parts, matrix-core pass 2
-PROFILED_PIN = {"mixed": "2,3,24,16,1", "fp64": "2,3,8,64,0"}
+# (0: FMA chains, 1 or 2: charge tiles of 32 per wave)
+PROFILED_PIN = {"mixed": "2,3,24,16,2", "fp64": "2,3,8,64,0"}
 
 
 def parse():
@@ -298,7 +299,7 @@ def main():
         eng.set_stencil(w, pcols)
         eng.set_kchunk(kzc, kparts)
         if mode == abi.MDX_MIXED:
-            eng.set_kspace_mfma(len(pin) > 4 and pin[4] != 0)
+            eng.set_kspace_mfma(pin[4] if len(pin) > 4 else 0)
         tune = (w, kzc, kparts, pcols)
     else:
         # the engine's autotuner picks the cell stencil, its split over

@@ -333,8 +333,10 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    kz chunks, parts asked, parts made, stencil columns per pair part,
    matrix-core pass 2 (0/1), pass-1 tiles (0 = FMA; n may be 19).
    mdx_set_kspace_mfma (MDX_MIXED): k-space pass 2 on the matrix cores
-   (FP32 MFMA, FP32 products and sums as the FMA variant) or as FP32 FMA
-   chains; mdx_autotune tries both.  Default: on in MDX_MIXED.
+   (FP32 MFMA, FP32 products and sums as the FMA variant) with on = 1 or
+   2 tiles of 32 charges per wave (2 share each coefficient operand), or
+   as FP32 FMA chains (on = 0); mdx_autotune tries all three.  Default: 2
+   in MDX_MIXED.
    mdx_set_kspace_pass1 (MDX_MIXED): k-space pass 1 as FP32 FMA chains
    (tiles = 0), on the matrix cores as units of 8 k rows with all their
    kz per wave (tiles = 1, the default where kz stays below 64: FP32 MFMA

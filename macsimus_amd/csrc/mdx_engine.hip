@@ -1468,7 +1468,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     return code;
   };
   c.mode = mode;
-  c.kmfma = mode == MDX_MIXED;       // pass 2 on the matrix cores by default
+  c.kmfma = mode == MDX_MIXED ? 2 : 0;   // pass 2 on the matrix cores, 64 charges per wave
   (void)hipGetDevice(&c.device);
   // set-up of this device: the address of the no-allocation adjoint's scratch
   if ((rc = adjoint_scratch_setup(c.device, nullptr))) return bail(rc);
@@ -1807,7 +1807,7 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
   };
   c.allpair = false;
   int rc = 0, best_w = c.wst, best_pc = c.pcols, best_k = c.kzc, best_p = c.kparts;
-  bool best_m = c.kmfma;
+  int best_m = c.kmfma;
   double tw = 1e30, tk = 1e30, fam[T_NFAM];
   for (int w = 1; w <= 3 && !rc; w++) {
     if (!grid_fits(c, w)) continue;
@@ -1826,8 +1826,8 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
     const int cand[] = {8, 16, 24, 32, 48};
     const int pcand[] = {4, 8, 16, 32, 64};
     // MDX_MIXED: the FP32 FMA and the matrix-core variants of pass 2
-    for (int mf = 0; mf < (c.mode == MDX_MIXED ? 2 : 1) && !rc; mf++) {
-      c.kmfma = mf != 0;
+    for (int mf = 0; mf < (c.mode == MDX_MIXED ? 3 : 1) && !rc; mf++) {
+      c.kmfma = mf;
       for (int kzc : cand)
         for (int parts : pcand) {
           if (rc) break;
@@ -1872,7 +1872,8 @@ extern "C" int mdx_set_kspace_pass1(mdx_ctx_t *h, int tiles) {
 extern "C" int mdx_set_kspace_mfma(mdx_ctx_t *h, int on) {
   if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_mfma: NULL");
   if (on && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_mfma: MDX_MIXED only");
-  h->c.kmfma = on != 0;
+  if (on < 0 || on > 2) return fail(MDX_E_ARG, "mdx_set_kspace_mfma: 0, 1 or 2");
+  h->c.kmfma = on;
   h->c.gen++;
   return 0;
 }
@@ -1930,7 +1931,7 @@ extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   Ctx &c = h->c;
   int v[19] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
                c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart,
-               c.pcols, c.kmfma ? 1 : 0, c.p1nt};
+               c.pcols, c.kmfma, c.p1nt};
   for (int i = 0; i < n && i < 19; i++) iv[i] = v[i];
   return 0;
 }

@@ -236,7 +236,7 @@ struct Ctx {
   // MDX_MIXED: pass 2 on the matrix cores (k_kforce_mfma) instead of FP32
   // FMA chains, from the coefficients also written per row group in the
   // operand order of the matrix instruction (coefg, zero padded)
-  bool kmfma = false;
+  int kmfma = 0;                // 0: FMA chains; 1, 2: matrix cores, charge tiles of 32 per wave
   int p1nt = 0;                 // MDX_MIXED pass 1: FP32 FMA (0) or matrix cores, 16 or 8 tiles per wave
   DBuf<int> d_grp_start, d_grp_base, d_grp_nb;
   DBuf<float> coefg;

@@ -744,29 +744,35 @@ __global__ __launch_bounds__(kBlock) void k_kforce(
 // gfx950 the results are the same bits).
 typedef float mfma_f32x16 __attribute__((ext_vector_type(16)));
 
-template <int KZC>
-__global__ __launch_bounds__(kBlock, 4) void k_kforce_mfma(
+template <int KZC, int NTC>
+__global__ __launch_bounds__(kBlock, NTC == 1 ? 4 : 3) void k_kforce_mfma(
     int nq, const double4 *__restrict__ qpos, double iLx, double iLy, double iLz, int kxmax,
     const int4 *__restrict__ parts, const int *__restrict__ chunk_kymax,
     const int *__restrict__ grp_start, const int *__restrict__ grp_base,
     const int *__restrict__ grp_nb, const float *__restrict__ coefg, double *__restrict__ kpf) {
+  // NTC tiles of 32 charges per wave share each coefficient operand: the
+  // products of tile u go to acc[u]
   constexpr int BPC = KZC / kKB;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = lane & 31, h = lane >> 5;
-  const int j = (blockIdx.x * (kBlock / 64) + wv) * 32 + c;
+  const int jw = (blockIdx.x * (kBlock / 64) + wv) * 32 * NTC + c;
   const int4 part = parts[blockIdx.y];
   const int ch = part.x;
-  const double4 qp = qpos[min(j, nq - 1)];
-  const double sx = frac1(qp.x * iLx), sy = frac1(qp.y * iLy), sz = frac1(qp.z * iLz);
   const float kxs = (float)iLx, kys = (float)iLy;
-  // e^{2 pi i sy} to the powers 1, 2, 4
-  double cb1, sb1;
-  cs2pi(sy, cb1, sb1);
-  const double cb2 = cb1 * cb1 - sb1 * sb1, sb2 = 2 * cb1 * sb1;
-  const double cb4 = cb2 * cb2 - sb2 * sb2, sb4 = 2 * cb2 * sb2;
-  // this lane's half of the kz table: cos (h = 0) or sin (h = 1)
-  float zt[KZC];
-  {
+  double sx[NTC], cb1[NTC], sb1[NTC], cb2[NTC], sb2[NTC], cb4[NTC], sb4[NTC];
+  float zt[NTC][KZC];
+#pragma unroll
+  for (int u = 0; u < NTC; u++) {
+    const double4 qp = qpos[min(jw + 32 * u, nq - 1)];
+    sx[u] = frac1(qp.x * iLx);
+    const double sy = frac1(qp.y * iLy), sz = frac1(qp.z * iLz);
+    // e^{2 pi i sy} to the powers 1, 2, 4
+    cs2pi(sy, cb1[u], sb1[u]);
+    cb2[u] = cb1[u] * cb1[u] - sb1[u] * sb1[u];
+    sb2[u] = 2 * cb1[u] * sb1[u];
+    cb4[u] = cb2[u] * cb2[u] - sb2[u] * sb2[u];
+    sb4[u] = 2 * cb2[u] * sb2[u];
+    // this lane's half of the kz table: cos (h = 0) or sin (h = 1)
     double cz1, sz1;
     cs2pi(sz, cz1, sz1);
 #pragma unroll
@@ -775,7 +781,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_kforce_mfma(
       cs2pi((double)(ch * KZC + 8 * m) * sz, cc, ss);
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        zt[8 * m + k] = (float)(h ? ss : cc);
+        zt[u][8 * m + k] = (float)(h ? ss : cc);
         const double nc = cc * cz1 - ss * sz1;
         ss = cc * sz1 + ss * cz1;
         cc = nc;
@@ -784,23 +790,32 @@ __global__ __launch_bounds__(kBlock, 4) void k_kforce_mfma(
   }
   const int *kym_c = chunk_kymax + ch * (kxmax + 1);
   const float *cfl = coefg + lane + (size_t)ch * KZC * 64;
-  double Fx = 0, Fy = 0, Fz = 0;
+  double Fx[NTC], Fy[NTC], Fz[NTC];
+#pragma unroll
+  for (int u = 0; u < NTC; u++) { Fx[u] = 0; Fy[u] = 0; Fz[u] = 0; }
   for (int kx = part.y; kx < part.z; kx++) {
     const int kym = kym_c[kx];
-    double ca, sa;
-    cs2pi(kx * sx, ca, sa);
-    const float rca = (float)ca, rsa = (float)sa;
+    float rca[NTC], rsa[NTC];
+    double cb[NTC], sb[NTC];
+#pragma unroll
+    for (int u = 0; u < NTC; u++) {
+      double ca, sa;
+      cs2pi(kx * sx[u], ca, sa);
+      rca[u] = (float)ca; rsa[u] = (float)sa;
+      // phase of row ky0 + h
+      cb[u] = h ? cb1[u] : 1.0; sb[u] = h ? sb1[u] : 0.0;
+    }
     const float kxl = (float)kx * kxs;
     const int g0 = grp_start[kx];
-    // phase of row ky0 + h
-    double cb = h ? cb1 : 1.0, sb = h ? sb1 : 0.0;
     for (int ky0 = 0; ky0 <= kym; ky0 += 4) {
       const int gi = g0 + (ky0 >> 2);
       const int nb = min(grp_nb[gi] - ch * BPC, BPC);   // wave-uniform, >= 1
       const float *ga = cfl + grp_base[gi];
-      mfma_f32x16 acc;
+      mfma_f32x16 acc[NTC];
 #pragma unroll
-      for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+      for (int u = 0; u < NTC; u++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[u][r] = 0.0f;
 #pragma unroll
       for (int b = 0; b < BPC; b++) {
         if (b < nb) {
@@ -809,36 +824,48 @@ __global__ __launch_bounds__(kBlock, 4) void k_kforce_mfma(
           for (int k = 0; k < kKB; k++) av[k] = ga[(b * kKB + k) * 64];
 #pragma unroll
           for (int k = 0; k < kKB; k++)
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k], zt[b * kKB + k], acc, 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < NTC; u++)
+              acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k], zt[u][b * kKB + k], acc[u], 0, 0, 0);
         }
       }
       // rows ky0 + h (registers 0-7) and ky0 + 2 + h (8-15); rows past kym
       // hold zeros
 #pragma unroll
-      for (int q = 0; q < 2; q++) {
-        const float G0 = acc[8 * q], G1 = acc[8 * q + 1], G2 = acc[8 * q + 2], G3 = acc[8 * q + 3];
-        const float H0 = acc[8 * q + 4], H1 = acc[8 * q + 5], H2 = acc[8 * q + 6], H3 = acc[8 * q + 7];
-        const double cq = q ? cb * cb2 - sb * sb2 : cb, sq = q ? cb * sb2 + sb * cb2 : sb;
-        const float rcb = (float)cq, rsb = (float)sq;
-        const float kyl = (float)(ky0 + 2 * q + h) * kys;
-        Fx += (double)(kxl * (rca * (rcb * G2 + rsb * G3) - rsa * (rcb * G0 + rsb * G1)));
-        Fy += (double)(kyl * (rca * (rcb * G1 - rsb * G0) + rsa * (rcb * G3 - rsb * G2)));
-        Fz += (double)(rca * (rcb * H0 + rsb * H1) + rsa * (rcb * H2 + rsb * H3));
+      for (int u = 0; u < NTC; u++) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          const float G0 = acc[u][8 * q], G1 = acc[u][8 * q + 1], G2 = acc[u][8 * q + 2],
+                      G3 = acc[u][8 * q + 3];
+          const float H0 = acc[u][8 * q + 4], H1 = acc[u][8 * q + 5], H2 = acc[u][8 * q + 6],
+                      H3 = acc[u][8 * q + 7];
+          const double cq = q ? cb[u] * cb2[u] - sb[u] * sb2[u] : cb[u];
+          const double sq = q ? cb[u] * sb2[u] + sb[u] * cb2[u] : sb[u];
+          const float rcb = (float)cq, rsb = (float)sq;
+          const float kyl = (float)(ky0 + 2 * q + h) * kys;
+          Fx[u] += (double)(kxl * (rca[u] * (rcb * G2 + rsb * G3) - rsa[u] * (rcb * G0 + rsb * G1)));
+          Fy[u] += (double)(kyl * (rca[u] * (rcb * G1 - rsb * G0) + rsa[u] * (rcb * G3 - rsb * G2)));
+          Fz[u] += (double)(rca[u] * (rcb * H0 + rsb * H1) + rsa[u] * (rcb * H2 + rsb * H3));
+        }
+        const double nc = cb[u] * cb4[u] - sb[u] * sb4[u];
+        sb[u] = cb[u] * sb4[u] + sb[u] * cb4[u];
+        cb[u] = nc;
       }
-      const double nc = cb * cb4 - sb * sb4;
-      sb = cb * sb4 + sb * cb4;
-      cb = nc;
     }
   }
-  // the two halves hold the even and odd rows of the same charges
-  Fx += __shfl_xor(Fx, 32, 64);
-  Fy += __shfl_xor(Fy, 32, 64);
-  Fz += __shfl_xor(Fz, 32, 64);
-  if (h || j >= nq) return;
-  double *o = kpf + ((size_t)blockIdx.y * nq + j) * 3;
-  o[0] = Fx;
-  o[1] = Fy;
-  o[2] = Fz;
+#pragma unroll
+  for (int u = 0; u < NTC; u++) {
+    // the two halves hold the even and odd rows of the same charges
+    const double fx = Fx[u] + __shfl_xor(Fx[u], 32, 64);
+    const double fy = Fy[u] + __shfl_xor(Fy[u], 32, 64);
+    const double fz = Fz[u] + __shfl_xor(Fz[u], 32, 64);
+    const int j = jw + 32 * u;
+    if (h || j >= nq) continue;
+    double *o = kpf + ((size_t)blockIdx.y * nq + j) * 3;
+    o[0] = fx;
+    o[1] = fy;
+    o[2] = fz;
+  }
 }
 
 // the cell-dipole field on a charge at r: ek_d = -cm_d M_d + cb_d r_d
@@ -949,17 +976,31 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   if constexpr (sizeof(R) == 4) {
     if (c.kmfma) {
 #define MDX_KM(KZV)                                                                         \
-  k_kforce_mfma<KZV><<<dim3(nblk(c.nq, 128), c.nkpart), kBlock, 0, st>>>(                   \
+  k_kforce_mfma<KZV, KNTC><<<dim3(nblk(c.nq, 128 * KNTC), c.nkpart), kBlock, 0, st>>>(      \
       c.nq, qpos, 1 / c.L[0], 1 / c.L[1], 1 / c.L[2], K.kxmax, (const int4 *)c.d_kparts.p,  \
       c.d_chunk_kymax.p, c.d_grp_start.p, c.d_grp_base.p, c.d_grp_nb.p, c.coefg.p, c.kpf.p)
-      switch (c.kzc) {
-        case 8: MDX_KM(8); break;
-        case 16: MDX_KM(16); break;
-        case 24: MDX_KM(24); break;
-        case 32: MDX_KM(32); break;
-        case 48: MDX_KM(48); break;
-        case 64: MDX_KM(64); break;
-        default: return fail(MDX_E_ARG, "k-space: kz chunk must be 8, 16, 24, 32, 48 or 64");
+      if (c.kmfma == 2) {
+        constexpr int KNTC = 2;
+        switch (c.kzc) {
+          case 8: MDX_KM(8); break;
+          case 16: MDX_KM(16); break;
+          case 24: MDX_KM(24); break;
+          case 32: MDX_KM(32); break;
+          case 48: MDX_KM(48); break;
+          case 64: MDX_KM(64); break;
+          default: return fail(MDX_E_ARG, "k-space: kz chunk must be 8, 16, 24, 32, 48 or 64");
+        }
+      } else {
+        constexpr int KNTC = 1;
+        switch (c.kzc) {
+          case 8: MDX_KM(8); break;
+          case 16: MDX_KM(16); break;
+          case 24: MDX_KM(24); break;
+          case 32: MDX_KM(32); break;
+          case 48: MDX_KM(48); break;
+          case 64: MDX_KM(64); break;
+          default: return fail(MDX_E_ARG, "k-space: kz chunk must be 8, 16, 24, 32, 48 or 64");
+        }
       }
 #undef MDX_KM
       MDX_HIP(hipGetLastError());
