@@ -23,9 +23,14 @@ Gradient of the loss L = sum_frames sum_i |F_i(theta) - F_i^target|^2:
     central difference of the forward call, [F(q + h) - F(q - h)] / 2h,
     is the exact derivative up to rounding, for any step h.
 
-`torch_loss(fm)` wraps this as a torch.autograd.Function on CPU tensors
-(torch is imported there only: its bundled HIP runtime must not share a
-process that computes on the GPU through torch).
+The fit (fit_lbfgs) runs L-BFGS in the variables FitVars: C12 and C6 of
+each Lennard-Jones pair, on which the forces depend linearly, and the
+charge parameters, preconditioned by the Cholesky factor of the
+Gauss-Newton matrix J^T J (ForceMatch.preconditioner).
+
+`torch_loss(fm)` wraps the loss as a torch.autograd.Function on CPU
+tensors (torch is imported there only: its bundled HIP runtime must not
+share a process that computes on the GPU through torch).
 """
 import ctypes as C
 
@@ -242,8 +247,197 @@ class ForceMatch:
         self.apply(theta)
         return E, g
 
+    def jtj(self, theta, frames=None, hrel=1e-4):
+        """J^T J of the force residuals at theta over the given frame
+        indices (default all), J by central differences of the forward
+        forces: h = hrel |theta_k| (exact for the charges and epsilon, on
+        which the forces depend quadratically resp. linearly)"""
+        theta = np.asarray(theta, dtype=np.float64)
+        idx = range(len(self.frames)) if frames is None else frames
+        J = np.zeros((len(idx), self.ns * 3, self.npar))
+        for k in range(self.npar):
+            h = hrel * max(abs(theta[k]), 1e-3)
+            e = np.zeros(self.npar)
+            e[k] = h
+            self.apply(theta + e)
+            fp = [self.eng.forces(self.frames[i])[0][:self.ns].ravel() for i in idx]
+            self.apply(theta - e)
+            fm = [self.eng.forces(self.frames[i])[0][:self.ns].ravel() for i in idx]
+            for n in range(len(idx)):
+                J[n, :, k] = (fp[n] - fm[n]) / (2 * h)
+        self.apply(theta)
+        return np.einsum("nik,nil->kl", J, J)
+
+    def preconditioner(self, theta, fv, frames=None):
+        """R with J_u^T J_u = R^T R (Cholesky), J_u the Jacobian of the force
+        residuals over the given frames (default all, scaled to all) with
+        respect to the fit variables u of fv (FitVars): in x = R (u - u0)
+        the Gauss-Newton Hessian of the loss is 2 I, so every parameter is
+        equally well conditioned for a quasi-Newton fit"""
+        A = self.jtj(theta, frames)
+        A *= len(self.frames) / (len(self.frames) if frames is None else len(frames))
+        D = fv.jac(np.asarray(theta, dtype=np.float64))
+        return np.linalg.cholesky(D.T @ A @ D).T
+
     def close(self):
         self.eng.close()
+
+
+class FitVars:
+    """Fit variables u of the parameters theta of a ForceMatch, relative to
+    theta0.  A Lennard-Jones pair has u = (C12 / C12_0, C6 / C6_0), C12 =
+    4 eps sigma^12, C6 = 4 eps sigma^6: its forces are linear in u (up to
+    the smooth-cutoff constants), where in (sigma, eps) the repulsive wall
+    goes as sigma^12 and its sigma and eps columns are nearly parallel.  An
+    exp-6 pair has u = log sigma, log eps, log a; a charge parameter
+    u = theta.  max_step(u, du): the largest t for a trial step u + t du
+    that lowers no C12 or C6 below half its value (keeping them positive),
+    changes no exp-6 parameter by more than a factor e^0.5 and no charge
+    parameter by more than max_q e."""
+
+    def __init__(self, fm, theta0, max_q=0.2):
+        self.npl, self.npar, self.exp6 = fm.npl, fm.npar, fm.exp6
+        th = np.asarray(theta0, dtype=np.float64)
+        s, e = th[0:self.npl:2], th[1:self.npl:2]
+        self.c0 = None if self.exp6 else np.stack([4 * e * s ** 12, 4 * e * s ** 6], 1).ravel()
+        self.max_q = max_q
+
+    def u(self, theta):
+        u = np.array(theta, dtype=np.float64)
+        if self.exp6:
+            u[:self.npl] = np.log(u[:self.npl])
+            return u
+        s, e = u[0:self.npl:2].copy(), u[1:self.npl:2].copy()
+        u[0:self.npl:2] = 4 * e * s ** 12 / self.c0[0::2]
+        u[1:self.npl:2] = 4 * e * s ** 6 / self.c0[1::2]
+        return u
+
+    def theta(self, u):
+        th = np.array(u, dtype=np.float64)
+        if self.exp6:
+            th[:self.npl] = np.exp(th[:self.npl])
+            return th
+        c12, c6 = th[0:self.npl:2] * self.c0[0::2], th[1:self.npl:2] * self.c0[1::2]
+        th[0:self.npl:2] = (c12 / c6) ** (1 / 6)
+        th[1:self.npl:2] = c6 * c6 / (4 * c12)
+        return th
+
+    def jac(self, theta):
+        """d theta / d u at theta"""
+        D = np.eye(self.npar)
+        th = np.asarray(theta, dtype=np.float64)
+        if self.exp6:
+            D[np.arange(self.npl), np.arange(self.npl)] = th[:self.npl]
+            return D
+        for k in range(0, self.npl, 2):
+            s, e = th[k], th[k + 1]
+            c12, c6 = 4 * e * s ** 12, 4 * e * s ** 6
+            a12, a6 = self.c0[k], self.c0[k + 1]
+            D[k, k], D[k, k + 1] = s / (6 * c12) * a12, -s / (6 * c6) * a6
+            D[k + 1, k], D[k + 1, k + 1] = -e / c12 * a12, 2 * e / c6 * a6
+        return D
+
+    def max_step(self, u, du):
+        t = self.max_q / np.maximum(np.abs(du[self.npl:]), 1e-300)
+        if self.exp6:
+            t = np.append(t, 0.5 / np.maximum(np.abs(du[:self.npl]), 1e-300))
+        else:
+            down = du[:self.npl] < 0
+            t = np.append(t, 0.5 * u[:self.npl][down] / -du[:self.npl][down])
+        return float(t.min()) if len(t) else np.inf
+
+
+def lbfgs(fun, x0, iters=40, memory=20, maxstep=None, ftol=None, rtol=1e-8, stall=2, log=None):
+    """Minimize fun(x) -> (f, g) by L-BFGS: the two-loop recursion for the
+    direction (Nocedal & Wright, Numerical Optimization, alg. 7.4, with
+    H0 = (s.y / y.y) I) and a backtracking line search on the Armijo
+    condition f(x + a d) <= f + 1e-4 a g.d, the first trial a = 1 cut to
+    maxstep(x, d) and a point where fun raises RuntimeError (parameters
+    the engine refuses) treated as f = inf.  Pairs with s.y <= 1e-12 |s||y|
+    are not stored.  Stops after iters iterations, at f <= ftol (if given),
+    when no step along d lowers f, or after `stall` iterations in a row
+    that lower f by no more than rtol |f| (the rounding floor of f).  Returns (x, [f of every accepted point])."""
+    x = np.array(x0, dtype=np.float64)
+    f, g = fun(x)
+    hist = [f]
+    S, Y = [], []
+    slow = 0
+    for it in range(iters):
+        if ftol is not None and f <= ftol:
+            break
+        q = g.copy()
+        al = []
+        for s, y in zip(reversed(S), reversed(Y)):
+            a = (s @ q) / (y @ s)
+            al.append(a)
+            q -= a * y
+        if S:
+            q *= (S[-1] @ Y[-1]) / (Y[-1] @ Y[-1])
+        for (s, y), a in zip(zip(S, Y), reversed(al)):
+            q += s * (a - (y @ q) / (y @ s))
+        d = -q
+        gd = g @ d
+        if gd >= 0:                       # not a descent direction: restart
+            S, Y = [], []
+            d, gd = -g, -(g @ g)
+        t = 1.0 if maxstep is None else min(1.0, maxstep(x, d))
+        for _ in range(30):
+            try:
+                fn, gn = fun(x + t * d)
+            except RuntimeError:
+                fn, gn = np.inf, None
+            if np.isfinite(fn) and fn <= f + 1e-4 * t * gd:
+                break
+            # minimum of the quadratic through f, g.d and fn, kept in [0.1 t, 0.5 t]
+            tq = -gd * t * t / (2 * (fn - f - gd * t)) if np.isfinite(fn) else 0.1 * t
+            t = min(max(tq, 0.1 * t), 0.5 * t)
+        else:
+            if log:
+                log("fit: no descent along the search direction at iteration %d" % it)
+            break
+        s, y = t * d, gn - g
+        if s @ y > 1e-12 * np.linalg.norm(s) * np.linalg.norm(y):
+            S.append(s)
+            Y.append(y)
+            if len(S) > memory:
+                S.pop(0)
+                Y.pop(0)
+        slow = slow + 1 if f - fn <= rtol * abs(f) else 0
+        x, f, g = x + s, fn, gn
+        hist.append(f)
+        if log:
+            log("fit: iteration %d, f %.6e, step %.3g, |g| %.3e" % (it + 1, f, t, np.linalg.norm(g)))
+        if slow >= stall:
+            break
+    return x, hist
+
+
+def fit_lbfgs(fm, theta0, R=None, iters=40, log=None, fv=None, ftol=1e-16):
+    """L-BFGS of the force-matching loss of fm in the variables
+    x = R (u - u0) / sqrt(L0/2), u the FitVars of theta (default FitVars(fm,
+    theta0)), L0 the loss at theta0 and R from fm.preconditioner (computed
+    here over all frames if not given): the Gauss-Newton model of L/L0 in
+    x has the identity as Hessian, so the first step has the right length
+    and every parameter is equally well conditioned.  Trial steps are
+    limited by fv.max_step; the fit stops at L <= ftol L0.  Returns (theta, loss history of the
+    accepted points)."""
+    theta0 = np.asarray(theta0, dtype=np.float64)
+    fv = FitVars(fm, theta0) if fv is None else fv
+    if R is None:
+        R = fm.preconditioner(theta0, fv)
+    u0 = fv.u(theta0)
+    L0, _ = fm.loss_and_grad(theta0)
+    Rinv = np.linalg.inv(R) * np.sqrt(L0 / 2)
+
+    def fun(x):
+        th = fv.theta(u0 + Rinv @ x)
+        L, g = fm.loss_and_grad(th)
+        return L / L0, Rinv.T @ (fv.jac(th).T @ g) / L0
+
+    def maxstep(x, d):
+        return fv.max_step(u0 + Rinv @ x, Rinv @ d)
+    x, hist = lbfgs(fun, np.zeros(fm.npar), iters=iters, maxstep=maxstep, ftol=ftol, log=log)
+    return fv.theta(u0 + Rinv @ x), [h * L0 for h in hist]
 
 
 def torch_loss(loss_and_grad):
@@ -301,11 +495,10 @@ def main():
     import json
     import sys
     import time
-    import torch   # before any GPU work in this process (CPU tensors only)
 
     ap = argparse.ArgumentParser(description="force matching of LJ sigma/epsilon and charges")
     ap.add_argument("--frames", type=int, default=64)
-    ap.add_argument("--iters", type=int, default=8, help="L-BFGS iterations")
+    ap.add_argument("--iters", type=int, default=40, help="L-BFGS iterations")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check")
     a = ap.parse_args()
     t0 = time.perf_counter()
@@ -344,33 +537,19 @@ def main():
     print("gradients done: %s" % json.dumps({k: v for k, v in res.items() if k.startswith(("grad", "ms", "loss"))}),
           file=sys.stderr, flush=True)
 
-    # fit with torch L-BFGS on the mixed engine, parameters relative to
-    # theta0; parameters for which setss has no smooth cutoff return a large
-    # loss (the line search then backtracks)
-    def scaled(x):
-        try:
-            L, g = fm32.loss_and_grad(x * th0)
-        except RuntimeError:
-            return 1e6, np.zeros_like(x)
-        return L / L64, g * th0 / L64
-    f = torch_loss(scaled)
-    x = torch.ones(len(th0), dtype=torch.float64, requires_grad=True)
-    opt = torch.optim.LBFGS([x], lr=0.1, max_iter=a.iters, line_search_fn="strong_wolfe")
-    hist = []
-
-    def closure():
-        opt.zero_grad()
-        L = f(x)
-        L.backward()
-        hist.append(float(L))
-        print("fit: evaluation %d, loss/loss0 %.6e" % (len(hist), float(L)), file=sys.stderr, flush=True)
-        return L
+    # L-BFGS on the mixed engine in preconditioned C12/C6 and charge
+    # variables (Gauss-Newton J^T J of 4 frames at theta0)
     t0 = time.perf_counter()
-    opt.step(closure)
+    fv = FitVars(fm32, th0)
+    R = fm32.preconditioner(th0, fv, frames=list(range(min(4, a.frames))))
+    res["precond_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    th, hist = fit_lbfgs(fm32, th0, R, iters=a.iters, fv=fv,
+                         log=lambda m: print(m, file=sys.stderr, flush=True))
     res["fit_s"] = time.perf_counter() - t0
-    th = x.detach().numpy() * th0
-    res["loss_history_rel"] = hist
+    res["loss_history_rel"] = [h / L64 for h in hist]
     res["theta_fit"] = th.tolist()
+    res["fit_param_rel_err"] = (np.abs(th - tht) / np.abs(tht)).tolist()
     res["fit_param_rel_err_max"] = float(np.abs(th - tht).max() / np.abs(tht).max())
     for fm in fms.values():
         fm.close()

@@ -1,0 +1,94 @@
+"""The force-matching workload at its size: 64 frames of the 23,712-site
+SPC/E + 0.5 mol/kg NaCl box made by the engine's own FP64 MD, targets from
++5 % sigma/epsilon of 4 type pairs and +-0.02 e charges.  The gradient of
+the mixed engine against the FP64 one over all 11 parameters, the FP64
+gradient against central differences of the loss (h = 1e-4 relative) and
+the energy gradient against differences of the energy on 4 frames, and a
+preconditioned L-BFGS fit in C12/C6 and charge variables that recovers every target parameter to 1 %."""
+import numpy as np
+import pytest
+
+from macsimus_amd import abi
+from macsimus_amd.fit import FIT_CHARGES, FIT_PAIRS, FitVars, ForceMatch, fit_lbfgs, make_frames
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c5():
+    wl, frames, info = make_frames(64)
+    kw = dict(alpha=info["alpha"], kappa=info["kappa"])
+    ref = ForceMatch(wl["ff"], wl["counts"], wl["L"], wl["cutoff"], frames,
+                     [np.zeros_like(f) for f in frames], FIT_PAIRS, FIT_CHARGES, **kw)
+    th0 = ref.theta0()
+    tht = th0 * np.array([1.05] * 8 + [1, 1, 1]) + np.array([0] * 8 + [-0.02, 0.02, -0.02])
+    targets = [f for f, _ in ref.forces(tht)]
+    ref.close()
+    print("C5: %d frames of %d sites" % (len(frames), frames[0].shape[0]), flush=True)
+    fms = {m: ForceMatch(wl["ff"], wl["counts"], wl["L"], wl["cutoff"], frames, targets,
+                         FIT_PAIRS, FIT_CHARGES, mode=m, **kw) for m in (abi.MDX_FP64, abi.MDX_MIXED)}
+    yield fms, th0, tht
+    for fm in fms.values():
+        fm.close()
+
+
+def test_c5_mixed_gradient_vs_fp64(c5):
+    fms, th0, _ = c5
+    L64, g64 = fms[abi.MDX_FP64].loss_and_grad(th0)
+    L32, g32 = fms[abi.MDX_MIXED].loss_and_grad(th0)
+    e = np.abs(g32 - g64) / np.abs(g64)
+    print("loss %.8e / %.8e, per-parameter gradient rel diff max %.2e" % (L64, L32, e.max()), flush=True)
+    assert abs(L32 - L64) < 1e-5 * L64
+    assert e.max() < 1e-4
+
+
+def sub(fm, idx):
+    """the same force matching on a subset of the frames"""
+    fm.frames, fm.targets = [fm.frames[i] for i in idx], [fm.targets[i] for i in idx]
+    return fm
+
+
+def test_c5_gradient_vs_central_differences(c5):
+    fms, th0, _ = c5
+    fm = fms[abi.MDX_FP64]
+    frames, targets = fm.frames, fm.targets
+    sub(fm, [0, 21, 42, 63])
+    try:
+        L, g = fm.loss_and_grad(th0)
+        gfd = np.zeros_like(g)
+        for k in range(len(th0)):
+            h = 1e-4 * abs(th0[k])
+            e = np.zeros_like(th0)
+            e[k] = h
+            gfd[k] = (fm.loss(th0 + e) - fm.loss(th0 - e)) / (2 * h)
+        err = np.abs(g - gfd) / np.abs(gfd)
+        print("grad vs central differences: per-parameter rel err max %.2e" % err.max(), flush=True)
+        assert err.max() < 1e-5
+        E, gE = fm.energy_and_grad(th0)
+        gEfd = np.zeros_like(gE)
+        for k in range(len(th0)):
+            h = 1e-4 * abs(th0[k])
+            e = np.zeros_like(th0)
+            e[k] = h
+            Ep = sum(o.En_pot + o.En_el for _, o in fm.forces(th0 + e, measure=1))
+            Em = sum(o.En_pot + o.En_el for _, o in fm.forces(th0 - e, measure=1))
+            gEfd[k] = (Ep - Em) / (2 * h)
+        eE = np.abs(gE - gEfd) / np.abs(gEfd).max()
+        print("energy grad vs differences: rel err max %.2e" % eE.max(), flush=True)
+        assert eE.max() < 1e-6
+    finally:
+        fm.frames, fm.targets = frames, targets
+
+
+def test_c5_fit_recovers_targets(c5):
+    fms, th0, tht = c5
+    fm = fms[abi.MDX_MIXED]
+    fv = FitVars(fm, th0)
+    R = fm.preconditioner(th0, fv, frames=[0, 21, 42, 63])
+    th, hist = fit_lbfgs(fm, th0, R, iters=40, fv=fv,
+                         log=lambda m: print(m, flush=True) if len(m) else None)
+    err = np.abs(th - tht) / np.abs(tht)
+    print("fit: %d evaluations, loss %.3e -> %.3e, parameter rel err max %.2e" % (
+        len(hist), hist[0], hist[-1], err.max()), flush=True)
+    assert err.max() < 1e-2
+    assert hist[-1] < 1e-6 * hist[0]
