@@ -358,7 +358,9 @@ int mdx_info(mdx_ctx_t *ctx, int *iv, int n);
    mdx_set_params: new site-site table sstab [ntypes^2] (mdx_setss) and/or
    charges (centre charges [ns], Drude charges [ns]) for the next calls;
    NULL keeps the old ones.  alpha, kappa, the k-vectors and the set of
-   charged sites stay.
+   charged sites stay; the net charge, its background energy and Ediag
+   follow the new charges (cook's refusal of a charged system without
+   el_bg is made by mdx_create only).
    mdx_adjoint_lj: adjoint of the LJ terms at positions r (as mdx_forces)
    for the weights w [ns*3] = dL/dF of a loss of the site forces; for each
    type pair (a,b) (a = central site) gF[(a*nt+b)*4 + k] = sum_i w_i .
@@ -388,6 +390,18 @@ int mdx_adjoint_lj_noalloc(mdx_ctx_t *ctx, const double *r, const double *w, dou
                            double *gE);
 int mdx_adjoint_exp6(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
                      double *gE);
+/* mdx_adjoint_charges: adjoint of the Ewald electrostatics (r-space with
+   cook's erfc spline, EXCL corrections, k-space on the force call's
+   k-vector set, self, background, Ediag and cell-dipole terms) with respect
+   to the site charges, at positions r (as mdx_forces) for the weights w
+   [ns*3] = dL/dF of a loss of the site forces: gF[i] = sum_k w_k . dF_k/dq_i
+   and gE[i] = dEn_el/dq_i (program units; 0 for uncharged sites).  Every
+   force term is bilinear in the charges, so gF and gE equal central
+   differences of mdx_forces in q_i up to rounding.  FP64 in either mode;
+   systems with Drude charges get MDX_E_ARG.  No counterpart in cook: it
+   replaces differences of the force call in charge fitting. */
+int mdx_adjoint_charges(mdx_ctx_t *ctx, const double *r, const double *w, double *gF,
+                        double *gE);
 
 /* ---- batched replicas ---------------------------------------------------
    Many independent atomic systems (one site per molecule: LJ with cook's

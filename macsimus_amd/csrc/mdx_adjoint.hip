@@ -66,6 +66,7 @@ int adjoint_scratch_setup(int dev, double **ptr) {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr double kPiA = 3.14159265358979323846;
 
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -208,8 +209,18 @@ extern "C" int mdx_set_params(mdx_ctx_t *h, const mdx_lj_t *sstab, const double 
     }
     if (charge) std::copy(charge, charge + c.ns, q.begin());
     if (chargepol) std::copy(chargepol, chargepol + c.ns, qp.begin());
+    // the charge sums and what depends on them: the background energy
+    // of a net charge and Ediag (cook's refusal of a charged system
+    // without el.bg is made at mdx_create only, so that a fit may pass
+    // through charged states)
     c.sumq2 = 0;
-    for (int i = 0; i < c.ns; i++) c.sumq2 += (q[i] + qp[i]) * (q[i] + qp[i]);
+    c.qnet = 0;
+    for (int i = 0; i < c.ns; i++) {
+      c.sumq2 += (q[i] + qp[i]) * (q[i] + qp[i]);
+      c.qnet += q[i] + qp[i];
+    }
+    c.bgE = -kPiA / 2 * (c.qnet / c.alpha) * (c.qnet / c.alpha);
+    c.Ediag = c.nq ? ewald_ediag(c.L, c.kappa, c.alpha, c.sumq2) : 0;
     MDX_HIP(hipMemcpy(c.charge.p, q.data(), sizeof(double) * c.ns, hipMemcpyHostToDevice));
     if (c.polar)
       MDX_HIP(hipMemcpy(c.chargepol.p, qp.data(), sizeof(double) * c.ns, hipMemcpyHostToDevice));

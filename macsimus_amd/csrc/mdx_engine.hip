@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "mdx_devfn.h"
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -993,29 +994,9 @@ __global__ void k_pair_sum(long n3, int nparts, const double *__restrict__ pf,
 }
 
 // --------------------------------------------------- exclusion corrections
-// exacterud_sqrt(): -erf(alpha r)/r and its force/r (sim/erexact.c:25-53),
-// added for same-molecule EXCL pairs inside the cutoff, FP64, one thread per
+// exact_erud (mdx_devfn.h): -erf(alpha r)/r and its force/r, added for
+// same-molecule EXCL pairs inside the cutoff, FP64, one thread per
 // molecule (no write conflicts: the thread owns the molecule's sites).
-__device__ inline double exact_erud(double rr, double alpha, double &erd) {
-  const double alphaq = alpha * alpha, A = alpha * (-2 / 1.772453850905516), B = A * alphaq * 2;
-  const double err = alphaq * rr;
-  if (err < 0.01) {
-    double su = 0, sd = 0, q = 1;
-    for (int n = 1;; ) {
-      const double t = q / (2 * n + 1);
-      sd += t * n;
-      su += t;
-      if (fabs(q) < 1e-16) break;
-      q *= -err / ++n;
-    }
-    erd = B * sd;
-    return A * (1 - su * err);
-  }
-  const double r = sqrt(rr), y = -erf(alpha * r) / r;
-  erd = (y - A * exp(-err)) / rr;
-  return y;
-}
-
 // adds the exact correction of one charge pair (separation d, FP64)
 template <bool MEASURE>
 __device__ inline void excl_term(double qq, const double (&d)[3], double alpha, double *fa,
@@ -1638,6 +1619,10 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.d_grp_nb.release();
   c.rdf_hist.release();
   c.qanc.release();
+  c.qadj_tab.release();
+  c.qadj_part.release();
+  c.qadj_buf.release();
+  c.qadj_rows.release();
   c.tm.release();
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);

@@ -185,6 +185,11 @@ struct Ctx {
   // matrix-core pass 1: FP32 anchors e^{2 pi i 8m s} per block of 8 [task][nq]
   // (q folded in along x), then the bases e^{2 pi i s} [3][nq]
   DBuf<float2> qanc;
+  // charge adjoint: triple records of the k-set with their S/U sums, the
+  // per-split partials of those sums, the per-call buffers
+  DBuf<double> qadj_tab, qadj_part, qadj_buf;
+  DBuf<int> qadj_rows;
+  int qadj_ntrip = 0, qadj_nrows = 0, qadj_nsplit = 1, qadj_ksplit = 1;
   DBuf<unsigned> excl;        // [ns]
   bool exp6 = false;           // site-site form: exp-6 table instead of LJ
   DBuf<double> lj64;          // [ntypes^2*8] LJ or exp-6 constants, C1q C2q A A4

@@ -220,6 +220,19 @@ class Engine:
         abi.check(self.lib, rc, "mdx_adjoint_lj_noalloc")
         return gF, gE
 
+    def adjoint_charges(self, r, w):
+        """(gF, gE) [ns]: sum_k w_k . dF_k/dq_i and dEn_el/dq_i for every
+        site charge q_i (program units; see include/mdx.h)"""
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.nrow, 3)
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.ns, 3)
+        gF = np.zeros(self.ns)
+        gE = np.zeros(self.ns)
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mdx_adjoint_charges(self.ctx, r.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                          gF.ctypes.data_as(dp), gE.ctypes.data_as(dp))
+        abi.check(self.lib, rc, "mdx_adjoint_charges")
+        return gF, gE
+
     def set_graph(self, on=True):
         rc = self.lib.mdx_set_graph(self.ctx, int(on))
         abi.check(self.lib, rc, "mdx_set_graph")
@@ -436,6 +449,7 @@ def declare_extras(lib):
     lib.mdx_adjoint_lj_noalloc.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
     lib.mdx_set_params_exp6.argtypes = [C.c_void_p, C.POINTER(abi.Exp6)]
     lib.mdx_adjoint_exp6.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
+    lib.mdx_adjoint_charges.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
     lib.mdx_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mdx_set_aspc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]

@@ -18,10 +18,14 @@ Gradient of the loss L = sum_frames sum_i |F_i(theta) - F_i^target|^2:
     chain to the parameters goes through setss() (mdx_setss,
     mdx_setss_exp6, which include the smooth-cutoff fixed point) by
     central differences of the constants on the host.
-  * charges: the site forces and the energy are quadratic forms in the
-    charges (r-space, exclusion, k-space and dipole terms alike), so the
-    central difference of the forward call, [F(q + h) - F(q - h)] / 2h,
-    is the exact derivative up to rounding, for any step h.
+  * charges: the engine's charge adjoint (mdx_adjoint_charges) gives
+    sum_i w_i . dF_i/dq_s and dE/dq_s for every site s; the chain to the
+    charge parameters is the affine charge map.  The site forces and the
+    energy are quadratic forms in the charges (r-space, exclusion, k-space
+    and dipole terms alike), so the central difference of the forward
+    call, [F(q + h) - F(q - h)] / 2h, is the exact derivative up to
+    rounding for any step h: it checks the adjoint, and it is the method
+    for systems with Drudes, which the charge adjoint does not take.
 
 The fit (fit_lbfgs) runs L-BFGS in the variables FitVars: C12 and C6 of
 each Lennard-Jones pair, on which the forces depend linearly, and the
@@ -190,13 +194,29 @@ class ForceMatch:
     def _adjoint(self, r, w):
         return self.eng.adjoint_exp6(r, w) if self.exp6 else self.eng.adjoint_lj(r, w)
 
-    def loss_and_grad(self, theta, hq=1e-3):
-        """L(theta) and dL/dtheta (adjoint for LJ, exact quadratic-form
-        differences for the charges)"""
+    def _dq_dtheta(self):
+        """d(site charge, program units)/d(charge parameter) [ns][nq]"""
+        D = np.zeros((self.ns, self.q_M.shape[1]))
+        st = np.asarray(self.site_type)[:self.ns]
+        for t, ty in enumerate(self.q_types):
+            D[st == ty] = self.q_M[t] * ELECTRON
+        return D
+
+    def charge_method(self):
+        """"adjoint" (mdx_adjoint_charges) unless the system has Drudes,
+        where the charge gradient is taken by exact differences"""
+        return "differences" if self.polar else "adjoint"
+
+    def loss_and_grad(self, theta, hq=1e-3, charges=None):
+        """L(theta) and dL/dtheta (adjoint for LJ; for the charges the
+        charge adjoint, or with charges="differences" the central
+        differences of the forward forces, exact for these quadratic forms)"""
         theta = np.asarray(theta, dtype=np.float64)
+        charges = charges or self.charge_method()
         fr = self.forces(theta)
         L = 0.0
         gF = 0.0
+        gq = 0.0
         ws = []
         for (f, _), t, r in zip(fr, self.targets, self.frames):
             d = f[:self.ns] - t
@@ -204,46 +224,59 @@ class ForceMatch:
             w = 2 * d
             ws.append(w)
             gF = gF + self._adjoint(r, w)[0]
+            if charges == "adjoint":
+                gq = gq + self.eng.adjoint_charges(r, w)[0]
         g = np.zeros(self.npar)
         pp = self.pp
         for k, (a, b) in enumerate(self.pairs):
             gc = gF[a, b] + (gF[b, a] if a != b else 0)          # d L / d(force constants)
             J = self._dconst(theta[pp * k:pp * (k + 1)])[self.FROWS[self.exp6]]
             g[pp * k:pp * (k + 1)] = gc @ J
-        for m in range(self.q_M.shape[1]):
-            e = np.zeros(self.npar)
-            e[self.npl + m] = hq
-            fp = self.forces(theta + e)
-            fm = self.forces(theta - e)
-            s = 0.0
-            for w, (a_, _), (b_, _) in zip(ws, fp, fm):
-                s += float((w * (a_[:self.ns] - b_[:self.ns])).sum()) / (2 * hq)
-            g[self.npl + m] = s
+        if charges == "adjoint":
+            g[self.npl:] = gq @ self._dq_dtheta()
+        else:
+            for m in range(self.q_M.shape[1]):
+                e = np.zeros(self.npar)
+                e[self.npl + m] = hq
+                fp = self.forces(theta + e)
+                fm = self.forces(theta - e)
+                s = 0.0
+                for w, (a_, _), (b_, _) in zip(ws, fp, fm):
+                    s += float((w * (a_[:self.ns] - b_[:self.ns])).sum()) / (2 * hq)
+                g[self.npl + m] = s
         self.apply(theta)
         return L, g
 
-    def energy_and_grad(self, theta, hq=1e-3):
+    def energy_and_grad(self, theta, hq=1e-3, charges=None):
         """sum over frames of En.pot + En.el and its gradient (adjoint gE for
-        LJ, exact quadratic-form differences for the charges)"""
+        LJ; for the charges the charge adjoint or exact differences, as
+        loss_and_grad)"""
         theta = np.asarray(theta, dtype=np.float64)
+        charges = charges or self.charge_method()
         outs = self.forces(theta, measure=1)
         E = sum(o.En_pot + o.En_el for _, o in outs)
         gE = 0.0
+        gq = 0.0
         zero = np.zeros((self.ns, 3))
         for r in self.frames:
             gE = gE + self._adjoint(r, zero)[1]
+            if charges == "adjoint":
+                gq = gq + self.eng.adjoint_charges(r, zero)[1]
         g = np.zeros(self.npar)
         pp = self.pp
         for k, (a, b) in enumerate(self.pairs):
             gc = gE[a, b] + (gE[b, a] if a != b else 0)          # d E / d(energy constants)
             J = self._dconst(theta[pp * k:pp * (k + 1)])
             g[pp * k:pp * (k + 1)] = gc @ J[self.EROWS[self.exp6]]
-        for m in range(self.q_M.shape[1]):
-            e = np.zeros(self.npar)
-            e[self.npl + m] = hq
-            Ep = sum(o.En_pot + o.En_el for _, o in self.forces(theta + e, measure=1))
-            Em = sum(o.En_pot + o.En_el for _, o in self.forces(theta - e, measure=1))
-            g[self.npl + m] = (Ep - Em) / (2 * hq)
+        if charges == "adjoint":
+            g[self.npl:] = gq @ self._dq_dtheta()
+        else:
+            for m in range(self.q_M.shape[1]):
+                e = np.zeros(self.npar)
+                e[self.npl + m] = hq
+                Ep = sum(o.En_pot + o.En_el for _, o in self.forces(theta + e, measure=1))
+                Em = sum(o.En_pot + o.En_el for _, o in self.forces(theta - e, measure=1))
+                g[self.npl + m] = (Ep - Em) / (2 * hq)
         self.apply(theta)
         return E, g
 

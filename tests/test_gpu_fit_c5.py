@@ -3,7 +3,8 @@ SPC/E + 0.5 mol/kg NaCl box made by the engine's own FP64 MD, targets from
 +5 % sigma/epsilon of 4 type pairs and +-0.02 e charges.  The gradient of
 the mixed engine against the FP64 one over all 11 parameters, the FP64
 gradient against central differences of the loss (h = 1e-4 relative) and
-the energy gradient against differences of the energy on 4 frames, and a
+the energy gradient against differences of the energy on 4 frames, the
+charge adjoint against exact differences of the force call, and a
 preconditioned L-BFGS fit in C12/C6 and charge variables that recovers every target parameter to 1 %."""
 import numpy as np
 import pytest
@@ -78,6 +79,33 @@ def test_c5_gradient_vs_central_differences(c5):
         assert eE.max() < 1e-6
     finally:
         fm.frames, fm.targets = frames, targets
+
+
+def test_c5_charge_adjoint_vs_exact_differences(c5):
+    import time
+    fms, th0, _ = c5
+    fm = fms[abi.MDX_FP64]
+    t0 = time.perf_counter()
+    La, ga = fm.loss_and_grad(th0)
+    ta = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Ld, gd = fm.loss_and_grad(th0, hq=0.05, charges="differences")
+    td = time.perf_counter() - t0
+    n = fm.npl
+    err = np.abs(ga[n:] - gd[n:]) / np.abs(gd[n:])
+    t0 = time.perf_counter()
+    fms[abi.MDX_MIXED].loss_and_grad(th0)
+    tm = time.perf_counter() - t0
+    print("charge gradient, adjoint vs exact differences: rel err max %.2e; loss+gradient "
+          "FP64 %.0f ms (differences %.0f ms), mixed %.0f ms" % (err.max(), ta * 1e3, td * 1e3, tm * 1e3),
+          flush=True)
+    assert La == Ld
+    assert err.max() < 1e-10
+    Ea, gEa = fm.energy_and_grad(th0)
+    Ed, gEd = fm.energy_and_grad(th0, hq=0.05, charges="differences")
+    eE = np.abs(gEa[n:] - gEd[n:]) / np.abs(gEd[n:])
+    print("energy charge gradient: rel err max %.2e" % eE.max(), flush=True)
+    assert eE.max() < 1e-10
 
 
 def test_c5_fit_recovers_targets(c5):
