@@ -445,14 +445,16 @@ def lbfgs(fun, x0, iters=40, memory=20, maxstep=None, ftol=None, rtol=1e-8, stal
     return x, hist
 
 
-def fit_lbfgs(fm, theta0, R=None, iters=40, log=None, fv=None, ftol=1e-16):
+def fit_lbfgs(fm, theta0, R=None, iters=40, log=None, fv=None, ftol=1e-16, rtol=1e-4):
     """L-BFGS of the force-matching loss of fm in the variables
     x = R (u - u0) / sqrt(L0/2), u the FitVars of theta (default FitVars(fm,
     theta0)), L0 the loss at theta0 and R from fm.preconditioner (computed
     here over all frames if not given): the Gauss-Newton model of L/L0 in
     x has the identity as Hessian, so the first step has the right length
     and every parameter is equally well conditioned.  Trial steps are
-    limited by fv.max_step; the fit stops at L <= ftol L0.  Returns (theta, loss history of the
+    limited by fv.max_step; the fit stops at L <= ftol L0 or after two
+    iterations in a row that lower L by no more than rtol L (the
+    rounding floor of the engine's loss).  Returns (theta, loss history of the
     accepted points)."""
     theta0 = np.asarray(theta0, dtype=np.float64)
     fv = FitVars(fm, theta0) if fv is None else fv
@@ -469,7 +471,8 @@ def fit_lbfgs(fm, theta0, R=None, iters=40, log=None, fv=None, ftol=1e-16):
 
     def maxstep(x, d):
         return fv.max_step(u0 + Rinv @ x, Rinv @ d)
-    x, hist = lbfgs(fun, np.zeros(fm.npar), iters=iters, maxstep=maxstep, ftol=ftol, log=log)
+    x, hist = lbfgs(fun, np.zeros(fm.npar), iters=iters, maxstep=maxstep, ftol=ftol, rtol=rtol,
+                    log=log)
     return fv.theta(u0 + Rinv @ x), [h * L0 for h in hist]
 
 
@@ -583,7 +586,7 @@ def main():
     res["loss_history_rel"] = [h / L64 for h in hist]
     res["theta_fit"] = th.tolist()
     res["fit_param_rel_err"] = (np.abs(th - tht) / np.abs(tht)).tolist()
-    res["fit_param_rel_err_max"] = float(np.abs(th - tht).max() / np.abs(tht).max())
+    res["fit_param_rel_err_max"] = float((np.abs(th - tht) / np.abs(tht)).max())
     for fm in fms.values():
         fm.close()
     print(json.dumps(res), flush=True)
