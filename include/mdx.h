@@ -128,6 +128,23 @@ typedef struct mdx_system_s {
   int el_corr;
   int el_bg;
   int el_diag;
+  /* POLAR&1 shell-core repulsion (cook/ljqqpol.c:41-46, 525-535,
+     561-648; cook/simdef.c:646-694, 988-992), for systems with Drude
+     charges and an exp-6 table (cook builds it for the Buckingham and
+     Busing forms).  Per site type: shell_qkappa, cook's ATOM kappa divided
+     by the electron charge unit (si->qkappa; 0: none), and shell_rep
+     (cook's rep column; nonzero: the type repels shells).  For a pair
+     inside the cutoff whose central type a has qkappa != 0 and whose
+     partner type b has rep, with urep(r) = eu e^(tau r) the exponential
+     part of the (a,b) table and frep = -urep'(r)/r:
+       centre-centre (both centres charged): + qkappa_a urep(r_cc), force
+         + qkappa_a frep(r_cc) (r_cc < C2);
+       Drude a - centre b (both charged): - qkappa_a urep(r_dc), force
+         - qkappa_a frep(r_dc) on the Drude (r_dc < C2);
+     and the same with the roles of the two sites swapped.  NULL (both)
+     for none. */
+  const double *shell_qkappa;
+  const int *shell_rep;
 } mdx_system_t;
 
 /* what forces() leaves in the globals of sim/simglob.h:436-577 */

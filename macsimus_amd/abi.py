@@ -55,6 +55,7 @@ class System(C.Structure):
         ("cons_ij", _ip), ("cons_bondq", _dp), ("mol_omega", _dp),
         ("sstab_exp6", C.POINTER(Exp6)),
         ("el_corr", C.c_int), ("el_bg", C.c_int), ("el_diag", C.c_int),
+        ("shell_qkappa", _dp), ("shell_rep", _ip),
     ]
 
 

@@ -223,7 +223,7 @@ def read_ble(path, form=None):
     L.find("site-types" if polar < 0 else
            {"exp6": "exp-6", "buck": "Buckingham", "busing": "Busing"}.get(form,
                                                                            "Lennard-Jones"))
-    types, mass, evdw, rvdw, alphas, pol, parm = [], [], [], [], [], {}, []
+    types, mass, evdw, rvdw, alphas, pol, parm, shell = [], [], [], [], [], {}, [], {}
     for i in range(nsites):
         L.tok()                                   # site number
         name = L.tok()
@@ -248,6 +248,18 @@ def read_ble(path, form=None):
             qpol = float(L.tok())
             if qpol:
                 pol[name] = (apol, qpol)
+            # Esat, kappa and (cook built with POLAR&1) rep, each optional
+            # (cook/simdef.c:646-694); kappa and rep are the shell-core data
+            rest = []
+            for _ in range(3):
+                t = L.tok(required=False)
+                if t is None:
+                    break
+                rest.append(t)
+            kap = float(rest[1]) if len(rest) > 1 else 0.0
+            rp = int(float(rest[2])) if len(rest) > 2 else 0
+            if kap or rp:
+                shell[name] = (kap, rp)
         types.append(name)
         mass.append(m)
         evdw.append(E)
@@ -326,12 +338,19 @@ def read_ble(path, form=None):
         ff["type_parm"] = [p[0] for p in parm]
     if pol:
         ff["polar"] = pol
+    if shell:
+        ff["shell"] = shell
     return ff
 
 
-def system_from_cook(ble_path, def_path, provider=None, form=None, exp6_cook_units=True, **kw):
+def system_from_cook(ble_path, def_path, provider=None, form=None, exp6_cook_units=True,
+                     shell_core=False, **kw):
     """(BuiltSystem, parameters) from a .ble force field and a .def deck:
     N[i], L[0..2], cutoff, LJcutoff, el.epsr, el.epsk, el.epsinf.
+
+    shell_core=True applies the kappa and rep columns of a polar file as
+    cook built with POLAR&1 does (the shell-core repulsion); by default
+    they are read and left unused, as in cook's POLAR=0 build.
 
     exp-6: cook's exp-6 initcombrule() multiplies EvdW by kcal/Eunit
     (sim/exp6/sitesite.c:52-54) after readblend() has converted it with
@@ -351,6 +370,8 @@ def system_from_cook(ble_path, def_path, provider=None, form=None, exp6_cook_uni
                 lj_cutoff=float(lj))
     if "el.epsinf" in d:
         args["epsinf"] = float(d["el.epsinf"])
+    if shell_core and ff.get("shell"):
+        args["shell"] = ff["shell"]
     args.update(kw)
     return build_system(ff, counts, L, cutoff, provider=provider, **args), d
 

@@ -373,6 +373,9 @@ struct PairTab {
   const float *sl = nullptr;
   int nsl = 0;
   double isgrid = 0;   // 1/sgrid
+  // POLAR&1 shell-core: per type qkappa and rep flag; NULL: none
+  const R *shk = nullptr;
+  const int *shrep = nullptr;
 };
 
 // eru/erd of the hyperbolic spline.  FP64: cook's form B/(C+x)+A
@@ -437,6 +440,39 @@ __device__ inline void qq_term(const PairTab<R> &tab, double sgrid, R qq, const 
   }
 }
 
+// the exponential repulsion of an exp-6 table row at rr: urep = eu
+// e^(tau r), frep = -urep'(r)/r = ef e^(tau r)/r (SS_MEASURE_rep of
+// sim/buck/sitesite.h:145-157 and sim/busing/sitesite.h)
+template <typename R>
+__device__ inline void exp_rep(const R *ss, double rr, R &urep, R &frep) {
+  const R x = sqrt((R)rr), y = exp(ss[0] * x);
+  urep = ss[1] * y;
+  frep = ss[2] * y * rcp<R>(x);
+}
+
+// POLAR&1 shell-core term of a Drude-centre pair at separation d (the
+// force on the central site's Drude or centre, whichever acc is):
+// -kappa urep, force -kappa frep d, inside C2 of the pair's table row
+// (cook/ljqqpol.c:561-592, 616-648)
+template <typename R, bool MEASURE>
+__device__ inline void shell_term(const R *ss, R kappa, const double (&d)[3], R (&acc)[3],
+                                  double &U, double &Vir, double (&pv)[6]) {
+  const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (rr >= (double)ss[5]) return;
+  R urep, frep;
+  exp_rep<R>(ss, rr, urep, frep);
+  const R f = -kappa * frep;
+  const R dx = (R)d[0], dy = (R)d[1], dz = (R)d[2];
+  const R fx = f * dx, fy = f * dy, fz = f * dz;
+  acc[0] += fx; acc[1] += fy; acc[2] += fz;
+  if (MEASURE) {
+    U -= (double)(kappa * urep);
+    Vir += rr * (double)f;
+    pv[0] += (double)(dx * fx); pv[1] += (double)(dy * fy); pv[2] += (double)(dz * fz);
+    pv[3] += (double)(dz * fy); pv[4] += (double)(dx * fz); pv[5] += (double)(dy * fx);
+  }
+}
+
 // POLAR (cook/ljqqpol.c, POLAR=0): the pair is chosen by the centre-centre
 // distance; the Drude of a site sits at r+rpol with charge qpol; the
 // central site collects the forces on its centre (cc, cp) and on its
@@ -457,6 +493,19 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
   const int tj = jj.x & 0xffff;
   const R *ss = ljrow + tj * 8;
   R fr = 0, u = 0;
+  // POLAR&1 shell-core (cook/ljqqpol.c:41-46): the central site's shell
+  // against a repelling partner (an1), and the reverse (an2)
+  bool an1 = false, an2 = false;
+  R k1 = 0, k2 = 0;
+  if constexpr (POLAR && E6) {
+    if (tab.shk) {
+      const int ti = ii.x & 0xffff;
+      k1 = tab.shk[ti];
+      k2 = tab.shk[tj];
+      an1 = k1 != 0 && tab.shrep[tj] != 0;
+      an2 = k2 != 0 && tab.shrep[ti] != 0;
+    }
+  }
   if (rr < (double)ss[5]) {
     if (rr < (double)ss[4]) {
       if (E6) {
@@ -485,6 +534,19 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
   }
   const R qq = qi * (R)pj.w;
   if (qq != 0) {
+    if constexpr (POLAR && E6) {
+      // centre-centre: + kappa urep (cook/ljqqpol.c:525-535)
+      if ((an1 || an2) && rr < (double)ss[5]) {
+        R urep, frep;
+        exp_rep<R>(ss, rr, urep, frep);
+        const R kk = (an1 ? k1 : (R)0) + (an2 ? k2 : (R)0);
+        fr += kk * frep;
+        if (MEASURE) {
+          u += kk * urep;
+          Vir += rr * (double)(kk * frep);
+        }
+      }
+    }
     R eru = 0, erd;
     spline<R>(tab, rr, sgrid, eru, erd, MEASURE);
     fr += qq * erd;
@@ -503,11 +565,13 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
     if (qpi != 0 && qj != 0) {        // Drude i - centre j
       const double d[3] = {dxr + ri.x, dyr + ri.y, dzr + ri.z};
       qq_term<R, MEASURE>(tab, sgrid, qpi * qj, d, ap, Eel, pv);
+      if (an1) shell_term<R, MEASURE>(ss, k1, d, ap, U, Vir, pv);
     }
     if (qpj != 0) {
       if (qi != 0) {                  // centre i - Drude j
         const double d[3] = {dxr - rj.x, dyr - rj.y, dzr - rj.z};
         qq_term<R, MEASURE>(tab, sgrid, qi * qpj, d, ac, Eel, pv);
+        if (an2) shell_term<R, MEASURE>(ss, k2, d, ac, U, Vir, pv);
       }
       if (qpi != 0) {                 // Drude i - Drude j
         const double d[3] = {dxr - rj.x + ri.x, dyr - rj.y + ri.y, dzr - rj.z + ri.z};
@@ -1185,6 +1249,11 @@ int launch_pair_t(Ctx &c, int measure) {
   if constexpr (sizeof(R) == 4) { tab.lj = c.lj32.p; tab.erf = c.erf32.p; }
   else { tab.lj = (const R *)c.lj64.p; tab.erf = (const R *)c.erf64.p; }
   tab.isgrid = 1 / c.erf_sgrid;
+  if (POLAR && E6 && c.shell) {
+    if constexpr (sizeof(R) == 4) tab.shk = c.shk32.p;
+    else tab.shk = (const R *)c.shk64.p;
+    tab.shrep = c.shrep.p;
+  }
   const double cq = c.cutoff * c.cutoff;
   const int np = pair_parts(c);
   const dim3 grid(nblk(c.ns), np);
@@ -1425,6 +1494,10 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
       qn += qt;
     }
     if (int rc0 = ewald_check(s->el_corr, s->el_bg, s->epsinf, qn, q2, "mdx_create")) return rc0;
+    if ((s->shell_qkappa || s->shell_rep) &&
+        (!s->shell_qkappa || !s->shell_rep || !s->chargepol || !s->sstab_exp6))
+      return fail(MDX_E_ARG, "mdx_create: the shell-core terms need shell_qkappa and shell_rep, "
+                             "Drude charges and an exp-6 table (Buckingham or Busing)");
   }
   for (int n = 0; n < s->nmol; n++) {
     if (s->mol_first[n + 1] - s->mol_first[n] > 32)
@@ -1561,6 +1634,22 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
                    !up(c.alpha_qq.p, s->alpha_qq, sizeof(double) * c.ns))))
     return bail(fail(MDX_E_GPU, "mdx_create: upload failed"));
 
+  // POLAR&1 shell-core (only when some type has a shell)
+  if (s->shell_qkappa) {
+    std::vector<double> k64(s->shell_qkappa, s->shell_qkappa + c.ntypes);
+    std::vector<float> k32(k64.begin(), k64.end());
+    for (double k : k64) c.shell = c.shell || k != 0;
+    if (c.shell) {
+      if ((rc = c.shk64.alloc(c.ntypes)) || (rc = c.shk32.alloc(c.ntypes)) ||
+          (rc = c.shrep.alloc(c.ntypes)))
+        return bail(rc);
+      if (!up(c.shk64.p, k64.data(), sizeof(double) * c.ntypes) ||
+          !up(c.shk32.p, k32.data(), sizeof(float) * c.ntypes) ||
+          !up(c.shrep.p, s->shell_rep, sizeof(int) * c.ntypes))
+        return bail(fail(MDX_E_GPU, "mdx_create: upload failed"));
+    }
+  }
+
   // dynamics data (optional)
   if (s->mass) {
     c.ncons = s->ncons;
@@ -1623,6 +1712,9 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.qadj_part.release();
   c.qadj_buf.release();
   c.qadj_rows.release();
+  c.shk64.release();
+  c.shk32.release();
+  c.shrep.release();
   c.tm.release();
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);

@@ -179,6 +179,11 @@ struct Ctx {
   DBuf<double> charge;        // [ns]
   bool polar = false;          // Drude charges present (cook POLAR)
   DBuf<double> chargepol, alpha_qq;  // [ns]
+  // POLAR&1 shell-core: per type qkappa (FP64, FP32) and rep flag
+  bool shell = false;
+  DBuf<double> shk64;
+  DBuf<float> shk32;
+  DBuf<int> shrep;
   DBuf<double> spol;           // [4ns] Drude offset, qpol (cell order)
   DBuf<double> qpos;           // [4nq] charge positions and charges
   DBuf<double> qtrig;          // [12nq] per direction cos, sin of 2 pi s and 16 pi s

@@ -129,7 +129,7 @@ class BuiltSystem:
 def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
                  rplus=2.5, rshift=3, epsinf=3e33, ncell=(0, 0, 0),
                  alpha=None, kappa=None, lj_cutoff=None, provider=None,
-                 omega=0.6, exp6_evdw_scale=1.0, el_corr=0, el_bg=0, el_diag=0):
+                 omega=0.6, exp6_evdw_scale=1.0, el_corr=0, el_bg=0, el_diag=0, shell=None):
     """ff: force-field dict as stored in tests/golden/*.json['forcefield']:
          types, type_mass, type_EvdW_kcal, type_RvdW,
          species[{site_type, site_charge_e, site_excl, bonds}]
@@ -151,7 +151,11 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
        el_corr, el_bg, el_diag: cook's el.corr (Yeh-Berkowitz slab
        correction, bits x=1 y=2 z=4, +8 energy and pressure only), el.bg
        (its charged-background force) and el.diag (1: Ediag in the
-       energy), sim/ewald.c:477-492, 865-997."""
+       energy), sim/ewald.c:477-492, 865-997.
+       shell: POLAR&1 shell-core repulsion, {type name: (kappa, rep)} with
+       cook's ATOM columns kappa (divided by the electron charge unit as
+       cook/simdef.c:692 does) and rep (nonzero: the type repels shells);
+       needs Drudes (ff["polar"]) and the Buckingham or Busing form."""
     prov = provider or engine_provider()
     bs = BuiltSystem()
     S = bs.struct
@@ -299,6 +303,17 @@ def build_system(ff, counts, L, cutoff, epsr=0.05, epsk=0.05, grid=256,
     S.el_corr = int(el_corr)
     S.el_bg = int(el_bg)
     S.el_diag = int(el_diag)
+    if shell:
+        if not polar or not exp6:
+            raise ValueError("shell-core terms need Drudes and the Buckingham or Busing form")
+        qk = np.zeros(nt)
+        rep = np.zeros(nt, np.int32)
+        for name, (kap, rp) in shell.items():
+            t = ff["types"].index(name)
+            qk[t] = kap / ELECTRON
+            rep[t] = 1 if rp else 0
+        S.shell_qkappa = bs.ptr(qk, C.c_double)
+        S.shell_rep = bs.ptr(rep, C.c_int)
     # No.f: 3 per site minus constraints (cook subtracts 3 for the total
     # momentum only when it is conserved; callers can override)
     bs.info = dict(alpha=alpha, kappa=kappa, nk=nk, ncell=list(ncell),

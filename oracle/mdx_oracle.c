@@ -853,16 +853,56 @@ static void orc_sitesite(orc_acc *a, int ti, int tj, double rr, double *U, doubl
   a->nlj += A != 0;
 }
 
+/* SS_MEASURE_rep of the Buckingham and Busing forms (sim/buck/sitesite.h:
+   145-157): the exponential part urep = eu e^(tau r) of the exp-6 table
+   row of types ti, tj and frep = -urep'(r)/r; 0 from C2 on */
+static double orc_rep(const mdx_system_t *S, int ti, int tj, double rr, double *frep)
+{
+  const mdx_exp6_t *e = S->sstab_exp6 + ti * S->ntypes + tj;
+  double x, y;
+  *frep = 0;
+  if (rr >= e->C2q) return 0;
+  x = sqrt(rr);
+  y = exp(e->tau * x);
+  *frep = e->ef * y / x;
+  return e->eu * y;
+}
+
+/* a POLAR&1 shell-core term on a Drude-centre pair (cook/ljqqpol.c:
+   561-592, 616-648): -kappa urep, force -kappa frep along d */
+static void orc_shell(orc_acc *a, int ti, int tj, double kappa, double *f1, double *f2,
+                      const double *d)
+{
+  double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], frep, urep;
+  if (rr >= a->S->sstab_exp6[ti * a->S->ntypes + tj].C2q) return;
+  urep = orc_rep(a->S, ti, tj, rr, &frep);
+  frep *= kappa;
+  a->Envir -= rr * frep;
+  a->U -= kappa * urep;
+  orc_addf(a, f1, f2, -frep, d);
+}
+
 /* polarLJqq[m] for POLAR=0 (cook/ljqqpol.c): the pair is chosen by the
    centre-centre distance; up to four charge-charge terms centre/Drude x
-   centre/Drude, Drudes at r+rpol, LJ between the centres only. */
+   centre/Drude, Drudes at r+rpol, LJ between the centres only.  With
+   POLAR&1 shell-core data (shell_qkappa, shell_rep): anion1 = the central
+   site has qkappa != 0 and the partner rep, anion2 the reverse
+   (cook/ljqqpol.c:41-46); their terms follow cook/ljqqpol.c:525-535
+   (centre-centre), 561-592 (Drude-centre) and 616-648 (centre-Drude). */
 static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
 {
   const mdx_system_t *S = a->S;
   double cq = S->cutoff * S->cutoff;
   for (; l; l = l->next) {
     double dr[3], d[3], rr, r2, f = 0, U = 0, qq, erd, u;
-    int k;
+    double k1 = 0, k2 = 0;
+    int k, an1 = 0, an2 = 0;
+    if (S->shell_qkappa && S->sstab_exp6) {
+      k1 = S->shell_qkappa[ls->st];
+      k2 = S->shell_qkappa[l->st];
+      an1 = k1 != 0 && S->shell_rep[l->st];
+      an2 = k2 != 0 && S->shell_rep[ls->st];
+    }
     for (k = 0; k < 3; k++) dr[k] = ls->r[k] - l->r[k];
     rr = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     if (rr >= cq) continue;
@@ -910,6 +950,11 @@ static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
     orc_sitesite(a, ls->st, l->st, rr, &U, &f);
     a->U += U;
     if ((qq = ls->q * l->q) != 0) {
+      if (an1 || an2) {
+        double frep, urep = orc_rep(S, ls->st, l->st, rr, &frep);
+        if (an1) { a->U += k1 * urep; a->Envir += k1 * frep * rr; f += k1 * frep; }
+        if (an2) { a->U += k2 * urep; a->Envir += k2 * frep * rr; f += k2 * frep; }
+      }
       u = orc_erud(S, rr, &erd);
       if (a->measure) a->Enel += u * qq;
       f += erd * qq;
@@ -921,6 +966,7 @@ static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
       u = orc_erud(S, r2, &erd);
       if (a->measure) a->Enel += u * qq;
       orc_addf(a, ls->fpol, l->f, erd * qq, d);
+      if (an1) orc_shell(a, ls->st, l->st, k1, ls->fpol, l->f, d);
     }
     if (l->qpol != 0) {
       for (k = 0; k < 3; k++) d[k] = dr[k] - l->rpol[k];
@@ -930,6 +976,7 @@ static void orc_ljqq_pol(orc_acc *a, orc_site *l, orc_site *ls)
         u = orc_erud(S, r2, &erd);
         if (a->measure) a->Enel += u * qq;
         orc_addf(a, ls->f, l->fpol, erd * qq, d);
+        if (an2) orc_shell(a, ls->st, l->st, k2, ls->f, l->fpol, d);
       }
       if (ls->qpol != 0) {
         qq = ls->qpol * l->qpol;

@@ -300,3 +300,27 @@ def test_nbfix_table(tmp_path):
     open(ble6, "w").write(open(ble6).read().replace("nnbfixes=0", "nnbfixes=1"))
     with pytest.raises(cookio.CookInputError):
         cookio.read_ble(ble6)
+
+
+def test_shell_core_columns(tmp_path):
+    """the kappa and rep columns of a polar file (cook/simdef.c:646-694):
+    read into ff["shell"]; applied only with shell_core=True (cook's POLAR&1
+    build), which needs the Buckingham or Busing form"""
+    d, meta = oracle.load_fixture("polar306")
+    species, types, _ = CASES["polar306"]()
+    polar = {k: tuple(v) for k, v in meta["forcefield"]["polar"].items()}
+    ble, dfn = write_deck(str(tmp_path), meta, species, types, polar)
+    lines = open(ble).read().splitlines()
+    for i, ln in enumerate(lines):
+        f = ln.split()
+        if len(f) == 10 and f[1] in ("CL", "O", "NA"):
+            f[9] = {"CL": "-245.27", "O": "-100", "NA": "0"}[f[1]]   # kappa
+            f.append("1")                         # rep
+            lines[i] = "  " + "  ".join(f)
+    open(ble, "w").write("\n".join(lines) + "\n")
+    ff = cookio.read_ble(ble)
+    assert ff["shell"] == {"O": (-100.0, 1), "NA": (0.0, 1), "CL": (-245.27, 1)}
+    bs, _ = cookio.system_from_cook(ble, dfn, provider=oracle.provider())
+    assert not bs.struct.shell_qkappa
+    with pytest.raises(ValueError, match="Buckingham or Busing"):
+        cookio.system_from_cook(ble, dfn, provider=oracle.provider(), shell_core=True)
