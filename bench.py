@@ -52,7 +52,7 @@ def parse():
                     help="W,PCOLS,KZC,KPARTS[,MFMA]: stencil, stencil columns per pair part, "
                          "k-space kz chunk and parts, pass 2 on the matrix cores (mixed) instead "
                          "of the autotuner (c4: W,KZC,KPARTS)")
-    ap.add_argument("--kpass1", type=int, default=-1, choices=[-1, 0, 1, 8, 16],
+    ap.add_argument("--kpass1", type=int, default=-1, choices=[-1, 0, 1, 2, 8, 16],
                     help="mixed mode: k-space pass 1 as FMA chains (0) or on the matrix cores "
                          "with 16 or 8 tiles per wave (-1: as pinned or tuned)")
     ap.add_argument("--tune", action="store_true",

@@ -356,10 +356,12 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    in MDX_MIXED.
    mdx_set_kspace_pass1 (MDX_MIXED): k-space pass 1 as FP32 FMA chains
    (tiles = 0), on the matrix cores as units of 8 k rows with all their
-   kz per wave (tiles = 1, the default where kz stays below 64: FP32 MFMA
-   16x16x4, sums added to FP32 totals every 128 charges and to FP64 per
-   charge split), or on the matrix cores with 16 or 8 tiles of 4 k rows
-   and one 8-kz block per wave (FP64 flush every 32 charges).
+   kz per wave (tiles = 1: FP32 MFMA 16x16x4, sums added to FP32 totals
+   every 128 charges and to FP64 per charge split), the same units with
+   two waves each, which share the row operands and split the kz blocks
+   (tiles = 2, the default where kz stays below 64), or on the matrix cores
+   with 16 or 8 tiles of 4 k rows and one 8-kz block per wave (FP64 flush
+   every 32 charges).
    mdx_autotune keeps the fastest. */
 int mdx_set_kspace_mfma(mdx_ctx_t *ctx, int on);
 int mdx_set_kspace_pass1(mdx_ctx_t *ctx, int tiles);

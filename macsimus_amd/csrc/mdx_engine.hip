@@ -1674,7 +1674,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   if (c.nq && (rc = kspace_setup(c))) return bail(rc);
   // mixed mode: pass 1 on the matrix cores as row units where the k-set
   // allows it (kz below 64)
-  if (mode == MDX_MIXED && c.nq && c.K.nblk <= 8) c.p1nt = 1;
+  if (mode == MDX_MIXED && c.nq && c.K.nblk <= 8) c.p1nt = 2;
   *out = h;
   return 0;
 }
@@ -1916,11 +1916,12 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
     c.kmfma = best_m;
     if (!rc) rc = kspace_chunks(c, best_k, best_p);
     if (!rc && c.mode == MDX_MIXED) {
-      // pass 1: FP32 FMA (0) or matrix cores with 16 or 8 tiles per wave
+      // pass 1: FP32 FMA (0), matrix-core row units of one or two waves
+      // (1, 2) or matrix cores with 16 or 8 tiles per wave
       int best_nt = c.p1nt;
       double t1 = 1e30;
-      for (int nt : {0, 1, 16, 8}) {
-        if (nt == 1 && c.K.nblk > 8) continue;
+      for (int nt : {0, 1, 2, 16, 8}) {
+        if ((nt == 1 || nt == 2) && c.K.nblk > 8) continue;
         c.p1nt = nt;
         if ((rc = run(fam))) break;
         if (fam[T_KPASS1] < t1) { t1 = fam[T_KPASS1]; best_nt = nt; }
@@ -1938,8 +1939,8 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
 
 extern "C" int mdx_set_kspace_pass1(mdx_ctx_t *h, int tiles) {
   if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: NULL");
-  if (tiles != 0 && tiles != 1 && tiles != 8 && tiles != 16)
-    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0, 1, 8 or 16");
+  if (tiles != 0 && tiles != 1 && tiles != 2 && tiles != 8 && tiles != 16)
+    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0, 1, 2, 8 or 16");
   if (tiles && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: MDX_MIXED only");
   h->c.p1nt = tiles;
   h->c.gen++;

@@ -345,13 +345,13 @@ struct SfTab {
 
 // the cos/sin tables of charges [jc, jc+JC) of the split ending at j1,
 // from the anchors of k_qgather: 7 FP32 recursion steps per block of 8
-template <int JC>
+template <int JC, int BLK>
 __device__ inline void sf_tables(float *Xf, float *Yf, float *Zf, int KXT, int KYT, int KZT,
                                  int jc, int j1, int nq, const float2 *__restrict__ qanc,
                                  const float2 *__restrict__ qbase) {
   constexpr int S = SfTab<JC>::S;
   const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
-  for (int t = threadIdx.x; t < JC * ntask; t += kBlock) {
+  for (int t = threadIdx.x; t < JC * ntask; t += BLK) {
     const int jj = t % JC, task = t / JC;
     int d, m, KT;
     float *dst;
@@ -376,16 +376,17 @@ __device__ inline void sf_tables(float *Xf, float *Yf, float *Zf, int KXT, int K
   }
 }
 
-// one wave's unit of NA N tiles over the charges [j0, j1): every wave of
-// the workgroup builds tables and meets the barriers alike
-template <int NA, int JC>
+// one wave's NA N tiles of a unit, the tiles t0, t0 + ts, ... (ts waves
+// share a unit), over the charges [j0, j1): every wave of the workgroup
+// (BLK threads) builds tables and meets the barriers alike
+template <int NA, int JC, int BLK>
 __device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT, int KZT,
                                int j0, int j1, int r0, int nrows, const int *__restrict__ row_kx,
                                const int *__restrict__ row_ky, const int *__restrict__ blk_rows,
                                const int *__restrict__ blk_seg0, int nseg,
                                double *__restrict__ kpart, int nq,
                                const float2 *__restrict__ qanc,
-                               const float2 *__restrict__ qbase) {
+                               const float2 *__restrict__ qbase, int t0, int ts) {
   constexpr int S = SfTab<JC>::S;
   const int lane = threadIdx.x & 63, i16 = lane & 15, kq = lane >> 4;
   // operand offsets (floats) for charge 0 of a chunk's first K step:
@@ -400,7 +401,7 @@ __device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT
     xo[m] = kx * S + 2 * kq + ((i16 >> 1) & 1);
     yo[m] = ky * S + 2 * kq + (i16 & 1);
   }
-  const int zo = (i16 >> 1) * S + 2 * kq + (i16 & 1);
+  const int zo = (i16 >> 1) * S + 2 * kq + (i16 & 1) + 8 * t0 * S;
   mfma_f32x4 acc[2][NA > 0 ? NA : 1];
   float mid[2][NA > 0 ? NA : 1][4];
 #pragma unroll
@@ -411,7 +412,7 @@ __device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT
       for (int r = 0; r < 4; r++) { acc[m][t][r] = 0.0f; mid[m][t][r] = 0; }
   int nchunk = 0;
   for (int jc = j0; jc < j1; jc += JC) {
-    sf_tables<JC>(Xf, Yf, Zf, KXT, KYT, KZT, jc, j1, nq, qanc, qbase);
+    sf_tables<JC, BLK>(Xf, Yf, Zf, KXT, KYT, KZT, jc, j1, nq, qanc, qbase);
     __syncthreads();
     if (NA > 0) {
 #pragma unroll 4
@@ -420,7 +421,7 @@ __device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT
         const float a1 = Xf[xo[1] + 2 * s] * Yf[yo[1] + 2 * s];
         float b[NA > 0 ? NA : 1];
 #pragma unroll
-        for (int t = 0; t < NA; t++) b[t] = Zf[zo + 8 * t * S + 2 * s];
+        for (int t = 0; t < NA; t++) b[t] = Zf[zo + 8 * ts * t * S + 2 * s];
 #pragma unroll
         for (int t = 0; t < NA; t++) {
           acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b[t], acc[0][t], 0, 0, 0);
@@ -445,8 +446,9 @@ __device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT
     const int row = r0 + 4 * m + kq;
 #pragma unroll
     for (int t = 0; t < NA; t++) {
-      if (row >= blk_rows[t]) continue;
-      const int seg = blk_seg0[t] + row / 64;
+      const int T = t0 + ts * t;
+      if (row >= blk_rows[T]) continue;
+      const int seg = blk_seg0[T] + row / 64;
       double *out = kpart + (((size_t)blockIdx.y * nseg + seg) * 64 + row % 64) * 64 +
                     (i16 >> 1) * 8 + (i16 & 1);
 #pragma unroll
@@ -455,8 +457,12 @@ __device__ inline void sf_unit(float *Xf, float *Yf, float *Zf, int KXT, int KYT
   }
 }
 
-template <int NTMAX, int JC>
-__global__ __launch_bounds__(kBlock) void k_sfac_mf(
+// WPU waves per unit of 8 rows (4 units per workgroup of 256 WPU
+// threads): the waves of a unit share its A operands and take every
+// WPU-th N tile, so the tables of a chunk serve 4 WPU waves and a wave
+// holds NTMAX/WPU tiles' accumulators
+template <int NTMAX, int JC, int WPU>
+__global__ __launch_bounds__(kBlock * WPU, WPU == 1 ? 3 : 4) void k_sfac_mf(
     int nq, const double4 *__restrict__ qpos, int nrows, const int *__restrict__ row_kx,
     const int *__restrict__ row_ky, const int *__restrict__ row_kzmax,
     const int *__restrict__ blk_rows, const int *__restrict__ blk_seg0, int nseg, int KXT,
@@ -464,26 +470,31 @@ __global__ __launch_bounds__(kBlock) void k_sfac_mf(
     const float2 *__restrict__ qbase) {
   extern __shared__ char smem[];
   constexpr int S = SfTab<JC>::S;
+  constexpr int BLK = kBlock * WPU;
   float *Xf = (float *)smem, *Yf = Xf + KXT * S, *Zf = Yf + KYT * S;
-  const int r0 = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 8;
-  // N tiles of the unit (wave-uniform), 0 for a wave past the rows
+  const int wv = threadIdx.x >> 6, hf = wv % WPU;
+  const int r0 = (blockIdx.x * 4 + wv / WPU) * 8;
+  // N tiles of the unit (wave-uniform), 0 for a wave past the rows; this
+  // wave's share: tiles hf, hf + WPU, ...
   int nt = 0;
   if (r0 < nrows) nt = min((row_kzmax[r0] + 8) / 8, NTMAX);
   nt = __builtin_amdgcn_readfirstlane(nt);
+  nt = (nt - hf + WPU - 1) / WPU;
   const int j0 = blockIdx.y * jper, j1 = min(nq, j0 + jper);
 #define MDX_U(NA)                                                                               \
-  sf_unit<NA, JC>(Xf, Yf, Zf, KXT, KYT, KZT, j0, j1, r0, nrows, row_kx, row_ky, blk_rows,       \
-                  blk_seg0, nseg, kpart, nq, qanc, qbase)
+  sf_unit<NA, JC, BLK>(Xf, Yf, Zf, KXT, KYT, KZT, j0, j1, r0, nrows, row_kx, row_ky, blk_rows,  \
+                       blk_seg0, nseg, kpart, nq, qanc, qbase, hf, WPU)
+  constexpr int NW = (NTMAX + WPU - 1) / WPU;   // most tiles of one wave
   switch (nt) {
     case 0: MDX_U(0); break;
     case 1: MDX_U(1); break;
-    case 2: if constexpr (NTMAX >= 2) MDX_U(2); break;
-    case 3: if constexpr (NTMAX >= 3) MDX_U(3); break;
-    case 4: if constexpr (NTMAX >= 4) MDX_U(4); break;
-    case 5: if constexpr (NTMAX >= 5) MDX_U(5); break;
-    case 6: if constexpr (NTMAX >= 6) MDX_U(6); break;
-    case 7: if constexpr (NTMAX >= 7) MDX_U(7); break;
-    default: if constexpr (NTMAX >= 8) MDX_U(8); break;
+    case 2: if constexpr (NW >= 2) MDX_U(2); break;
+    case 3: if constexpr (NW >= 3) MDX_U(3); break;
+    case 4: if constexpr (NW >= 4) MDX_U(4); break;
+    case 5: if constexpr (NW >= 5) MDX_U(5); break;
+    case 6: if constexpr (NW >= 6) MDX_U(6); break;
+    case 7: if constexpr (NW >= 7) MDX_U(7); break;
+    default: if constexpr (NW >= 8) MDX_U(8); break;
   }
 #undef MDX_U
 }
@@ -907,7 +918,7 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   dim3 g1((K.nseg + 3) / 4, c.nsplit);
   const double4 *qpos = (const double4 *)c.qpos.p;
   // anchors of the matrix-core pass 1 (mixed mode, row units)
-  const bool anc = sizeof(R) == 4 && c.p1nt == 1;
+  const bool anc = sizeof(R) == 4 && (c.p1nt == 1 || c.p1nt == 2);
   const int3 nb = make_int3((KXT + 7) / 8, (KYT + 7) / 8, (KZT + 7) / 8);
   const int3 boff = make_int3(0, nb.x, nb.x + nb.y);
   if (anc && (size_t)(nb.x + nb.y + nb.z + 3) * c.nq > c.qanc.n)
@@ -923,25 +934,33 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   bool pass1_mfma = false;
   int nsplit1 = c.nsplit;              // charge splits written to kpart
   if constexpr (sizeof(R) == 4) pass1_mfma = c.p1nt != 0;
-  if (pass1_mfma && c.p1nt == 1) {
-    // row units of 8 rows, 4 waves per workgroup; splits for about 1024
-    // workgroups
+  if (pass1_mfma && (c.p1nt == 1 || c.p1nt == 2)) {
+    // row units of 8 rows, 4 units per workgroup (one or two waves each);
+    // splits for about 1024 workgroups
     const int wgx = ((K.nrows + 7) / 8 + 3) / 4;
     nsplit1 = std::max(1, std::min(c.nsplit, 1024 / wgx));
     const int jp = (c.nq + nsplit1 - 1) / nsplit1;
     const dim3 gm(wgx, nsplit1);
     constexpr int JC = 64;
     const size_t shm = sizeof(float) * SfTab<JC>::S * (KXT + KYT + KZT);
-#define MDX_SF(NT)                                                                            \
-  k_sfac_mf<NT, JC><<<gm, kBlock, shm, st>>>(c.nq, qpos, K.nrows, c.d_row_kx.p, c.d_row_ky.p, \
-                                           c.d_row_kzmax.p, c.d_blk_rows.p, c.d_blk_seg0.p,  \
-                                           K.nseg, KXT, KYT, KZT, jp, c.kpart.p, c.qanc.p,   \
-                                           c.qanc.p + (size_t)(nb.x + nb.y + nb.z) * c.nq)
-    if (K.nblk <= 2) MDX_SF(2);
-    else if (K.nblk <= 4) MDX_SF(4);
-    else if (K.nblk <= 6) MDX_SF(6);
-    else if (K.nblk <= 8) MDX_SF(8);
-    else return fail(MDX_E_ARG, "k-space pass 1 on row units: at most 8 kz blocks (kz < 64)");
+#define MDX_SF(NT, WPU)                                                                       \
+  k_sfac_mf<NT, JC, WPU><<<gm, kBlock * WPU, shm, st>>>(                                      \
+      c.nq, qpos, K.nrows, c.d_row_kx.p, c.d_row_ky.p, c.d_row_kzmax.p, c.d_blk_rows.p,        \
+      c.d_blk_seg0.p, K.nseg, KXT, KYT, KZT, jp, c.kpart.p, c.qanc.p,                          \
+      c.qanc.p + (size_t)(nb.x + nb.y + nb.z) * c.nq)
+    if (K.nblk > 8)
+      return fail(MDX_E_ARG, "k-space pass 1 on row units: at most 8 kz blocks (kz < 64)");
+    if (c.p1nt == 2) {
+      if (K.nblk <= 2) MDX_SF(2, 2);
+      else if (K.nblk <= 4) MDX_SF(4, 2);
+      else if (K.nblk <= 6) MDX_SF(6, 2);
+      else MDX_SF(8, 2);
+    } else {
+      if (K.nblk <= 2) MDX_SF(2, 1);
+      else if (K.nblk <= 4) MDX_SF(4, 1);
+      else if (K.nblk <= 6) MDX_SF(6, 1);
+      else MDX_SF(8, 1);
+    }
 #undef MDX_SF
   } else if (pass1_mfma && c.p1nt == 8)
     k_sfac_mfma<8><<<dim3((K.nseg + 1) / 2, c.nsplit), kBlock, shmem, st>>>(

@@ -13,7 +13,7 @@ from macsimus_amd.system import build_system, engine_provider
 
 pytestmark = pytest.mark.gpu
 
-TILES = (0, 1, 16, 8)
+TILES = (0, 1, 2, 16, 8)
 
 
 def rel_rms(a, b):
