@@ -105,7 +105,9 @@ def traffic_lookup(workload, mode, kernel, info):
         if kernel == "pair" and (e["stencil_w"], e.get("pair_cols_per_part", 0)) != (
                 info["stencil"], info.get("pair_cols", 0)):
             continue
-        if kernel == "kpass2" and bool(e.get("kspace_pass2_mfma", False)) != bool(info.get("kmfma", 0)):
+        if kernel == "kpass2" and int(e.get("kspace_pass2_mfma", 0)) != int(info.get("kmfma", 0)):
+            continue
+        if kernel == "kpass1" and int(e.get("kspace_pass1_tiles", 0)) != int(info.get("kpass1_tiles", 0)):
             continue
         return e["bytes"]
     return None
