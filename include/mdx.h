@@ -187,6 +187,9 @@ int mdx_forces(mdx_ctx_t *ctx, const double *r, double *f, mdx_out_t *out,
 /* Box change between calls (box.L); k-vector set stays as in
    sim/ewald.c:570-613 (ekk recomputed, same ktab). */
 int mdx_set_box(mdx_ctx_t *ctx, const double L[3]);
+/* The context's box (box.L): as created or set, or as the NPT steps of
+   mdx_md_steps left it. */
+int mdx_get_box(mdx_ctx_t *ctx, double L[3]);
 
 /* A new system in the same context (No.s, molecules, types, tables or box
    changed: cook re-allocates linklist()'s and Ewald()'s scratch on such a
@@ -211,7 +214,9 @@ typedef struct mdx_step_s {
                           sim/thermo.c:180-200); not with Drudes;
                           11, 12, 13 = the decoupled friction thermostats
                           of molecular translations, of the intramolecular
-                          motion, of both (sim/thermo.c:107-135) */
+                          motion, of both (sim/thermo.c:107-135);
+                          22, 23 = MTK, Hoover isotropic NPT (below): every
+                          step is measured (the barostat needs the pressure) */
   double T;            /* target temperature [K] */
   double tau;          /* thermostat time constant tau.T [ps] */
   double nf;           /* No.f degrees of freedom */
@@ -230,6 +235,21 @@ typedef struct mdx_step_s {
      freedom (No.f_in = nf - nf_tr), and T_tr_in = T_tr/T_in of 13 */
   double nf_tr;
   double T_tr_in;
+  /* 22 = MTK, 23 = Hoover isotropic barostat with the Nose thermostat
+     (sim/thermo.c:58-99, sim/trvpscale.c:45-70, sim/siminit.c:604-626;
+     cook's rescale="xyz": every site scaled, molecules distorted as in
+     rescalecfg(), sim/norm.c:909-1016); not with Drudes.  P: the target
+     pressure No.P in program units (Pa / Punit, sim/units.h); tau_P:
+     tau.P [ps] (>= 0.999e6: no barostat degree of freedom in No.fx);
+     virial: the pressure En.Pref that drives the box, 1 = Pevir (virial
+     theorem) or 3 (or 0) = Ptr (trace of the pressure tensor),
+     sim/constrd.c:256-262; Ecorr: En.corr of mdx_cutcorr added as
+     Ecorr/V^2 (cook's corr&2; 0: none).  tau, T, nf and trvp_k as for
+     Nose-Hoover; No.f = nf. */
+  double P;
+  double tau_P;
+  double Ecorr;
+  int virial;
 } mdx_step_t;
 
 typedef struct mdx_step_out_s {
@@ -247,6 +267,13 @@ typedef struct mdx_step_out_s {
      the step and En.dlogs (sim/thermo.c:180-200) */
   double En_ext, logs, dlogs;
   double En_kin_tr;    /* kinetic energy of the molecular translations (En.kin_tr/2) */
+  /* NPT (22, 23), last step: box after it; En.Pref that drove it; En.V =
+     the volume it was measured at; En.lambda[0] = log L(t) and
+     En.dlambda[0] (predicted); En_ext there is No.fx T logs(t) + the
+     barostat and thermostat kinetic terms + V P (sim/thermo.c:69-84) */
+  double L[3];
+  double Pref, V;
+  double npt_lambda, npt_dlambda;
 } mdx_step_out_t;
 
 /* Replaces one selffield() iteration (sim/scf.c:12-240, POLAR=0):
@@ -292,6 +319,20 @@ int mdx_set_overlap(mdx_ctx_t *ctx, int on);
    aspc_k than the context holds restarts the predictor (kk = 0). */
 int mdx_set_aspc(mdx_ctx_t *ctx, int k, int kk, const double *hist);
 int mdx_get_aspc(mdx_ctx_t *ctx, int *k, int *kk, double *hist);
+
+/* Isotropic NPT state (thermostats 22, 23): logs = cfg[0]->logs, logs1 =
+   cfg[1]->logs, lambda1 = cfg[1]->lambda[0] (log L(t) - log L(t-h); the
+   box itself is the context's, mdx_set_box), the velocity predictor's
+   length k (0..9), the length reached kk (0..k) and its history
+   hist [k][2 + 3 ns]: per level logs1, lambda1, then r1 of the sites
+   (hist = NULL: zeros).  cook keeps k, kk and the history in statics of
+   sim/constrd.c:673-684 (rows of No.eq: logs, lambda[3], rp) and logs,
+   lambda in the cfg.  The predictor (k, kk and the site rows) is the one
+   Nose-Hoover uses; logs, logs1 and lambda1 are separate from its state. */
+int mdx_set_npt(mdx_ctx_t *ctx, double logs, double logs1, double lambda1, int k, int kk,
+                const double *hist);
+int mdx_get_npt(mdx_ctx_t *ctx, double *logs, double *logs1, double *lambda1, int *k,
+                int *kk, double *hist);
 
 /* Nose-Hoover state: logs = cfg[0]->logs, logs1 = cfg[1]->logs
    (logs(t) - logs(t-h)), the velocity predictor's length k (0..9), the
@@ -449,7 +490,8 @@ int mdx_batch_forces(mdx_batch_t *b, const double *r, double *f, mdx_out_t *out,
 int mdx_batch_upload_state(mdx_batch_t *b, const double *r, const double *r1);
 int mdx_batch_download_state(mdx_batch_t *b, double *r, double *r1);
 /* nsteps leap-frog steps of all systems; p[nsys] per system (h, T, tau,
-   nf, thermostat are used; measure_every of p[0] for all); out[nsys]
+   nf, thermostat are used, thermostat 0 or 1 (Berendsen) only;
+   measure_every of p[0] for all); out[nsys]
    filled at the last measured step (frc, En_kin, Pkin, lambda).
    Unmeasured steps are replayed from one hipGraph of the whole batch step
    (mdx_batch_set_graph, default on), re-captured when p changes. */

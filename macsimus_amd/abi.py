@@ -79,6 +79,7 @@ class Step(C.Structure):
         ("nf", C.c_double), ("measure_every", C.c_int),
         ("aspc_k", C.c_int), ("scf_iter", C.c_int), ("scf_omega", C.c_double),
         ("trvp_k", C.c_int), ("nf_tr", C.c_double), ("T_tr_in", C.c_double),
+        ("P", C.c_double), ("tau_P", C.c_double), ("Ecorr", C.c_double), ("virial", C.c_int),
     ]
 
 
@@ -90,6 +91,8 @@ class StepOut(C.Structure):
         ("lambda_", C.c_double), ("shake_iter_max", C.c_int),
         ("En_ext", C.c_double), ("logs", C.c_double), ("dlogs", C.c_double),
         ("En_kin_tr", C.c_double),
+        ("L", C.c_double * 3), ("Pref", C.c_double), ("V", C.c_double),
+        ("npt_lambda", C.c_double), ("npt_dlambda", C.c_double),
     ]
 
 

@@ -747,6 +747,8 @@ extern "C" int mdx_batch_md_steps(mdx_batch_t *h, const mdx_step_t *p, int nstep
   std::vector<double> sp(5L * b.nsys);
   for (int s = 0; s < b.nsys; s++) {
     if (!(p[s].h > 0)) return fail(MDX_E_ARG, "mdx_batch_md_steps: h must be > 0");
+    if (p[s].thermostat != 0 && p[s].thermostat != 1)
+      return fail(MDX_E_ARG, "mdx_batch_md_steps: thermostat 0 (none) or 1 (Berendsen) only");
     if (p[s].thermostat && !(p[s].tau > 0 && p[s].nf > 0 && p[s].T > 0))
       return fail(MDX_E_ARG, "mdx_batch_md_steps: Berendsen needs T, tau, nf > 0");
     sp[5 * s] = p[s].h; sp[5 * s + 1] = p[s].T; sp[5 * s + 2] = p[s].tau; sp[5 * s + 3] = p[s].nf;

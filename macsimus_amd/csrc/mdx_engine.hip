@@ -1754,6 +1754,12 @@ extern "C" int mdx_set_box(mdx_ctx_t *h, const double L[3]) {
   return 0;  // k-space uses c.L at every call (ekk recomputed, same k-set)
 }
 
+extern "C" int mdx_get_box(mdx_ctx_t *h, double L[3]) {
+  if (!h || !L) return fail(MDX_E_ARG, "mdx_get_box: NULL argument");
+  for (int d = 0; d < 3; d++) L[d] = h->c.L[d];
+  return 0;
+}
+
 // a new system in the same context: everything mdx_create builds is built
 // again for it (a complete new context is made and swapped in, the old one
 // destroyed; on failure the old system stays); the settings carry over

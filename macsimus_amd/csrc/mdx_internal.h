@@ -274,6 +274,10 @@ struct Ctx {
   // velocity predictor's history [trvp_k][1 + 3 ns] of length trvp_kk
   DBuf<double> nose, trvp_hist;
   int trvp_k = -1, trvp_kk = 0;
+  // isotropic NPT (host): logs, logs1, lambda1 = log L(t) - log L(t-h) and
+  // the predictor's histories of logs1 and lambda1 (the site rows are in
+  // trvp_hist, whose logs slots NPT leaves unused)
+  double npt_logs = 0, npt_logs1 = 0, npt_lam1 = 0, npt_hlogs[9] = {}, npt_hlam[9] = {};
   // configuration generation: bumped by every change of buffers, grid or
   // k-space split, so that a captured step graph is rebuilt
   unsigned gen = 0;

@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "mdx_internal.h"
 
@@ -59,27 +61,36 @@ Trvp trvp_coef(int kk, int kmax) {
 
 // one thread per molecule: p = h^2 f/m + r + r1, SHAKE sweeps until the
 // largest relative violation is <= eps, then r1 = p - r, r = p.
-// Nose-Hoover (nose != NULL, sim/verlet.c:22-36): p -= Vfactor vpred with
-// the predicted h v of each site and h dlogs/dt from b and the history
-// (level 0 = the current r1 and logs1, level j = hist[j-1], stride 1+3ns:
-// logs1 then the sites); each thread then shifts its sites' history (the
-// logs slots are shifted by k_nose after all threads have read them)
+// The velocity predictor (pred != PRED_NONE, sim/verlet.c:22-36): p -=
+// Vfactor vpred with the predicted h v of each site from b and the history
+// (level 0 = the current r1, level j = hist[j-1], stride 1+3ns: a logs
+// slot then the sites); Vfactor is h dlogs/dt predicted from the device
+// Nose variables (PRED_NOSE: nose[1] and the logs slots) or given by the
+// host (PRED_HOST, the NPT Vfactor of sim/trvpscale.c:62).  Each thread
+// then shifts its sites' history (the logs slots are shifted by k_nose
+// after all threads have read them).  bscale multiplies p_i - p_j in the
+// constraint (the predicted box scaling of NPT, sim/constrd.c:820; 1 else).
+enum { PRED_NONE = 0, PRED_NOSE = 1, PRED_HOST = 2 };
 __global__ __launch_bounds__(kBlock) void k_shake(
     int nmol, const int *__restrict__ molfirst, const int *__restrict__ cons_first,
     const int *__restrict__ cons_ij, const double *__restrict__ bondq,
     const double *__restrict__ omega, const double *__restrict__ mass, double h, double eps,
     int maxit, int measure, double *__restrict__ r, double *__restrict__ r1,
-    const double *__restrict__ f, double *__restrict__ p, double *__restrict__ part,
-    const double *__restrict__ nose, double *__restrict__ hist, long stride, Trvp tv) {
+    const double *__restrict__ f, double *__restrict__ p, double *__restrict__ part, int pred,
+    const double *__restrict__ nose, double vfac_host, double bscale, double *__restrict__ hist,
+    long stride, Trvp tv) {
   const int m = blockIdx.x * kBlock + threadIdx.x;
   double ek = 0, pk[6] = {0, 0, 0, 0, 0, 0}, virc = 0, ptv[6] = {0, 0, 0, 0, 0, 0};
   double its = 0, bad = 0, ktr = 0;
   if (m < nmol) {
     const double hh = h * h;
     const int b = molfirst[m], e = molfirst[m + 1];
-    if (nose) {
-      double vfac = tv.b[0] * nose[1];
-      for (int j = 1; j <= tv.kk; j++) vfac += tv.b[j] * hist[(j - 1) * stride];
+    if (pred != PRED_NONE) {
+      double vfac = vfac_host;
+      if (pred == PRED_NOSE) {
+        vfac = tv.b[0] * nose[1];
+        for (int j = 1; j <= tv.kk; j++) vfac += tv.b[j] * hist[(j - 1) * stride];
+      }
       for (int i = b; i < e; i++) {
         const double im = hh / mass[i];
         for (int k = 0; k < 3; k++) {
@@ -110,7 +121,10 @@ __global__ __launch_bounds__(kBlock) void k_shake(
           const int a = cons_ij[2 * al], c = cons_ij[2 * al + 1];
           const double bq = bondq[al], mi = 1 / mass[a], mj = 1 / mass[c];
           double pij[3], rij[3];
-          for (int k = 0; k < 3; k++) { pij[k] = p[3 * a + k] - p[3 * c + k]; rij[k] = r[3 * a + k] - r[3 * c + k]; }
+          for (int k = 0; k < 3; k++) {
+            pij[k] = (p[3 * a + k] - p[3 * c + k]) * bscale;
+            rij[k] = r[3 * a + k] - r[3 * c + k];
+          }
           double sq = bq - (pij[0] * pij[0] + pij[1] * pij[1] + pij[2] * pij[2]);
           maxsq = fmax(maxsq, fabs(sq / bq));
           sq *= om / (mi + mj);
@@ -207,6 +221,13 @@ __global__ void k_berendsen(int n3, double *r1, const double *acc, double h, dou
   const double lam = exp(b * -h / tau);
   if (i < n3) r1[i] *= lam;
   if (i == 0 && lambda_out) *lambda_out = lam;
+}
+
+// the isotropic NPT rescaling of the sites (rescalecfg(cfg[0], RESCALE_XYZ
+// | RESCALE_L, 0, scale), sim/norm.c:989-1000): r(t+h) *= scale; r1 stays
+__global__ void k_rescale(int n3, double *r, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) r[i] *= scale;
 }
 
 // Nose-Hoover, after SHAKE (sim/thermo.c:180-200): shifts the logs slots
@@ -597,13 +618,144 @@ extern "C" int mdx_get_nose(mdx_ctx_t *h, double *logs, double *logs1, int *k, i
   return 0;
 }
 
+extern "C" int mdx_set_npt(mdx_ctx_t *h, double logs, double logs1, double lambda1, int k, int kk,
+                           const double *hist) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_npt: NULL handle");
+  Ctx &c = h->c;
+  if (!c.mass.p) return fail(MDX_E_ARG, "mdx_set_npt: system has no masses");
+  if (k < 0 || k > 9 || kk < 0 || kk > k) return fail(MDX_E_ARG, "NPT: need 0 <= kk <= k <= 9");
+  MDX_HIP(hipSetDevice(c.device));
+  const size_t row = 1 + 3 * (size_t)c.ns;
+  std::vector<double> dev;
+  if (hist && k > 0) {
+    // the device rows: a logs slot (unused by NPT) and the sites
+    dev.resize(row * k);
+    for (int j = 0; j < k; j++) {
+      dev[j * row] = hist[j * (row + 1)];
+      std::memcpy(&dev[j * row + 1], hist + j * (row + 1) + 2, sizeof(double) * 3 * c.ns);
+    }
+  }
+  int rc;
+  if ((rc = set_trvp(c, k, kk, dev.empty() ? nullptr : dev.data()))) return rc;
+  c.npt_logs = logs;
+  c.npt_logs1 = logs1;
+  c.npt_lam1 = lambda1;
+  for (int j = 0; j < 9; j++) {
+    c.npt_hlogs[j] = hist && j < k ? hist[j * (row + 1)] : 0.0;
+    c.npt_hlam[j] = hist && j < k ? hist[j * (row + 1) + 1] : 0.0;
+  }
+  return 0;
+}
+
+extern "C" int mdx_get_npt(mdx_ctx_t *h, double *logs, double *logs1, double *lambda1, int *k,
+                           int *kk, double *hist) {
+  if (!h) return fail(MDX_E_ARG, "mdx_get_npt: NULL handle");
+  Ctx &c = h->c;
+  if (!c.mass.p) return fail(MDX_E_ARG, "mdx_get_npt: system has no masses");
+  const int kn = std::max(c.trvp_k, 0);
+  if (hist && kn > 0) {
+    MDX_HIP(hipSetDevice(c.device));
+    const size_t row = 1 + 3 * (size_t)c.ns;
+    std::vector<double> dev(row * kn);
+    MDX_HIP(hipMemcpyAsync(dev.data(), c.trvp_hist.p, sizeof(double) * dev.size(),
+                           hipMemcpyDeviceToHost, c.stream));
+    MDX_HIP(hipStreamSynchronize(c.stream));
+    for (int j = 0; j < kn; j++) {
+      hist[j * (row + 1)] = c.npt_hlogs[j];
+      hist[j * (row + 1) + 1] = c.npt_hlam[j];
+      std::memcpy(hist + j * (row + 1) + 2, &dev[j * row + 1], sizeof(double) * 3 * c.ns);
+    }
+  }
+  if (logs) *logs = c.npt_logs;
+  if (logs1) *logs1 = c.npt_logs1;
+  if (lambda1) *lambda1 = c.npt_lam1;
+  if (k) *k = kn;
+  if (kk) *kk = c.trvp_kk;
+  return 0;
+}
+
 namespace {
+
+// isotropic NPT, before the forces (sim/trvpscale.c:43-70 with rescale =
+// "xyz"): the predicted h dlogs/dt (Vp0) and h dlambda/dt (Vp1) from the
+// current logs1, lambda1 and their history, Vfactor = Vp0 + No.NPT Vp1
+// (No.NPT = 1 + 3/No.f for MTK, 1 for Hoover, sim/siminit.c:607), the
+// constraint scaling exp(2 lambda1 - lambda1(t-h)) (exp(lambda1) while the
+// predictor is shorter than 2), then the history shift of both scalars
+struct NptPred {
+  double Vp0, Vp1, vfac, bscale;
+};
+
+NptPred npt_predict(Ctx &c, const mdx_step_t *p, int kk) {
+  const Trvp tv = trvp_coef(kk, c.trvp_k);
+  NptPred q{};
+  for (int j = 0; j <= kk; j++) {
+    q.Vp0 += tv.b[j] * (j ? c.npt_hlogs[j - 1] : c.npt_logs1);
+    q.Vp1 += tv.b[j] * (j ? c.npt_hlam[j - 1] : c.npt_lam1);
+  }
+  const double NPT = 1. + (p->thermostat == 22) * 3. / p->nf;
+  q.vfac = q.Vp0 + NPT * q.Vp1;
+  q.bscale = kk > 1 ? std::exp(2 * c.npt_lam1 - c.npt_hlam[0]) : std::exp(c.npt_lam1);
+  for (int j = tv.knew; j >= 2; j--) {
+    c.npt_hlogs[j - 1] = c.npt_hlogs[j - 2];
+    c.npt_hlam[j - 1] = c.npt_hlam[j - 2];
+  }
+  if (tv.knew >= 1) {
+    c.npt_hlogs[0] = c.npt_logs1;
+    c.npt_hlam[0] = c.npt_lam1;
+  }
+  return q;
+}
+
+// isotropic NPT after SHAKE (sim/thermo.c:58-99, masses No.M_T, No.M_P and
+// No.fx of sim/siminit.c:604-626 with No.ncoord = 3): En.Pref at the
+// current volume, the leap-frog of logs and of lambda = log L, then the
+// sites r(t+h) and the box scaled by exp(lambda(t+h) - lambda(t)).  kin
+// is En.kin (doubled); out has this step's measured outputs.
+int npt_update(mdx_ctx_t *h, const mdx_step_t *p, const NptPred &q, double kin,
+               mdx_step_out_t *out) {
+  Ctx &c = h->c;
+  const double hh = p->h * p->h, T = p->T, V = c.L[0] * c.L[1] * c.L[2];
+  const double invf = (double)(p->thermostat == 22) / p->nf;
+  const double fx = p->nf + (p->tau_P < .999e6 ? 1 : 0);
+  const double M_T = p->tau * p->tau * T * p->nf, M_Th = M_T / 2;
+  const double M_P = p->tau_P * p->tau_P * T * (p->nf + 3) * 3, M_Ph = M_P / 2;
+  mdx_pressure_t pr;
+  int rc = mdx_pressure(out, c.L, 1.0, p->Ecorr, 0, &pr);
+  if (rc) return rc;
+  const double Pref = p->virial == 1 ? pr.Pevir_c : pr.Ptr_c;
+  const double ldlogs = c.npt_logs1, lam0 = std::log(c.L[0]);
+  double ext = fx * T * c.npt_logs;
+  const double kinT = M_Th / hh * (q.Vp0 * q.Vp0), kinP = M_Ph / hh * (q.Vp1 * q.Vp1);
+  c.npt_logs1 += hh * (kin + 6 * kinP - fx * T) / M_T;
+  c.npt_logs += c.npt_logs1;
+  ext += kinT + kinP * 3 + V * p->P;
+  c.npt_lam1 += hh * (V * (Pref - p->P) + kin * invf) / M_P - q.Vp0 * q.Vp1;
+  const double scale = std::exp((lam0 + c.npt_lam1) - lam0);
+  out->En_ext = ext;
+  out->logs = c.npt_logs;
+  out->dlogs = (ldlogs + c.npt_logs1) / (2 * p->h);
+  out->Pref = Pref;
+  out->V = V;
+  out->npt_lambda = lam0;
+  out->npt_dlambda = q.Vp1 / p->h;
+  k_rescale<<<nblk(3L * c.ns), kBlock, 0, c.stream>>>(3 * c.ns, c.r.p, scale);
+  if (hipGetLastError() != hipSuccess) return fail(MDX_E_GPU, "mdx_md_steps: launch");
+  const double L[3] = {c.L[0] * scale, c.L[1] * scale, c.L[2] * scale};
+  if ((rc = mdx_set_box(h, L))) return rc;
+  for (int d = 0; d < 3; d++) out->L[d] = c.L[d];
+  return 0;
+}
+
+inline bool is_npt(int thermostat) { return thermostat == 22 || thermostat == 23; }
 
 // the launches of one MD step (no host synchronization): for Drudes the
 // ASPC prediction with predictor length kk and scf_iter forces()+selffield()
 // with mixing omega, then forces (others), SHAKE + leap-frog, the reduction
-// and the Berendsen scaling
-int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) {
+// and the Berendsen scaling.  NPT: vfac and bscale of this step from the
+// host (its thermostat update and rescaling follow on the host)
+int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega,
+                double vfac = 0, double bscale = 1) {
   const int nb = nblk(c.nmol);
   int rc;
   if (c.polar) {
@@ -618,14 +770,14 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) 
     return rc;
   }
   if (c.timing) c.tm.start(T_SHAKE, c.stream);
-  const bool nose = p->thermostat == 2;
+  const bool nose = p->thermostat == 2, npt = is_npt(p->thermostat);
   const long stride = 1 + 3L * c.ns;
-  const Trvp tv = trvp_coef(nose ? kk : 0, nose ? c.trvp_k : 0);
+  const Trvp tv = trvp_coef(nose || npt ? kk : 0, nose || npt ? c.trvp_k : 0);
   k_shake<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.cons_first.p, c.cons_ij.p,
                                        c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
                                        p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
-                                       c.mdpart.p, nose ? c.nose.p : nullptr, c.trvp_hist.p,
-                                       stride, tv);
+                                       c.mdpart.p, nose ? PRED_NOSE : npt ? PRED_HOST : PRED_NONE,
+                                       c.nose.p, vfac, bscale, c.trvp_hist.p, stride, tv);
   if (c.timing) c.tm.stop(T_SHAKE, c.stream);
   k_md_reduce<<<1, kBlock, 0, c.stream>>>(c.mdpart.p, nb, c.mdacc.p, c.mdflag.p);
   if (p->thermostat == 1)
@@ -634,7 +786,7 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega) 
   if (nose)
     k_nose<<<1, 64, 0, c.stream>>>(c.mdacc.p, c.nose.p, c.trvp_hist.p, stride, tv.knew, p->h,
                                    p->T, p->tau, p->nf);
-  if (p->thermostat >= 11)
+  if (p->thermostat >= 11 && p->thermostat <= 13)
     k_friction<<<nb, kBlock, 0, c.stream>>>(c.nmol, c.molfirst.p, c.mass.p, c.r1.p, c.mdacc.p,
                                             p->h, p->T, p->tau, p->nf, p->nf_tr, p->T_tr_in,
                                             p->thermostat);
@@ -647,7 +799,8 @@ bool same_step(const mdx_step_t &a, const mdx_step_t &b) {
   return a.h == b.h && a.eps == b.eps && a.maxit == b.maxit && a.thermostat == b.thermostat &&
          a.T == b.T && a.tau == b.tau && a.nf == b.nf && a.aspc_k == b.aspc_k &&
          a.scf_iter == b.scf_iter && a.scf_omega == b.scf_omega && a.trvp_k == b.trvp_k &&
-         a.nf_tr == b.nf_tr && a.T_tr_in == b.T_tr_in;
+         a.nf_tr == b.nf_tr && a.T_tr_in == b.T_tr_in && a.P == b.P && a.tau_P == b.tau_P &&
+         a.Ecorr == b.Ecorr && a.virial == b.virial;
 }
 
 // an unmeasured step as a hipGraph, captured once per step parameters and
@@ -698,10 +851,16 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
   if (!c.state_on_device) return fail(MDX_E_ARG, "mdx_md_steps: upload the state first");
   if (!(p->h > 0) || !(p->eps > 0)) return fail(MDX_E_ARG, "mdx_md_steps: h and eps must be > 0");
   const int th = p->thermostat;
-  if (!(th >= 0 && th <= 2) && !(th >= 11 && th <= 13))
-    return fail(MDX_E_ARG, "mdx_md_steps: thermostat 0 (none), 1 (Berendsen), 2 (Nose-Hoover) "
-                           "or 11-13 (friction on translations, intramolecular, both)");
-  if (th >= 11 && (c.polar || !(p->nf_tr > 0 && p->nf - p->nf_tr > 0) ||
+  const bool npt = is_npt(th);
+  if (!(th >= 0 && th <= 2) && !(th >= 11 && th <= 13) && !npt)
+    return fail(MDX_E_ARG, "mdx_md_steps: thermostat 0 (none), 1 (Berendsen), 2 (Nose-Hoover), "
+                           "11-13 (friction on translations, intramolecular, both) or 22, 23 "
+                           "(MTK, Hoover NPT)");
+  if (npt && (c.polar || !(p->tau_P > 0) || !std::isfinite(p->P) || !std::isfinite(p->Ecorr) ||
+              !(p->virial == 0 || p->virial == 1 || p->virial == 3)))
+    return fail(MDX_E_ARG, "mdx_md_steps: NPT needs tau_P > 0, a finite P and Ecorr, virial 0, "
+                           "1 or 3, and no Drude charges");
+  if (th >= 11 && th <= 13 && (c.polar || !(p->nf_tr > 0 && p->nf - p->nf_tr > 0) ||
                    (th == 13 && !(p->T_tr_in > 0))))
     return fail(MDX_E_ARG, "mdx_md_steps: the friction thermostats need 0 < nf_tr < nf (and "
                            "T_tr_in > 0 for 13), not for Drude systems");
@@ -709,15 +868,18 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
     return fail(MDX_E_ARG, "mdx_md_steps: the thermostat needs T, tau, nf > 0");
   if (p->thermostat == 2 && c.polar)
     return fail(MDX_E_ARG, "mdx_md_steps: Nose-Hoover is not carried for Drude systems");
-  if (p->thermostat == 2 && (p->trvp_k < 0 || p->trvp_k > 9))
+  if ((p->thermostat == 2 || npt) && (p->trvp_k < 0 || p->trvp_k > 9))
     return fail(MDX_E_ARG, "mdx_md_steps: trvp_k in 0..9");
   if (c.polar && !(p->scf_omega <= 1))
     return fail(MDX_E_ARG, "mdx_md_steps: scf_omega must be <= 1 (<= 0: stability value)");
   MDX_HIP(hipSetDevice(c.device));
   int rc;
   if (c.polar && c.aspc_k != p->aspc_k && (rc = set_aspc(c, p->aspc_k, 0, nullptr))) return rc;
-  if (p->thermostat == 2 && c.trvp_k != p->trvp_k && (rc = set_trvp(c, p->trvp_k, 0, nullptr)))
-    return rc;
+  if ((p->thermostat == 2 || npt) && c.trvp_k != p->trvp_k) {
+    if ((rc = set_trvp(c, p->trvp_k, 0, nullptr))) return rc;
+    std::fill(c.npt_hlogs, c.npt_hlogs + 9, 0.0);
+    std::fill(c.npt_hlam, c.npt_hlam + 9, 0.0);
+  }
   std::memset(out, 0, sizeof(*out));
   out->lambda = 1;
   const int every = p->measure_every > 0 ? p->measure_every : nsteps;
@@ -725,26 +887,27 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
   reset_flags(c);
   rc = 0;
   for (int s = 1; s <= nsteps && !rc; s++) {
-    const int measure = (s % every == 0) || s == nsteps;
+    const int measure = npt || (s % every == 0) || s == nsteps;
     // predictor length of this step (ASPC: grows to aspc_k, sim/pred2.c:
-    // 67-77; Nose-Hoover: grows to trvp_k, sim/trvpscale.c:95)
-    const bool nose = p->thermostat == 2;
+    // 67-77; Nose-Hoover and NPT: grows to trvp_k, sim/trvpscale.c:95)
+    const bool nose = p->thermostat == 2 || npt;
     const int kk = nose ? c.trvp_kk : c.aspc_kk;
     if (c.polar && c.aspc_kk < c.aspc_k) c.aspc_kk++;
     if (nose && c.trvp_kk < c.trvp_k) c.trvp_kk++;
+    const NptPred np = npt ? npt_predict(c, p, kk) : NptPred{0, 0, 0, 1};
     const double omega = !c.polar ? 0.0
                          : p->scf_omega > 0 ? p->scf_omega
                                             : (c.aspc_kk + 2) / (double)(2 * c.aspc_kk + 3);
     // unmeasured steps of the steady state replay a captured graph
     const bool steady = nose ? kk == c.trvp_k : (!c.polar || kk == c.aspc_k);
     if (!measure && steady && c.use_graph && !c.timing) rc = graph_step(c, p, kk, omega);
-    else rc = launch_step(c, p, measure, kk, omega);
+    else rc = launch_step(c, p, measure, kk, omega, np.vfac, np.bscale);
     if (rc) break;
     if (measure) {
       if (hipMemcpyAsync(acc, c.acc.p, sizeof(acc), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
           hipMemcpyAsync(md, c.mdacc.p, sizeof(md), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
           hipMemcpyAsync(flag, c.mdflag.p, sizeof(flag), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-          (nose && hipMemcpyAsync(nv, c.nose.p, sizeof(nv), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
+          (th == 2 && hipMemcpyAsync(nv, c.nose.p, sizeof(nv), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
           (c.polar && hipMemcpyAsync(sf, c.sfacc.p, sizeof(sf), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
           hipStreamSynchronize(c.stream) != hipSuccess) {
         rc = fail(MDX_E_GPU, "mdx_md_steps: download");
@@ -766,7 +929,7 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
       out->virc = md[MD_VIRC] / hh;
       out->shake_iter_max = (int)flag[2];
       out->lambda = flag[0];
-      if (nose) {
+      if (th == 2) {
         out->logs = nv[0];
         out->En_ext = nv[2];
         out->dlogs = nv[3];
@@ -777,6 +940,7 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
         out->En_self = sf[SF_SELF] / 2;
         out->scf_maxerr = sqrt(sf[SF_ERR]);
       }
+      if (npt && (rc = npt_update(h, p, np, md[MD_EKIN] * hscale, out))) break;
     }
   }
   (void)hipStreamSynchronize(c.stream);

@@ -112,6 +112,44 @@ class Engine:
             abi.check(self.lib, rc, "mdx_get_nose")
         return a.value, b.value, k.value, kk.value, hist
 
+    def set_npt(self, logs=0.0, logs1=0.0, lambda1=0.0, k=0, kk=0, hist=None):
+        """isotropic NPT state: logs, logs1, lambda1 = log L(t) - log L(t-h),
+        predictor length k, length reached kk, history [k][2+3ns] (per
+        level logs1, lambda1, the sites' r1; None: zeros)"""
+        ptr = None
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, dtype=np.float64)
+            assert hist.shape == (k, 2 + 3 * self.ns)
+            ptr = hist.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.mdx_set_npt(self.ctx, float(logs), float(logs1), float(lambda1), int(k),
+                                  int(kk), ptr)
+        abi.check(self.lib, rc, "mdx_set_npt")
+
+    def get_npt(self):
+        """(logs, logs1, lambda1, k, kk, hist [k][2+3ns])"""
+        a, b, l1, k, kk = C.c_double(), C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        rc = self.lib.mdx_get_npt(self.ctx, C.byref(a), C.byref(b), C.byref(l1), C.byref(k),
+                                  C.byref(kk), None)
+        abi.check(self.lib, rc, "mdx_get_npt")
+        hist = np.zeros((k.value, 2 + 3 * self.ns))
+        if k.value:
+            rc = self.lib.mdx_get_npt(self.ctx, None, None, None, None, None,
+                                      hist.ctypes.data_as(C.POINTER(C.c_double)))
+            abi.check(self.lib, rc, "mdx_get_npt")
+        return a.value, b.value, l1.value, k.value, kk.value, hist
+
+    def set_box(self, L):
+        """a new box (k-vector set kept, ekk recomputed)"""
+        rc = self.lib.mdx_set_box(self.ctx, (C.c_double * 3)(*[float(x) for x in L]))
+        abi.check(self.lib, rc, "mdx_set_box")
+
+    def box(self):
+        """the context's box L (changed by NPT steps and set_box)"""
+        L = (C.c_double * 3)()
+        rc = self.lib.mdx_get_box(self.ctx, L)
+        abi.check(self.lib, rc, "mdx_get_box")
+        return np.array(L[:])
+
     def cook_cells(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         cid = np.empty(self.ns, np.int32)
@@ -459,6 +497,12 @@ def declare_extras(lib):
                                  C.POINTER(C.c_double)]
     lib.mdx_get_nose.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.mdx_set_npt.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                C.POINTER(C.c_double)]
+    lib.mdx_get_npt.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_double)]
+    lib.mdx_get_box.argtypes = [C.c_void_p, C.c_double * 3]
     lib.mdx_resize.argtypes = [C.c_void_p, C.POINTER(abi.System)]
     lib.mdx_set_rdf.argtypes = [C.c_void_p, C.c_int, C.c_double]
     lib.mdx_get_rdf.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_long,

@@ -25,6 +25,7 @@
  *   other site-site forms           exp-6 sim/exp6/sitesite.{c,h}, Buckingham sim/buck,
  *                                   Busing sim/busing (POW = 0), all in the exp-6 table
  *   RDF, Nose-Hoover, frictions     cook/ljqqnp.c:218-225, sim/trvpscale.c, sim/thermo.c:107-135
+ *   MTK / Hoover isotropic NPT      sim/trvpscale.c:43-70, sim/thermo.c:58-99, sim/norm.c:909-1016
  * (these later functions state their own citations and how they are pinned)
  * Everything is double precision, serial except the k-space sums, which
  * split the charges over OpenMP threads as cook's PARALLEL=1 does
@@ -437,6 +438,13 @@ typedef struct {
   int ntrip, nk;
 } orc_kset;
 
+/* the box the k-vector set and Ediag belong to: cook keeps the k-vectors
+   chosen for the starting box when the box changes and recomputes only
+   ekk with the new one (sim/ewald.c:570-613); Ediag is computed at the
+   initialization only (sim/ewald.c:475-492).  NULL: the system's box.
+   Set for the duration of one NPT step. */
+static const double *orc_kbox = NULL;
+
 static void orc_kset_build(orc_kset *K, double kappa, const double L[3])
 {
   double Kk = kappa * kappa;
@@ -582,7 +590,7 @@ static void orc_ewald_pass(int pass, const orc_kset *K, const double *L,
    each, times charges.sumq/(2 pi V) */
 static double orc_ediag(const mdx_system_t *S, double sumq)
 {
-  const double *L = S->L;
+  const double *L = orc_kbox ? orc_kbox : S->L;
   double qkappax = sqr(S->kappa) + sqr(S->alpha), kappax = sqrt(qkappax), kk;
   double q = -sqr(PI / S->alpha), E = 0;
   int Kx = kappax * L[0], Ky = kappax * L[1], Kz = kappax * L[2], kx, ky, kz;
@@ -609,7 +617,7 @@ static double orc_ewald(const mdx_system_t *S, int nq, const int *idx,
   double (*F)[3];
   int i, kx, ky, kz, ikk, iQ, t;
 
-  orc_kset_build(&K, S->kappa, L);
+  orc_kset_build(&K, S->kappa, orc_kbox ? orc_kbox : L);
   ekk = malloc(sizeof(double) * K.ntrip);
   ikk = 0;
   for (kx = 0; kx <= K.kxmax; kx++)
@@ -1311,11 +1319,12 @@ int oracle_selffield(const mdx_system_t *S, double *r, double *f, mdx_out_t *out
 /* ------------------------------------------------------------- MD step */
 
 /* One Shake() step (sim/constrd.c:622-1081) for SHAKE=1, VERLET=3,
-   thermostat none or Berendsen, no Nose/NPT.  r = cfg[0], r1 = cfg[1]
+   thermostat none, Berendsen or frictions (the Nose-Hoover and NPT steps
+   below add their predictor through vpred, vfac, bscale).  r = cfg[0], r1 = cfg[1]
    are updated in place; f is scratch (cfg[2]).  Energies returned in out. */
 static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
                           const mdx_step_t *p, int measure, mdx_step_out_t *out,
-                          const double *vpred, double vfac);
+                          const double *vpred, double vfac, double bscale);
 
 int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
                    const mdx_step_t *p, int measure, mdx_step_out_t *out,
@@ -1325,7 +1334,7 @@ int oracle_md_step(const mdx_system_t *S, double *r, double *r1, double *f,
   memset(out, 0, sizeof(*out));
   err = oracle_forces(S, r, f, &out->frc, measure, nthreads, NULL);
   if (err) return err;
-  return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0);
+  return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0, 1);
 }
 
 /* ASPC coefficients B_1..B_{kk+2} (sim/pred2.c:94-98) */
@@ -1380,7 +1389,7 @@ int oracle_md_step_polar(const mdx_system_t *S, double *r, double *r1, double *f
     out->En_self = es;
     out->scf_maxerr = me;
   }
-  return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0);
+  return orc_shake_step(S, r, r1, f, p, measure, out, NULL, 0, 1);
 }
 
 /* blogT(): ln(E/T)/2 clamped to [-1, 1] (sim/constrd.c:98-107) */
@@ -1392,9 +1401,11 @@ static double orc_blogT(double E, double T)
   return log(E / T) / 2;
 }
 
+/* bscale: the NPT constraint scaling of p_i - p_j (sim/constrd.c:820,
+   the isotropic scale of sim/trvpscale.c:66-68; 1 otherwise) */
 static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const double *f,
                           const mdx_step_t *p, int measure, mdx_step_out_t *out,
-                          const double *vpred, double vfac)
+                          const double *vpred, double vfac, double bscale)
 {
   int ns = S->ns, n, i, k, al;
   double hh = p->h * p->h, hscale = 0.5 / hh;
@@ -1423,7 +1434,7 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
         int a = S->cons_ij[2 * al], b = S->cons_ij[2 * al + 1];
         double bondq = S->cons_bondq[al], sq = bondq, pij[3], rij[3];
         double mi = 1 / S->mass[a], mj = 1 / S->mass[b];
-        for (k = 0; k < 3; k++) pij[k] = pp[3 * a + k] - pp[3 * b + k];
+        for (k = 0; k < 3; k++) pij[k] = (pp[3 * a + k] - pp[3 * b + k]) * bscale;
         sq -= pij[0] * pij[0] + pij[1] * pij[1] + pij[2] * pij[2];
         if (fabs(sq / bondq) > maxsq) maxsq = fabs(sq / bondq);
         for (k = 0; k < 3; k++) rij[k] = r[3 * a + k] - r[3 * b + k];
@@ -1558,7 +1569,7 @@ int oracle_md_step_nose(const mdx_system_t *S, double *r, double *r1, double *f,
     memcpy(hist + 1, r1, sizeof(double) * 3 * ns);
   }
   *kk = knew;
-  err = orc_shake_step(S, r, r1, f, p, measure, out, vpred, vfac);
+  err = orc_shake_step(S, r, r1, f, p, measure, out, vpred, vfac, 1);
   free(vpred);
   if (err) return err;
   /* logs by leap-frog with the doubled kinetic energy (thermo.c) */
@@ -1570,6 +1581,96 @@ int oracle_md_step_nose(const mdx_system_t *S, double *r, double *r1, double *f,
   nose[3] = (ld + nose[1]) / (2 * p->h);
   nose[2] = (logs0 + (p->tau / p->h) * (p->tau / p->h) * (ld * ld + nose[1] * nose[1]) / 4)
             * p->nf * p->T;
+  return 0;
+}
+
+/* One Shake() step with the isotropic MTK (thermostat 22) or Hoover (23)
+   barostat and the Nose thermostat, rescale="xyz": the predictor of
+   sim/trvpscale.c:43-70 for logs, lambda and the sites (Vfactor = Vp0 +
+   No.NPT Vp1, No.NPT = 1 + 3/No.f for MTK and 1 for Hoover, and the
+   constraint scaling exp(2 lambda1 - lambda1(t-h)), or exp(lambda1) while
+   the predictor is shorter than 2), the Verlet step of sim/verlet.c:22-36,
+   SHAKE with the scaled p_i - p_j (sim/constrd.c:815-839), the pressure
+   En.Pref of measurePconstraints() (sim/constrd.c:185-262: Ptr, the trace
+   of the pressure tensor, or Pevir for virial = 1, + Ecorr/V^2), the
+   leap-frog of logs and lambda of sim/thermo.c:58-99 with No.fx, No.M_T,
+   No.M_P of sim/siminit.c:604-626 (No.ncoord = 3, No.Pkinq = 1), and the
+   rescaling of the sites and the box by exp(lambda(t+h) - lambda(t))
+   (rescalecfg(), sim/norm.c:989-1005).  L [3]: the box, updated; L0: the
+   box of the k-vector set.  npt[0..2] = logs, logs1 (cfg[1]->logs),
+   lambda1 (cfg[1]->lambda[0]), updated; npt[3] = En.ext, npt[4] = En.Pref,
+   npt[5] = En.dlogs out.  hist [k][2 + 3 ns]: per level logs1, lambda1,
+   then the h v of the sites; *kk as for Nose-Hoover.  No Drudes. */
+int oracle_md_step_npt(const mdx_system_t *S, double *r, double *r1, double *f, double *L,
+                       const double *L0, const mdx_step_t *p, double npt[6], int *kk,
+                       double *hist, mdx_step_out_t *out, int nthreads)
+{
+  int ns = S->ns, stride = 2 + 3 * ns, k = *kk, j, i, err, knew;
+  double b[10], hh = p->h * p->h, *vpred, Vp0 = 0, Vp1 = 0, vfac, bscale, NPT;
+  double V, tr[3], Pref, kin, vir, lam0, scale, ldlogs, kinT, kinP, ext;
+  double invf, fx, M_T, M_P, T = p->T;
+  mdx_system_t SL = *S;
+  if (!(p->thermostat == 22 || p->thermostat == 23) || p->trvp_k < 0 || p->trvp_k > 9 ||
+      k > p->trvp_k || S->chargepol || !(p->tau_P > 0))
+    return MDX_E_ARG;
+  for (j = 0; j < 3; j++) SL.L[j] = L[j];
+  orc_kbox = L0;
+  err = oracle_forces(&SL, r, f, &out->frc, 1, nthreads, NULL);
+  orc_kbox = NULL;
+  if (err) return err;
+  b[0] = (double)(2 * k + 1) / (k + 1);
+  for (j = 0; j < k; j++) b[j + 1] = -b[j] * (double)(k - j) / (k + j + 2);
+  for (j = 0; j <= k; j++) {
+    Vp0 += b[j] * (j ? hist[(size_t)(j - 1) * stride] : npt[1]);
+    Vp1 += b[j] * (j ? hist[(size_t)(j - 1) * stride + 1] : npt[2]);
+  }
+  NPT = 1. + (p->thermostat == 22) * 3. / p->nf;
+  vfac = Vp0 + NPT * Vp1;
+  bscale = k > 1 ? exp(2 * npt[2] - hist[1]) : exp(npt[2]);
+  vpred = malloc(sizeof(double) * 3 * ns);
+  for (i = 0; i < 3 * ns; i++) {
+    double v = b[0] * r1[i];
+    for (j = 1; j <= k; j++) v += b[j] * hist[(size_t)(j - 1) * stride + 2 + i];
+    vpred[i] = v;
+  }
+  knew = k + 1 < p->trvp_k ? k + 1 : p->trvp_k;
+  for (j = knew; j >= 2; j--)
+    memcpy(hist + (size_t)(j - 1) * stride, hist + (size_t)(j - 2) * stride, sizeof(double) * stride);
+  if (knew >= 1) {
+    hist[0] = npt[1];
+    hist[1] = npt[2];
+    memcpy(hist + 2, r1, sizeof(double) * 3 * ns);
+  }
+  *kk = knew;
+  err = orc_shake_step(&SL, r, r1, f, p, 1, out, vpred, vfac, bscale);
+  free(vpred);
+  if (err) return err;
+  /* En.Pref at the volume of this step */
+  V = L[0] * L[1] * L[2];
+  kin = 2 * out->En_kin;                    /* En.kin, doubled */
+  for (j = 0; j < 3; j++) tr[j] = (out->frc.Pvir[j] + out->PTvirc[j]) / V + out->Pkin[j] / V;
+  vir = out->frc.En_vir + out->virc + out->frc.En_el;
+  Pref = (p->virial == 1 ? (kin + vir) / 3 / V : (tr[0] + tr[1] + tr[2]) / 3) + p->Ecorr / (V * V);
+  /* the leap-frog of logs and lambda (sim/thermo.c:58-99) */
+  invf = (double)(p->thermostat == 22) / p->nf;
+  fx = p->nf + (p->tau_P < .999e6 ? 1 : 0);
+  M_T = p->tau * p->tau * T * p->nf;
+  M_P = p->tau_P * p->tau_P * T * (p->nf + 3) * 3;
+  ldlogs = npt[1];
+  ext = fx * T * npt[0];
+  kinT = M_T / 2 / hh * (Vp0 * Vp0);
+  kinP = M_P / 2 / hh * (Vp1 * Vp1);
+  npt[1] += hh * (kin + 6 * kinP - fx * T) / M_T;
+  npt[0] += npt[1];
+  npt[5] = (ldlogs + npt[1]) / (2 * p->h);
+  ext += kinT + kinP * 3 + V * p->P;
+  lam0 = log(L[0]);
+  npt[2] += hh * (V * (Pref - p->P) + kin * invf) / M_P - Vp0 * Vp1;
+  scale = exp((lam0 + npt[2]) - lam0);
+  npt[3] = ext;
+  npt[4] = Pref;
+  for (i = 0; i < 3 * ns; i++) r[i] *= scale;
+  for (j = 0; j < 3; j++) L[j] *= scale;
   return 0;
 }
 

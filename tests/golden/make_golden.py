@@ -407,6 +407,52 @@ def nose_case(name, species, types, counts, L, cutoff, nsteps, tauT, pred):
     print(name, len(dumps), "dumps", len(cfg0), len(cfg1), "cfg bytes")
 
 
+def npt_case(name, species, types, counts, L, cutoff, nsteps, tauT, tauP, P_Pa, pred,
+             thermostat="MTK"):
+    """The isotropic MTK (or Hoover) barostat with the Nose thermostat and
+    the velocity predictor in cook's Shake() (sim/trvpscale.c, sim/verlet.c,
+    sim/constrd.c:705-762, sim/thermo.c:11-95): run 1 (crystal, Berendsen,
+    5 steps) saves md.cfg; run 2 starts from it with thermostat="MTK",
+    tau.T, tau.P, P [Pa], rescale="xyz" and -p<pred>, every force call
+    dumped by -v128 (positions, forces and the box of each step).  Stored:
+    both .cfg files as bytes and the dumps."""
+    with tempfile.TemporaryDirectory() as td:
+        write_ble(os.path.join(td, "sys.ble"), species, types)
+        lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
+        lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
+                  "cutoff=%g" % cutoff, "LJcutoff=cutoff", "el.epsr=0.05", "el.epsk=0.05",
+                  "h=0.002", "T=298", "epsc=1e-12", ";"]
+        open(os.path.join(td, "sys.def"), "w").write("\n".join(lines) + "\n")
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="crystal"\nthermostat="Berendsen"\ntau.T=0.1\nnoint=5\nno=1;\n')
+        run([COOK, "sys", "md"], td)
+        cfg0 = open(os.path.join(td, "md.cfg"), "rb").read()
+        open(os.path.join(td, "md.get"), "w").write(
+            'init="start"\nthermostat="%s"\ntau.T=%g\ntau.P=%g\nP=%.10g\nrescale="xyz"\n'
+            'noint=%d\nno=1;\n' % (thermostat, tauT, tauP, P_Pa, nsteps))
+        run([COOK, "sys", "md", "-v128", "-p%d" % pred], td)
+        cfg1 = open(os.path.join(td, "md.cfg"), "rb").read()
+        dumps = []
+        k = 0
+        while os.path.exists(os.path.join(td, "md.%d.dump" % k)):
+            dumps.append(parse_dump(os.path.join(td, "md.%d.dump" % k)))
+            k += 1
+        meta = parse_prt(os.path.join(td, "md.prt"))
+        txt = open(os.path.join(td, "md.prt")).read()
+    out = dict(L=np.stack([d["L"] for d in dumps]), r=np.stack([d["r"] for d in dumps]),
+               f=np.stack([d["f"] for d in dumps]),
+               cfg0=np.frombuffer(cfg0, np.uint8), cfg1=np.frombuffer(cfg1, np.uint8))
+    meta.update(dict(name=name, counts=counts, L=L, cutoff=cutoff, epsr=0.05, epsk=0.05,
+                     h=0.002, T=298.0, tauT=tauT, tauP=tauP, P_Pa=P_Pa, pred=pred,
+                     thermostat=thermostat, nsteps=nsteps, ndumps=len(dumps),
+                     forcefield=ff_arrays(species, types)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(name, len(dumps), "dumps", len(cfg0), len(cfg1), "cfg bytes")
+    return txt
+
+
 def parse_rdf(path):
     """cook's binary .rdf file (saverdf(), sim/simmeas.c:812-825): records
     <int4 size><data><int4 checksum> (gen/varfile.c); the grid, the number
@@ -599,6 +645,13 @@ def main(argv):
         # decoupled friction thermostats on 216 SPC/E from a saved configuration
         "spce216_friction": lambda: friction_case(
             "spce216_friction", [spce], ["H", "O"], [216], 18.6206, 8.0, 6),
+        # MTK and Hoover isotropic NPT (Nose thermostat, TRVP k=3) on 216
+        # SPC/E from a saved configuration at 100 MPa
+        "spce216_npt": lambda: npt_case(
+            "spce216_npt", [spce], ["H", "O"], [216], 18.6206, 8.0, 8, 0.05, 0.2, 1e8, 3),
+        "spce216_hoover": lambda: npt_case(
+            "spce216_hoover", [spce], ["H", "O"], [216], 18.6206, 8.0, 8, 0.05, 0.2, 1e8, 3,
+            thermostat="Hoover"),
         # Nose-Hoover (TRVP k=3) on 216 SPC/E from a saved configuration
         "spce216_nose": lambda: nose_case(
             "spce216_nose", [spce], ["H", "O"], [216], 18.6206, 8.0, 8, 0.05, 3),

@@ -193,6 +193,32 @@ def aspc_state(d, post, k):
     return r, r1, hist
 
 
+def md_step_npt(bs, r, r1, L, L0, npt, kk, hist, p, nthreads=None):
+    """one Shake() step with the isotropic MTK/Hoover barostat; r, r1, the
+    box L [3], npt [6] (logs, logs1, lambda1, En.ext, En.Pref, dlogs) and
+    hist [k][2+3ns] updated in place; L0 is the box of the k-vector set;
+    returns (StepOut, new kk)"""
+    lib = load()
+    dp = C.POINTER(C.c_double)
+    lib.oracle_md_step_npt.argtypes = [C.POINTER(abi.System), dp, dp, dp, dp, dp,
+                                       C.POINTER(abi.Step), dp, C.POINTER(C.c_int), dp,
+                                       C.POINTER(abi.StepOut), C.c_int]
+    for a in (r, r1, L, L0, npt, hist):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    assert hist.shape[0] >= max(p.trvp_k, 1) and hist.shape[1] == 2 + r.size
+    f = np.zeros_like(r)
+    out = abi.StepOut()
+    k = C.c_int(kk)
+    rc = lib.oracle_md_step_npt(C.byref(bs.struct), r.ctypes.data_as(dp), r1.ctypes.data_as(dp),
+                                f.ctypes.data_as(dp), L.ctypes.data_as(dp), L0.ctypes.data_as(dp),
+                                C.byref(p), npt.ctypes.data_as(dp), C.byref(k),
+                                hist.ctypes.data_as(dp), C.byref(out),
+                                nthreads or os.cpu_count() or 1)
+    if rc:
+        raise RuntimeError("oracle_md_step_npt failed (%d)" % rc)
+    return out, k.value
+
+
 def md_step_nose(bs, r, r1, nose, kk, hist, p, measure=1, nthreads=None):
     """one Shake() step with Nose-Hoover + TRVP; r, r1, nose [4] (logs,
     logs1, En.ext, dlogs) and hist [k][1+3ns] updated in place; returns
