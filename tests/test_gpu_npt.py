@@ -115,3 +115,49 @@ def test_npt_refuses_bad_parameters(tmp_path):
     r, _ = e.download_state()
     assert np.array_equal(r, c0["cfgs"][0]["rp"]) and np.array_equal(e.box(), c0["L"])
     e.close()
+
+
+def test_extended_energy_fluctuation_is_second_order_in_h(tmp_path):
+    """the conserved quantity of the MTK step, En.tot = En.U + En.ext
+    (sim/constrd.c:1060-1068, sim/thermo.c:69-84, with the cutoff
+    correction Ecorr/V of the current box), over 2 ps from cook's start
+    at h = 2 fs and 1 fs: its fluctuation is that of the leap-frog energy
+    estimate, second order in h (a wrong barostat force would leave an
+    h-independent error), and it does not drift by more than it fluctuates;
+    Nose-Hoover at 2 fs from the same start fluctuates as much"""
+    d, meta, (c0, c1) = load_case("spce216_npt", tmp_path)
+    bs = oracle.fixture_system(meta, provider_=engine_provider())
+    p = npt_params(meta, bs)
+
+    def series(h, th):
+        e = Engine(bs, abi.MDX_FP64)
+        e.upload_state(c0["cfgs"][0]["rp"], c0["cfgs"][1]["rp"] * (h / meta["h"]))
+        q = abi.Step.from_buffer_copy(p)
+        q.h, q.thermostat = h, th
+        if th == 2:
+            e.set_nose(0, 0, 0, 0)
+        else:
+            e.set_npt(0, 0, 0, 0, 0)
+        n = int(round(2.0 / h))
+        blk = n // 200
+        H = []
+        for _ in range(200):
+            so = e.md_steps(q, blk)
+            V = so.V if th != 2 else meta["L"] ** 3
+            H.append(so.En_kin + so.frc.En_pot + so.frc.En_el + p.Ecorr / V + so.En_ext)
+        L = e.box()[0]
+        e.close()
+        H = np.array(H)
+        t = np.arange(len(H))
+        drift = np.polyval(np.polyfit(t, H, 1), [0, len(H) - 1])
+        return H.std(), abs(drift[1] - drift[0]), L
+
+    s2, d2, L2 = series(0.002, 22)
+    s1, d1, L1 = series(0.001, 22)
+    sn, dn, _ = series(0.002, 2)
+    print("En.tot std %.1f K at 2 fs, %.1f K at 1 fs (ratio %.2f), drift over 2 ps %.1f / %.1f K; "
+          "Nose-Hoover std %.1f K; L %.4f / %.4f" % (s2, s1, s2 / s1, d2, d1, sn, L2, L1), flush=True)
+    assert 3.0 < s2 / s1 < 5.5
+    assert d2 < 2 * s2 and d1 < 2 * s1
+    assert 0.5 < s2 / sn < 2
+    assert abs(L2 / L1 - 1) < 0.02
