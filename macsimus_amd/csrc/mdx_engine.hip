@@ -366,17 +366,59 @@ __global__ void k_cook_cellid(const double *__restrict__ r, int ns, CookGrid g,
 template <typename R>
 struct PairTab {
   const R *lj;    // [nt*nt*8] Sq E4 E24 E48 C1q C2q A A4
+  bool ljlds = false;  // lj points into LDS (a staged copy)
   const R *erf;   // [nerf*8] f0u bu Du f0d bd Dd (stable form) or Au Bu Cu Ad Bd Cd
   // FP32 force splines staged in LDS (structure of arrays f0d, bd, Dd of
   // the first nsl intervals), read in place of erf when only forces are
   // wanted; NULL: erf only
   const float *sl = nullptr;
   int nsl = 0;
+  // k_pairc: the number of lj values it copies to LDS first (0: none)
+  int ljstage = 0;
   double isgrid = 0;   // 1/sgrid
   // POLAR&1 shell-core: per type qkappa and rep flag; NULL: none
   const R *shk = nullptr;
   const int *shrep = nullptr;
 };
+
+typedef __attribute__((address_space(3))) const float lds_float;
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const vf4 lds_vf4;
+typedef __attribute__((address_space(3))) const vd2 lds_vd2;
+
+// the 8 constants of a type pair's table row, read at once (16-byte loads)
+// from global memory or from a copy staged in LDS (lds: a generic pointer
+// to LDS), so that the branches of the pair terms wait on no further loads
+template <typename R>
+struct Row {
+  R v[8];
+  __device__ R operator[](int k) const { return v[k]; }
+};
+
+__device__ inline Row<float> load_row(const float *p, bool lds) {
+  vf4 a, b;
+  if (lds) {
+    a = ((const lds_vf4 *)p)[0];
+    b = ((const lds_vf4 *)p)[1];
+  } else {
+    a = ((const vf4 *)p)[0];
+    b = ((const vf4 *)p)[1];
+  }
+  return Row<float>{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
+}
+
+__device__ inline Row<double> load_row(const double *p, bool lds) {
+  vd2 a, b, c, d;
+  if (lds) {
+    const lds_vd2 *q = (const lds_vd2 *)p;
+    a = q[0]; b = q[1]; c = q[2]; d = q[3];
+  } else {
+    const vd2 *q = (const vd2 *)p;
+    a = q[0]; b = q[1]; c = q[2]; d = q[3];
+  }
+  return Row<double>{{a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y}};
+}
 
 // eru/erd of the hyperbolic spline.  FP64: cook's form B/(C+x)+A
 // (sim/elst.h:122-130).  FP32: the same function written around the
@@ -405,7 +447,9 @@ __device__ inline void spline<float>(const PairTab<float> &tab, double rr, doubl
   // quotient; t only enters the FP32 form)
   const float t = (float)(rr - i * tab.isgrid);
   if (!want_u && tab.sl && i < tab.nsl) {
-    const float f0 = tab.sl[i], b = tab.sl[tab.nsl + i], D = tab.sl[2 * tab.nsl + i];
+    // tab.sl is a generic pointer to LDS: read it as such (ds_read, not flat)
+    const lds_float *sl = (const lds_float *)tab.sl;
+    const float f0 = sl[i], b = sl[tab.nsl + i], D = sl[2 * tab.nsl + i];
     erd = f0 - b * t * rcp32(D + t);
     return;
   }
@@ -444,7 +488,7 @@ __device__ inline void qq_term(const PairTab<R> &tab, double sgrid, R qq, const 
 // e^(tau r), frep = -urep'(r)/r = ef e^(tau r)/r (SS_MEASURE_rep of
 // sim/buck/sitesite.h:145-157 and sim/busing/sitesite.h)
 template <typename R>
-__device__ inline void exp_rep(const R *ss, double rr, R &urep, R &frep) {
+__device__ inline void exp_rep(const Row<R> &ss, double rr, R &urep, R &frep) {
   const R x = sqrt((R)rr), y = exp(ss[0] * x);
   urep = ss[1] * y;
   frep = ss[2] * y * rcp<R>(x);
@@ -455,7 +499,7 @@ __device__ inline void exp_rep(const R *ss, double rr, R &urep, R &frep) {
 // -kappa urep, force -kappa frep d, inside C2 of the pair's table row
 // (cook/ljqqpol.c:561-592, 616-648)
 template <typename R, bool MEASURE>
-__device__ inline void shell_term(const R *ss, R kappa, const double (&d)[3], R (&acc)[3],
+__device__ inline void shell_term(const Row<R> &ss, R kappa, const double (&d)[3], R (&acc)[3],
                                   double &U, double &Vir, double (&pv)[6]) {
   const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   if (rr >= (double)ss[5]) return;
@@ -491,7 +535,7 @@ __device__ inline void pair_body(const int4 ii, const R *ljrow, R qi, R qpi, con
                                  double &Eel, double &Vir, double (&pv)[6], double &nlj) {
   if (jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1)) return;  // EXCL
   const int tj = jj.x & 0xffff;
-  const R *ss = ljrow + tj * 8;
+  const Row<R> ss = load_row(ljrow + tj * 8, tab.ljlds);
   R fr = 0, u = 0;
   // POLAR&1 shell-core (cook/ljqqpol.c:41-46): the central site's shell
   // against a repelling partner (an1), and the reverse (an2)
@@ -694,6 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(
 constexpr int kPairBlock = 512;
 constexpr int kMaxJ = 768;
 constexpr int kMaxSplineLds = 48 * 1024;   // bytes of force splines staged in LDS
+constexpr int kMaxLjLds = 4 * 1024;        // bytes of the type-pair table staged in LDS
 constexpr int kMaxZ = 64;      // buffer cells of a round (z cells + 2w)
 
 template <typename R, bool MEASURE, bool POLAR, bool E6>
@@ -711,7 +756,17 @@ __global__ __launch_bounds__(kPairBlock) void k_pairc(
   __shared__ int joff[kMaxZ + 1];
   __shared__ int jsrc[kMaxZ];
   __shared__ double jsz[kMaxZ];
-  extern __shared__ float lsp[];       // force splines (nsl intervals x 3), see PairTab
+  // dynamic LDS: the type-pair table (tab.ljstage values, 16-byte
+  // multiple), then the force splines (nsl intervals x 3), see PairTab
+  extern __shared__ __align__(16) unsigned char dyn[];
+  const int ljbytes = (tab.ljstage * (int)sizeof(R) + 15) & ~15;
+  if (tab.ljstage) {
+    R *l = (R *)dyn;
+    for (int i = threadIdx.x; i < tab.ljstage; i += kPairBlock) l[i] = tab.lj[i];
+    tab.lj = l;
+    tab.ljlds = true;
+  }
+  float *lsp = (float *)(dyn + ljbytes);
   if (tab.nsl) {
     for (int i = threadIdx.x; i < tab.nsl; i += kPairBlock) {
       const float *e = (const float *)tab.erf + 8 * i;
@@ -1278,11 +1333,17 @@ int launch_pair_t(Ctx &c, int measure) {
       const dim3 gc(c.nc[0] * c.nc[1], np);
       const double band = make_cook_grid(c).band;
       size_t shm = 0;
+      // the type-pair table in LDS where it is small (it is read once per pair)
+      const int ljn = c.ntypes * c.ntypes * 8;
+      if (ljn * (int)sizeof(R) <= kMaxLjLds) {
+        tab.ljstage = ljn;
+        shm = (size_t)((ljn * (int)sizeof(R) + 15) & ~15);
+      }
       if constexpr (sizeof(R) == 4) {
         const int nsl = std::min(c.nerf, (int)((cq + band) * c.erf_sgrid) + 2);
         if (!measure && !POLAR && nsl * 12 <= kMaxSplineLds) {
           tab.nsl = nsl;
-          shm = (size_t)nsl * 12;
+          shm += (size_t)nsl * 12;
         }
       }
       if (measure)
