@@ -25,10 +25,11 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-PEAK = {"fp32_valu": 157.3e12, "fp64_valu": 78.6e12, "fp32_mfma": 157.3e12, "hbm": 8.0e12}
+PEAK = {"fp32_valu": 157.3e12, "fp64_valu": 78.6e12, "fp32_mfma": 157.3e12,
+        "fp16_mfma": 2.5e15, "hbm": 8.0e12}
 # stencil w, stencil columns per pair part, kz chunk, k parts, matrix-core pass 2
 # (0: FMA chains, 1 or 2: charge tiles of 32 per wave)
-PROFILED_PIN = {"mixed": "2,3,24,16,2", "fp64": "2,3,8,64,0"}
+PROFILED_PIN = {"mixed": "2,3,32,16,1", "fp64": "2,3,8,64,0"}
 
 
 def parse():
@@ -55,6 +56,9 @@ def parse():
     ap.add_argument("--kpass1", type=int, default=-1, choices=[-1, 0, 1, 2, 8, 16],
                     help="mixed mode: k-space pass 1 as FMA chains (0) or on the matrix cores "
                          "with 16 or 8 tiles per wave (-1: as pinned or tuned)")
+    ap.add_argument("--kprod", type=int, default=-1, choices=[-1, 0, 3, 6],
+                    help="mixed mode: pass 2's products of FP32 operands (0) or of split-FP16 "
+                         "operands, 3 or 6 products (-1: the engine's default, 6)")
     ap.add_argument("--tune", action="store_true",
                     help="c2: run the autotuner instead of the profiled configuration")
     ap.add_argument("--no-overlap", action="store_true",
@@ -106,6 +110,8 @@ def traffic_lookup(workload, mode, kernel, info):
                 info["stencil"], info.get("pair_cols", 0)):
             continue
         if kernel == "kpass2" and int(e.get("kspace_pass2_mfma", 0)) != int(info.get("kmfma", 0)):
+            continue
+        if kernel == "kpass2" and int(e.get("kspace_pass2_products", 0)) != int(info.get("kprod", 0)):
             continue
         if kernel == "kpass1" and int(e.get("kspace_pass1_tiles", 0)) != int(info.get("kpass1_tiles", 0)):
             continue
@@ -310,6 +316,8 @@ def main():
         tune = eng.autotune(wl["r_full"], reps=2)[:4]
     if a.kpass1 >= 0 and mode == abi.MDX_MIXED:
         eng.set_kspace_pass1(a.kpass1)
+    if a.kprod >= 0 and mode == abi.MDX_MIXED:
+        eng.set_kspace_products(a.kprod)
     if a.no_overlap:
         eng.set_overlap(False)
     info = eng.info()
@@ -379,16 +387,23 @@ def main():
     force_ms = sum(kern[k][0] for k in ("bin", "pair", "excl", "kpass1", "kcoef", "kpass2") if k in kern)
 
     # every hot kernel against its own roof: FLOP/s for the pair kernel
-    # and both k-space passes (matrix cores where they run there), HBM
-    # bytes/s for binning (44 B per site + 8 per cell) and the SHAKE step
-    # (96 B per site: r, p, r1 read, r, r1 written)
+    # and both k-space passes (matrix cores where they run there; with
+    # split-FP16 operands each FP32 product costs kprod FP16 products, so
+    # the roof is the FP16 matrix peak / kprod), HBM bytes/s for binning
+    # (44 B per site + 8 per cell) and the SHAKE step (96 B per site: r, p,
+    # r1 read, r, r1 written)
     ncell = info["ncx"] * info["ncy"] * info["ncz"]
     mfma = {"kpass1": bool(info["kpass1_tiles"]), "kpass2": bool(info["kmfma"])}
+
+    def mfma_peak(k):
+        if k == "kpass2" and info["kprod"]:
+            return PEAK["fp16_mfma"] / info["kprod"]
+        return PEAK["fp32_mfma"]
     rooflines = {}
     for k in ("pair", "kpass1", "kpass2"):
         if k not in kern:
             continue
-        pk = PEAK["fp32_mfma"] if (mode == abi.MDX_MIXED and mfma.get(k)) else peak
+        pk = mfma_peak(k) if (mode == abi.MDX_MIXED and mfma.get(k)) else peak
         ach = work[k] / (kern[k][0] * 1e-3)
         rooflines[k] = {"bound": "mfma" if (mode == abi.MDX_MIXED and mfma.get(k)) else "valu",
                         "ms": kern[k][0], "achieved": ach / 1e12, "peak": pk / 1e12,
@@ -463,13 +478,13 @@ def main():
                    "timestep_fs": h * 1e3, "precision": a.mode, "parallelism": "1 GPU",
                    "tuned": {"stencil_w": tune[0], "kz_chunk": tune[1], "k_parts": tune[2],
                              "pair_cols_per_part": tune[3], "kspace_pass2_mfma": bool(info["kmfma"]),
+                             "kspace_pass2_products": info["kprod"],
                              "kspace_pass1_tiles": info["kpass1_tiles"]}},
         "us_per_force": force_ms * 1e3,
         "kernel_ms": {k: v[0] for k, v in kern.items()},
-        "roofline": {"bound": "mfma" if ((dom == "kpass2" and info["kmfma"]) or
-                                         (dom == "kpass1" and info["kpass1_tiles"])) else "valu",
-                     "kernel": dom, "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak,
+        "roofline": {"bound": rooflines[dom]["bound"],
+                     "kernel": dom, "achieved": achieved / 1e12, "peak": rooflines[dom]["peak"],
+                     "unit": "TFLOP/s", "frac": rooflines[dom]["frac"],
                      "traffic": traffic_lookup(wl["name"], a.mode, dom, info),
                      "work_per_launch": work[dom]},
         "rooflines": rooflines,

@@ -389,11 +389,12 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    mdx_info: iv[0..n) = engine cells x,y,z, stencil w, charges, k rows,
    kz blocks, k segments, k triples, k-sum splits, kx max, mode, kz chunk,
    kz chunks, parts asked, parts made, stencil columns per pair part,
-   matrix-core pass 2 (0/1), pass-1 tiles (0 = FMA; n may be 19).
+   matrix-core pass 2 (0/1/2), pass-1 tiles (0 = FMA), pass-2 products
+   (0/3/6) (n may be 20).
    mdx_set_kspace_mfma (MDX_MIXED): k-space pass 2 on the matrix cores
    (FP32 MFMA, FP32 products and sums as the FMA variant) with on = 1 or
    2 tiles of 32 charges per wave (2 share each coefficient operand), or
-   as FP32 FMA chains (on = 0); mdx_autotune tries all three.  Default: 2
+   as FP32 FMA chains (on = 0); mdx_autotune tries all three.  Default: 1
    in MDX_MIXED.
    mdx_set_kspace_pass1 (MDX_MIXED): k-space pass 1 as FP32 FMA chains
    (tiles = 0), on the matrix cores as units of 8 k rows with all their
@@ -403,8 +404,18 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    (tiles = 2, the default where kz stays below 64), or on the matrix cores
    with 16 or 8 tiles of 4 k rows and one 8-kz block per wave (FP64 flush
    every 32 charges).
-   mdx_autotune keeps the fastest. */
+   mdx_autotune keeps the fastest.
+   mdx_set_kspace_products (MDX_MIXED, with the matrix-core pass 2): np =
+   0 forms pass 2's products of FP32 operands on the FP32 matrix cores;
+   np = 3 writes each FP32 operand as the sum of two FP16 parts (22 of its
+   24 bits) and forms the three products of parts of order <= 1 on the
+   FP16 matrix cores with FP32 sums (terms left out: 2^-22 relative);
+   np = 6 uses three parts (all 24 bits) and the six products of order <=
+   2 (terms left out: 2^-33).  The operands are scaled by powers of two
+   (per 4-row group for the coefficients) to stay in FP16's normal range.
+   The tuner leaves this choice alone.  Default: 6 in MDX_MIXED. */
 int mdx_set_kspace_mfma(mdx_ctx_t *ctx, int on);
+int mdx_set_kspace_products(mdx_ctx_t *ctx, int np);
 int mdx_set_kspace_pass1(mdx_ctx_t *ctx, int tiles);
 int mdx_cook_cells(mdx_ctx_t *ctx, const double *r, int *cellid);
 int mdx_set_timing(mdx_ctx_t *ctx, int on);

@@ -1583,7 +1583,10 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     return code;
   };
   c.mode = mode;
-  c.kmfma = mode == MDX_MIXED ? 2 : 0;   // pass 2 on the matrix cores, 64 charges per wave
+  // MDX_MIXED: pass 2 on the matrix cores, 32 charges per wave, with
+  // split-FP16 operands of all 24 bits (six products)
+  c.kmfma = mode == MDX_MIXED ? 1 : 0;
+  c.kprod = mode == MDX_MIXED ? 6 : 0;
   (void)hipGetDevice(&c.device);
   // set-up of this device: the address of the no-allocation adjoint's scratch
   if ((rc = adjoint_scratch_setup(c.device, nullptr))) return bail(rc);
@@ -1767,6 +1770,10 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.d_grp_start.release();
   c.d_grp_base.release();
   c.d_grp_nb.release();
+  c.coefh.release();
+  c.gscale.release();
+  c.d_grp_hb.release();
+  c.d_grp_nb8.release();
   c.rdf_hist.release();
   c.qanc.release();
   c.qadj_tab.release();
@@ -1837,6 +1844,7 @@ extern "C" int mdx_resize(mdx_ctx_t *h, const mdx_system_t *s) {
   // that a failing setter leaves the old system in place
   Ctx &n = t->c;
   n.kmfma = c.kmfma;
+  n.kprod = c.kprod;
   n.p1nt = c.p1nt;                     // same precision, so still valid
   n.overlap = c.overlap;
   n.use_graph = c.use_graph;
@@ -2023,6 +2031,15 @@ extern "C" int mdx_set_kspace_mfma(mdx_ctx_t *h, int on) {
   return 0;
 }
 
+extern "C" int mdx_set_kspace_products(mdx_ctx_t *h, int np) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_products: NULL");
+  if (np != 0 && np != 3 && np != 6) return fail(MDX_E_ARG, "mdx_set_kspace_products: 0, 3 or 6");
+  if (np && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_products: MDX_MIXED only");
+  h->c.kprod = np;
+  h->c.gen++;
+  return 0;
+}
+
 extern "C" int mdx_set_rdf(mdx_ctx_t *h, int grid, double cutoff) {
   if (!h) return fail(MDX_E_ARG, "mdx_set_rdf: NULL");
   Ctx &c = h->c;
@@ -2074,9 +2091,9 @@ extern "C" int mdx_get_rdf(mdx_ctx_t *h, unsigned long long *hist, long n, int *
 extern "C" int mdx_info(mdx_ctx_t *h, int *iv, int n) {
   if (!h || !iv) return fail(MDX_E_ARG, "mdx_info: NULL");
   Ctx &c = h->c;
-  int v[19] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
+  int v[20] = {c.nc[0], c.nc[1], c.nc[2], c.allpair ? 0 : c.wst, c.nq, c.K.nrows, c.K.nblk, c.K.nseg,
                c.K.ntrip, c.nsplit, c.K.kxmax, c.mode, c.kzc, c.nchunk, c.kparts, c.nkpart,
-               c.pcols, c.kmfma, c.p1nt};
-  for (int i = 0; i < n && i < 19; i++) iv[i] = v[i];
+               c.pcols, c.kmfma, c.p1nt, c.kprod};
+  for (int i = 0; i < n && i < 20; i++) iv[i] = v[i];
   return 0;
 }

@@ -150,6 +150,10 @@ struct KLayout {
   // and its 4-kz blocks (those of its first, longest row)
   std::vector<int> grp_start, grp_base, grp_nb;  // [kxmax+2], [ngrp], [ngrp]
   long grp_floats = 0;
+  // the same groups in 8-kz blocks for the split-FP16 pass 2: per group
+  // its first block slot and its number of blocks (ceil of 4 grp_nb / 8)
+  std::vector<int> grp_hb, grp_nb8;
+  long grp_h8 = 0;                     // block slots of all groups
 };
 
 struct Ctx {
@@ -250,6 +254,15 @@ struct Ctx {
   int p1nt = 0;                 // MDX_MIXED pass 1: FP32 FMA (0) or matrix cores, 16 or 8 tiles per wave
   DBuf<int> d_grp_start, d_grp_base, d_grp_nb;
   DBuf<float> coefg;
+  // products of the matrix-core pass 2: 0 FP32 operands; 3 (6): each FP32
+  // operand as the sum of 2 (3) FP16 parts, scaled by a power of two per
+  // row group (coefficients) and 2^8 (phases), and the products of parts
+  // whose orders add up to at most 1 (2) formed on the FP16 matrix cores
+  // with FP32 accumulation (coefh [slot][part][64 lanes][8], gscale)
+  int kprod = 0;
+  DBuf<uint16_t> coefh;
+  DBuf<float> gscale;
+  DBuf<int> d_grp_hb, d_grp_nb8;
   DBuf<int> d_kparts;
   DBuf<double> kpf;
   DBuf<double> kpart;          // pass-1 partials [nsplit][nseg][64][64]

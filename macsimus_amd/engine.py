@@ -159,11 +159,11 @@ class Engine:
         return cid
 
     def info(self):
-        iv = (C.c_int * 19)()
-        self.lib.mdx_info(self.ctx, iv, 19)
+        iv = (C.c_int * 20)()
+        self.lib.mdx_info(self.ctx, iv, 20)
         keys = ("ncx", "ncy", "ncz", "stencil", "nq", "krows", "kblocks", "ksegments",
                 "ktriples", "ksplits", "kxmax", "mode", "kchunk", "nkchunks", "kparts_asked",
-                "kparts", "pair_cols", "kmfma", "kpass1_tiles")
+                "kparts", "pair_cols", "kmfma", "kpass1_tiles", "kprod")
         return dict(zip(keys, list(iv)))
 
     def set_kchunk(self, kzc, parts=16):
@@ -278,6 +278,11 @@ class Engine:
     def set_kspace_mfma(self, on=True):
         rc = self.lib.mdx_set_kspace_mfma(self.ctx, int(on))
         abi.check(self.lib, rc, "mdx_set_kspace_mfma")
+
+    def set_kspace_products(self, np_=0):
+        """pass 2's products: 0 FP32 operands, 3 or 6 split-FP16 products"""
+        rc = self.lib.mdx_set_kspace_products(self.ctx, int(np_))
+        abi.check(self.lib, rc, "mdx_set_kspace_products")
 
     def set_kspace_pass1(self, tiles=0):
         rc = self.lib.mdx_set_kspace_pass1(self.ctx, int(tiles))
@@ -481,6 +486,7 @@ def declare_extras(lib):
     lib.mdx_set_overlap.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_kspace_mfma.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_kspace_pass1.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_kspace_products.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4

@@ -136,27 +136,37 @@ def test_kspace_pass2_on_matrix_cores(name):
     d, meta = oracle.load_fixture(name)
     bs = oracle.fixture_system(meta, provider_=engine_provider())
     eng = Engine(bs, abi.MDX_MIXED)
-    for kzc in (8, 16, 24, 32):
+    assert eng.info()["kmfma"] == 1 and eng.info()["kprod"] == 6     # the defaults
+    for kzc in (8, 16, 24, 32, 48, 64):
         for parts in (1, 5, 16):
             eng.set_kchunk(kzc, parts)
             eng.set_kspace_mfma(0)
             f0, o0 = eng.forces(d["r"], 1)
-            for tiles in (1, 2):
-                # 1 or 2 tiles of 32 charges per wave: the same products
-                eng.set_kspace_mfma(tiles)
-                assert eng.info()["kmfma"] == tiles
-                f1, o1 = eng.forces(d["r"], 1)
-                e, d01 = rel_rms(f1, d["f"]), rel_rms(f1, f0)
-                print(name, kzc, parts, tiles, "mfma vs cook %.2e, vs fma %.2e" % (e, d01), flush=True)
-                assert e < TOL[abi.MDX_MIXED] and d01 < 2e-6
-                assert o1.En_el == o0.En_el and o1.npair == o0.npair
-                if tiles == 1:
-                    f_one = f1
-                else:
-                    assert np.array_equal(f1, f_one)
+            # FP32 operands, split-FP16 operands with 3 or 6 products
+            for kp in (0, 3, 6):
+                eng.set_kspace_products(kp)
+                for tiles in (1, 2):
+                    # 1 or 2 tiles of 32 charges per wave: the same products
+                    eng.set_kspace_mfma(tiles)
+                    assert eng.info()["kmfma"] == tiles and eng.info()["kprod"] == kp
+                    f1, o1 = eng.forces(d["r"], 1)
+                    e, d01 = rel_rms(f1, d["f"]), rel_rms(f1, f0)
+                    print(name, kzc, parts, kp, tiles, "mfma vs cook %.2e, vs fma %.2e" % (e, d01),
+                          flush=True)
+                    assert e < TOL[abi.MDX_MIXED] and d01 < 2e-6
+                    assert o1.En_el == o0.En_el and o1.npair == o0.npair
+                    if tiles == 1:
+                        f_one = f1
+                    else:
+                        assert np.array_equal(f1, f_one)
     e64 = Engine(bs, abi.MDX_FP64)
     with pytest.raises(RuntimeError):
         e64.set_kspace_mfma(True)
+    with pytest.raises(RuntimeError):
+        e64.set_kspace_products(6)
+    with pytest.raises(RuntimeError):
+        eng.set_kspace_products(4)
+    e64.close()
     eng.close()
     e64.close()
 
