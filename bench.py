@@ -53,9 +53,10 @@ def parse():
                     help="W,PCOLS,KZC,KPARTS[,MFMA]: stencil, stencil columns per pair part, "
                          "k-space kz chunk and parts, pass 2 on the matrix cores (mixed) instead "
                          "of the autotuner (c4: W,KZC,KPARTS)")
-    ap.add_argument("--kpass1", type=int, default=-1, choices=[-1, 0, 1, 2, 8, 16],
-                    help="mixed mode: k-space pass 1 as FMA chains (0) or on the matrix cores "
-                         "with 16 or 8 tiles per wave (-1: as pinned or tuned)")
+    ap.add_argument("--kpass1", type=int, default=-1, choices=[-1, 0, 1, 2, 4, 5, 6, 7, 8, 16],
+                    help="mixed mode: k-space pass 1 as FMA chains (0), on the FP32 matrix cores "
+                         "(1, 2: row units; 16, 8: tiles per wave) or on the FP16 matrix cores with "
+                         "split operands (4..7: row-unit shapes) (-1: the engine's default)")
     ap.add_argument("--kprod", type=int, default=-1, choices=[-1, 0, 3, 6],
                     help="mixed mode: pass 2's products of FP32 operands (0) or of split-FP16 "
                          "operands, 3 or 6 products (-1: the engine's default, 6)")
@@ -398,6 +399,8 @@ def main():
     def mfma_peak(k):
         if k == "kpass2" and info["kprod"]:
             return PEAK["fp16_mfma"] / info["kprod"]
+        if k == "kpass1" and 4 <= info["kpass1_tiles"] <= 7:
+            return PEAK["fp16_mfma"] / (3 if info["kprod"] == 3 else 6)
         return PEAK["fp32_mfma"]
     rooflines = {}
     for k in ("pair", "kpass1", "kpass2"):

@@ -401,9 +401,16 @@ int mdx_get_rdf(mdx_ctx_t *ctx, unsigned long long *hist, long n, int *nrec, int
    kz per wave (tiles = 1: FP32 MFMA 16x16x4, sums added to FP32 totals
    every 128 charges and to FP64 per charge split), the same units with
    two waves each, which share the row operands and split the kz blocks
-   (tiles = 2, the default where kz stays below 64), or on the matrix cores
-   with 16 or 8 tiles of 4 k rows and one 8-kz block per wave (FP64 flush
-   every 32 charges).
+   (tiles = 2), on the FP16 matrix cores with split-FP16 operands (the
+   products of mdx_set_kspace_products: 3, or 6 when it is 0 or 6) as row
+   units (tiles = 4..7: units of 8 rows, 8 per workgroup, 64
+   charges per table chunk (4); units of 4 rows, 16 per workgroup, 64
+   charges (5, the default where kz stays below 64 and the system has no
+   Drudes; with Drudes the default is 2); units of 4 rows, 16
+   per workgroup, two table sets of 32 charges (6); units of 8 rows, 8 per
+   workgroup, two sets of 32 charges (7)), or on the FP32 matrix cores with
+   16 or 8 tiles of 4 k rows and one 8-kz block per wave (FP64 flush every
+   32 charges).
    mdx_autotune keeps the fastest.
    mdx_set_kspace_products (MDX_MIXED, with the matrix-core pass 2): np =
    0 forms pass 2's products of FP32 operands on the FP32 matrix cores;

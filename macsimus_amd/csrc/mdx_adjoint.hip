@@ -215,9 +215,11 @@ extern "C" int mdx_set_params(mdx_ctx_t *h, const mdx_lj_t *sstab, const double 
     // through charged states)
     c.sumq2 = 0;
     c.qnet = 0;
+    c.qmax = 0;
     for (int i = 0; i < c.ns; i++) {
       c.sumq2 += (q[i] + qp[i]) * (q[i] + qp[i]);
       c.qnet += q[i] + qp[i];
+      c.qmax = std::max(c.qmax, std::max(std::fabs(q[i]), std::fabs(qp[i])));
     }
     c.bgE = -kPiA / 2 * (c.qnet / c.alpha) * (c.qnet / c.alpha);
     c.Ediag = c.nq ? ewald_ediag(c.L, c.kappa, c.alpha, c.sumq2) : 0;

@@ -1628,6 +1628,7 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
     // self term: squared total site charge (sim/ewald.c:402-424)
     c.sumq2 += (s->charge[i] + qp) * (s->charge[i] + qp);
     c.qnet += s->charge[i] + qp;
+    c.qmax = std::max(c.qmax, std::max(std::fabs(s->charge[i]), std::fabs(qp)));
   }
   c.nq = (int)qsite.size();
   // Ewald corrections: the background energy of a net charge
@@ -1737,8 +1738,12 @@ extern "C" int mdx_create(mdx_ctx_t **out, const mdx_system_t *s, int mode) {
   if ((rc = choose_grid(c, 2)) || (rc = alloc_grid(c))) return bail(rc);
   if (c.nq && (rc = kspace_setup(c))) return bail(rc);
   // mixed mode: pass 1 on the matrix cores as row units where the k-set
-  // allows it (kz below 64)
-  if (mode == MDX_MIXED && c.nq && c.K.nblk <= 8) c.p1nt = 2;
+  // allows it (kz below 64), with split-FP16 operands except for Drude
+  // systems: there a centre and its Drude, next to each other in the
+  // charge list, cancel inside one instruction's sum, and the FP16
+  // instruction sums 32 products less exactly than the FP32 one sums 4
+  // (measured: 2x the force error of the FP32 matrix cores)
+  if (mode == MDX_MIXED && c.nq && c.K.nblk <= 8) c.p1nt = c.polar ? 2 : 5;
   *out = h;
   return 0;
 }
@@ -1995,8 +2000,8 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
       // (1, 2) or matrix cores with 16 or 8 tiles per wave
       int best_nt = c.p1nt;
       double t1 = 1e30;
-      for (int nt : {0, 1, 2, 16, 8}) {
-        if ((nt == 1 || nt == 2) && c.K.nblk > 8) continue;
+      for (int nt : {0, 1, 2, 4, 5, 6, 7, 16, 8}) {
+        if ((nt == 1 || nt == 2 || (nt >= 4 && nt <= 7)) && c.K.nblk > 8) continue;
         c.p1nt = nt;
         if ((rc = run(fam))) break;
         if (fam[T_KPASS1] < t1) { t1 = fam[T_KPASS1]; best_nt = nt; }
@@ -2014,8 +2019,9 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
 
 extern "C" int mdx_set_kspace_pass1(mdx_ctx_t *h, int tiles) {
   if (!h) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: NULL");
-  if (tiles != 0 && tiles != 1 && tiles != 2 && tiles != 8 && tiles != 16)
-    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0, 1, 2, 8 or 16");
+  if (!(tiles >= 0 && tiles <= 8) && tiles != 16)
+    return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0..8 or 16");
+  if (tiles == 3) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: tiles must be 0..8 or 16");
   if (tiles && h->c.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_set_kspace_pass1: MDX_MIXED only");
   h->c.p1nt = tiles;
   h->c.gen++;

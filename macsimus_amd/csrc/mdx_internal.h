@@ -171,6 +171,7 @@ struct Ctx {
   double L[3] = {0, 0, 0};
   double cutoff = 0, alpha = 0, alphar = 0, kappa = 0, epsinf = 3e33;
   double sumq2 = 0;
+  double qmax = 0;                     // largest |charge| (centres, Drudes)
   // Ewald corrections (sim/ewald.c:477-492, 865-997): el.corr, el.bg,
   // el.diag, the net charge, the background energy -pi/2 (Q/alpha)^2
   // (divided by V at each call) and Ediag of the box at set-up

@@ -499,6 +499,245 @@ __global__ __launch_bounds__(kBlock * WPU, WPU == 1 ? 3 : 4) void k_sfac_mf(
 #undef MDX_U
 }
 
+// Pass 1 with split-FP16 operands (split16 below): the GEMM of k_sfac_mf
+// on the FP16 instruction v_mfma_f32_16x16x32_f16 (K = 32 charges), one
+// wave per unit of 8 rows with all its N tiles, U units per workgroup.
+// Per chunk of JC charges the workgroup builds the cos and sin rows of X
+// and Y in FP32 (rows [2k + p][charge], q folded into X) and those of Z as
+// NPART FP16 parts scaled by 2^14 ([part][2 kz + r][charge]); a wave then
+// forms its A operands, 8 charges of q X_p Y_q per lane for each M tile,
+// scaled by asc and split, and reads its B operands from the Z parts.
+// Lane l of the instruction holds A[l%16][8(l/16)..+8) and B[8(l/16)..+8)
+// [l%16] (the same 8 charges for A and B) and receives D[4(l/16) + r]
+// [l%16] as k_sfac_mf.  Sums: FP32 per 64 charges, FP32 second level per
+// 128, FP64 per split, scaled back by 2^-14/asc.
+template <int NPART>
+__device__ inline void split16(float x, _Float16 (&p)[NPART]);
+
+template <int JC>
+struct SfTabH {
+  static constexpr int SX = JC + 8;     // floats per X / Y row
+  static constexpr int SZ = JC + 8;     // halves per Z row
+};
+
+template <int JC, int BLK, int NPART>
+__device__ inline void sf_tables_h(float *Xf, float *Yf, _Float16 *Zh, int KXT, int KYT,
+                                   int KZT, int jc, int j1, int nq,
+                                   const float2 *__restrict__ qanc,
+                                   const float2 *__restrict__ qbase) {
+  constexpr int SX = SfTabH<JC>::SX, SZ = SfTabH<JC>::SZ;
+  const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
+  for (int t = threadIdx.x; t < JC * ntask; t += BLK) {
+    const int jj = t % JC, task = t / JC;
+    int d, m, KT;
+    if (task < ntx) { d = 0; m = task; KT = KXT; }
+    else if (task < ntx + nty) { d = 1; m = task - ntx; KT = KYT; }
+    else { d = 2; m = task - ntx - nty; KT = KZT; }
+    const int j = jc + jj;
+    const int k1 = min(8 * m + 8, KT);
+    float ec = 0, es = 0, bc = 0, bs = 0;
+    if (j < j1) {
+      const float2 a = qanc[(size_t)task * nq + j], b = qbase[(size_t)d * nq + j];
+      ec = a.x; es = a.y; bc = b.x; bs = b.y;
+    }
+    for (int k = 8 * m; k < k1; k++) {
+      if (d < 2) {
+        float *dst = (d == 0 ? Xf : Yf) + 2 * k * SX + jj;
+        dst[0] = ec;
+        dst[SX] = es;
+      } else {
+        _Float16 pc[NPART], ps[NPART];
+        split16<NPART>(ec * 16384.0f, pc);
+        split16<NPART>(es * 16384.0f, ps);
+#pragma unroll
+        for (int q = 0; q < NPART; q++) {
+          Zh[(q * KZT + k) * 2 * SZ + jj] = pc[q];
+          Zh[((q * KZT + k) * 2 + 1) * SZ + jj] = ps[q];
+        }
+      }
+      const float nc = ec * bc - es * bs;
+      es = ec * bs + es * bc;
+      ec = nc;
+    }
+  }
+}
+
+typedef _Float16 half8p1 __attribute__((ext_vector_type(8)));
+
+template <int NA, int MT, int JC, int BLK, int NPART, bool DB>
+__device__ inline void sf_unit_h(char *smem, size_t set_bytes, int KXT, int KYT, int KZT, int j0,
+                                 int j1, int r0, int nrows, const int *__restrict__ row_kx,
+                                 const int *__restrict__ row_ky, const int *__restrict__ blk_rows,
+                                 const int *__restrict__ blk_seg0, int nseg,
+                                 double *__restrict__ kpart, int nq,
+                                 const float2 *__restrict__ qanc,
+                                 const float2 *__restrict__ qbase, float asc) {
+  constexpr int SX = SfTabH<JC>::SX, SZ = SfTabH<JC>::SZ;
+  const int lane = threadIdx.x & 63, i16 = lane & 15, g = lane >> 4;
+  // A of M tile m: row r0 + 4m + i16/4, X_p by bit 1 of i16, Y_q by bit 0;
+  // B of N tile t: kz 8t + i16/2, cos/sin by bit 0; this lane's 8 charges
+  // 8g..8g+7 of each 32
+  int xo[MT], yo[MT];
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int row = r0 + 4 * m + (i16 >> 2);
+    int kx = 0, ky = 0;
+    if (row < nrows) { kx = row_kx[row]; ky = row_ky[row]; }
+    xo[m] = (2 * kx + ((i16 >> 1) & 1)) * SX + 8 * g;
+    yo[m] = (2 * ky + (i16 & 1)) * SX + 8 * g;
+  }
+  const int zo = (2 * (i16 >> 1) + (i16 & 1)) * SZ + 8 * g;
+  typedef float mf4 __attribute__((ext_vector_type(4)));
+  mf4 acc[MT][NA > 0 ? NA : 1], acch[MT][NA > 0 ? NA : 1];
+  float mid[MT][NA > 0 ? NA : 1][4];
+#pragma unroll
+  for (int m = 0; m < MT; m++)
+#pragma unroll
+    for (int t = 0; t < NA; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) { acc[m][t][r] = 0.0f; acch[m][t][r] = 0.0f; mid[m][t][r] = 0; }
+  const float unsc = 1.0f / (asc * 16384.0f);
+  // table set k (DB: two sets, the next chunk's built while this one's
+  // products run, one barrier per chunk)
+  auto tabs = [&](int k, float *&X, float *&Y, _Float16 *&Z) {
+    X = (float *)(smem + (DB ? k : 0) * set_bytes);
+    Y = X + 2 * KXT * SX;
+    Z = (_Float16 *)(Y + 2 * KYT * SX);
+  };
+  int nchunk = 0, set = 0;
+  if (DB && j0 < j1) {
+    float *X, *Y;
+    _Float16 *Z;
+    tabs(0, X, Y, Z);
+    sf_tables_h<JC, BLK, NPART>(X, Y, Z, KXT, KYT, KZT, j0, j1, nq, qanc, qbase);
+    __syncthreads();
+  }
+  for (int jc = j0; jc < j1; jc += JC) {
+    float *Xc, *Yc;
+    _Float16 *Zc;
+    tabs(set, Xc, Yc, Zc);
+    const float *Xf = Xc, *Yf = Yc;
+    const _Float16 *Zh = Zc;
+    if (DB) {
+      // the other set was last read before the previous barrier
+      if (jc + JC < j1) {
+        float *X, *Y;
+        _Float16 *Z;
+        tabs(set ^ 1, X, Y, Z);
+        sf_tables_h<JC, BLK, NPART>(X, Y, Z, KXT, KYT, KZT, jc + JC, j1, nq, qanc, qbase);
+      }
+    } else {
+      sf_tables_h<JC, BLK, NPART>(Xc, Yc, Zc, KXT, KYT, KZT, jc, j1, nq, qanc, qbase);
+      __syncthreads();
+    }
+    if (NA > 0) {
+      for (int s = 0; s < JC; s += 32) {
+        half8p1 a[MT][NPART];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+          const float4 x0 = *(const float4 *)(Xf + xo[m] + s), x1 = *(const float4 *)(Xf + xo[m] + s + 4);
+          const float4 y0 = *(const float4 *)(Yf + yo[m] + s), y1 = *(const float4 *)(Yf + yo[m] + s + 4);
+          const float v[8] = {x0.x * y0.x, x0.y * y0.y, x0.z * y0.z, x0.w * y0.w,
+                              x1.x * y1.x, x1.y * y1.y, x1.z * y1.z, x1.w * y1.w};
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            _Float16 pp[NPART];
+            split16<NPART>(v[i] * asc, pp);
+#pragma unroll
+            for (int q = 0; q < NPART; q++) a[m][q][i] = pp[q];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < NA; t++) {
+          half8p1 b[NPART];
+#pragma unroll
+          for (int q = 0; q < NPART; q++)
+            b[q] = *(const half8p1 *)(Zh + (q * KZT + 8 * t) * 2 * SZ + zo + s);
+          // products of parts (ia, ib): the leading one (0, 0) into acch,
+          // the smaller ones into acc (the instruction drops low bits of
+          // addends far below its accumulator, so terms of like size
+          // share one); the two M tiles' chains interleaved
+          constexpr int pa[6] = {NPART == 3 ? 2 : 1, 1, 0, 1, 0, 0};
+          constexpr int pb[6] = {0, NPART == 3 ? 1 : 0, NPART == 3 ? 2 : 1, 0, 1, 0};
+#pragma unroll
+          for (int k = (NPART == 3 ? 0 : 3); k < 5; k++)
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+              acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[m][pa[k]], b[pb[k]], acc[m][t], 0, 0, 0);
+#pragma unroll
+          for (int m = 0; m < MT; m++)
+            acch[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[m][0], b[0], acch[m][t], 0, 0, 0);
+        }
+      }
+      // the matrix sums of 2 JC charges go to the second-level sums
+      if (++nchunk % (128 / JC) == 0 || jc + JC >= j1) {
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+          for (int t = 0; t < NA; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              mid[m][t][r] += (acch[m][t][r] + acc[m][t][r]) * unsc;
+              acc[m][t][r] = 0.0f;
+              acch[m][t][r] = 0.0f;
+            }
+      }
+    }
+    __syncthreads();
+    if (DB) set ^= 1;
+  }
+  // lane l: row r0 + 4m + l/16, kz 8t + i16/2, T index 2r + i16%2
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int row = r0 + 4 * m + g;
+#pragma unroll
+    for (int t = 0; t < NA; t++) {
+      if (row >= blk_rows[t]) continue;
+      const int seg = blk_seg0[t] + row / 64;
+      double *out = kpart + (((size_t)blockIdx.y * nseg + seg) * 64 + row % 64) * 64 +
+                    (i16 >> 1) * 8 + (i16 & 1);
+#pragma unroll
+      for (int r = 0; r < 4; r++) out[2 * r] = (double)mid[m][t][r];
+    }
+  }
+}
+
+// U units of 4 MT rows per workgroup (one wave each); NTMAX: most N tiles
+template <int NTMAX, int MT, int JC, int U, int NPART, bool DB>
+__global__ __launch_bounds__(64 * U, 8 / U * (MT == 1 ? 4 : 2) > 0 ? 8 / U * (MT == 1 ? 4 : 2) : 1) void k_sfac_h(
+    int nq, int nrows, const int *__restrict__ row_kx, const int *__restrict__ row_ky,
+    const int *__restrict__ row_kzmax, const int *__restrict__ blk_rows,
+    const int *__restrict__ blk_seg0, int nseg, int KXT, int KYT, int KZT, int jper,
+    double *__restrict__ kpart, const float2 *__restrict__ qanc, const float2 *__restrict__ qbase,
+    float asc) {
+  extern __shared__ __align__(16) char smem[];
+  constexpr int SX = SfTabH<JC>::SX, SZ = SfTabH<JC>::SZ;
+  constexpr int BLK = 64 * U;
+  const size_t set_bytes = sizeof(float) * 2 * (KXT + KYT) * SX + sizeof(_Float16) * NPART * KZT * 2 * SZ;
+  const int wv = threadIdx.x >> 6;
+  const int r0 = (blockIdx.x * U + wv) * 4 * MT;
+  int nt = 0;
+  if (r0 < nrows) nt = min((row_kzmax[r0] + 8) / 8, NTMAX);
+  nt = __builtin_amdgcn_readfirstlane(nt);
+  const int j0 = blockIdx.y * jper, j1 = min(nq, j0 + jper);
+#define MDX_U(NA)                                                                              \
+  sf_unit_h<NA, MT, JC, BLK, NPART, DB>(smem, set_bytes, KXT, KYT, KZT, j0, j1, r0, nrows,     \
+                                row_kx, row_ky, blk_rows, blk_seg0, nseg, kpart, nq, qanc,     \
+                                qbase, asc)
+  switch (nt) {
+    case 0: MDX_U(0); break;
+    case 1: MDX_U(1); break;
+    case 2: if constexpr (NTMAX >= 2) MDX_U(2); break;
+    case 3: if constexpr (NTMAX >= 3) MDX_U(3); break;
+    case 4: if constexpr (NTMAX >= 4) MDX_U(4); break;
+    case 5: if constexpr (NTMAX >= 5) MDX_U(5); break;
+    case 6: if constexpr (NTMAX >= 6) MDX_U(6); break;
+    case 7: if constexpr (NTMAX >= 7) MDX_U(7); break;
+    default: if constexpr (NTMAX >= 8) MDX_U(8); break;
+  }
+#undef MDX_U
+}
+
 // sum of the charge splits: tsum[i] = sum_s kpart[s*n + i]
 __global__ void k_ksum(const double *__restrict__ kpart, int nsplit, size_t n,
                        double *__restrict__ tsum) {
@@ -1059,11 +1298,14 @@ __global__ __launch_bounds__(kBlock, NTC == 1 ? 3 : 2) void k_kforce_h(
     const int gi = grp_start[kx] + (ky0 >> 2);
     const int nb = min(grp_nb8[gi] - ch * BPC, BPC);   // wave-uniform, >= 1
     const half8 *ga = stage[buf] + lane;
-    mfma_f32x16 acc[NTC];
+    // the leading product (0, 0) of each block into acc, the smaller ones
+    // into acl (the instruction drops low bits of addends far below its
+    // accumulator, so terms of like size share one)
+    mfma_f32x16 acc[NTC], acl[NTC];
 #pragma unroll
     for (int u = 0; u < NTC; u++)
 #pragma unroll
-      for (int r = 0; r < 16; r++) acc[u][r] = 0.0f;
+      for (int r = 0; r < 16; r++) { acc[u][r] = 0.0f; acl[u][r] = 0.0f; }
 #pragma unroll
     for (int b = 0; b < BPC; b++) {
       if (b < nb) {
@@ -1072,18 +1314,21 @@ __global__ __launch_bounds__(kBlock, NTC == 1 ? 3 : 2) void k_kforce_h(
         for (int k = 0; k < NPART; k++) ev[k] = ga[(b * NPART + k) * 64];
 #pragma unroll
         for (int u = 0; u < NTC; u++) {
-          // the smaller products first
           if constexpr (NPART == 3) {
-            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[2], zt[u][b][0], acc[u], 0, 0, 0);
-            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][1], acc[u], 0, 0, 0);
-            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][2], acc[u], 0, 0, 0);
+            acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[2], zt[u][b][0], acl[u], 0, 0, 0);
+            acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][1], acl[u], 0, 0, 0);
+            acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][2], acl[u], 0, 0, 0);
           }
-          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][0], acc[u], 0, 0, 0);
-          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][1], acc[u], 0, 0, 0);
+          acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][0], acl[u], 0, 0, 0);
+          acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][1], acl[u], 0, 0, 0);
           acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][0], acc[u], 0, 0, 0);
         }
       }
     }
+#pragma unroll
+    for (int u = 0; u < NTC; u++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[u][r] += acl[u][r];
     // the group's scale goes into the row phases
     const float sc = gscale[gi] * (1.0f / 256.0f);
 #pragma unroll
@@ -1185,7 +1430,7 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
   dim3 g1((K.nseg + 3) / 4, c.nsplit);
   const double4 *qpos = (const double4 *)c.qpos.p;
   // anchors of the matrix-core pass 1 (mixed mode, row units)
-  const bool anc = sizeof(R) == 4 && (c.p1nt == 1 || c.p1nt == 2);
+  const bool anc = sizeof(R) == 4 && (c.p1nt == 1 || c.p1nt == 2 || (c.p1nt >= 4 && c.p1nt <= 7));
   const int3 nb = make_int3((KXT + 7) / 8, (KYT + 7) / 8, (KZT + 7) / 8);
   const int3 boff = make_int3(0, nb.x, nb.x + nb.y);
   if (anc && (size_t)(nb.x + nb.y + nb.z + 3) * c.nq > c.qanc.n)
@@ -1229,6 +1474,45 @@ int run_kspace(mdx::Ctx &c, int measure, hipStream_t st) {
       else MDX_SF(8, 1);
     }
 #undef MDX_SF
+  } else if (pass1_mfma && c.p1nt >= 4 && c.p1nt <= 7) {
+    // split-FP16 row units, one wave each: (M tiles per wave, waves per
+    // workgroup, charges per chunk, table sets) = (2, 8, 64, 1),
+    // (1, 16, 64, 1), (1, 16, 32, 2), (2, 8, 32, 2) for p1nt 4..7
+    if (K.nblk > 8)
+      return fail(MDX_E_ARG, "k-space pass 1 on row units: at most 8 kz blocks (kz < 64)");
+    const int cfg = c.p1nt - 4;
+    const int MTv = cfg == 0 || cfg == 3 ? 2 : 1, Uv = cfg == 1 || cfg == 2 ? 16 : 8;
+    const int JCv = cfg >= 2 ? 32 : 64, nset = cfg >= 2 ? 2 : 1;
+    const int wgx = ((K.nrows + 4 * MTv - 1) / (4 * MTv) + Uv - 1) / Uv;
+    nsplit1 = std::max(1, std::min(c.nsplit, 1024 / wgx));
+    const int jp = (c.nq + nsplit1 - 1) / nsplit1;
+    const int npart = c.kprod == 3 ? 2 : 3;
+    const size_t shm = nset * (sizeof(float) * 2 * (KXT + KYT) * (JCv + 8) +
+                               sizeof(_Float16) * npart * KZT * 2 * (JCv + 8));
+    // A = q X Y scaled into [2^13, 2^14) at the largest |q|
+    const float asc = std::ldexp(1.0f, 13 - std::ilogb(std::max(c.qmax, 1e-30)));
+    const dim3 gm(wgx, nsplit1);
+#define MDX_SH(NT, MT, JC, U, NP)                                                            \
+  k_sfac_h<NT, MT, JC, U, NP, JC == 32><<<gm, 64 * U, shm, st>>>(                            \
+      c.nq, K.nrows, c.d_row_kx.p, c.d_row_ky.p, c.d_row_kzmax.p, c.d_blk_rows.p,           \
+      c.d_blk_seg0.p, K.nseg, KXT, KYT, KZT, jp, c.kpart.p, c.qanc.p,                       \
+      c.qanc.p + (size_t)(nb.x + nb.y + nb.z) * c.nq, asc)
+#define MDX_SHN(MT, JC, U, NP)                       \
+  if (K.nblk <= 2) MDX_SH(2, MT, JC, U, NP);         \
+  else if (K.nblk <= 4) MDX_SH(4, MT, JC, U, NP);    \
+  else if (K.nblk <= 6) MDX_SH(6, MT, JC, U, NP);    \
+  else MDX_SH(8, MT, JC, U, NP);
+#define MDX_SHC(NP)                                  \
+  switch (cfg) {                                     \
+    case 0: MDX_SHN(2, 64, 8, NP); break;            \
+    case 1: MDX_SHN(1, 64, 16, NP); break;           \
+    case 2: MDX_SHN(1, 32, 16, NP); break;           \
+    default: MDX_SHN(2, 32, 8, NP); break;           \
+  }
+    if (npart == 3) { MDX_SHC(3); } else { MDX_SHC(2); }
+#undef MDX_SHC
+#undef MDX_SHN
+#undef MDX_SH
   } else if (pass1_mfma && c.p1nt == 8)
     k_sfac_mfma<8><<<dim3((K.nseg + 1) / 2, c.nsplit), kBlock, shmem, st>>>(
         c.nq, qpos, K.nseg, c.d_seg_blk.p, c.d_seg_row0.p, c.d_seg_rowend.p, c.d_row_kx.p,

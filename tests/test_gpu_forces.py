@@ -132,7 +132,8 @@ def test_all_pair_path(name, parts, mode):
 def test_kspace_pass2_on_matrix_cores(name):
     """the mixed engine's pass 2 on the matrix cores gives cook's forces
     within the mixed bar and agrees with the FMA variant to FP32 rounding,
-    for every kz chunk and part split (energies do not depend on pass 2)"""
+    for every kz chunk and part split and each choice of products
+    (energies do not depend on pass 2)"""
     d, meta = oracle.load_fixture(name)
     bs = oracle.fixture_system(meta, provider_=engine_provider())
     eng = Engine(bs, abi.MDX_MIXED)
@@ -140,11 +141,12 @@ def test_kspace_pass2_on_matrix_cores(name):
     for kzc in (8, 16, 24, 32, 48, 64):
         for parts in (1, 5, 16):
             eng.set_kchunk(kzc, parts)
-            eng.set_kspace_mfma(0)
-            f0, o0 = eng.forces(d["r"], 1)
-            # FP32 operands, split-FP16 operands with 3 or 6 products
+            # FP32 operands, split-FP16 operands with 3 or 6 products (the
+            # default pass 1 takes its operand parts from the same setting)
             for kp in (0, 3, 6):
                 eng.set_kspace_products(kp)
+                eng.set_kspace_mfma(0)
+                f0, o0 = eng.forces(d["r"], 1)
                 for tiles in (1, 2):
                     # 1 or 2 tiles of 32 charges per wave: the same products
                     eng.set_kspace_mfma(tiles)

@@ -92,6 +92,9 @@ def test_polar_c3_mixed_vs_fp64(c3):
     res = {}
     for mode in (abi.MDX_FP64, abi.MDX_MIXED):
         eng = Engine(bs, mode)
+        if mode == abi.MDX_MIXED:
+            # Drudes: pass 1 on the FP32 matrix cores by default
+            assert eng.info()["kpass1_tiles"] == 2
         res[mode] = [eng.forces(r, measure=m) for m in (0, 1)]
         eng.close()
     # measure=0: FP32 four-term pair kernel in the mixed mode
