@@ -274,9 +274,8 @@ def read_ble(path, form=None):
         for _ in range(nnbfixes):
             names = []
             for _k in range(2):
+                # by name only, as atomn() (cook/simdef.c:77-120)
                 t = L.tok()
-                if t not in types and t.isdigit() and int(t) < len(types):
-                    t = types[int(t)]
                 if t not in types:
                     raise CookInputError("nbfix: unknown site type %s" % t)
                 names.append(t)

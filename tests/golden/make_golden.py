@@ -71,13 +71,14 @@ def ion_species(name, q):
     return dict(name=name, sites=[(name, q, (0.0, 0.0, 0.0), [])], bonds=[])
 
 
-def write_ble(path, species, types, polar=None):
-    """polar: dict type -> (alphapol [A^3], chargepol [e]) for Drude sites."""
+def write_ble(path, species, types, polar=None, nbfix=()):
+    """polar: dict type -> (alphapol [A^3], chargepol [e]) for Drude sites;
+    nbfix: rows (type, type, Emin [kcal/mol], R [A]) of the nbfixes table."""
     out = []
     out.append("! hand-written force field for golden fixtures (LJ form)")
     out.append("nspec=%d" % len(species))
-    out.append("  nsites=%d  nnbfixes=0  factor14=0.5  distance14=0  comb_rule=1"
-               % len(types))
+    out.append("  nsites=%d  nnbfixes=%d  factor14=0.5  distance14=0  comb_rule=1"
+               % (len(types), len(nbfix)))
     out.append("  polar=%d  nparms=0;" % (1 if polar else 0))
     out.append("")
     out.append("Lennard-Jones")
@@ -94,6 +95,8 @@ def write_ble(path, species, types, polar=None):
     out.append("")
     out.append("nbfixes")
     out.append("! atom atom Eref R")
+    for a, b, e, r in nbfix:
+        out.append("%s %s %.12g %.12g" % (a, b, e, r))
     out.append("")
     for isp, sp in enumerate(species):
         out.append("!!!!!!!!!!")
@@ -219,7 +222,7 @@ def ff_arrays(species, types, polar=None):
 
 
 def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
-               extra_def="", polar=None, scf=None, md_steps=0, ewald=None):
+               extra_def="", polar=None, scf=None, md_steps=0, ewald=None, nbfix=()):
     """One Ewaldtest force evaluation on cook's own initial configuration.
 
     md_steps>0 first runs that many MD steps (SHAKE, leap-frog, Berendsen)
@@ -230,7 +233,7 @@ def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
     if ewald:
         extra_def = " ".join([extra_def] + ["el.%s=%d" % (k[3:], v) for k, v in ewald.items()])
     with tempfile.TemporaryDirectory() as td:
-        write_ble(os.path.join(td, "sys.ble"), species, types, polar)
+        write_ble(os.path.join(td, "sys.ble"), species, types, polar, nbfix)
         lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(counts)]
         lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L),
                   "cutoff=%g" % cutoff, "LJcutoff=cutoff",
@@ -257,7 +260,7 @@ def force_case(name, species, types, counts, L, cutoff, epsr, epsk,
         dump = parse_dump(os.path.join(td, "t.0.dump"), polar=bool(polar))
         meta = parse_prt(os.path.join(td, "t.prt"))
     meta.update(dict(name=name, counts=counts, L=L, epsr=epsr, epsk=epsk,
-                     md_steps=md_steps, ewald=ewald or {},
+                     md_steps=md_steps, ewald=ewald or {}, nbfix=[list(x) for x in nbfix],
                      forcefield=ff_arrays(species, types, polar)))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **dump)
     with open(os.path.join(HERE, name + ".json"), "w") as f:
@@ -331,13 +334,18 @@ def plb_case(name, species, types, counts, L, cutoff, nsteps):
     print(name, len(dumps), "dumps", len(raw), "plb bytes")
 
 
-def friction_case(name, species, types, counts, L, cutoff, nsteps):
+FRICTION_RUNS = [("tr", "BerendsenCM", ""), ("in", "BerendsenIN", ""),
+                 ("fr", "frictions", "T_tr_in=1.3\n")]
+
+
+def friction_case(name, species, types, counts, L, cutoff, nsteps, runs=FRICTION_RUNS):
     """the decoupled friction thermostats of sim/thermo.c:107-135: from the
     configuration of 5 Berendsen steps (cfg0), three runs of nsteps each
     with thermostat="BerendsenCM" (translations), "BerendsenIN" (the
     intramolecular motion) and "frictions" (both, T_tr_in=1.3), tau.T=0.05;
-    stored: cfg0 and each run's final configuration"""
-    runs = [("tr", "BerendsenCM", ""), ("in", "BerendsenIN", ""), ("fr", "frictions", "T_tr_in=1.3\n")]
+    stored: cfg0 and each run's final configuration.  Other runs (tag,
+    thermostat, extra lines) may be given, e.g. the plain Berendsen
+    thermostat of sim/thermo.c:141-160."""
     out = {}
     with tempfile.TemporaryDirectory() as td:
         write_ble(os.path.join(td, "sys.ble"), species, types)
@@ -613,6 +621,11 @@ def main(argv):
             "brine506", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
             ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 0.05, 0.05,
             md_steps=20),
+        # the same with an Na-O nbfix (Emin 0.2 kcal/mol at 3.0 A)
+        "brine506_nbfix": lambda: force_case(
+            "brine506_nbfix", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
+            ["H", "O", "NA", "CL"], [3, 3, 500], 24.86, 9.0, 0.05, 0.05,
+            md_steps=20, nbfix=[("NA", "O", -0.2, 3.0)]),
         # polarizable: Drude on O (1.44 A^3, -8 e) and on Cl (3.5 A^3, -8 e)
         "polar306": lambda: polar_case(
             "polar306", [ion_species("NA", 1.0), ion_species("CL", -1.0), spce],
@@ -661,6 +674,11 @@ def main(argv):
         # NVE trajectory of 216 SPC/E (SHAKE + leap-frog)
         "spce216_nve": lambda: traj_case(
             "spce216_nve", [spce], ["H", "O"], [216], 18.6206, 8.0, 8),
+        # the plain Berendsen thermostat (tau.T = 0.05 ps, T = 298 K), 8 steps
+        # from the configuration of 5 Berendsen steps (cook's md.cfg)
+        "spce216_ber": lambda: friction_case(
+            "spce216_ber", [spce], ["H", "O"], [216], 18.6206, 8.0, 8,
+            runs=[("ber", "Berendsen", "")]),
     }
     todo = argv or list(cases)
     for c in todo:

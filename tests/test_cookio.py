@@ -25,9 +25,9 @@ CASES = {
 }
 
 
-def write_deck(td, meta, species, types, polar):
+def write_deck(td, meta, species, types, polar, nbfix=()):
     ble = os.path.join(td, "sys.ble")
-    mg.write_ble(ble, species, types, polar)
+    mg.write_ble(ble, species, types, polar, nbfix)
     L = meta["L"]
     lines = ["N[%d]=%d" % (i, n) for i, n in enumerate(meta["counts"])]
     lines += ["L[0]=%.10g L[1]=%.10g L[2]=%.10g" % (L, L, L), "cutoff=%g" % meta["cutoff"],
@@ -289,12 +289,16 @@ def test_nbfix_table(tmp_path):
     f0, o0 = oracle.forces(bs0, r, 1)
     f1, o1 = oracle.forces(bs1, r, 1)
     assert np.abs(f1 - f0).max() < 1e-9 * np.abs(f0).max()
-    bs2 = with_row("1 1 -0.2 3.4")                  # type numbers, other values
+    bs2 = with_row("O O -0.2 3.4")                  # other values
     t2 = bs2.info["sstab"][3]
     assert t2.E4 == pytest.approx(0.8 * EUNIT_KCAL, rel=1e-13)
     assert t2.Sq == pytest.approx(3.4 ** 2 * 2 ** (-1 / 3), rel=1e-13)
     for k in (0, 1, 2):
         assert bs2.info["sstab"][k].E4 == bs0.info["sstab"][k].E4
+    # types by name only (atomn(), cook/simdef.c:77-120): a type number is
+    # not a type
+    with pytest.raises(cookio.CookInputError):
+        with_row("1 1 -0.2 3.4")
     # the exp-6 form refuses nbfixes
     ble6, _, _ = exp6_ble(tmp_path, {"H": 14.0, "O": 13.5})
     open(ble6, "w").write(open(ble6).read().replace("nnbfixes=0", "nnbfixes=1"))
@@ -324,3 +328,26 @@ def test_shell_core_columns(tmp_path):
     assert not bs.struct.shell_qkappa
     with pytest.raises(ValueError, match="Buckingham or Busing"):
         cookio.system_from_cook(ble, dfn, provider=oracle.provider(), shell_core=True)
+
+
+def test_nbfix_deck_gives_cooks_forces(tmp_path):
+    """cook's own run of the brine deck with an Na-O nbfix (tests/golden/
+    brine506_nbfix: Emin 0.2 kcal/mol at 3.0 A): the deck read by cookio
+    gives, through the oracle, cook's forces, Epot and tensor; without the
+    nbfix row the forces differ"""
+    d, meta = oracle.load_fixture("brine506_nbfix")
+    assert meta["nbfix"] == [["NA", "O", -0.2, 3.0]]
+    species, types, _ = CASES["brine506"]()
+    ble, dfn = write_deck(str(tmp_path), meta, species, types, None,
+                          [tuple(x) for x in meta["nbfix"]])
+    assert cookio.read_ble(ble)["nbfix"][0][:2] == ["NA", "O"]
+    bs, _ = cookio.system_from_cook(ble, dfn, provider=oracle.provider())
+    f, out = oracle.forces(bs, d["r"], 1)
+    err = float(np.sqrt(((f - d["f"]) ** 2).sum() / (d["f"] ** 2).sum()))
+    assert err < 1e-10, err
+    assert abs(out.En_pot + out.En_el - meta["Epot"]) < 1e-9 * abs(meta["Epot"])
+    ble0, dfn0 = write_deck(str(tmp_path / "plain"), meta, species, types, None) \
+        if (tmp_path / "plain").mkdir() is None else (None, None)
+    bs0, _ = cookio.system_from_cook(ble0, dfn0, provider=oracle.provider())
+    f0, _ = oracle.forces(bs0, d["r"], 1)
+    assert float(np.sqrt(((f0 - d["f"]) ** 2).sum() / (d["f"] ** 2).sum())) > 1e-5

@@ -3,7 +3,7 @@ that the reference cook dumped (tests/golden/spce216_nve.*)."""
 import numpy as np
 
 import oracle
-from macsimus_amd import abi
+from macsimus_amd import abi, cookio, workloads
 
 
 def test_nve_trajectory_matches_reference():
@@ -33,3 +33,45 @@ def test_constraints_hold_after_steps():
     dr = r[cij[:, 0]] - r[cij[:, 1]]
     rel = np.abs((dr ** 2).sum(1) / bs.info["cons_bondq"] - 1)
     assert rel.max() < 1e-9
+
+
+
+def berendsen_case(tmp_path):
+    """cook's spce216_ber runs: cfg0 (5 Berendsen steps from the crystal)
+    and the configuration cook saved after 8 more steps with
+    thermostat="Berendsen", tau.T = 0.05 ps, T = 298 K"""
+    d, meta = oracle.load_fixture("spce216_ber")
+    cfgs = {}
+    for nm in ("cfg0", "cfg_ber"):
+        p = tmp_path / (nm + ".cfg")
+        p.write_bytes(d[nm].tobytes())
+        cfgs[nm] = cookio.read_cfg(p)
+    nf = workloads.degrees_of_freedom(meta["forcefield"], meta["counts"])
+    step = abi.Step(h=meta["h"], eps=1e-12, maxit=0, thermostat=1, T=meta["T"], tau=meta["tauT"],
+                    nf=float(nf), measure_every=1)
+    return meta, cfgs, step
+
+
+def test_berendsen_run_ends_on_cooks_configuration(tmp_path):
+    """the plain Berendsen branch of Shake() (sim/thermo.c:141-160): the
+    oracle started from cook's cfg0 ends on cook's saved configuration; the
+    same steps without the thermostat do not"""
+    meta, cfgs, p = berendsen_case(tmp_path)
+    bs = oracle.fixture_system(meta)
+    c0, c1 = cfgs["cfg0"], cfgs["cfg_ber"]
+    assert c1["thermostat"] == 1
+    r, r1 = c0["cfgs"][0]["rp"].copy(), c0["cfgs"][1]["rp"].copy()
+    lams = []
+    for _ in range(meta["nsteps"]):
+        out = oracle.md_step(bs, r, r1, p, measure=1)
+        lams.append(out.lambda_)
+    dr = np.abs(r - c1["cfgs"][0]["rp"]).max()
+    dr1 = np.abs(r1 - c1["cfgs"][1]["rp"]).max()
+    print("Berendsen: r %.2e r1 %.2e, lambda %s" % (dr, dr1, lams))
+    assert dr < 1e-10 and dr1 < 1e-11
+    assert max(abs(x - 1) for x in lams) > 1e-5          # the thermostat acts
+    p.thermostat = 0
+    r, r1 = c0["cfgs"][0]["rp"].copy(), c0["cfgs"][1]["rp"].copy()
+    for _ in range(meta["nsteps"]):
+        oracle.md_step(bs, r, r1, p, measure=1)
+    assert np.abs(r1 - c1["cfgs"][1]["rp"]).max() > 1e-8
