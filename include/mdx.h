@@ -274,6 +274,10 @@ typedef struct mdx_step_out_s {
   double L[3];
   double Pref, V;
   double npt_lambda, npt_dlambda;
+  /* SHAKE sweeps summed over the molecules with constraints and over the
+     steps of the call: cook's constrit[sp].nit[COR_BONDS] summed over the
+     species (sim/constrd.c:930); per species by mdx_get_shake_nit */
+  long long shake_nit;
 } mdx_step_out_t;
 
 /* Replaces one selffield() iteration (sim/scf.c:12-240, POLAR=0):
@@ -431,6 +435,24 @@ int mdx_set_stencil(mdx_ctx_t *ctx, int w, int pcols);
 int mdx_set_kchunk(mdx_ctx_t *ctx, int kzc, int parts);
 int mdx_autotune(mdx_ctx_t *ctx, const double *r, int reps, int *choice, double *ms);
 int mdx_info(mdx_ctx_t *ctx, int *iv, int n);
+
+/* ---- SHAKE relaxation omega per species (cook's omegac optimization) ----
+   cook's default omegac < 0 tunes the relaxation of each species during the
+   run from the sweep counts (doconstrit(), sim/norm.c:32-78, with cook's own
+   random trial steps).  The engine keeps that on the caller's side: it
+   counts the sweeps per species and takes the new omega per species.
+   mdx_set_shake_species: species index of each molecule (mol_species[nmol],
+     values in [0, nspec), nspec <= 64; NULL: one species).
+   mdx_set_shake_omega: the relaxation omega (= omegac/2, constrit[sp].omega)
+     of each species for the following steps (omega[nspec]; each in (0, 2)).
+   mdx_get_shake_nit: the sweeps of the molecules with constraints of each
+     species summed over the steps of the last mdx_md_steps call (nit[nspec],
+     cook's constrit[sp].nit[COR_BONDS] of those steps).
+   mdx_resize goes back to one species and keeps the omega of the new
+   system's mol_omega. */
+int mdx_set_shake_species(mdx_ctx_t *ctx, const int *mol_species, int nspec);
+int mdx_set_shake_omega(mdx_ctx_t *ctx, const double *omega);
+int mdx_get_shake_nit(mdx_ctx_t *ctx, long long *nit, int nspec);
 
 /* ---- force-field fitting (no counterpart in cook) ----------------------
    mdx_set_params: new site-site table sstab [ntypes^2] (mdx_setss) and/or

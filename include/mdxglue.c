@@ -9,6 +9,7 @@
 #include "interpot.h"
 #include "elst.h"
 #include "simdef.h"                       /* nsites: number of site types */
+#include "norm.h"                         /* constrit, doconstrit() */
 #include "mdx.h"
 
 static mdx_ctx_t *mdx;
@@ -36,7 +37,8 @@ void mdxinit(int mode) /* after initss(), initrspace(), Ewald(0), lcsetup() */
   loop (n, 0, No.N) {                      /* molecule-major, as A->rp */
     molecule_t *mn = molec + n;
     specinfo_t *sp = spec[mn->sp];
-    mdx_molfirst[n] = s;  mdx_consfirst[n] = c;  mdx_omega[n] = 0.6;
+    mdx_molfirst[n] = s;  mdx_consfirst[n] = c;
+    mdx_omega[n] = constrit ? constrit[mn->sp].omega : fabs(omegac) / 2;
     loop (i, 0, mn->ns) {
       siteinfo_t *si = sp->si + i;
       exception_t *e;
@@ -98,6 +100,27 @@ void mdxinit(int mode) /* after initss(), initrspace(), Ewald(0), lcsetup() */
   }
 #endif
   mdxcheck(mdx_create(&mdx, &S, mode), "mdx_create");
+  {                                        /* species of the molecules: sweep counts */
+    int *sp;
+    allocarray(sp, No.N);
+    loop (n, 0, No.N) sp[n] = molec[n].sp;
+    mdxcheck(mdx_set_shake_species(mdx, sp, nspec), "mdx_set_shake_species");
+    free(sp);
+  }
+}
+
+/* cook's omegac optimization (sim/main.c:1621, doconstrit() of sim/norm.c)
+   with the engine's sweep counts, after every step of mdxsteps(.., 1) */
+void mdxconstrit(void)
+{
+  long long nit[64];
+  double om[64];
+  int sp;
+  mdxcheck(mdx_get_shake_nit(mdx, nit, nspec), "mdx_get_shake_nit");
+  loop (sp, 0, nspec) constrit[sp].nit[COR_BONDS] = (int)nit[sp];
+  doconstrit(omegac);
+  loop (sp, 0, nspec) om[sp] = constrit[sp].omega;
+  mdxcheck(mdx_set_shake_omega(mdx, om), "mdx_set_shake_omega");
 }
 
 /* forces(B,A) body under MDX; cook/forces.c:93 */

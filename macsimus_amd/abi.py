@@ -93,6 +93,7 @@ class StepOut(C.Structure):
         ("En_kin_tr", C.c_double),
         ("L", C.c_double * 3), ("Pref", C.c_double), ("V", C.c_double),
         ("npt_lambda", C.c_double), ("npt_dlambda", C.c_double),
+        ("shake_nit", C.c_longlong),
     ]
 
 

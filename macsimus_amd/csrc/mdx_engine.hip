@@ -1788,6 +1788,8 @@ extern "C" int mdx_destroy(mdx_ctx_t *h) {
   c.shk64.release();
   c.shk32.release();
   c.shrep.release();
+  c.shk_sp.release();
+  c.shk_nit.release();
   c.tm.release();
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);

@@ -275,6 +275,12 @@ struct Ctx {
   DBuf<double> mass;
   DBuf<int> cons_first, cons_ij;
   DBuf<double> cons_bondq, mol_omega;
+  // SHAKE sweep counts per species (cook's constrit nit) and the species of
+  // each molecule; one species unless mdx_set_shake_species gave more
+  int shk_nsp = 1;
+  std::vector<int> shk_sp_host;
+  DBuf<int> shk_sp;
+  DBuf<unsigned long long> shk_nit;
   int ncons = 0;
   bool state_on_device = false;
   // reduction buffers of the MD step: SHAKE/kinetic partials and sums,

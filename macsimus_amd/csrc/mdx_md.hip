@@ -78,10 +78,12 @@ __global__ __launch_bounds__(kBlock) void k_shake(
     int maxit, int measure, double *__restrict__ r, double *__restrict__ r1,
     const double *__restrict__ f, double *__restrict__ p, double *__restrict__ part, int pred,
     const double *__restrict__ nose, double vfac_host, double bscale, double *__restrict__ hist,
-    long stride, Trvp tv) {
+    long stride, Trvp tv, const int *__restrict__ msp, unsigned long long *__restrict__ nit) {
   const int m = blockIdx.x * kBlock + threadIdx.x;
   double ek = 0, pk[6] = {0, 0, 0, 0, 0, 0}, virc = 0, ptv[6] = {0, 0, 0, 0, 0, 0};
   double its = 0, bad = 0, ktr = 0;
+  unsigned long long nsw = 0;          // sweeps of a molecule with constraints
+  int sp = -1;                         // its species
   if (m < nmol) {
     const double hh = h * h;
     const int b = molfirst[m], e = molfirst[m + 1];
@@ -145,6 +147,8 @@ __global__ __launch_bounds__(kBlock) void k_shake(
       if (maxsq > eps) bad = 1;
     }
     its = it;
+    nsw = nc ? it : 0;
+    sp = msp ? msp[m] : 0;
     double vm1[3] = {0, 0, 0}, vm2[3] = {0, 0, 0}, mm = 0;
     for (int i = b; i < e; i++) {
       double V1[3], V2[3];
@@ -164,6 +168,20 @@ __global__ __launch_bounds__(kBlock) void k_shake(
     // translational kinetic energy of the molecule (sim/shakev3.c:35)
     ktr = (vm1[0] * vm1[0] + vm1[1] * vm1[1] + vm1[2] * vm1[2] +
            vm2[0] * vm2[0] + vm2[1] * vm2[1] + vm2[2] * vm2[2]) / mm;
+  }
+  // sweep counts per species (cook's constrit nit, sim/constrd.c:930):
+  // one atomic per wave where the wave's molecules share a species (the
+  // molecules of a species are consecutive), else one per molecule;
+  // integer sums, so the order does not matter
+  {
+    const int s0 = __shfl(sp, 0, 64);
+    if (__all(sp == s0 || sp < 0)) {
+      unsigned long long t = nsw;
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      if ((threadIdx.x & 63) == 0 && t && s0 >= 0) atomicAdd(nit + s0, t);
+    } else if (sp >= 0 && nsw) {
+      atomicAdd(nit + sp, nsw);
+    }
   }
   __shared__ double red[kBlock / 64][MD_N];
   const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -488,7 +506,7 @@ int set_trvp(Ctx &c, int k, int kk, const double *hist) {
 
 int mdx::md_setup(Ctx &c) {
   int rc;
-  if ((rc = c.mdflag.alloc(4))) return rc;
+  if ((rc = c.mdflag.alloc(4)) || (rc = c.shk_nit.alloc(64))) return rc;
   if (c.mass.p && ((rc = c.mdpart.alloc((size_t)nblk(c.nmol) * MD_N)) || (rc = c.mdacc.alloc(MD_N)) ||
                     (rc = c.nose.alloc(4))))
     return rc;
@@ -503,9 +521,15 @@ int mdx::md_setup(Ctx &c) {
 
 namespace {
 // flags: lambda = 1, the sticky ones 0 (a kernel, so that the step can be captured)
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 void reset_flags(Ctx &c) {
   k_fill<<<1, 64, 0, c.stream>>>(4, c.mdflag.p, 0.0);
   k_fill<<<1, 64, 0, c.stream>>>(1, c.mdflag.p, 1.0);
+  k_zero_u64<<<1, 64, 0, c.stream>>>(c.shk_nit.p, c.shk_nsp);
 }
 }  // namespace
 
@@ -777,7 +801,8 @@ int launch_step(Ctx &c, const mdx_step_t *p, int measure, int kk, double omega,
                                        c.cons_bondq.p, c.mol_omega.p, c.mass.p, p->h, p->eps,
                                        p->maxit, measure, c.r.p, c.r1.p, c.f.p, c.pwork.p,
                                        c.mdpart.p, nose ? PRED_NOSE : npt ? PRED_HOST : PRED_NONE,
-                                       c.nose.p, vfac, bscale, c.trvp_hist.p, stride, tv);
+                                       c.nose.p, vfac, bscale, c.trvp_hist.p, stride, tv,
+                                       c.shk_nsp > 1 ? c.shk_sp.p : nullptr, c.shk_nit.p);
   if (c.timing) c.tm.stop(T_SHAKE, c.stream);
   k_md_reduce<<<1, kBlock, 0, c.stream>>>(c.mdpart.p, nb, c.mdacc.p, c.mdflag.p);
   if (p->thermostat == 1)
@@ -943,6 +968,66 @@ extern "C" int mdx_md_steps(mdx_ctx_t *h, const mdx_step_t *p, int nsteps, mdx_s
       if (npt && (rc = npt_update(h, p, np, md[MD_EKIN] * hscale, out))) break;
     }
   }
+  if (!rc) {
+    unsigned long long nit[64];
+    if (hipMemcpyAsync(nit, c.shk_nit.p, sizeof(unsigned long long) * c.shk_nsp,
+                       hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess)
+      return fail(MDX_E_GPU, "mdx_md_steps: download");
+    for (int k = 0; k < c.shk_nsp; k++) out->shake_nit += (long long)nit[k];
+  }
   (void)hipStreamSynchronize(c.stream);
   return rc;
+}
+
+extern "C" int mdx_set_shake_species(mdx_ctx_t *h, const int *msp, int nspec) {
+  if (!h) return fail(MDX_E_ARG, "mdx_set_shake_species: NULL");
+  Ctx &c = h->c;
+  if (!msp) nspec = 1;
+  if (nspec < 1 || nspec > 64) return fail(MDX_E_ARG, "mdx_set_shake_species: nspec in 1..64");
+  std::vector<int> v(c.nmol, 0);
+  if (msp)
+    for (int m = 0; m < c.nmol; m++) {
+      if (msp[m] < 0 || msp[m] >= nspec)
+        return fail(MDX_E_ARG, "mdx_set_shake_species: species index out of [0, nspec)");
+      v[m] = msp[m];
+    }
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  int rc;
+  if ((rc = c.shk_sp.alloc(std::max(1, c.nmol)))) return rc;
+  if (c.nmol)
+    MDX_HIP(hipMemcpy(c.shk_sp.p, v.data(), sizeof(int) * c.nmol, hipMemcpyHostToDevice));
+  c.shk_sp_host = msp ? v : std::vector<int>();
+  c.shk_nsp = nspec;
+  c.gen++;                             // captured steps hold the species pointer
+  return 0;
+}
+
+extern "C" int mdx_set_shake_omega(mdx_ctx_t *h, const double *omega) {
+  if (!h || !omega) return fail(MDX_E_ARG, "mdx_set_shake_omega: NULL");
+  Ctx &c = h->c;
+  for (int k = 0; k < c.shk_nsp; k++)
+    if (!(omega[k] > 0 && omega[k] < 2))
+      return fail(MDX_E_ARG, "mdx_set_shake_omega: omega in (0, 2)");
+  if (!c.nmol) return 0;
+  std::vector<double> om(c.nmol);
+  for (int m = 0; m < c.nmol; m++) om[m] = omega[c.shk_sp_host.empty() ? 0 : c.shk_sp_host[m]];
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  // the same buffer as before: captured steps read the new values
+  MDX_HIP(hipMemcpy(c.mol_omega.p, om.data(), sizeof(double) * c.nmol, hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int mdx_get_shake_nit(mdx_ctx_t *h, long long *nit, int nspec) {
+  if (!h || !nit) return fail(MDX_E_ARG, "mdx_get_shake_nit: NULL");
+  Ctx &c = h->c;
+  if (nspec != c.shk_nsp) return fail(MDX_E_ARG, "mdx_get_shake_nit: nspec differs from the species set");
+  MDX_HIP(hipSetDevice(c.device));
+  MDX_HIP(hipStreamSynchronize(c.stream));
+  unsigned long long v[64];
+  MDX_HIP(hipMemcpy(v, c.shk_nit.p, sizeof(unsigned long long) * nspec, hipMemcpyDeviceToHost));
+  for (int k = 0; k < nspec; k++) nit[k] = (long long)v[k];
+  return 0;
 }

@@ -284,6 +284,33 @@ class Engine:
         rc = self.lib.mdx_set_kspace_products(self.ctx, int(np_))
         abi.check(self.lib, rc, "mdx_set_kspace_products")
 
+    def set_shake_species(self, mol_species=None, nspec=1):
+        """species index of each molecule for the SHAKE sweep counts and
+        omega (None: one species)"""
+        if mol_species is None:
+            rc = self.lib.mdx_set_shake_species(self.ctx, None, 1)
+        else:
+            a = np.ascontiguousarray(mol_species, dtype=np.int32)
+            self._shake_sp = a
+            rc = self.lib.mdx_set_shake_species(self.ctx, a.ctypes.data_as(C.POINTER(C.c_int)),
+                                                int(nspec))
+        abi.check(self.lib, rc, "mdx_set_shake_species")
+        self._shake_nsp = 1 if mol_species is None else int(nspec)
+
+    def set_shake_omega(self, omega):
+        """SHAKE relaxation omega (= omegac/2) per species"""
+        a = np.ascontiguousarray(np.atleast_1d(omega), dtype=np.float64)
+        rc = self.lib.mdx_set_shake_omega(self.ctx, a.ctypes.data_as(C.POINTER(C.c_double)))
+        abi.check(self.lib, rc, "mdx_set_shake_omega")
+
+    def shake_nit(self):
+        """SHAKE sweeps per species over the last md_steps call"""
+        n = getattr(self, "_shake_nsp", 1)
+        v = (C.c_longlong * n)()
+        rc = self.lib.mdx_get_shake_nit(self.ctx, v, n)
+        abi.check(self.lib, rc, "mdx_get_shake_nit")
+        return np.array(list(v), dtype=np.int64)
+
     def set_kspace_pass1(self, tiles=0):
         rc = self.lib.mdx_set_kspace_pass1(self.ctx, int(tiles))
         abi.check(self.lib, rc, "mdx_set_kspace_pass1")
@@ -487,6 +514,9 @@ def declare_extras(lib):
     lib.mdx_set_kspace_mfma.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_kspace_pass1.argtypes = [C.c_void_p, C.c_int]
     lib.mdx_set_kspace_products.argtypes = [C.c_void_p, C.c_int]
+    lib.mdx_set_shake_species.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
+    lib.mdx_set_shake_omega.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.mdx_get_shake_nit.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]
     lib.mdx_set_params.argtypes = [C.c_void_p, C.POINTER(abi.LJ), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     lib.mdx_adjoint_lj.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4

@@ -1413,6 +1413,7 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
   double Ktr = 0;
   double *pp = malloc(sizeof(double) * 3 * ns);
   int maxit0 = -16 * log(p->eps * 0.1), itmax = 0;
+  long long nit = 0;                   /* constrit nit, sim/constrd.c:930 */
   for (n = 0; n < S->nmol; n++) {
     int i0 = S->mol_first[n], i1 = S->mol_first[n + 1];
     int c0 = S->cons_first[n], c1 = S->cons_first[n + 1], nc = c1 - c0, it = 0;
@@ -1458,6 +1459,7 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
     } while (maxsq > p->eps && it < maxit);
     if (nc && it == maxit && maxsq > p->eps) { free(pp); return MDX_E_NOCONV; }
     if (it > itmax) itmax = it;
+    if (nc) nit += it;
     double vm1[3] = {0, 0, 0}, vm2[3] = {0, 0, 0}, mm = 0;
     for (i = i0; i < i1; i++) {        /* shakev3.c */
       double V1[3], V2[3], m = S->mass[i];
@@ -1528,6 +1530,7 @@ static int orc_shake_step(const mdx_system_t *S, double *r, double *r1, const do
   for (k = 0; k < 6; k++) { out->Pkin[k] = Pkin[k]; out->PTvirc[k] = PTvirc[k]; }
   out->virc = virc;
   out->shake_iter_max = itmax;
+  out->shake_nit = nit;
   free(pp);
   return 0;
 }

@@ -75,3 +75,25 @@ def test_berendsen_run_ends_on_cooks_configuration(tmp_path):
     for _ in range(meta["nsteps"]):
         oracle.md_step(bs, r, r1, p, measure=1)
     assert np.abs(r1 - c1["cfgs"][1]["rp"]).max() > 1e-8
+
+
+def test_shake_sweep_count():
+    """the oracle's SHAKE sweep count (cook's constrit nit, sim/constrd.c:
+    930: the sweeps of every molecule with constraints): at least one per
+    molecule and at most shake_iter_max each; another relaxation omega
+    changes it but not the converged positions (within the tolerance)"""
+    d, meta = oracle.load_fixture("spce216_nve")
+    bs = oracle.fixture_system(meta)
+    R = d["r"]
+    p = abi.Step(h=0.002, eps=1e-12, maxit=0, thermostat=0, T=298, tau=0.1, nf=1, measure_every=1)
+    nmol = bs.struct.nmol
+    res = {}
+    for om in (0.6, 0.9):
+        for m in range(nmol):
+            bs.struct.mol_omega[m] = om
+        r, r1 = R[2].copy(), R[2] - R[1]
+        out = oracle.md_step(bs, r, r1, p, measure=1)
+        res[om] = (out.shake_nit, r)
+        assert nmol <= out.shake_nit <= nmol * out.shake_iter_max
+    assert res[0.6][0] != res[0.9][0]
+    assert np.abs(res[0.6][1] - res[0.9][1]).max() < 1e-9
