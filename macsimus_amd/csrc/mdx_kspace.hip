@@ -514,6 +514,25 @@ __global__ __launch_bounds__(kBlock * WPU, WPU == 1 ? 3 : 4) void k_sfac_mf(
 template <int NPART>
 __device__ inline void split16(float x, _Float16 (&p)[NPART]);
 
+typedef _Float16 half2p1 __attribute__((ext_vector_type(2)));
+
+// the three-part split of two values at once by the packed conversion
+// (truncating: the remainder x - x_0 has at most 13 bits and x - x_0 - x_1
+// at most 2, so every remainder is exact in FP32 and the three parts hold
+// all 24 bits).  Two parts use split16 (rounded to nearest: after a
+// truncated first part the second would lose two bits instead of one).
+template <int NPART>
+__device__ inline void split16x2(float x0, float x1, half2p1 (&p)[NPART]) {
+  static_assert(NPART == 3, "the packed split is for three parts");
+  float r0 = x0, r1 = x1;
+#pragma unroll
+  for (int k = 0; k < NPART; k++) {
+    p[k] = __builtin_bit_cast(half2p1, __builtin_amdgcn_cvt_pkrtz(r0, r1));
+    r0 -= (float)p[k][0];
+    r1 -= (float)p[k][1];
+  }
+}
+
 template <int JC>
 struct SfTabH {
   static constexpr int SX = JC + 8;     // floats per X / Y row
@@ -524,7 +543,7 @@ template <int JC, int BLK, int NPART>
 __device__ inline void sf_tables_h(float *Xf, float *Yf, _Float16 *Zh, int KXT, int KYT,
                                    int KZT, int jc, int j1, int nq,
                                    const float2 *__restrict__ qanc,
-                                   const float2 *__restrict__ qbase) {
+                                   const float2 *__restrict__ qbase, float asc) {
   constexpr int SX = SfTabH<JC>::SX, SZ = SfTabH<JC>::SZ;
   const int ntx = (KXT + 7) / 8, nty = (KYT + 7) / 8, ntz = (KZT + 7) / 8, ntask = ntx + nty + ntz;
   for (int t = threadIdx.x; t < JC * ntask; t += BLK) {
@@ -542,9 +561,12 @@ __device__ inline void sf_tables_h(float *Xf, float *Yf, _Float16 *Zh, int KXT, 
     }
     for (int k = 8 * m; k < k1; k++) {
       if (d < 2) {
+        // with three parts X carries the operand scale asc of A = q X Y (a
+        // power of two); with two, the wave scales A itself
         float *dst = (d == 0 ? Xf : Yf) + 2 * k * SX + jj;
-        dst[0] = ec;
-        dst[SX] = es;
+        const float sc = d == 0 && NPART == 3 ? asc : 1.0f;
+        dst[0] = ec * sc;
+        dst[SX] = es * sc;
       } else {
         _Float16 pc[NPART], ps[NPART];
         split16<NPART>(ec * 16384.0f, pc);
@@ -609,7 +631,7 @@ __device__ inline void sf_unit_h(char *smem, size_t set_bytes, int KXT, int KYT,
     float *X, *Y;
     _Float16 *Z;
     tabs(0, X, Y, Z);
-    sf_tables_h<JC, BLK, NPART>(X, Y, Z, KXT, KYT, KZT, j0, j1, nq, qanc, qbase);
+    sf_tables_h<JC, BLK, NPART>(X, Y, Z, KXT, KYT, KZT, j0, j1, nq, qanc, qbase, asc);
     __syncthreads();
   }
   for (int jc = j0; jc < j1; jc += JC) {
@@ -624,10 +646,10 @@ __device__ inline void sf_unit_h(char *smem, size_t set_bytes, int KXT, int KYT,
         float *X, *Y;
         _Float16 *Z;
         tabs(set ^ 1, X, Y, Z);
-        sf_tables_h<JC, BLK, NPART>(X, Y, Z, KXT, KYT, KZT, jc + JC, j1, nq, qanc, qbase);
+        sf_tables_h<JC, BLK, NPART>(X, Y, Z, KXT, KYT, KZT, jc + JC, j1, nq, qanc, qbase, asc);
       }
     } else {
-      sf_tables_h<JC, BLK, NPART>(Xc, Yc, Zc, KXT, KYT, KZT, jc, j1, nq, qanc, qbase);
+      sf_tables_h<JC, BLK, NPART>(Xc, Yc, Zc, KXT, KYT, KZT, jc, j1, nq, qanc, qbase, asc);
       __syncthreads();
     }
     if (NA > 0) {
@@ -639,12 +661,22 @@ __device__ inline void sf_unit_h(char *smem, size_t set_bytes, int KXT, int KYT,
           const float4 y0 = *(const float4 *)(Yf + yo[m] + s), y1 = *(const float4 *)(Yf + yo[m] + s + 4);
           const float v[8] = {x0.x * y0.x, x0.y * y0.y, x0.z * y0.z, x0.w * y0.w,
                               x1.x * y1.x, x1.y * y1.y, x1.z * y1.z, x1.w * y1.w};
+          if constexpr (NPART == 3) {
 #pragma unroll
-          for (int i = 0; i < 8; i++) {
-            _Float16 pp[NPART];
-            split16<NPART>(v[i] * asc, pp);
+            for (int i = 0; i < 8; i += 2) {
+              half2p1 pp[NPART];
+              split16x2<NPART>(v[i], v[i + 1], pp);
 #pragma unroll
-            for (int q = 0; q < NPART; q++) a[m][q][i] = pp[q];
+              for (int q = 0; q < NPART; q++) { a[m][q][i] = pp[q][0]; a[m][q][i + 1] = pp[q][1]; }
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+              _Float16 pp[NPART];
+              split16<NPART>(v[i] * asc, pp);
+#pragma unroll
+              for (int q = 0; q < NPART; q++) a[m][q][i] = pp[q];
+            }
           }
         }
 #pragma unroll
@@ -1298,14 +1330,16 @@ __global__ __launch_bounds__(kBlock, NTC == 1 ? 3 : 2) void k_kforce_h(
     const int gi = grp_start[kx] + (ky0 >> 2);
     const int nb = min(grp_nb8[gi] - ch * BPC, BPC);   // wave-uniform, >= 1
     const half8 *ga = stage[buf] + lane;
-    // the leading product (0, 0) of each block into acc, the smaller ones
-    // into acl (the instruction drops low bits of addends far below its
-    // accumulator, so terms of like size share one)
-    mfma_f32x16 acc[NTC], acl[NTC];
+    // per 8-kz block the smaller products first, then the leading one
+    // (0, 0): a group's sums run over at most KZC / 8 blocks, short enough
+    // that one accumulator costs no accuracy to measure (force error 4.07e-7
+    // against 4.06e-7 with the leading products apart; pass 1 sums over
+    // thousands of charges and keeps them apart)
+    mfma_f32x16 acc[NTC];
 #pragma unroll
     for (int u = 0; u < NTC; u++)
 #pragma unroll
-      for (int r = 0; r < 16; r++) { acc[u][r] = 0.0f; acl[u][r] = 0.0f; }
+      for (int r = 0; r < 16; r++) acc[u][r] = 0.0f;
 #pragma unroll
     for (int b = 0; b < BPC; b++) {
       if (b < nb) {
@@ -1315,20 +1349,16 @@ __global__ __launch_bounds__(kBlock, NTC == 1 ? 3 : 2) void k_kforce_h(
 #pragma unroll
         for (int u = 0; u < NTC; u++) {
           if constexpr (NPART == 3) {
-            acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[2], zt[u][b][0], acl[u], 0, 0, 0);
-            acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][1], acl[u], 0, 0, 0);
-            acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][2], acl[u], 0, 0, 0);
+            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[2], zt[u][b][0], acc[u], 0, 0, 0);
+            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][1], acc[u], 0, 0, 0);
+            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][2], acc[u], 0, 0, 0);
           }
-          acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][0], acl[u], 0, 0, 0);
-          acl[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][1], acl[u], 0, 0, 0);
+          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][0], acc[u], 0, 0, 0);
+          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][1], acc[u], 0, 0, 0);
           acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][0], acc[u], 0, 0, 0);
         }
       }
     }
-#pragma unroll
-    for (int u = 0; u < NTC; u++)
-#pragma unroll
-      for (int r = 0; r < 16; r++) acc[u][r] += acl[u][r];
     // the group's scale goes into the row phases
     const float sc = gscale[gi] * (1.0f / 256.0f);
 #pragma unroll
