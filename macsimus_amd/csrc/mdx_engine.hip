@@ -741,8 +741,13 @@ constexpr int kMaxSplineLds = 48 * 1024;   // bytes of force splines staged in L
 constexpr int kMaxLjLds = 4 * 1024;        // bytes of the type-pair table staged in LDS
 constexpr int kMaxZ = 64;      // buffer cells of a round (z cells + 2w)
 
-template <typename R, bool MEASURE, bool POLAR, bool E6>
-__global__ __launch_bounds__(kPairBlock) void k_pairc(
+// ST: unmeasured FP32 forces without Drudes, with the type-pair table and
+// the force splines staged in LDS (the launch guarantees both): the pair
+// terms of pair_body in the same arithmetic, with selects in place of its
+// branches (the lanes of a wave take different branches) and integer
+// census counts.
+template <typename R, bool MEASURE, bool POLAR, bool E6, bool ST>
+__global__ __launch_bounds__(kPairBlock, (POLAR || MEASURE || sizeof(R) == 8) ? 1 : 4) void k_pairc(
     int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
     const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
@@ -781,8 +786,11 @@ __global__ __launch_bounds__(kPairBlock) void k_pairc(
   const int span = 2 * w + 1, col0 = blockIdx.y * cpp, col1 = min(col0 + cpp, span * span);
   const int ib0 = cstart[c0], ib1 = cstart[c0 + g.nz];
   const int lane = threadIdx.x & 63;
+  static_assert(!ST || (sizeof(R) == 4 && !MEASURE && !POLAR), "ST: unmeasured FP32 atoms");
   double U = 0, Eel = 0, Vir = 0, pv[6] = {0, 0, 0, 0, 0, 0};
   double npair = 0, nlj = 0;
+  int inpair = 0, inlj = 0;   // ST counts
+  const lds_float *stsl = (const lds_float *)tab.sl;   // ST: staged splines
   for (int rb = ib0; rb < ib1; rb += kPairBlock) {
     const int s = rb + threadIdx.x;
     const bool act = s < ib1;
@@ -793,6 +801,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairc(
     const int4 ii = sinfo[sl];
     const R qi = (R)pi.w, qpi = (R)ri.w;
     const R *ljrow = tab.lj + (size_t)(ii.x & 0xffff) * nt * 8;
+    const lds_vf4 *stlj = (const lds_vf4 *)(const float *)ljrow;   // ST: staged table row
     const int czi = scell[sl] - c0;
     // z cells of the round and its buffer cells z0-w .. z1+w
     const int z0 = scell[rb] - c0, z1 = scell[min(rb + kPairBlock, ib1) - 1] - c0;
@@ -827,18 +836,24 @@ __global__ __launch_bounds__(kPairBlock) void k_pairc(
         if (lane == 0) joff[0] = 0;
       }
       __syncthreads();
-      const bool lds = joff[nzb] <= kMaxJ;
+      const int ntot = joff[nzb];
+      const bool lds = ntot <= kMaxJ;
       if (lds) {
-        if (threadIdx.x < 32) jpos[joff[nzb] + threadIdx.x] = make_double4(1e30, 1e30, 1e30, 0);
-        for (int k = 0; k < nzb; k++) {
-          const int n = joff[k + 1] - joff[k], src = jsrc[k], dst = joff[k];
-          const double sz = jsz[k];
-          for (int m = threadIdx.x; m < n; m += kPairBlock) {
-            const double4 p = spos[src + m];
-            jpos[dst + m] = make_double4(p.x + sx, p.y + sy, p.z + sz, p.w);
-            jinf[dst + m] = sinfo[src + m];
-            if (POLAR) jpol[dst + m] = spol[src + m];
-          }
+        if (threadIdx.x < 32) jpos[ntot + threadIdx.x] = make_double4(1e30, 1e30, 1e30, 0);
+        // all buffer cells at once: staged site m belongs to the last
+        // buffer cell k with joff[k] <= m (a search over at most 64 offsets;
+        // empty cells share their offset with the next one)
+#pragma unroll 1
+        for (int m = threadIdx.x; m < ntot; m += kPairBlock) {
+          int k = 0;
+#pragma unroll
+          for (int step = kMaxZ / 2; step > 0; step >>= 1)
+            if (k + step < nzb && joff[k + step] <= m) k += step;
+          const int src = jsrc[k] + (m - joff[k]);
+          const double4 p = spos[src];
+          jpos[m] = make_double4(p.x + sx, p.y + sy, p.z + jsz[k], p.w);
+          jinf[m] = sinfo[src];
+          if (POLAR) jpol[m] = spol[src];
         }
       }
       __syncthreads();
@@ -876,6 +891,35 @@ __global__ __launch_bounds__(kPairBlock) void k_pairc(
               const int slot = atomicAdd(bcount, 1);
               if (slot < bcap) blist[slot] = make_int4(ii.w, jj.w, rr < cq, 0);
               if (rr >= cq) continue;
+            }
+            if constexpr (ST) {
+              inpair++;
+              const bool ex = jj.y == ii.y && ((ii.z >> (jj.x >> 16)) & 1);   // EXCL
+              const int tj = jj.x & 0xffff;
+              const vf4 sa = stlj[2 * tj], sb = stlj[2 * tj + 1];
+              float fin;
+              if constexpr (E6) {
+                const float q = (float)rr, x = sqrt(q), y = exp(sa.x * x), iz = rcp32(q * q * q);
+                fin = sa.z * y * rcp32(x) - 6.0f * sa.w * iz * rcp32(q);
+              } else {
+                const float x = sa.x * rcp32((float)rr), y = x * x * x;
+                fin = sa.w * y * (y - 0.5f) * x;
+              }
+              float fout = sb.w * (float)(rr - (double)sb.y);
+              // both forms in registers before the selects (no branch)
+              asm volatile("" : "+v"(fin), "+v"(fout));
+              const bool in5 = rr < (double)sb.y;
+              float fr = rr < (double)sb.x ? fin : (in5 ? fout : 0.0f);
+              inlj += (!ex && in5 && sb.z != 0) ? 1 : 0;
+              const int k = (int)(rr * sgrid);
+              const float t = (float)(rr - k * tab.isgrid);
+              const float erd = stsl[k] - stsl[tab.nsl + k] * t * rcp32(stsl[2 * tab.nsl + k] + t);
+              fr += (float)qi * (float)pj.w * erd;
+              fr = ex ? 0.0f : fr;
+              ac[0] += fr * (float)dxr;
+              ac[1] += fr * (float)dyr;
+              ac[2] += fr * (float)dzr;
+              continue;
             }
             npair += 1;
             double4 rj = make_double4(0, 0, 0, 0);
@@ -926,6 +970,7 @@ __global__ __launch_bounds__(kPairBlock) void k_pairc(
     }
   }
   const size_t row = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (ST) { npair += inpair; nlj += inlj; }
   if (MEASURE) {
     double v[12] = {0.5 * U, 0.5 * Eel, 0.5 * Vir, 0.5 * pv[0], 0.5 * pv[1], 0.5 * pv[2],
                     0.5 * pv[3], 0.5 * pv[4], 0.5 * pv[5], 0.5 * npair, 0.5 * nlj, 0};
@@ -1346,16 +1391,23 @@ int launch_pair_t(Ctx &c, int measure) {
           shm += (size_t)nsl * 12;
         }
       }
-      if (measure)
-        k_pairc<R, true, POLAR, E6><<<gc, kPairBlock, shm, c.stream>>>(
-            c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
-            (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-            c.part.p, band, (int4 *)c.blist.p, c.bcount.p, c.bcap);
-      else
-        k_pairc<R, false, POLAR, E6><<<gc, kPairBlock, shm, c.stream>>>(
-            c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
-            (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,
-            c.part.p, band, (int4 *)c.blist.p, c.bcount.p, c.bcap);
+#define MDX_PAIRC(M, S)                                                                     \
+  k_pairc<R, M, POLAR, E6, S><<<gc, kPairBlock, shm, c.stream>>>(                            \
+      c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,          \
+      (const int4 *)c.sinfo.p, c.scan_tmp.p, c.cstart.p, c.ntypes, tab, c.erf_sgrid, out,      \
+      c.part.p, band, (int4 *)c.blist.p, c.bcount.p, c.bcap)
+      if (measure) {
+        MDX_PAIRC(true, false);
+      } else {
+        bool st = false;
+        if constexpr (sizeof(R) == 4 && !POLAR) st = tab.nsl > 0 && tab.ljstage > 0;
+        if (st) {
+          if constexpr (sizeof(R) == 4 && !POLAR) MDX_PAIRC(false, true);
+        } else {
+          MDX_PAIRC(false, false);
+        }
+      }
+#undef MDX_PAIRC
     } else if (measure)
       k_pair<R, true, POLAR, E6><<<grid, kBlock, 0, c.stream>>>(
           c.ns, g, c.wst, cpp, cq, (const double4 *)c.spos.p, (const double4 *)c.spol.p,
@@ -1999,11 +2051,14 @@ extern "C" int mdx_autotune(mdx_ctx_t *h, const double *r, int reps, int *choice
     if (!rc) rc = kspace_chunks(c, best_k, best_p);
     if (!rc && c.mode == MDX_MIXED) {
       // pass 1: FP32 FMA (0), matrix-core row units of one or two waves
-      // (1, 2) or matrix cores with 16 or 8 tiles per wave
+      // (1, 2) or matrix cores with 16 or 8 tiles per wave; the split-FP16
+      // units (4..7) not for Drude systems, whose centre and Drude charges
+      // cancel inside the FP16 instruction's sums (force error near 1e-5)
       int best_nt = c.p1nt;
       double t1 = 1e30;
       for (int nt : {0, 1, 2, 4, 5, 6, 7, 16, 8}) {
         if ((nt == 1 || nt == 2 || (nt >= 4 && nt <= 7)) && c.K.nblk > 8) continue;
+        if (nt >= 4 && nt <= 7 && c.polar) continue;
         c.p1nt = nt;
         if ((rc = run(fam))) break;
         if (fam[T_KPASS1] < t1) { t1 = fam[T_KPASS1]; best_nt = nt; }
