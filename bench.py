@@ -218,6 +218,9 @@ def bench_replicas(a):
     work = {"kpass1": 8.0 * qk, "kpass2": 12.0 * qk, "pair": 24.0 * npair + 7.0 * nlj}
     dom = max(("kpass1", "kpass2", "pair"), key=lambda k: kern.get(k, (0, 0))[0])
     peak = PEAK["fp64_valu"] if mode == abi.MDX_FP64 else PEAK["fp32_valu"]
+    if dom == "kpass2" and info["kmfma"]:
+        # FP32 matrix cores, or 6 FP16 products per FP32 product
+        peak = PEAK["fp32_mfma"] if info["kmfma"] == 1 else PEAK["fp16_mfma"] / 6
     achieved = work[dom] / (kern[dom][0] * 1e-3)
     force_ms = sum(kern[k][0] for k in ("bin", "pair", "kpass1", "kcoef", "kpass2") if k in kern)
 
@@ -255,7 +258,8 @@ def bench_replicas(a):
                    "charges": info["charges"], "k_segments": info["kseg"], "pairs": npair,
                    "timestep_ps": h, "precision": a.mode, "parallelism": "1 GPU",
                    "tuned": {"stencil_w": w, "kz_chunk": kzc, "k_parts": kparts,
-                             "kspace_pass2_mfma": bool(info["kmfma"]), "force_ms": tune_ms}},
+                             # 0 FP32 FMA, 1 FP32 matrix cores, 2 split-FP16 matrix cores
+                             "kspace_pass2_mfma": int(info["kmfma"]), "force_ms": tune_ms}},
         "us_per_force": force_ms * 1e3,
         "replica_steps_per_s": nsys / (ms_step * 1e-3),
         "atom_steps_per_s": info["sites"] / (ms_step * 1e-3),

@@ -545,8 +545,10 @@ int mdx_batch_set_stencil(mdx_batch_t *b, int w);
    about kparts parts (1..256) for the largest k-set, the others in
    proportion to their k-sets */
 int mdx_batch_set_kspace(mdx_batch_t *b, int kzc, int kparts);
-/* MDX_MIXED: k-space pass 2 on the matrix cores (as mdx_set_kspace_mfma);
-   mdx_batch_autotune tries both.  Default: on in MDX_MIXED. */
+/* MDX_MIXED: k-space pass 2 on the matrix cores: on = 1 with FP32
+   operands (as mdx_set_kspace_mfma), on = 2 with split-FP16 operands of
+   3 parts and 6 products per FP32 product (as mdx_set_kspace_products 6),
+   0 FP32 FMA; mdx_batch_autotune tries all three.  Default: 2 in MDX_MIXED. */
 int mdx_batch_set_kspace_mfma(mdx_batch_t *b, int on);
 /* times the force call on r (HIP events): the stencil w = 1..3, then kz
    chunk x parts; keeps the fastest; choice[3] = {w, kz chunk, parts},

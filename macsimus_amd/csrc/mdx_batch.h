@@ -39,7 +39,8 @@ struct KSys {
 struct BK {
   int nks = 0, nqt = 0, nseg = 0, n1 = 0, n2 = 0, n2m = 0, nkcb = 0, kzc = 16, kparts = 8;
   // MDX_MIXED: pass 2 on the matrix cores (units of 128 charges, units2m)
-  bool kmfma = false;
+  // with FP32 operands (1) or split-FP16 operands of 3 parts (2)
+  int kmfma = 0;
   int shmem_max = 0;         // pass-1 LDS tables of the largest system
   long ncoef = 0;
   std::vector<KSys> hks;
@@ -51,6 +52,12 @@ struct BK {
   DBuf<int> units1, units2, units2m, parts;  // int4 each
   DBuf<int> grp_start, grp_base, grp_nb;     // row groups of all systems
   DBuf<float> coefg;
+  // kmfma 2: the groups' split-FP16 coefficient blocks (global slots
+  // grp_hb, grp_nb8 blocks of 8 kz each) and per-group scales
+  DBuf<int> grp_hb, grp_nb8;
+  DBuf<uint16_t> coefh;
+  DBuf<float> gscale;
+  int ngrp = 0;
   DBuf<int> lex_row0, lex_meta, chunk_kymax;
   DBuf<double> kpart, tsum, kpf, kcpart;
   DBuf<double> coef64;

@@ -514,7 +514,7 @@ extern "C" int mdx_batch_create(mdx_batch_t **out, int nsys, const mdx_system_t 
   auto *h = new mdx_batch_s;
   Batch &b = h->b;
   b.mode = mode;
-  b.k.kmfma = mode == MDX_MIXED;      // pass 2 on the matrix cores by default
+  b.k.kmfma = mode == MDX_MIXED ? 2 : 0;   // pass 2 on the split-FP16 matrix cores by default
   b.nsys = nsys;
   auto bail = [&](int rc) { release(b); delete h; return rc; };
   (void)hipGetDevice(&b.device);
@@ -842,10 +842,11 @@ extern "C" int mdx_batch_set_stencil(mdx_batch_t *h, int w) {
 
 extern "C" int mdx_batch_set_kspace_mfma(mdx_batch_t *h, int on) {
   if (!h) return fail(MDX_E_ARG, "mdx_batch_set_kspace_mfma: NULL");
+  if (on < 0 || on > 2) return fail(MDX_E_ARG, "mdx_batch_set_kspace_mfma: 0, 1 or 2");
   if (on && h->b.mode != MDX_MIXED) return fail(MDX_E_ARG, "mdx_batch_set_kspace_mfma: MDX_MIXED only");
   MDX_HIP(hipSetDevice(h->b.device));
   MDX_HIP(hipStreamSynchronize(h->b.stream));
-  h->b.k.kmfma = on != 0;
+  h->b.k.kmfma = on;
   h->b.graph_ok = false;
   return 0;
 }
@@ -898,10 +899,10 @@ extern "C" int mdx_batch_autotune(mdx_batch_t *h, const double *r, int *choice, 
   }
   if (!rc) rc = layout_cells(b, bw);
   const int kzs[4] = {8, 16, 24, 32}, kps[6] = {1, 2, 4, 8, 16, 32};
-  bool bm = b.k.kmfma;
-  // MDX_MIXED: the FP32 FMA and the matrix-core variants of pass 2
-  for (int mf = 0; mf < (b.mode == MDX_MIXED ? 2 : 1) && !rc && b.k.nks; mf++) {
-    b.k.kmfma = mf != 0;
+  int bm = b.k.kmfma;
+  // MDX_MIXED: the FP32 FMA and the two matrix-core variants of pass 2
+  for (int mf = 0; mf < (b.mode == MDX_MIXED ? 3 : 1) && !rc && b.k.nks; mf++) {
+    b.k.kmfma = mf;
     for (int iz = 0; iz < 4 && !rc; iz++)
       for (int ip = 0; ip < 6 && !rc; ip++) {
         if ((rc = bk_setup(b, kzs[iz], kps[ip])) || (rc = time_forces(b, e0, e1, &t))) break;
@@ -926,7 +927,7 @@ extern "C" int mdx_batch_info(mdx_batch_t *h, int *iv, int n) {
   const Batch &b = h->b;
   const BK &k = b.k;
   const int v[14] = {b.nsys, b.nreal, b.npad, b.ncell, k.nks, b.mode, b.hs.empty() ? 0 : b.hs[0].w,
-                     k.nqt, k.nseg, k.n1, k.kmfma ? k.n2m : k.n2, k.kzc, k.kparts, k.kmfma ? 1 : 0};
+                     k.nqt, k.nseg, k.n1, k.kmfma ? k.n2m : k.n2, k.kzc, k.kparts, k.kmfma};
   for (int i = 0; i < n && i < 14; i++) iv[i] = v[i];
   return 0;
 }

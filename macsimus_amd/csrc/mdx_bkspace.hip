@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "mdx_batch.h"
+#include "mdx_ksplit.h"
 
 using namespace mdx;
 using namespace mdx::batch;
@@ -494,6 +495,25 @@ __global__ __launch_bounds__(kBlock, 4) void k_bk_kforce_mfma(
   o[2] = Fz;
 }
 
+// Pass 2 with split-FP16 operands (kmfma 2): kforce_h_body of mdx_ksplit.h
+// with the unit's system descriptor, one tile of 32 charges per wave;
+// unit = (system, block of 128 of its charges, part)
+template <int KZC, int NPART>
+__global__ __launch_bounds__(kBlock, 3) void k_bk_kforce_h(
+    const int4 *__restrict__ units, const KSys *__restrict__ ks, const double4 *__restrict__ qpos,
+    const int4 *__restrict__ parts, const int *__restrict__ chunk_kymax,
+    const int *__restrict__ grp_start, const int *__restrict__ grp_hb,
+    const int *__restrict__ grp_nb8, const half8 *__restrict__ coefh,
+    const float *__restrict__ gscale, double *__restrict__ kpf) {
+  const int4 u = units[blockIdx.x];
+  const KSys &K = ks[u.x];
+  const int4 part = parts[u.z];
+  kforce_h_body<KZC, 1, NPART>(K.nq, qpos + K.q_off, K.iL, K.iL, K.iL, K.kxmax, part, u.y,
+                               chunk_kymax + K.chunk_off, grp_start + K.grps_off, grp_hb,
+                               grp_nb8, coefh, gscale,
+                               kpf + ((size_t)K.kpf_off + (size_t)part.w * K.nq) * 3);
+}
+
 // sums a charge's parts in order and adds -q grad Psi and the dipole field
 __global__ void k_bk_kforce_sum(int nqt, const KSys *__restrict__ ks, const int *__restrict__ qks,
                                 const int *__restrict__ qrow, const double4 *__restrict__ qpos,
@@ -548,7 +568,26 @@ int run(Batch &b, int measure) {
   if (b.timing) { b.tm.stop(T_KCOEF, b.stream); b.tm.start(T_KPASS2, b.stream); }
   bool done2 = false;
   if constexpr (sizeof(R) == 4) {
-    if (k.nks && k.kmfma) {
+    if (k.nks && k.kmfma == 2) {
+      // the groups' coefficients as 3 FP16 parts, then the contraction
+      k_coefh<3><<<k.ngrp, kBlock, 0, b.stream>>>(k.grp_base.p, k.grp_nb.p, k.grp_hb.p, k.grp_nb8.p,
+                                                k.coefg.p, (half8 *)k.coefh.p, k.gscale.p);
+#define MDX_BKH(KZV)                                                                          \
+  k_bk_kforce_h<KZV, 3><<<k.n2m, kBlock, 0, b.stream>>>(                                     \
+      (const int4 *)k.units2m.p, ks, qpos, (const int4 *)k.parts.p, k.chunk_kymax.p,          \
+      k.grp_start.p, k.grp_hb.p, k.grp_nb8.p, (const half8 *)k.coefh.p, k.gscale.p, k.kpf.p)
+      switch (k.kzc) {
+        case 8: MDX_BKH(8); break;
+        case 16: MDX_BKH(16); break;
+        case 24: MDX_BKH(24); break;
+        case 32: MDX_BKH(32); break;
+        default: return fail(MDX_E_ARG, "batch k-space: kz chunk must be 8, 16, 24 or 32");
+      }
+#undef MDX_BKH
+      k_bk_kforce_sum<<<nblk(k.nqt), kBlock, 0, b.stream>>>(k.nqt, ks, k.qks.p, k.qrow.p, qpos,
+                                                             k.kpf.p, b.sacc.p, b.f.p);
+      done2 = true;
+    } else if (k.nks && k.kmfma) {
 #define MDX_BKM(KZV)                                                                          \
   k_bk_kforce_mfma<KZV><<<k.n2m, kBlock, 0, b.stream>>>((const int4 *)k.units2m.p, ks, qpos,  \
                                                        (const int4 *)k.parts.p,              \
@@ -599,12 +638,14 @@ void BK::release() {
   ks.release();
   DBuf<int> *n[] = {&qrow, &qks, &row_kx, &row_ky, &row_kzmax, &row_coef0, &seg_blk, &seg_row0,
                     &seg_rowend, &seg_ks, &units1, &units2, &units2m, &parts, &lex_row0, &lex_meta,
-                    &chunk_kymax, &grp_start, &grp_base, &grp_nb};
+                    &chunk_kymax, &grp_start, &grp_base, &grp_nb, &grp_hb, &grp_nb8};
   for (auto *x : n) x->release();
   DBuf<double> *d[] = {&qpos, &kpart, &tsum, &kpf, &kcpart, &coef64};
   for (auto *x : d) x->release();
   coef32.release();
   coefg.release();
+  coefh.release();
+  gscale.release();
 }
 
 int bk_setup(Batch &b, int kzc, int kparts) {
@@ -615,8 +656,8 @@ int bk_setup(Batch &b, int kzc, int kparts) {
   std::vector<KSys> hks;
   std::vector<int> qrow, qks, row_kx, row_ky, row_kzmax, row_coef0, seg_blk, seg_row0, seg_rowend,
       seg_ks, units1, units2, units2m, parts, lex_row0, lex_meta, chunk_kymax, grp_start, grp_base,
-      grp_nb;
-  long grp_floats = 0;
+      grp_nb, grp_hb, grp_nb8;
+  long grp_floats = 0, grp_h8 = 0;
   std::vector<double> cost1;
   std::vector<KLayout> lay;
   // the k-sets first: the largest sets the parts of the others
@@ -711,6 +752,9 @@ int bk_setup(Batch &b, int kzc, int kparts) {
     for (int v : K.grp_base) grp_base.push_back((int)(v + grp_floats));
     grp_nb.insert(grp_nb.end(), K.grp_nb.begin(), K.grp_nb.end());
     grp_floats += K.grp_floats;
+    for (int v : K.grp_hb) grp_hb.push_back((int)(v + grp_h8));
+    grp_nb8.insert(grp_nb8.end(), K.grp_nb8.begin(), K.grp_nb8.end());
+    grp_h8 += K.grp_h8;
     x.kcb0 = 2 * nseg;
     x.kcb1 = 2 * (nseg + K.nseg);
     nrows += K.nrows;
@@ -739,6 +783,7 @@ int bk_setup(Batch &b, int kzc, int kparts) {
   k.n1 = (int)units1.size() / 4;
   k.n2 = (int)units2.size() / 4;
   k.n2m = (int)units2m.size() / 4;
+  k.ngrp = (int)grp_hb.size();
   k.nkcb = 2 * nseg;
   k.kzc = kzc;
   k.kparts = kparts;
@@ -765,8 +810,12 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       (rc = k.kpf.alloc(one((size_t)kpf_off * 3))) || (rc = k.kcpart.alloc(one((size_t)k.nkcb * 7))))
     return rc;
   if (b.mode == MDX_MIXED) {
-    if ((rc = k.coef32.alloc(one((size_t)ncoef * 16))) || (rc = k.coefg.alloc(one((size_t)grp_floats))))
+    if ((rc = k.coef32.alloc(one((size_t)ncoef * 16))) || (rc = k.coefg.alloc(one((size_t)grp_floats))) ||
+        (rc = k.coefh.alloc(one((size_t)grp_h8 * 3 * 512))) ||
+        (rc = k.gscale.alloc(one(grp_hb.size()))) || (rc = k.grp_hb.alloc(one(grp_hb.size()))) ||
+        (rc = k.grp_nb8.alloc(one(grp_nb8.size()))))
       return rc;
+    if (grp_h8 * 3 * 512 >= (1L << 31)) return fail(MDX_E_ARG, "batch k-space: too many row groups");
     MDX_HIP(hipMemset(k.coef32.p, 0, sizeof(float) * one((size_t)ncoef * 16)));
     MDX_HIP(hipMemset(k.coefg.p, 0, sizeof(float) * one((size_t)grp_floats)));
   } else {
@@ -784,6 +833,7 @@ int bk_setup(Batch &b, int kzc, int kparts) {
       !upi(k.seg_rowend, seg_rowend) || !upi(k.seg_ks, seg_ks) || !upi(k.units1, units1) ||
       !upi(k.units2, units2) || !upi(k.units2m, units2m) || !upi(k.grp_start, grp_start) ||
       !upi(k.grp_base, grp_base) || !upi(k.grp_nb, grp_nb) ||
+      (b.mode == MDX_MIXED && (!upi(k.grp_hb, grp_hb) || !upi(k.grp_nb8, grp_nb8))) ||
       !upi(k.parts, parts) || !upi(k.lex_row0, lex_row0) ||
       !upi(k.lex_meta, lex_meta) || !upi(k.chunk_kymax, chunk_kymax))
     return fail(MDX_E_GPU, "batch k-space: upload failed");

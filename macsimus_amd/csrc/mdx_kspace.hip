@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "mdx_internal.h"
+#include "mdx_ksplit.h"
 
 namespace {
 
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(kBlock * WPU, WPU == 1 ? 3 : 4) void k_sfac_mf(
 #undef MDX_U
 }
 
-// Pass 1 with split-FP16 operands (split16 below): the GEMM of k_sfac_mf
+// Pass 1 with split-FP16 operands (split16, mdx_ksplit.h): the GEMM of k_sfac_mf
 // on the FP16 instruction v_mfma_f32_16x16x32_f16 (K = 32 charges), one
 // wave per unit of 8 rows with all its N tiles, U units per workgroup.
 // Per chunk of JC charges the workgroup builds the cos and sin rows of X
@@ -1150,84 +1151,8 @@ __global__ __launch_bounds__(kBlock, NTC == 1 ? 4 : 3) void k_kforce_mfma(
   }
 }
 
-// ------------------------------------------- pass 2, split-FP16 operands
-// Each FP32 operand value x is written as the sum of NPART FP16 parts,
-// x_0 = fp16(x), x_1 = fp16(x - x_0), x_2 = fp16(x - x_0 - x_1) (the
-// remainders are exact in FP32); NPART = 3 holds all 24 bits of x.  The
-// product x y is then the sum of x_a y_b over a + b < NPART: the terms
-// left out are of relative size 2^-22 (NPART = 2) or 2^-33 (NPART = 3).
-// A product of two FP16 values is exact in FP32, so the FP16 matrix
-// instruction v_mfma_f32_32x32x16_f16 with FP32 accumulation forms these
-// sums at 16 times the FP32 rate.  So that the parts stay in FP16's
-// normal range, the coefficients of a row group are scaled by 2^(14-e),
-// with e the exponent of their largest magnitude, and the phases by 2^8;
-// the epilogue scales the sums back (powers of two: exact).
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-
-template <int NPART>
-__device__ inline void split16(float x, _Float16 (&p)[NPART]) {
-  float r = x;
-#pragma unroll
-  for (int k = 0; k < NPART; k++) {
-    p[k] = (_Float16)r;
-    r -= (float)p[k];
-  }
-}
-
-// the coefficients of a row group (coefg, FP32, operand order of
-// k_kforce_mfma) as NPART FP16 parts in 8-kz blocks: slot b of group gi,
-// part k, lane l holds kz 8b..8b+7 of coefg lane l (M index l % 32, cos or
-// sin by l / 32), zeros past the group's kz slots.  One workgroup per
-// group; gscale[gi] = 2^(e-14) undoes the group's scale.
-template <int NPART>
-__global__ __launch_bounds__(kBlock) void k_coefh(const int *__restrict__ grp_base,
-                                                  const int *__restrict__ grp_nb,
-                                                  const int *__restrict__ grp_hb,
-                                                  const int *__restrict__ grp_nb8,
-                                                  const float *__restrict__ coefg,
-                                                  half8 *__restrict__ coefh,
-                                                  float *__restrict__ gscale) {
-  const int gi = blockIdx.x;
-  const float *src = coefg + grp_base[gi];
-  const int nkz = grp_nb[gi] * kKB;
-  float m = 0;
-  for (int i = threadIdx.x; i < nkz * 64; i += kBlock) m = fmaxf(m, fabsf(src[i]));
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-  __shared__ float red[kBlock / 64];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  m = red[0];
-  for (int w = 1; w < kBlock / 64; w++) m = fmaxf(m, red[w]);
-  const int e = m > 0 ? ilogbf(m) : 0;           // m in [2^e, 2^(e+1))
-  const float sc = ldexpf(1.0f, 14 - e);
-  if (threadIdx.x == 0) gscale[gi] = ldexpf(1.0f, e - 14);
-  const int nb8 = grp_nb8[gi];
-  half8 *dst = coefh + (size_t)grp_hb[gi] * NPART * 64;
-  for (int t = threadIdx.x; t < nb8 * 64; t += kBlock) {
-    const int b = t >> 6, l = t & 63;
-    half8 v[NPART];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int kz = 8 * b + i;
-      _Float16 p[NPART];
-      split16<NPART>(kz < nkz ? src[(size_t)kz * 64 + l] * sc : 0.0f, p);
-#pragma unroll
-      for (int k = 0; k < NPART; k++) v[k][i] = p[k];
-    }
-#pragma unroll
-    for (int k = 0; k < NPART; k++) dst[((size_t)b * NPART + k) * 64 + l] = v[k];
-  }
-}
-
-// Pass 2 of k_kforce_mfma with split-FP16 operands.  Same work split, row
-// groups, lane roles and epilogue; the K index of the instruction is (cos
-// or sin by the lane half, 8 kz), so lane l supplies 8 kz of one M index's
-// coefficients and 8 kz of its charge's cos (h = 0) or sin (h = 1) of c,
-// and one instruction per product of parts covers 8 kz of the chunk.  The
-// four waves of a workgroup take different charges through the same row
-// groups, so each group's coefficient blocks are loaded once per
-// workgroup into LDS (double buffered: the next group's loads are in
-// flight while this group's products run; one barrier per group).
+// Pass 2 with split-FP16 operands (kforce_h_body, mdx_ksplit.h): grid
+// (blocks of 32 NTC kBlock / 64 charges, parts)
 template <int KZC, int NTC, int NPART>
 __global__ __launch_bounds__(kBlock, NTC == 1 ? 3 : 2) void k_kforce_h(
     int nq, const double4 *__restrict__ qpos, double iLx, double iLy, double iLz, int kxmax,
@@ -1235,190 +1160,9 @@ __global__ __launch_bounds__(kBlock, NTC == 1 ? 3 : 2) void k_kforce_h(
     const int *__restrict__ grp_start, const int *__restrict__ grp_hb,
     const int *__restrict__ grp_nb8, const half8 *__restrict__ coefh,
     const float *__restrict__ gscale, double *__restrict__ kpf) {
-  constexpr int BPC = KZC / 8;         // 8-kz blocks per chunk
-  constexpr int NV = BPC * NPART * 64;  // operand vectors of a group's chunk
-  constexpr int NL = (NV + kBlock - 1) / kBlock;   // of them per thread
-  __shared__ half8 stage[2][NV];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int c = lane & 31, h = lane >> 5;
-  const int jw = (blockIdx.x * (kBlock / 64) + wv) * 32 * NTC + c;
-  const int4 part = parts[blockIdx.y];
-  const int ch = part.x;
-  const float kxs = (float)iLx, kys = (float)iLy;
-  double sx[NTC], cb1[NTC], sb1[NTC], cb2[NTC], sb2[NTC], cb4[NTC], sb4[NTC];
-  half8 zt[NTC][BPC][NPART];
-#pragma unroll
-  for (int u = 0; u < NTC; u++) {
-    const double4 qp = qpos[min(jw + 32 * u, nq - 1)];
-    sx[u] = frac1(qp.x * iLx);
-    const double sy = frac1(qp.y * iLy), sz = frac1(qp.z * iLz);
-    cs2pi(sy, cb1[u], sb1[u]);
-    cb2[u] = cb1[u] * cb1[u] - sb1[u] * sb1[u];
-    sb2[u] = 2 * cb1[u] * sb1[u];
-    cb4[u] = cb2[u] * cb2[u] - sb2[u] * sb2[u];
-    sb4[u] = 2 * cb2[u] * sb2[u];
-    double cz1, sz1;
-    cs2pi(sz, cz1, sz1);
-#pragma unroll
-    for (int m = 0; m < BPC; m++) {
-      double cc, ss;
-      cs2pi((double)(ch * KZC + 8 * m) * sz, cc, ss);
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        _Float16 p[NPART];
-        split16<NPART>((float)(h ? ss : cc) * 256.0f, p);
-#pragma unroll
-        for (int q = 0; q < NPART; q++) zt[u][m][q][k] = p[q];
-        const double nc = cc * cz1 - ss * sz1;
-        ss = cc * sz1 + ss * cz1;
-        cc = nc;
-      }
-    }
-  }
-  const int *kym_c = chunk_kymax + ch * (kxmax + 1);
-  const half8 *cfl = coefh + (size_t)ch * BPC * NPART * 64;
-  // the operand vectors of group gi's chunk (blocks past the group's last
-  // are not read by the products)
-  auto fetch = [&](int gi, half8 (&v)[NL]) {
-    const int n = min(grp_nb8[gi] - ch * BPC, BPC) * NPART * 64;
-    const half8 *src = cfl + (size_t)grp_hb[gi] * NPART * 64;
-#pragma unroll
-    for (int i = 0; i < NL; i++) {
-      const int t = threadIdx.x + i * kBlock;
-      if (t < n) v[i] = src[t];
-    }
-  };
-  auto put = [&](int buf, const half8 (&v)[NL]) {
-#pragma unroll
-    for (int i = 0; i < NL; i++) {
-      const int t = threadIdx.x + i * kBlock;
-      if (t < NV) stage[buf][t] = v[i];
-    }
-  };
-  // Per kx the force of a row is kx/Lx (ca B - sa A) along x, ca C + sa D
-  // along y and ca E + sa F along z, with A = cb G0 + sb G1, B = cb G2 +
-  // sb G3, C = ky/Ly (cb G1 - sb G0), D = ky/Ly (cb G3 - sb G2), E = cb H0
-  // + sb H1, F = cb H2 + sb H3 (ca, sa: phase of a; cb, sb: of b): A..F
-  // are summed over the rows of a kx in FP32, the kx terms in FP64.
-  double Fx[NTC], Fy[NTC], Fz[NTC];
-  float PA[NTC], PB[NTC], PC[NTC], PD[NTC], PE[NTC], PF[NTC];
-#pragma unroll
-  for (int u = 0; u < NTC; u++) {
-    Fx[u] = 0; Fy[u] = 0; Fz[u] = 0;
-    PA[u] = 0; PB[u] = 0; PC[u] = 0; PD[u] = 0; PE[u] = 0; PF[u] = 0;
-  }
-  double cb[NTC], sb[NTC];
-  int kx = part.y, ky0 = 0, buf = 0;
-  {
-    half8 v[NL];
-    fetch(grp_start[kx], v);
-    put(0, v);
-  }
-  __syncthreads();
-  while (true) {
-    // the group after this one: next 4 rows of kx, or the first of kx + 1
-    int nkx = kx, nky0 = ky0 + 4;
-    if (nky0 > kym_c[kx]) { nkx = kx + 1; nky0 = 0; }
-    const bool more = nkx < part.z;
-    half8 v[NL];
-    if (more) fetch(grp_start[nkx] + (nky0 >> 2), v);
-    if (ky0 == 0) {
-#pragma unroll
-      for (int u = 0; u < NTC; u++) { cb[u] = h ? cb1[u] : 1.0; sb[u] = h ? sb1[u] : 0.0; }
-    }
-    const float kyl = (float)(ky0 + h) * kys, kyl2 = (float)(ky0 + 2 + h) * kys;
-    const int gi = grp_start[kx] + (ky0 >> 2);
-    const int nb = min(grp_nb8[gi] - ch * BPC, BPC);   // wave-uniform, >= 1
-    const half8 *ga = stage[buf] + lane;
-    // per 8-kz block the smaller products first, then the leading one
-    // (0, 0): a group's sums run over at most KZC / 8 blocks, short enough
-    // that one accumulator costs no accuracy to measure (force error 4.07e-7
-    // against 4.06e-7 with the leading products apart; pass 1 sums over
-    // thousands of charges and keeps them apart)
-    mfma_f32x16 acc[NTC];
-#pragma unroll
-    for (int u = 0; u < NTC; u++)
-#pragma unroll
-      for (int r = 0; r < 16; r++) acc[u][r] = 0.0f;
-#pragma unroll
-    for (int b = 0; b < BPC; b++) {
-      if (b < nb) {
-        half8 ev[NPART];
-#pragma unroll
-        for (int k = 0; k < NPART; k++) ev[k] = ga[(b * NPART + k) * 64];
-#pragma unroll
-        for (int u = 0; u < NTC; u++) {
-          if constexpr (NPART == 3) {
-            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[2], zt[u][b][0], acc[u], 0, 0, 0);
-            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][1], acc[u], 0, 0, 0);
-            acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][2], acc[u], 0, 0, 0);
-          }
-          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[1], zt[u][b][0], acc[u], 0, 0, 0);
-          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][1], acc[u], 0, 0, 0);
-          acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ev[0], zt[u][b][0], acc[u], 0, 0, 0);
-        }
-      }
-    }
-    // the group's scale goes into the row phases
-    const float sc = gscale[gi] * (1.0f / 256.0f);
-#pragma unroll
-    for (int u = 0; u < NTC; u++) {
-#pragma unroll
-      for (int q = 0; q < 2; q++) {
-        const float G0 = acc[u][8 * q], G1 = acc[u][8 * q + 1], G2 = acc[u][8 * q + 2],
-                    G3 = acc[u][8 * q + 3];
-        const float H0 = acc[u][8 * q + 4], H1 = acc[u][8 * q + 5], H2 = acc[u][8 * q + 6],
-                    H3 = acc[u][8 * q + 7];
-        const double cq = q ? cb[u] * cb2[u] - sb[u] * sb2[u] : cb[u];
-        const double sq = q ? cb[u] * sb2[u] + sb[u] * cb2[u] : sb[u];
-        const float rcb = (float)cq * sc, rsb = (float)sq * sc;
-        const float ky = q ? kyl2 : kyl;
-        PA[u] = fmaf(rcb, G0, fmaf(rsb, G1, PA[u]));
-        PB[u] = fmaf(rcb, G2, fmaf(rsb, G3, PB[u]));
-        PC[u] = fmaf(ky, rcb * G1 - rsb * G0, PC[u]);
-        PD[u] = fmaf(ky, rcb * G3 - rsb * G2, PD[u]);
-        PE[u] = fmaf(rcb, H0, fmaf(rsb, H1, PE[u]));
-        PF[u] = fmaf(rcb, H2, fmaf(rsb, H3, PF[u]));
-      }
-      const double nc = cb[u] * cb4[u] - sb[u] * sb4[u];
-      sb[u] = cb[u] * sb4[u] + sb[u] * cb4[u];
-      cb[u] = nc;
-    }
-    if (nkx != kx) {
-      // the kx terms of the rows summed so far
-      const float kxl = (float)kx * kxs;
-#pragma unroll
-      for (int u = 0; u < NTC; u++) {
-        double ca, sa;
-        cs2pi(kx * sx[u], ca, sa);
-        const float rca = (float)ca, rsa = (float)sa;
-        Fx[u] += (double)(kxl * (rca * PB[u] - rsa * PA[u]));
-        Fy[u] += (double)(rca * PC[u] + rsa * PD[u]);
-        Fz[u] += (double)(rca * PE[u] + rsa * PF[u]);
-        PA[u] = 0; PB[u] = 0; PC[u] = 0; PD[u] = 0; PE[u] = 0; PF[u] = 0;
-      }
-    }
-    if (!more) break;
-    // every wave has read stage[buf ^ 1] (last group) before the previous
-    // barrier, so it can take the next group now
-    put(buf ^ 1, v);
-    __syncthreads();
-    buf ^= 1;
-    kx = nkx;
-    ky0 = nky0;
-  }
-#pragma unroll
-  for (int u = 0; u < NTC; u++) {
-    const double fx = Fx[u] + __shfl_xor(Fx[u], 32, 64);
-    const double fy = Fy[u] + __shfl_xor(Fy[u], 32, 64);
-    const double fz = Fz[u] + __shfl_xor(Fz[u], 32, 64);
-    const int j = jw + 32 * u;
-    if (h || j >= nq) continue;
-    double *o = kpf + ((size_t)blockIdx.y * nq + j) * 3;
-    o[0] = fx;
-    o[1] = fy;
-    o[2] = fz;
-  }
+  kforce_h_body<KZC, NTC, NPART>(nq, qpos, iLx, iLy, iLz, kxmax, parts[blockIdx.y], blockIdx.x,
+                                 chunk_kymax, grp_start, grp_hb, grp_nb8, coefh, gscale,
+                                 kpf + (size_t)blockIdx.y * nq * 3);
 }
 
 // the cell-dipole field on a charge at r: ek_d = -cm_d M_d + cb_d r_d

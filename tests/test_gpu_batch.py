@@ -139,13 +139,13 @@ def test_batch_kspace_splits_agree(mode):
     rs = [sd["r"] for sd, _ in sy]
     ref = b.forces(rs, 1)
     tol = 1e-12 if mode == abi.MDX_FP64 else 1e-6
-    for mf in ((False, True) if mode == abi.MDX_MIXED else (False,)):
+    for mf in ((0, 1, 2) if mode == abi.MDX_MIXED else (0,)):
         if mode == abi.MDX_MIXED:
             b.set_kspace_mfma(mf)
         for kzc in (8, 16, 24, 32):
             for kparts in (1, 3, 32):
                 b.set_kspace(kzc, kparts)
-                assert b.info()["kmfma"] == int(mf)
+                assert b.info()["kmfma"] == mf
                 fs, outs = b.forces(rs, 1)
                 for f, f0, o, o0 in zip(fs, ref[0], outs, ref[1]):
                     assert rel_rms(f, f0) < tol, (mf, kzc, kparts)
@@ -153,6 +153,9 @@ def test_batch_kspace_splits_agree(mode):
     if mode == abi.MDX_FP64:
         with pytest.raises(RuntimeError):
             b.set_kspace_mfma(True)
+    else:
+        with pytest.raises(RuntimeError):
+            b.set_kspace_mfma(3)
     b.close()
 
 
