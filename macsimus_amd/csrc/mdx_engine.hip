@@ -747,7 +747,7 @@ constexpr int kMaxZ = 64;      // buffer cells of a round (z cells + 2w)
 // branches (the lanes of a wave take different branches) and integer
 // census counts.
 template <typename R, bool MEASURE, bool POLAR, bool E6, bool ST>
-__global__ __launch_bounds__(kPairBlock, (POLAR || MEASURE || sizeof(R) == 8) ? 1 : 4) void k_pairc(
+__global__ __launch_bounds__(kPairBlock, (MEASURE || sizeof(R) == 8 || (POLAR && E6)) ? 1 : 4) void k_pairc(
     int ns, Grid g, int w, int cpp, double cq, const double4 *__restrict__ spos,
     const double4 *__restrict__ spol, const int4 *__restrict__ sinfo,
     const int *__restrict__ scell, const int *__restrict__ cstart, int nt, PairTab<R> tab,
