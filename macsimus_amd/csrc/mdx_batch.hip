@@ -219,57 +219,84 @@ __global__ __launch_bounds__(kBlock) void k_bpair(
       double sy_ = 0;
       if (ny < 0) { ny += nc; sy_ = -S.L; } else if (ny >= nc) { ny -= nc; sy_ = S.L; }
       R ac[3] = {0, 0, 0};
-      for (int dz = -w; dz <= w; dz++) {
-        int nz = cz + dz;
-        double sz = 0;
-        if (nz < 0) { nz += nc; sz = -S.L; } else if (nz >= nc) { nz -= nc; sz = S.L; }
-        const int cc = S.cell_off + (nx * nc + ny) * nc + nz;
-        for (int j = cstart[cc]; j < cstart[cc + 1]; j++) {
-          const double4 pj = spos[j];
-          const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy_), dzr = pi.z - (pj.z + sz);
-          const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
-          if (rr >= S.cq || j == s) continue;
-          npair += 1;
-          const R *ss = ljrow + sinfo[j].x * 8;
-          R fr = 0, u = 0;
-          if (rr < (double)ss[5]) {
-            if (rr < (double)ss[4]) {
-              if (E6) {
-                const R q = (R)rr, x = sqrt(q), y = exp(ss[0] * x), iz = brcp<R>(q * q * q);
-                if (MEASURE) u = ss[1] * y - ss[3] * iz;
-                fr = ss[2] * y * brcp<R>(x) - (R)6 * ss[3] * iz * brcp<R>(q);
-              } else {
-                const R x = ss[0] * brcp<R>((R)rr), y = x * x * x;
-                if (MEASURE) {
-                  const R z = y * y;
-                  u = ss[1] * (z - y);
-                  fr = ss[2] * (z + z - y) * x;
+      // the z cells of the column in dz order, as runs of consecutive cells
+      // with one image shift (a run is a contiguous range of sorted sites);
+      // per run 32 candidates at a time are tested into a bit mask and the
+      // pair terms run only for the set bits, in the order of the cells
+      auto wrapz = [&](int z, double &sh) {
+        sh = 0;
+        if (z < 0) { z += nc; sh = -S.L; } else if (z >= nc) { z -= nc; sh = S.L; }
+        return z;
+      };
+      for (int dz = -w; dz <= w;) {
+        double sz;
+        const int nz = wrapz(cz + dz, sz);
+        int n1 = nz;
+        for (dz++; dz <= w; dz++) {
+          double sh;
+          const int z = wrapz(cz + dz, sh);
+          if (z != n1 + 1 || sh != sz) break;
+          n1 = z;
+        }
+        const int c0 = S.cell_off + (nx * nc + ny) * nc;
+        const int jlo = cstart[c0 + nz], jhi = cstart[c0 + n1 + 1];
+        for (int jb = jlo; jb < jhi; jb += 32) {
+          unsigned mask = 0;
+          const int nb = min(32, jhi - jb);
+          for (int t = 0; t < nb; t++) {
+            const double4 pj = spos[jb + t];
+            const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy_), dzr = pi.z - (pj.z + sz);
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            mask |= (unsigned)(rr < S.cq && jb + t != s) << t;
+          }
+          while (mask) {
+            const int j = jb + __ffs(mask) - 1;
+            mask &= mask - 1;
+            const double4 pj = spos[j];
+            const double dxr = pi.x - (pj.x + sx), dyr = pi.y - (pj.y + sy_), dzr = pi.z - (pj.z + sz);
+            const double rr = dxr * dxr + dyr * dyr + dzr * dzr;
+            npair += 1;
+            const R *ss = ljrow + sinfo[j].x * 8;
+            R fr = 0, u = 0;
+            if (rr < (double)ss[5]) {
+              if (rr < (double)ss[4]) {
+                if (E6) {
+                  const R q = (R)rr, x = sqrt(q), y = exp(ss[0] * x), iz = brcp<R>(q * q * q);
+                  if (MEASURE) u = ss[1] * y - ss[3] * iz;
+                  fr = ss[2] * y * brcp<R>(x) - (R)6 * ss[3] * iz * brcp<R>(q);
                 } else {
-                  fr = ss[3] * y * (y - (R)0.5) * x;
+                  const R x = ss[0] * brcp<R>((R)rr), y = x * x * x;
+                  if (MEASURE) {
+                    const R z = y * y;
+                    u = ss[1] * (z - y);
+                    fr = ss[2] * (z + z - y) * x;
+                  } else {
+                    fr = ss[3] * y * (y - (R)0.5) * x;
+                  }
                 }
+              } else {
+                const R x = (R)(rr - (double)ss[5]);
+                if (MEASURE) u = x * x * ss[6];
+                fr = ss[7] * x;
               }
-            } else {
-              const R x = (R)(rr - (double)ss[5]);
-              if (MEASURE) u = x * x * ss[6];
-              fr = ss[7] * x;
+              if (MEASURE) Vir += rr * (double)fr;
+              nlj += ss[6] != 0;
             }
-            if (MEASURE) Vir += rr * (double)fr;
-            nlj += ss[6] != 0;
-          }
-          const R qq = qi * (R)pj.w;
-          if (qq != 0) {
-            R eru = 0, erd;
-            bspline<R>(et, rr, S.sgrid, eru, erd, MEASURE);
-            fr += qq * erd;
-            if (MEASURE) Eel += (double)(qq * eru);
-          }
-          const R ddx = (R)dxr, ddy = (R)dyr, ddz = (R)dzr;
-          const R fdx = fr * ddx, fdy = fr * ddy, fdz = fr * ddz;
-          ac[0] += fdx; ac[1] += fdy; ac[2] += fdz;
-          if (MEASURE) {
-            U += u;
-            pv[0] += (double)(ddx * fdx); pv[1] += (double)(ddy * fdy); pv[2] += (double)(ddz * fdz);
-            pv[3] += (double)(ddz * fdy); pv[4] += (double)(ddx * fdz); pv[5] += (double)(ddy * fdx);
+            const R qq = qi * (R)pj.w;
+            if (qq != 0) {
+              R eru = 0, erd;
+              bspline<R>(et, rr, S.sgrid, eru, erd, MEASURE);
+              fr += qq * erd;
+              if (MEASURE) Eel += (double)(qq * eru);
+            }
+            const R ddx = (R)dxr, ddy = (R)dyr, ddz = (R)dzr;
+            const R fdx = fr * ddx, fdy = fr * ddy, fdz = fr * ddz;
+            ac[0] += fdx; ac[1] += fdy; ac[2] += fdz;
+            if (MEASURE) {
+              U += u;
+              pv[0] += (double)(ddx * fdx); pv[1] += (double)(ddy * fdy); pv[2] += (double)(ddz * fdz);
+              pv[3] += (double)(ddz * fdy); pv[4] += (double)(ddx * fdz); pv[5] += (double)(ddy * fdx);
+            }
           }
         }
       }
